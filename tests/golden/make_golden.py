@@ -1,0 +1,159 @@
+"""Write the golden fixtures of tests/golden/*.npz.
+
+Each case is a deterministic synthetic NumCalc project (mesh2hrtf_amd.project) run
+through the reference NumCalc compiled from its own sources by ``make -C oracle ref``
+(oracle/_ref/NumCalc).  The fixture stores the inputs needed to rebuild the problem
+(mesh, evaluation points, BC/source spec as JSON) and the reference be.out values.
+
+Run from the repository root:  python tests/golden/make_golden.py [case ...]
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from mesh2hrtf_amd import project as P  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def cube_sphere(n, radius):
+    """Quad mesh: every cube face split n x n, projected on a sphere (outward order)."""
+    pts, index, quads = [], {}, []
+
+    def pid(p):
+        key = tuple(np.round(p, 12))
+        if key not in index:
+            index[key] = len(pts)
+            pts.append(p)
+        return index[key]
+
+    g = np.linspace(-1.0, 1.0, n + 1)
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            ids = {}
+            for a in range(n + 1):
+                for b in range(n + 1):
+                    p = np.zeros(3)
+                    p[axis] = sign
+                    p[(axis + 1) % 3] = g[a]
+                    p[(axis + 2) % 3] = g[b]
+                    ids[a, b] = pid(p / np.linalg.norm(p) * radius)
+            for a in range(n):
+                for b in range(n):
+                    q = [ids[a, b], ids[a + 1, b], ids[a + 1, b + 1], ids[a, b + 1]]
+                    quads.append(q)
+    nodes = np.asarray(pts)
+    quads = np.asarray(quads)
+    # NumCalc quad normal = dx/ds x dx/dt with s along 1->0 ... orient by centroid test
+    out = []
+    for q in quads:
+        x = nodes[q]
+        c = x.mean(axis=0)
+        xs = 0.25 * ((x[0] - x[1]) + (x[3] - x[2]))
+        xt = 0.25 * ((x[0] - x[3]) + (x[1] - x[2]))
+        if np.dot(np.cross(xs, xt), c) < 0:
+            q = q[::-1]
+        out.append(q)
+    return P.Mesh(nodes, np.asarray(out))
+
+
+def cases():
+    ico2 = P.icosphere(2, 0.1)
+    ne = len(ico2.elems)
+    C = {}
+    C["c1"] = dict(prj=P.config1_project([100.0, 1000.0, 2000.0]))
+    C["soft_pw"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[500.0, 2500.0], eval_grid=P.fibonacci_grid(100, 0.5),
+        boundary=[P.Boundary(0, ne - 1, "PRES", 0.0, 0.0)],
+        plane_waves=[(0.0, -1.0, 0.0, 1.0, 0.0)], title="soft sphere plane wave"))
+    C["adm_ps"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[800.0, 3000.0], eval_grid=P.fibonacci_grid(100, 0.6),
+        boundary=[P.Boundary(0, ne - 1, "ADMI", 0.5, 0.2)],
+        point_sources=[(0.05, 0.0, 0.25, 1.0, 0.5)], title="admittance sphere point source"))
+    C["velo_patch"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[1000.0, 4000.0], eval_grid=P.fibonacci_grid(60, 1.0),
+        boundary=[P.Boundary(0, ne - 1, "VELO", 0.0, 0.0), P.Boundary(17, 19, "VELO", 0.1, 0.0)],
+        title="rigid sphere with vibrating patch"))
+    C["mixed_pres"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[1500.0], eval_grid=P.fibonacci_grid(60, 0.8),
+        boundary=[P.Boundary(0, 99, "PRES", 0.3, -0.1), P.Boundary(200, 210, "VELO", 0.05, 0.02),
+                  P.Boundary(250, 260, "IMPE", 2.0, 1.0)],
+        point_sources=[(0.0, 0.2, 0.0, 1.0, 0.0)], title="mixed BCs"))
+    C["quad"] = dict(prj=P.Project(
+        mesh=cube_sphere(6, 0.1), freqs=[700.0, 2100.0], eval_grid=P.fibonacci_grid(80, 0.7),
+        point_sources=[(0.0, 0.0, 0.3, 1.0, 0.0)], title="quad cube-sphere"))
+    head = P.head_like_mesh(freq=8, seed=0)
+    src = P.ear_sources(head)[0]
+    C["head8"] = dict(prj=P.Project(
+        mesh=head, freqs=[2000.0, 6000.0], eval_grid=P.fibonacci_grid(150, 1.2),
+        point_sources=[(src[0], src[1], src[2], 1.0, 0.0)], title="head-like mesh nu=8"))
+    for pc in (1, 2):
+        C["c1_precond%d" % pc] = dict(prj=P.Project(
+            mesh=P.icosphere(3, 0.1), freqs=[1000.0], eval_grid=P.fibonacci_grid(2030, 1.2),
+            point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], precond=pc))
+    C["gauss"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[1200.0], eval_grid=P.fibonacci_grid(50, 0.9), method_solver=4,
+        point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], title="direct solver"))
+    return C
+
+
+def spec_of(prj):
+    return dict(freqs=list(prj.freqs),
+                boundary=[(b.lo, b.hi, b.kind, b.re, b.im) for b in prj.boundary],
+                point_sources=[list(map(float, s)) for s in prj.point_sources],
+                plane_waves=[list(map(float, s)) for s in prj.plane_waves],
+                method_bem=prj.method_bem, method_solver=prj.method_solver,
+                precond=prj.precond, c=prj.c, rho=prj.rho)
+
+
+def run_case(name, prj):
+    with tempfile.TemporaryDirectory() as d:
+        P.write_project(d, prj)
+        t0 = time.time()
+        r = subprocess.run([REF], cwd=d, capture_output=True, text=True)
+        wall = time.time() - t0
+        if r.returncode != 0:
+            raise RuntimeError("%s: NumCalc failed\n%s" % (name, r.stdout[-2000:]))
+        out = {}
+        nout = open(os.path.join(d, "NC.out")).read()
+        iters = [int(l.split("=")[1].split(",")[0]) for l in nout.splitlines()
+                 if l.startswith("CGS solver: number of iterations")]
+        for key in ("pBoundary", "vBoundary", "pEvalGrid", "vEvalGrid"):
+            vals = []
+            for s in range(len(prj.freqs)):
+                ids, v = P.read_be_file(os.path.join(d, "be.out", "be.%d" % (s + 1), key))
+                vals.append(v)
+            out[key] = np.asarray(vals)
+            out[key + "_ids"] = ids
+    m = prj.mesh
+    g = prj.eval_grid
+    np.savez_compressed(
+        os.path.join(OUT, "case_%s.npz" % name),
+        nodes=m.nodes, elems=m.elems, elem_ids=m.elem_ids,
+        eval_nodes=g.nodes if g is not None else np.zeros((0, 3)),
+        spec=np.array(json.dumps(spec_of(prj))),
+        cgs_iterations=np.asarray(iters, dtype=np.int64),
+        ref_wall_s=np.float64(wall), **out)
+    print("%-12s N=%d M=%d steps=%d wall=%.1fs iters=%s" % (
+        name, len(m.elems), 0 if g is None else len(g.nodes), len(prj.freqs), wall, iters))
+
+
+def main(argv):
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: make -C oracle ref")
+    C = cases()
+    for name in (argv or list(C)):
+        run_case(name, C[name]["prj"])
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

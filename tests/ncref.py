@@ -1,0 +1,203 @@
+"""Test helpers: golden fixtures, NumCalc input transforms, and the oracle binding.
+
+The transforms restate what NumCalc does between NC.inp and the solve:
+  * boundary lines -> per-element kind/value (src/NC_Input.cpp:1163-1406);
+  * per-frequency scaling (NC_UpdateFreqCurves, src/NC_EquationSystem.cpp:1815-1961):
+    pressure / (i rho omega h), admittance * (i h k), sources / (i rho omega h);
+  * outputs p = i rho omega h phi (src/NC_PostProcessing.cpp:541-545, 744).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLDEN, "case_%s.npz" % name))   # allow_pickle stays False
+    d = {k: z[k] for k in z.files}
+    d["spec"] = json.loads(str(d["spec"]))
+    return d
+
+
+def case_names():
+    return sorted(f[5:-4] for f in os.listdir(GOLDEN) if f.startswith("case_") and f.endswith(".npz"))
+
+
+def element_bc(case):
+    """Per-element (kind, nodal value (n,4), admittance) as given in NC.inp, before scaling."""
+    elems = case["elems"]
+    n = len(elems)
+    ext = case["elem_ids"]
+    loc = {int(e): i for i, e in enumerate(ext)}
+    kind = np.zeros(n, dtype=np.int32)
+    val = np.zeros((n, 4), dtype=np.complex128)
+    adm = np.zeros(n, dtype=np.complex128)
+    for lo, hi, key, re, im in case["spec"]["boundary"]:
+        z = complex(re, im)
+        nb = {"VELO": 0, "PRES": 1, "ADMI": 2, "IMPE": 2}[key]
+        if key == "IMPE":
+            if re == 0 and im == 0:
+                nb = 1
+            else:
+                z = 1.0 / z
+        for e in range(lo, hi + 1):
+            if e not in loc:
+                continue
+            i = loc[e]
+            if nb == 0:
+                val[i, :] = z
+            elif nb == 1:
+                val[i, :] = z
+                kind[i] = 1
+            else:
+                adm[i] = z
+                if kind[i] == 0:
+                    kind[i] = 2
+    return kind, val, adm
+
+
+def problem_arrays(case, freq, h=1.0):
+    """Arrays of the oracle's ncp_problem at one frequency (host-side scaling applied)."""
+    spec = case["spec"]
+    nodes, elems = case["nodes"], case["elems"]
+    n = len(elems)
+    nv = np.full(n, elems.shape[1], dtype=np.int32)
+    xyz = np.zeros((n, 4, 3))
+    xyz[:, :elems.shape[1], :] = nodes[elems]
+    kind, val, adm = element_bc(case)
+    omega = 2.0 * np.pi * freq
+    k = omega / spec["c"]
+    rpfact = spec["rho"] * omega * h
+    val = val.copy()
+    val[kind == 1] = val[kind == 1] / (1j * rpfact)
+    adm = np.where(kind == 2, adm * (1j * h * k), 0.0)
+    ps = np.array([[s[0], s[1], s[2], *(lambda z: (z.real, z.imag))(complex(s[3], s[4]) / (1j * rpfact))]
+                   for s in spec["point_sources"]]).reshape(-1, 5)
+    pw = []
+    for w in spec["plane_waves"]:
+        d = np.asarray(w[:3], dtype=np.float64)
+        d = d / np.sqrt(d @ d)
+        z = complex(w[3], w[4]) / (1j * rpfact)
+        pw.append([*d, z.real, z.imag])
+    pw = np.asarray(pw).reshape(-1, 5)
+    return dict(n=n, nv=nv, xyz=xyz, kind=kind, bc=val, adm=adm, ps=ps, pw=pw,
+                k=k, h=h, rpfact=rpfact)
+
+
+def surface_outputs(pa, sol):
+    """pBoundary and vBoundary values from the solved unknowns."""
+    kind = pa["kind"]
+    phi = np.where(kind == 1, pa["bc"][:, 0], sol)
+    v = np.where(kind == 1, sol, pa["bc"][:, 0] - np.where(kind == 2, sol * pa["adm"], 0.0))
+    return 1j * pa["rpfact"] * phi, v
+
+
+# ----------------------------------------------------------------------------- oracle
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nv", ctypes.c_void_p), ("xyz", ctypes.c_void_p),
+                ("bc_kind", ctypes.c_void_p), ("bc", ctypes.c_void_p), ("adm", ctypes.c_void_p),
+                ("n_ps", ctypes.c_int), ("ps", ctypes.c_void_p), ("n_pw", ctypes.c_int),
+                ("pw", ctypes.c_void_p), ("k", ctypes.c_double), ("h", ctypes.c_double),
+                ("interior", ctypes.c_int)]
+
+
+class Oracle:
+    """ctypes binding of oracle/libncport.so (the C restatement); test use only."""
+
+    def __init__(self):
+        path = os.path.join(ROOT, "oracle", "libncport.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: make -C oracle port")
+        self.lib = ctypes.CDLL(path)
+
+    def _prob(self, pa):
+        keep = dict(nv=np.ascontiguousarray(pa["nv"], np.int32),
+                    xyz=np.ascontiguousarray(pa["xyz"], np.float64),
+                    kind=np.ascontiguousarray(pa["kind"], np.int32),
+                    bc=np.ascontiguousarray(pa["bc"], np.complex128),
+                    adm=np.ascontiguousarray(pa["adm"], np.complex128),
+                    ps=np.ascontiguousarray(pa["ps"], np.float64),
+                    pw=np.ascontiguousarray(pa["pw"], np.float64))
+        p = _Problem(pa["n"], keep["nv"].ctypes.data, keep["xyz"].ctypes.data,
+                     keep["kind"].ctypes.data, keep["bc"].ctypes.data, keep["adm"].ctypes.data,
+                     len(keep["ps"]), keep["ps"].ctypes.data, len(keep["pw"]), keep["pw"].ctypes.data,
+                     pa["k"], pa["h"], 0)
+        return p, keep
+
+    def assemble(self, pa):
+        p, keep = self._prob(pa)
+        n = pa["n"]
+        A = np.zeros((n, n), np.complex128)
+        b = np.zeros(n, np.complex128)
+        rc = self.lib.ncp_assemble(ctypes.byref(p), ctypes.c_void_p(A.ctypes.data),
+                                   ctypes.c_void_p(b.ctypes.data))
+        if rc:
+            raise RuntimeError("ncp_assemble failed")
+        return A, b
+
+    def integrals(self, pa, i, j):
+        p, keep = self._prob(pa)
+        out = np.zeros(5, np.complex128)
+        rc = self.lib.ncp_element_integrals(ctypes.byref(p), i, j, ctypes.c_void_p(out.ctypes.data))
+        if rc:
+            raise RuntimeError("ncp_element_integrals failed")
+        return out
+
+    def subelements(self, pa, j, x):
+        p, keep = self._prob(pa)
+        s0, t0, fa = (np.zeros(220) for _ in range(3))
+        order = np.zeros(220, np.int32)
+        x = np.ascontiguousarray(x, np.float64)
+        n = self.lib.ncp_subelements(ctypes.byref(p), j, ctypes.c_void_p(x.ctypes.data),
+                                     ctypes.c_void_p(s0.ctypes.data), ctypes.c_void_p(t0.ctypes.data),
+                                     ctypes.c_void_p(fa.ctypes.data), ctypes.c_void_p(order.ctypes.data))
+        return s0[:n], t0[:n], fa[:n], order[:n]
+
+    def cgs(self, A, b, precond=2, maxit=250, tol=1e-9):
+        n = len(b)
+        A = np.ascontiguousarray(A, np.complex128)
+        b = np.ascontiguousarray(b, np.complex128)
+        x = np.zeros(n, np.complex128)
+        err = ctypes.c_double(0)
+        it = self.lib.ncp_cgs(n, ctypes.c_void_p(A.ctypes.data), ctypes.c_void_p(b.ctypes.data),
+                              ctypes.c_void_p(x.ctypes.data), precond, maxit, ctypes.c_double(tol),
+                              ctypes.byref(err))
+        return x, it, err.value
+
+    def lu_solve(self, A, b):
+        A = np.array(A, np.complex128, order="C")
+        b = np.array(b, np.complex128)
+        self.lib.ncp_lu_solve(len(b), ctypes.c_void_p(A.ctypes.data), ctypes.c_void_p(b.ctypes.data))
+        return b
+
+    def field(self, pa, sol, pts):
+        p, keep = self._prob(pa)
+        pts = np.ascontiguousarray(pts, np.float64)
+        sol = np.ascontiguousarray(sol, np.complex128)
+        m = len(pts)
+        phi = np.zeros(m, np.complex128)
+        grad = np.zeros((m, 3), np.complex128)
+        rc = self.lib.ncp_field(ctypes.byref(p), ctypes.c_void_p(sol.ctypes.data), m,
+                                ctypes.c_void_p(pts.ctypes.data), ctypes.c_void_p(phi.ctypes.data),
+                                ctypes.c_void_p(grad.ctypes.data))
+        if rc:
+            raise RuntimeError("ncp_field failed")
+        return phi, grad
+
+    def geometry(self, pa):
+        p, keep = self._prob(pa)
+        n = pa["n"]
+        c, nn, a = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        self.lib.ncp_element_geometry(ctypes.byref(p), ctypes.c_void_p(c.ctypes.data),
+                                      ctypes.c_void_p(nn.ctypes.data), ctypes.c_void_p(a.ctypes.data))
+        return c, nn, a
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
