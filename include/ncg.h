@@ -1,0 +1,152 @@
+/*
+ * ncg.h — C ABI of libncg, the MI355X (gfx950) implementation of NumCalc's
+ * Burton–Miller collocation BEM solve.
+ *
+ * The reference exposes no library API for this path: its boundary is the NumCalc
+ * executable (mesh2hrtf/NumCalc/src/NC_Main.cpp:137-281) driven through files.  These
+ * entry points are the seams inside that executable, one per phase of a frequency
+ * step of NC_ControlProgram (src/NC_Main.cpp:370-957, src/ = mesh2hrtf/NumCalc/src/).
+ * The drop-in `NumCalc` host program built beside this library calls them; a caller
+ * that wants the solver without files (ctypes, a serving loop) calls them directly.
+ *
+ * Conventions: plain C types only; complex numbers are interleaved (re, im) doubles;
+ * every function returns 0 on success or a negative NCG_E* code, with a message in
+ * ncg_last_error().  No function calls exit().  A problem handle owns its device
+ * memory and one HIP stream; calls on one handle must not overlap, handles on
+ * different threads are independent.
+ */
+#ifndef NCG_H
+#define NCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NCG_OK = 0,
+    NCG_E_ARG = -1,        /* invalid argument                                    */
+    NCG_E_DEVICE = -2,     /* HIP runtime error or no device                      */
+    NCG_E_MEMORY = -3,     /* device allocation failed                            */
+    NCG_E_SUBDIV = -4,     /* adaptive subdivision limits exceeded (see message)  */
+    NCG_E_STATE = -5,      /* call order violated (e.g. solve before assemble)    */
+    NCG_E_NOCONV = -6      /* iterative solver reached its iteration limit        */
+};
+
+/* element boundary-condition kinds (ibval of src/NC_Input.cpp:1163-1406) */
+enum { NCG_BC_VELOCITY = 0, NCG_BC_PRESSURE = 1, NCG_BC_ADMITTANCE = 2 };
+
+/* arithmetic of the bulk quadrature and of the stored operator */
+enum {
+    NCG_FP64 = 0,          /* plain method: everything FP64                                   */
+    NCG_MIXED = 1          /* FP32-complex bulk quadrature, operator and Krylov matvec;
+                              FP64 singular/near integrals, Krylov dots and refinement residual */
+};
+
+enum { NCG_SOLVER_CGS = 0, NCG_SOLVER_GMRES = 1, NCG_SOLVER_LU = 4 };
+enum { NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2 };
+
+/* Frequency-independent description of one boundary-element problem.
+ * Replaces the geometry arrays NumCalc builds in NC_ReadMesh
+ * (src/NC_Input.cpp:353-737): element node coordinates, the boundary-condition
+ * kind per element (it decides the unknown: potential or normal velocity), and
+ * the evaluation-grid nodes of NC_WriteResultsEvaluationGrid (src/NC_PostProcessing.cpp:581). */
+typedef struct {
+    int32_t n_elements;                 /* boundary elements = unknowns                    */
+    const int32_t *nodes_per_element;   /* [n] 3 (triangle) or 4 (quadrilateral)          */
+    const double *node_xyz;             /* [n][4][3]; 4th node ignored for triangles       */
+    const int32_t *bc_kind;             /* [n] NCG_BC_*                                    */
+    int32_t n_eval;                     /* evaluation points (may be 0)                    */
+    const double *eval_xyz;             /* [n_eval][3]                                     */
+    double harmonic_time_factor;        /* +1: e^{-i w t} (NumCalc default), -1: e^{+i w t} */
+    int32_t interior;                   /* 0 exterior problem, 1 interior                  */
+} ncg_geometry;
+
+/* One frequency step with n_rhs load cases that share the operator.  Values are
+ * the per-frequency ones of NC_UpdateFreqCurves (src/NC_EquationSystem.cpp:1815-1961):
+ * prescribed pressures and source strengths already divided by i*rho*omega*h,
+ * admittances already multiplied by i*h*k. */
+typedef struct {
+    double k;                           /* wave number omega/c                              */
+    int32_t n_rhs;                      /* load cases (1 for a NumCalc step)                */
+    const double *bc_value;             /* [n_rhs][n] complex: velocity or pressure per element */
+    const double *admittance;           /* [n] complex, used where bc_kind == ADMITTANCE    */
+    int32_t n_point_sources;
+    const double *point_sources;        /* [ns][6]: x y z re im rhs_index                   */
+    int32_t n_plane_waves;
+    const double *plane_waves;          /* [nw][6]: unit direction, re im rhs_index         */
+} ncg_frequency;
+
+typedef struct {
+    int32_t solver;                     /* NCG_SOLVER_*                                      */
+    int32_t precond;                    /* NCG_PRECOND_*                                     */
+    int32_t max_iterations;             /* NumCalc default 250 (src/NC_Main.cpp:83)          */
+    double tolerance;                   /* NumCalc 1e-9 (src/NC_CommonFunctions.cpp:39)      */
+    int32_t restart;                    /* GMRES restart length                              */
+    int32_t refinement_steps;           /* NCG_MIXED: max outer FP64 refinement steps        */
+    double refinement_tolerance;        /* NCG_MIXED: FP64 residual target                   */
+} ncg_solver_options;
+
+typedef struct {
+    int32_t iterations[64];             /* per load case (first 64)                          */
+    double relative_residual[64];
+    int32_t converged_all;
+    int32_t refinement_steps;
+    double ms_assemble, ms_solve, ms_field;   /* device time of the last calls (HIP events)  */
+} ncg_solve_stats;
+
+typedef struct ncg_problem ncg_problem;
+
+/* Library / device info.  ncg_device_count replaces nothing in the reference. */
+const char *ncg_last_error(void);
+int ncg_device_count(int32_t *count);
+
+/* Upload geometry, compute element data and the frequency-independent near-pair
+ * table (the "which pairs need subdivision" plan) on the device.
+ * Replaces the geometry part of NC_ReadMesh (src/NC_Input.cpp:522-605). */
+int ncg_problem_create(const ncg_geometry *geometry, int32_t device, ncg_problem **out);
+int ncg_problem_destroy(ncg_problem *p);
+
+/* Dense Burton–Miller system and right-hand sides of one frequency on the device.
+ * Replaces NC_UpdateFreqCurves' use sites + NC_SetupEquationSystem / NC_BuildSystemTBEM
+ * (src/NC_EquationSystem.cpp:79-302, 1408-1812). precision: NCG_FP64 or NCG_MIXED. */
+int ncg_assemble(ncg_problem *p, const ncg_frequency *freq, int32_t precision);
+
+/* Solve the assembled system for all load cases.
+ * Replaces NC_IterativeSolverCGS / Tfactor_usy+Tfbelim (src/NC_CommonFunctions.cpp:1066-1208,
+ * src/NC_TypeDefinition.h:542-594, selected at src/NC_Main.cpp:833-866). */
+int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *stats);
+
+/* Solved unknowns to host: [n_rhs][n] complex (potential for velocity/admittance
+ * elements, normal velocity for pressure elements).  Replaces reading zrhs after the
+ * solve (src/NC_PostProcessing.cpp:367-578). */
+int ncg_get_solution(ncg_problem *p, double *x);
+
+/* Potential and its gradient at the evaluation points, incident field included:
+ * phi [n_rhs][n_eval] complex, grad [n_rhs][n_eval][3] complex (grad may be NULL).
+ * Replaces the TBEM branch of NC_WriteResultsEvaluationGrid
+ * (src/NC_PostProcessing.cpp:581-866, 869-975, 1217-1322). */
+int ncg_field(ncg_problem *p, double *phi, double *grad);
+
+/* ---- inspection hooks used by the parity tests and benchmarks -------------------- */
+
+/* Copy the assembled operator (row-major, complex128 even in NCG_MIXED) and the
+ * right-hand sides [n_rhs][n] to host. */
+int ncg_get_system(ncg_problem *p, double *A, double *b);
+
+/* y = A x for n_rhs vectors [n_rhs][n] (host buffers), with the stored operator. */
+int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
+
+/* Per-phase device timings of the last assemble/solve/field calls, and the average
+ * duration of the dominant kernel of each phase over the calls since the last reset
+ * (HIP events on the problem's stream).  Layout of t[8]:
+ * [0] assemble ms  [1] solve ms  [2] field ms  [3] far-field quadrature kernel avg ms
+ * [4] matvec kernel avg ms  [5] matvec launches  [6] quadrature evaluations of the
+ * last assembly (far + near + singular)  [7] bytes of the stored operator. */
+int ncg_timings(ncg_problem *p, double *t, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NCG_H */

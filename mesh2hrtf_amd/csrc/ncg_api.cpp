@@ -1,0 +1,868 @@
+// libncg C ABI (include/ncg.h): problem handles, device buffers, and the host-side
+// orchestration of assembly, Krylov solvers and field evaluation.  All arithmetic on
+// the solve path runs in the kernels of ncg_kernels.hip; this file only sequences
+// launches, moves the few scalars the solvers branch on, and times phases with HIP
+// events on the problem's stream.
+#include "ncg.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "ncg_kernels.hpp"
+
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define NCG_HIP(call)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return fail(NCG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Gauss-Legendre nodes/weights by Newton iteration on P_n (the rules NumCalc tabulates
+// in src/NC_IntegrationConstants.h:24-1900), packed rule n at n(n-1)/2, n = 1..8.
+void gauss_tables(double *x, double *w)
+{
+    for (int n = 1; n <= 8; ++n) {
+        double *xn = x + n * (n - 1) / 2, *wn = w + n * (n - 1) / 2;
+        for (int i = 0; i < (n + 1) / 2; ++i) {
+            double z = cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1.0, p1 = 0.0;
+                for (int k = 1; k <= n; ++k) {
+                    double p2 = p1;
+                    p1 = p0;
+                    p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+                }
+                pp = n * (z * p0 - p1) / (z * z - 1.0);
+                double dz = p0 / pp;
+                z -= dz;
+                if (fabs(dz) < 1e-16) break;
+            }
+            double p0 = 1.0, p1 = 0.0;
+            for (int k = 1; k <= n; ++k) {
+                double p2 = p1;
+                p1 = p0;
+                p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+            }
+            pp = n * (z * p0 - p1) / (z * z - 1.0);
+            xn[i] = -z;
+            xn[n - 1 - i] = z;
+            wn[i] = wn[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+        }
+        if (n % 2) xn[n / 2] = 0.0;
+    }
+}
+
+template <typename T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+struct ncg_problem {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n = 0, m = 0;
+    double h = 1.0;
+    int interior = 0;
+    // geometry (device)
+    double *X = nullptr, *cen = nullptr, *nrm = nullptr, *area = nullptr, *jac = nullptr, *eval = nullptr;
+    int8_t *nv = nullptr, *kind = nullptr;
+    std::vector<int8_t> kind_h;
+    long long nnear = 0;
+    int *near_rows = nullptr, *near_cols = nullptr;
+    // frequency data
+    double k = 0.0;
+    int nrhs = 0, precision = NCG_FP64;
+    int assembled = 0, solved = 0;
+    void *A = nullptr;
+    size_t A_cap = 0;
+    double *adm = nullptr;
+    double *bcval = nullptr, *rhs = nullptr, *rhs_src = nullptr, *x = nullptr;
+    size_t vec_cap = 0;   // complex entries of each [nrhs][n] vector
+    double *ps = nullptr, *pw = nullptr;
+    int nps = 0, npw = 0, ps_cap = 0, pw_cap = 0;
+    int *src = nullptr;
+    int nsrc = 0;
+    int *err = nullptr;
+    unsigned long long *evals = nullptr;
+    // Krylov workspace
+    std::vector<double *> work;
+    size_t work_cap = 0;
+    double *scal = nullptr, *partial = nullptr, *dotout = nullptr, *dvec = nullptr;
+    int *active = nullptr;
+    double *basis = nullptr;
+    size_t basis_cap = 0;
+    // timings
+    EventPair ev_asm, ev_far, ev_solve, ev_field;
+    std::vector<EventPair> ev_mv;
+    size_t mv_used = 0;
+    double t_asm = 0, t_solve = 0, t_field = 0;
+    double far_ms_sum = 0, mv_ms_sum = 0;
+    long long far_calls = 0, mv_calls = 0;
+    double last_evals = 0;
+
+    ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
+    ncgk::BMParams bm() const
+    {
+        ncgk::BMParams b;
+        b.k = k;
+        b.hk = h * k;
+        b.tau = interior ? -1.0 : 1.0;
+        b.beta_im = interior ? 0.0 : h / k;
+        return b;
+    }
+};
+
+namespace {
+
+int ensure_vectors(ncg_problem *p, int nrhs)
+{
+    const size_t need = (size_t)nrhs * p->n;
+    if (need <= p->vec_cap) return 0;
+    hipFree(p->bcval);
+    hipFree(p->rhs);
+    hipFree(p->rhs_src);
+    hipFree(p->x);
+    p->vec_cap = 0;
+    int rc;
+    if ((rc = dalloc(&p->bcval, 2 * need)) || (rc = dalloc(&p->rhs, 2 * need)) || (rc = dalloc(&p->rhs_src, 2 * need)) ||
+        (rc = dalloc(&p->x, 2 * need)))
+        return rc;
+    p->vec_cap = need;
+    return 0;
+}
+
+int ensure_work(ncg_problem *p, int count)
+{
+    const size_t need = (size_t)p->nrhs * p->n;
+    if ((int)p->work.size() >= count && p->work_cap >= need) return 0;
+    for (double *w : p->work) hipFree(w);
+    p->work.assign(count, nullptr);
+    for (int i = 0; i < count; ++i) {
+        int rc = dalloc(&p->work[i], 2 * need);
+        if (rc) return rc;
+    }
+    p->work_cap = need;
+    return 0;
+}
+
+void record(ncg_problem *p, hipEvent_t e) { hipEventRecord(e, p->stream); }
+
+double elapsed(const EventPair &ev)
+{
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev.a, ev.b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+// matvec with per-call timing (the Krylov loops' dominant kernel)
+void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
+{
+    if (p->mv_used == p->ev_mv.size()) {
+        EventPair e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        p->ev_mv.push_back(e);
+    }
+    EventPair &e = p->ev_mv[p->mv_used++];
+    record(p, e.a);
+    ncgk::matvec(p->stream, p->A, p->precision == NCG_MIXED, acc64, p->n, p->nrhs, x, d, y);
+    record(p, e.b);
+}
+
+void harvest_mv(ncg_problem *p)
+{
+    hipStreamSynchronize(p->stream);
+    for (size_t i = 0; i < p->mv_used; ++i) {
+        p->mv_ms_sum += elapsed(p->ev_mv[i]);
+        p->mv_calls++;
+    }
+    p->mv_used = 0;
+}
+
+int check_err(ncg_problem *p)
+{
+    int e = 0;
+    NCG_HIP(hipMemcpyAsync(&e, p->err, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    if (e) {
+        NCG_HIP(hipMemsetAsync(p->err, 0, sizeof(int), p->stream));
+        if (e & 1) return fail(NCG_E_SUBDIV, "number of subels. which are subdivided in a loop must <= 15!");
+        if (e & 2) return fail(NCG_E_SUBDIV, "nsbe must be < MSBE(= 220)");
+        return fail(NCG_E_SUBDIV, "adaptive subdivision deeper than 24 levels (point on an element?)");
+    }
+    return 0;
+}
+
+// d-vector helpers ------------------------------------------------------------------
+int dotv(ncg_problem *p, int ndots, const double *const *u, const double *const *v, int conj_mask, std::vector<cd> &out)
+{
+    ncgk::dots(p->stream, p->n, p->nrhs, ndots, (const void *const *)u, (const void *const *)v, conj_mask, p->partial,
+               p->dotout);
+    out.resize((size_t)p->nrhs * 3);
+    NCG_HIP(hipMemcpyAsync(out.data(), p->dotout, sizeof(cd) * 3 * p->nrhs, hipMemcpyDeviceToHost, p->stream));
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------- CGS
+// Right-preconditioned CGS of NC_IterativeSolverCGS (src/NC_CommonFunctions.cpp:1066-1208):
+// x0 = 0, unconjugated bilinear form, relative error measured against the residual
+// after the first iteration, stop below tol or at max_iterations.
+int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+{
+    const int n = p->n, nr = p->nrhs;
+    int rc = ensure_work(p, 8);
+    if (rc) return rc;
+    double *R0 = p->work[0], *R = p->work[1], *P = p->work[2], *V = p->work[3], *Q = p->work[4], *U = p->work[5],
+           *UQ = p->work[6], *AUQ = p->work[7], *X = p->x;
+    const size_t bytes = sizeof(cd) * (size_t)nr * n;
+    const double *d = nullptr;
+    if (o->precond == NCG_PRECOND_SCALING) {
+        ncgk::row_scaling(p->stream, p->A, p->precision == NCG_MIXED, n, p->dvec);
+        d = p->dvec;
+    }
+    NCG_HIP(hipMemsetAsync(X, 0, bytes, p->stream));
+    for (double *w : {R0, R, P, U}) NCG_HIP(hipMemcpyAsync(w, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+    std::vector<int> active(nr, 1);
+    NCG_HIP(hipMemcpyAsync(p->active, active.data(), sizeof(int) * nr, hipMemcpyHostToDevice, p->stream));
+    std::vector<double> err0(nr, 0.0), err(nr, 1.0);
+    std::vector<int> iters(nr, 0);
+    std::vector<cd> dv;
+    int j;
+    for (j = 0; j <= o->max_iterations; ++j) {
+        mv(p, 1, P, d, V);
+        {
+            const double *u[2] = {R, V}, *v[2] = {R0, R0};
+            ncgk::dots(p->stream, n, nr, 2, (const void *const *)u, (const void *const *)v, 0, p->partial, p->dotout);
+            ncgk::cgs_alpha(p->stream, nr, j == 0, p->dotout, p->scal, p->active);
+        }
+        ncgk::cgs_update1(p->stream, n, nr, p->scal, p->active, U, V, Q, UQ);
+        mv(p, 1, UQ, d, AUQ);
+        ncgk::cgs_update2(p->stream, n, nr, p->scal, p->active, UQ, AUQ, X, R);
+        {
+            const double *u[2] = {R, R}, *v[2] = {R, R0};
+            rc = dotv(p, 2, u, v, 1, dv);
+            if (rc) return rc;
+            ncgk::cgs_beta(p->stream, nr, p->dotout, p->scal, p->active);
+        }
+        bool any = false, changed = false;
+        for (int r = 0; r < nr; ++r) {
+            if (!active[r]) continue;
+            const double rr = sqrt(dv[3 * r].real());
+            if (j == 0) err0[r] = rr;
+            else err[r] = err0[r] > 0.0 ? rr / err0[r] : 0.0;
+            if (err[r] < o->tolerance || j == o->max_iterations) {
+                active[r] = 0;
+                iters[r] = j;
+                changed = true;
+            } else {
+                any = true;
+            }
+        }
+        if (changed)
+            NCG_HIP(hipMemcpyAsync(p->active, active.data(), sizeof(int) * nr, hipMemcpyHostToDevice, p->stream));
+        if (!any) break;
+        ncgk::cgs_update3(p->stream, n, nr, p->scal, p->active, R, Q, U, P);
+    }
+    if (d) ncgk::scale_vec(p->stream, n, nr, d, X);
+    harvest_mv(p);
+    int all = 1;
+    for (int r = 0; r < nr; ++r) {
+        if (err[r] >= o->tolerance) all = 0;
+        if (st && r < 64) {
+            st->iterations[r] = iters[r];
+            st->relative_residual[r] = err[r];
+        }
+    }
+    if (st) st->converged_all = all;
+    return all ? 0 : fail(NCG_E_NOCONV, "Warning: Maximum number of iterations is reached!");
+}
+
+// ---------------------------------------------------------------------------------- GMRES
+// Restarted GMRES(m) per load case on A x = b with Hermitian inner products, classical
+// Gram-Schmidt applied twice (FP64 dots and updates on the device), Givens rotations on
+// the host.  acc64 selects the matvec accumulation (FP32 operator: 0 = FP32 inner solve).
+// Solves for the correction in 'x' starting from its current content.
+int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, double tol, int acc64,
+          std::vector<int> &iters, std::vector<double> &relres)
+{
+    const int n = p->n, nr = p->nrhs;
+    const size_t vlen = (size_t)nr * n;
+    const int m = std::max(2, std::min(restart, 200));
+    const size_t need = (size_t)(m + 1) * vlen * 2;
+    if (need > p->basis_cap) {
+        hipFree(p->basis);
+        int rc = dalloc(&p->basis, need);
+        if (rc) return rc;
+        p->basis_cap = need;
+    }
+    int rc = ensure_work(p, 8);
+    if (rc) return rc;
+    double *W = p->work[0], *T = p->work[1];
+    double *V = p->basis;
+    std::vector<cd> coef(2 * nr), hh((size_t)nr * (m + 1)), dv;
+    std::vector<double> bnorm(nr);
+    {
+        const double *u[1] = {b}, *v[1] = {b};
+        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+        for (int r = 0; r < nr; ++r) bnorm[r] = sqrt(dv[3 * r].real());
+    }
+    iters.assign(nr, 0);
+    relres.assign(nr, 1.0);
+    std::vector<int> done(nr, 0);
+    int total = 0;
+    while (total < maxit) {
+        // r = b - A x  -> W
+        mv(p, acc64, x, nullptr, T);
+        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(0, 0); }
+        NCG_HIP(hipMemcpyAsync(W, b, sizeof(cd) * vlen, hipMemcpyDeviceToDevice, p->stream));
+        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
+        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+        ncgk::axpby(p->stream, n, nr, p->scal, T, n, W, n);
+        {
+            const double *u[1] = {W}, *v[1] = {W};
+            if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+        }
+        std::vector<double> beta(nr);
+        bool all_done = true;
+        for (int r = 0; r < nr; ++r) {
+            beta[r] = sqrt(dv[3 * r].real());
+            relres[r] = bnorm[r] > 0 ? beta[r] / bnorm[r] : 0.0;
+            done[r] = relres[r] < tol;
+            if (!done[r]) all_done = false;
+        }
+        if (all_done) break;
+        // V0 = W / beta
+        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(beta[r] > 0 ? 1.0 / beta[r] : 0.0, 0); coef[2 * r + 1] = cd(0, 0); }
+        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+        ncgk::axpby(p->stream, n, nr, p->scal, W, n, V, n);
+        std::vector<std::vector<cd>> H(nr, std::vector<cd>((size_t)(m + 1) * m));
+        std::vector<std::vector<cd>> g(nr, std::vector<cd>(m + 1)), cs(nr, std::vector<cd>(m)), sn(nr, std::vector<cd>(m));
+        std::vector<int> kk(nr, 0);
+        for (int r = 0; r < nr; ++r) g[r][0] = beta[r];
+        int k;
+        for (k = 0; k < m && total < maxit; ++k, ++total) {
+            double *Vk = V + 2 * (size_t)k * vlen;
+            mv(p, acc64, Vk, nullptr, W);
+            std::vector<cd> h1((size_t)nr * (k + 1)), h2((size_t)nr * (k + 1));
+            for (int pass = 0; pass < 2; ++pass) {
+                ncgk::multidot(p->stream, n, nr, k + 1, V, W, p->dotout);
+                NCG_HIP(hipMemcpyAsync(pass ? h2.data() : h1.data(), p->dotout, sizeof(cd) * nr * (k + 1),
+                                       hipMemcpyDeviceToHost, p->stream));
+                ncgk::multiaxpy(p->stream, n, nr, k + 1, V, p->dotout, W, 0);
+            }
+            {
+                const double *u[1] = {W}, *v[1] = {W};
+                if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+            }
+            bool all = true;
+            for (int r = 0; r < nr; ++r) {
+                const double hn = sqrt(dv[3 * r].real());
+                coef[2 * r] = cd(hn > 1e-300 ? 1.0 / hn : 0.0, 0);
+                coef[2 * r + 1] = cd(0, 0);
+                if (done[r]) continue;
+                std::vector<cd> &Hr = H[r];
+                for (int i = 0; i <= k; ++i) Hr[(size_t)i * m + k] = h1[(size_t)r * (k + 1) + i] + h2[(size_t)r * (k + 1) + i];
+                Hr[(size_t)(k + 1) * m + k] = hn;
+                for (int i = 0; i < k; ++i) {   // apply previous rotations
+                    const cd a = Hr[(size_t)i * m + k], bb = Hr[(size_t)(i + 1) * m + k];
+                    Hr[(size_t)i * m + k] = std::conj(cs[r][i]) * a + std::conj(sn[r][i]) * bb;
+                    Hr[(size_t)(i + 1) * m + k] = -sn[r][i] * a + cs[r][i] * bb;
+                }
+                const cd a = Hr[(size_t)k * m + k], bb = Hr[(size_t)(k + 1) * m + k];
+                const double den = sqrt(std::norm(a) + std::norm(bb));
+                cs[r][k] = den > 0 ? a / den : cd(1, 0);
+                sn[r][k] = den > 0 ? bb / den : cd(0, 0);
+                Hr[(size_t)k * m + k] = den;
+                Hr[(size_t)(k + 1) * m + k] = 0.0;
+                g[r][k + 1] = -sn[r][k] * g[r][k];
+                g[r][k] = std::conj(cs[r][k]) * g[r][k];
+                kk[r] = k + 1;
+                iters[r]++;
+                relres[r] = bnorm[r] > 0 ? std::abs(g[r][k + 1]) / bnorm[r] : 0.0;
+                if (relres[r] < tol) done[r] = 1;
+                if (!done[r]) all = false;
+            }
+            NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+            ncgk::axpby(p->stream, n, nr, p->scal, W, n, V + 2 * (size_t)(k + 1) * vlen, n);
+            if (all) { ++total; ++k; break; }
+        }
+        // y = H^-1 g per load case; x += V y
+        const int kmax = *std::max_element(kk.begin(), kk.end());
+        std::vector<cd> y((size_t)nr * std::max(kmax, 1), cd(0, 0));
+        for (int r = 0; r < nr; ++r) {
+            for (int i = kk[r] - 1; i >= 0; --i) {
+                cd s = g[r][i];
+                for (int c = i + 1; c < kk[r]; ++c) s -= H[r][(size_t)i * m + c] * y[(size_t)r * kmax + c];
+                y[(size_t)r * kmax + i] = s / H[r][(size_t)i * m + i];
+            }
+        }
+        if (kmax > 0) {
+            NCG_HIP(hipMemcpyAsync(p->dotout, y.data(), sizeof(cd) * nr * kmax, hipMemcpyHostToDevice, p->stream));
+            ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, x, 1);
+        }
+        bool fin = true;
+        for (int r = 0; r < nr; ++r) if (!done[r]) fin = false;
+        if (fin) break;
+    }
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int solve_gmres(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+{
+    const size_t bytes = sizeof(cd) * (size_t)p->nrhs * p->n;
+    NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
+    std::vector<int> it;
+    std::vector<double> rr;
+    int rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance,
+                   1, it, rr);
+    if (rc) return rc;
+    harvest_mv(p);
+    int all = 1;
+    for (int r = 0; r < p->nrhs; ++r) {
+        if (rr[r] >= o->tolerance) all = 0;
+        if (st && r < 64) { st->iterations[r] = it[r]; st->relative_residual[r] = rr[r]; }
+    }
+    if (st) st->converged_all = all;
+    return all ? 0 : fail(NCG_E_NOCONV, "GMRES: maximum number of iterations reached");
+}
+
+// Mixed precision: FP32 operator; inner GMRES with FP32 matvec accumulation and FP64
+// Arnoldi; outer refinement x += d with the residual b - A x accumulated in FP64.
+int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+{
+    const int n = p->n, nr = p->nrhs;
+    const size_t vlen = (size_t)nr * n, bytes = sizeof(cd) * vlen;
+    int rc = ensure_work(p, 10);
+    if (rc) return rc;
+    double *Rr = p->work[8], *Dd = p->work[9];
+    NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
+    NCG_HIP(hipMemcpyAsync(Rr, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+    std::vector<double> bnorm(nr), rel(nr, 1.0);
+    std::vector<int> total(nr, 0);
+    std::vector<cd> dv, coef(2 * nr);
+    {
+        const double *u[1] = {p->rhs}, *v[1] = {p->rhs};
+        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+        for (int r = 0; r < nr; ++r) bnorm[r] = sqrt(dv[3 * r].real());
+    }
+    const double target = o->refinement_tolerance > 0 ? o->refinement_tolerance : o->tolerance;
+    const int steps = o->refinement_steps > 0 ? o->refinement_steps : 5;
+    int s;
+    for (s = 0; s < steps; ++s) {
+        const double *u[1] = {Rr}, *v[1] = {Rr};
+        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+        bool ok = true;
+        for (int r = 0; r < nr; ++r) {
+            rel[r] = bnorm[r] > 0 ? sqrt(dv[3 * r].real()) / bnorm[r] : 0.0;
+            if (rel[r] >= target) ok = false;
+        }
+        if (ok) break;
+        // inner solve A d = r in FP32 matvecs to ~1e-5 relative (FP32 operator accuracy)
+        NCG_HIP(hipMemsetAsync(Dd, 0, bytes, p->stream));
+        std::vector<int> it;
+        std::vector<double> rr;
+        const double inner = std::max(1e-5, target * 0.5);
+        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, it, rr))) return rc;
+        for (int r = 0; r < nr; ++r) total[r] += it[r];
+        // x += d ; r = b - A x (FP64 accumulation)
+        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(1, 0); }
+        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+        ncgk::axpby(p->stream, n, nr, p->scal, Dd, n, p->x, n);
+        mv(p, 1, p->x, nullptr, Dd);
+        NCG_HIP(hipMemcpyAsync(Rr, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
+        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+        ncgk::axpby(p->stream, n, nr, p->scal, Dd, n, Rr, n);
+    }
+    harvest_mv(p);
+    int all = 1;
+    for (int r = 0; r < nr; ++r) {
+        if (rel[r] >= target) all = 0;
+        if (st && r < 64) { st->iterations[r] = total[r]; st->relative_residual[r] = rel[r]; }
+    }
+    if (st) { st->converged_all = all; st->refinement_steps = s; }
+    return all ? 0 : fail(NCG_E_NOCONV, "mixed-precision refinement did not reach its tolerance");
+}
+
+}  // namespace
+
+// ==================================================================================== API
+
+extern "C" {
+
+const char *ncg_last_error(void) { return g_err.c_str(); }
+
+int ncg_device_count(int32_t *count)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return 0;
+}
+
+int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
+{
+    if (!g || !out || g->n_elements <= 0 || !g->nodes_per_element || !g->node_xyz || !g->bc_kind)
+        return fail(NCG_E_ARG, "ncg_problem_create: missing geometry");
+    if (g->n_eval > 0 && !g->eval_xyz) return fail(NCG_E_ARG, "ncg_problem_create: missing evaluation points");
+    const int n = g->n_elements;
+    std::vector<int8_t> nvh(n), kh(n);
+    for (int e = 0; e < n; ++e) {
+        if (g->nodes_per_element[e] != 3 && g->nodes_per_element[e] != 4)
+            return fail(NCG_E_ARG, "number of nodes of an element must be 3 or 4!");
+        if (g->bc_kind[e] < 0 || g->bc_kind[e] > 2) return fail(NCG_E_ARG, "bc_kind must be 0, 1 or 2");
+        nvh[e] = (int8_t)g->nodes_per_element[e];
+        kh[e] = (int8_t)g->bc_kind[e];
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NCG_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(NCG_E_ARG, "device index out of range");
+    NCG_HIP(hipSetDevice(device));
+    ncg_problem *p = new ncg_problem();
+    p->device = device;
+    p->n = n;
+    p->m = g->n_eval > 0 ? g->n_eval : 0;
+    p->h = g->harmonic_time_factor == -1.0 ? -1.0 : 1.0;
+    p->interior = g->interior ? 1 : 0;
+    p->kind_h = kh;
+    auto bail = [&](int rc) {
+        ncg_problem_destroy(p);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(NCG_E_DEVICE, "hipStreamCreate failed"));
+    for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
+        hipEventCreate(&e->a);
+        hipEventCreate(&e->b);
+    }
+    double glx[36], glw[36];
+    gauss_tables(glx, glw);
+    if (ncgk::set_gauss_tables(glx, glw) != hipSuccess) return bail(fail(NCG_E_DEVICE, "constant table upload failed"));
+    std::vector<double> Xh((size_t)12 * n, 0.0);
+    for (int e = 0; e < n; ++e)
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) Xh[(size_t)(3 * a + d) * n + e] = g->node_xyz[(size_t)e * 12 + 3 * a + d];
+    int rc;
+    if ((rc = dalloc(&p->X, 12 * (size_t)n)) || (rc = dalloc(&p->cen, 3 * (size_t)n)) ||
+        (rc = dalloc(&p->nrm, 3 * (size_t)n)) || (rc = dalloc(&p->area, n)) || (rc = dalloc(&p->jac, n)) ||
+        (rc = dalloc(&p->nv, n)) || (rc = dalloc(&p->kind, n)) || (rc = dalloc(&p->eval, 3 * (size_t)p->m)) ||
+        (rc = dalloc(&p->adm, 2 * (size_t)n)) || (rc = dalloc(&p->err, 1)) || (rc = dalloc(&p->evals, 1)) ||
+        (rc = dalloc(&p->dvec, n)) || (rc = dalloc(&p->active, 64)))
+        return bail(rc);
+    const int nb = ncgk::dot_blocks(n);
+    if ((rc = dalloc(&p->partial, (size_t)2 * nb * 3 * 64)) || (rc = dalloc(&p->dotout, (size_t)2 * 3 * 64 * 256)) ||
+        (rc = dalloc(&p->scal, (size_t)2 * 8 * 64 * 2)))
+        return bail(rc);
+    hipStream_t s = p->stream;
+    hipMemcpyAsync(p->X, Xh.data(), sizeof(double) * Xh.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(p->nv, nvh.data(), n, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(p->kind, kh.data(), n, hipMemcpyHostToDevice, s);
+    if (p->m) hipMemcpyAsync(p->eval, g->eval_xyz, sizeof(double) * 3 * p->m, hipMemcpyHostToDevice, s);
+    hipMemsetAsync(p->err, 0, sizeof(int), s);
+    ncgk::element_geometry(s, n, p->X, p->nv, p->cen, p->nrm, p->area, p->jac);
+    // near-pair table
+    int *cnt = nullptr;
+    long long *off = nullptr;
+    if ((rc = dalloc(&cnt, n)) || (rc = dalloc(&off, (size_t)n + 1))) { hipFree(cnt); return bail(rc); }
+    ncgk::near_count(s, p->elements(), cnt);
+    std::vector<int> ch(n);
+    hipMemcpyAsync(ch.data(), cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) { hipFree(cnt); hipFree(off); return bail(fail(NCG_E_DEVICE, "geometry kernels failed")); }
+    std::vector<long long> oh((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) oh[i + 1] = oh[i] + ch[i];
+    p->nnear = oh[n];
+    if ((rc = dalloc(&p->near_rows, (size_t)p->nnear)) || (rc = dalloc(&p->near_cols, (size_t)p->nnear))) {
+        hipFree(cnt); hipFree(off); return bail(rc);
+    }
+    hipMemcpyAsync(off, oh.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s);
+    ncgk::near_fill(s, p->elements(), off, p->near_rows, p->near_cols);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(cnt);
+    hipFree(off);
+    if (e != hipSuccess) return bail(fail(NCG_E_DEVICE, std::string("near table: ") + hipGetErrorString(e)));
+    *out = p;
+    return 0;
+}
+
+int ncg_problem_destroy(ncg_problem *p)
+{
+    if (!p) return 0;
+    hipSetDevice(p->device);
+    if (p->stream) hipStreamSynchronize(p->stream);
+    for (void *q : {(void *)p->X, (void *)p->cen, (void *)p->nrm, (void *)p->area, (void *)p->jac, (void *)p->eval,
+                    (void *)p->nv, (void *)p->kind, (void *)p->near_rows, (void *)p->near_cols, p->A, (void *)p->adm,
+                    (void *)p->bcval, (void *)p->rhs, (void *)p->rhs_src, (void *)p->x, (void *)p->ps, (void *)p->pw,
+                    (void *)p->src, (void *)p->err, (void *)p->evals, (void *)p->scal, (void *)p->partial,
+                    (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis})
+        if (q) hipFree(q);
+    for (double *w : p->work) hipFree(w);
+    for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
+        if (e->a) hipEventDestroy(e->a);
+        if (e->b) hipEventDestroy(e->b);
+    }
+    for (EventPair &e : p->ev_mv) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    if (p->stream) hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
+{
+    if (!p || !f) return fail(NCG_E_ARG, "ncg_assemble: null argument");
+    if (!(f->k > 0.0)) return fail(NCG_E_ARG, "ncg_assemble: wave number must be positive");
+    if (f->n_rhs < 1 || f->n_rhs > 64) return fail(NCG_E_ARG, "ncg_assemble: n_rhs must be in [1, 64]");
+    if (precision != NCG_FP64 && precision != NCG_MIXED) return fail(NCG_E_ARG, "ncg_assemble: unknown precision");
+    if ((f->n_point_sources > 0 && !f->point_sources) || (f->n_plane_waves > 0 && !f->plane_waves))
+        return fail(NCG_E_ARG, "ncg_assemble: missing source data");
+    NCG_HIP(hipSetDevice(p->device));
+    const int n = p->n, nr = f->n_rhs;
+    hipStream_t s = p->stream;
+    p->k = f->k;
+    p->nrhs = nr;
+    p->precision = precision;
+    p->assembled = p->solved = 0;
+    int rc;
+    if ((rc = ensure_vectors(p, nr))) return rc;
+    const size_t abytes = (size_t)n * n * (precision == NCG_MIXED ? 8 : 16);
+    if (abytes > p->A_cap) {
+        hipFree(p->A);
+        p->A = nullptr;
+        p->A_cap = 0;
+        if (hipMalloc(&p->A, abytes) != hipSuccess) return fail(NCG_E_MEMORY, "cannot allocate the dense operator");
+        p->A_cap = abytes;
+    }
+    // admittances only where the element kind says so; prescribed values per load case
+    std::vector<double> admh(2 * (size_t)n, 0.0);
+    if (f->admittance)
+        for (int e = 0; e < n; ++e)
+            if (p->kind_h[e] == NCG_BC_ADMITTANCE) { admh[2 * e] = f->admittance[2 * e]; admh[2 * e + 1] = f->admittance[2 * e + 1]; }
+    std::vector<double> bch(2 * (size_t)nr * n, 0.0);
+    if (f->bc_value) memcpy(bch.data(), f->bc_value, sizeof(double) * bch.size());
+    // source columns: elements with a nonzero prescribed value in some load case
+    // (the Ifcrh3 test of src/NC_EquationSystem.cpp:268-279)
+    std::vector<int> srch;
+    for (int e = 0; e < n; ++e)
+        for (int r = 0; r < nr; ++r) {
+            const double re = bch[2 * ((size_t)r * n + e)], im = bch[2 * ((size_t)r * n + e) + 1];
+            if (sqrt(re * re + im * im) > 1.0e-14) { srch.push_back(e); break; }
+        }
+    if ((int)srch.size() > p->nsrc || !p->src) {
+        hipFree(p->src);
+        if ((rc = dalloc(&p->src, srch.size() + 1))) return rc;
+    }
+    p->nsrc = (int)srch.size();
+    auto upload_src = [&](double **dst, int *cap, const double *srcp, int cnt) -> int {
+        if (cnt > *cap || !*dst) {
+            hipFree(*dst);
+            int r2 = dalloc(dst, 6 * (size_t)std::max(cnt, 1));
+            if (r2) return r2;
+            *cap = std::max(cnt, 1);
+        }
+        if (cnt) hipMemcpyAsync(*dst, srcp, sizeof(double) * 6 * cnt, hipMemcpyHostToDevice, s);
+        return 0;
+    };
+    for (int q = 0; q < f->n_point_sources; ++q)
+        if (f->point_sources[6 * q + 5] < 0 || f->point_sources[6 * q + 5] >= nr)
+            return fail(NCG_E_ARG, "point source rhs index out of range");
+    for (int q = 0; q < f->n_plane_waves; ++q)
+        if (f->plane_waves[6 * q + 5] < 0 || f->plane_waves[6 * q + 5] >= nr)
+            return fail(NCG_E_ARG, "plane wave rhs index out of range");
+    if ((rc = upload_src(&p->ps, &p->ps_cap, f->point_sources, f->n_point_sources))) return rc;
+    if ((rc = upload_src(&p->pw, &p->pw_cap, f->plane_waves, f->n_plane_waves))) return rc;
+    p->nps = f->n_point_sources;
+    p->npw = f->n_plane_waves;
+    NCG_HIP(hipMemcpyAsync(p->adm, admh.data(), sizeof(double) * admh.size(), hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(p->bcval, bch.data(), sizeof(double) * bch.size(), hipMemcpyHostToDevice, s));
+    if (p->nsrc) NCG_HIP(hipMemcpyAsync(p->src, srch.data(), sizeof(int) * p->nsrc, hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemsetAsync(p->evals, 0, sizeof(unsigned long long), s));
+    const ncgk::BMParams b = p->bm();
+    const int mixed = precision == NCG_MIXED;
+    record(p, p->ev_asm.a);
+    record(p, p->ev_far.a);
+    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->evals);
+    record(p, p->ev_far.b);
+    ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err);
+    if (p->nsrc) ncgk::rhs_sources(s, p->elements(), b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err);
+    ncgk::assemble_diag(s, p->elements(), b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src,
+                        p->nsrc > 0, p->A, mixed, p->rhs);
+    record(p, p->ev_asm.b);
+    NCG_HIP(hipGetLastError());
+    if ((rc = check_err(p))) return rc;
+    unsigned long long ev = 0;
+    NCG_HIP(hipMemcpy(&ev, p->evals, sizeof(ev), hipMemcpyDeviceToHost));
+    p->last_evals = (double)ev;
+    p->t_asm = elapsed(p->ev_asm);
+    p->far_ms_sum += elapsed(p->ev_far);
+    p->far_calls++;
+    p->assembled = 1;
+    return 0;
+}
+
+int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+{
+    if (!p || !o) return fail(NCG_E_ARG, "ncg_solve: null argument");
+    if (!p->assembled) return fail(NCG_E_STATE, "ncg_solve: assemble first");
+    if (o->max_iterations < 0) return fail(NCG_E_ARG, "ncg_solve: negative iteration limit");
+    NCG_HIP(hipSetDevice(p->device));
+    if (st) memset(st, 0, sizeof(*st));
+    record(p, p->ev_solve.a);
+    int rc;
+    if (o->solver == NCG_SOLVER_LU) {
+        if (p->precision != NCG_FP64) return fail(NCG_E_ARG, "the direct solver needs the FP64 operator");
+        const size_t bytes = sizeof(cd) * (size_t)p->nrhs * p->n;
+        NCG_HIP(hipMemcpyAsync(p->x, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+        ncgk::lu_factor(p->stream, p->A, p->n);
+        ncgk::lu_solve(p->stream, p->A, p->n, p->nrhs, p->x);
+        NCG_HIP(hipGetLastError());
+        p->assembled = 0;   // the operator now holds its factors
+        rc = 0;
+        if (st) st->converged_all = 1;
+    } else if (p->precision == NCG_MIXED) {
+        rc = solve_mixed(p, o, st);
+    } else if (o->solver == NCG_SOLVER_GMRES) {
+        rc = solve_gmres(p, o, st);
+    } else {
+        rc = solve_cgs(p, o, st);
+    }
+    record(p, p->ev_solve.b);
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    p->t_solve = elapsed(p->ev_solve);
+    if (st) {
+        st->ms_assemble = p->t_asm;
+        st->ms_solve = p->t_solve;
+        st->ms_field = p->t_field;
+    }
+    if (rc == 0 || rc == NCG_E_NOCONV) p->solved = 1;
+    return rc;
+}
+
+int ncg_get_solution(ncg_problem *p, double *x)
+{
+    if (!p || !x) return fail(NCG_E_ARG, "ncg_get_solution: null argument");
+    if (!p->solved) return fail(NCG_E_STATE, "ncg_get_solution: solve first");
+    NCG_HIP(hipSetDevice(p->device));
+    NCG_HIP(hipMemcpy(x, p->x, sizeof(cd) * (size_t)p->nrhs * p->n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ncg_field(ncg_problem *p, double *phi, double *grad)
+{
+    if (!p || !phi) return fail(NCG_E_ARG, "ncg_field: null argument");
+    if (!p->solved) return fail(NCG_E_STATE, "ncg_field: solve first");
+    if (p->m == 0) return 0;
+    NCG_HIP(hipSetDevice(p->device));
+    const int n = p->n, m = p->m, nr = p->nrhs;
+    int rc = ensure_work(p, 8);
+    if (rc) return rc;
+    double *pot = p->work[0], *vel = p->work[1];
+    double *dphi = nullptr, *dgrad = nullptr;
+    if ((rc = dalloc(&dphi, 2 * (size_t)nr * m)) || (rc = dalloc(&dgrad, 6 * (size_t)nr * m))) {
+        hipFree(dphi);
+        return rc;
+    }
+    record(p, p->ev_field.a);
+    ncgk::field_prep(p->stream, n, nr, p->kind, p->adm, p->interior ? -1.0 : 1.0, p->bcval, p->x, pot, vel);
+    ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
+                grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
+    record(p, p->ev_field.b);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(phi, dphi, sizeof(cd) * nr * m, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, dgrad, sizeof(cd) * 3 * nr * m, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    hipFree(dphi);
+    hipFree(dgrad);
+    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("ncg_field: ") + hipGetErrorString(e));
+    p->t_field = elapsed(p->ev_field);
+    return check_err(p);
+}
+
+int ncg_get_system(ncg_problem *p, double *A, double *b)
+{
+    if (!p) return fail(NCG_E_ARG, "ncg_get_system: null argument");
+    if (!p->assembled) return fail(NCG_E_STATE, "ncg_get_system: assemble first");
+    NCG_HIP(hipSetDevice(p->device));
+    const size_t nn = (size_t)p->n * p->n;
+    if (A) {
+        if (p->precision == NCG_MIXED) {
+            std::vector<float> tmp(2 * nn);
+            NCG_HIP(hipMemcpy(tmp.data(), p->A, sizeof(float) * 2 * nn, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 2 * nn; ++i) A[i] = tmp[i];
+        } else {
+            NCG_HIP(hipMemcpy(A, p->A, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost));
+        }
+    }
+    if (b) NCG_HIP(hipMemcpy(b, p->rhs, sizeof(cd) * (size_t)p->nrhs * p->n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
+{
+    if (!p || !x || !y) return fail(NCG_E_ARG, "ncg_apply: null argument");
+    if (!p->assembled) return fail(NCG_E_STATE, "ncg_apply: assemble first");
+    if (n_rhs < 1 || n_rhs > p->nrhs) return fail(NCG_E_ARG, "ncg_apply: n_rhs must be in [1, assembled n_rhs]");
+    NCG_HIP(hipSetDevice(p->device));
+    int rc = ensure_work(p, 8);
+    if (rc) return rc;
+    const size_t bytes = sizeof(cd) * (size_t)n_rhs * p->n;
+    const int keep = p->nrhs;
+    p->nrhs = n_rhs;
+    NCG_HIP(hipMemcpyAsync(p->work[0], x, bytes, hipMemcpyHostToDevice, p->stream));
+    mv(p, 1, p->work[0], nullptr, p->work[1]);
+    p->nrhs = keep;
+    NCG_HIP(hipMemcpyAsync(y, p->work[1], bytes, hipMemcpyDeviceToHost, p->stream));
+    harvest_mv(p);
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int ncg_timings(ncg_problem *p, double *t, int32_t reset)
+{
+    if (!p || !t) return fail(NCG_E_ARG, "ncg_timings: null argument");
+    t[0] = p->t_asm;
+    t[1] = p->t_solve;
+    t[2] = p->t_field;
+    t[3] = p->far_calls ? p->far_ms_sum / p->far_calls : 0.0;
+    t[4] = p->mv_calls ? p->mv_ms_sum / p->mv_calls : 0.0;
+    t[5] = (double)p->mv_calls;
+    t[6] = p->last_evals;
+    t[7] = (double)p->n * p->n * (p->precision == NCG_MIXED ? 8.0 : 16.0);
+    if (reset) {
+        p->far_ms_sum = p->mv_ms_sum = 0.0;
+        p->far_calls = p->mv_calls = 0;
+    }
+    return 0;
+}
+
+}  // extern "C"

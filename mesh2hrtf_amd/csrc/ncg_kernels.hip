@@ -1,0 +1,936 @@
+// libncg kernels for gfx950 (wave64).  One translation unit holds every kernel so the
+// quadrature tables in constant memory exist once per code object.
+//
+// Layouts in HBM (all row-major, complex interleaved):
+//   A      [n][n]  cx<S>, S = double (NCG_FP64) or float (NCG_MIXED)
+//   vectors [nrhs][n] cx<double>
+//   element data: structure of arrays (see ncg::Elements)
+#include "ncg_kernels.hpp"
+
+#include "ncg_device.hpp"
+
+namespace ncg {
+
+// ======================================================================================
+// geometry: centre, unit normal, area, Jacobian per element (NC_ReadMesh,
+// src/NC_Input.cpp:522-605; triangle centre s = t = 1/3 and area 0.5*J, quad centre
+// (0, 0) and area 4*J)
+// ======================================================================================
+__global__ void k_element_geometry(int n, const double *X, const int8_t *nv, double *cen, double *nrm,
+                                   double *area, double *jac)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    Elements E{};
+    E.n = n;
+    E.X = X;
+    LocalElem L;
+    L.nv = nv[e];
+    for (int a = 0; a < L.nv; ++a)
+        for (int d = 0; d < 3; ++d) L.p[a][d] = X[(3 * a + d) * n + e];
+    const double s = L.nv == 3 ? 1.0 / 3.0 : 0.0;
+    double q[3], ny[3], J;
+    if (L.nv == 3) {
+        // NC_ComputeUnitVectorOnElement for a flat triangle: dx/ds = x0 - x2, dx/dt = x1 - x2
+        double xs[3], xt[3];
+        for (int d = 0; d < 3; ++d) { xs[d] = L.p[0][d] - L.p[2][d]; xt[d] = L.p[1][d] - L.p[2][d]; }
+        double n0 = xs[1] * xt[2] - xs[2] * xt[1], n1 = xs[2] * xt[0] - xs[0] * xt[2], n2 = xs[0] * xt[1] - xs[1] * xt[0];
+        J = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        ny[0] = n0 / J; ny[1] = n1 / J; ny[2] = n2 / J;
+        for (int d = 0; d < 3; ++d) q[d] = s * L.p[0][d] + s * L.p[1][d] + (1.0 - s - s) * L.p[2][d];
+    } else {
+        L.jac = 0.0;
+        point_on(L, s, s, q, ny, J);
+    }
+    for (int d = 0; d < 3; ++d) { cen[d * n + e] = q[d]; nrm[d * n + e] = ny[d]; }
+    area[e] = J * (L.nv == 3 ? 0.5 : 4.0);
+    jac[e] = J;
+}
+
+// ======================================================================================
+// near-pair table: pairs (i, j != i) whose element j must be subdivided as seen from
+// the collocation point of i (ratio < 1.3 at level 0).  Frequency independent.
+// ======================================================================================
+__global__ void k_near_count(Elements E, int *count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E.n) return;
+    const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
+    int c = 0;
+    for (int j = 0; j < E.n; ++j) c += (j != i && pair_is_near(E, j, x)) ? 1 : 0;
+    count[i] = c;
+}
+
+__global__ void k_near_fill(Elements E, const long long *offset, int *rows, int *cols)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E.n) return;
+    const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
+    long long o = offset[i];
+    for (int j = 0; j < E.n; ++j)
+        if (j != i && pair_is_near(E, j, x)) { rows[o] = i; cols[o] = j; ++o; }
+}
+
+// ======================================================================================
+// far pairs: the bulk of the dense operator.  Block = 256 threads = 4 waves, one column
+// element per thread (held in registers), rows_per_block collocation rows walked by the
+// whole block (row data is wave-uniform -> scalar loads).  Pairs that are near or on
+// the diagonal are left to k_assemble_near / k_assemble_diag.  Writes are coalesced:
+// consecutive lanes write consecutive columns of one row.
+// ======================================================================================
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, const double *adm,
+                                                       cx<S> *A, int rows_per_block, unsigned long long *evals)
+{
+    const int n = E.n;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.y * rows_per_block;
+    const int i1 = min(i0 + rows_per_block, n);
+    if (j >= n) return;
+    const int nv = E.nv[j];
+    const int kind = E.kind[j];
+    const cx<T> admj = {T(adm[2 * j]), T(adm[2 * j + 1])};
+    const double cj[3] = {E.cen[j], E.cen[n + j], E.cen[2 * n + j]};
+    const double areaj = E.area[j];
+    const T nyx = T(E.nrm[j]), nyy = T(E.nrm[n + j]), nyz = T(E.nrm[2 * n + j]);
+    const T jacj = T(E.jac[j]);
+    double P[4][3];
+    for (int a = 0; a < nv; ++a)
+        for (int d = 0; d < 3; ++d) P[a][d] = E.X[(3 * a + d) * n + j];
+    // triangle: x(s,t) = x2 + s (x0 - x2) + t (x1 - x2)
+    const T e1x = T(P[0][0] - P[2][0]), e1y = T(P[0][1] - P[2][1]), e1z = T(P[0][2] - P[2][2]);
+    const T e2x = T(P[1][0] - P[2][0]), e2y = T(P[1][1] - P[2][1]), e2z = T(P[1][2] - P[2][2]);
+    const T hk = T(bm.hk), k2 = T(bm.k2), tau = T(bm.tau), bim = T(bm.beta_im);
+    unsigned long long nev = 0;
+    for (int i = i0; i < i1; ++i) {
+        if (i == j) continue;
+        const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
+        const double ratdis = ratio_to_centre(cj[0], cj[1], cj[2], x, areaj);
+        if (ratdis < kTolf) continue;
+        const T nxx = T(E.nrm[i]), nxy = T(E.nrm[n + i]), nxz = T(E.nrm[2 * n + i]);
+        const int m = gauss_order(ratdis);
+        BMAcc<T> acc;
+        acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
+        if (nv == 3) {
+            const T px = T(P[2][0] - x[0]), py = T(P[2][1] - x[1]), pz = T(P[2][2] - x[2]);
+            const int off = tri_rule_offset(m), cnt = tri_rule_count(m);
+            for (int g = 0; g < cnt; ++g) {
+                const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), w = T(c_tri[off + g][2]);
+                const T qx = px + s * e1x + t * e2x, qy = py + s * e1y + t * e2y, qz = pz + s * e1z + t * e2z;
+                bm_point<T>(acc, qx, qy, qz, nyx, nyy, nyz, nxx, nxy, nxz, w * jacj, hk, k2);
+            }
+            nev += cnt;
+        } else {
+            LocalElem L;
+            L.nv = 4;
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) L.p[a][d] = P[a][d] - x[d];
+            const int base = m * (m - 1) / 2;
+            for (int u = 0; u < m; ++u)
+                for (int v = 0; v < m; ++v) {
+                    double q[3], ny[3], J;
+                    point_on(L, c_gl_x[base + u], c_gl_x[base + v], q, ny, J);
+                    const double w = c_gl_w[base + u] * c_gl_w[base + v] * J;
+                    bm_point<T>(acc, T(q[0]), T(q[1]), T(q[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz, T(w), hk, k2);
+                }
+            nev += m * m;
+        }
+        const cx<T> a = bm_entry<T>(acc, kind, admj, tau, bim);
+        A[(size_t)i * n + j] = cx<S>{S(a.re), S(a.im)};
+    }
+    if (evals) atomicAdd(evals, nev);
+}
+
+// ======================================================================================
+// near pairs: adaptive subdivision in FP64, one thread per pair
+// ======================================================================================
+template <typename S>
+__global__ void __launch_bounds__(128) k_assemble_near(Elements E, BMConst bm, const double *adm, const int *rows,
+                                                        const int *cols, long long npairs, cx<S> *A, int *err)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const int i = rows[p], j = cols[p];
+    BMAcc<double> acc;
+    const int e = bm_pair(E, i, j, bm.hk, bm.k2, acc);
+    if (e) atomicOr(err, e);
+    const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+    A[(size_t)i * E.n + j] = cx<S>{S(a.re), S(a.im)};
+}
+
+// ======================================================================================
+// right-hand-side contributions of prescribed boundary values: b_r[i] += sum over the
+// source columns j (elements with a nonzero prescribed value in some load case) of
+// c_ij * bc_r[j].  One wave per row, lanes over the source columns; FP64.
+// ======================================================================================
+constexpr int kRhsChunk = 8;
+
+__global__ void __launch_bounds__(256) k_rhs_sources(Elements E, BMConst bm, const int *src, int nsrc,
+                                                      const cx<double> *bcval, int nrhs, int rhs0,
+                                                      cx<double> *rhs_src, int *err)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= E.n) return;
+    const int nr = min(kRhsChunk, nrhs - rhs0);
+    cx<double> part[kRhsChunk];
+    for (int r = 0; r < kRhsChunk; ++r) part[r] = mk(0.0, 0.0);
+    for (int q = lane; q < nsrc; q += 64) {
+        const int j = src[q];
+        BMAcc<double> acc;
+        const int e = bm_pair(E, i, j, bm.hk, bm.k2, acc);
+        if (e) atomicOr(err, e);
+        const cx<double> c = bm_rhs_coef<double>(acc, E.kind[j], bm.tau, bm.beta_im);
+        for (int r = 0; r < nr; ++r) part[r] += c * bcval[(size_t)(rhs0 + r) * E.n + j];
+    }
+    for (int r = 0; r < nr; ++r) {
+        double re = part[r].re, im = part[r].im;
+        for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o);
+            im += __shfl_xor(im, o);
+        }
+        if (lane == 0) rhs_src[(size_t)(rhs0 + r) * E.n + i] = mk(re, im);
+    }
+}
+
+// ======================================================================================
+// diagonal: singular integrals + free terms, and the right-hand sides
+// (NC_ComputeEntriesForTBEM free terms src/NC_EquationSystem.cpp:218-237 and
+// NC_IncidentWaveRHS src/NC_EquationSystem.cpp:1731-1812)
+// ======================================================================================
+__global__ void __launch_bounds__(64) k_assemble_diag(Elements E, BMConst bm, const double *adm, const cx<double> *bcval,
+                                                      int nrhs, const double *ps, int nps, const double *pw, int npw,
+                                                      const cx<double> *rhs_src, int has_src, void *Aout,
+                                                      int mixed, cx<double> *b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E.n) return;
+    const int n = E.n;
+    BMAcc<double> acc;
+    bm_pair(E, i, i, bm.hk, bm.k2, acc);
+    const int kind = E.kind[i];
+    const cx<double> admi = mk(adm[2 * i], adm[2 * i + 1]);
+    const cx<double> beta = mk(0.0, bm.beta_im);
+    cx<double> a = bm_entry<double>(acc, kind, admi, bm.tau, bm.beta_im);
+    if (kind == 1) a -= beta * (0.5 * bm.tau);
+    else a += (admi * beta - mk(1.0, 0.0)) * 0.5;
+    if (mixed) ((cx<float> *)Aout)[(size_t)i * n + i] = cx<float>{float(a.re), float(a.im)};
+    else ((cx<double> *)Aout)[(size_t)i * n + i] = a;
+
+    const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
+    const double nx[3] = {E.nrm[i], E.nrm[n + i], E.nrm[2 * n + i]};
+    const double hk = bm.hk;
+    const cx<double> btau = beta * bm.tau;
+    for (int r = 0; r < nrhs; ++r) {
+        const cx<double> v = bcval[(size_t)r * n + i];
+        cx<double> z = kind == 1 ? v * (bm.tau * 0.5) : v * beta * (bm.tau * 0.5);
+        for (int w = 0; w < npw; ++w) {
+            const double *d = pw + 6 * w;
+            if ((int)d[5] != r) continue;
+            const double ph = (d[0] * x[0] + d[1] * x[1] + d[2] * x[2]) * hk;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const cx<double> g = mk(cs, sn) * mk(d[3], d[4]);
+            const double c = hk * (d[0] * nx[0] + d[1] * nx[1] + d[2] * nx[2]);
+            const cx<double> vv = mk(-g.im * c, g.re * c);
+            z -= g + vv * btau;
+        }
+        for (int s = 0; s < nps; ++s) {
+            const double *y = ps + 6 * s;
+            if ((int)y[5] != r) continue;
+            double v2[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+            const double R = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+            v2[0] /= R; v2[1] /= R; v2[2] /= R;
+            double sn, cs;
+            sincos(hk * R, &sn, &cs);
+            const double f = 1.0 / kPi4 / R;
+            const cx<double> g = mk(cs * f, sn * f) * mk(y[3], y[4]);
+            const double rn = v2[0] * nx[0] + v2[1] * nx[1] + v2[2] * nx[2];
+            const cx<double> vv = mk(-rn / R, hk * rn) * g;
+            z -= g + vv * btau;
+        }
+        if (has_src) z += rhs_src[(size_t)r * n + i];
+        b[(size_t)r * n + i] = z;
+    }
+}
+
+// ======================================================================================
+// Krylov primitives
+// ======================================================================================
+// y[r] = A (d .* x[r]) for r < nr (<= 4), wave = 4 rows, block = 4 waves = 16 rows.
+// Lanes stride the columns: each load instruction reads 64 consecutive entries of a row
+// (1 KiB FP64, 512 B FP32), so HBM traffic is the operator once per call plus the
+// vectors once per 16 rows.
+template <typename S, typename Acc, int NR>
+__global__ void __launch_bounds__(256) k_matvec(const cx<S> *__restrict__ A, int n, const cx<double> *__restrict__ x,
+                                                 const double *__restrict__ d, cx<double> *__restrict__ y)
+{
+    constexpr int ROWS = 4;
+    const int lane = threadIdx.x & 63;
+    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * ROWS;
+    if (row0 >= n) return;
+    cx<Acc> acc[ROWS][NR];
+#pragma unroll
+    for (int a = 0; a < ROWS; ++a)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[a][r] = mk(Acc(0), Acc(0));
+    const cx<S> *rowp[ROWS];
+#pragma unroll
+    for (int a = 0; a < ROWS; ++a) rowp[a] = A + (size_t)min(row0 + a, n - 1) * n;
+    for (int j = lane; j < n; j += 64) {
+        cx<Acc> xv[NR];
+        const double dj = d ? d[j] : 1.0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const cx<double> v = x[(size_t)r * n + j];
+            xv[r] = mk(Acc(v.re * dj), Acc(v.im * dj));
+        }
+#pragma unroll
+        for (int a = 0; a < ROWS; ++a) {
+            const cx<S> e = rowp[a][j];
+            const cx<Acc> ea = mk(Acc(e.re), Acc(e.im));
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[a][r] += ea * xv[r];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < ROWS; ++a)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double re = double(acc[a][r].re), im = double(acc[a][r].im);
+            for (int o = 32; o > 0; o >>= 1) {
+                re += __shfl_xor(re, o);
+                im += __shfl_xor(im, o);
+            }
+            if (lane == 0 && row0 + a < n) y[(size_t)r * n + row0 + a] = mk(re, im);
+        }
+}
+
+// Block partial sums of up to 3 bilinear forms per load case:
+//   out[b][r][q] = sum_i u_q[r][i] * v_q[r][i]   (bit q of conj_mask: conjugate u_q)
+// Deterministic: fixed partition, reduced by k_reduce_partials.
+constexpr int kDotBlocks = 256;
+
+__global__ void __launch_bounds__(256) k_dots(int n, int nrhs, int ndots, const cx<double> *u0, const cx<double> *v0,
+                                              const cx<double> *u1, const cx<double> *v1, const cx<double> *u2,
+                                              const cx<double> *v2, int conj_mask, cx<double> *partial)
+{
+    __shared__ double sre[256], sim[256];
+    const cx<double> *U[3] = {u0, u1, u2}, *V[3] = {v0, v1, v2};
+    for (int r = 0; r < nrhs; ++r)
+        for (int q = 0; q < ndots; ++q) {
+            double re = 0.0, im = 0.0;
+            const cx<double> *u = U[q] + (size_t)r * n, *v = V[q] + (size_t)r * n;
+            for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+                cx<double> a = u[i];
+                if ((conj_mask >> q) & 1) a.im = -a.im;
+                const cx<double> p = a * v[i];
+                re += p.re;
+                im += p.im;
+            }
+            sre[threadIdx.x] = re;
+            sim[threadIdx.x] = im;
+            __syncthreads();
+            for (int s = 128; s > 0; s >>= 1) {
+                if (threadIdx.x < s) {
+                    sre[threadIdx.x] += sre[threadIdx.x + s];
+                    sim[threadIdx.x] += sim[threadIdx.x + s];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) partial[((size_t)blockIdx.x * nrhs + r) * 3 + q] = mk(sre[0], sim[0]);
+            __syncthreads();
+        }
+}
+
+// out[r][q] = sum over blocks of partial; one block.
+__global__ void k_reduce_partials(int nblocks, int nrhs, int ndots, const cx<double> *partial, cx<double> *out)
+{
+    for (int t = threadIdx.x; t < nrhs * ndots; t += blockDim.x) {
+        const int r = t / ndots, q = t % ndots;
+        cx<double> s = mk(0.0, 0.0);
+        for (int b = 0; b < nblocks; ++b) s += partial[((size_t)b * nrhs + r) * 3 + q];
+        out[r * 3 + q] = s;
+    }
+}
+
+// CGS scalars, one thread per load case.  sc layout per r (complex):
+// [0] rho0 [1] rho1 [2] sigma [3] alpha [4] beta [5] (rr, -) [6] err0 [7] flags
+__global__ void k_cgs_alpha(int nrhs, int first, const cx<double> *dots, cx<double> *sc, const int *active)
+{
+    const int r = threadIdx.x;
+    if (r >= nrhs || !active[r]) return;
+    cx<double> *s = sc + 8 * r;
+    // dots: [r][0] = <R,R0> (only valid in the first iteration), [r][1] = <V,R0>
+    if (first) s[0] = dots[r * 3 + 0];
+    else s[0] = s[1];
+    s[2] = dots[r * 3 + 1];
+    s[3] = cdiv(s[0], s[2]);
+}
+
+__global__ void k_cgs_beta(int nrhs, const cx<double> *dots, cx<double> *sc, const int *active)
+{
+    const int r = threadIdx.x;
+    if (r >= nrhs || !active[r]) return;
+    cx<double> *s = sc + 8 * r;
+    // dots: [r][0] = <R,R> (real), [r][1] = <R,R0>
+    s[5] = dots[r * 3 + 0];
+    s[1] = dots[r * 3 + 1];
+    s[4] = cdiv(s[1], s[0]);
+}
+
+// q = u - alpha v ; uq = u + q
+__global__ void k_cgs_update1(int n, int nrhs, const cx<double> *sc, const int *active, const cx<double> *u,
+                              const cx<double> *v, cx<double> *q, cx<double> *uq)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    if (!active[r]) return;
+    const cx<double> alpha = sc[8 * r + 3];
+    const cx<double> qq = u[t] - alpha * v[t];
+    q[t] = qq;
+    uq[t] = u[t] + qq;
+}
+
+// x += alpha uq ; r -= alpha auq
+__global__ void k_cgs_update2(int n, int nrhs, const cx<double> *sc, const int *active, const cx<double> *uq,
+                              const cx<double> *auq, cx<double> *x, cx<double> *rr)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    if (!active[r]) return;
+    const cx<double> alpha = sc[8 * r + 3];
+    x[t] += alpha * uq[t];
+    rr[t] -= alpha * auq[t];
+}
+
+// u = r + beta q ; p = u + beta (q + beta p)
+__global__ void k_cgs_update3(int n, int nrhs, const cx<double> *sc, const int *active, const cx<double> *rr,
+                              const cx<double> *q, cx<double> *u, cx<double> *p)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    if (!active[r]) return;
+    const cx<double> beta = sc[8 * r + 4];
+    const cx<double> uu = rr[t] + beta * q[t];
+    u[t] = uu;
+    p[t] = uu + beta * (q[t] + beta * p[t]);
+}
+
+// y = alpha x + beta y elementwise per load case with per-case coefficients
+// (coef[r] = (alpha, beta) complex pairs); used by GMRES / refinement.
+__global__ void k_axpby(int n, int nrhs, const cx<double> *coef, const cx<double> *x, size_t ldx, cx<double> *y,
+                        size_t ldy)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    const size_t i = t % n;
+    const cx<double> a = coef[2 * r], b = coef[2 * r + 1];
+    y[r * ldy + i] = a * x[r * ldx + i] + b * y[r * ldy + i];
+}
+
+__global__ void k_scale_vec(int n, int nrhs, const double *d, cx<double> *x)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    x[t] = x[t] * d[t % n];
+}
+
+// row scaling of NC_ComputeScalingVector (src/NC_CommonFunctions.cpp:1322-1354):
+// d_i = sqrt(n / sum_j |a_ij|^2), one wave per row
+template <typename S>
+__global__ void k_row_scaling(const cx<S> *A, int n, double *d)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) {
+        const cx<S> a = A[(size_t)i * n + j];
+        s += double(a.re) * double(a.re) + double(a.im) * double(a.im);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) d[i] = sqrt((double)n / s);
+}
+
+// GMRES: h[r][k] = <V_k, w_r> for k < kcount (Hermitian), one block per (k, r) pair.
+// V layout: [basis k][rhs r][n]; w: [r][n].
+__global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, const cx<double> *V, const cx<double> *w,
+                                                  cx<double> *h)
+{
+    __shared__ double sre[256], sim[256];
+    const int k = blockIdx.x, r = blockIdx.y;
+    const cx<double> *v = V + ((size_t)k * nrhs + r) * n, *ww = w + (size_t)r * n;
+    double re = 0.0, im = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const cx<double> a = v[i], b = ww[i];
+        re += a.re * b.re + a.im * b.im;
+        im += a.re * b.im - a.im * b.re;
+    }
+    sre[threadIdx.x] = re;
+    sim[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sre[threadIdx.x] += sre[threadIdx.x + s];
+            sim[threadIdx.x] += sim[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) h[(size_t)r * kcount + k] = mk(sre[0], sim[0]);
+}
+
+// w[r] -= sum_k V_k[r] h[r][k]
+__global__ void k_multiaxpy(int n, int nrhs, int kcount, const cx<double> *V, const cx<double> *h, cx<double> *w)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    const size_t i = t % n;
+    cx<double> s = w[t];
+    for (int k = 0; k < kcount; ++k) s -= V[((size_t)k * nrhs + r) * n + i] * h[(size_t)r * kcount + k];
+    w[t] = s;
+}
+
+// x[r] += sum_k V_k[r] y[r][k]
+__global__ void k_multiaxpy_add(int n, int nrhs, int kcount, const cx<double> *V, const cx<double> *y, cx<double> *x)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = t / n;
+    const size_t i = t % n;
+    cx<double> s = x[t];
+    for (int k = 0; k < kcount; ++k) s += V[((size_t)k * nrhs + r) * n + i] * y[(size_t)r * kcount + k];
+    x[t] = s;
+}
+
+// ======================================================================================
+// unpivoted LU (Tfactor_usy / Tfbelim, src/NC_TypeDefinition.h:542-594) — FP64
+// ======================================================================================
+// trailing update of step i: A[j][k] -= A[i][k] * A[j][i] / A[i][i], j, k > i
+__global__ void k_lu_update(cx<double> *A, int n, int i)
+{
+    const int k = i + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = i + 1 + blockIdx.y;
+    if (k >= n || j >= n) return;
+    const cx<double> dia = A[(size_t)i * n + i];
+    const cx<double> l = cdiv(A[(size_t)j * n + i], dia);
+    A[(size_t)j * n + k] -= A[(size_t)i * n + k] * l;
+}
+
+// after the update: column i below and row i right of the diagonal divided by A[i][i]
+__global__ void k_lu_scale(cx<double> *A, int n, int i)
+{
+    const int t = i + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const cx<double> dia = A[(size_t)i * n + i];
+    A[(size_t)t * n + i] = cdiv(A[(size_t)t * n + i], dia);
+    A[(size_t)i * n + t] = cdiv(A[(size_t)i * n + t], dia);
+}
+
+// forward elimination, diagonal division and back substitution, one block per load case
+__global__ void __launch_bounds__(1024) k_lu_solve(const cx<double> *A, int n, cx<double> *bb)
+{
+    __shared__ double sre[1024], sim[1024];
+    cx<double> *b = bb + (size_t)blockIdx.x * n;
+    for (int i = 1; i < n; ++i) {
+        double re = 0.0, im = 0.0;
+        for (int j = threadIdx.x; j < i; j += blockDim.x) {
+            const cx<double> p = A[(size_t)i * n + j] * b[j];
+            re += p.re;
+            im += p.im;
+        }
+        sre[threadIdx.x] = re;
+        sim[threadIdx.x] = im;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                sre[threadIdx.x] += sre[threadIdx.x + s];
+                sim[threadIdx.x] += sim[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) b[i] -= mk(sre[0], sim[0]);
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) b[i] = cdiv(b[i], A[(size_t)i * n + i]);
+    __syncthreads();
+    for (int i = n - 1; i > 0; --i) {
+        const cx<double> bi = b[i];
+        for (int j = threadIdx.x; j < i; j += blockDim.x) b[j] -= bi * A[(size_t)j * n + i];
+        __syncthreads();
+    }
+}
+
+// ======================================================================================
+// evaluation grid: potential and gradient (NC_IntegrationTBEM, src/NC_PostProcessing.cpp:1264-1318)
+// one wave per evaluation point, lanes over the boundary elements.
+// ======================================================================================
+// pot/vel of every element for every load case from the solution
+// (NC_WriteResultsObjectMesh, src/NC_PostProcessing.cpp:399-421)
+__global__ void k_field_prep(int n, int nrhs, const int8_t *kind, const double *adm, double tau,
+                             const cx<double> *bcval, const cx<double> *x, cx<double> *pot, cx<double> *vel)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int j = t % n;
+    if (kind[j] == 1) {
+        pot[t] = bcval[t];
+        vel[t] = x[t];
+    } else {
+        pot[t] = x[t];
+        cx<double> v = bcval[t];
+        if (kind[j] == 2) v -= x[t] * mk(adm[2 * j], adm[2 * j + 1]) * tau;
+        vel[t] = v;
+    }
+}
+
+struct FieldAcc {
+    cx<double> K1, K2, K3[3], K4[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void field_point(FieldAcc &f, T qx, T qy, T qz, T nyx, T nyy, T nyz, T wj, T hk, T k2)
+{
+    const T r = sqrt(qx * qx + qy * qy + qz * qz);
+    const T ir = T(1) / r;
+    const T vx = qx * ir, vy = qy * ir, vz = qz * ir;
+    T sn, cs;
+    sincos_t(hk * r, &sn, &cs);
+    const T a = T(1.0 / kPi4) * ir * wj;
+    const cx<T> G = {cs * a, sn * a};                               // G * w J
+    const cx<T> Gi = {-ir * G.re - hk * G.im, -ir * G.im + hk * G.re};
+    const T drdn = vx * nyx + vy * nyy + vz * nyz;
+    const cx<T> f1 = G * mk(drdn * (k2 - T(3) * ir * ir), hk * T(3) * drdn * ir);
+    const cx<T> f2 = Gi * (-ir);
+    const cx<T> dgdn = Gi * drdn;
+    f.K1 += mk(double(dgdn.re), double(dgdn.im));
+    f.K2 += mk(double(G.re), double(G.im));
+    const T v[3] = {vx, vy, vz}, ny[3] = {nyx, nyy, nyz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const cx<T> t3 = f1 * v[d] + f2 * ny[d];
+        const cx<T> t4 = Gi * (-v[d]);
+        f.K3[d] += mk(double(t3.re), double(t3.im));
+        f.K4[d] += mk(double(t4.re), double(t4.im));
+    }
+}
+
+constexpr int kFieldChunk = 4;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const double *pts, int m, int nrhs, int rhs0,
+                                               const cx<double> *pot, const cx<double> *vel, const double *ps, int nps,
+                                               const double *pw, int npw, cx<double> *phi, cx<double> *grad, int *err)
+{
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= m) return;
+    const int n = E.n;
+    const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    const int nr = min(kFieldChunk, nrhs - rhs0);
+    cx<double> aphi[kFieldChunk], agr[kFieldChunk][3];
+    for (int r = 0; r < kFieldChunk; ++r) {
+        aphi[r] = mk(0.0, 0.0);
+        agr[r][0] = agr[r][1] = agr[r][2] = mk(0.0, 0.0);
+    }
+    const T hk = T(bm.hk), k2 = T(bm.k2);
+    for (int j = lane; j < n; j += 64) {
+        FieldAcc f;
+        f.K1 = f.K2 = mk(0.0, 0.0);
+        for (int d = 0; d < 3; ++d) f.K3[d] = f.K4[d] = mk(0.0, 0.0);
+        LocalElem L;
+        load_local(E, j, x, L);
+        const double ratdis = ratio_to_centre(E.cen[j], E.cen[n + j], E.cen[2 * n + j], x, L.area);
+        if (ratdis < kTolf) {
+            const int e = visit_adaptive(L, [&](double s, double t, double w) {
+                double q[3], ny[3], J;
+                point_on(L, s, t, q, ny, J);
+                field_point<double>(f, q[0], q[1], q[2], ny[0], ny[1], ny[2], w * J, bm.hk, bm.k2);
+            });
+            if (e) atomicOr(err, e);
+        } else {
+            const int mo = gauss_order(ratdis);
+            if (L.nv == 3) {
+                const int off = tri_rule_offset(mo), cnt = tri_rule_count(mo);
+                for (int g = 0; g < cnt; ++g) {
+                    double q[3], ny[3], J;
+                    point_on(L, c_tri[off + g][0], c_tri[off + g][1], q, ny, J);
+                    field_point<T>(f, T(q[0]), T(q[1]), T(q[2]), T(ny[0]), T(ny[1]), T(ny[2]),
+                                   T(c_tri[off + g][2] * J), hk, k2);
+                }
+            } else {
+                const int base = mo * (mo - 1) / 2;
+                for (int u = 0; u < mo; ++u)
+                    for (int v = 0; v < mo; ++v) {
+                        double q[3], ny[3], J;
+                        point_on(L, c_gl_x[base + u], c_gl_x[base + v], q, ny, J);
+                        field_point<T>(f, T(q[0]), T(q[1]), T(q[2]), T(ny[0]), T(ny[1]), T(ny[2]),
+                                       T(c_gl_w[base + u] * c_gl_w[base + v] * J), hk, k2);
+                    }
+            }
+        }
+        for (int r = 0; r < nr; ++r) {
+            const cx<double> pj = pot[(size_t)(rhs0 + r) * n + j], vj = vel[(size_t)(rhs0 + r) * n + j];
+            aphi[r] += f.K1 * pj - f.K2 * vj;
+            for (int d = 0; d < 3; ++d) agr[r][d] += f.K3[d] * pj - f.K4[d] * vj;
+        }
+    }
+    for (int r = 0; r < nr; ++r) {
+        double v[8] = {aphi[r].re, aphi[r].im, agr[r][0].re, agr[r][0].im, agr[r][1].re, agr[r][1].im,
+                       agr[r][2].re, agr[r][2].im};
+        for (int c = 0; c < 8; ++c)
+            for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+        if (lane != 0) continue;
+        cx<double> ph = mk(v[0], v[1]) * bm.tau;
+        cx<double> gr[3] = {mk(v[2], v[3]) * bm.tau, mk(v[4], v[5]) * bm.tau, mk(v[6], v[7]) * bm.tau};
+        // incident field (NC_ContributionIncidentWaves, src/NC_PostProcessing.cpp:776-866)
+        const int rr = rhs0 + r;
+        for (int w = 0; w < npw; ++w) {
+            const double *d = pw + 6 * w;
+            if ((int)d[5] != rr) continue;
+            double sn, cs;
+            sincos((d[0] * x[0] + d[1] * x[1] + d[2] * x[2]) * bm.hk, &sn, &cs);
+            const cx<double> g = mk(cs, sn) * mk(d[3], d[4]);
+            ph += g;
+            const cx<double> gv = mk(-g.im * bm.hk, g.re * bm.hk);
+            for (int q = 0; q < 3; ++q) gr[q] += gv * d[q];
+        }
+        for (int s = 0; s < nps; ++s) {
+            const double *y = ps + 6 * s;
+            if ((int)y[5] != rr) continue;
+            double v2[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+            const double R = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+            v2[0] /= R; v2[1] /= R; v2[2] /= R;
+            double sn, cs;
+            sincos(bm.hk * R, &sn, &cs);
+            const double f = 1.0 / kPi4 / R;
+            const cx<double> g = mk(cs * f, sn * f) * mk(y[3], y[4]);
+            ph += g;
+            const cx<double> gv = mk(-1.0 / R, bm.hk) * g;
+            for (int q = 0; q < 3; ++q) gr[q] += gv * v2[q];
+        }
+        phi[(size_t)rr * m + p] = ph;
+        if (grad)
+            for (int q = 0; q < 3; ++q) grad[((size_t)rr * m + p) * 3 + q] = gr[q];
+    }
+}
+
+}  // namespace ncg
+
+// ======================================================================================
+// host launchers (C++ linkage, declared in ncg_kernels.hpp)
+// ======================================================================================
+namespace ncgk {
+using namespace ncg;
+
+static inline unsigned blocks_for(size_t work, unsigned per) { return (unsigned)((work + per - 1) / per); }
+
+hipError_t set_gauss_tables(const double *x, const double *w)
+{
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_gl_x), x, sizeof(double) * 36);
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_gl_w), w, sizeof(double) * 36);
+}
+
+void element_geometry(hipStream_t s, int n, const double *X, const int8_t *nv, double *cen, double *nrm, double *area,
+                      double *jac)
+{
+    k_element_geometry<<<blocks_for(n, 128), 128, 0, s>>>(n, X, nv, cen, nrm, area, jac);
+}
+
+void near_count(hipStream_t s, const DevElements &D, int *count)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    k_near_count<<<blocks_for(D.n, 64), 64, 0, s>>>(E, count);
+}
+
+void near_fill(hipStream_t s, const DevElements &D, const long long *offset, int *rows, int *cols)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    k_near_fill<<<blocks_for(D.n, 64), 64, 0, s>>>(E, offset, rows, cols);
+}
+
+static BMConst make_bm(const BMParams &b)
+{
+    BMConst c;
+    c.tau = b.tau;
+    c.beta_im = b.beta_im;
+    c.hk = b.hk;
+    c.k2 = b.k * b.k;
+    c.k = b.k;
+    return c;
+}
+
+void assemble_far(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, void *A, int mixed,
+                  int rows_per_block, unsigned long long *evals)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    dim3 grid(blocks_for(D.n, 256), blocks_for(D.n, rows_per_block));
+    if (mixed)
+        k_assemble_far<float, float><<<grid, 256, 0, s>>>(E, make_bm(b), adm, (cx<float> *)A, rows_per_block, evals);
+    else
+        k_assemble_far<double, double><<<grid, 256, 0, s>>>(E, make_bm(b), adm, (cx<double> *)A, rows_per_block, evals);
+}
+
+void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
+                   const int *cols, long long npairs, void *A, int mixed, int *err)
+{
+    if (npairs == 0) return;
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    const unsigned g = blocks_for(npairs, 128);
+    if (mixed) k_assemble_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<float> *)A, err);
+    else k_assemble_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<double> *)A, err);
+}
+
+void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
+                 int nrhs, void *rhs_src, int *err)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk)
+        k_rhs_sources<<<blocks_for(D.n, 4), 256, 0, s>>>(E, make_bm(b), src, nsrc, (const cx<double> *)bcval, nrhs, r0,
+                                                         (cx<double> *)rhs_src, err);
+}
+
+void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const void *bcval,
+                   int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
+                   void *A, int mixed, void *rhs)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    k_assemble_diag<<<blocks_for(D.n, 64), 64, 0, s>>>(E, make_bm(b), adm, (const cx<double> *)bcval, nrhs, ps, nps, pw,
+                                                       npw, (const cx<double> *)rhs_src, has_src, A, mixed,
+                                                       (cx<double> *)rhs);
+}
+
+void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d, void *y)
+{
+    const unsigned g = blocks_for(n, 16);
+    for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        const int nr = std::min(4, nrhs - r0);
+        const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
+        cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
+#define NCG_MV(S, ACC)                                                                                 \
+    switch (nr) {                                                                                      \
+    case 1: k_matvec<S, ACC, 1><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
+    case 2: k_matvec<S, ACC, 2><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
+    case 3: k_matvec<S, ACC, 3><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
+    default: k_matvec<S, ACC, 4><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;            \
+    }
+        if (!mixed) { NCG_MV(double, double) }
+        else if (acc64) { NCG_MV(float, double) }
+        else { NCG_MV(float, float) }
+#undef NCG_MV
+    }
+}
+
+int dot_blocks(int n) { return std::min(kDotBlocks, (int)blocks_for(n, 256)); }
+
+void dots(hipStream_t s, int n, int nrhs, int ndots, const void *const *u, const void *const *v, int conj_mask,
+          void *partial, void *out)
+{
+    const int nb = dot_blocks(n);
+    k_dots<<<nb, 256, 0, s>>>(n, nrhs, ndots, (const cx<double> *)u[0], (const cx<double> *)v[0],
+                              (const cx<double> *)(ndots > 1 ? u[1] : u[0]), (const cx<double> *)(ndots > 1 ? v[1] : v[0]),
+                              (const cx<double> *)(ndots > 2 ? u[2] : u[0]), (const cx<double> *)(ndots > 2 ? v[2] : v[0]),
+                              conj_mask, (cx<double> *)partial);
+    k_reduce_partials<<<1, 256, 0, s>>>(nb, nrhs, ndots, (const cx<double> *)partial, (cx<double> *)out);
+}
+
+void cgs_alpha(hipStream_t s, int nrhs, int first, const void *dts, void *sc, const int *active)
+{
+    k_cgs_alpha<<<1, 64, 0, s>>>(nrhs, first, (const cx<double> *)dts, (cx<double> *)sc, active);
+}
+void cgs_beta(hipStream_t s, int nrhs, const void *dts, void *sc, const int *active)
+{
+    k_cgs_beta<<<1, 64, 0, s>>>(nrhs, (const cx<double> *)dts, (cx<double> *)sc, active);
+}
+void cgs_update1(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *u, const void *v, void *q,
+                 void *uq)
+{
+    k_cgs_update1<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+                                                                     (const cx<double> *)u, (const cx<double> *)v,
+                                                                     (cx<double> *)q, (cx<double> *)uq);
+}
+void cgs_update2(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *uq, const void *auq,
+                 void *x, void *r)
+{
+    k_cgs_update2<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+                                                                     (const cx<double> *)uq, (const cx<double> *)auq,
+                                                                     (cx<double> *)x, (cx<double> *)r);
+}
+void cgs_update3(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *r, const void *q, void *u,
+                 void *p)
+{
+    k_cgs_update3<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+                                                                     (const cx<double> *)r, (const cx<double> *)q,
+                                                                     (cx<double> *)u, (cx<double> *)p);
+}
+void axpby(hipStream_t s, int n, int nrhs, const void *coef, const void *x, size_t ldx, void *y, size_t ldy)
+{
+    k_axpby<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)coef, (const cx<double> *)x,
+                                                               ldx, (cx<double> *)y, ldy);
+}
+void scale_vec(hipStream_t s, int n, int nrhs, const double *d, void *x)
+{
+    k_scale_vec<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, d, (cx<double> *)x);
+}
+void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d)
+{
+    if (mixed) k_row_scaling<float><<<blocks_for(n, 4), 256, 0, s>>>((const cx<float> *)A, n, d);
+    else k_row_scaling<double><<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d);
+}
+void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h)
+{
+    k_multidot<<<dim3(kcount, nrhs), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V, (const cx<double> *)w,
+                                                  (cx<double> *)h);
+}
+void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add)
+{
+    if (add)
+        k_multiaxpy_add<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V,
+                                                                           (const cx<double> *)h, (cx<double> *)w);
+    else
+        k_multiaxpy<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V,
+                                                                       (const cx<double> *)h, (cx<double> *)w);
+}
+void lu_factor(hipStream_t s, void *A, int n)
+{
+    for (int i = 0; i < n - 1; ++i) {
+        const int rem = n - 1 - i;
+        k_lu_update<<<dim3(blocks_for(rem, 256), rem), 256, 0, s>>>((cx<double> *)A, n, i);
+        k_lu_scale<<<blocks_for(rem, 256), 256, 0, s>>>((cx<double> *)A, n, i);
+    }
+}
+void lu_solve(hipStream_t s, const void *A, int n, int nrhs, void *b)
+{
+    k_lu_solve<<<nrhs, 1024, 0, s>>>((const cx<double> *)A, n, (cx<double> *)b);
+}
+void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double *adm, double tau, const void *bcval,
+                const void *x, void *pot, void *vel)
+{
+    k_field_prep<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kind, adm, tau, (const cx<double> *)bcval,
+                                                                    (const cx<double> *)x, (cx<double> *)pot,
+                                                                    (cx<double> *)vel);
+}
+void field(hipStream_t s, const DevElements &D, const BMParams &b, const double *pts, int m, int nrhs, const void *pot,
+           const void *vel, const double *ps, int nps, const double *pw, int npw, void *phi, void *grad, int fp32,
+           int *err)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    for (int r0 = 0; r0 < nrhs; r0 += kFieldChunk) {
+        if (fp32)
+            k_field<float><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
+                                                            (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
+                                                            (cx<double> *)grad, err);
+        else
+            k_field<double><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
+                                                             (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
+                                                             (cx<double> *)grad, err);
+    }
+}
+
+}  // namespace ncgk

@@ -1,0 +1,174 @@
+"""ctypes binding of libncg (include/ncg.h) — the GPU BEM solve behind the drop-in NumCalc.
+
+The library is the product path; there is no CPU fallback.  Loading fails loudly when
+the built ``libncg.so`` is missing (build it with ``__graft_entry__.build()`` or
+``make -C mesh2hrtf_amd/csrc``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libncg.so")
+NUMCALC = os.path.join(PKG, "bin", "NumCalc")
+
+FP64, MIXED = 0, 1
+CGS, GMRES, LU = 0, 1, 4
+SCALING, NONE = 1, 2
+BC_VELOCITY, BC_PRESSURE, BC_ADMITTANCE = 0, 1, 2
+
+
+class NcgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libncg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Geometry(ctypes.Structure):
+    _fields_ = [("n_elements", ctypes.c_int32), ("nodes_per_element", ctypes.c_void_p),
+                ("node_xyz", ctypes.c_void_p), ("bc_kind", ctypes.c_void_p), ("n_eval", ctypes.c_int32),
+                ("eval_xyz", ctypes.c_void_p), ("harmonic_time_factor", ctypes.c_double),
+                ("interior", ctypes.c_int32)]
+
+
+class _Frequency(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_double), ("n_rhs", ctypes.c_int32), ("bc_value", ctypes.c_void_p),
+                ("admittance", ctypes.c_void_p), ("n_point_sources", ctypes.c_int32),
+                ("point_sources", ctypes.c_void_p), ("n_plane_waves", ctypes.c_int32),
+                ("plane_waves", ctypes.c_void_p)]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("precond", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("restart", ctypes.c_int32), ("refinement_steps", ctypes.c_int32),
+                ("refinement_tolerance", ctypes.c_double)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32 * 64), ("relative_residual", ctypes.c_double * 64),
+                ("converged_all", ctypes.c_int32), ("refinement_steps", ctypes.c_int32),
+                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_field", ctypes.c_double)]
+
+
+# every entry point include/ncg.h declares
+EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_problem_create", "ncg_problem_destroy", "ncg_assemble",
+           "ncg_solve", "ncg_get_solution", "ncg_field", "ncg_get_system", "ncg_apply", "ncg_timings")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libncg.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.ncg_last_error.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise NcgError(rc, load().ncg_last_error().decode())
+
+
+def device_count():
+    c = ctypes.c_int32(0)
+    load().ncg_device_count(ctypes.byref(c))
+    return c.value
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class Problem:
+    """One boundary-element problem resident on a GPU (geometry, near-pair table)."""
+
+    def __init__(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, device=0):
+        self.lib = load()
+        self.nv = np.ascontiguousarray(nv, np.int32)
+        n = len(self.nv)
+        x = np.zeros((n, 4, 3))
+        xyz = np.asarray(xyz, np.float64)
+        x[:, :xyz.shape[1], :] = xyz
+        self.xyz = np.ascontiguousarray(x)
+        self.kind = np.ascontiguousarray(kind, np.int32)
+        self.eval = np.ascontiguousarray(eval_xyz if eval_xyz is not None else np.zeros((0, 3)), np.float64)
+        self.n, self.m = n, len(self.eval)
+        g = _Geometry(n, self.nv.ctypes.data, self.xyz.ctypes.data, self.kind.ctypes.data, self.m,
+                      self.eval.ctypes.data if self.m else None, float(h), int(interior))
+        self._h = ctypes.c_void_p()
+        _check(self.lib.ncg_problem_create(ctypes.byref(g), int(device), ctypes.byref(self._h)))
+        self.nrhs = 0
+
+    def close(self):
+        if self._h:
+            self.lib.ncg_problem_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble(self, k, bc=None, adm=None, point_sources=(), plane_waves=(), precision=FP64, nrhs=None):
+        """bc: (nrhs, n) complex prescribed values; sources: rows (x, y, z, re, im, rhs)."""
+        if bc is None:
+            bc = np.zeros((nrhs or 1, self.n), np.complex128)
+        bc = np.ascontiguousarray(np.atleast_2d(bc), np.complex128)
+        self.nrhs = bc.shape[0]
+        self._bc = bc
+        self._adm = np.ascontiguousarray(adm if adm is not None else np.zeros(self.n), np.complex128)
+        self._ps = np.ascontiguousarray(np.asarray(point_sources, np.float64).reshape(-1, 6))
+        self._pw = np.ascontiguousarray(np.asarray(plane_waves, np.float64).reshape(-1, 6))
+        f = _Frequency(float(k), self.nrhs, bc.ctypes.data, self._adm.ctypes.data, len(self._ps), _ptr(self._ps),
+                       len(self._pw), _ptr(self._pw))
+        _check(self.lib.ncg_assemble(self._h, ctypes.byref(f), int(precision)))
+
+    def solve(self, solver=CGS, precond=NONE, max_iterations=250, tolerance=1e-9, restart=50,
+              refinement_steps=8, refinement_tolerance=1e-6, allow_noconv=False):
+        o = _Options(solver, precond, max_iterations, tolerance, restart, refinement_steps, refinement_tolerance)
+        st = _Stats()
+        rc = self.lib.ncg_solve(self._h, ctypes.byref(o), ctypes.byref(st))
+        if rc != 0 and not (allow_noconv and rc == -6):
+            _check(rc)
+        return dict(iterations=list(st.iterations[:self.nrhs]),
+                    relative_residual=list(st.relative_residual[:self.nrhs]),
+                    converged=bool(st.converged_all), refinement_steps=st.refinement_steps,
+                    ms_assemble=st.ms_assemble, ms_solve=st.ms_solve)
+
+    def solution(self):
+        x = np.zeros((self.nrhs, self.n), np.complex128)
+        _check(self.lib.ncg_get_solution(self._h, ctypes.c_void_p(x.ctypes.data)))
+        return x
+
+    def field(self, gradient=True):
+        phi = np.zeros((self.nrhs, self.m), np.complex128)
+        grad = np.zeros((self.nrhs, self.m, 3), np.complex128) if gradient else None
+        _check(self.lib.ncg_field(self._h, ctypes.c_void_p(phi.ctypes.data), _ptr(grad) if gradient else None))
+        return phi, grad
+
+    def system(self):
+        A = np.zeros((self.n, self.n), np.complex128)
+        b = np.zeros((self.nrhs, self.n), np.complex128)
+        _check(self.lib.ncg_get_system(self._h, ctypes.c_void_p(A.ctypes.data), ctypes.c_void_p(b.ctypes.data)))
+        return A, b
+
+    def apply(self, x):
+        x = np.ascontiguousarray(np.atleast_2d(x), np.complex128)
+        y = np.zeros_like(x)
+        _check(self.lib.ncg_apply(self._h, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data), len(x)))
+        return y
+
+    def timings(self, reset=False):
+        t = (ctypes.c_double * 8)()
+        _check(self.lib.ncg_timings(self._h, t, int(reset)))
+        keys = ("ms_assemble", "ms_solve", "ms_field", "far_kernel_ms", "matvec_kernel_ms", "matvec_calls",
+                "far_evaluations", "operator_bytes")
+        return dict(zip(keys, list(t)))
