@@ -52,6 +52,15 @@ enum { NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2 };
  * (src/NC_Input.cpp:353-737): element node coordinates, the boundary-condition
  * kind per element (it decides the unknown: potential or normal velocity), and
  * the evaluation-grid nodes of NC_WriteResultsEvaluationGrid (src/NC_PostProcessing.cpp:581). */
+/* Method of NC.inp "Main Parameters I" and the FMM inputs of "Main Parameters II"
+ * (src/NC_Input.cpp:246-323); the per-frequency choice between dense BEM, SL-FMM and
+ * ML-FMM then follows NC_DeclareArrays (src/NC_Addresses.cpp:58-287). */
+typedef struct {
+    int32_t method_bem;                 /* 0 dense (TBEM), 1 SL-FMM, 4 ML-FMM               */
+    int32_t min_expansion_terms;        /* 5..10, 0 -> 8                                    */
+    double cluster_edge;                /* ClusEdgL0; <= 0: computed as NumCalc does        */
+} ncg_method;
+
 typedef struct {
     int32_t n_elements;                 /* boundary elements = unknowns                    */
     const int32_t *nodes_per_element;   /* [n] 3 (triangle) or 4 (quadrilateral)          */
@@ -61,7 +70,27 @@ typedef struct {
     const double *eval_xyz;             /* [n_eval][3]                                     */
     double harmonic_time_factor;        /* +1: e^{-i w t} (NumCalc default), -1: e^{+i w t} */
     int32_t interior;                   /* 0 exterior problem, 1 interior                  */
+    const int32_t *node_index;          /* [n][4] mesh node ids (NULL: nodes with equal
+                                           coordinates are one node); FMM cluster centres  */
+    const int32_t *element_group;       /* [n] element group ids (NULL: one group)         */
+    ncg_method method;
 } ncg_geometry;
+
+/* FMM plan summary: the cluster and expansion figures NumCalc writes to NC.out
+ * (NC_FrequencyInformations src/NC_Main.cpp:1051-1153, src/NC_Main.cpp:491-522,
+ * NC_AllocateSDTmtxs* src/NC_Addresses.cpp:1476-1654). */
+typedef struct {
+    int32_t method;                     /* 0 dense, 1 SL-FMM, 3 ML-FMM (methodFMM_)         */
+    int32_t levels;
+    int32_t clusters[10];
+    double radius_max[10], radius_min[10];
+    int32_t expansion_terms[10], points_theta[10], points_phi[10], points_sphere[10];
+    int64_t nnz_near, nnz_t[10], nnz_d[10], nnz_s[10];
+    int64_t canceled_pairs[10], far_pairs[10];
+    int32_t eval_clusters;
+    double eval_radius_max, eval_radius_min;
+    char warning[256];
+} ncg_fmm_info;
 
 /* One frequency step with n_rhs load cases that share the operator.  Values are
  * the per-frequency ones of NC_UpdateFreqCurves (src/NC_EquationSystem.cpp:1815-1961):
@@ -102,6 +131,12 @@ typedef struct ncg_problem ncg_problem;
 const char *ncg_last_error(void);
 int ncg_device_count(int32_t *count);
 
+/* Method and FMM plan NumCalc would use at wave number k for this geometry, without a
+ * device (host computation only).  Replaces the tree part of NC_DeclareArrays
+ * (src/NC_Addresses.cpp:58-287, 290-1314, 1476-1937) and the expansion lengths of
+ * src/NC_Main.cpp:421-489.  Returns NCG_E_ARG with NumCalc's message on its fatal cases. */
+int ncg_fmm_plan(const ncg_geometry *geometry, double k, ncg_fmm_info *info);
+
 /* Upload geometry, compute element data and the frequency-independent near-pair
  * table (the "which pairs need subdivision" plan) on the device.
  * Replaces the geometry part of NC_ReadMesh (src/NC_Input.cpp:522-605). */
@@ -117,6 +152,9 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *freq, int32_t precision);
  * Replaces NC_IterativeSolverCGS / Tfactor_usy+Tfbelim (src/NC_CommonFunctions.cpp:1066-1208,
  * src/NC_TypeDefinition.h:542-594, selected at src/NC_Main.cpp:833-866). */
 int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *stats);
+
+/* The method and FMM plan of the last ncg_assemble (what NumCalc logs per step). */
+int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info);
 
 /* Solved unknowns to host: [n_rhs][n] complex (potential for velocity/admittance
  * elements, normal velocity for pressure elements).  Replaces reading zrhs after the

@@ -15,7 +15,11 @@
 #include <string>
 #include <vector>
 
+#include "ncg_fmm.hpp"
 #include "ncg_kernels.hpp"
+
+#include <array>
+#include <map>
 
 using cd = std::complex<double>;
 
@@ -85,6 +89,53 @@ struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
+// device arrays of one FMM level (layouts in ncg_kernels.hip)
+struct DevLevel {
+    int ncl = 0, P = 0, L = 0, nfan = 0;
+    double *ccen = nullptr, *u = nullptr, *w = nullptr;
+    int *cl_ptr = nullptr, *cl_elem = nullptr, *elem_cl = nullptr, *elem_pos = nullptr;
+    int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
+    void *T = nullptr, *D = nullptr, *S = nullptr;
+    double *tw = nullptr, *sw = nullptr, *tv = nullptr;   // work [nrhs][ncl*P]
+};
+
+// device arrays of the FMM operator of one frequency
+struct DevFmm {
+    std::vector<DevLevel> lev;
+    long long nnz = 0;
+    long long *near_ptr = nullptr, *diag_pos = nullptr;
+    int *near_row = nullptr, *near_col = nullptr;
+    void *near_val = nullptr;
+    int *elem_cl_fine = nullptr;
+    unsigned char *cl_near = nullptr;
+    int ncl_fine = 0;
+    double *xs = nullptr;   // scaled copy of the input vector
+    std::vector<void *> owned;
+    void release()
+    {
+        for (void *q : owned) hipFree(q);
+        owned.clear();
+        lev.clear();
+        nnz = 0;
+        near_ptr = diag_pos = nullptr;
+        near_row = near_col = nullptr;
+        near_val = nullptr;
+        elem_cl_fine = nullptr;
+        cl_near = nullptr;
+        xs = nullptr;
+    }
+    template <typename T>
+    int alloc(T **ptr, size_t count)
+    {
+        *ptr = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)ptr, sizeof(T) * count);
+        if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc (FMM): ") + hipGetErrorString(e));
+        owned.push_back((void *)*ptr);
+        return 0;
+    }
+};
+
 }  // namespace
 
 struct ncg_problem {
@@ -129,6 +180,12 @@ struct ncg_problem {
     double far_ms_sum = 0, mv_ms_sum = 0;
     long long far_calls = 0, mv_calls = 0;
     double last_evals = 0;
+    // fast multipole state
+    ncgf::Mesh fm;
+    ncgf::Options fo;
+    ncgf::Plan plan;
+    DevFmm fmm;
+    size_t nrhs_fmm = 0;
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -195,7 +252,24 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
     }
     EventPair &e = p->ev_mv[p->mv_used++];
     record(p, e.a);
-    ncgk::matvec(p->stream, p->A, p->precision == NCG_MIXED, acc64, p->n, p->nrhs, x, d, y);
+    if (p->plan.method == 0) {
+        ncgk::matvec(p->stream, p->A, p->precision == NCG_MIXED, acc64, p->n, p->nrhs, x, d, y);
+    } else {
+        // y = Near x + sum over levels S (D (T x))  (src/NC_CommonFunctions.cpp:1247-1316)
+        const int mixed = p->precision == NCG_MIXED;
+        const double *xin = x;
+        if (d) {
+            hipMemcpyAsync(p->fmm.xs, x, sizeof(cd) * (size_t)p->nrhs * p->n, hipMemcpyDeviceToDevice, p->stream);
+            ncgk::scale_vec(p->stream, p->n, p->nrhs, d, p->fmm.xs);
+            xin = p->fmm.xs;
+        }
+        ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val, mixed, xin, y);
+        for (DevLevel &L : p->fmm.lev) {
+            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T, mixed, xin, L.tw);
+            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D, mixed, L.tw, L.sw);
+            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S, mixed, L.sw, y);
+        }
+    }
     record(p, e.b);
 }
 
@@ -220,6 +294,267 @@ int check_err(ncg_problem *p)
         if (e & 2) return fail(NCG_E_SUBDIV, "nsbe must be < MSBE(= 220)");
         return fail(NCG_E_SUBDIV, "adaptive subdivision deeper than 24 levels (point on an element?)");
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------- FMM
+// host mesh of the planner from the ABI geometry
+int build_fmm_mesh(const ncg_geometry *g, ncgf::Mesh &M)
+{
+    const int n = g->n_elements;
+    M = ncgf::Mesh();
+    M.n = n;
+    M.nv.assign(g->nodes_per_element, g->nodes_per_element + n);
+    M.X.assign(g->node_xyz, g->node_xyz + 12 * (size_t)n);
+    M.node.assign(4 * (size_t)n, -1);
+    if (g->node_index) {
+        for (size_t i = 0; i < 4 * (size_t)n; ++i) M.node[i] = g->node_index[i];
+    } else {
+        // nodes with bitwise equal coordinates are one mesh node
+        std::vector<std::pair<std::array<double, 3>, int>> keys;
+        std::map<std::array<double, 3>, int> id;
+        for (int e = 0; e < n; ++e)
+            for (int a = 0; a < M.nv[e]; ++a) {
+                std::array<double, 3> k3 = {M.X[12 * (size_t)e + 3 * a], M.X[12 * (size_t)e + 3 * a + 1],
+                                            M.X[12 * (size_t)e + 3 * a + 2]};
+                auto it = id.find(k3);
+                if (it == id.end()) it = id.emplace(k3, (int)id.size()).first;
+                M.node[4 * (size_t)e + a] = it->second;
+            }
+    }
+    M.group.assign(n, 0);
+    if (g->element_group) M.group.assign(g->element_group, g->element_group + n);
+    M.m = g->n_eval > 0 ? g->n_eval : 0;
+    if (M.m) M.eval.assign(g->eval_xyz, g->eval_xyz + 3 * (size_t)M.m);
+    ncgf::prepare_mesh(M);
+    return 0;
+}
+
+ncgf::Options fmm_options(const ncg_geometry *g)
+{
+    ncgf::Options o;
+    o.method_bem = g->method.method_bem;
+    o.min_terms = g->method.min_expansion_terms == 0 ? 8 : g->method.min_expansion_terms;
+    o.clus_edge = g->method.cluster_edge;
+    return o;
+}
+
+// near-field nonzeros of row DOFs in cluster c: elements of its near clusters
+long long near_count(const ncgf::Level &L, int c)
+{
+    long long s = 0;
+    for (int j : L.cl[c].near) s += (long long)L.cl[j].elems.size();
+    return s;
+}
+
+void fill_info(const ncgf::Plan &P, int n, ncg_fmm_info *info)
+{
+    memset(info, 0, sizeof(*info));
+    info->method = P.method;
+    info->levels = (int)P.lev.size();
+    for (size_t l = 0; l < P.lev.size() && l < 10; ++l) {
+        const ncgf::Level &L = P.lev[l];
+        info->clusters[l] = (int)L.cl.size();
+        info->radius_max[l] = L.rmax;
+        info->radius_min[l] = L.rmin;
+        info->expansion_terms[l] = L.L;
+        info->points_theta[l] = L.ntheta;
+        info->points_phi[l] = L.nphi;
+        info->points_sphere[l] = L.P;
+        long long fan = 0;
+        for (const ncgf::Cluster &c : L.cl) fan += (long long)c.fan.size();
+        info->nnz_t[l] = (long long)n * L.P;
+        info->nnz_d[l] = fan * L.P;
+        info->nnz_s[l] = (long long)n * L.P;
+        info->canceled_pairs[l] = L.n_canceled;
+        info->far_pairs[l] = L.n_far_geom;
+    }
+    if (!P.lev.empty()) {
+        const ncgf::Level &F = P.lev.back();
+        for (int c = 0; c < (int)F.cl.size(); ++c) info->nnz_near += (long long)F.cl[c].elems.size() * near_count(F, c);
+    }
+    info->eval_clusters = (int)P.ev.size();
+    info->eval_radius_max = P.ev_rmax;
+    info->eval_radius_min = P.ev.empty() ? 0.0 : P.ev_rmin;
+    snprintf(info->warning, sizeof(info->warning), "%s", P.warning.c_str());
+}
+
+template <typename T>
+int upload(DevFmm &F, hipStream_t s, T **dst, const std::vector<T> &src)
+{
+    int rc = F.alloc(dst, src.size());
+    if (rc) return rc;
+    if (!src.empty()) {
+        hipError_t e = hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("FMM upload: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// Build and fill the device FMM operator of the current plan (NC_BuildSystemFMBEM,
+// src/NC_EquationSystem.cpp:305-365, with the CSR layouts of NC_AllocateNmtx and
+// NC_AllocateSDTmtxs*, src/NC_Addresses.cpp:1476-1937).
+int assemble_fmm(ncg_problem *p, const std::vector<int> &srch)
+{
+    hipStream_t s = p->stream;
+    DevFmm &F = p->fmm;
+    hipStreamSynchronize(s);
+    F.release();
+    const int n = p->n, nr = p->nrhs;
+    const int mixed = p->precision == NCG_MIXED;
+    const size_t se = mixed ? 8 : 16;
+    int rc;
+    const ncgk::BMParams b = p->bm();
+    const double h = p->h;
+    // levels
+    for (const ncgf::Level &L : p->plan.lev) {
+        DevLevel D;
+        D.ncl = (int)L.cl.size();
+        D.P = L.P;
+        D.L = L.L;
+        std::vector<double> cc(3 * (size_t)D.ncl);
+        std::vector<int> cl_ptr(D.ncl + 1, 0), cl_elem, elem_cl(n, -1), elem_pos(n, -1), fan_ptr(D.ncl + 1, 0), fan_idx,
+            fan_row;
+        for (int c = 0; c < D.ncl; ++c) {
+            for (int d = 0; d < 3; ++d) cc[3 * c + d] = L.cl[c].cen[d];
+            for (int e : L.cl[c].elems) {
+                elem_cl[e] = c;
+                elem_pos[e] = (int)cl_elem.size();
+                cl_elem.push_back(e);
+            }
+            cl_ptr[c + 1] = (int)cl_elem.size();
+            for (int j : L.cl[c].fan) {
+                fan_idx.push_back(j);
+                fan_row.push_back(c);
+            }
+            fan_ptr[c + 1] = (int)fan_idx.size();
+        }
+        D.nfan = (int)fan_idx.size();
+        if ((rc = upload(F, s, &D.ccen, cc)) || (rc = upload(F, s, &D.u, L.u)) || (rc = upload(F, s, &D.w, L.w)) ||
+            (rc = upload(F, s, &D.cl_ptr, cl_ptr)) || (rc = upload(F, s, &D.cl_elem, cl_elem)) ||
+            (rc = upload(F, s, &D.elem_cl, elem_cl)) || (rc = upload(F, s, &D.elem_pos, elem_pos)) ||
+            (rc = upload(F, s, &D.fan_ptr, fan_ptr)) || (rc = upload(F, s, &D.fan_idx, fan_idx)) ||
+            (rc = upload(F, s, &D.fan_row, fan_row)))
+            return rc;
+        char *T, *Dv, *S;
+        if ((rc = F.alloc(&T, se * (size_t)n * D.P)) || (rc = F.alloc(&Dv, se * (size_t)std::max(D.nfan, 1) * D.P)) ||
+            (rc = F.alloc(&S, se * (size_t)n * D.P)) || (rc = F.alloc(&D.tw, 2 * (size_t)nr * D.ncl * D.P)) ||
+            (rc = F.alloc(&D.sw, 2 * (size_t)nr * D.ncl * D.P)) || (rc = F.alloc(&D.tv, 2 * (size_t)nr * D.ncl * D.P)))
+            return rc;
+        D.T = T;
+        D.D = Dv;
+        D.S = S;
+        F.lev.push_back(D);
+    }
+    // near field of the finest level: row i -> elements of the near clusters of its cluster
+    const ncgf::Level &FL = p->plan.lev.back();
+    const int ncf = (int)FL.cl.size();
+    std::vector<int> cl_of(n, -1);
+    for (int c = 0; c < ncf; ++c)
+        for (int e : FL.cl[c].elems) cl_of[e] = c;
+    std::vector<long long> ptr(n + 1, 0), diag(n, -1);
+    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + near_count(FL, cl_of[i]);
+    F.nnz = ptr[n];
+    std::vector<int> rows(F.nnz), cols(F.nnz);
+    std::vector<unsigned char> cnear((size_t)ncf * ncf, 0);
+    for (int c = 0; c < ncf; ++c)
+        for (int j : FL.cl[c].near) cnear[(size_t)c * ncf + j] = 1;
+    for (int i = 0; i < n; ++i) {
+        long long q = ptr[i];
+        for (int c = 0; c < ncf; ++c) {
+            if (!cnear[(size_t)cl_of[i] * ncf + c]) continue;
+            for (int e : FL.cl[c].elems) {
+                rows[q] = i;
+                if (e == i) diag[i] = q;
+                cols[q++] = e;
+            }
+        }
+    }
+    F.ncl_fine = ncf;
+    if ((rc = upload(F, s, &F.near_ptr, ptr)) || (rc = upload(F, s, &F.diag_pos, diag)) ||
+        (rc = upload(F, s, &F.near_row, rows)) || (rc = upload(F, s, &F.near_col, cols)) ||
+        (rc = upload(F, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F, s, &F.cl_near, cnear)))
+        return rc;
+    char *nv;
+    if ((rc = F.alloc(&nv, se * (size_t)F.nnz)) || (rc = F.alloc(&F.xs, 2 * (size_t)nr * n))) return rc;
+    F.near_val = nv;
+    // values
+    const ncgk::DevElements E = p->elements();
+    ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val, mixed, p->err);
+    if (p->nsrc)
+        ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near, ncf);
+    ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0, F.near_val,
+                        mixed, p->rhs, F.diag_pos);
+    for (DevLevel &D : F.lev) {
+        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u, D.D, mixed);
+        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u, D.T, mixed);
+        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u, D.w, D.S, mixed);
+        if (p->nsrc) {
+            // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
+            ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u, p->bcval, nr, D.tv);
+            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D, mixed, D.tv, D.sw);
+            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S, mixed, D.sw, p->rhs);
+        }
+    }
+    (void)srch;
+    return 0;
+}
+
+// evaluation grid through the level-0 expansions (NC_ContributionFMM,
+// src/NC_PostProcessing.cpp:978-1214)
+int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dgrad)
+{
+    hipStream_t s = p->stream;
+    const ncgf::Level &L0 = p->plan.lev.front();
+    DevLevel &D0 = p->fmm.lev.front();
+    const int nr = p->nrhs, m = p->m, nev = (int)p->plan.ev.size();
+    const double fac = 2.0 / sqrt(3.0);
+    std::vector<int> pt_ev(m, 0), ev_ptr(nev + 1, 0), pair_ev, pair_cl, el_ptr(nev + 1, 0), el_idx;
+    std::vector<double> ecen(3 * (size_t)nev);
+    for (int a = 0; a < nev; ++a) {
+        const ncgf::EvalCluster &E = p->plan.ev[a];
+        for (int d = 0; d < 3; ++d) ecen[3 * a + d] = E.cen[d];
+        for (int x : E.pts) pt_ev[x] = a;
+        for (int c = 0; c < (int)L0.cl.size(); ++c) {
+            const ncgf::Cluster &C = L0.cl[c];
+            const double dx = C.cen[0] - E.cen[0], dy = C.cen[1] - E.cen[1], dz = C.cen[2] - E.cen[2];
+            const double di = sqrt(dx * dx + dy * dy + dz * dz), dk = C.radius + E.radius;
+            if (di > fac * dk && di > p->plan.min_dist) {
+                pair_ev.push_back(a);
+                pair_cl.push_back(c);
+            } else {
+                el_idx.insert(el_idx.end(), C.elems.begin(), C.elems.end());
+            }
+        }
+        ev_ptr[a + 1] = (int)pair_ev.size();
+        el_ptr[a + 1] = (int)el_idx.size();
+    }
+    DevFmm T;   // per-call buffers
+    int rc;
+    int *d_pt_ev, *d_ev_ptr, *d_pair_ev, *d_pair_cl, *d_el_ptr, *d_el_idx;
+    double *d_ecen, *zpair, *z, *fphi, *fgrad;
+    const int npair = (int)pair_ev.size();
+    if ((rc = upload(T, s, &d_pt_ev, pt_ev)) || (rc = upload(T, s, &d_ev_ptr, ev_ptr)) ||
+        (rc = upload(T, s, &d_pair_ev, pair_ev)) || (rc = upload(T, s, &d_pair_cl, pair_cl)) ||
+        (rc = upload(T, s, &d_el_ptr, el_ptr)) || (rc = upload(T, s, &d_el_idx, el_idx)) ||
+        (rc = upload(T, s, &d_ecen, ecen)) || (rc = T.alloc(&zpair, 2 * (size_t)nr * std::max(npair, 1) * L0.P)) ||
+        (rc = T.alloc(&z, 2 * (size_t)nr * nev * L0.P)) || (rc = T.alloc(&fphi, 2 * (size_t)nr * m)) ||
+        (rc = T.alloc(&fgrad, 6 * (size_t)nr * m))) {
+        hipStreamSynchronize(s);
+        T.release();
+        return rc;
+    }
+    const ncgk::BMParams b = p->bm();
+    ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u, pot, vel, nr, D0.tv);
+    ncgk::fmm_eval_pairs(s, D0.L, D0.P, p->k, p->h, d_ecen, D0.ccen, d_pair_ev, d_pair_cl, npair, D0.u, D0.w, D0.tv,
+                         D0.ncl, nr, zpair);
+    ncgk::fmm_eval_sum(s, nev, D0.P, npair, d_ev_ptr, nr, zpair, z);
+    ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, d_pt_ev, d_ecen, D0.u, nr, z, fphi, fgrad);
+    ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad,
+                p->precision == NCG_MIXED, p->err, d_pt_ev, d_el_ptr, d_el_idx, fphi, fgrad);
+    hipError_t e = hipStreamSynchronize(s);
+    T.release();
+    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("FMM field: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -248,7 +583,8 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     const size_t bytes = sizeof(cd) * (size_t)nr * n;
     const double *d = nullptr;
     if (o->precond == NCG_PRECOND_SCALING) {
-        ncgk::row_scaling(p->stream, p->A, p->precision == NCG_MIXED, n, p->dvec);
+        if (p->plan.method == 0) ncgk::row_scaling(p->stream, p->A, p->precision == NCG_MIXED, n, p->dvec);
+        else ncgk::row_scaling_csr(p->stream, n, p->fmm.near_ptr, p->fmm.near_val, p->precision == NCG_MIXED, p->dvec);
         d = p->dvec;
     }
     NCG_HIP(hipMemsetAsync(X, 0, bytes, p->stream));
@@ -550,8 +886,14 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NCG_E_DEVICE, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(NCG_E_ARG, "device index out of range");
     NCG_HIP(hipSetDevice(device));
+    if (g->method.method_bem != 0 && g->method.method_bem != 1 && g->method.method_bem != 4)
+        return fail(NCG_E_ARG, "methodBEM_ must be in {0,1,4}!");
+    if (g->method.min_expansion_terms != 0 && (g->method.min_expansion_terms < 5 || g->method.min_expansion_terms > 10))
+        return fail(NCG_E_ARG, "minExpansionTermsFMM_  must be in [5 10]!");
     ncg_problem *p = new ncg_problem();
     p->device = device;
+    build_fmm_mesh(g, p->fm);
+    p->fo = fmm_options(g);
     p->n = n;
     p->m = g->n_eval > 0 ? g->n_eval : 0;
     p->h = g->harmonic_time_factor == -1.0 ? -1.0 : 1.0;
@@ -628,6 +970,7 @@ int ncg_problem_destroy(ncg_problem *p)
                     (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis})
         if (q) hipFree(q);
     for (double *w : p->work) hipFree(w);
+    p->fmm.release();
     for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
         if (e->a) hipEventDestroy(e->a);
         if (e->b) hipEventDestroy(e->b);
@@ -655,7 +998,9 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
     p->assembled = p->solved = 0;
     int rc;
     if ((rc = ensure_vectors(p, nr))) return rc;
-    const size_t abytes = (size_t)n * n * (precision == NCG_MIXED ? 8 : 16);
+    // method and FMM plan of this frequency (NC_DeclareArrays, src/NC_Addresses.cpp:58-287)
+    if (!ncgf::make_plan(p->fm, p->fo, f->k, p->plan)) return fail(NCG_E_ARG, p->plan.error);
+    const size_t abytes = p->plan.method ? 0 : (size_t)n * n * (precision == NCG_MIXED ? 8 : 16);
     if (abytes > p->A_cap) {
         hipFree(p->A);
         p->A = nullptr;
@@ -709,6 +1054,19 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
     NCG_HIP(hipMemsetAsync(p->evals, 0, sizeof(unsigned long long), s));
     const ncgk::BMParams b = p->bm();
     const int mixed = precision == NCG_MIXED;
+    if (p->plan.method != 0) {
+        record(p, p->ev_asm.a);
+        record(p, p->ev_far.a);
+        if ((rc = assemble_fmm(p, srch))) return rc;
+        record(p, p->ev_far.b);
+        record(p, p->ev_asm.b);
+        NCG_HIP(hipGetLastError());
+        if ((rc = check_err(p))) return rc;
+        p->last_evals = 0.0;
+        p->t_asm = elapsed(p->ev_asm);
+        p->assembled = 1;
+        return 0;
+    }
     record(p, p->ev_asm.a);
     record(p, p->ev_far.a);
     ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->evals);
@@ -741,6 +1099,7 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     int rc;
     if (o->solver == NCG_SOLVER_LU) {
         if (p->precision != NCG_FP64) return fail(NCG_E_ARG, "the direct solver needs the FP64 operator");
+        if (p->plan.method != 0) return fail(NCG_E_ARG, "the direct solver needs the dense operator (TBEM)");
         const size_t bytes = sizeof(cd) * (size_t)p->nrhs * p->n;
         NCG_HIP(hipMemcpyAsync(p->x, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
         ncgk::lu_factor(p->stream, p->A, p->n);
@@ -794,8 +1153,17 @@ int ncg_field(ncg_problem *p, double *phi, double *grad)
     }
     record(p, p->ev_field.a);
     ncgk::field_prep(p->stream, n, nr, p->kind, p->adm, p->interior ? -1.0 : 1.0, p->bcval, p->x, pot, vel);
-    ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
-                grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
+    if (p->plan.method != 0) {
+        rc = field_fmm(p, pot, vel, dphi, grad ? dgrad : nullptr);
+        if (rc) {
+            hipFree(dphi);
+            hipFree(dgrad);
+            return rc;
+        }
+    } else {
+        ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
+                    grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
+    }
     record(p, p->ev_field.b);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(phi, dphi, sizeof(cd) * nr * m, hipMemcpyDeviceToHost, p->stream);
@@ -812,6 +1180,7 @@ int ncg_get_system(ncg_problem *p, double *A, double *b)
 {
     if (!p) return fail(NCG_E_ARG, "ncg_get_system: null argument");
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_get_system: assemble first");
+    if (p->plan.method != 0) return fail(NCG_E_STATE, "ncg_get_system: the FMM operator is not stored densely");
     NCG_HIP(hipSetDevice(p->device));
     const size_t nn = (size_t)p->n * p->n;
     if (A) {
@@ -847,6 +1216,26 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
     return 0;
 }
 
+int ncg_fmm_plan(const ncg_geometry *g, double k, ncg_fmm_info *info)
+{
+    if (!g || !info || g->n_elements <= 0 || !g->nodes_per_element || !g->node_xyz)
+        return fail(NCG_E_ARG, "ncg_fmm_plan: missing geometry");
+    if (!(k > 0.0)) return fail(NCG_E_ARG, "ncg_fmm_plan: wave number must be positive");
+    ncgf::Mesh M;
+    build_fmm_mesh(g, M);
+    ncgf::Plan P;
+    if (!ncgf::make_plan(M, fmm_options(g), k, P)) return fail(NCG_E_ARG, P.error);
+    fill_info(P, M.n, info);
+    return 0;
+}
+
+int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info)
+{
+    if (!p || !info) return fail(NCG_E_ARG, "ncg_problem_fmm_info: null argument");
+    fill_info(p->plan, p->n, info);
+    return 0;
+}
+
 int ncg_timings(ncg_problem *p, double *t, int32_t reset)
 {
     if (!p || !t) return fail(NCG_E_ARG, "ncg_timings: null argument");
@@ -857,7 +1246,15 @@ int ncg_timings(ncg_problem *p, double *t, int32_t reset)
     t[4] = p->mv_calls ? p->mv_ms_sum / p->mv_calls : 0.0;
     t[5] = (double)p->mv_calls;
     t[6] = p->last_evals;
-    t[7] = (double)p->n * p->n * (p->precision == NCG_MIXED ? 8.0 : 16.0);
+    const double se = p->precision == NCG_MIXED ? 8.0 : 16.0;
+    if (p->plan.method == 0) t[7] = (double)p->n * p->n * se;
+    else {
+        ncg_fmm_info info;
+        fill_info(p->plan, p->n, &info);
+        double nnz = (double)info.nnz_near;
+        for (int l = 0; l < info.levels; ++l) nnz += (double)(info.nnz_t[l] + info.nnz_d[l] + info.nnz_s[l]);
+        t[7] = nnz * se;
+    }
     if (reset) {
         p->far_ms_sum = p->mv_ms_sum = 0.0;
         p->far_calls = p->mv_calls = 0;

@@ -167,7 +167,8 @@ constexpr int kRhsChunk = 8;
 
 __global__ void __launch_bounds__(256) k_rhs_sources(Elements E, BMConst bm, const int *src, int nsrc,
                                                       const cx<double> *bcval, int nrhs, int rhs0,
-                                                      cx<double> *rhs_src, int *err)
+                                                      cx<double> *rhs_src, int *err, const int *elem_cluster,
+                                                      const unsigned char *cluster_near, int nclusters)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -177,6 +178,8 @@ __global__ void __launch_bounds__(256) k_rhs_sources(Elements E, BMConst bm, con
     for (int r = 0; r < kRhsChunk; ++r) part[r] = mk(0.0, 0.0);
     for (int q = lane; q < nsrc; q += 64) {
         const int j = src[q];
+        // FMM: only the near field is integrated here; far sources enter through the T-vector
+        if (cluster_near && !cluster_near[(size_t)elem_cluster[i] * nclusters + elem_cluster[j]]) continue;
         BMAcc<double> acc;
         const int e = bm_pair(E, i, j, bm.hk, bm.k2, acc);
         if (e) atomicOr(err, e);
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(256) k_rhs_sources(Elements E, BMConst bm, con
 __global__ void __launch_bounds__(64) k_assemble_diag(Elements E, BMConst bm, const double *adm, const cx<double> *bcval,
                                                       int nrhs, const double *ps, int nps, const double *pw, int npw,
                                                       const cx<double> *rhs_src, int has_src, void *Aout,
-                                                      int mixed, cx<double> *b)
+                                                      int mixed, cx<double> *b, const long long *diag_index)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E.n) return;
@@ -214,8 +217,10 @@ __global__ void __launch_bounds__(64) k_assemble_diag(Elements E, BMConst bm, co
     cx<double> a = bm_entry<double>(acc, kind, admi, bm.tau, bm.beta_im);
     if (kind == 1) a -= beta * (0.5 * bm.tau);
     else a += (admi * beta - mk(1.0, 0.0)) * 0.5;
-    if (mixed) ((cx<float> *)Aout)[(size_t)i * n + i] = cx<float>{float(a.re), float(a.im)};
-    else ((cx<double> *)Aout)[(size_t)i * n + i] = a;
+    // dense operator: entry (i, i); near-field CSR: its diagonal position
+    const size_t at = diag_index ? (size_t)diag_index[i] : (size_t)i * n + i;
+    if (mixed) ((cx<float> *)Aout)[at] = cx<float>{float(a.re), float(a.im)};
+    else ((cx<double> *)Aout)[at] = a;
 
     const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
     const double nx[3] = {E.nrm[i], E.nrm[n + i], E.nrm[2 * n + i]};
@@ -457,6 +462,23 @@ __global__ void k_row_scaling(const cx<S> *A, int n, double *d)
     if (lane == 0) d[i] = sqrt((double)n / s);
 }
 
+// the same scaling for the FMM operator, over each row's near-field entries
+// (src/NC_CommonFunctions.cpp:1344-1352)
+template <typename S>
+__global__ void k_row_scaling_csr(int n, const long long *ptr, const cx<S> *val, double *d)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double s = 0.0;
+    for (long long k = ptr[i] + lane; k < ptr[i + 1]; k += 64) {
+        const cx<S> a = val[k];
+        s += double(a.re) * double(a.re) + double(a.im) * double(a.im);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) d[i] = sqrt((double)(ptr[i + 1] - ptr[i]) / s);
+}
+
 // GMRES: h[r][k] = <V_k, w_r> for k < kcount (Hermitian), one block per (k, r) pair.
 // V layout: [basis k][rhs r][n]; w: [r][n].
 __global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, const cx<double> *V, const cx<double> *w,
@@ -625,7 +647,9 @@ constexpr int kFieldChunk = 4;
 template <typename T>
 __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const double *pts, int m, int nrhs, int rhs0,
                                                const cx<double> *pot, const cx<double> *vel, const double *ps, int nps,
-                                               const double *pw, int npw, cx<double> *phi, cx<double> *grad, int *err)
+                                               const double *pw, int npw, cx<double> *phi, cx<double> *grad, int *err,
+                                               const int *pt_ev, const int *el_ptr, const int *el_idx,
+                                               const cx<double> *fphi, const cx<double> *fgrad)
 {
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -639,7 +663,11 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
         agr[r][0] = agr[r][1] = agr[r][2] = mk(0.0, 0.0);
     }
     const T hk = T(bm.hk), k2 = T(bm.k2);
-    for (int j = lane; j < n; j += 64) {
+    // all elements (dense method) or the elements of the near clusters of the point's
+    // evaluation cluster (FMM)
+    const int k0 = el_ptr ? el_ptr[pt_ev[p]] : 0, k1 = el_ptr ? el_ptr[pt_ev[p] + 1] : n;
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        const int j = el_ptr ? el_idx[kk] : kk;
         FieldAcc f;
         f.K1 = f.K2 = mk(0.0, 0.0);
         for (int d = 0; d < 3; ++d) f.K3[d] = f.K4[d] = mk(0.0, 0.0);
@@ -688,8 +716,12 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
         if (lane != 0) continue;
         cx<double> ph = mk(v[0], v[1]) * bm.tau;
         cx<double> gr[3] = {mk(v[2], v[3]) * bm.tau, mk(v[4], v[5]) * bm.tau, mk(v[6], v[7]) * bm.tau};
-        // incident field (NC_ContributionIncidentWaves, src/NC_PostProcessing.cpp:776-866)
         const int rr = rhs0 + r;
+        if (fphi) {   // far clusters through the multipole expansion
+            ph += fphi[(size_t)rr * m + p];
+            for (int q = 0; q < 3; ++q) gr[q] += fgrad[((size_t)rr * m + p) * 3 + q];
+        }
+        // incident field (NC_ContributionIncidentWaves, src/NC_PostProcessing.cpp:776-866)
         for (int w = 0; w < npw; ++w) {
             const double *d = pw + 6 * w;
             if ((int)d[5] != rr) continue;
@@ -717,6 +749,470 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
         phi[(size_t)rr * m + p] = ph;
         if (grad)
             for (int q = 0; q < 3; ++q) grad[((size_t)rr * m + p) * 3 + q] = gr[q];
+    }
+}
+
+// ======================================================================================
+// Fast multipole operators (NumCalc's SL-FMM and direct ML-FMM: per level an aggregation
+// T, a diagonal translation D between interacting clusters and a disaggregation S;
+// src/NC_EquationSystem.cpp:305-1240, src/NC_CommonFunctions.cpp:1247-1316).
+// Layouts (p, the unit-sphere point, fastest so that lanes coalesce):
+//   T[(cl_ptr[c] + local element) * P + p]   S[e * P + p]   D[q * P + p], q = interaction
+//   t, s work vectors [nrhs][ncl * P]
+// ======================================================================================
+constexpr int kMaxTerms = 80;
+
+// h_n = j_n + i h y_n, n < N (NC_SphericalHankel, src/NC_3dFunctions.cpp:821-886): y_n by
+// upward recurrence, j_n by downward recurrence from the continued fraction of
+// j_N/j_{N-1} (modified Lentz), normalised with j_0 = sin x / x (or j_0' when j_0 ~ 0).
+__device__ void spherical_hankel(int N, double x, double h, double *hre, double *him)
+{
+    double gg[kMaxTerms], dg[kMaxTerms];
+    const double sx = sin(x), cxs = cos(x);
+    him[0] = -cxs / x;
+    if (N > 1) him[1] = -(cxs / x + sx) / x;
+    for (int i = 2; i < N; ++i) him[i] = him[i - 1] * (double)(2 * i - 1) / x - him[i - 2];
+    const double nu = (double)(N - 1);
+    double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
+    for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
+        const double b = (2.0 * (nu + j + 1.0) + 1.0) / x;
+        D = b - D;
+        if (D == 0.0) D = 1.0e-30;
+        D = 1.0 / D;
+        C = b - 1.0 / C;
+        if (C == 0.0) C = 1.0e-30;
+        di = di * C * D;
+        err = fabs(C * D - 1.0);
+    }
+    const double gnu = nu / x - 1.0 / di;
+    gg[N - 1] = 1.0;
+    dg[N - 1] = gnu;
+    for (int i = N - 2; i >= 0; --i) {
+        gg[i] = ((double)i + 2.0) / x * gg[i + 1] + dg[i + 1];
+        dg[i] = (double)i / x * gg[i] - gg[i + 1];
+    }
+    const double dp = fabs(gg[0]) > 1.0e-5 ? sx / x / gg[0] : (cxs - sx / x) / x / dg[0];
+    for (int i = 0; i < N; ++i) {
+        hre[i] = dp * gg[i];
+        him[i] = h * him[i];
+    }
+}
+
+// translation function mu(s) = sum_{n<L} (i h)^n (2n+1) h_n P_n(dhat . s)
+// (NC_BuildDforSLFMM, src/NC_EquationSystem.cpp:517-537)
+__device__ __forceinline__ cx<double> translation(int L, double c, double h, const double *hre, const double *him)
+{
+    cx<double> mu = mk(0.0, 0.0), zi = mk(1.0, 0.0);
+    double p0 = 1.0, p1 = c;
+    for (int n = 0; n < L; ++n) {
+        double pn;
+        if (n == 0) pn = 1.0;
+        else if (n == 1) pn = c;
+        else {
+            pn = (p1 * c * (double)(2 * n - 1) - p0 * (double)(n - 1)) / (double)n;
+            p0 = p1;
+            p1 = pn;
+        }
+        mu += zi * mk(hre[n], him[n]) * (pn * (double)(2 * n + 1));
+        zi = mk(-zi.im * h, zi.re * h);
+    }
+    return mu;
+}
+
+// D values of one level: one block per interaction q = (cluster i, far cluster j)
+template <typename S>
+__global__ void __launch_bounds__(128) k_fmm_D(int L, int P, double k, double h, const double *ccen, int ncl,
+                                               const int *fan_row, const int *fan_idx, const double *u, cx<S> *Dv)
+{
+    __shared__ double hre[kMaxTerms], him[kMaxTerms], dhat[3];
+    const int q = blockIdx.x;
+    const int i = fan_row[q], j = fan_idx[q];
+    if (threadIdx.x == 0) {
+        double d[3] = {ccen[3 * i] - ccen[3 * j], ccen[3 * i + 1] - ccen[3 * j + 1], ccen[3 * i + 2] - ccen[3 * j + 2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        dhat[0] = d[0] / r; dhat[1] = d[1] / r; dhat[2] = d[2] / r;
+        spherical_hankel(L, r * k, h, hre, him);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        const double c = dhat[0] * u[3 * p] + dhat[1] * u[3 * p + 1] + dhat[2] * u[3 * p + 2];
+        const cx<double> mu = translation(L, c, h, hre, him);
+        Dv[(size_t)q * P + p] = cx<S>{S(mu.re), S(mu.im)};
+    }
+}
+
+// Gauss points of the FMM aggregation rule (order 2: TR4 on triangles, 2x2 on quads,
+// GauFMBOrd3 = 2 of src/NC_EquationSystem.cpp:73)
+__device__ __forceinline__ int fmm_gauss(int nv, int g, double &s, double &t, double &w)
+{
+    if (nv == 3) {
+        s = c_tri[g][0]; t = c_tri[g][1]; w = c_tri[g][2];
+        return 4;
+    }
+    s = c_gl_x[1 + g / 2]; t = c_gl_x[1 + g % 2]; w = c_gl_w[1 + g / 2] * c_gl_w[1 + g % 2];
+    return 4;
+}
+
+// T values of one level: thread per (element, sphere point)
+// (NC_ComputeEntriesTforFMM, src/NC_EquationSystem.cpp:1150-1240)
+template <typename S>
+__global__ void __launch_bounds__(256) k_fmm_T(Elements E, BMConst bm, const double *adm, int P, const double *ccen,
+                                               const int *elem_cl, const int *elem_pos, const double *u, cx<S> *Tv)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)E.n * P) return;
+    const int e = (int)(t / P), p = (int)(t % P);
+    const int c = elem_cl[e];
+    const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
+    const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
+    LocalElem L;
+    load_local(E, e, C, L);   // nodes relative to the cluster centre
+    const int kind = E.kind[e];
+    const cx<double> ad = mk(adm[2 * e], adm[2 * e + 1]);
+    const bool has_adm = kind == 2 && (ad.re != 0.0 || ad.im != 0.0);
+    cx<double> acc = mk(0.0, 0.0);
+    for (int g = 0; g < 4; ++g) {
+        double s, tt, w;
+        fmm_gauss(L.nv, g, s, tt, w);
+        double q[3], ny[3], J;
+        point_on(L, s, tt, q, ny, J);   // q = y - C
+        const double wga = w * J;
+        const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
+        double sn, cs;
+        sincos(dw, &sn, &cs);
+        cx<double> z = mk(cs, sn);
+        if (kind == 1) {
+            z = z * wga;
+            z = mk(z.im, -z.re);   // times -i
+        } else {
+            const double d1 = (ny[0] * su[0] + ny[1] * su[1] + ny[2] * su[2]) * bm.k * bm.tau;
+            if (!has_adm) z = z * (d1 * wga);
+            else z = z * (mk(d1 - bm.hk / bm.k * ad.im, bm.hk / bm.k * ad.re) * wga);
+        }
+        acc += z;
+    }
+    acc = acc * (bm.k / kPi4);
+    Tv[(size_t)elem_pos[e] * P + p] = cx<S>{S(acc.re), S(acc.im)};
+}
+
+// T-vector of prescribed values of one level (the right-hand side's far part), thread per
+// (cluster, sphere point), summed over the cluster's elements
+__global__ void __launch_bounds__(256) k_fmm_Tvec_bc(Elements E, BMConst bm, int P, int ncl, const double *ccen,
+                                                     const int *cl_ptr, const int *cl_elem, const double *u,
+                                                     const cx<double> *bcval, int nrhs, cx<double> *tv)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncl * P) return;
+    const int c = (int)(t / P), p = (int)(t % P);
+    const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
+    const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
+    for (int r = 0; r < nrhs; ++r) {
+        cx<double> acc = mk(0.0, 0.0);
+        for (int k = cl_ptr[c]; k < cl_ptr[c + 1]; ++k) {
+            const int e = cl_elem[k];
+            const cx<double> bv = bcval[(size_t)r * E.n + e];
+            if (bv.re == 0.0 && bv.im == 0.0) continue;
+            LocalElem L;
+            load_local(E, e, C, L);
+            const int kind = E.kind[e];
+            for (int g = 0; g < 4; ++g) {
+                double s, tt, w;
+                fmm_gauss(L.nv, g, s, tt, w);
+                double q[3], ny[3], J;
+                point_on(L, s, tt, q, ny, J);
+                const double wga = w * J;
+                const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
+                double sn, cs;
+                sincos(dw, &sn, &cs);
+                const cx<double> z = mk(cs, sn);
+                if (kind == 1) {
+                    const cx<double> zr = mk((z * wga).im, -(z * wga).re);
+                    const double d1 = (ny[0] * su[0] + ny[1] * su[1] + ny[2] * su[2]) * bm.k * bm.tau;
+                    acc -= zr * bv * d1;
+                } else {
+                    acc += z * (bv * wga);
+                }
+            }
+        }
+        const double f = bm.k / kPi4 * bm.tau * (bm.hk / bm.k);
+        tv[(size_t)r * ncl * P + t] = mk(-acc.im * f, acc.re * f);
+    }
+}
+
+// S values of one level: thread per (element, sphere point)
+// (NC_BuildSforSLFMM, src/NC_EquationSystem.cpp:550-609)
+template <typename S>
+__global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, const double *ccen, const int *elem_cl,
+                                               const double *u, const double *wsph, cx<S> *Sv)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)E.n * P) return;
+    const int e = (int)(t / P), p = (int)(t % P);
+    const int c = elem_cl[e];
+    const double d[3] = {E.cen[e] - ccen[3 * c], E.cen[E.n + e] - ccen[3 * c + 1], E.cen[2 * E.n + e] - ccen[3 * c + 2]};
+    const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
+    const double d0 = (d[0] * su[0] + d[1] * su[1] + d[2] * su[2]) * bm.hk;
+    double sn, cs;
+    sincos(d0, &sn, &cs);
+    const cx<double> z0 = mk(cs, sn) * wsph[p];
+    const double dn = (su[0] * E.nrm[e] + su[1] * E.nrm[E.n + e] + su[2] * E.nrm[2 * E.n + e]) * bm.hk;
+    const cx<double> zd = mk(-z0.im * dn, z0.re * dn);
+    const cx<double> beta_tau = mk(0.0, bm.beta_im * bm.tau);
+    const cx<double> v = z0 + zd * beta_tau;
+    Sv[(size_t)e * P + p] = cx<S>{S(v.re), S(v.im)};
+}
+
+// near-field entries of the FMM operator: one thread per off-diagonal nonzero
+template <typename S>
+__global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *rows,
+                                                  const int *cols, long long nnz, cx<S> *vals, int *err)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    const int i = rows[q], j = cols[q];
+    if (i == j) return;   // diagonal: k_assemble_diag
+    BMAcc<double> acc;
+    const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
+    const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
+    LocalElem L;
+    load_local(E, j, x, L);
+    const double ratdis = ratio_to_centre(E.cen[j], E.cen[E.n + j], E.cen[2 * E.n + j], x, L.area);
+    acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
+    if (ratdis < kTolf) {
+        const int e = visit_adaptive(L, [&](double s, double t, double w) {
+            double q2[3], ny[3], J;
+            point_on(L, s, t, q2, ny, J);
+            bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * J, bm.hk, bm.k2);
+        });
+        if (e) atomicOr(err, e);
+    } else {
+        const int m = gauss_order(ratdis);
+        if (L.nv == 3) {
+            const int off = tri_rule_offset(m), cnt = tri_rule_count(m);
+            for (int g = 0; g < cnt; ++g) {
+                double q2[3], ny[3], J;
+                point_on(L, c_tri[off + g][0], c_tri[off + g][1], q2, ny, J);
+                bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], c_tri[off + g][2] * J,
+                                 bm.hk, bm.k2);
+            }
+        } else {
+            const int base = m * (m - 1) / 2;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) {
+                    double q2[3], ny[3], J;
+                    point_on(L, c_gl_x[base + a], c_gl_x[base + b], q2, ny, J);
+                    bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2],
+                                     c_gl_w[base + a] * c_gl_w[base + b] * J, bm.hk, bm.k2);
+                }
+        }
+    }
+    const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+    vals[q] = cx<S>{S(a.re), S(a.im)};
+}
+
+// ---- FMM operator apply ---------------------------------------------------------------
+// near field: one wave per row
+template <typename S>
+__global__ void __launch_bounds__(256) k_near_spmv(int n, int nrhs, const long long *ptr, const int *col,
+                                                   const cx<S> *val, const cx<double> *x, cx<double> *y)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    for (int r = 0; r < nrhs; ++r) {
+        const cx<double> *xr = x + (size_t)r * n;
+        double re = 0.0, im = 0.0;
+        for (long long k = ptr[i] + lane; k < ptr[i + 1]; k += 64) {
+            const cx<S> a = val[k];
+            const cx<double> v = xr[col[k]];
+            re += double(a.re) * v.re - double(a.im) * v.im;
+            im += double(a.re) * v.im + double(a.im) * v.re;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o);
+            im += __shfl_xor(im, o);
+        }
+        if (lane == 0) y[(size_t)r * n + i] = mk(re, im);
+    }
+}
+
+// t[c, p] = sum over the cluster's elements of T * x
+template <typename S>
+__global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, int nrhs, const int *cl_ptr,
+                                                     const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
+                                                     cx<double> *tw)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncl * P) return;
+    const int c = (int)(t / P), p = (int)(t % P);
+    for (int r = 0; r < nrhs; ++r) {
+        cx<double> acc = mk(0.0, 0.0);
+        for (int k = cl_ptr[c]; k < cl_ptr[c + 1]; ++k) {
+            const cx<S> a = Tv[(size_t)k * P + p];
+            acc += mk(double(a.re), double(a.im)) * x[(size_t)r * n + cl_elem[k]];
+        }
+        tw[(size_t)r * ncl * P + t] = acc;
+    }
+}
+
+// s[i, p] = sum over the interaction list of D * t
+template <typename S>
+__global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx,
+                                                     const cx<S> *Dv, const cx<double> *tw, cx<double> *sw)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncl * P) return;
+    const int i = (int)(t / P), p = (int)(t % P);
+    for (int r = 0; r < nrhs; ++r) {
+        cx<double> acc = mk(0.0, 0.0);
+        for (int q = fan_ptr[i]; q < fan_ptr[i + 1]; ++q) {
+            const cx<S> a = Dv[(size_t)q * P + p];
+            acc += mk(double(a.re), double(a.im)) * tw[(size_t)r * ncl * P + (size_t)fan_idx[q] * P + p];
+        }
+        sw[(size_t)r * ncl * P + t] = acc;
+    }
+}
+
+// y[e] += sum_p S[e, p] s[cluster(e), p]; one wave per element
+template <typename S>
+__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int nrhs, const int *elem_cl,
+                                                     const cx<S> *Sv, const cx<double> *sw, cx<double> *y)
+{
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= n) return;
+    const int c = elem_cl[e];
+    for (int r = 0; r < nrhs; ++r) {
+        double re = 0.0, im = 0.0;
+        for (int p = lane; p < P; p += 64) {
+            const cx<S> a = Sv[(size_t)e * P + p];
+            const cx<double> v = sw[(size_t)r * ncl * P + (size_t)c * P + p];
+            re += double(a.re) * v.re - double(a.im) * v.im;
+            im += double(a.re) * v.im + double(a.im) * v.re;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o);
+            im += __shfl_xor(im, o);
+        }
+        if (lane == 0) y[(size_t)r * n + e] += mk(re, im);
+    }
+}
+
+// ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
+// far-field signature of the solution at level 0 (NC_ContributionTvecFMM,
+// src/NC_PostProcessing.cpp:302-364): thread per (cluster, sphere point)
+__global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, int P, int ncl, const double *ccen,
+                                                      const int *cl_ptr, const int *cl_elem, const double *u,
+                                                      const cx<double> *pot, const cx<double> *vel, int nrhs,
+                                                      cx<double> *tv)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncl * P) return;
+    const int c = (int)(t / P), p = (int)(t % P);
+    const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
+    const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
+    const double h = bm.hk / bm.k;
+    for (int r = 0; r < nrhs; ++r) {
+        cx<double> acc = mk(0.0, 0.0);
+        for (int k = cl_ptr[c]; k < cl_ptr[c + 1]; ++k) {
+            const int e = cl_elem[k];
+            LocalElem L;
+            load_local(E, e, C, L);
+            const cx<double> pe = pot[(size_t)r * E.n + e], ve = vel[(size_t)r * E.n + e];
+            for (int g = 0; g < 4; ++g) {
+                double s, tt, w;
+                fmm_gauss(L.nv, g, s, tt, w);
+                double q[3], ny[3], J;
+                point_on(L, s, tt, q, ny, J);
+                const double wga = w * J;
+                const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
+                double sn, cs;
+                sincos(dw, &sn, &cs);
+                const cx<double> zg = mk(cs, sn);
+                const double dn = (ny[0] * su[0] + ny[1] * su[1] + ny[2] * su[2]) * bm.k;
+                const cx<double> zH = zg * (dn * wga);
+                const cx<double> zG = mk(-zg.im * wga * h, zg.re * wga * h);
+                acc += zH * pe - zG * ve;
+            }
+        }
+        acc = acc * (bm.k / kPi4);
+        if (bm.tau != 1.0) acc = acc * bm.tau;
+        tv[(size_t)r * ncl * P + t] = acc;
+    }
+}
+
+// translation of the level-0 signatures to the evaluation clusters:
+// Z[ipc, p] = sum over far BE clusters of mu(C_ipc - C_ic, s_p) T[ic, p] w_p.
+// One block per far pair computes its Hankel functions; pairs of one evaluation cluster
+// are contiguous and reduced in a second pass.
+__global__ void __launch_bounds__(128) k_fmm_eval_pair(int L, int P, double k, double h, const double *ecen,
+                                                       const double *ccen, const int *pair_ev, const int *pair_cl,
+                                                       const double *u, const double *wsph, const cx<double> *tv,
+                                                       int ncl, int nrhs, cx<double> *zpair)
+{
+    __shared__ double hre[kMaxTerms], him[kMaxTerms], dhat[3];
+    const int q = blockIdx.x;
+    const int a = pair_ev[q], c = pair_cl[q];
+    if (threadIdx.x == 0) {
+        double d[3] = {ecen[3 * a] - ccen[3 * c], ecen[3 * a + 1] - ccen[3 * c + 1], ecen[3 * a + 2] - ccen[3 * c + 2]};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        dhat[0] = d[0] / r; dhat[1] = d[1] / r; dhat[2] = d[2] / r;
+        spherical_hankel(L, r * k, h, hre, him);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        const double cc = dhat[0] * u[3 * p] + dhat[1] * u[3 * p + 1] + dhat[2] * u[3 * p + 2];
+        const cx<double> mu = translation(L, cc, h, hre, him) * wsph[p];
+        for (int r = 0; r < nrhs; ++r)
+            zpair[((size_t)r * gridDim.x + q) * P + p] = mu * tv[(size_t)r * ncl * P + (size_t)c * P + p];
+    }
+}
+
+__global__ void k_fmm_eval_sum(int nev, int P, int npair, const int *ev_ptr, int nrhs, const cx<double> *zpair,
+                               cx<double> *z)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nev * P) return;
+    const int a = (int)(t / P), p = (int)(t % P);
+    for (int r = 0; r < nrhs; ++r) {
+        cx<double> s = mk(0.0, 0.0);
+        for (int q = ev_ptr[a]; q < ev_ptr[a + 1]; ++q) s += zpair[((size_t)r * npair + q) * P + p];
+        z[(size_t)r * nev * P + t] = s;
+    }
+}
+
+// far part at each evaluation point: phi = sum_p e^{i h k s_p.(x - C)} Z[p], grad likewise
+__global__ void __launch_bounds__(256) k_fmm_eval_far(int m, int P, int nev, double hk, const double *pts,
+                                                      const int *pt_ev, const double *ecen, const double *u,
+                                                      int nrhs, const cx<double> *z, cx<double> *fphi,
+                                                      cx<double> *fgrad)
+{
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (x >= m) return;
+    const int a = pt_ev[x];
+    const double d[3] = {pts[3 * x] - ecen[3 * a], pts[3 * x + 1] - ecen[3 * a + 1], pts[3 * x + 2] - ecen[3 * a + 2]};
+    for (int r = 0; r < nrhs; ++r) {
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int p = lane; p < P; p += 64) {
+            const double sp = (d[0] * u[3 * p] + d[1] * u[3 * p + 1] + d[2] * u[3 * p + 2]) * hk;
+            double sn, cs;
+            sincos(sp, &sn, &cs);
+            const cx<double> zi = mk(cs, sn) * z[(size_t)r * nev * P + (size_t)a * P + p];
+            v[0] += zi.re;
+            v[1] += zi.im;
+            for (int kk = 0; kk < 3; ++kk) {
+                const double dk = hk * u[3 * p + kk];
+                v[2 + 2 * kk] += -zi.im * dk;
+                v[3 + 2 * kk] += zi.re * dk;
+            }
+        }
+        for (int c = 0; c < 8; ++c)
+            for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+        if (lane == 0) {
+            fphi[(size_t)r * m + x] = mk(v[0], v[1]);
+            for (int kk = 0; kk < 3; ++kk) fgrad[((size_t)r * m + x) * 3 + kk] = mk(v[2 + 2 * kk], v[3 + 2 * kk]);
+        }
     }
 }
 
@@ -788,22 +1284,24 @@ void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const
 }
 
 void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
-                 int nrhs, void *rhs_src, int *err)
+                 int nrhs, void *rhs_src, int *err, const int *elem_cluster, const unsigned char *cluster_near,
+                 int nclusters)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk)
         k_rhs_sources<<<blocks_for(D.n, 4), 256, 0, s>>>(E, make_bm(b), src, nsrc, (const cx<double> *)bcval, nrhs, r0,
-                                                         (cx<double> *)rhs_src, err);
+                                                         (cx<double> *)rhs_src, err, elem_cluster, cluster_near,
+                                                         nclusters);
 }
 
 void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const void *bcval,
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
-                   void *A, int mixed, void *rhs)
+                   void *A, int mixed, void *rhs, const long long *diag_index)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     k_assemble_diag<<<blocks_for(D.n, 64), 64, 0, s>>>(E, make_bm(b), adm, (const cx<double> *)bcval, nrhs, ps, nps, pw,
                                                        npw, (const cx<double> *)rhs_src, has_src, A, mixed,
-                                                       (cx<double> *)rhs);
+                                                       (cx<double> *)rhs, diag_index);
 }
 
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d, void *y)
@@ -883,6 +1381,11 @@ void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d)
     if (mixed) k_row_scaling<float><<<blocks_for(n, 4), 256, 0, s>>>((const cx<float> *)A, n, d);
     else k_row_scaling<double><<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d);
 }
+void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val, int mixed, double *d)
+{
+    if (mixed) k_row_scaling_csr<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<float> *)val, d);
+    else k_row_scaling_csr<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<double> *)val, d);
+}
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h)
 {
     k_multidot<<<dim3(kcount, nrhs), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V, (const cx<double> *)w,
@@ -918,19 +1421,132 @@ void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double
 }
 void field(hipStream_t s, const DevElements &D, const BMParams &b, const double *pts, int m, int nrhs, const void *pot,
            const void *vel, const double *ps, int nps, const double *pw, int npw, void *phi, void *grad, int fp32,
-           int *err)
+           int *err, const int *pt_ev, const int *el_ptr, const int *el_idx, const void *fphi, const void *fgrad)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     for (int r0 = 0; r0 < nrhs; r0 += kFieldChunk) {
         if (fp32)
             k_field<float><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                             (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
-                                                            (cx<double> *)grad, err);
+                                                            (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
+                                                            (const cx<double> *)fphi, (const cx<double> *)fgrad);
         else
             k_field<double><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                              (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
-                                                             (cx<double> *)grad, err);
+                                                             (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
+                                                             (const cx<double> *)fphi, (const cx<double> *)fgrad);
     }
+}
+
+// ---- FMM launchers -----------------------------------------------------------------
+void fmm_D(hipStream_t s, int L, int P, double k, double h, const double *ccen, int ncl, const int *fan_row,
+           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed)
+{
+    if (nfan == 0) return;
+    if (mixed) k_fmm_D<float><<<nfan, 128, 0, s>>>(L, P, k, h, ccen, ncl, fan_row, fan_idx, u, (cx<float> *)Dv);
+    else k_fmm_D<double><<<nfan, 128, 0, s>>>(L, P, k, h, ccen, ncl, fan_row, fan_idx, u, (cx<double> *)Dv);
+}
+void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, int P, const double *ccen,
+           const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    const unsigned g = blocks_for((size_t)D.n * P, 256);
+    if (mixed) k_fmm_T<float><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<float> *)Tv);
+    else k_fmm_T<double><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<double> *)Tv);
+}
+void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
+                 const int *cl_ptr, const int *cl_elem, const double *u, const void *bcval, int nrhs, void *tv)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    k_fmm_Tvec_bc<<<blocks_for((size_t)ncl * P, 256), 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u,
+                                                                   (const cx<double> *)bcval, nrhs, (cx<double> *)tv);
+}
+void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
+           const double *u, const double *w, void *Sv, int mixed)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    const unsigned g = blocks_for((size_t)D.n * P, 256);
+    if (mixed) k_fmm_S<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<float> *)Sv);
+    else k_fmm_S<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<double> *)Sv);
+}
+void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
+              long long nnz, void *vals, int mixed, int *err)
+{
+    if (nnz == 0) return;
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    const unsigned g = blocks_for(nnz, 128);
+    if (mixed) k_fmm_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
+    else k_fmm_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
+}
+void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
+               const void *x, void *y)
+{
+    if (mixed)
+        k_near_spmv<float><<<blocks_for(n, 4), 256, 0, s>>>(n, nrhs, ptr, col, (const cx<float> *)val,
+                                                            (const cx<double> *)x, (cx<double> *)y);
+    else
+        k_near_spmv<double><<<blocks_for(n, 4), 256, 0, s>>>(n, nrhs, ptr, col, (const cx<double> *)val,
+                                                             (const cx<double> *)x, (cx<double> *)y);
+}
+void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
+                 int mixed, const void *x, void *tw)
+{
+    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    if (mixed)
+        k_fmm_apply_T<float><<<g, 256, 0, s>>>(n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<float> *)Tv,
+                                               (const cx<double> *)x, (cx<double> *)tw);
+    else
+        k_fmm_apply_T<double><<<g, 256, 0, s>>>(n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<double> *)Tv,
+                                                (const cx<double> *)x, (cx<double> *)tw);
+}
+void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
+                 int mixed, const void *tw, void *sw)
+{
+    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    if (mixed)
+        k_fmm_apply_D<float><<<g, 256, 0, s>>>(ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv,
+                                               (const cx<double> *)tw, (cx<double> *)sw);
+    else
+        k_fmm_apply_D<double><<<g, 256, 0, s>>>(ncl, P, nrhs, fan_ptr, fan_idx, (const cx<double> *)Dv,
+                                                (const cx<double> *)tw, (cx<double> *)sw);
+}
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const void *Sv, int mixed,
+                 const void *sw, void *y)
+{
+    if (mixed)
+        k_fmm_apply_S<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ncl, P, nrhs, elem_cl, (const cx<float> *)Sv,
+                                                              (const cx<double> *)sw, (cx<double> *)y);
+    else
+        k_fmm_apply_S<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ncl, P, nrhs, elem_cl, (const cx<double> *)Sv,
+                                                               (const cx<double> *)sw, (cx<double> *)y);
+}
+void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
+                  const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
+                  void *tv)
+{
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    k_fmm_Tvec_sol<<<blocks_for((size_t)ncl * P, 256), 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u,
+                                                                    (const cx<double> *)pot, (const cx<double> *)vel,
+                                                                    nrhs, (cx<double> *)tv);
+}
+void fmm_eval_pairs(hipStream_t s, int L, int P, double k, double h, const double *ecen, const double *ccen,
+                    const int *pair_ev, const int *pair_cl, int npair, const double *u, const double *w, const void *tv,
+                    int ncl, int nrhs, void *zpair)
+{
+    if (npair == 0) return;
+    k_fmm_eval_pair<<<npair, 128, 0, s>>>(L, P, k, h, ecen, ccen, pair_ev, pair_cl, u, w, (const cx<double> *)tv, ncl,
+                                          nrhs, (cx<double> *)zpair);
+}
+void fmm_eval_sum(hipStream_t s, int nev, int P, int npair, const int *ev_ptr, int nrhs, const void *zpair, void *z)
+{
+    k_fmm_eval_sum<<<blocks_for((size_t)nev * P, 256), 256, 0, s>>>(nev, P, npair, ev_ptr, nrhs,
+                                                                    (const cx<double> *)zpair, (cx<double> *)z);
+}
+void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
+                  const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad)
+{
+    k_fmm_eval_far<<<blocks_for(m, 4), 256, 0, s>>>(m, P, nev, hk, pts, pt_ev, ecen, u, nrhs, (const cx<double> *)z,
+                                                    (cx<double> *)fphi, (cx<double> *)fgrad);
 }
 
 }  // namespace ncgk

@@ -39,10 +39,11 @@ void assemble_far(hipStream_t s, const DevElements &D, const BMParams &b, const 
 void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
                    const int *cols, long long npairs, void *A, int mixed, int *err);
 void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
-                 int nrhs, void *rhs_src, int *err);
+                 int nrhs, void *rhs_src, int *err, const int *elem_cluster = nullptr,
+                 const unsigned char *cluster_near = nullptr, int nclusters = 0);
 void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const void *bcval,
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
-                   void *A, int mixed, void *rhs);
+                   void *A, int mixed, void *rhs, const long long *diag_index = nullptr);
 
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d,
             void *y);
@@ -60,6 +61,7 @@ void cgs_update3(hipStream_t s, int n, int nrhs, const void *sc, const int *acti
 void axpby(hipStream_t s, int n, int nrhs, const void *coef, const void *x, size_t ldx, void *y, size_t ldy);
 void scale_vec(hipStream_t s, int n, int nrhs, const double *d, void *x);
 void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d);
+void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val, int mixed, double *d);
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h);
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add);
 void lu_factor(hipStream_t s, void *A, int n);
@@ -68,6 +70,36 @@ void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double
                 const void *x, void *pot, void *vel);
 void field(hipStream_t s, const DevElements &D, const BMParams &b, const double *pts, int m, int nrhs, const void *pot,
            const void *vel, const double *ps, int nps, const double *pw, int npw, void *phi, void *grad, int fp32,
-           int *err);
+           int *err, const int *pt_ev = nullptr, const int *el_ptr = nullptr, const int *el_idx = nullptr,
+           const void *fphi = nullptr, const void *fgrad = nullptr);
+
+// fast multipole operators (layouts in ncg_kernels.hip)
+void fmm_D(hipStream_t s, int L, int P, double k, double h, const double *ccen, int ncl, const int *fan_row,
+           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed);
+void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, int P, const double *ccen,
+           const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed);
+void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
+                 const int *cl_ptr, const int *cl_elem, const double *u, const void *bcval, int nrhs, void *tv);
+void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
+           const double *u, const double *w, void *Sv, int mixed);
+void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
+              long long nnz, void *vals, int mixed, int *err);
+void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
+               const void *x, void *y);
+void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
+                 int mixed, const void *x, void *tw);
+void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
+                 int mixed, const void *tw, void *sw);
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const void *Sv, int mixed,
+                 const void *sw, void *y);
+void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
+                  const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
+                  void *tv);
+void fmm_eval_pairs(hipStream_t s, int L, int P, double k, double h, const double *ecen, const double *ccen,
+                    const int *pair_ev, const int *pair_cl, int npair, const double *u, const double *w, const void *tv,
+                    int ncl, int nrhs, void *zpair);
+void fmm_eval_sum(hipStream_t s, int nev, int P, int npair, const int *ev_ptr, int nrhs, const void *zpair, void *z);
+void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
+                  const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad);
 
 }  // namespace ncgk

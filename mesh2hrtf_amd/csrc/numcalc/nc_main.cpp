@@ -36,6 +36,101 @@ void stamp(std::ostream &os, const char *what)
     nc::fatal(log, std::string(where) + " failed (" + std::to_string(rc) + "): " + ncg_last_error());
 }
 
+// analysis type, cluster and expansion tables of one frequency step
+// (NC_FrequencyInformations src/NC_Main.cpp:1051-1153 and src/NC_Main.cpp:491-522)
+void method_info(std::ostream &os, const ncg_fmm_info &fi, bool first, int n)
+{
+    static const char *names[4] = {"Traditional BEM", "Single level fast multipole BEM", "",
+                                   "Direct multilevel fast multipole BEM"};
+    os << "\n" << names[fi.method] << std::endl;
+    if (first) {
+        os << "\nNumber of equations = " << n << std::endl;
+        if (fi.method == 0) os << "Number of entries of the coefficient matrix = " << (long long)n * n << std::endl;
+    }
+    if (fi.method == 0) return;
+    os << "\nInformations about clusters:" << std::endl;
+    os << "   Level    Num. of clusters    Maximum radius   Minimum radius" << std::endl;
+    for (int l = 0; l < fi.levels; ++l)
+        os << "   " << std::setw(3) << l << "       " << std::setw(6) << fi.clusters[l] << "                " << std::setw(6)
+           << fi.radius_max[l] << "         " << std::setw(6) << fi.radius_min[l] << std::endl;
+    if (fi.eval_clusters)
+        os << " resu. nets" << "  " << std::setw(6) << fi.eval_clusters << "                " << std::setw(6)
+           << fi.eval_radius_max << "         " << std::setw(6) << fi.eval_radius_min << std::endl;
+    os << "\nInformations about the FMM expansion:" << std::endl;
+    os << "   Level    N. expa. terms    N. p. theta    N. p. phi    N. p. sphere" << std::endl;
+    for (int l = 0; l < fi.levels; ++l)
+        os << "   " << std::setw(3) << l << "       " << std::setw(5) << fi.expansion_terms[l] << "           "
+           << std::setw(5) << fi.points_theta[l] << "           " << std::setw(5) << fi.points_phi[l] << "         "
+           << std::setw(5) << fi.points_sphere[l] << std::endl;
+}
+
+// near/T/D/S nonzero summary written when the FMM matrices are set up
+// (NC_AllocateSDTmtxsSLFMM src/NC_Addresses.cpp:1580-1595, ...MLFMM :1611-1653)
+void fmm_nonzeros(std::ostream &os, const ncg_fmm_info &fi)
+{
+    long long st = 0, sd = 0, ss = 0;
+    for (int l = 0; l < fi.levels; ++l) {
+        st += fi.nnz_t[l];
+        sd += fi.nnz_d[l];
+        ss += fi.nnz_s[l];
+    }
+    os << "\nNonzeros of the near field matrix ..................... = " << fi.nnz_near << std::endl;
+    os << "Nonzeros of the T-matrix .............................. = " << st << std::endl;
+    if (fi.method == 3)
+        for (int l = 0; l < fi.levels; ++l)
+            os << "    Nonzeros of T_lev_" << l << " ............................... = " << fi.nnz_t[l] << std::endl;
+    os << (fi.method == 3 ? "Nonzeros of the D-matrices ............................ = "
+                          : "Nonzeros of the D-matrix .............................. = ")
+       << sd << std::endl;
+    if (fi.method == 3)
+        for (int l = 0; l < fi.levels; ++l)
+            os << "    Nonzeros of D_lev_" << l << " ............................... = " << fi.nnz_d[l] << std::endl;
+    os << "Nonzeros of the S-matrix .............................. = " << ss << std::endl;
+    if (fi.method == 3)
+        for (int l = 0; l < fi.levels; ++l)
+            os << "    Nonzeros of S_lev_" << l << " ............................... = " << fi.nnz_s[l] << std::endl;
+    os << "Sum of nonzeros of the FMBEM matrices ................. = " << fi.nnz_near + st + sd + ss << std::endl;
+}
+
+// RAM estimate of the -estimate_ram run (src/NC_Main.cpp:524-731), with the
+// reference's 32-bit counters
+double ram_estimate(const ncg_fmm_info &fi, int n)
+{
+    unsigned int nint = 0, ncmplx = (unsigned int)n;
+    if (fi.method == 0) {
+        std::cout << "TBEM: \n";
+        std::cout << "Number of entries right hand side: " << ncmplx << "\n";
+        std::cout << "Number of Matrix entries: " << ncmplx * ncmplx << "\n";
+        ncmplx += (unsigned int)n * (unsigned int)n;
+    } else {
+        const int fin = fi.levels - 1;
+        ncmplx += (unsigned int)fi.points_sphere[fin] * 2u;
+        nint += (unsigned int)fi.nnz_near + (unsigned int)n + 1u;
+        ncmplx += (unsigned int)fi.nnz_near;
+        std::cout << (fi.method == 1 ? "SLFMM:\nNearfield: " : "MLFMM: \nNearfield ") << fi.nnz_near << "\n";
+        for (int l = 0; l < fi.levels; ++l) {
+            const unsigned int P = (unsigned int)fi.points_sphere[l], nc = (unsigned int)fi.clusters[l];
+            nint += (unsigned int)fi.nnz_t[l] + nc * P + 1u;
+            ncmplx += (unsigned int)fi.nnz_t[l];
+            std::cout << (fi.method == 1 ? "T Matrix: " : "Tmatrix level " + std::to_string(l) + ": ") << fi.nnz_t[l] << "\n";
+        }
+        for (int l = 0; l < fi.levels; ++l) {
+            const unsigned int P = (unsigned int)fi.points_sphere[l], nc = (unsigned int)fi.clusters[l];
+            nint += (unsigned int)fi.nnz_d[l] + nc * P + (fi.method == 3 ? 1u : 0u);
+            ncmplx += (unsigned int)fi.nnz_d[l];
+            std::cout << (fi.method == 1 ? "D Matrix: " : "D Matrix level " + std::to_string(l) + ": ") << fi.nnz_d[l] << "\n";
+        }
+        for (int l = 0; l < fi.levels; ++l) {
+            nint += (unsigned int)fi.nnz_s[l] + (unsigned int)n + 1u;
+            ncmplx += (unsigned int)fi.nnz_s[l];
+            std::cout << (fi.method == 1 ? "S Matrix: " : "S Matrix level " + std::to_string(l) + ": ") << fi.nnz_s[l] << "\n";
+        }
+    }
+    std::cout << "Total Nr. of Integers: " << nint << "\n";
+    std::cout << "Total Nr. of Cmplxdble: " << ncmplx << "\n";
+    return (double)(nint * sizeof(int) + ncmplx * 16) / 1000.0 / 1000.0 / 1000.0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -128,25 +223,34 @@ int main(int argc, char **argv)
     const int m = (int)evalp.size() / 3;
 
     ncg_problem *prob = nullptr;
-    std::vector<int> nvv(n), kind(n);
+    std::vector<int> nvv(n), kind(n), nodes(4 * (size_t)n, -1), groups(n);
     std::vector<double> xyz(12 * (size_t)n, 0.0);
     for (int q = 0; q < n; ++q) {
         const int e = be[q];
         nvv[q] = P.elem_nv[e];
         kind[q] = P.ibval[e];
-        for (int a = 0; a < nvv[q]; ++a)
+        groups[q] = P.elem_group[e];
+        for (int a = 0; a < nvv[q]; ++a) {
+            nodes[4 * (size_t)q + a] = P.elem_conn[4 * (size_t)e + a];
             for (int d = 0; d < 3; ++d) xyz[12 * (size_t)q + 3 * a + d] = P.node_xyz[3 * (size_t)P.elem_conn[4 * (size_t)e + a] + d];
+        }
     }
+    ncg_geometry g;
+    memset(&g, 0, sizeof(g));
+    g.node_index = nodes.data();
+    g.element_group = groups.data();
+    g.method.method_bem = P.method_bem;
+    g.method.min_expansion_terms = P.min_terms;
+    g.method.cluster_edge = P.clus_edge;
+    g.n_elements = n;
+    g.nodes_per_element = nvv.data();
+    g.node_xyz = xyz.data();
+    g.bc_kind = kind.data();
+    g.n_eval = m;
+    g.eval_xyz = m ? evalp.data() : nullptr;
+    g.harmonic_time_factor = P.h;
+    g.interior = P.interior;
     if (!estimate_ram) {
-        ncg_geometry g;
-        g.n_elements = n;
-        g.nodes_per_element = nvv.data();
-        g.node_xyz = xyz.data();
-        g.bc_kind = kind.data();
-        g.n_eval = m;
-        g.eval_xyz = m ? evalp.data() : nullptr;
-        g.harmonic_time_factor = P.h;
-        g.interior = P.interior;
         const int rc = ncg_problem_create(&g, device, &prob);
         if (rc) gpu_fatal(log, "ncg_problem_create", rc);
     }
@@ -166,18 +270,13 @@ int main(int argc, char **argv)
         std::cout << "\nStep " << step + 1 << ", Frequency = " << freq << " Hz" << std::endl;
         log << "\n\n\n>> S T E P   N U M B E R   A N D   F R E Q U E N C Y <<\n" << std::endl;
         log << "Step " << step + 1 << ", Frequency = " << freq << " Hz" << std::endl;
-        if (step == istart) {
-            for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log}) {
-                *os << "\nTraditional BEM" << std::endl;
-                *os << "\nNumber of equations = " << n << std::endl;
-                *os << "Number of entries of the coefficient matrix = " << (long long)n * n << std::endl;
-            }
-        }
         if (estimate_ram) {
-            // TBEM estimate of src/NC_Main.cpp:535-541, 722-731 (32-bit counters as in the reference)
-            unsigned int nint = 0, ncmplx = (unsigned int)n;
-            ncmplx += (unsigned int)n * (unsigned int)n;
-            const double gb = (double)(nint * sizeof(int) + ncmplx * 16) / 1000.0 / 1000.0 / 1000.0;
+            ncg_fmm_info fi;
+            const int rc = ncg_fmm_plan(&g, k, &fi);
+            if (rc) gpu_fatal(log, "ncg_fmm_plan", rc);
+            for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log})
+                method_info(*os, fi, step == istart, n);
+            const double gb = ram_estimate(fi, n);
             std::cout << "RAM Estimation: " << gb << " GByte\n";
             estim << step + 1 << " " << freq << " " << gb << "\n";
             continue;
@@ -226,6 +325,13 @@ int main(int argc, char **argv)
         fr.plane_waves = pw.empty() ? nullptr : pw.data();
         int rc = ncg_assemble(prob, &fr, precision);
         if (rc) gpu_fatal(log, "ncg_assemble", rc);
+        ncg_fmm_info fi;
+        rc = ncg_problem_fmm_info(prob, &fi);
+        if (rc) gpu_fatal(log, "ncg_problem_fmm_info", rc);
+        for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log})
+            method_info(*os, fi, step == istart, n);
+        if (fi.warning[0]) nc::warning(log, fi.warning);
+        if (fi.method) fmm_nonzeros(log, fi);
         const time_t ta = time(nullptr);
         // solve (src/NC_Main.cpp:833-866, NC_IterativeSolverCGS)
         ncg_solver_options so;
@@ -245,7 +351,7 @@ int main(int argc, char **argv)
             log << "\n" << (so.solver == NCG_SOLVER_CGS && precision == NCG_FP64 ? "CGS" : "GMRES")
                 << " solver: number of iterations = " << st.iterations[0] << ", relative error = " << st.relative_residual[0]
                 << std::endl;
-            if (rc == NCG_E_NOCONV && precision == NCG_FP64) {
+            if (rc == NCG_E_NOCONV && precision == NCG_FP64 && fi.method == 0) {
                 // TBEM fallback of src/NC_CommonFunctions.cpp:1181-1191, solved with the
                 // assembled right-hand side (the reference passes the unconverged iterate)
                 nc::warning(log, "Iteration method CGS failed, direct method is used!");

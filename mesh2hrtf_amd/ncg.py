@@ -27,11 +27,41 @@ class NcgError(RuntimeError):
         self.code = code
 
 
+class _Method(ctypes.Structure):
+    _fields_ = [("method_bem", ctypes.c_int32), ("min_expansion_terms", ctypes.c_int32),
+                ("cluster_edge", ctypes.c_double)]
+
+
 class _Geometry(ctypes.Structure):
     _fields_ = [("n_elements", ctypes.c_int32), ("nodes_per_element", ctypes.c_void_p),
                 ("node_xyz", ctypes.c_void_p), ("bc_kind", ctypes.c_void_p), ("n_eval", ctypes.c_int32),
                 ("eval_xyz", ctypes.c_void_p), ("harmonic_time_factor", ctypes.c_double),
-                ("interior", ctypes.c_int32)]
+                ("interior", ctypes.c_int32), ("node_index", ctypes.c_void_p), ("element_group", ctypes.c_void_p),
+                ("method", _Method)]
+
+
+class _FmmInfo(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("levels", ctypes.c_int32), ("clusters", ctypes.c_int32 * 10),
+                ("radius_max", ctypes.c_double * 10), ("radius_min", ctypes.c_double * 10),
+                ("expansion_terms", ctypes.c_int32 * 10), ("points_theta", ctypes.c_int32 * 10),
+                ("points_phi", ctypes.c_int32 * 10), ("points_sphere", ctypes.c_int32 * 10),
+                ("nnz_near", ctypes.c_int64), ("nnz_t", ctypes.c_int64 * 10), ("nnz_d", ctypes.c_int64 * 10),
+                ("nnz_s", ctypes.c_int64 * 10), ("canceled_pairs", ctypes.c_int64 * 10),
+                ("far_pairs", ctypes.c_int64 * 10), ("eval_clusters", ctypes.c_int32),
+                ("eval_radius_max", ctypes.c_double), ("eval_radius_min", ctypes.c_double),
+                ("warning", ctypes.c_char * 256)]
+
+
+def _info_dict(i):
+    L = i.levels
+    return dict(method=i.method, levels=L, clusters=list(i.clusters[:L]), radius_max=list(i.radius_max[:L]),
+                radius_min=list(i.radius_min[:L]), expansion_terms=list(i.expansion_terms[:L]),
+                points_theta=list(i.points_theta[:L]), points_phi=list(i.points_phi[:L]),
+                points_sphere=list(i.points_sphere[:L]), nnz_near=i.nnz_near, nnz_t=list(i.nnz_t[:L]),
+                nnz_d=list(i.nnz_d[:L]), nnz_s=list(i.nnz_s[:L]), canceled_pairs=list(i.canceled_pairs[:L]),
+                far_pairs=list(i.far_pairs[:L]), eval_clusters=i.eval_clusters,
+                eval_radius_max=i.eval_radius_max, eval_radius_min=i.eval_radius_min,
+                warning=i.warning.decode())
 
 
 class _Frequency(ctypes.Structure):
@@ -54,8 +84,9 @@ class _Stats(ctypes.Structure):
 
 
 # every entry point include/ncg.h declares
-EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_problem_create", "ncg_problem_destroy", "ncg_assemble",
-           "ncg_solve", "ncg_get_solution", "ncg_field", "ncg_get_system", "ncg_apply", "ncg_timings")
+EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_create", "ncg_problem_destroy",
+           "ncg_assemble", "ncg_solve", "ncg_problem_fmm_info", "ncg_get_solution", "ncg_field", "ncg_get_system",
+           "ncg_apply", "ncg_timings")
 
 _lib = None
 
@@ -86,25 +117,60 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
 
 
-class Problem:
-    """One boundary-element problem resident on a GPU (geometry, near-pair table)."""
+class _GeometryArrays:
+    """Keeps the arrays an _Geometry points to alive."""
 
-    def __init__(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, device=0):
-        self.lib = load()
+    def __init__(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, node_index=None, group=None,
+                 method_bem=0, min_terms=0, cluster_edge=0.0):
         self.nv = np.ascontiguousarray(nv, np.int32)
         n = len(self.nv)
         x = np.zeros((n, 4, 3))
         xyz = np.asarray(xyz, np.float64)
         x[:, :xyz.shape[1], :] = xyz
         self.xyz = np.ascontiguousarray(x)
-        self.kind = np.ascontiguousarray(kind, np.int32)
+        self.kind = np.ascontiguousarray(kind if kind is not None else np.zeros(n), np.int32)
         self.eval = np.ascontiguousarray(eval_xyz if eval_xyz is not None else np.zeros((0, 3)), np.float64)
+        self.node = None
+        if node_index is not None:
+            ni = np.full((n, 4), -1, np.int32)
+            node_index = np.asarray(node_index)
+            ni[:, :node_index.shape[1]] = node_index
+            self.node = np.ascontiguousarray(ni)
+        self.group = None if group is None else np.ascontiguousarray(group, np.int32)
         self.n, self.m = n, len(self.eval)
-        g = _Geometry(n, self.nv.ctypes.data, self.xyz.ctypes.data, self.kind.ctypes.data, self.m,
-                      self.eval.ctypes.data if self.m else None, float(h), int(interior))
+        self.g = _Geometry(n, self.nv.ctypes.data, self.xyz.ctypes.data, self.kind.ctypes.data, self.m,
+                           self.eval.ctypes.data if self.m else None, float(h), int(interior),
+                           self.node.ctypes.data if self.node is not None else None,
+                           self.group.ctypes.data if self.group is not None else None,
+                           _Method(int(method_bem), int(min_terms), float(cluster_edge)))
+
+
+def fmm_plan(nv, xyz, k, eval_xyz=None, node_index=None, method_bem=4, min_terms=0, cluster_edge=0.0):
+    """NumCalc's method and FMM plan at wave number k (host only, no device)."""
+    ga = _GeometryArrays(nv, xyz, None, eval_xyz, node_index=node_index, method_bem=method_bem,
+                         min_terms=min_terms, cluster_edge=cluster_edge)
+    info = _FmmInfo()
+    _check(load().ncg_fmm_plan(ctypes.byref(ga.g), ctypes.c_double(k), ctypes.byref(info)))
+    return _info_dict(info)
+
+
+class Problem:
+    """One boundary-element problem resident on a GPU (geometry, near-pair table)."""
+
+    def __init__(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, device=0, node_index=None, group=None,
+                 method_bem=0, min_terms=0, cluster_edge=0.0):
+        self.lib = load()
+        self._ga = _GeometryArrays(nv, xyz, kind, eval_xyz, h, interior, node_index, group, method_bem, min_terms,
+                                   cluster_edge)
+        self.n, self.m = self._ga.n, self._ga.m
         self._h = ctypes.c_void_p()
-        _check(self.lib.ncg_problem_create(ctypes.byref(g), int(device), ctypes.byref(self._h)))
+        _check(self.lib.ncg_problem_create(ctypes.byref(self._ga.g), int(device), ctypes.byref(self._h)))
         self.nrhs = 0
+
+    def fmm_info(self):
+        info = _FmmInfo()
+        _check(self.lib.ncg_problem_fmm_info(self._h, ctypes.byref(info)))
+        return _info_dict(info)
 
     def close(self):
         if self._h:

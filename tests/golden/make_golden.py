@@ -103,6 +103,31 @@ def cases():
     C["gauss"] = dict(prj=P.Project(
         mesh=ico2, freqs=[1200.0], eval_grid=P.fibonacci_grid(50, 0.9), method_solver=4,
         point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], title="direct solver"))
+    # fast multipole projects (NC.inp methodBEM 1 = SL-FMM, 4 = ML-FMM)
+    ico3 = P.icosphere(3, 0.1)
+    n3 = len(ico3.elems)
+    C["fmm_sl_point"] = dict(prj=P.Project(
+        mesh=ico3, freqs=[1000.0, 3000.0], eval_grid=P.fibonacci_grid(100, 0.5), method_bem=1,
+        point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], title="SL-FMM rigid sphere point source"))
+    C["fmm_sl_adm"] = dict(prj=P.Project(
+        mesh=ico3, freqs=[2000.0], eval_grid=P.fibonacci_grid(60, 0.8), method_bem=1,
+        boundary=[P.Boundary(0, n3 - 1, "ADMI", 0.3, 0.1)],
+        plane_waves=[(0.0, 0.6, 0.8, 1.0, 0.0)], point_sources=[(0.1, 0.1, 0.15, 0.5, 0.5)],
+        title="SL-FMM admittance sphere"))
+    C["fmm_sl_soft"] = dict(prj=P.Project(
+        mesh=ico3, freqs=[1500.0], eval_grid=P.fibonacci_grid(60, 0.7), method_bem=1,
+        boundary=[P.Boundary(0, n3 - 1, "PRES", 0.0, 0.0)], plane_waves=[(1.0, 0.0, 0.0, 1.0, 0.0)],
+        title="SL-FMM soft sphere plane wave"))
+    head12 = P.head_like_mesh(freq=12, seed=0)
+    src12 = P.ear_sources(head12)[0]
+    C["fmm_ml_head"] = dict(prj=P.Project(
+        mesh=head12, freqs=[3000.0, 8000.0], eval_grid=P.fibonacci_grid(120, 1.2), method_bem=4,
+        point_sources=[(src12[0], src12[1], src12[2], 1.0, 0.0)], title="ML-FMM head-like mesh nu=12"))
+    n12 = len(head12.elems)
+    C["fmm_ml_patch"] = dict(prj=P.Project(
+        mesh=head12, freqs=[5000.0], eval_grid=P.fibonacci_grid(80, 1.0), method_bem=4,
+        boundary=[P.Boundary(0, n12 - 1, "VELO", 0.0, 0.0), P.Boundary(100, 103, "VELO", 0.1, 0.0)],
+        title="ML-FMM head-like mesh vibrating patch"))
     return C
 
 

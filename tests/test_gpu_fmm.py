@@ -1,0 +1,111 @@
+"""GPU fast-multipole path against reference NumCalc's own FMM outputs.
+
+* tests/golden/shtf.npz: the reference's SHTF project (2412-element sphere, reciprocal
+  velocity source, SL-FMM steps 1-11 and 2-level direct ML-FMM steps 12-60), be.out of
+  steps 1, 5, 11, 12, 30, 41, 60 for both sources;
+* tests/golden/case_fmm_*.npz: SL-FMM and ML-FMM projects (point sources, plane waves,
+  admittance, sound-soft, vibrating patch) written by the reference compiled from its
+  sources.
+
+Tolerance: rel-L2 1e-5 on every be.out quantity (the files print 7 digits; FMM and
+CGS at tolerance 1e-9 add iteration-path differences), and the method (SL / ML) the
+reference logged.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import ncref as R
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+CASES = ["fmm_sl_point", "fmm_sl_adm", "fmm_sl_soft", "fmm_ml_head", "fmm_ml_patch"]
+
+
+@pytest.fixture(scope="module")
+def ncg():
+    from mesh2hrtf_amd import ncg as N
+    if N.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return N
+
+
+def check(ours, ref, tol=TOL):
+    ours, ref = np.asarray(ours), np.asarray(ref)
+    if np.linalg.norm(ref) == 0:
+        assert np.abs(ours).max() < 1e-12
+    else:
+        assert R.rel_l2(ours, ref) < tol, R.rel_l2(ours, ref)
+
+
+@pytest.mark.parametrize("src", [1, 2])
+def test_shtf_matches_reference(ncg, src):
+    z = np.load(R.GOLDEN + "/shtf.npz")
+    figs = json.loads(str(z["figures"]))[str(src)]
+    elems = z["elems"]
+    xyz = z["nodes"][elems]
+    n = len(elems)
+    prob = ncg.Problem(np.full(n, 3), xyz, np.zeros(n, np.int32), z["eval_nodes"], node_index=elems, method_bem=4)
+    j = int(np.where(z["elem_ids"] == (2410 if src == 1 else 2411))[0][0])
+    bc = np.zeros((1, n), complex)
+    bc[0, j] = 0.1
+    method = None
+    for s, step in enumerate(z["steps"]):
+        f = 100.0 * step
+        omega = 2 * np.pi * f
+        k = omega / 343.0
+        rp = 1.1839 * omega
+        print("SHTF source", src, "step", step, flush=True)
+        prob.assemble(k, bc=bc)
+        fig = figs[str(step)]
+        method = fig.get("method", method)
+        info = prob.fmm_info()
+        assert info["method"] == method
+        st = prob.solve(solver=ncg.CGS)
+        assert st["converged"]
+        x = prob.solution()[0]
+        phi, grad = prob.field()
+        check(1j * rp * x, z["s%d_pBoundary" % src][s])
+        check(1j * rp * phi[0], z["s%d_pEvalGrid" % src][s])
+        check(grad[0], z["s%d_vEvalGrid" % src][s])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fmm_cases_match_reference(ncg, name):
+    case = R.load_case(name)
+    spec = case["spec"]
+    prob = None
+    for s, f in enumerate(spec["freqs"]):
+        pa = R.problem_arrays(case, f)
+        print("case", name, "f", f, flush=True)
+        if prob is None:
+            prob = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], case["eval_nodes"], node_index=case["elems"],
+                               method_bem=spec["method_bem"])
+        ps = [list(v) + [0] for v in pa["ps"]]
+        pw = [list(v) + [0] for v in pa["pw"]]
+        prob.assemble(pa["k"], bc=pa["bc"][:, 0][None, :], adm=pa["adm"], point_sources=ps, plane_waves=pw)
+        assert prob.fmm_info()["method"] in (1, 3)
+        st = prob.solve(solver=ncg.CGS)
+        assert st["converged"]
+        x = prob.solution()[0]
+        pB, vB = R.surface_outputs(pa, x)
+        check(pB, case["pBoundary"][s])
+        check(vB, case["vBoundary"][s])
+        phi, grad = prob.field()
+        check(1j * pa["rpfact"] * phi[0], case["pEvalGrid"][s])
+        check(grad[0], case["vEvalGrid"][s])
+
+
+def test_fmm_scaling_preconditioner_same_solution(ncg):
+    case = R.load_case("fmm_ml_head")
+    f = case["spec"]["freqs"][0]
+    pa = R.problem_arrays(case, f)
+    prob = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], case["eval_nodes"], node_index=case["elems"], method_bem=4)
+    ps = [list(v) + [0] for v in pa["ps"]]
+    sols = []
+    for pc in (ncg.NONE, ncg.SCALING):
+        prob.assemble(pa["k"], point_sources=ps)
+        prob.solve(solver=ncg.CGS, precond=pc)
+        sols.append(prob.solution()[0])
+    assert R.rel_l2(sols[1], sols[0]) < 1e-7
