@@ -36,12 +36,43 @@ void stamp(std::ostream &os, const char *what)
     nc::fatal(log, std::string(where) + " failed (" + std::to_string(rc) + "): " + ncg_last_error());
 }
 
-// analysis type, cluster and expansion tables of one frequency step
-// (NC_FrequencyInformations src/NC_Main.cpp:1051-1153 and src/NC_Main.cpp:491-522)
-void method_info(std::ostream &os, const ncg_fmm_info &fi, bool first, int n)
+// Which analysis blocks a step writes.  NumCalc rebuilds (and reports) its cluster
+// structure at the first step, when the method or the number of levels changes, or
+// when the previous structure had cluster pairs cancelled into the near field
+// (NC_DeclareArrays src/NC_Addresses.cpp:119-137, 222-223).
+struct PlanLog {
+    int method = -1, levels = -1;
+    long long cancelled = 0;
+
+    bool rebuilt(const ncg_fmm_info &fi, bool first)
+    {
+        const bool r = first || fi.method != method || fi.levels != levels || cancelled > 0;
+        method = fi.method;
+        levels = fi.levels;
+        if (r) {
+            cancelled = 0;
+            if (fi.method)
+                for (int l = 0; l < fi.levels; ++l) cancelled += fi.canceled_pairs[l];
+        }
+        return r;
+    }
+};
+
+// cancelled-pair table (src/NC_Addresses.cpp:225-255), analysis type and cluster table
+// (NC_FrequencyInformations src/NC_Main.cpp:1051-1153), written when the structure is rebuilt
+void plan_info(std::ostream &os, const ncg_fmm_info &fi, bool first, int n)
 {
     static const char *names[4] = {"Traditional BEM", "Single level fast multipole BEM", "",
                                    "Direct multilevel fast multipole BEM"};
+    if (fi.method && (fi.canceled_pairs[0] || fi.canceled_pairs[fi.levels - 1])) {
+        os << "\nInformations about the canceled interacting cluster pairs:" << std::endl;
+        os << "   Level   N. canceled pairs   N. interacting pairs   Cancel ratio (%)" << std::endl;
+        for (int l = 0; l < fi.levels; ++l) {
+            const double r = fi.canceled_pairs[l] > 0 ? (double)fi.canceled_pairs[l] / (double)fi.far_pairs[l] : 0.0;
+            os << "   " << std::setw(3) << l << "     " << std::setw(10) << fi.canceled_pairs[l] << "           "
+               << std::setw(10) << fi.far_pairs[l] << "               " << std::setw(5) << r * 100.0 << std::endl;
+        }
+    }
     os << "\n" << names[fi.method] << std::endl;
     if (first) {
         os << "\nNumber of equations = " << n << std::endl;
@@ -56,6 +87,12 @@ void method_info(std::ostream &os, const ncg_fmm_info &fi, bool first, int n)
     if (fi.eval_clusters)
         os << " resu. nets" << "  " << std::setw(6) << fi.eval_clusters << "                " << std::setw(6)
            << fi.eval_radius_max << "         " << std::setw(6) << fi.eval_radius_min << std::endl;
+}
+
+// FMM expansion table of every FMM step (src/NC_Main.cpp:491-522)
+void expansion_info(std::ostream &os, const ncg_fmm_info &fi)
+{
+    if (fi.method == 0) return;
     os << "\nInformations about the FMM expansion:" << std::endl;
     os << "   Level    N. expa. terms    N. p. theta    N. p. phi    N. p. sphere" << std::endl;
     for (int l = 0; l < fi.levels; ++l)
@@ -261,6 +298,15 @@ int main(int argc, char **argv)
     if (istart > 0) istart--;
     long lti_eqa = 0, lti_sol = 0, lti_pos = 0;
     time_t tlast = t2;
+    PlanLog plan_log;
+    auto log_plan = [&](const ncg_fmm_info &fi, bool first) {
+        const bool rebuilt = plan_log.rebuilt(fi, first);
+        for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log}) {
+            if (rebuilt) plan_info(*os, fi, first, n);
+            expansion_info(*os, fi);
+        }
+        if (fi.warning[0]) nc::warning(log, fi.warning);
+    };
     for (int step = istart; step < iend; ++step) {
         const time_t ts = time(nullptr);
         const double freq = P.freqs[step];
@@ -274,8 +320,7 @@ int main(int argc, char **argv)
             ncg_fmm_info fi;
             const int rc = ncg_fmm_plan(&g, k, &fi);
             if (rc) gpu_fatal(log, "ncg_fmm_plan", rc);
-            for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log})
-                method_info(*os, fi, step == istart, n);
+            log_plan(fi, step == istart);
             const double gb = ram_estimate(fi, n);
             std::cout << "RAM Estimation: " << gb << " GByte\n";
             estim << step + 1 << " " << freq << " " << gb << "\n";
@@ -328,9 +373,7 @@ int main(int argc, char **argv)
         ncg_fmm_info fi;
         rc = ncg_problem_fmm_info(prob, &fi);
         if (rc) gpu_fatal(log, "ncg_problem_fmm_info", rc);
-        for (std::ostream *os : {(std::ostream *)&std::cout, (std::ostream *)&log})
-            method_info(*os, fi, step == istart, n);
-        if (fi.warning[0]) nc::warning(log, fi.warning);
+        log_plan(fi, step == istart);
         if (fi.method) fmm_nonzeros(log, fi);
         const time_t ta = time(nullptr);
         // solve (src/NC_Main.cpp:833-866, NC_IterativeSolverCGS)

@@ -18,8 +18,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from mesh2hrtf_amd import project as P  # noqa: E402
+from ncref import nc_tables  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -159,6 +161,11 @@ def run_case(name, prj):
                 vals.append(v)
             out[key] = np.asarray(vals)
             out[key + "_ids"] = ids
+        out["nc_tables"] = np.array(json.dumps(nc_tables(nout)))
+        r = subprocess.run([REF, "-estimate_ram"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s: NumCalc -estimate_ram failed\n%s" % (name, r.stdout[-2000:]))
+        out["memory_txt"] = np.array(open(os.path.join(d, "Memory.txt")).read())
     m = prj.mesh
     g = prj.eval_grid
     np.savez_compressed(

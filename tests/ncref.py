@@ -201,3 +201,50 @@ class Oracle:
 def rel_l2(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+TABLE_KEYS = ("Traditional BEM", "Single level fast multipole BEM", "Direct multilevel fast multipole BEM",
+              "Number of equations", "Number of entries", "Nonzeros", "    Nonzeros", "Sum of nonzeros")
+TABLE_HEADS = ("Informations about clusters", "Informations about the FMM expansion",
+               "Informations about the canceled interacting cluster pairs")
+
+
+def nc_tables(nout):
+    """Per frequency step of an NC.out text: the lines describing the analysis
+    (method, cancelled-pair, cluster and expansion tables, nonzero counts),
+    whitespace-normalised."""
+    steps = nout.split(">> S T E P   N U M B E R   A N D   F R E Q U E N C Y <<")[1:]
+    res = []
+    for blk in steps:
+        lines, table = [], False
+        for l in blk.splitlines():
+            if l.startswith(TABLE_HEADS):
+                table = True
+                lines.append(l.strip())
+                continue
+            if table and (l.startswith("   ") or l.startswith(" resu. nets")) and not l.startswith("    Nonzeros"):
+                lines.append(" ".join(l.split()))
+                continue
+            table = False
+            if l.startswith(TABLE_KEYS):
+                lines.append(" ".join(l.split()))
+        res.append(lines)
+    return res
+
+
+def same_table_line(a, b, rtol=2e-5):
+    """Two normalised NC.out lines agree: words equal, numbers to rtol (the
+    printed 6 significant digits of radii may round differently)."""
+    ta, tb = a.split(), b.split()
+    if len(ta) != len(tb):
+        return False
+    for x, y in zip(ta, tb):
+        if x == y:
+            continue
+        try:
+            fx, fy = float(x), float(y)
+        except ValueError:
+            return False
+        if abs(fx - fy) > rtol * max(abs(fx), abs(fy)):
+            return False
+    return True

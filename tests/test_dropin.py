@@ -4,6 +4,7 @@ CPU tests cover what runs without a device: the command line (src/NC_Main.cpp:15
 the RAM estimate (Memory.txt), the C-ABI exports.  GPU tests run whole projects through
 ``mesh2hrtf_amd/bin/NumCalc`` and compare its be.out with the reference's.
 """
+import json
 import os
 import re
 import subprocess
@@ -38,7 +39,7 @@ def project_from_case(folder, name, steps=None):
                     eval_grid=grid, boundary=[P.Boundary(*b) for b in spec["boundary"]],
                     point_sources=[tuple(s) for s in spec["point_sources"]],
                     plane_waves=[tuple(s) for s in spec["plane_waves"]], method_solver=spec["method_solver"],
-                    precond=spec["precond"], c=spec["c"], rho=spec["rho"])
+                    method_bem=spec["method_bem"], precond=spec["precond"], c=spec["c"], rho=spec["rho"])
     P.write_project(folder, prj)
     return c
 
@@ -97,11 +98,40 @@ def test_estimate_ram_memory_txt(tmp_path):
     assert os.path.isdir(tmp_path / "be.out" / "be.1")   # estimate keeps be.out
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch"])
-def test_dropin_matches_reference_be_out(tmp_path, name):
+def check_tables(ours, ref, skip_nonzeros=False):
+    """NC.out analysis lines of every step equal the reference's."""
+    assert len(ours) == len(ref)
+    for s, (a, b) in enumerate(zip(ours, ref)):
+        if skip_nonzeros:
+            b = [l for l in b if "onzeros" not in l]
+        assert len(a) == len(b), (s, a, b)
+        for x, y in zip(a, b):
+            assert R.same_table_line(x, y), (s, x, y)
+
+
+@pytest.mark.parametrize("name", ["c1", "quad", "fmm_sl_point", "fmm_sl_adm", "fmm_ml_head", "fmm_ml_patch"])
+def test_estimate_ram_matches_reference(tmp_path, name):
+    """-estimate_ram: Memory.txt and the method / cluster / expansion tables of NC.out
+    equal the reference's (TBEM, SL-FMM and ML-FMM estimates of src/NC_Main.cpp:524-731;
+    the cluster plan comes from the host planner, no device needed)."""
     _need_binary()
     case = project_from_case(str(tmp_path), name)
+    r = subprocess.run([NUMCALC, "-estimate_ram"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert open(tmp_path / "Memory.txt").read() == str(case["memory_txt"])
+    check_tables(R.nc_tables(open(tmp_path / "NC.out").read()), json.loads(str(case["nc_tables"])),
+                 skip_nonzeros=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch", "fmm_sl_point",
+                                  "fmm_ml_head"])
+def test_dropin_matches_reference_be_out(tmp_path, name):
+    """be.out of every step (rel-L2 1e-6 dense, 1e-5 FMM: the FMM solve differs from the
+    reference's by CGS iteration-path rounding) and the NC.out analysis tables."""
+    _need_binary()
+    case = project_from_case(str(tmp_path), name)
+    tol = 1e-5 if name.startswith("fmm_") else 1e-6
     print("drop-in case", name, flush=True)
     r = subprocess.run([NUMCALC], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
@@ -116,9 +146,10 @@ def test_dropin_matches_reference_be_out(tmp_path, name):
             if np.linalg.norm(ref) == 0:
                 assert np.abs(vals).max() < 1e-12
             else:
-                assert R.rel_l2(vals, ref) < 1e-6, (key, s)
+                assert R.rel_l2(vals, ref) < tol, (key, s)
     if case["spec"]["method_solver"] != 4:
         assert re.search(r"CGS solver: number of iterations = \d+, relative error = ", log)
+    check_tables(R.nc_tables(log), json.loads(str(case["nc_tables"])))
 
 
 @pytest.mark.gpu
