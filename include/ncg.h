@@ -45,7 +45,9 @@ enum {
 };
 
 enum { NCG_SOLVER_CGS = 0, NCG_SOLVER_GMRES = 1, NCG_SOLVER_LU = 4 };
-enum { NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2 };
+/* IPRECOND of NC.inp (src/NC_Input.cpp:298-323): 0 (and -1..-3, which choose the
+ * scanning degree) incomplete LU, 1 row scaling, 2 none */
+enum { NCG_PRECOND_ILU = 0, NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2 };
 
 /* Frequency-independent description of one boundary-element problem.
  * Replaces the geometry arrays NumCalc builds in NC_ReadMesh
@@ -115,6 +117,8 @@ typedef struct {
     int32_t restart;                    /* GMRES restart length                              */
     int32_t refinement_steps;           /* NCG_MIXED: max outer FP64 refinement steps        */
     double refinement_tolerance;        /* NCG_MIXED: FP64 residual target                   */
+    int32_t ilu_scanning_degree;        /* NCG_PRECOND_ILU: 0..3 (-IPRECOND), sets the drop
+                                           threshold (src/NC_CommonFunctions.cpp:1367-1403) */
 } ncg_solver_options;
 
 typedef struct {
@@ -123,6 +127,8 @@ typedef struct {
     int32_t converged_all;
     int32_t refinement_steps;
     double ms_assemble, ms_solve, ms_field;   /* device time of the last calls (HIP events)  */
+    int64_t ilu_nonzeros;               /* NCG_PRECOND_ILU: entries kept in L + U            */
+    int32_t ilu_levels[2];              /* dependency depth of the L and U solves            */
 } ncg_solve_stats;
 
 typedef struct ncg_problem ncg_problem;

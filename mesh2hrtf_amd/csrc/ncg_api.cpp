@@ -136,6 +136,30 @@ struct DevFmm {
     }
 };
 
+// device state of the incomplete LU preconditioner (pattern CSR and level schedules)
+struct DevIlu {
+    int nnz = 0, nlevL = 0, nlevU = 0;
+    int *rp = nullptr, *col = nullptr, *diag = nullptr;
+    int *levL_ptr = nullptr, *levL_rows = nullptr, *levU_ptr = nullptr, *levU_rows = nullptr;
+    double *val = nullptr;
+    std::vector<void *> owned;
+    void release()
+    {
+        for (void *q : owned) hipFree(q);
+        owned.clear();
+        nnz = nlevL = nlevU = 0;
+    }
+    template <typename T>
+    int alloc(T **ptr, size_t count)
+    {
+        *ptr = nullptr;
+        hipError_t e = hipMalloc((void **)ptr, sizeof(T) * std::max<size_t>(count, 1));
+        if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc (ILU): ") + hipGetErrorString(e));
+        owned.push_back((void *)*ptr);
+        return 0;
+    }
+};
+
 }  // namespace
 
 struct ncg_problem {
@@ -186,6 +210,7 @@ struct ncg_problem {
     ncgf::Plan plan;
     DevFmm fmm;
     size_t nrhs_fmm = 0;
+    DevIlu ilu;
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -290,6 +315,7 @@ int check_err(ncg_problem *p)
     NCG_HIP(hipStreamSynchronize(p->stream));
     if (e) {
         NCG_HIP(hipMemsetAsync(p->err, 0, sizeof(int), p->stream));
+        if (e & 8) return fail(NCG_E_ARG, "incomplete LU decomposition: zero pivot");
         if (e & 1) return fail(NCG_E_SUBDIV, "number of subels. which are subdivided in a loop must <= 15!");
         if (e & 2) return fail(NCG_E_SUBDIV, "nsbe must be < MSBE(= 220)");
         return fail(NCG_E_SUBDIV, "adaptive subdivision deeper than 24 levels (point on an element?)");
@@ -569,6 +595,105 @@ int dotv(ncg_problem *p, int ndots, const double *const *u, const double *const 
     return 0;
 }
 
+// ---------------------------------------------------------------------------------- ILU
+// Incomplete LU of NC_IncompleteLUDecomposition (src/NC_CommonFunctions.cpp:1356-1659):
+// rows scaled by d_i = sqrt(n_i / sum |a_ij|^2) over the dense row or the near-field row,
+// pattern = scaled entries above the method's threshold plus the diagonal, ILU(0) on it.
+// The scaling stays implicit: the preconditioned operator is D A (LU)^-1 and the
+// right-hand side D b, which is what the reference's in-place scaling of A, S and b gives.
+// Pattern counts, entries and factors are computed on the device; the host only turns
+// the column lists into level schedules.
+int ilu_setup(ncg_problem *p, int degree)
+{
+    static const double thr_tbem[4] = {1.2, 1.0, 0.8, 0.6}, thr_sl[4] = {0.9, 0.35, 0.07, 0.01},
+                        thr_ml[4] = {0.65, 0.15, 0.05, 0.005};
+    if (p->precision != NCG_FP64) return fail(NCG_E_ARG, "the ILU preconditioner needs the FP64 operator");
+    const int deg = std::min(std::max(degree, 0), 3);
+    const double thr = p->plan.method == 0 ? thr_tbem[deg] : p->plan.method == 1 ? thr_sl[deg] : thr_ml[deg];
+    hipStream_t s = p->stream;
+    const int n = p->n;
+    DevIlu &I = p->ilu;
+    NCG_HIP(hipStreamSynchronize(s));
+    I.release();
+    const bool dense = p->plan.method == 0;
+    const void *A = dense ? p->A : p->fmm.near_val;
+    const long long *ptr = dense ? nullptr : p->fmm.near_ptr;
+    const int *ncol = dense ? nullptr : p->fmm.near_col;
+    int rc;
+    int *cnt;
+    if ((rc = I.alloc(&cnt, n)) || (rc = I.alloc(&I.rp, n + 1)) || (rc = I.alloc(&I.diag, n))) return rc;
+    if (dense) ncgk::row_scaling(s, A, 0, n, p->dvec);
+    else ncgk::row_scaling_csr(s, n, ptr, A, 0, p->dvec);
+    ncgk::ilu_count(s, A, n, ptr, ncol, p->dvec, thr, cnt);
+    std::vector<int> c(n), rp(n + 1, 0);
+    NCG_HIP(hipMemcpyAsync(c.data(), cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    NCG_HIP(hipStreamSynchronize(s));
+    long long tot = 0;
+    for (int i = 0; i < n; ++i) {
+        rp[i] = (int)tot;
+        tot += c[i];
+        if (tot > 0x7fffffffLL) return fail(NCG_E_MEMORY, "incomplete LU pattern exceeds 2^31 entries");
+    }
+    rp[n] = (int)tot;
+    I.nnz = (int)tot;
+    if ((rc = I.alloc(&I.col, tot)) || (rc = I.alloc(&I.val, 2 * (size_t)tot))) return rc;
+    NCG_HIP(hipMemcpyAsync(I.rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    ncgk::ilu_fill(s, A, n, ptr, ncol, p->dvec, thr, I.rp, I.col, I.val);
+    std::vector<int> col(tot);
+    NCG_HIP(hipMemcpyAsync(col.data(), I.col, sizeof(int) * tot, hipMemcpyDeviceToHost, s));
+    NCG_HIP(hipStreamSynchronize(s));
+    // diagonal positions and the level of every row in the L and U dependency graphs
+    std::vector<int> diag(n, -1), lvL(n, 0), lvU(n, 0);
+    int nL = 0, nU = 0;
+    for (int i = 0; i < n; ++i) {
+        int l = 0;
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            if (col[q] < i) l = std::max(l, lvL[col[q]] + 1);
+            else if (col[q] == i) diag[i] = q;
+        }
+        if (diag[i] < 0) return fail(NCG_E_STATE, "incomplete LU: diagonal missing from the pattern");
+        lvL[i] = l;
+        nL = std::max(nL, l + 1);
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        int l = 0;
+        for (int q = diag[i] + 1; q < rp[i + 1]; ++q) l = std::max(l, lvU[col[q]] + 1);
+        lvU[i] = l;
+        nU = std::max(nU, l + 1);
+    }
+    auto schedule = [&](const std::vector<int> &lv, int nlev, std::vector<int> &lp, std::vector<int> &rows) {
+        lp.assign(nlev + 1, 0);
+        for (int i = 0; i < n; ++i) lp[lv[i] + 1]++;
+        for (int l = 0; l < nlev; ++l) lp[l + 1] += lp[l];
+        rows.assign(n, 0);
+        std::vector<int> fill(lp.begin(), lp.end() - 1);
+        for (int i = 0; i < n; ++i) rows[fill[lv[i]]++] = i;
+    };
+    std::vector<int> lpL, rowsL, lpU, rowsU;
+    schedule(lvL, nL, lpL, rowsL);
+    schedule(lvU, nU, lpU, rowsU);
+    if ((rc = I.alloc(&I.levL_ptr, nL + 1)) || (rc = I.alloc(&I.levL_rows, n)) || (rc = I.alloc(&I.levU_ptr, nU + 1)) ||
+        (rc = I.alloc(&I.levU_rows, n)))
+        return rc;
+    NCG_HIP(hipMemcpyAsync(I.diag, diag.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.levL_ptr, lpL.data(), sizeof(int) * (nL + 1), hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.levL_rows, rowsL.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.levU_ptr, lpU.data(), sizeof(int) * (nU + 1), hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.levU_rows, rowsU.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    I.nlevL = nL;
+    I.nlevU = nU;
+    ncgk::ilu_factor(s, nL, I.levL_ptr, I.levL_rows, I.rp, I.col, I.diag, I.val, p->err);
+    NCG_HIP(hipGetLastError());
+    return check_err(p);
+}
+
+void ilu_apply(ncg_problem *p, const double *x, double *y)
+{
+    DevIlu &I = p->ilu;
+    ncgk::ilu_solve(p->stream, p->n, p->nrhs, I.nlevL, I.levL_ptr, I.levL_rows, I.nlevU, I.levU_ptr, I.levU_rows, I.rp,
+                    I.col, I.diag, I.val, x, y);
+}
+
 // ---------------------------------------------------------------------------------- CGS
 // Right-preconditioned CGS of NC_IterativeSolverCGS (src/NC_CommonFunctions.cpp:1066-1208):
 // x0 = 0, unconjugated bilinear form, relative error measured against the residual
@@ -576,19 +701,44 @@ int dotv(ncg_problem *p, int ndots, const double *const *u, const double *const 
 int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 {
     const int n = p->n, nr = p->nrhs;
-    int rc = ensure_work(p, 8);
+    const bool ilu = o->precond == NCG_PRECOND_ILU;
+    int rc = ensure_work(p, ilu ? 10 : 8);
     if (rc) return rc;
     double *R0 = p->work[0], *R = p->work[1], *P = p->work[2], *V = p->work[3], *Q = p->work[4], *U = p->work[5],
            *UQ = p->work[6], *AUQ = p->work[7], *X = p->x;
     const size_t bytes = sizeof(cd) * (size_t)nr * n;
     const double *d = nullptr;
+    const double *B = p->rhs;
+    double *T = nullptr;
     if (o->precond == NCG_PRECOND_SCALING) {
         if (p->plan.method == 0) ncgk::row_scaling(p->stream, p->A, p->precision == NCG_MIXED, n, p->dvec);
         else ncgk::row_scaling_csr(p->stream, n, p->fmm.near_ptr, p->fmm.near_val, p->precision == NCG_MIXED, p->dvec);
         d = p->dvec;
+    } else if (ilu) {
+        if ((rc = ilu_setup(p, o->ilu_scanning_degree))) return rc;
+        double *Bs = p->work[8];
+        T = p->work[9];
+        NCG_HIP(hipMemcpyAsync(Bs, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+        ncgk::scale_vec(p->stream, n, nr, p->dvec, Bs);
+        B = Bs;
+        if (st) {
+            st->ilu_nonzeros = p->ilu.nnz;
+            st->ilu_levels[0] = p->ilu.nlevL;
+            st->ilu_levels[1] = p->ilu.nlevU;
+        }
     }
+    // operator of the Krylov loop: A diag(d) (scaling), D A (LU)^-1 (ILU) or A
+    auto op = [&](const double *in, double *out) {
+        if (!ilu) {
+            mv(p, 1, in, d, out);
+            return;
+        }
+        ilu_apply(p, in, T);
+        mv(p, 1, T, nullptr, out);
+        ncgk::scale_vec(p->stream, n, nr, p->dvec, out);
+    };
     NCG_HIP(hipMemsetAsync(X, 0, bytes, p->stream));
-    for (double *w : {R0, R, P, U}) NCG_HIP(hipMemcpyAsync(w, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
+    for (double *w : {R0, R, P, U}) NCG_HIP(hipMemcpyAsync(w, B, bytes, hipMemcpyDeviceToDevice, p->stream));
     std::vector<int> active(nr, 1);
     NCG_HIP(hipMemcpyAsync(p->active, active.data(), sizeof(int) * nr, hipMemcpyHostToDevice, p->stream));
     std::vector<double> err0(nr, 0.0), err(nr, 1.0);
@@ -596,14 +746,14 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     std::vector<cd> dv;
     int j;
     for (j = 0; j <= o->max_iterations; ++j) {
-        mv(p, 1, P, d, V);
+        op(P, V);
         {
             const double *u[2] = {R, V}, *v[2] = {R0, R0};
             ncgk::dots(p->stream, n, nr, 2, (const void *const *)u, (const void *const *)v, 0, p->partial, p->dotout);
             ncgk::cgs_alpha(p->stream, nr, j == 0, p->dotout, p->scal, p->active);
         }
         ncgk::cgs_update1(p->stream, n, nr, p->scal, p->active, U, V, Q, UQ);
-        mv(p, 1, UQ, d, AUQ);
+        op(UQ, AUQ);
         ncgk::cgs_update2(p->stream, n, nr, p->scal, p->active, UQ, AUQ, X, R);
         {
             const double *u[2] = {R, R}, *v[2] = {R, R0};
@@ -631,6 +781,7 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
         ncgk::cgs_update3(p->stream, n, nr, p->scal, p->active, R, Q, U, P);
     }
     if (d) ncgk::scale_vec(p->stream, n, nr, d, X);
+    if (ilu) ilu_apply(p, X, X);
     harvest_mv(p);
     int all = 1;
     for (int r = 0; r < nr; ++r) {
@@ -971,6 +1122,7 @@ int ncg_problem_destroy(ncg_problem *p)
         if (q) hipFree(q);
     for (double *w : p->work) hipFree(w);
     p->fmm.release();
+    p->ilu.release();
     for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
         if (e->a) hipEventDestroy(e->a);
         if (e->b) hipEventDestroy(e->b);
@@ -1093,6 +1245,8 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     if (!p || !o) return fail(NCG_E_ARG, "ncg_solve: null argument");
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_solve: assemble first");
     if (o->max_iterations < 0) return fail(NCG_E_ARG, "ncg_solve: negative iteration limit");
+    if (o->precond < NCG_PRECOND_ILU || o->precond > NCG_PRECOND_NONE)
+        return fail(NCG_E_ARG, "ncg_solve: precond must be NCG_PRECOND_ILU, _SCALING or _NONE");
     NCG_HIP(hipSetDevice(p->device));
     if (st) memset(st, 0, sizeof(*st));
     record(p, p->ev_solve.a);

@@ -589,6 +589,165 @@ __global__ void __launch_bounds__(1024) k_lu_solve(const cx<double> *A, int n, c
 }
 
 // ======================================================================================
+// incomplete LU preconditioner (NC_IncompleteLUDecomposition / NC_IncompleteLUForBack,
+// src/NC_CommonFunctions.cpp:1356-1721) — FP64.
+// Rows are scaled by d_i (the scaling vector); the pattern keeps entries with
+// |d_i a_ij| > threshold and the diagonal.  ILU(0) on that pattern is unique, so the
+// factors are computed row by row (IKJ, unit lower L, U with the diagonal) in level
+// order: rows of one level depend only on rows of earlier levels.
+// Pattern CSR: rp[n+1] (int), col ascending per row, val = scaled entries.
+// ======================================================================================
+__device__ __forceinline__ bool ilu_keep(cx<double> a, double d, double thr)
+{
+#pragma clang fp contract(off)
+    const double re = a.re * d, im = a.im * d;
+    return sqrt(re * re + im * im) > thr;
+}
+
+// kept entries per row of the dense operator, one wave per row
+__global__ void k_ilu_count_dense(const cx<double> *A, int n, const double *d, double thr, int *cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double di = d[i];
+    int c = 0;
+    for (int j = lane; j < n; j += 64) c += (j == i || ilu_keep(A[(size_t)i * n + j], di, thr)) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) cnt[i] = c;
+}
+
+// kept entries written in column order: ballot over 64 columns at a time
+__global__ void k_ilu_fill_dense(const cx<double> *A, int n, const double *d, double thr, const int *rp, int *col,
+                                 cx<double> *val)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double di = d[i];
+    int base = rp[i];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        cx<double> a = mk(0.0, 0.0);
+        bool keep = false;
+        if (j < n) {
+            a = A[(size_t)i * n + j];
+            keep = j == i || ilu_keep(a, di, thr);
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int q = base + __popcll(m & below);
+            col[q] = j;
+            val[q] = a * di;
+        }
+        base += __popcll(m);
+    }
+}
+
+// the same over the near-field CSR rows (SL/ML-FMM)
+__global__ void k_ilu_count_csr(int n, const long long *ptr, const int *ncol, const cx<double> *nval, const double *d,
+                                double thr, int *cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double di = d[i];
+    int c = 0;
+    for (long long k = ptr[i] + lane; k < ptr[i + 1]; k += 64) c += (ncol[k] == i || ilu_keep(nval[k], di, thr)) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) cnt[i] = c;
+}
+
+// one thread per row: copy the kept entries, then insertion-sort them by column
+// (near-field rows are ordered by cluster, not by column)
+__global__ void k_ilu_fill_csr(int n, const long long *ptr, const int *ncol, const cx<double> *nval, const double *d,
+                               double thr, const int *rp, int *col, cx<double> *val)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double di = d[i];
+    const int b = rp[i];
+    int q = b;
+    for (long long k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const int j = ncol[k];
+        const cx<double> a = nval[k];
+        if (j == i || ilu_keep(a, di, thr)) {
+            int t = q++;
+            while (t > b && col[t - 1] > j) {
+                col[t] = col[t - 1];
+                val[t] = val[t - 1];
+                --t;
+            }
+            col[t] = j;
+            val[t] = a * di;
+        }
+    }
+}
+
+// IKJ factorization in place, one work-group walking the levels of the L dependency graph
+__global__ void __launch_bounds__(1024) k_ilu_factor(int nlev, const int *lev_ptr, const int *lev_rows, const int *rp,
+                                                     const int *col, const int *diag, cx<double> *val, int *err)
+{
+    for (int l = 0; l < nlev; ++l) {
+        for (int t = lev_ptr[l] + threadIdx.x; t < lev_ptr[l + 1]; t += blockDim.x) {
+            const int i = lev_rows[t];
+            const int end = rp[i + 1];
+            for (int p = rp[i]; p < diag[i]; ++p) {
+                const int k = col[p];
+                const cx<double> piv = val[diag[k]];
+                if (piv.re == 0.0 && piv.im == 0.0) {
+                    atomicOr(err, 8);
+                    continue;
+                }
+                const cx<double> lik = cdiv(val[p], piv);
+                val[p] = lik;
+                int q = diag[k] + 1;
+                const int qe = rp[k + 1];
+                for (int p2 = p + 1; p2 < end && q < qe; ++p2) {
+                    const int j = col[p2];
+                    while (q < qe && col[q] < j) ++q;
+                    if (q < qe && col[q] == j) val[p2] -= lik * val[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// y = (L U)^-1 x for every load case: forward over the L levels, backward over the U
+// levels; one work-group (x and y may be the same vector)
+__global__ void __launch_bounds__(1024) k_ilu_solve(int n, int nrhs, int nlevL, const int *levL_ptr,
+                                                    const int *levL_rows, int nlevU, const int *levU_ptr,
+                                                    const int *levU_rows, const int *rp, const int *col,
+                                                    const int *diag, const cx<double> *val, const cx<double> *x,
+                                                    cx<double> *y)
+{
+    for (int l = 0; l < nlevL; ++l) {
+        const int b = levL_ptr[l], w = levL_ptr[l + 1] - b;
+        for (int u = threadIdx.x; u < w * nrhs; u += blockDim.x) {
+            const int r = u / w, i = levL_rows[b + u % w];
+            const cx<double> *yr = y + (size_t)r * n;
+            cx<double> s = x[(size_t)r * n + i];
+            for (int p = rp[i]; p < diag[i]; ++p) s -= val[p] * yr[col[p]];
+            y[(size_t)r * n + i] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < nlevU; ++l) {
+        const int b = levU_ptr[l], w = levU_ptr[l + 1] - b;
+        for (int u = threadIdx.x; u < w * nrhs; u += blockDim.x) {
+            const int r = u / w, i = levU_rows[b + u % w];
+            const cx<double> *yr = y + (size_t)r * n;
+            cx<double> s = yr[i];
+            for (int p = diag[i] + 1; p < rp[i + 1]; ++p) s -= val[p] * yr[col[p]];
+            y[(size_t)r * n + i] = cdiv(s, val[diag[i]]);
+        }
+        __syncthreads();
+    }
+}
+
+// ======================================================================================
 // evaluation grid: potential and gradient (NC_IntegrationTBEM, src/NC_PostProcessing.cpp:1264-1318)
 // one wave per evaluation point, lanes over the boundary elements.
 // ======================================================================================
@@ -1399,6 +1558,36 @@ void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const 
     else
         k_multiaxpy<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V,
                                                                        (const cx<double> *)h, (cx<double> *)w);
+}
+void ilu_count(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
+               int *cnt)
+{
+    if (ptr)
+        k_ilu_count_csr<<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, ncol, (const cx<double> *)A, d, thr, cnt);
+    else
+        k_ilu_count_dense<<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d, thr, cnt);
+}
+void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
+              const int *rp, int *col, void *val)
+{
+    if (ptr)
+        k_ilu_fill_csr<<<blocks_for(n, 256), 256, 0, s>>>(n, ptr, ncol, (const cx<double> *)A, d, thr, rp, col,
+                                                           (cx<double> *)val);
+    else
+        k_ilu_fill_dense<<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d, thr, rp, col,
+                                                          (cx<double> *)val);
+}
+void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
+                const int *diag, void *val, int *err)
+{
+    k_ilu_factor<<<1, 1024, 0, s>>>(nlev, lev_ptr, lev_rows, rp, col, diag, (cx<double> *)val, err);
+}
+void ilu_solve(hipStream_t s, int n, int nrhs, int nlevL, const int *levL_ptr, const int *levL_rows, int nlevU,
+               const int *levU_ptr, const int *levU_rows, const int *rp, const int *col, const int *diag,
+               const void *val, const void *x, void *y)
+{
+    k_ilu_solve<<<1, 1024, 0, s>>>(n, nrhs, nlevL, levL_ptr, levL_rows, nlevU, levU_ptr, levU_rows, rp, col, diag,
+                                   (const cx<double> *)val, (const cx<double> *)x, (cx<double> *)y);
 }
 void lu_factor(hipStream_t s, void *A, int n)
 {

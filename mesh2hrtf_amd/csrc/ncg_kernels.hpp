@@ -64,6 +64,17 @@ void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d);
 void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val, int mixed, double *d);
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h);
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add);
+// incomplete LU of the row-scaled operator (dense A when ptr == nullptr, else the
+// near-field CSR ptr/ncol/A); pattern CSR rp/col/val, level schedules from the host
+void ilu_count(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
+               int *cnt);
+void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
+              const int *rp, int *col, void *val);
+void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
+                const int *diag, void *val, int *err);
+void ilu_solve(hipStream_t s, int n, int nrhs, int nlevL, const int *levL_ptr, const int *levL_rows, int nlevU,
+               const int *levU_ptr, const int *levU_rows, const int *rp, const int *col, const int *diag,
+               const void *val, const void *x, void *y);
 void lu_factor(hipStream_t s, void *A, int n);
 void lu_solve(hipStream_t s, const void *A, int n, int nrhs, void *b);
 void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double *adm, double tau, const void *bcval,

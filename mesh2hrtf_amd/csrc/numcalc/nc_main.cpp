@@ -380,7 +380,8 @@ int main(int argc, char **argv)
         ncg_solver_options so;
         memset(&so, 0, sizeof(so));
         so.solver = P.method_solver == 4 ? NCG_SOLVER_LU : (gmres ? NCG_SOLVER_GMRES : NCG_SOLVER_CGS);
-        so.precond = P.precond == 1 ? NCG_PRECOND_SCALING : NCG_PRECOND_NONE;
+        so.precond = P.precond == 0 ? NCG_PRECOND_ILU : P.precond == 1 ? NCG_PRECOND_SCALING : NCG_PRECOND_NONE;
+        so.ilu_scanning_degree = P.scan_degree;
         so.max_iterations = nitermax;
         so.tolerance = 1.0e-9;
         so.restart = 50;

@@ -17,7 +17,7 @@ NUMCALC = os.path.join(PKG, "bin", "NumCalc")
 
 FP64, MIXED = 0, 1
 CGS, GMRES, LU = 0, 1, 4
-SCALING, NONE = 1, 2
+ILU, SCALING, NONE = 0, 1, 2
 BC_VELOCITY, BC_PRESSURE, BC_ADMITTANCE = 0, 1, 2
 
 
@@ -74,13 +74,14 @@ class _Frequency(ctypes.Structure):
 class _Options(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("precond", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
                 ("tolerance", ctypes.c_double), ("restart", ctypes.c_int32), ("refinement_steps", ctypes.c_int32),
-                ("refinement_tolerance", ctypes.c_double)]
+                ("refinement_tolerance", ctypes.c_double), ("ilu_scanning_degree", ctypes.c_int32)]
 
 
 class _Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32 * 64), ("relative_residual", ctypes.c_double * 64),
                 ("converged_all", ctypes.c_int32), ("refinement_steps", ctypes.c_int32),
-                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_field", ctypes.c_double)]
+                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_field", ctypes.c_double),
+                ("ilu_nonzeros", ctypes.c_int64), ("ilu_levels", ctypes.c_int32 * 2)]
 
 
 # every entry point include/ncg.h declares
@@ -198,8 +199,9 @@ class Problem:
         _check(self.lib.ncg_assemble(self._h, ctypes.byref(f), int(precision)))
 
     def solve(self, solver=CGS, precond=NONE, max_iterations=250, tolerance=1e-9, restart=50,
-              refinement_steps=8, refinement_tolerance=1e-6, allow_noconv=False):
-        o = _Options(solver, precond, max_iterations, tolerance, restart, refinement_steps, refinement_tolerance)
+              refinement_steps=8, refinement_tolerance=1e-6, allow_noconv=False, ilu_scanning_degree=0):
+        o = _Options(solver, precond, max_iterations, tolerance, restart, refinement_steps, refinement_tolerance,
+                     ilu_scanning_degree)
         st = _Stats()
         rc = self.lib.ncg_solve(self._h, ctypes.byref(o), ctypes.byref(st))
         if rc != 0 and not (allow_noconv and rc == -6):
@@ -207,7 +209,8 @@ class Problem:
         return dict(iterations=list(st.iterations[:self.nrhs]),
                     relative_residual=list(st.relative_residual[:self.nrhs]),
                     converged=bool(st.converged_all), refinement_steps=st.refinement_steps,
-                    ms_assemble=st.ms_assemble, ms_solve=st.ms_solve)
+                    ms_assemble=st.ms_assemble, ms_solve=st.ms_solve,
+                    ilu_nonzeros=st.ilu_nonzeros, ilu_levels=tuple(st.ilu_levels))
 
     def solution(self):
         x = np.zeros((self.nrhs, self.n), np.complex128)

@@ -567,13 +567,92 @@ static cplx dotu(int n, const cplx *a, const cplx *b)
 
 /* NC_IterativeSolverCGS (src/NC_CommonFunctions.cpp:1066-1208) with the scaling
  * preconditioner of NC_ComputeScalingVector (:1322-1354) applied on the right. */
-int ncp_cgs(int n, const double *A, const double *b, double *xout, int precond,
+/* Incomplete LU of the TBEM branch of NC_IncompleteLUDecomposition
+ * (src/NC_CommonFunctions.cpp:1356-1659), dense storage: the matrix and right-hand side
+ * are scaled row by row (d_i = sqrt(n / sum_j |a_ij|^2)), the pattern keeps
+ * |a_ij| > threshold and the diagonal, and the Crout recurrences of the reference fill
+ * L (lower, with the diagonal) column by column and U (unit upper) row by row. */
+typedef struct {
+    int n;
+    unsigned char *mask;   /* [n][n] pattern */
+    cplx *LU;              /* [n][n]: L on and below the diagonal, U above */
+} ilu_t;
+
+static void ilu_build(int n, cplx *As, cplx *bs, int degree, ilu_t *f)
+{
+    static const double thr_tbem[4] = {1.2, 1.0, 0.8, 0.6};
+    const double thr = thr_tbem[degree < 0 ? 0 : degree > 3 ? 3 : degree];
+    f->n = n;
+    f->mask = calloc((size_t)n * n, 1);
+    f->LU = calloc((size_t)n * n, sizeof(cplx));
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += cabs2(As[(size_t)i * n + j]);
+        const double d = sqrt((double)n / s);
+        for (int j = 0; j < n; j++) {
+            cplx *a = As + (size_t)i * n + j;
+            *a = cscal(*a, d);
+            if (sqrt(cabs2(*a)) > thr || i == j) {
+                f->mask[(size_t)i * n + j] = 1;
+                f->LU[(size_t)i * n + j] = *a;
+            }
+        }
+        bs[i] = cscal(bs[i], d);
+    }
+    const unsigned char *M = f->mask;
+    cplx *F = f->LU;
+    for (int i = 0; i < n; i++) {
+        for (int k = i; k < n; k++) {          /* column i of L */
+            if (!M[(size_t)k * n + i]) continue;
+            cplx s = F[(size_t)k * n + i];
+            for (int m = 0; m < i; m++)
+                if (M[(size_t)k * n + m] && M[(size_t)m * n + i]) s = csub(s, cmul(F[(size_t)k * n + m], F[(size_t)m * n + i]));
+            F[(size_t)k * n + i] = s;
+        }
+        for (int k = i + 1; k < n; k++) {      /* row i of U */
+            if (!M[(size_t)i * n + k]) continue;
+            cplx s = F[(size_t)i * n + k];
+            for (int m = 0; m < i; m++)
+                if (M[(size_t)i * n + m] && M[(size_t)m * n + k]) s = csub(s, cmul(F[(size_t)i * n + m], F[(size_t)m * n + k]));
+            F[(size_t)i * n + k] = cdiv(s, F[(size_t)i * n + i]);
+        }
+    }
+}
+
+/* NC_IncompleteLUForBack (src/NC_CommonFunctions.cpp:1673-1721): y = (L U)^-1 x */
+static void ilu_apply(const ilu_t *f, const cplx *x, cplx *y)
+{
+    const int n = f->n;
+    const cplx *F = f->LU;
+    const unsigned char *M = f->mask;
+    for (int i = 0; i < n; i++) y[i] = x[i];
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < i; j++)
+            if (M[(size_t)i * n + j]) y[i] = csub(y[i], cmul(F[(size_t)i * n + j], y[j]));
+        y[i] = cdiv(y[i], F[(size_t)i * n + i]);
+    }
+    for (int i = n - 1; i > 0; i--)
+        for (int j = 0; j < i; j++)
+            if (M[(size_t)j * n + i]) y[j] = csub(y[j], cmul(F[(size_t)j * n + i], y[i]));
+}
+
+int ncp_cgs(int n, const double *A0, const double *b0, double *xout, int precond,
             int maxit, double tol, double *relerr)
 {
-    cplx *w = calloc((size_t)9 * n, sizeof(cplx));
+    cplx *w = calloc((size_t)10 * n, sizeof(cplx));
     double *sc = malloc(sizeof(double) * n);
     cplx *X = w, *R0 = w + n, *R = w + 2 * n, *P = w + 3 * n, *V = w + 4 * n, *Q = w + 5 * n,
-         *U = w + 6 * n, *UQ = w + 7 * n, *T = w + 8 * n;
+         *U = w + 6 * n, *UQ = w + 7 * n, *T = w + 8 * n, *B = w + 9 * n;
+    const double *A = A0;
+    cplx *As = NULL;
+    ilu_t ilu = {0, NULL, NULL};
+    for (int i = 0; i < n; i++) B[i] = cld(b0 + 2 * i);
+    if (precond <= 0) {
+        As = malloc(sizeof(cplx) * (size_t)n * n);
+        memcpy(As, A0, sizeof(cplx) * (size_t)n * n);
+        ilu_build(n, As, B, -precond, &ilu);
+        A = (const double *)As;
+    }
     for (int i = 0; i < n; i++) {
         if (precond == 1) {
             double s = 0.0;
@@ -581,18 +660,24 @@ int ncp_cgs(int n, const double *A, const double *b, double *xout, int precond,
             sc[i] = sqrt((double)n / s);
         } else sc[i] = 1.0;
     }
+    /* M^-1 applied to the Krylov vector before the product */
+#define PRECOND_IN(src)                                                 \
+    do {                                                                \
+        if (precond <= 0) ilu_apply(&ilu, (src), T);                    \
+        else for (int i_ = 0; i_ < n; i_++) T[i_] = cscal((src)[i_], sc[i_]); \
+    } while (0)
     /* x0 = 0, so A M^-1 x0 = 0 and r0 = b */
-    for (int i = 0; i < n; i++) U[i] = R[i] = P[i] = R0[i] = cld(b + 2 * i);
+    for (int i = 0; i < n; i++) U[i] = R[i] = P[i] = R0[i] = B[i];
     double err0 = 0.0, err = 1.0;
     cplx rho0 = cmk(0, 0), rho1 = cmk(0, 0);
     int j;
     for (j = 0; j <= maxit; j++) {
-        for (int i = 0; i < n; i++) T[i] = cscal(P[i], sc[i]);
+        PRECOND_IN(P);
         matvec(n, A, T, V);
         rho0 = j == 0 ? dotu(n, R, R0) : rho1;
         cplx alpha = cdiv(rho0, dotu(n, V, R0));
         for (int i = 0; i < n; i++) { Q[i] = csub(U[i], cmul(alpha, V[i])); UQ[i] = cadd(U[i], Q[i]); }
-        for (int i = 0; i < n; i++) T[i] = cscal(UQ[i], sc[i]);
+        PRECOND_IN(UQ);
         matvec(n, A, T, V);
         double s = 0.0;
         for (int i = 0; i < n; i++) {
@@ -607,7 +692,16 @@ int ncp_cgs(int n, const double *A, const double *b, double *xout, int precond,
         for (int i = 0; i < n; i++) U[i] = cadd(R[i], cmul(beta, Q[i]));
         for (int i = 0; i < n; i++) P[i] = cadd(U[i], cmul(beta, cadd(Q[i], cmul(beta, P[i]))));
     }
-    for (int i = 0; i < n; i++) { cplx v = cscal(X[i], sc[i]); xout[2 * i] = v.re; xout[2 * i + 1] = v.im; }
+#undef PRECOND_IN
+    if (precond <= 0) {
+        ilu_apply(&ilu, X, T);
+        for (int i = 0; i < n; i++) { xout[2 * i] = T[i].re; xout[2 * i + 1] = T[i].im; }
+        free(ilu.mask);
+        free(ilu.LU);
+        free(As);
+    } else {
+        for (int i = 0; i < n; i++) { cplx v = cscal(X[i], sc[i]); xout[2 * i] = v.re; xout[2 * i + 1] = v.im; }
+    }
     if (relerr) *relerr = err;
     free(w);
     free(sc);

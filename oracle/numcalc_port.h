@@ -55,7 +55,9 @@ int ncp_element_integrals(const ncp_problem *p, int i, int j, double *out);
 int ncp_assemble(const ncp_problem *p, double *A, double *b);
 
 /* Right-preconditioned CGS with the unconjugated bilinear form
- * (src/NC_CommonFunctions.cpp:1066-1208).  precond: 1 row-scaling, 2 none.
+ * (src/NC_CommonFunctions.cpp:1066-1208).  precond: 1 row-scaling, 2 none, 0..-3
+ * incomplete LU with scanning degree -precond (the dense TBEM thresholds of
+ * src/NC_CommonFunctions.cpp:1367-1376).
  * Returns iterations; *relerr receives the final relative residual. */
 int ncp_cgs(int n, const double *A, const double *b, double *x, int precond,
             int maxit, double tol, double *relerr);

@@ -154,6 +154,8 @@ def run_case(name, prj):
         nout = open(os.path.join(d, "NC.out")).read()
         iters = [int(l.split("=")[1].split(",")[0]) for l in nout.splitlines()
                  if l.startswith("CGS solver: number of iterations")]
+        relerr = [float(l.split("relative error =")[1]) for l in nout.splitlines()
+                  if l.startswith("CGS solver: number of iterations")]
         for key in ("pBoundary", "vBoundary", "pEvalGrid", "vEvalGrid"):
             vals = []
             for s in range(len(prj.freqs)):
@@ -173,7 +175,7 @@ def run_case(name, prj):
         nodes=m.nodes, elems=m.elems, elem_ids=m.elem_ids,
         eval_nodes=g.nodes if g is not None else np.zeros((0, 3)),
         spec=np.array(json.dumps(spec_of(prj))),
-        cgs_iterations=np.asarray(iters, dtype=np.int64),
+        cgs_iterations=np.asarray(iters, dtype=np.int64), cgs_relerr=np.asarray(relerr),
         ref_wall_s=np.float64(wall), **out)
     print("%-12s N=%d M=%d steps=%d wall=%.1fs iters=%s" % (
         name, len(m.elems), 0 if g is None else len(g.nodes), len(prj.freqs), wall, iters))

@@ -148,7 +148,8 @@ def test_dropin_matches_reference_be_out(tmp_path, name):
             else:
                 assert R.rel_l2(vals, ref) < tol, (key, s)
     if case["spec"]["method_solver"] != 4:
-        assert re.search(r"CGS solver: number of iterations = \d+, relative error = ", log)
+        its = [int(v) for v in re.findall(r"CGS solver: number of iterations = (\d+), relative error = ", log)]
+        assert its == list(case["cgs_iterations"])
     check_tables(R.nc_tables(log), json.loads(str(case["nc_tables"])))
 
 

@@ -62,8 +62,9 @@ def test_shtf_matches_reference(ncg, src):
         method = fig.get("method", method)
         info = prob.fmm_info()
         assert info["method"] == method
-        st = prob.solve(solver=ncg.CGS)
+        st = prob.solve(solver=ncg.CGS, precond=ncg.ILU)   # the project's IPRECOND 0
         assert st["converged"]
+        assert st["iterations"][0] == fig["cgs_iterations"], (step, st["iterations"])
         x = prob.solution()[0]
         phi, grad = prob.field()
         check(1j * rp * x, z["s%d_pBoundary" % src][s])
@@ -86,8 +87,9 @@ def test_fmm_cases_match_reference(ncg, name):
         pw = [list(v) + [0] for v in pa["pw"]]
         prob.assemble(pa["k"], bc=pa["bc"][:, 0][None, :], adm=pa["adm"], point_sources=ps, plane_waves=pw)
         assert prob.fmm_info()["method"] in (1, 3)
-        st = prob.solve(solver=ncg.CGS)
+        st = prob.solve(solver=ncg.CGS, precond=ncg.ILU)
         assert st["converged"]
+        assert st["iterations"][0] == case["cgs_iterations"][s], (s, st["iterations"])
         x = prob.solution()[0]
         pB, vB = R.surface_outputs(pa, x)
         check(pB, case["pBoundary"][s])

@@ -108,6 +108,28 @@ def test_cgs_follows_numcalc_iteration(ncg, oracle):
     assert R.rel_l2(prob.solution()[0], x1) < 1e-8
 
 
+@pytest.mark.parametrize("name", ["soft_pw", "adm_ps", "velo_patch", "mixed_pres", "quad", "head8", "c1"])
+def test_cgs_ilu_matches_reference(ncg, oracle, name):
+    """Incomplete-LU CGS (IPRECOND 0): the reference's iteration count and relative
+    error at every step, and the oracle's ILU-CGS solution."""
+    case = R.load_case(name)
+    prob = None
+    for s, f in enumerate(case["spec"]["freqs"]):
+        pa = R.problem_arrays(case, f)
+        print("ILU case", name, "f", f, flush=True)
+        if prob is None:
+            prob = gpu_problem(ncg, case, pa)
+        assemble(prob, pa)
+        st = prob.solve(solver=ncg.CGS, precond=ncg.ILU)
+        assert st["converged"]
+        assert st["iterations"][0] == case["cgs_iterations"][s], (s, st)
+        assert abs(st["relative_residual"][0] - case["cgs_relerr"][s]) < 0.2 * case["cgs_relerr"][s]
+        assert st["ilu_nonzeros"] >= pa["n"]
+        A0, b0 = oracle.assemble(pa)
+        x0, it0, _ = oracle.cgs(A0, b0, precond=0)
+        assert R.rel_l2(prob.solution()[0], x0) < 1e-8
+
+
 def test_solvers_agree(ncg):
     case = R.load_case("head8")
     pa = R.problem_arrays(case, 6000.0)

@@ -230,3 +230,19 @@ def test_dense_solvers_against_numpy(oracle):
     x1, it1, err1 = oracle.cgs(A, b, precond=1)
     assert R.rel_l2(x1, ref) < 1e-8
     assert R.rel_l2(oracle.lu_solve(A, b), ref) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["soft_pw", "adm_ps", "velo_patch", "mixed_pres", "quad", "head8", "c1"])
+def test_cgs_ilu_matches_reference_iterations(oracle, name):
+    """The incomplete-LU preconditioned CGS (IPRECOND 0, the value mesh2input writes)
+    takes the reference's iteration count at every step; its last relative residual
+    agrees to the noise of a residual near 1e-10; the solution is the system's."""
+    case = R.load_case(name)
+    assert case["spec"]["precond"] == 0
+    for s, f in enumerate(case["spec"]["freqs"]):
+        pa = R.problem_arrays(case, f)
+        A, b = oracle.assemble(pa)
+        x, it, err = oracle.cgs(A, b, precond=0)
+        assert it == case["cgs_iterations"][s], (s, it)
+        assert abs(err - case["cgs_relerr"][s]) < 0.2 * case["cgs_relerr"][s]
+        assert R.rel_l2(x, np.linalg.solve(A, b)) < 1e-8
