@@ -89,40 +89,13 @@ struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
-// device arrays of one FMM level (layouts in ncg_kernels.hip)
-struct DevLevel {
-    int ncl = 0, P = 0, L = 0, nfan = 0;
-    double *ccen = nullptr, *u = nullptr, *w = nullptr;
-    int *cl_ptr = nullptr, *cl_elem = nullptr, *elem_cl = nullptr, *elem_pos = nullptr;
-    int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
-    void *T = nullptr, *D = nullptr, *S = nullptr;
-    double *tw = nullptr, *sw = nullptr, *tv = nullptr;   // work [nrhs][ncl*P]
-};
-
-// device arrays of the FMM operator of one frequency
-struct DevFmm {
-    std::vector<DevLevel> lev;
-    long long nnz = 0;
-    long long *near_ptr = nullptr, *diag_pos = nullptr;
-    int *near_row = nullptr, *near_col = nullptr;
-    void *near_val = nullptr;
-    int *elem_cl_fine = nullptr;
-    unsigned char *cl_near = nullptr;
-    int ncl_fine = 0;
-    double *xs = nullptr;   // scaled copy of the input vector
+// device allocations released together
+struct Pool {
     std::vector<void *> owned;
     void release()
     {
         for (void *q : owned) hipFree(q);
         owned.clear();
-        lev.clear();
-        nnz = 0;
-        near_ptr = diag_pos = nullptr;
-        near_row = near_col = nullptr;
-        near_val = nullptr;
-        elem_cl_fine = nullptr;
-        cl_near = nullptr;
-        xs = nullptr;
     }
     template <typename T>
     int alloc(T **ptr, size_t count)
@@ -133,6 +106,101 @@ struct DevFmm {
         if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc (FMM): ") + hipGetErrorString(e));
         owned.push_back((void *)*ptr);
         return 0;
+    }
+};
+
+// grow-only device buffer (kept across frequencies)
+struct Grow {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return 0;
+        hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc (FMM): ") + hipGetErrorString(e));
+        cap = std::max<size_t>(bytes, 16);
+        return 0;
+    }
+    void release()
+    {
+        hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    double *d() const { return (double *)p; }
+};
+
+// device arrays of one FMM level (layouts in ncg_kernels.hip).  The cluster structure
+// is kept while the plan's structure is unchanged; the frequency-dependent arrays (sphere
+// points, T, D, S, work vectors) are grow-only buffers.
+struct DevLevel {
+    int ncl = 0, nfan = 0;
+    double *ccen = nullptr;
+    int *cl_ptr = nullptr, *cl_elem = nullptr, *elem_cl = nullptr, *elem_pos = nullptr;
+    int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
+    int P = 0, L = 0;
+    Grow u, w, T, D, S, tw, sw, tv;   // tw, sw, tv: [nrhs][ncl*P]
+    void release()
+    {
+        for (Grow *g : {&u, &w, &T, &D, &S, &tw, &sw, &tv}) g->release();
+    }
+};
+
+// device arrays of the FMM operator
+struct DevFmm {
+    unsigned long long sig = 0;   // structure signature of the plan the arrays belong to
+    Pool st;
+    std::vector<DevLevel> lev;
+    long long nnz = 0;
+    long long *near_ptr = nullptr, *diag_pos = nullptr;
+    int *near_row = nullptr, *near_col = nullptr;
+    int *elem_cl_fine = nullptr;
+    unsigned char *cl_near = nullptr;
+    int ncl_fine = 0;
+    Grow near_val, xs;            // xs: scaled copy of the input vector
+    // evaluation-grid pairs (rebuilt when the pair lists change)
+    unsigned long long ev_sig = 0;
+    Pool evp;
+    int *pt_ev = nullptr, *ev_ptr = nullptr, *pair_ev = nullptr, *pair_cl = nullptr, *el_ptr = nullptr,
+        *el_idx = nullptr;
+    double *ecen = nullptr;
+    int npair = 0, nev = 0;
+    Grow tab, z, fphi, fgrad;
+    void release_structure()
+    {
+        for (DevLevel &L : lev) L.release();
+        lev.clear();
+        st.release();
+        near_val.release();
+        sig = 0;
+        nnz = 0;
+        evp.release();
+        ev_sig = 0;
+    }
+    void release()
+    {
+        release_structure();
+        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad}) g->release();
+    }
+};
+
+// FNV-1a over integer sequences: structure signatures of plans and pair lists
+struct Fnv {
+    unsigned long long h = 1469598103934665603ull;
+    void add(long long v)
+    {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    }
+    void add(const std::vector<int> &v)
+    {
+        add((long long)v.size());
+        for (int x : v) add((long long)x);
     }
 };
 
@@ -284,15 +352,15 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         const int mixed = p->precision == NCG_MIXED;
         const double *xin = x;
         if (d) {
-            hipMemcpyAsync(p->fmm.xs, x, sizeof(cd) * (size_t)p->nrhs * p->n, hipMemcpyDeviceToDevice, p->stream);
-            ncgk::scale_vec(p->stream, p->n, p->nrhs, d, p->fmm.xs);
-            xin = p->fmm.xs;
+            hipMemcpyAsync(p->fmm.xs.p, x, sizeof(cd) * (size_t)p->nrhs * p->n, hipMemcpyDeviceToDevice, p->stream);
+            ncgk::scale_vec(p->stream, p->n, p->nrhs, d, p->fmm.xs.p);
+            xin = p->fmm.xs.d();
         }
-        ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val, mixed, xin, y);
+        ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val.p, mixed, xin, y);
         for (DevLevel &L : p->fmm.lev) {
-            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T, mixed, xin, L.tw);
-            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D, mixed, L.tw, L.sw);
-            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S, mixed, L.sw, y);
+            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, mixed, xin, L.tw.p);
+            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, mixed, L.tw.p, L.sw.p);
+            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, mixed, L.sw.p, y);
         }
     }
     record(p, e.b);
@@ -406,9 +474,9 @@ void fill_info(const ncgf::Plan &P, int n, ncg_fmm_info *info)
 }
 
 template <typename T>
-int upload(DevFmm &F, hipStream_t s, T **dst, const std::vector<T> &src)
+int upload(Pool &pool, hipStream_t s, T **dst, const std::vector<T> &src)
 {
-    int rc = F.alloc(dst, src.size());
+    int rc = pool.alloc(dst, src.size());
     if (rc) return rc;
     if (!src.empty()) {
         hipError_t e = hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s);
@@ -417,30 +485,50 @@ int upload(DevFmm &F, hipStream_t s, T **dst, const std::vector<T> &src)
     return 0;
 }
 
-// Build and fill the device FMM operator of the current plan (NC_BuildSystemFMBEM,
-// src/NC_EquationSystem.cpp:305-365, with the CSR layouts of NC_AllocateNmtx and
-// NC_AllocateSDTmtxs*, src/NC_Addresses.cpp:1476-1937).
-int assemble_fmm(ncg_problem *p, const std::vector<int> &srch)
+template <typename T>
+int upload_grow(Grow &g, hipStream_t s, const std::vector<T> &src)
+{
+    int rc = g.ensure(sizeof(T) * src.size());
+    if (rc) return rc;
+    if (!src.empty()) {
+        hipError_t e = hipMemcpyAsync(g.p, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("FMM upload: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// structure signature of a plan: method, clusters, interaction lists, finest near lists
+unsigned long long plan_signature(const ncgf::Plan &P)
+{
+    Fnv f;
+    f.add((long long)P.method);
+    f.add((long long)P.lev.size());
+    for (const ncgf::Level &L : P.lev) {
+        f.add((long long)L.cl.size());
+        for (const ncgf::Cluster &c : L.cl) {
+            f.add(c.elems);
+            f.add(c.fan);
+        }
+    }
+    if (!P.lev.empty())
+        for (const ncgf::Cluster &c : P.lev.back().cl) f.add(c.near);
+    return f.h;
+}
+
+// cluster structure of every level and the near-field CSR on the device
+int fmm_structure(ncg_problem *p)
 {
     hipStream_t s = p->stream;
     DevFmm &F = p->fmm;
-    hipStreamSynchronize(s);
-    F.release();
-    const int n = p->n, nr = p->nrhs;
-    const int mixed = p->precision == NCG_MIXED;
-    const size_t se = mixed ? 8 : 16;
+    const int n = p->n;
     int rc;
-    const ncgk::BMParams b = p->bm();
-    const double h = p->h;
-    // levels
     for (const ncgf::Level &L : p->plan.lev) {
         DevLevel D;
         D.ncl = (int)L.cl.size();
-        D.P = L.P;
-        D.L = L.L;
         std::vector<double> cc(3 * (size_t)D.ncl);
         std::vector<int> cl_ptr(D.ncl + 1, 0), cl_elem, elem_cl(n, -1), elem_pos(n, -1), fan_ptr(D.ncl + 1, 0), fan_idx,
             fan_row;
+        cl_elem.reserve(n);
         for (int c = 0; c < D.ncl; ++c) {
             for (int d = 0; d < 3; ++d) cc[3 * c + d] = L.cl[c].cen[d];
             for (int e : L.cl[c].elems) {
@@ -456,73 +544,96 @@ int assemble_fmm(ncg_problem *p, const std::vector<int> &srch)
             fan_ptr[c + 1] = (int)fan_idx.size();
         }
         D.nfan = (int)fan_idx.size();
-        if ((rc = upload(F, s, &D.ccen, cc)) || (rc = upload(F, s, &D.u, L.u)) || (rc = upload(F, s, &D.w, L.w)) ||
-            (rc = upload(F, s, &D.cl_ptr, cl_ptr)) || (rc = upload(F, s, &D.cl_elem, cl_elem)) ||
-            (rc = upload(F, s, &D.elem_cl, elem_cl)) || (rc = upload(F, s, &D.elem_pos, elem_pos)) ||
-            (rc = upload(F, s, &D.fan_ptr, fan_ptr)) || (rc = upload(F, s, &D.fan_idx, fan_idx)) ||
-            (rc = upload(F, s, &D.fan_row, fan_row)))
+        if ((rc = upload(F.st, s, &D.ccen, cc)) || (rc = upload(F.st, s, &D.cl_ptr, cl_ptr)) ||
+            (rc = upload(F.st, s, &D.cl_elem, cl_elem)) || (rc = upload(F.st, s, &D.elem_cl, elem_cl)) ||
+            (rc = upload(F.st, s, &D.elem_pos, elem_pos)) || (rc = upload(F.st, s, &D.fan_ptr, fan_ptr)) ||
+            (rc = upload(F.st, s, &D.fan_idx, fan_idx)) || (rc = upload(F.st, s, &D.fan_row, fan_row)))
             return rc;
-        char *T, *Dv, *S;
-        if ((rc = F.alloc(&T, se * (size_t)n * D.P)) || (rc = F.alloc(&Dv, se * (size_t)std::max(D.nfan, 1) * D.P)) ||
-            (rc = F.alloc(&S, se * (size_t)n * D.P)) || (rc = F.alloc(&D.tw, 2 * (size_t)nr * D.ncl * D.P)) ||
-            (rc = F.alloc(&D.sw, 2 * (size_t)nr * D.ncl * D.P)) || (rc = F.alloc(&D.tv, 2 * (size_t)nr * D.ncl * D.P)))
-            return rc;
-        D.T = T;
-        D.D = Dv;
-        D.S = S;
         F.lev.push_back(D);
     }
-    // near field of the finest level: row i -> elements of the near clusters of its cluster
+    // near field of the finest level: row i -> elements of the near clusters of its
+    // cluster; row lengths are per-cluster sums, the columns are written on the device
     const ncgf::Level &FL = p->plan.lev.back();
+    const DevLevel &DF = F.lev.back();
     const int ncf = (int)FL.cl.size();
-    std::vector<int> cl_of(n, -1);
-    for (int c = 0; c < ncf; ++c)
-        for (int e : FL.cl[c].elems) cl_of[e] = c;
-    std::vector<long long> ptr(n + 1, 0), diag(n, -1);
-    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + near_count(FL, cl_of[i]);
-    F.nnz = ptr[n];
-    std::vector<int> rows(F.nnz), cols(F.nnz);
+    std::vector<long long> rowlen(ncf, 0);
+    std::vector<int> nn_ptr(ncf + 1, 0), nn_idx, cl_of(n, -1);
     std::vector<unsigned char> cnear((size_t)ncf * ncf, 0);
-    for (int c = 0; c < ncf; ++c)
-        for (int j : FL.cl[c].near) cnear[(size_t)c * ncf + j] = 1;
-    for (int i = 0; i < n; ++i) {
-        long long q = ptr[i];
-        for (int c = 0; c < ncf; ++c) {
-            if (!cnear[(size_t)cl_of[i] * ncf + c]) continue;
-            for (int e : FL.cl[c].elems) {
-                rows[q] = i;
-                if (e == i) diag[i] = q;
-                cols[q++] = e;
-            }
+    for (int c = 0; c < ncf; ++c) {
+        for (int j : FL.cl[c].near) {
+            rowlen[c] += (long long)FL.cl[j].elems.size();
+            nn_idx.push_back(j);
+            cnear[(size_t)c * ncf + j] = 1;
         }
+        nn_ptr[c + 1] = (int)nn_idx.size();
+        for (int e : FL.cl[c].elems) cl_of[e] = c;
     }
+    std::vector<long long> ptr(n + 1, 0);
+    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + rowlen[cl_of[i]];
+    F.nnz = ptr[n];
     F.ncl_fine = ncf;
-    if ((rc = upload(F, s, &F.near_ptr, ptr)) || (rc = upload(F, s, &F.diag_pos, diag)) ||
-        (rc = upload(F, s, &F.near_row, rows)) || (rc = upload(F, s, &F.near_col, cols)) ||
-        (rc = upload(F, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F, s, &F.cl_near, cnear)))
+    int *d_nn_ptr, *d_nn_idx;
+    if ((rc = upload(F.st, s, &F.near_ptr, ptr)) || (rc = F.st.alloc(&F.diag_pos, n)) ||
+        (rc = F.st.alloc(&F.near_row, F.nnz)) || (rc = F.st.alloc(&F.near_col, F.nnz)) ||
+        (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
+        (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)))
         return rc;
-    char *nv;
-    if ((rc = F.alloc(&nv, se * (size_t)F.nnz)) || (rc = F.alloc(&F.xs, 2 * (size_t)nr * n))) return rc;
-    F.near_val = nv;
+    ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, F.near_row, F.near_col,
+                   F.diag_pos);
+    F.sig = plan_signature(p->plan);
+    return 0;
+}
+
+// Build and fill the device FMM operator of the current plan (NC_BuildSystemFMBEM,
+// src/NC_EquationSystem.cpp:305-365, with the CSR layouts of NC_AllocateNmtx and
+// NC_AllocateSDTmtxs*, src/NC_Addresses.cpp:1476-1937).  The structure is rebuilt only
+// when the plan's clusters, interaction lists or near lists change.
+int assemble_fmm(ncg_problem *p)
+{
+    hipStream_t s = p->stream;
+    DevFmm &F = p->fmm;
+    const int n = p->n, nr = p->nrhs;
+    const int mixed = p->precision == NCG_MIXED;
+    const size_t se = mixed ? 8 : 16;
+    int rc;
+    const ncgk::BMParams b = p->bm();
+    const double h = p->h;
+    if (F.sig == 0 || F.sig != plan_signature(p->plan) || F.lev.size() != p->plan.lev.size()) {
+        NCG_HIP(hipStreamSynchronize(s));
+        F.release_structure();
+        if ((rc = fmm_structure(p))) return rc;
+    }
+    for (size_t l = 0; l < F.lev.size(); ++l) {
+        const ncgf::Level &L = p->plan.lev[l];
+        DevLevel &D = F.lev[l];
+        D.P = L.P;
+        D.L = L.L;
+        if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(se * (size_t)n * D.P)) ||
+            (rc = D.D.ensure(se * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(se * (size_t)n * D.P)) ||
+            (rc = D.tw.ensure(16 * (size_t)nr * D.ncl * D.P)) || (rc = D.sw.ensure(16 * (size_t)nr * D.ncl * D.P)) ||
+            (rc = D.tv.ensure(16 * (size_t)nr * D.ncl * D.P)))
+            return rc;
+    }
+    if ((rc = F.near_val.ensure(se * (size_t)F.nnz)) || (rc = F.xs.ensure(16 * (size_t)nr * n))) return rc;
     // values
     const ncgk::DevElements E = p->elements();
-    ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val, mixed, p->err);
+    ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val.p, mixed, p->err);
     if (p->nsrc)
-        ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near, ncf);
-    ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0, F.near_val,
-                        mixed, p->rhs, F.diag_pos);
+        ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near,
+                          F.ncl_fine);
+    ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0,
+                        F.near_val.p, mixed, p->rhs, F.diag_pos);
     for (DevLevel &D : F.lev) {
-        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u, D.D, mixed);
-        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u, D.T, mixed);
-        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u, D.w, D.S, mixed);
+        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, mixed);
+        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, mixed);
+        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u.d(), D.w.d(), D.S.p, mixed);
         if (p->nsrc) {
             // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
-            ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u, p->bcval, nr, D.tv);
-            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D, mixed, D.tv, D.sw);
-            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S, mixed, D.sw, p->rhs);
+            ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u.d(), p->bcval, nr, D.tv.d());
+            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, mixed, D.tv.d(), D.sw.d());
+            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S.p, mixed, D.sw.d(), p->rhs);
         }
     }
-    (void)srch;
     return 0;
 }
 
@@ -531,8 +642,9 @@ int assemble_fmm(ncg_problem *p, const std::vector<int> &srch)
 int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dgrad)
 {
     hipStream_t s = p->stream;
+    DevFmm &F = p->fmm;
     const ncgf::Level &L0 = p->plan.lev.front();
-    DevLevel &D0 = p->fmm.lev.front();
+    DevLevel &D0 = F.lev.front();
     const int nr = p->nrhs, m = p->m, nev = (int)p->plan.ev.size();
     const double fac = 2.0 / sqrt(3.0);
     std::vector<int> pt_ev(m, 0), ev_ptr(nev + 1, 0), pair_ev, pair_cl, el_ptr(nev + 1, 0), el_idx;
@@ -555,31 +667,37 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
         ev_ptr[a + 1] = (int)pair_ev.size();
         el_ptr[a + 1] = (int)el_idx.size();
     }
-    DevFmm T;   // per-call buffers
+    Fnv f;
+    f.add((long long)F.sig);
+    for (const std::vector<int> *v : {&pt_ev, &ev_ptr, &pair_ev, &pair_cl, &el_ptr, &el_idx}) f.add(*v);
     int rc;
-    int *d_pt_ev, *d_ev_ptr, *d_pair_ev, *d_pair_cl, *d_el_ptr, *d_el_idx;
-    double *d_ecen, *zpair, *z, *fphi, *fgrad;
-    const int npair = (int)pair_ev.size();
-    if ((rc = upload(T, s, &d_pt_ev, pt_ev)) || (rc = upload(T, s, &d_ev_ptr, ev_ptr)) ||
-        (rc = upload(T, s, &d_pair_ev, pair_ev)) || (rc = upload(T, s, &d_pair_cl, pair_cl)) ||
-        (rc = upload(T, s, &d_el_ptr, el_ptr)) || (rc = upload(T, s, &d_el_idx, el_idx)) ||
-        (rc = upload(T, s, &d_ecen, ecen)) || (rc = T.alloc(&zpair, 2 * (size_t)nr * std::max(npair, 1) * L0.P)) ||
-        (rc = T.alloc(&z, 2 * (size_t)nr * nev * L0.P)) || (rc = T.alloc(&fphi, 2 * (size_t)nr * m)) ||
-        (rc = T.alloc(&fgrad, 6 * (size_t)nr * m))) {
-        hipStreamSynchronize(s);
-        T.release();
-        return rc;
+    if (f.h != F.ev_sig) {
+        NCG_HIP(hipStreamSynchronize(s));
+        F.evp.release();
+        F.ev_sig = 0;
+        if ((rc = upload(F.evp, s, &F.pt_ev, pt_ev)) || (rc = upload(F.evp, s, &F.ev_ptr, ev_ptr)) ||
+            (rc = upload(F.evp, s, &F.pair_ev, pair_ev)) || (rc = upload(F.evp, s, &F.pair_cl, pair_cl)) ||
+            (rc = upload(F.evp, s, &F.el_ptr, el_ptr)) || (rc = upload(F.evp, s, &F.el_idx, el_idx)) ||
+            (rc = upload(F.evp, s, &F.ecen, ecen)))
+            return rc;
+        F.ev_sig = f.h;
     }
+    F.npair = (int)pair_ev.size();
+    F.nev = nev;
+    if ((rc = F.tab.ensure(sizeof(double) * (size_t)std::max(F.npair, 1) * (4 + 2 * D0.L))) ||
+        (rc = F.z.ensure(16 * (size_t)nr * nev * L0.P)) || (rc = F.fphi.ensure(16 * (size_t)nr * m)) ||
+        (rc = F.fgrad.ensure(48 * (size_t)nr * m)))
+        return rc;
     const ncgk::BMParams b = p->bm();
-    ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u, pot, vel, nr, D0.tv);
-    ncgk::fmm_eval_pairs(s, D0.L, D0.P, p->k, p->h, d_ecen, D0.ccen, d_pair_ev, d_pair_cl, npair, D0.u, D0.w, D0.tv,
-                         D0.ncl, nr, zpair);
-    ncgk::fmm_eval_sum(s, nev, D0.P, npair, d_ev_ptr, nr, zpair, z);
-    ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, d_pt_ev, d_ecen, D0.u, nr, z, fphi, fgrad);
+    ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
+                       D0.tv.d());
+    ncgk::fmm_eval_hankel(s, D0.L, p->k, p->h, F.ecen, D0.ccen, F.pair_ev, F.pair_cl, F.npair, F.tab.d());
+    ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
+                     nr, F.z.p);
+    ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
     ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad,
-                p->precision == NCG_MIXED, p->err, d_pt_ev, d_el_ptr, d_el_idx, fphi, fgrad);
+                p->precision == NCG_MIXED, p->err, F.pt_ev, F.el_ptr, F.el_idx, F.fphi.p, F.fgrad.p);
     hipError_t e = hipStreamSynchronize(s);
-    T.release();
     if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("FMM field: ") + hipGetErrorString(e));
     return 0;
 }
@@ -616,7 +734,7 @@ int ilu_setup(ncg_problem *p, int degree)
     NCG_HIP(hipStreamSynchronize(s));
     I.release();
     const bool dense = p->plan.method == 0;
-    const void *A = dense ? p->A : p->fmm.near_val;
+    const void *A = dense ? p->A : p->fmm.near_val.p;
     const long long *ptr = dense ? nullptr : p->fmm.near_ptr;
     const int *ncol = dense ? nullptr : p->fmm.near_col;
     int rc;
@@ -712,7 +830,7 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     double *T = nullptr;
     if (o->precond == NCG_PRECOND_SCALING) {
         if (p->plan.method == 0) ncgk::row_scaling(p->stream, p->A, p->precision == NCG_MIXED, n, p->dvec);
-        else ncgk::row_scaling_csr(p->stream, n, p->fmm.near_ptr, p->fmm.near_val, p->precision == NCG_MIXED, p->dvec);
+        else ncgk::row_scaling_csr(p->stream, n, p->fmm.near_ptr, p->fmm.near_val.p, p->precision == NCG_MIXED, p->dvec);
         d = p->dvec;
     } else if (ilu) {
         if ((rc = ilu_setup(p, o->ilu_scanning_degree))) return rc;
@@ -1209,7 +1327,7 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
     if (p->plan.method != 0) {
         record(p, p->ev_asm.a);
         record(p, p->ev_far.a);
-        if ((rc = assemble_fmm(p, srch))) return rc;
+        if ((rc = assemble_fmm(p))) return rc;
         record(p, p->ev_far.b);
         record(p, p->ev_asm.b);
         NCG_HIP(hipGetLastError());

@@ -19,15 +19,24 @@ double dist(const double *a, const double *b)
     return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
 }
 
-// unique node ids of a list of elements, first-appearance order
+// unique node ids of a list of elements, first-appearance order (a per-thread stamp
+// array marks the ids already seen)
 void unique_nodes(const Mesh &M, const std::vector<int> &elems, std::vector<int> &nodes, std::vector<const double *> &xyz)
 {
+    thread_local std::vector<unsigned> seen;
+    thread_local unsigned stamp = 0;
+    if (++stamp == 0) {
+        std::fill(seen.begin(), seen.end(), 0u);
+        stamp = 1;
+    }
     nodes.clear();
     xyz.clear();
     for (int e : elems)
         for (int a = 0; a < M.nv[e]; ++a) {
             const int id = M.node[4 * (size_t)e + a];
-            if (std::find(nodes.begin(), nodes.end(), id) != nodes.end()) continue;
+            if (id >= (int)seen.size()) seen.resize((size_t)id + 1 + id / 2, 0u);
+            if (seen[id] == stamp) continue;
+            seen[id] = stamp;
             nodes.push_back(id);
             xyz.push_back(&M.X[12 * (size_t)e + 3 * a]);
         }
@@ -433,39 +442,34 @@ bool make_plan(const Mesh &M, const Options &O, double k, Plan &P)
                 P.clus_edge = O.clus_edge;
                 if (P.clus_edge < sqrt(4.0 * M.avg_area)) { P.error = "The read cluster diameter is too small!"; return false; }
             }
-            Level L;
-            for (int g : M.group_order) {
-                std::vector<int> elems;
-                for (int e = 0; e < M.n; ++e)
-                    if (M.group[e] == g) elems.push_back(e);
-                if (elems.empty()) continue;
-                auto parts = split_group(M, elems, P.clus_edge, false, err);
-                cancel_small(M, parts);
-                for (auto &c : parts)
-                    if (!c.empty()) { Cluster cl; cl.elems = c; L.cl.push_back(cl); }
-            }
-            if (!err.empty()) { P.error = err; return false; }
-            finish_level(M, L);
-            lists(L, P.min_dist);
-            P.lev.push_back(L);
         } else {
             P.clus_edge = edge_ml;
-            for (int lv = 0; lv < nlev; ++lv) {
+        }
+        // the cluster tree depends on the mesh, the method and the number of levels only
+        const int tlev = method == 1 ? 1 : nlev;
+        const Tree *tree = nullptr;
+        for (const Tree &t : M.trees)
+            if (t.method == method && t.nlev == tlev) tree = &t;
+        if (!tree) {
+            Tree t;
+            t.method = method;
+            t.nlev = tlev;
+            for (int lv = 0; lv < tlev; ++lv) {
                 Level L;
-                const double edge = edge_ml / pow(2.0, lv);
+                const double edge = P.clus_edge / pow(2.0, lv);
                 if (lv == 0) {
                     for (int g : M.group_order) {
                         std::vector<int> elems;
                         for (int e = 0; e < M.n; ++e)
                             if (M.group[e] == g) elems.push_back(e);
                         if (elems.empty()) continue;
-                        auto parts = split_group(M, elems, edge_ml, false, err);
+                        auto parts = split_group(M, elems, P.clus_edge, false, err);
                         cancel_small(M, parts);
                         for (auto &c : parts)
                             if (!c.empty()) { Cluster cl; cl.elems = c; L.cl.push_back(cl); }
                     }
                 } else {
-                    const Level &F = P.lev[lv - 1];
+                    const Level &F = t.lev[lv - 1];
                     for (int f = 0; f < (int)F.cl.size(); ++f) {
                         auto parts = split_group(M, F.cl[f].elems, edge, true, err);
                         cancel_small(M, parts);
@@ -475,9 +479,14 @@ bool make_plan(const Mesh &M, const Options &O, double k, Plan &P)
                 }
                 if (!err.empty()) { P.error = err; return false; }
                 finish_level(M, L);
-                lists(L, P.min_dist);
-                P.lev.push_back(L);
+                t.lev.push_back(L);
             }
+            M.trees.push_back(std::move(t));
+            tree = &M.trees.back();
+        }
+        P.lev = tree->lev;
+        for (Level &L : P.lev) lists(L, P.min_dist);
+        if (method == 3) {
             // interaction lists (NC_AllocateDmtxMLFMM, src/NC_Addresses.cpp:1699-1767)
             for (int lv = 0; lv < nlev; ++lv) {
                 Level &L = P.lev[lv];

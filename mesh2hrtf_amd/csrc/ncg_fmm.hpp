@@ -37,6 +37,13 @@ struct EvalCluster {
     double radius = 0.0;
 };
 
+// Cluster tree of one (method, number of levels): clusters with their elements, centres,
+// radii and fathers, before the frequency-dependent near/far split.
+struct Tree {
+    int method = -1, nlev = 0;
+    std::vector<Level> lev;
+};
+
 // Frequency-independent mesh data the plan needs (boundary elements only).
 struct Mesh {
     int n = 0;
@@ -50,6 +57,7 @@ struct Mesh {
     double avg_area = 0.0, delta = 0.0, diameter = 0.0;
     std::vector<double> eval;        // [m][3]
     int m = 0;
+    mutable std::vector<Tree> trees; // trees built so far (they depend on the mesh only)
 };
 
 struct Options {

@@ -1169,91 +1169,168 @@ __global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const 
     vals[q] = cx<S>{S(a.re), S(a.im)};
 }
 
-// ---- FMM operator apply ---------------------------------------------------------------
-// near field: one wave per row
-template <typename S>
-__global__ void __launch_bounds__(256) k_near_spmv(int n, int nrhs, const long long *ptr, const int *col,
-                                                   const cx<S> *val, const cx<double> *x, cx<double> *y)
+// near-field CSR of the finest level: row i holds the elements of the near clusters of
+// its cluster, cluster by cluster (near lists ascending, elements in cluster order).
+// One wave per row; ptr comes from the host (row lengths are per-cluster sums).
+__global__ void __launch_bounds__(256) k_near_csr(int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem,
+                                                  const int *nn_ptr, const int *nn_idx, const long long *ptr,
+                                                  int *rows, int *cols, long long *diag)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    for (int r = 0; r < nrhs; ++r) {
-        const cx<double> *xr = x + (size_t)r * n;
-        double re = 0.0, im = 0.0;
-        for (long long k = ptr[i] + lane; k < ptr[i + 1]; k += 64) {
-            const cx<S> a = val[k];
-            const cx<double> v = xr[col[k]];
-            re += double(a.re) * v.re - double(a.im) * v.im;
-            im += double(a.re) * v.im + double(a.im) * v.re;
+    const int c = elem_cl[i];
+    long long q = ptr[i];
+    for (int t = nn_ptr[c]; t < nn_ptr[c + 1]; ++t) {
+        const int j = nn_idx[t];
+        const int b = cl_ptr[j], sz = cl_ptr[j + 1] - b;
+        for (int u = lane; u < sz; u += 64) {
+            const int e = cl_elem[b + u];
+            rows[q + u] = i;
+            cols[q + u] = e;
+            if (e == i) diag[i] = q + u;
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            re += __shfl_xor(re, o);
-            im += __shfl_xor(im, o);
-        }
-        if (lane == 0) y[(size_t)r * n + i] = mk(re, im);
+        q += sz;
     }
 }
 
-// t[c, p] = sum over the cluster's elements of T * x
-template <typename S>
-__global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, int nrhs, const int *cl_ptr,
-                                                     const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
-                                                     cx<double> *tw)
+// ---- FMM operator apply ---------------------------------------------------------------
+// Each kernel handles NR (1..4) load cases per pass so the operator is read once for
+// all of them; vectors are [r][...] with the given strides.
+__device__ __forceinline__ void cmac(double &re, double &im, double ar, double ai, cx<double> v)
+{
+    re += ar * v.re - ai * v.im;
+    im += ar * v.im + ai * v.re;
+}
+
+// near field: one wave per row, lanes over the row's nonzeros
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_near_spmv(int n, const long long *ptr, const int *col, const cx<S> *val,
+                                                   const cx<double> *x, cx<double> *y)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double re[NR], im[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    const long long b = ptr[i], e = ptr[i + 1];
+    for (long long k = b + lane; k < e; k += 64) {
+        const cx<S> a = val[k];
+        const int j = col[k];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + j]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        for (int o = 32; o > 0; o >>= 1) {
+            re[r] += __shfl_xor(re[r], o);
+            im[r] += __shfl_xor(im[r], o);
+        }
+        if (lane == 0) y[(size_t)r * n + i] = mk(re[r], im[r]);
+    }
+}
+
+// t[c, p] = sum over the cluster's elements of T * x; thread per (cluster, sphere point),
+// the element loop unrolled by 4 for independent loads
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
+                                                     const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)ncl * P) return;
     const int c = (int)(t / P), p = (int)(t % P);
-    for (int r = 0; r < nrhs; ++r) {
-        cx<double> acc = mk(0.0, 0.0);
-        for (int k = cl_ptr[c]; k < cl_ptr[c + 1]; ++k) {
-            const cx<S> a = Tv[(size_t)k * P + p];
-            acc += mk(double(a.re), double(a.im)) * x[(size_t)r * n + cl_elem[k]];
+    double re[NR], im[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    const int k1 = cl_ptr[c + 1];
+    int k = cl_ptr[c];
+    for (; k + 4 <= k1; k += 4) {
+        cx<S> a[4];
+        int e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = Tv[(size_t)(k + u) * P + p];
+            e[u] = cl_elem[k + u];
         }
-        tw[(size_t)r * ncl * P + t] = acc;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a[u].re), double(a[u].im), x[(size_t)r * n + e[u]]);
     }
+    for (; k < k1; ++k) {
+        const cx<S> a = Tv[(size_t)k * P + p];
+        const int e = cl_elem[k];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + e]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) tw[(size_t)r * ncl * P + t] = mk(re[r], im[r]);
 }
 
-// s[i, p] = sum over the interaction list of D * t
-template <typename S>
-__global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx,
+// s[i, p] = sum over the interaction list of D * t (D is diagonal in p)
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
                                                      const cx<S> *Dv, const cx<double> *tw, cx<double> *sw)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)ncl * P) return;
     const int i = (int)(t / P), p = (int)(t % P);
-    for (int r = 0; r < nrhs; ++r) {
-        cx<double> acc = mk(0.0, 0.0);
-        for (int q = fan_ptr[i]; q < fan_ptr[i + 1]; ++q) {
-            const cx<S> a = Dv[(size_t)q * P + p];
-            acc += mk(double(a.re), double(a.im)) * tw[(size_t)r * ncl * P + (size_t)fan_idx[q] * P + p];
+    const size_t ld = (size_t)ncl * P;
+    double re[NR], im[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    const int q1 = fan_ptr[i + 1];
+    int q = fan_ptr[i];
+    for (; q + 4 <= q1; q += 4) {
+        cx<S> a[4];
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = Dv[(size_t)(q + u) * P + p];
+            j[u] = fan_idx[q + u];
         }
-        sw[(size_t)r * ncl * P + t] = acc;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                cmac(re[r], im[r], double(a[u].re), double(a[u].im), tw[r * ld + (size_t)j[u] * P + p]);
     }
+    for (; q < q1; ++q) {
+        const cx<S> a = Dv[(size_t)q * P + p];
+        const int j = fan_idx[q];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), tw[r * ld + (size_t)j * P + p]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sw[r * ld + t] = mk(re[r], im[r]);
 }
 
 // y[e] += sum_p S[e, p] s[cluster(e), p]; one wave per element
-template <typename S>
-__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int nrhs, const int *elem_cl,
-                                                     const cx<S> *Sv, const cx<double> *sw, cx<double> *y)
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, const int *elem_cl, const cx<S> *Sv,
+                                                     const cx<double> *sw, cx<double> *y)
 {
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= n) return;
     const int c = elem_cl[e];
-    for (int r = 0; r < nrhs; ++r) {
-        double re = 0.0, im = 0.0;
-        for (int p = lane; p < P; p += 64) {
-            const cx<S> a = Sv[(size_t)e * P + p];
-            const cx<double> v = sw[(size_t)r * ncl * P + (size_t)c * P + p];
-            re += double(a.re) * v.re - double(a.im) * v.im;
-            im += double(a.re) * v.im + double(a.im) * v.re;
-        }
+    const size_t ld = (size_t)ncl * P;
+    double re[NR], im[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    for (int p = lane; p < P; p += 64) {
+        const cx<S> a = Sv[(size_t)e * P + p];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), sw[r * ld + (size_t)c * P + p]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
         for (int o = 32; o > 0; o >>= 1) {
-            re += __shfl_xor(re, o);
-            im += __shfl_xor(im, o);
+            re[r] += __shfl_xor(re[r], o);
+            im[r] += __shfl_xor(im[r], o);
         }
-        if (lane == 0) y[(size_t)r * n + e] += mk(re, im);
+        if (lane == 0) y[(size_t)r * n + e] += mk(re[r], im[r]);
     }
 }
 
@@ -1302,41 +1379,77 @@ __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, in
 
 // translation of the level-0 signatures to the evaluation clusters:
 // Z[ipc, p] = sum over far BE clusters of mu(C_ipc - C_ic, s_p) T[ic, p] w_p.
-// One block per far pair computes its Hankel functions; pairs of one evaluation cluster
-// are contiguous and reduced in a second pass.
-__global__ void __launch_bounds__(128) k_fmm_eval_pair(int L, int P, double k, double h, const double *ecen,
-                                                       const double *ccen, const int *pair_ev, const int *pair_cl,
-                                                       const double *u, const double *wsph, const cx<double> *tv,
-                                                       int ncl, int nrhs, cx<double> *zpair)
+// First pass: one thread per far pair computes the direction and the spherical Hankel
+// functions h_n(k |C_ipc - C_ic|), n < L, into [pair][4 + 2 L] doubles.
+__global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, double k, double h, const double *ecen,
+                                                         const double *ccen, const int *pair_ev, const int *pair_cl,
+                                                         double *tab)
 {
-    __shared__ double hre[kMaxTerms], him[kMaxTerms], dhat[3];
-    const int q = blockIdx.x;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npair) return;
     const int a = pair_ev[q], c = pair_cl[q];
-    if (threadIdx.x == 0) {
-        double d[3] = {ecen[3 * a] - ccen[3 * c], ecen[3 * a + 1] - ccen[3 * c + 1], ecen[3 * a + 2] - ccen[3 * c + 2]};
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        dhat[0] = d[0] / r; dhat[1] = d[1] / r; dhat[2] = d[2] / r;
-        spherical_hankel(L, r * k, h, hre, him);
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        const double cc = dhat[0] * u[3 * p] + dhat[1] * u[3 * p + 1] + dhat[2] * u[3 * p + 2];
-        const cx<double> mu = translation(L, cc, h, hre, him) * wsph[p];
-        for (int r = 0; r < nrhs; ++r)
-            zpair[((size_t)r * gridDim.x + q) * P + p] = mu * tv[(size_t)r * ncl * P + (size_t)c * P + p];
+    double d[3] = {ecen[3 * a] - ccen[3 * c], ecen[3 * a + 1] - ccen[3 * c + 1], ecen[3 * a + 2] - ccen[3 * c + 2]};
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double hre[kMaxTerms], him[kMaxTerms];
+    spherical_hankel(L, r * k, h, hre, him);
+    double *o = tab + (size_t)q * (4 + 2 * L);
+    o[0] = d[0] / r;
+    o[1] = d[1] / r;
+    o[2] = d[2] / r;
+    o[3] = 0.0;
+    for (int n = 0; n < L; ++n) {
+        o[4 + 2 * n] = hre[n];
+        o[5 + 2 * n] = him[n];
     }
 }
 
-__global__ void k_fmm_eval_sum(int nev, int P, int npair, const int *ev_ptr, int nrhs, const cx<double> *zpair,
-                               cx<double> *z)
+// Second pass: one block per evaluation cluster, threads over sphere points, summing
+// its far pairs in registers (pairs of one evaluation cluster are contiguous)
+template <int NR>
+__global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, double h, const int *ev_ptr,
+                                                    const int *pair_cl, const double *tab, const double *u,
+                                                    const double *wsph, const cx<double> *tv, int ncl,
+                                                    cx<double> *z)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nev * P) return;
-    const int a = (int)(t / P), p = (int)(t % P);
-    for (int r = 0; r < nrhs; ++r) {
-        cx<double> s = mk(0.0, 0.0);
-        for (int q = ev_ptr[a]; q < ev_ptr[a + 1]; ++q) s += zpair[((size_t)r * npair + q) * P + p];
-        z[(size_t)r * nev * P + t] = s;
+    __shared__ double sh[4 + 2 * kMaxTerms];
+    const int a = blockIdx.x;
+    const size_t ldt = (size_t)ncl * P, ldz = (size_t)nev * P;
+    for (int p0 = 0; p0 < P; p0 += blockDim.x) {
+        const int p = p0 + threadIdx.x;
+        const bool on = p < P;
+        double su[3] = {0.0, 0.0, 0.0};
+        if (on) { su[0] = u[3 * p]; su[1] = u[3 * p + 1]; su[2] = u[3 * p + 2]; }
+        cx<double> acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = mk(0.0, 0.0);
+        for (int q = ev_ptr[a]; q < ev_ptr[a + 1]; ++q) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < 4 + 2 * L; t += blockDim.x) sh[t] = tab[(size_t)q * (4 + 2 * L) + t];
+            __syncthreads();
+            if (!on) continue;
+            const double cc = sh[0] * su[0] + sh[1] * su[1] + sh[2] * su[2];
+            cx<double> mu = mk(0.0, 0.0), zi = mk(1.0, 0.0);
+            double pm1 = 1.0, pm0 = cc;
+            for (int nn = 0; nn < L; ++nn) {
+                double pn;
+                if (nn == 0) pn = 1.0;
+                else if (nn == 1) pn = cc;
+                else {
+                    pn = (pm0 * cc * (double)(2 * nn - 1) - pm1 * (double)(nn - 1)) / (double)nn;
+                    pm1 = pm0;
+                    pm0 = pn;
+                }
+                mu += zi * mk(sh[4 + 2 * nn], sh[5 + 2 * nn]) * (pn * (double)(2 * nn + 1));
+                zi = mk(-zi.im * h, zi.re * h);
+            }
+            mu = mu * wsph[p];
+            const int c = pair_cl[q];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += mu * tv[r * ldt + (size_t)c * P + p];
+        }
+        if (on)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) z[r * ldz + (size_t)a * P + p] = acc[r];
     }
 }
 
@@ -1667,47 +1780,97 @@ void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const doub
     if (mixed) k_fmm_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
     else k_fmm_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
 }
+void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
+              const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag)
+{
+    k_near_csr<<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, cl_ptr, cl_elem, nn_ptr, nn_idx, ptr, rows, cols, diag);
+}
+
+// load cases in chunks of up to 4 per launch: fn(NR, r0) launches one chunk
+template <typename F>
+static void rhs_chunks(int nrhs, F &&fn)
+{
+    for (int r0 = 0; r0 < nrhs; r0 += 4) {
+        const int nr = std::min(4, nrhs - r0);
+        fn(nr, r0);
+    }
+}
+
+#define NCG_NR_DISPATCH(NR, KERNEL, T, GRID, ...)                                       \
+    switch (NR) {                                                                       \
+    case 1: KERNEL<T, 1><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
+    case 2: KERNEL<T, 2><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
+    case 3: KERNEL<T, 3><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
+    default: KERNEL<T, 4><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                     \
+    }
+
+template <typename S>
+static void near_spmv_t(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const cx<S> *val,
+                        const cx<double> *x, cx<double> *y)
+{
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        NCG_NR_DISPATCH(nr, k_near_spmv, S, blocks_for(n, 4), n, ptr, col, val, x + (size_t)r0 * n,
+                        y + (size_t)r0 * n);
+    });
+}
 void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
                const void *x, void *y)
 {
-    if (mixed)
-        k_near_spmv<float><<<blocks_for(n, 4), 256, 0, s>>>(n, nrhs, ptr, col, (const cx<float> *)val,
-                                                            (const cx<double> *)x, (cx<double> *)y);
-    else
-        k_near_spmv<double><<<blocks_for(n, 4), 256, 0, s>>>(n, nrhs, ptr, col, (const cx<double> *)val,
-                                                             (const cx<double> *)x, (cx<double> *)y);
+    if (mixed) near_spmv_t(s, n, nrhs, ptr, col, (const cx<float> *)val, (const cx<double> *)x, (cx<double> *)y);
+    else near_spmv_t(s, n, nrhs, ptr, col, (const cx<double> *)val, (const cx<double> *)x, (cx<double> *)y);
+}
+template <typename S>
+static void apply_T_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem,
+                      const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
+{
+    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        NCG_NR_DISPATCH(nr, k_fmm_apply_T, S, g, n, ncl, P, cl_ptr, cl_elem, Tv, x + (size_t)r0 * n,
+                        tw + (size_t)r0 * ncl * P);
+    });
 }
 void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
                  int mixed, const void *x, void *tw)
 {
-    const unsigned g = blocks_for((size_t)ncl * P, 256);
     if (mixed)
-        k_fmm_apply_T<float><<<g, 256, 0, s>>>(n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<float> *)Tv,
-                                               (const cx<double> *)x, (cx<double> *)tw);
+        apply_T_t(s, n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<float> *)Tv, (const cx<double> *)x, (cx<double> *)tw);
     else
-        k_fmm_apply_T<double><<<g, 256, 0, s>>>(n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<double> *)Tv,
-                                                (const cx<double> *)x, (cx<double> *)tw);
+        apply_T_t(s, n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<double> *)Tv, (const cx<double> *)x, (cx<double> *)tw);
+}
+template <typename S>
+static void apply_D_t(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const cx<S> *Dv,
+                      const cx<double> *tw, cx<double> *sw)
+{
+    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        NCG_NR_DISPATCH(nr, k_fmm_apply_D, S, g, ncl, P, fan_ptr, fan_idx, Dv, tw + (size_t)r0 * ncl * P,
+                        sw + (size_t)r0 * ncl * P);
+    });
 }
 void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
                  int mixed, const void *tw, void *sw)
 {
-    const unsigned g = blocks_for((size_t)ncl * P, 256);
     if (mixed)
-        k_fmm_apply_D<float><<<g, 256, 0, s>>>(ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv,
-                                               (const cx<double> *)tw, (cx<double> *)sw);
+        apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv, (const cx<double> *)tw, (cx<double> *)sw);
     else
-        k_fmm_apply_D<double><<<g, 256, 0, s>>>(ncl, P, nrhs, fan_ptr, fan_idx, (const cx<double> *)Dv,
-                                                (const cx<double> *)tw, (cx<double> *)sw);
+        apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<double> *)Dv, (const cx<double> *)tw, (cx<double> *)sw);
+}
+template <typename S>
+static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const cx<S> *Sv,
+                      const cx<double> *sw, cx<double> *y)
+{
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        NCG_NR_DISPATCH(nr, k_fmm_apply_S, S, blocks_for(n, 4), n, ncl, P, elem_cl, Sv, sw + (size_t)r0 * ncl * P,
+                        y + (size_t)r0 * n);
+    });
 }
 void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const void *Sv, int mixed,
                  const void *sw, void *y)
 {
     if (mixed)
-        k_fmm_apply_S<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ncl, P, nrhs, elem_cl, (const cx<float> *)Sv,
-                                                              (const cx<double> *)sw, (cx<double> *)y);
+        apply_S_t(s, n, ncl, P, nrhs, elem_cl, (const cx<float> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
     else
-        k_fmm_apply_S<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ncl, P, nrhs, elem_cl, (const cx<double> *)Sv,
-                                                               (const cx<double> *)sw, (cx<double> *)y);
+        apply_S_t(s, n, ncl, P, nrhs, elem_cl, (const cx<double> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
 }
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
@@ -1718,18 +1881,26 @@ void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P,
                                                                     (const cx<double> *)pot, (const cx<double> *)vel,
                                                                     nrhs, (cx<double> *)tv);
 }
-void fmm_eval_pairs(hipStream_t s, int L, int P, double k, double h, const double *ecen, const double *ccen,
-                    const int *pair_ev, const int *pair_cl, int npair, const double *u, const double *w, const void *tv,
-                    int ncl, int nrhs, void *zpair)
+void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
+                     const int *pair_ev, const int *pair_cl, int npair, double *tab)
 {
     if (npair == 0) return;
-    k_fmm_eval_pair<<<npair, 128, 0, s>>>(L, P, k, h, ecen, ccen, pair_ev, pair_cl, u, w, (const cx<double> *)tv, ncl,
-                                          nrhs, (cx<double> *)zpair);
+    k_fmm_pair_hankel<<<blocks_for(npair, 128), 128, 0, s>>>(npair, L, k, h, ecen, ccen, pair_ev, pair_cl, tab);
 }
-void fmm_eval_sum(hipStream_t s, int nev, int P, int npair, const int *ev_ptr, int nrhs, const void *zpair, void *z)
+void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
+                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z)
 {
-    k_fmm_eval_sum<<<blocks_for((size_t)nev * P, 256), 256, 0, s>>>(nev, P, npair, ev_ptr, nrhs,
-                                                                    (const cx<double> *)zpair, (cx<double> *)z);
+    if (nev == 0) return;
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        const cx<double> *t = (const cx<double> *)tv + (size_t)r0 * ncl * P;
+        cx<double> *zz = (cx<double> *)z + (size_t)r0 * nev * P;
+        switch (nr) {
+        case 1: k_fmm_eval_z<1><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        case 2: k_fmm_eval_z<2><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        case 3: k_fmm_eval_z<3><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        default: k_fmm_eval_z<4><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        }
+    });
 }
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad)

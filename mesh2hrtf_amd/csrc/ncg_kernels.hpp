@@ -95,6 +95,8 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
            const double *u, const double *w, void *Sv, int mixed);
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
               long long nnz, void *vals, int mixed, int *err);
+void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
+              const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag);
 void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
                const void *x, void *y);
 void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
@@ -106,10 +108,12 @@ void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
                   void *tv);
-void fmm_eval_pairs(hipStream_t s, int L, int P, double k, double h, const double *ecen, const double *ccen,
-                    const int *pair_ev, const int *pair_cl, int npair, const double *u, const double *w, const void *tv,
-                    int ncl, int nrhs, void *zpair);
-void fmm_eval_sum(hipStream_t s, int nev, int P, int npair, const int *ev_ptr, int nrhs, const void *zpair, void *z);
+// evaluation clusters: per far pair the direction and h_n (tab [npair][4 + 2L] doubles),
+// then z[r][ev][p] = sum over the pairs of mu * w * tv
+void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
+                     const int *pair_ev, const int *pair_cl, int npair, double *tab);
+void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
+                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z);
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad);
 
