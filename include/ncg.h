@@ -46,8 +46,10 @@ enum {
 
 enum { NCG_SOLVER_CGS = 0, NCG_SOLVER_GMRES = 1, NCG_SOLVER_LU = 4 };
 /* IPRECOND of NC.inp (src/NC_Input.cpp:298-323): 0 (and -1..-3, which choose the
- * scanning degree) incomplete LU, 1 row scaling, 2 none */
-enum { NCG_PRECOND_ILU = 0, NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2 };
+ * scanning degree) incomplete LU, 1 row scaling, 2 none.  NCG_PRECOND_BLOCK has no
+ * counterpart in NumCalc: block-Jacobi with the inverses of the finest clusters'
+ * near-field self blocks (FMM plans only), for the served GMRES / mixed solves. */
+enum { NCG_PRECOND_ILU = 0, NCG_PRECOND_SCALING = 1, NCG_PRECOND_NONE = 2, NCG_PRECOND_BLOCK = 3 };
 
 /* Frequency-independent description of one boundary-element problem.
  * Replaces the geometry arrays NumCalc builds in NC_ReadMesh

@@ -161,6 +161,11 @@ struct DevFmm {
     unsigned char *cl_near = nullptr;
     int ncl_fine = 0;
     Grow near_val, xs;            // xs: scaled copy of the input vector
+    // block-Jacobi preconditioner: self-block offsets (structure) and inverses (values)
+    int *self_off = nullptr;
+    long long *boff = nullptr, bj_total = 0;
+    Grow bj_blk, bj_work;
+    int bj_ready = 0;
     // evaluation-grid pairs (rebuilt when the pair lists change)
     unsigned long long ev_sig = 0;
     Pool evp;
@@ -175,6 +180,7 @@ struct DevFmm {
         lev.clear();
         st.release();
         near_val.release();
+        bj_ready = 0;
         sig = 0;
         nnz = 0;
         evp.release();
@@ -183,7 +189,7 @@ struct DevFmm {
     void release()
     {
         release_structure();
-        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad}) g->release();
+        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad, &bj_blk, &bj_work}) g->release();
     }
 };
 
@@ -384,6 +390,7 @@ int check_err(ncg_problem *p)
     if (e) {
         NCG_HIP(hipMemsetAsync(p->err, 0, sizeof(int), p->stream));
         if (e & 8) return fail(NCG_E_ARG, "incomplete LU decomposition: zero pivot");
+        if (e & 16) return fail(NCG_E_ARG, "block-Jacobi preconditioner: singular cluster block");
         if (e & 1) return fail(NCG_E_SUBDIV, "number of subels. which are subdivided in a loop must <= 15!");
         if (e & 2) return fail(NCG_E_SUBDIV, "nsbe must be < MSBE(= 220)");
         return fail(NCG_E_SUBDIV, "adaptive subdivision deeper than 24 levels (point on an element?)");
@@ -570,13 +577,26 @@ int fmm_structure(ncg_problem *p)
     }
     std::vector<long long> ptr(n + 1, 0);
     for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + rowlen[cl_of[i]];
+    // block-Jacobi layout: offset of the own cluster inside its rows, block offsets
+    std::vector<int> self_off(ncf, 0);
+    std::vector<long long> boff(ncf + 1, 0);
+    for (int c = 0; c < ncf; ++c) {
+        for (int j : FL.cl[c].near) {
+            if (j == c) break;
+            self_off[c] += (int)FL.cl[j].elems.size();
+        }
+        const long long m = (long long)FL.cl[c].elems.size();
+        boff[c + 1] = boff[c] + m * m;
+    }
+    F.bj_total = boff[ncf];
     F.nnz = ptr[n];
     F.ncl_fine = ncf;
     int *d_nn_ptr, *d_nn_idx;
     if ((rc = upload(F.st, s, &F.near_ptr, ptr)) || (rc = F.st.alloc(&F.diag_pos, n)) ||
         (rc = F.st.alloc(&F.near_row, F.nnz)) || (rc = F.st.alloc(&F.near_col, F.nnz)) ||
         (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
-        (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)))
+        (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)) ||
+        (rc = upload(F.st, s, &F.self_off, self_off)) || (rc = upload(F.st, s, &F.boff, boff)))
         return rc;
     ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, F.near_row, F.near_col,
                    F.diag_pos);
@@ -615,6 +635,7 @@ int assemble_fmm(ncg_problem *p)
             return rc;
     }
     if ((rc = F.near_val.ensure(se * (size_t)F.nnz)) || (rc = F.xs.ensure(16 * (size_t)nr * n))) return rc;
+    F.bj_ready = 0;
     // values
     const ncgk::DevElements E = p->elements();
     ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val.p, mixed, p->err);
@@ -812,6 +833,32 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
                     I.col, I.diag, I.val, x, y);
 }
 
+// ---------------------------------------------------------------------------------- block-Jacobi
+// Inverse of every finest cluster's self block of the near field (FMM plans only),
+// computed once per assembly on first use.
+int bj_prepare(ncg_problem *p)
+{
+    DevFmm &F = p->fmm;
+    if (p->plan.method == 0) return fail(NCG_E_ARG, "the block-Jacobi preconditioner needs an FMM plan");
+    if (F.bj_ready) return 0;
+    const DevLevel &D = F.lev.back();
+    int rc;
+    if ((rc = F.bj_blk.ensure(16 * (size_t)F.bj_total)) || (rc = F.bj_work.ensure(32 * (size_t)F.bj_total))) return rc;
+    ncgk::bj_setup(p->stream, p->n, D.ncl, D.elem_cl, D.elem_pos, D.cl_ptr, F.near_ptr, F.self_off, F.near_val.p,
+                   p->precision == NCG_MIXED, F.boff, F.bj_blk.p, F.bj_work.p, p->err);
+    NCG_HIP(hipGetLastError());
+    if ((rc = check_err(p))) return rc;
+    F.bj_ready = 1;
+    return 0;
+}
+
+void bj_apply_vec(ncg_problem *p, const double *x, double *y)
+{
+    const DevFmm &F = p->fmm;
+    const DevLevel &D = F.lev.back();
+    ncgk::bj_apply(p->stream, p->n, p->nrhs, D.elem_cl, D.elem_pos, D.cl_ptr, D.cl_elem, F.boff, F.bj_blk.p, x, y);
+}
+
 // ---------------------------------------------------------------------------------- CGS
 // Right-preconditioned CGS of NC_IterativeSolverCGS (src/NC_CommonFunctions.cpp:1066-1208):
 // x0 = 0, unconjugated bilinear form, relative error measured against the residual
@@ -819,8 +866,8 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
 int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 {
     const int n = p->n, nr = p->nrhs;
-    const bool ilu = o->precond == NCG_PRECOND_ILU;
-    int rc = ensure_work(p, ilu ? 10 : 8);
+    const bool ilu = o->precond == NCG_PRECOND_ILU, blk = o->precond == NCG_PRECOND_BLOCK;
+    int rc = ensure_work(p, ilu || blk ? 10 : 8);
     if (rc) return rc;
     double *R0 = p->work[0], *R = p->work[1], *P = p->work[2], *V = p->work[3], *Q = p->work[4], *U = p->work[5],
            *UQ = p->work[6], *AUQ = p->work[7], *X = p->x;
@@ -844,9 +891,17 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
             st->ilu_levels[0] = p->ilu.nlevL;
             st->ilu_levels[1] = p->ilu.nlevU;
         }
+    } else if (blk) {
+        if ((rc = bj_prepare(p))) return rc;
+        T = p->work[9];
     }
     // operator of the Krylov loop: A diag(d) (scaling), D A (LU)^-1 (ILU) or A
     auto op = [&](const double *in, double *out) {
+        if (blk) {
+            bj_apply_vec(p, in, T);
+            mv(p, 1, T, nullptr, out);
+            return;
+        }
         if (!ilu) {
             mv(p, 1, in, d, out);
             return;
@@ -900,6 +955,10 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     }
     if (d) ncgk::scale_vec(p->stream, n, nr, d, X);
     if (ilu) ilu_apply(p, X, X);
+    if (blk) {
+        bj_apply_vec(p, X, T);
+        NCG_HIP(hipMemcpyAsync(X, T, bytes, hipMemcpyDeviceToDevice, p->stream));
+    }
     harvest_mv(p);
     int all = 1;
     for (int r = 0; r < nr; ++r) {
@@ -914,11 +973,14 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 }
 
 // ---------------------------------------------------------------------------------- GMRES
-// Restarted GMRES(m) per load case on A x = b with Hermitian inner products, classical
-// Gram-Schmidt applied twice (FP64 dots and updates on the device), Givens rotations on
-// the host.  acc64 selects the matvec accumulation (FP32 operator: 0 = FP32 inner solve).
-// Solves for the correction in 'x' starting from its current content.
-int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, double tol, int acc64,
+// Restarted GMRES(m) per load case with Hermitian inner products and classical
+// Gram-Schmidt applied twice.  The Arnoldi vectors, projections and norms stay on the
+// device; one copy of the new Hessenberg column per iteration goes to the host for the
+// Givens rotations and the convergence test.  Optional right preconditioning by the
+// block-Jacobi inverse (A M^-1 u = b, x = M^-1 u).  acc64 selects the matvec
+// accumulation (FP32 operator: 0 = FP32 inner solve).  Solves for the correction in 'x'
+// starting from its current content.
+int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, double tol, int acc64, int bj,
           std::vector<int> &iters, std::vector<double> &relres)
 {
     const int n = p->n, nr = p->nrhs;
@@ -933,9 +995,9 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
     }
     int rc = ensure_work(p, 8);
     if (rc) return rc;
-    double *W = p->work[0], *T = p->work[1];
+    double *W = p->work[0], *T = p->work[1], *Z = p->work[2];
     double *V = p->basis;
-    std::vector<cd> coef(2 * nr), hh((size_t)nr * (m + 1)), dv;
+    std::vector<cd> coef(2 * nr), dv;
     std::vector<double> bnorm(nr);
     {
         const double *u[1] = {b}, *v[1] = {b};
@@ -945,11 +1007,11 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
     iters.assign(nr, 0);
     relres.assign(nr, 1.0);
     std::vector<int> done(nr, 0);
+    std::vector<cd> hcol;
     int total = 0;
     while (total < maxit) {
         // r = b - A x  -> W
         mv(p, acc64, x, nullptr, T);
-        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(0, 0); }
         NCG_HIP(hipMemcpyAsync(W, b, sizeof(cd) * vlen, hipMemcpyDeviceToDevice, p->stream));
         for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
         NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
@@ -967,10 +1029,8 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
             if (!done[r]) all_done = false;
         }
         if (all_done) break;
-        // V0 = W / beta
-        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(beta[r] > 0 ? 1.0 / beta[r] : 0.0, 0); coef[2 * r + 1] = cd(0, 0); }
-        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-        ncgk::axpby(p->stream, n, nr, p->scal, W, n, V, n);
+        // V0 = W / beta (the norm is still in p->dotout)
+        ncgk::scale_norm(p->stream, n, nr, p->dotout, W, V);
         std::vector<std::vector<cd>> H(nr, std::vector<cd>((size_t)(m + 1) * m));
         std::vector<std::vector<cd>> g(nr, std::vector<cd>(m + 1)), cs(nr, std::vector<cd>(m)), sn(nr, std::vector<cd>(m));
         std::vector<int> kk(nr, 0);
@@ -978,27 +1038,36 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
         int k;
         for (k = 0; k < m && total < maxit; ++k, ++total) {
             double *Vk = V + 2 * (size_t)k * vlen;
-            mv(p, acc64, Vk, nullptr, W);
-            std::vector<cd> h1((size_t)nr * (k + 1)), h2((size_t)nr * (k + 1));
-            for (int pass = 0; pass < 2; ++pass) {
-                ncgk::multidot(p->stream, n, nr, k + 1, V, W, p->dotout);
-                NCG_HIP(hipMemcpyAsync(pass ? h2.data() : h1.data(), p->dotout, sizeof(cd) * nr * (k + 1),
-                                       hipMemcpyDeviceToHost, p->stream));
-                ncgk::multiaxpy(p->stream, n, nr, k + 1, V, p->dotout, W, 0);
+            if (bj) {
+                bj_apply_vec(p, Vk, Z);
+                mv(p, acc64, Z, nullptr, W);
+            } else {
+                mv(p, acc64, Vk, nullptr, W);
             }
+            // projections h1, h2 (two Gram-Schmidt passes) and the norm, all on the device:
+            // dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3]
+            cd *h1 = (cd *)p->dotout, *h2 = h1 + (size_t)nr * (k + 1), *hn = h2 + (size_t)nr * (k + 1);
+            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h1);
+            ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h1, W, 0);
+            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h2);
+            ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h2, W, 0);
             {
-                const double *u[1] = {W}, *v[1] = {W};
-                if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
+                const void *u[1] = {W}, *v[1] = {W};
+                ncgk::dots(p->stream, n, nr, 1, u, v, 1, p->partial, hn);
             }
+            ncgk::scale_norm(p->stream, n, nr, hn, W, V + 2 * (size_t)(k + 1) * vlen);
+            const size_t ncol = 2 * (size_t)nr * (k + 1) + 3 * (size_t)nr;
+            hcol.resize(ncol);
+            NCG_HIP(hipMemcpyAsync(hcol.data(), h1, sizeof(cd) * ncol, hipMemcpyDeviceToHost, p->stream));
+            NCG_HIP(hipStreamSynchronize(p->stream));
             bool all = true;
             for (int r = 0; r < nr; ++r) {
-                const double hn = sqrt(dv[3 * r].real());
-                coef[2 * r] = cd(hn > 1e-300 ? 1.0 / hn : 0.0, 0);
-                coef[2 * r + 1] = cd(0, 0);
+                const double hnr = sqrt(hcol[2 * (size_t)nr * (k + 1) + 3 * r].real());
                 if (done[r]) continue;
                 std::vector<cd> &Hr = H[r];
-                for (int i = 0; i <= k; ++i) Hr[(size_t)i * m + k] = h1[(size_t)r * (k + 1) + i] + h2[(size_t)r * (k + 1) + i];
-                Hr[(size_t)(k + 1) * m + k] = hn;
+                for (int i = 0; i <= k; ++i)
+                    Hr[(size_t)i * m + k] = hcol[(size_t)r * (k + 1) + i] + hcol[(size_t)nr * (k + 1) + (size_t)r * (k + 1) + i];
+                Hr[(size_t)(k + 1) * m + k] = hnr;
                 for (int i = 0; i < k; ++i) {   // apply previous rotations
                     const cd a = Hr[(size_t)i * m + k], bb = Hr[(size_t)(i + 1) * m + k];
                     Hr[(size_t)i * m + k] = std::conj(cs[r][i]) * a + std::conj(sn[r][i]) * bb;
@@ -1018,11 +1087,9 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
                 if (relres[r] < tol) done[r] = 1;
                 if (!done[r]) all = false;
             }
-            NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-            ncgk::axpby(p->stream, n, nr, p->scal, W, n, V + 2 * (size_t)(k + 1) * vlen, n);
             if (all) { ++total; ++k; break; }
         }
-        // y = H^-1 g per load case; x += V y
+        // y = H^-1 g per load case; x += M^-1 V y
         const int kmax = *std::max_element(kk.begin(), kk.end());
         std::vector<cd> y((size_t)nr * std::max(kmax, 1), cd(0, 0));
         for (int r = 0; r < nr; ++r) {
@@ -1034,7 +1101,16 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
         }
         if (kmax > 0) {
             NCG_HIP(hipMemcpyAsync(p->dotout, y.data(), sizeof(cd) * nr * kmax, hipMemcpyHostToDevice, p->stream));
-            ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, x, 1);
+            if (bj) {
+                NCG_HIP(hipMemsetAsync(Z, 0, sizeof(cd) * vlen, p->stream));
+                ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, Z, 1);
+                bj_apply_vec(p, Z, T);
+                for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(1, 0); }
+                NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+                ncgk::axpby(p->stream, n, nr, p->scal, T, n, x, n);
+            } else {
+                ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, x, 1);
+            }
         }
         bool fin = true;
         for (int r = 0; r < nr; ++r) if (!done[r]) fin = false;
@@ -1047,11 +1123,13 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
 int solve_gmres(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 {
     const size_t bytes = sizeof(cd) * (size_t)p->nrhs * p->n;
+    int rc;
+    const int bj = o->precond == NCG_PRECOND_BLOCK;
+    if (bj && (rc = bj_prepare(p))) return rc;
     NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
     std::vector<int> it;
     std::vector<double> rr;
-    int rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance,
-                   1, it, rr);
+    rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance, 1, bj, it, rr);
     if (rc) return rc;
     harvest_mv(p);
     int all = 1;
@@ -1071,6 +1149,8 @@ int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st
     const size_t vlen = (size_t)nr * n, bytes = sizeof(cd) * vlen;
     int rc = ensure_work(p, 10);
     if (rc) return rc;
+    const int bj = o->precond == NCG_PRECOND_BLOCK;
+    if (bj && (rc = bj_prepare(p))) return rc;
     double *Rr = p->work[8], *Dd = p->work[9];
     NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
     NCG_HIP(hipMemcpyAsync(Rr, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
@@ -1099,7 +1179,8 @@ int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st
         std::vector<int> it;
         std::vector<double> rr;
         const double inner = std::max(1e-5, target * 0.5);
-        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, it, rr))) return rc;
+        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, bj, it, rr)))
+            return rc;
         for (int r = 0; r < nr; ++r) total[r] += it[r];
         // x += d ; r = b - A x (FP64 accumulation)
         for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(1, 0); }
@@ -1363,8 +1444,8 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     if (!p || !o) return fail(NCG_E_ARG, "ncg_solve: null argument");
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_solve: assemble first");
     if (o->max_iterations < 0) return fail(NCG_E_ARG, "ncg_solve: negative iteration limit");
-    if (o->precond < NCG_PRECOND_ILU || o->precond > NCG_PRECOND_NONE)
-        return fail(NCG_E_ARG, "ncg_solve: precond must be NCG_PRECOND_ILU, _SCALING or _NONE");
+    if (o->precond < NCG_PRECOND_ILU || o->precond > NCG_PRECOND_BLOCK)
+        return fail(NCG_E_ARG, "ncg_solve: precond must be NCG_PRECOND_ILU, _SCALING, _NONE or _BLOCK");
     NCG_HIP(hipSetDevice(p->device));
     if (st) memset(st, 0, sizeof(*st));
     record(p, p->ev_solve.a);

@@ -1194,14 +1194,142 @@ __global__ void __launch_bounds__(256) k_near_csr(int n, const int *elem_cl, con
     }
 }
 
-// ---- FMM operator apply ---------------------------------------------------------------
-// Each kernel handles NR (1..4) load cases per pass so the operator is read once for
-// all of them; vectors are [r][...] with the given strides.
+// complex multiply-accumulate into separate real and imaginary sums
 __device__ __forceinline__ void cmac(double &re, double &im, double ar, double ai, cx<double> v)
 {
     re += ar * v.re - ai * v.im;
     im += ar * v.im + ai * v.re;
 }
+
+// ---- block-Jacobi preconditioner over the finest clusters ----------------------------
+// The self block of cluster c (its elements against themselves) is a contiguous segment
+// of every row of c in the near-field CSR, starting self_off[c] after the row start.
+// Blocks are stored row-major, m_c x m_c, at boff[c].
+constexpr int kMaxBlock = 1024;
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+                                                   const long long *near_ptr, const int *self_off, const cx<S> *nval,
+                                                   const long long *boff, cx<double> *blk)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int c = elem_cl[i], m = cl_ptr[c + 1] - cl_ptr[c], a = elem_pos[i] - cl_ptr[c];
+    if (m > kMaxBlock) return;
+    const cx<S> *src = nval + near_ptr[i] + self_off[c];
+    cx<double> *dst = blk + boff[c] + (size_t)a * m;
+    for (int b = lane; b < m; b += 64) dst[b] = mk(double(src[b].re), double(src[b].im));
+}
+
+// Gauss-Jordan inversion with partial pivoting, one work-group per cluster; work holds
+// the augmented [A | I] (m x 2m at 2*boff[c]); the inverse overwrites the block.
+__global__ void __launch_bounds__(256) k_bj_invert(int ncl, const int *cl_ptr, const long long *boff, cx<double> *blk,
+                                                   cx<double> *work, int *err)
+{
+    __shared__ cx<double> fac[kMaxBlock];
+    __shared__ int piv;
+    __shared__ cx<double> pval;
+    const int c = blockIdx.x;
+    const int m = cl_ptr[c + 1] - cl_ptr[c];
+    if (m > kMaxBlock) return;
+    const int w2 = 2 * m;
+    cx<double> *A = work + 2 * boff[c];
+    const cx<double> *B = blk + boff[c];
+    for (int t = threadIdx.x; t < m * w2; t += blockDim.x) {
+        const int i = t / w2, j = t % w2;
+        A[t] = j < m ? B[(size_t)i * m + j] : (j - m == i ? mk(1.0, 0.0) : mk(0.0, 0.0));
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        if (threadIdx.x == 0) {
+            int best = k;
+            double bv = -1.0;
+            for (int i = k; i < m; ++i) {
+                const cx<double> v = A[(size_t)i * w2 + k];
+                const double a2 = v.re * v.re + v.im * v.im;
+                if (a2 > bv) { bv = a2; best = i; }
+            }
+            piv = best;
+            if (bv <= 0.0) atomicOr(err, 16);
+        }
+        __syncthreads();
+        if (piv != k)
+            for (int j = threadIdx.x; j < w2; j += blockDim.x) {
+                const cx<double> t0 = A[(size_t)k * w2 + j];
+                A[(size_t)k * w2 + j] = A[(size_t)piv * w2 + j];
+                A[(size_t)piv * w2 + j] = t0;
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) pval = A[(size_t)k * w2 + k];
+        __syncthreads();
+        const cx<double> pv = pval;
+        if (pv.re == 0.0 && pv.im == 0.0) return;   // singular block: flagged above
+        for (int j = threadIdx.x; j < w2; j += blockDim.x) A[(size_t)k * w2 + j] = cdiv(A[(size_t)k * w2 + j], pv);
+        for (int i = threadIdx.x; i < m; i += blockDim.x) fac[i] = A[(size_t)i * w2 + k];
+        __syncthreads();
+        for (int t = threadIdx.x; t < m * w2; t += blockDim.x) {
+            const int i = t / w2, j = t % w2;
+            if (i != k) A[t] -= fac[i] * A[(size_t)k * w2 + j];
+        }
+        __syncthreads();
+    }
+    cx<double> *O = blk + boff[c];
+    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+        const int i = t / m, j = t % m;
+        O[t] = A[(size_t)i * w2 + m + j];
+    }
+}
+
+// y = M^-1 x: one wave per row; rows of clusters too large for a block are copied
+template <int NR>
+__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+                                                  const int *cl_elem, const long long *boff, const cx<double> *minv,
+                                                  const cx<double> *x, cx<double> *y)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int c = elem_cl[i], b0 = cl_ptr[c], m = cl_ptr[c + 1] - b0, a = elem_pos[i] - b0;
+    if (m > kMaxBlock) {
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) y[(size_t)r * n + i] = x[(size_t)r * n + i];
+        return;
+    }
+    const cx<double> *row = minv + boff[c] + (size_t)a * m;
+    double re[NR], im[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    for (int b = lane; b < m; b += 64) {
+        const cx<double> v = row[b];
+        const int j = cl_elem[b0 + b];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, x[(size_t)r * n + j]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        for (int o = 32; o > 0; o >>= 1) {
+            re[r] += __shfl_xor(re[r], o);
+            im[r] += __shfl_xor(im[r], o);
+        }
+        if (lane == 0) y[(size_t)r * n + i] = mk(re[r], im[r]);
+    }
+}
+
+// v_r = w_r / sqrt(re <w_r, w_r>) with the norm read from a device dot result [r][3]
+__global__ void k_scale_norm(int n, int nrhs, const cx<double> *dts, const cx<double> *w, cx<double> *v)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * nrhs) return;
+    const int r = (int)(t / n);
+    const double s = sqrt(dts[3 * r].re);
+    v[t] = s > 1e-300 ? w[t] * (1.0 / s) : mk(0.0, 0.0);
+}
+
+// ---- FMM operator apply ---------------------------------------------------------------
+// Each kernel handles NR (1..4) load cases per pass so the operator is read once for
+// all of them; vectors are [r][...] with the given strides.
 
 // near field: one wave per row, lanes over the row's nonzeros
 template <typename S, int NR>
@@ -1794,6 +1922,39 @@ static void rhs_chunks(int nrhs, F &&fn)
         const int nr = std::min(4, nrhs - r0);
         fn(nr, r0);
     }
+}
+
+void bj_setup(hipStream_t s, int n, int ncl, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+              const long long *near_ptr, const int *self_off, const void *nval, int mixed, const long long *boff,
+              void *blk, void *work, int *err)
+{
+    if (mixed)
+        k_bj_gather<float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, near_ptr, self_off,
+                                                            (const cx<float> *)nval, boff, (cx<double> *)blk);
+    else
+        k_bj_gather<double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, near_ptr, self_off,
+                                                             (const cx<double> *)nval, boff, (cx<double> *)blk);
+    k_bj_invert<<<ncl, 256, 0, s>>>(ncl, cl_ptr, boff, (cx<double> *)blk, (cx<double> *)work, err);
+}
+void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+              const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y)
+{
+    rhs_chunks(nrhs, [&](int nr, int r0) {
+        const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
+        cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
+        const cx<double> *M = (const cx<double> *)minv;
+        switch (nr) {
+        case 1: k_bj_apply<1><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
+        case 2: k_bj_apply<2><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
+        case 3: k_bj_apply<3><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
+        default: k_bj_apply<4><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
+        }
+    });
+}
+void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v)
+{
+    k_scale_norm<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)dts,
+                                                                    (const cx<double> *)w, (cx<double> *)v);
 }
 
 #define NCG_NR_DISPATCH(NR, KERNEL, T, GRID, ...)                                       \

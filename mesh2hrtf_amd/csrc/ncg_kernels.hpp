@@ -95,6 +95,15 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
            const double *u, const double *w, void *Sv, int mixed);
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
               long long nnz, void *vals, int mixed, int *err);
+// block-Jacobi preconditioner from the finest clusters' self blocks of the near field
+// (blocks and work sized by boff: sum of m_c^2 and twice that, complex FP64)
+void bj_setup(hipStream_t s, int n, int ncl, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+              const long long *near_ptr, const int *self_off, const void *nval, int mixed, const long long *boff,
+              void *blk, void *work, int *err);
+void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+              const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y);
+// v = w / ||w|| per load case, the norm read from a dots() result
+void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v);
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag);
 void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,

@@ -17,7 +17,7 @@ NUMCALC = os.path.join(PKG, "bin", "NumCalc")
 
 FP64, MIXED = 0, 1
 CGS, GMRES, LU = 0, 1, 4
-ILU, SCALING, NONE = 0, 1, 2
+ILU, SCALING, NONE, BLOCK = 0, 1, 2, 3
 BC_VELOCITY, BC_PRESSURE, BC_ADMITTANCE = 0, 1, 2
 
 
