@@ -142,6 +142,7 @@ struct DevLevel {
     int *cl_ptr = nullptr, *cl_elem = nullptr, *elem_cl = nullptr, *elem_pos = nullptr;
     int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
     int P = 0, L = 0;
+    int f32 = 0;                      // T, D, S stored in FP32 (mixed form, regular levels only)
     Grow u, w, T, D, S, tw, sw, tv;   // tw, sw, tv: [nrhs][ncl*P]
     void release()
     {
@@ -161,13 +162,16 @@ struct DevFmm {
     unsigned char *cl_near = nullptr;
     int ncl_fine = 0;
     Grow near_val, xs;            // xs: scaled copy of the input vector
-    // block-Jacobi preconditioner: self-block offsets (structure) and inverses (values)
-    int *self_off = nullptr;
+    // block-Jacobi preconditioner: block layout (structure) and inverses (values)
+    int *self_off = nullptr, *elem_blk = nullptr, *blk_cl = nullptr, *blk_b0 = nullptr, *blk_m = nullptr;
     long long *boff = nullptr, bj_total = 0;
-    Grow bj_blk, bj_work;
+    int nblk = 0;
+    Grow bj_blk;
     int bj_ready = 0;
     // evaluation-grid pairs (rebuilt when the pair lists change)
-    unsigned long long ev_sig = 0;
+    unsigned long long ev_sig = 0, ev_struct = 0;
+    int ev_pure = 0;
+    double ev_dmin_far = 0.0;
     Pool evp;
     int *pt_ev = nullptr, *ev_ptr = nullptr, *pair_ev = nullptr, *pair_cl = nullptr, *el_ptr = nullptr,
         *el_idx = nullptr;
@@ -189,7 +193,7 @@ struct DevFmm {
     void release()
     {
         release_structure();
-        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad, &bj_blk, &bj_work}) g->release();
+        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad, &bj_blk}) g->release();
     }
 };
 
@@ -285,6 +289,7 @@ struct ncg_problem {
     DevFmm fmm;
     size_t nrhs_fmm = 0;
     DevIlu ilu;
+    Grow out_phi, out_grad;   // evaluation-grid results before the copy to the host
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -364,9 +369,9 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         }
         ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val.p, mixed, xin, y);
         for (DevLevel &L : p->fmm.lev) {
-            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, mixed, xin, L.tw.p);
-            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, mixed, L.tw.p, L.sw.p);
-            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, mixed, L.sw.p, y);
+            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, xin, L.tw.p);
+            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, L.tw.p, L.sw.p);
+            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, L.f32, L.sw.p, y);
         }
     }
     record(p, e.b);
@@ -395,6 +400,45 @@ int check_err(ncg_problem *p)
         if (e & 2) return fail(NCG_E_SUBDIV, "nsbe must be < MSBE(= 220)");
         return fail(NCG_E_SUBDIV, "adaptive subdivision deeper than 24 levels (point on an element?)");
     }
+    return 0;
+}
+
+// Frequency-independent table of the dense method's near pairs (the pairs NumCalc
+// subdivides), built on the first dense assembly.
+int ensure_near_table(ncg_problem *p)
+{
+    if (p->near_rows) return 0;
+    hipStream_t s = p->stream;
+    const int n = p->n;
+    int rc;
+    int *cnt = nullptr;
+    long long *off = nullptr;
+    if ((rc = dalloc(&cnt, n)) || (rc = dalloc(&off, (size_t)n + 1))) {
+        hipFree(cnt);
+        return rc;
+    }
+    ncgk::near_count(s, p->elements(), cnt);
+    std::vector<int> ch(n);
+    hipMemcpyAsync(ch.data(), cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    std::vector<long long> oh((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) oh[i + 1] = oh[i] + ch[i];
+    if (e == hipSuccess) {
+        if ((rc = dalloc(&p->near_rows, (size_t)oh[n])) || (rc = dalloc(&p->near_cols, (size_t)oh[n]))) {
+            hipFree(p->near_rows);
+            p->near_rows = nullptr;
+            hipFree(cnt);
+            hipFree(off);
+            return rc;
+        }
+        p->nnear = oh[n];
+        hipMemcpyAsync(off, oh.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s);
+        ncgk::near_fill(s, p->elements(), off, p->near_rows, p->near_cols);
+        e = hipStreamSynchronize(s);
+    }
+    hipFree(cnt);
+    hipFree(off);
+    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("near table: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -577,18 +621,32 @@ int fmm_structure(ncg_problem *p)
     }
     std::vector<long long> ptr(n + 1, 0);
     for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + rowlen[cl_of[i]];
-    // block-Jacobi layout: offset of the own cluster inside its rows, block offsets
-    std::vector<int> self_off(ncf, 0);
-    std::vector<long long> boff(ncf + 1, 0);
+    // block-Jacobi layout: offset of the own cluster inside its rows; blocks are runs of
+    // at most 64 consecutive elements of a cluster's list, of balanced sizes
+    std::vector<int> self_off(ncf, 0), elem_blk(n, -1), blk_cl, blk_b0, blk_m;
+    std::vector<long long> boff(1, 0);
+    int cstart = 0;
     for (int c = 0; c < ncf; ++c) {
         for (int j : FL.cl[c].near) {
             if (j == c) break;
             self_off[c] += (int)FL.cl[j].elems.size();
         }
-        const long long m = (long long)FL.cl[c].elems.size();
-        boff[c + 1] = boff[c] + m * m;
+        const int m = (int)FL.cl[c].elems.size(), nch = (m + 63) / 64;
+        int start = 0;
+        for (int q = 0; q < nch; ++q) {
+            const int mb = m / nch + (q < m % nch ? 1 : 0);
+            const int b = (int)blk_m.size();
+            blk_cl.push_back(c);
+            blk_b0.push_back(cstart + start);
+            blk_m.push_back(mb);
+            boff.push_back(boff.back() + (long long)mb * mb);
+            for (int t = start; t < start + mb; ++t) elem_blk[FL.cl[c].elems[t]] = b;
+            start += mb;
+        }
+        cstart += m;
     }
-    F.bj_total = boff[ncf];
+    F.nblk = (int)blk_m.size();
+    F.bj_total = boff.back();
     F.nnz = ptr[n];
     F.ncl_fine = ncf;
     int *d_nn_ptr, *d_nn_idx;
@@ -596,7 +654,9 @@ int fmm_structure(ncg_problem *p)
         (rc = F.st.alloc(&F.near_row, F.nnz)) || (rc = F.st.alloc(&F.near_col, F.nnz)) ||
         (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
         (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)) ||
-        (rc = upload(F.st, s, &F.self_off, self_off)) || (rc = upload(F.st, s, &F.boff, boff)))
+        (rc = upload(F.st, s, &F.self_off, self_off)) || (rc = upload(F.st, s, &F.boff, boff)) ||
+        (rc = upload(F.st, s, &F.elem_blk, elem_blk)) || (rc = upload(F.st, s, &F.blk_cl, blk_cl)) ||
+        (rc = upload(F.st, s, &F.blk_b0, blk_b0)) || (rc = upload(F.st, s, &F.blk_m, blk_m)))
         return rc;
     ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, F.near_row, F.near_col,
                    F.diag_pos);
@@ -628,8 +688,13 @@ int assemble_fmm(ncg_problem *p)
         DevLevel &D = F.lev[l];
         D.P = L.P;
         D.L = L.L;
-        if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(se * (size_t)n * D.P)) ||
-            (rc = D.D.ensure(se * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(se * (size_t)n * D.P)) ||
+        // A level whose expansion length sits at the minimum (small k R) sums large
+        // translation values against O(1) aggregation and disaggregation entries with
+        // cancellation: its T, D and S stay FP64 in the mixed form.
+        D.f32 = mixed && L.L > p->fo.min_terms;
+        const size_t sf = D.f32 ? 8 : 16;
+        if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(sf * (size_t)n * D.P)) ||
+            (rc = D.D.ensure(sf * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(sf * (size_t)n * D.P)) ||
             (rc = D.tw.ensure(16 * (size_t)nr * D.ncl * D.P)) || (rc = D.sw.ensure(16 * (size_t)nr * D.ncl * D.P)) ||
             (rc = D.tv.ensure(16 * (size_t)nr * D.ncl * D.P)))
             return rc;
@@ -645,14 +710,14 @@ int assemble_fmm(ncg_problem *p)
     ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0,
                         F.near_val.p, mixed, p->rhs, F.diag_pos);
     for (DevLevel &D : F.lev) {
-        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, mixed);
-        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, mixed);
-        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u.d(), D.w.d(), D.S.p, mixed);
+        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32);
+        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, D.f32);
+        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u.d(), D.w.d(), D.S.p, D.f32);
         if (p->nsrc) {
             // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
             ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u.d(), p->bcval, nr, D.tv.d());
-            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, mixed, D.tv.d(), D.sw.d());
-            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S.p, mixed, D.sw.d(), p->rhs);
+            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, D.f32, D.tv.d(), D.sw.d());
+            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S.p, D.f32, D.sw.d(), p->rhs);
         }
     }
     return 0;
@@ -668,58 +733,70 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
     DevLevel &D0 = F.lev.front();
     const int nr = p->nrhs, m = p->m, nev = (int)p->plan.ev.size();
     const double fac = 2.0 / sqrt(3.0);
-    std::vector<int> pt_ev(m, 0), ev_ptr(nev + 1, 0), pair_ev, pair_cl, el_ptr(nev + 1, 0), el_idx;
-    std::vector<double> ecen(3 * (size_t)nev);
-    for (int a = 0; a < nev; ++a) {
-        const ncgf::EvalCluster &E = p->plan.ev[a];
-        for (int d = 0; d < 3; ++d) ecen[3 * a + d] = E.cen[d];
-        for (int x : E.pts) pt_ev[x] = a;
-        for (int c = 0; c < (int)L0.cl.size(); ++c) {
-            const ncgf::Cluster &C = L0.cl[c];
-            const double dx = C.cen[0] - E.cen[0], dy = C.cen[1] - E.cen[1], dz = C.cen[2] - E.cen[2];
-            const double di = sqrt(dx * dx + dy * dy + dz * dz), dk = C.radius + E.radius;
-            if (di > fac * dk && di > p->plan.min_dist) {
-                pair_ev.push_back(a);
-                pair_cl.push_back(c);
-            } else {
-                el_idx.insert(el_idx.end(), C.elems.begin(), C.elems.end());
-            }
-        }
-        ev_ptr[a + 1] = (int)pair_ev.size();
-        el_ptr[a + 1] = (int)el_idx.size();
-    }
-    Fnv f;
-    f.add((long long)F.sig);
-    for (const std::vector<int> *v : {&pt_ev, &ev_ptr, &pair_ev, &pair_cl, &el_ptr, &el_idx}) f.add(*v);
     int rc;
-    if (f.h != F.ev_sig) {
-        NCG_HIP(hipStreamSynchronize(s));
-        F.evp.release();
-        F.ev_sig = 0;
-        if ((rc = upload(F.evp, s, &F.pt_ev, pt_ev)) || (rc = upload(F.evp, s, &F.ev_ptr, ev_ptr)) ||
-            (rc = upload(F.evp, s, &F.pair_ev, pair_ev)) || (rc = upload(F.evp, s, &F.pair_cl, pair_cl)) ||
-            (rc = upload(F.evp, s, &F.el_ptr, el_ptr)) || (rc = upload(F.evp, s, &F.el_idx, el_idx)) ||
-            (rc = upload(F.evp, s, &F.ecen, ecen)))
-            return rc;
-        F.ev_sig = f.h;
+    // The pair lists depend on the level-0 clusters and, through the cancelled-distance
+    // test di > min_dist, on k.  Lists built while min_dist was below every geometric-far
+    // distance hold for any smaller min_dist as well: they are reused without a rebuild.
+    const bool reuse = F.ev_sig != 0 && F.ev_struct == F.sig && F.ev_pure && p->plan.min_dist < F.ev_dmin_far;
+    if (!reuse) {
+        std::vector<int> pt_ev(m, 0), ev_ptr(nev + 1, 0), pair_ev, pair_cl, el_ptr(nev + 1, 0), el_idx;
+        std::vector<double> ecen(3 * (size_t)nev);
+        double dmin_far = 1e300;
+        bool pure = true;
+        for (int a = 0; a < nev; ++a) {
+            const ncgf::EvalCluster &E = p->plan.ev[a];
+            for (int d = 0; d < 3; ++d) ecen[3 * a + d] = E.cen[d];
+            for (int x : E.pts) pt_ev[x] = a;
+            for (int c = 0; c < (int)L0.cl.size(); ++c) {
+                const ncgf::Cluster &C = L0.cl[c];
+                const double dx = C.cen[0] - E.cen[0], dy = C.cen[1] - E.cen[1], dz = C.cen[2] - E.cen[2];
+                const double di = sqrt(dx * dx + dy * dy + dz * dz), dk = C.radius + E.radius;
+                const bool geo_far = di > fac * dk;
+                if (geo_far) dmin_far = std::min(dmin_far, di);
+                if (geo_far && di > p->plan.min_dist) {
+                    pair_ev.push_back(a);
+                    pair_cl.push_back(c);
+                } else {
+                    if (geo_far) pure = false;
+                    el_idx.insert(el_idx.end(), C.elems.begin(), C.elems.end());
+                }
+            }
+            ev_ptr[a + 1] = (int)pair_ev.size();
+            el_ptr[a + 1] = (int)el_idx.size();
+        }
+        Fnv f;
+        f.add((long long)F.sig);
+        for (const std::vector<int> *v : {&pt_ev, &ev_ptr, &pair_ev, &pair_cl, &el_ptr, &el_idx}) f.add(*v);
+        if (f.h != F.ev_sig) {
+            NCG_HIP(hipStreamSynchronize(s));
+            F.evp.release();
+            F.ev_sig = 0;
+            if ((rc = upload(F.evp, s, &F.pt_ev, pt_ev)) || (rc = upload(F.evp, s, &F.ev_ptr, ev_ptr)) ||
+                (rc = upload(F.evp, s, &F.pair_ev, pair_ev)) || (rc = upload(F.evp, s, &F.pair_cl, pair_cl)) ||
+                (rc = upload(F.evp, s, &F.el_ptr, el_ptr)) || (rc = upload(F.evp, s, &F.el_idx, el_idx)) ||
+                (rc = upload(F.evp, s, &F.ecen, ecen)))
+                return rc;
+            F.ev_sig = f.h;
+        }
+        F.ev_struct = F.sig;
+        F.ev_pure = pure;
+        F.ev_dmin_far = dmin_far;
+        F.npair = (int)pair_ev.size();
+        F.nev = nev;
     }
-    F.npair = (int)pair_ev.size();
-    F.nev = nev;
     if ((rc = F.tab.ensure(sizeof(double) * (size_t)std::max(F.npair, 1) * (4 + 2 * D0.L))) ||
         (rc = F.z.ensure(16 * (size_t)nr * nev * L0.P)) || (rc = F.fphi.ensure(16 * (size_t)nr * m)) ||
         (rc = F.fgrad.ensure(48 * (size_t)nr * m)))
         return rc;
     const ncgk::BMParams b = p->bm();
     ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
-                       D0.tv.d());
+                       D0.tv.d(), D0.f32);
     ncgk::fmm_eval_hankel(s, D0.L, p->k, p->h, F.ecen, D0.ccen, F.pair_ev, F.pair_cl, F.npair, F.tab.d());
     ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
                      nr, F.z.p);
     ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
     ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad,
                 p->precision == NCG_MIXED, p->err, F.pt_ev, F.el_ptr, F.el_idx, F.fphi.p, F.fgrad.p);
-    hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("FMM field: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -843,9 +920,9 @@ int bj_prepare(ncg_problem *p)
     if (F.bj_ready) return 0;
     const DevLevel &D = F.lev.back();
     int rc;
-    if ((rc = F.bj_blk.ensure(16 * (size_t)F.bj_total)) || (rc = F.bj_work.ensure(32 * (size_t)F.bj_total))) return rc;
-    ncgk::bj_setup(p->stream, p->n, D.ncl, D.elem_cl, D.elem_pos, D.cl_ptr, F.near_ptr, F.self_off, F.near_val.p,
-                   p->precision == NCG_MIXED, F.boff, F.bj_blk.p, F.bj_work.p, p->err);
+    if ((rc = F.bj_blk.ensure(16 * (size_t)F.bj_total))) return rc;
+    ncgk::bj_setup(p->stream, p->n, F.nblk, F.elem_blk, D.elem_pos, F.blk_cl, F.blk_b0, F.blk_m, D.cl_ptr, F.near_ptr,
+                   F.self_off, F.near_val.p, p->precision == NCG_MIXED, F.boff, F.bj_blk.p, p->err);
     NCG_HIP(hipGetLastError());
     if ((rc = check_err(p))) return rc;
     F.bj_ready = 1;
@@ -856,7 +933,8 @@ void bj_apply_vec(ncg_problem *p, const double *x, double *y)
 {
     const DevFmm &F = p->fmm;
     const DevLevel &D = F.lev.back();
-    ncgk::bj_apply(p->stream, p->n, p->nrhs, D.elem_cl, D.elem_pos, D.cl_ptr, D.cl_elem, F.boff, F.bj_blk.p, x, y);
+    ncgk::bj_apply(p->stream, p->n, p->nrhs, F.elem_blk, D.elem_pos, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, x,
+                   y);
 }
 
 // ---------------------------------------------------------------------------------- CGS
@@ -1284,26 +1362,8 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     if (p->m) hipMemcpyAsync(p->eval, g->eval_xyz, sizeof(double) * 3 * p->m, hipMemcpyHostToDevice, s);
     hipMemsetAsync(p->err, 0, sizeof(int), s);
     ncgk::element_geometry(s, n, p->X, p->nv, p->cen, p->nrm, p->area, p->jac);
-    // near-pair table
-    int *cnt = nullptr;
-    long long *off = nullptr;
-    if ((rc = dalloc(&cnt, n)) || (rc = dalloc(&off, (size_t)n + 1))) { hipFree(cnt); return bail(rc); }
-    ncgk::near_count(s, p->elements(), cnt);
-    std::vector<int> ch(n);
-    hipMemcpyAsync(ch.data(), cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess) { hipFree(cnt); hipFree(off); return bail(fail(NCG_E_DEVICE, "geometry kernels failed")); }
-    std::vector<long long> oh((size_t)n + 1, 0);
-    for (int i = 0; i < n; ++i) oh[i + 1] = oh[i] + ch[i];
-    p->nnear = oh[n];
-    if ((rc = dalloc(&p->near_rows, (size_t)p->nnear)) || (rc = dalloc(&p->near_cols, (size_t)p->nnear))) {
-        hipFree(cnt); hipFree(off); return bail(rc);
-    }
-    hipMemcpyAsync(off, oh.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s);
-    ncgk::near_fill(s, p->elements(), off, p->near_rows, p->near_cols);
     hipError_t e = hipStreamSynchronize(s);
-    hipFree(cnt);
-    hipFree(off);
-    if (e != hipSuccess) return bail(fail(NCG_E_DEVICE, std::string("near table: ") + hipGetErrorString(e)));
+    if (e != hipSuccess) return bail(fail(NCG_E_DEVICE, std::string("geometry kernels: ") + hipGetErrorString(e)));
     *out = p;
     return 0;
 }
@@ -1322,6 +1382,8 @@ int ncg_problem_destroy(ncg_problem *p)
     for (double *w : p->work) hipFree(w);
     p->fmm.release();
     p->ilu.release();
+    p->out_phi.release();
+    p->out_grad.release();
     for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
         if (e->a) hipEventDestroy(e->a);
         if (e->b) hipEventDestroy(e->b);
@@ -1418,6 +1480,7 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
         p->assembled = 1;
         return 0;
     }
+    if ((rc = ensure_near_table(p))) return rc;
     record(p, p->ev_asm.a);
     record(p, p->ev_far.a);
     ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->evals);
@@ -1499,20 +1562,13 @@ int ncg_field(ncg_problem *p, double *phi, double *grad)
     int rc = ensure_work(p, 8);
     if (rc) return rc;
     double *pot = p->work[0], *vel = p->work[1];
-    double *dphi = nullptr, *dgrad = nullptr;
-    if ((rc = dalloc(&dphi, 2 * (size_t)nr * m)) || (rc = dalloc(&dgrad, 6 * (size_t)nr * m))) {
-        hipFree(dphi);
-        return rc;
-    }
+    if ((rc = p->out_phi.ensure(16 * (size_t)nr * m)) || (rc = p->out_grad.ensure(48 * (size_t)nr * m))) return rc;
+    double *dphi = p->out_phi.d(), *dgrad = p->out_grad.d();
     record(p, p->ev_field.a);
     ncgk::field_prep(p->stream, n, nr, p->kind, p->adm, p->interior ? -1.0 : 1.0, p->bcval, p->x, pot, vel);
     if (p->plan.method != 0) {
         rc = field_fmm(p, pot, vel, dphi, grad ? dgrad : nullptr);
-        if (rc) {
-            hipFree(dphi);
-            hipFree(dgrad);
-            return rc;
-        }
+        if (rc) return rc;
     } else {
         ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
                     grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
@@ -1522,8 +1578,6 @@ int ncg_field(ncg_problem *p, double *phi, double *grad)
     if (e == hipSuccess) e = hipMemcpyAsync(phi, dphi, sizeof(cd) * nr * m, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, dgrad, sizeof(cd) * 3 * nr * m, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    hipFree(dphi);
-    hipFree(dgrad);
     if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("ncg_field: ") + hipGetErrorString(e));
     p->t_field = elapsed(p->ev_field);
     return check_err(p);

@@ -1121,8 +1121,10 @@ __global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, co
     Sv[(size_t)e * P + p] = cx<S>{S(v.re), S(v.im)};
 }
 
-// near-field entries of the FMM operator: one thread per off-diagonal nonzero
-template <typename S>
+// near-field entries of the FMM operator: one thread per off-diagonal nonzero.  Pairs
+// NumCalc subdivides are integrated in FP64; the regular rule of the other near pairs
+// runs in T (FP32 in the mixed form, with the geometry differences formed in FP64).
+template <typename S, typename T>
 __global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *rows,
                                                   const int *cols, long long nnz, cx<S> *vals, int *err)
 {
@@ -1130,42 +1132,51 @@ __global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const 
     if (q >= nnz) return;
     const int i = rows[q], j = cols[q];
     if (i == j) return;   // diagonal: k_assemble_diag
-    BMAcc<double> acc;
     const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
     const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
     LocalElem L;
     load_local(E, j, x, L);
     const double ratdis = ratio_to_centre(E.cen[j], E.cen[E.n + j], E.cen[2 * E.n + j], x, L.area);
-    acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
     if (ratdis < kTolf) {
+        BMAcc<double> acc;
+        acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
         const int e = visit_adaptive(L, [&](double s, double t, double w) {
             double q2[3], ny[3], J;
             point_on(L, s, t, q2, ny, J);
             bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * J, bm.hk, bm.k2);
         });
         if (e) atomicOr(err, e);
-    } else {
-        const int m = gauss_order(ratdis);
-        if (L.nv == 3) {
-            const int off = tri_rule_offset(m), cnt = tri_rule_count(m);
-            for (int g = 0; g < cnt; ++g) {
-                double q2[3], ny[3], J;
-                point_on(L, c_tri[off + g][0], c_tri[off + g][1], q2, ny, J);
-                bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], c_tri[off + g][2] * J,
-                                 bm.hk, bm.k2);
-            }
-        } else {
-            const int base = m * (m - 1) / 2;
-            for (int a = 0; a < m; ++a)
-                for (int b = 0; b < m; ++b) {
-                    double q2[3], ny[3], J;
-                    point_on(L, c_gl_x[base + a], c_gl_x[base + b], q2, ny, J);
-                    bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2],
-                                     c_gl_w[base + a] * c_gl_w[base + b] * J, bm.hk, bm.k2);
-                }
-        }
+        const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+        vals[q] = cx<S>{S(a.re), S(a.im)};
+        return;
     }
-    const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+    BMAcc<T> acc;
+    acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
+    const T nxx = T(nx[0]), nxy = T(nx[1]), nxz = T(nx[2]), hk = T(bm.hk), k2 = T(bm.k2);
+    const int m = gauss_order(ratdis);
+    if (L.nv == 3) {
+        // x(s,t) - x_i = p2 + s (p0 - p2) + t (p1 - p2), nodes already relative to x_i
+        const T e1x = T(L.p[0][0] - L.p[2][0]), e1y = T(L.p[0][1] - L.p[2][1]), e1z = T(L.p[0][2] - L.p[2][2]);
+        const T e2x = T(L.p[1][0] - L.p[2][0]), e2y = T(L.p[1][1] - L.p[2][1]), e2z = T(L.p[1][2] - L.p[2][2]);
+        const T px = T(L.p[2][0]), py = T(L.p[2][1]), pz = T(L.p[2][2]);
+        const T nyx = T(E.nrm[j]), nyy = T(E.nrm[E.n + j]), nyz = T(E.nrm[2 * E.n + j]), jac = T(E.jac[j]);
+        const int off = tri_rule_offset(m), cnt = tri_rule_count(m);
+        for (int g = 0; g < cnt; ++g) {
+            const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), w = T(c_tri[off + g][2]);
+            bm_point<T>(acc, px + s * e1x + t * e2x, py + s * e1y + t * e2y, pz + s * e1z + t * e2z, nyx, nyy, nyz, nxx,
+                        nxy, nxz, w * jac, hk, k2);
+        }
+    } else {
+        const int base = m * (m - 1) / 2;
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                double q2[3], ny[3], J;
+                point_on(L, c_gl_x[base + a], c_gl_x[base + b], q2, ny, J);
+                bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
+                            T(c_gl_w[base + a] * c_gl_w[base + b] * J), hk, k2);
+            }
+    }
+    const cx<T> a = bm_entry<T>(acc, E.kind[j], cx<T>{T(adm[2 * j]), T(adm[2 * j + 1])}, T(bm.tau), T(bm.beta_im));
     vals[q] = cx<S>{S(a.re), S(a.im)};
 }
 
@@ -1202,110 +1213,114 @@ __device__ __forceinline__ void cmac(double &re, double &im, double ar, double a
 }
 
 // ---- block-Jacobi preconditioner over the finest clusters ----------------------------
-// The self block of cluster c (its elements against themselves) is a contiguous segment
-// of every row of c in the near-field CSR, starting self_off[c] after the row start.
-// Blocks are stored row-major, m_c x m_c, at boff[c].
-constexpr int kMaxBlock = 1024;
+// Blocks are runs of at most kBlk consecutive elements of a finest cluster's element
+// list.  A block's rows hold its columns as a contiguous piece of the near-field CSR
+// row: the cluster's own segment starts self_off[c] after the row start, the block
+// blk_b0[b] - cl_ptr[c] further.  Inverses are stored row-major m x m at boff[b].
+constexpr int kBlk = 64;
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
+__global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_blk, const int *elem_pos, const int *blk_cl,
+                                                   const int *blk_b0, const int *blk_m, const int *cl_ptr,
                                                    const long long *near_ptr, const int *self_off, const cx<S> *nval,
                                                    const long long *boff, cx<double> *blk)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    const int c = elem_cl[i], m = cl_ptr[c + 1] - cl_ptr[c], a = elem_pos[i] - cl_ptr[c];
-    if (m > kMaxBlock) return;
-    const cx<S> *src = nval + near_ptr[i] + self_off[c];
-    cx<double> *dst = blk + boff[c] + (size_t)a * m;
-    for (int b = lane; b < m; b += 64) dst[b] = mk(double(src[b].re), double(src[b].im));
+    const int b = elem_blk[i], c = blk_cl[b], m = blk_m[b], a = elem_pos[i] - blk_b0[b];
+    const cx<S> *src = nval + near_ptr[i] + self_off[c] + (blk_b0[b] - cl_ptr[c]);
+    cx<double> *dst = blk + boff[b] + (size_t)a * m;
+    for (int j = lane; j < m; j += 64) dst[j] = mk(double(src[j].re), double(src[j].im));
 }
 
-// Gauss-Jordan inversion with partial pivoting, one work-group per cluster; work holds
-// the augmented [A | I] (m x 2m at 2*boff[c]); the inverse overwrites the block.
-__global__ void __launch_bounds__(256) k_bj_invert(int ncl, const int *cl_ptr, const long long *boff, cx<double> *blk,
-                                                   cx<double> *work, int *err)
+// In-place Gauss-Jordan inversion with partial (row) pivoting, one work-group per block,
+// the block held in LDS; the row interchanges are undone as column interchanges at the end.
+__global__ void __launch_bounds__(256) k_bj_invert(const int *blk_m, const long long *boff, cx<double> *blk, int *err)
 {
-    __shared__ cx<double> fac[kMaxBlock];
+    __shared__ cx<double> A[kBlk * kBlk];
+    __shared__ cx<double> fac[kBlk];
+    __shared__ int perm[kBlk];
     __shared__ int piv;
-    __shared__ cx<double> pval;
-    const int c = blockIdx.x;
-    const int m = cl_ptr[c + 1] - cl_ptr[c];
-    if (m > kMaxBlock) return;
-    const int w2 = 2 * m;
-    cx<double> *A = work + 2 * boff[c];
-    const cx<double> *B = blk + boff[c];
-    for (int t = threadIdx.x; t < m * w2; t += blockDim.x) {
-        const int i = t / w2, j = t % w2;
-        A[t] = j < m ? B[(size_t)i * m + j] : (j - m == i ? mk(1.0, 0.0) : mk(0.0, 0.0));
-    }
+    const int b = blockIdx.x, m = blk_m[b], tid = threadIdx.x;
+    cx<double> *G = blk + boff[b];
+    for (int t = tid; t < m * m; t += blockDim.x) A[(t / m) * kBlk + t % m] = G[t];
     __syncthreads();
     for (int k = 0; k < m; ++k) {
-        if (threadIdx.x == 0) {
-            int best = k;
-            double bv = -1.0;
-            for (int i = k; i < m; ++i) {
-                const cx<double> v = A[(size_t)i * w2 + k];
-                const double a2 = v.re * v.re + v.im * v.im;
-                if (a2 > bv) { bv = a2; best = i; }
+        if (tid < 64) {
+            const int i = k + tid;
+            double v = -1.0;
+            int idx = i;
+            if (i < m) {
+                const cx<double> a = A[i * kBlk + k];
+                v = a.re * a.re + a.im * a.im;
             }
-            piv = best;
-            if (bv <= 0.0) atomicOr(err, 16);
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(v, o);
+                const int i2 = __shfl_xor(idx, o);
+                if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+            }
+            if (tid == 0) {
+                piv = idx;
+                perm[k] = idx;
+                if (!(v > 0.0)) atomicOr(err, 16);
+            }
         }
         __syncthreads();
-        if (piv != k)
-            for (int j = threadIdx.x; j < w2; j += blockDim.x) {
-                const cx<double> t0 = A[(size_t)k * w2 + j];
-                A[(size_t)k * w2 + j] = A[(size_t)piv * w2 + j];
-                A[(size_t)piv * w2 + j] = t0;
+        const int pr = piv;
+        if (pr != k)
+            for (int j = tid; j < m; j += blockDim.x) {
+                const cx<double> t0 = A[k * kBlk + j];
+                A[k * kBlk + j] = A[pr * kBlk + j];
+                A[pr * kBlk + j] = t0;
             }
         __syncthreads();
-        if (threadIdx.x == 0) pval = A[(size_t)k * w2 + k];
+        const cx<double> pv = A[k * kBlk + k];
+        const cx<double> inv = cdiv(mk(1.0, 0.0), pv);
         __syncthreads();
-        const cx<double> pv = pval;
-        if (pv.re == 0.0 && pv.im == 0.0) return;   // singular block: flagged above
-        for (int j = threadIdx.x; j < w2; j += blockDim.x) A[(size_t)k * w2 + j] = cdiv(A[(size_t)k * w2 + j], pv);
-        for (int i = threadIdx.x; i < m; i += blockDim.x) fac[i] = A[(size_t)i * w2 + k];
+        for (int j = tid; j < m; j += blockDim.x) A[k * kBlk + j] = j == k ? inv : A[k * kBlk + j] * inv;
+        for (int i = tid; i < m; i += blockDim.x) fac[i] = i == k ? mk(0.0, 0.0) : A[i * kBlk + k];
         __syncthreads();
-        for (int t = threadIdx.x; t < m * w2; t += blockDim.x) {
-            const int i = t / w2, j = t % w2;
-            if (i != k) A[t] -= fac[i] * A[(size_t)k * w2 + j];
+        for (int t = tid; t < m * m; t += blockDim.x) {
+            const int i = t / m, j = t % m;
+            if (i == k) continue;
+            const cx<double> base = j == k ? mk(0.0, 0.0) : A[i * kBlk + j];
+            A[i * kBlk + j] = base - fac[i] * A[k * kBlk + j];
         }
         __syncthreads();
     }
-    cx<double> *O = blk + boff[c];
-    for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-        const int i = t / m, j = t % m;
-        O[t] = A[(size_t)i * w2 + m + j];
+    for (int k = m - 1; k >= 0; --k) {
+        const int pk = perm[k];
+        if (pk != k)
+            for (int i = tid; i < m; i += blockDim.x) {
+                const cx<double> t0 = A[i * kBlk + k];
+                A[i * kBlk + k] = A[i * kBlk + pk];
+                A[i * kBlk + pk] = t0;
+            }
+        __syncthreads();
     }
+    for (int t = tid; t < m * m; t += blockDim.x) G[t] = A[(t / m) * kBlk + t % m];
 }
 
-// y = M^-1 x: one wave per row; rows of clusters too large for a block are copied
+// y = M^-1 x: one wave per row
 template <int NR>
-__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
-                                                  const int *cl_elem, const long long *boff, const cx<double> *minv,
-                                                  const cx<double> *x, cx<double> *y)
+__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *elem_blk, const int *elem_pos, const int *blk_b0,
+                                                  const int *blk_m, const int *cl_elem, const long long *boff,
+                                                  const cx<double> *minv, const cx<double> *x, cx<double> *y)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    const int c = elem_cl[i], b0 = cl_ptr[c], m = cl_ptr[c + 1] - b0, a = elem_pos[i] - b0;
-    if (m > kMaxBlock) {
-        if (lane == 0)
-#pragma unroll
-            for (int r = 0; r < NR; ++r) y[(size_t)r * n + i] = x[(size_t)r * n + i];
-        return;
-    }
-    const cx<double> *row = minv + boff[c] + (size_t)a * m;
+    const int b = elem_blk[i], b0 = blk_b0[b], m = blk_m[b], a = elem_pos[i] - b0;
+    const cx<double> *row = minv + boff[b] + (size_t)a * m;
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
-    for (int b = lane; b < m; b += 64) {
-        const cx<double> v = row[b];
-        const int j = cl_elem[b0 + b];
+    for (int j = lane; j < m; j += 64) {
+        const cx<double> v = row[j];
+        const int e = cl_elem[b0 + j];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, x[(size_t)r * n + j]);
+        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, x[(size_t)r * n + e]);
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -1465,6 +1480,9 @@ __global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, cons
 // ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
 // far-field signature of the solution at level 0 (NC_ContributionTvecFMM,
 // src/NC_PostProcessing.cpp:302-364): thread per (cluster, sphere point)
+// T is the arithmetic of the phase and weight (FP32 in the mixed form, after the
+// geometry differences are formed in FP64); sums are FP64.
+template <typename T>
 __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, int P, int ncl, const double *ccen,
                                                       const int *cl_ptr, const int *cl_elem, const double *u,
                                                       const cx<double> *pot, const cx<double> *vel, int nrhs,
@@ -1474,8 +1492,9 @@ __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, in
     if (t >= (size_t)ncl * P) return;
     const int c = (int)(t / P), p = (int)(t % P);
     const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
-    const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
+    const T su[3] = {T(u[3 * p]), T(u[3 * p + 1]), T(u[3 * p + 2])};
     const double h = bm.hk / bm.k;
+    const T hk = T(bm.hk), kk = T(bm.k);
     for (int r = 0; r < nrhs; ++r) {
         cx<double> acc = mk(0.0, 0.0);
         for (int k = cl_ptr[c]; k < cl_ptr[c + 1]; ++k) {
@@ -1489,11 +1508,11 @@ __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, in
                 double q[3], ny[3], J;
                 point_on(L, s, tt, q, ny, J);
                 const double wga = w * J;
-                const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
-                double sn, cs;
-                sincos(dw, &sn, &cs);
-                const cx<double> zg = mk(cs, sn);
-                const double dn = (ny[0] * su[0] + ny[1] * su[1] + ny[2] * su[2]) * bm.k;
+                const T dw = -(T(q[0]) * su[0] + T(q[1]) * su[1] + T(q[2]) * su[2]) * hk;
+                T sn, cs;
+                sincos_t(dw, &sn, &cs);
+                const cx<double> zg = mk(double(cs), double(sn));
+                const double dn = double((T(ny[0]) * su[0] + T(ny[1]) * su[1] + T(ny[2]) * su[2]) * kk);
                 const cx<double> zH = zg * (dn * wga);
                 const cx<double> zG = mk(-zg.im * wga * h, zg.re * wga * h);
                 acc += zH * pe - zG * ve;
@@ -1905,8 +1924,8 @@ void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const doub
     if (nnz == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for(nnz, 128);
-    if (mixed) k_fmm_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
-    else k_fmm_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
+    if (mixed) k_fmm_near<float, float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
+    else k_fmm_near<double, double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
 }
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag)
@@ -1924,30 +1943,32 @@ static void rhs_chunks(int nrhs, F &&fn)
     }
 }
 
-void bj_setup(hipStream_t s, int n, int ncl, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
-              const long long *near_ptr, const int *self_off, const void *nval, int mixed, const long long *boff,
-              void *blk, void *work, int *err)
+void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
+              const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
+              const void *nval, int mixed, const long long *boff, void *blk, int *err)
 {
     if (mixed)
-        k_bj_gather<float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, near_ptr, self_off,
-                                                            (const cx<float> *)nval, boff, (cx<double> *)blk);
+        k_bj_gather<float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr, near_ptr,
+                                                            self_off, (const cx<float> *)nval, boff, (cx<double> *)blk);
     else
-        k_bj_gather<double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, near_ptr, self_off,
-                                                             (const cx<double> *)nval, boff, (cx<double> *)blk);
-    k_bj_invert<<<ncl, 256, 0, s>>>(ncl, cl_ptr, boff, (cx<double> *)blk, (cx<double> *)work, err);
+        k_bj_gather<double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
+                                                             near_ptr, self_off, (const cx<double> *)nval, boff,
+                                                             (cx<double> *)blk);
+    k_bj_invert<<<nblk, 256, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
 }
-void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
-              const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y)
+void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_blk, const int *elem_pos, const int *blk_b0,
+              const int *blk_m, const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
         const cx<double> *M = (const cx<double> *)minv;
+        const unsigned g = blocks_for(n, 4);
         switch (nr) {
-        case 1: k_bj_apply<1><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
-        case 2: k_bj_apply<2><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
-        case 3: k_bj_apply<3><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
-        default: k_bj_apply<4><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, elem_pos, cl_ptr, cl_elem, boff, M, xx, yy); break;
+        case 1: k_bj_apply<1><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 2: k_bj_apply<2><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 3: k_bj_apply<3><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        default: k_bj_apply<4><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
         }
     });
 }
@@ -2035,12 +2056,16 @@ void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem
 }
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
-                  void *tv)
+                  void *tv, int fp32)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    k_fmm_Tvec_sol<<<blocks_for((size_t)ncl * P, 256), 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u,
-                                                                    (const cx<double> *)pot, (const cx<double> *)vel,
-                                                                    nrhs, (cx<double> *)tv);
+    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    if (fp32)
+        k_fmm_Tvec_sol<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
+                                                (const cx<double> *)vel, nrhs, (cx<double> *)tv);
+    else
+        k_fmm_Tvec_sol<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
+                                                 (const cx<double> *)vel, nrhs, (cx<double> *)tv);
 }
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
                      const int *pair_ev, const int *pair_cl, int npair, double *tab)

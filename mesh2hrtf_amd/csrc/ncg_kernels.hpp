@@ -95,13 +95,13 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
            const double *u, const double *w, void *Sv, int mixed);
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
               long long nnz, void *vals, int mixed, int *err);
-// block-Jacobi preconditioner from the finest clusters' self blocks of the near field
-// (blocks and work sized by boff: sum of m_c^2 and twice that, complex FP64)
-void bj_setup(hipStream_t s, int n, int ncl, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
-              const long long *near_ptr, const int *self_off, const void *nval, int mixed, const long long *boff,
-              void *blk, void *work, int *err);
-void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_cl, const int *elem_pos, const int *cl_ptr,
-              const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y);
+// block-Jacobi preconditioner: blocks of at most 64 consecutive elements of a finest
+// cluster, gathered from the near-field CSR and inverted in place (complex FP64)
+void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
+              const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
+              const void *nval, int mixed, const long long *boff, void *blk, int *err);
+void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_blk, const int *elem_pos, const int *blk_b0,
+              const int *blk_m, const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y);
 // v = w / ||w|| per load case, the norm read from a dots() result
 void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v);
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
@@ -116,7 +116,7 @@ void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem
                  const void *sw, void *y);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
-                  void *tv);
+                  void *tv, int fp32);
 // evaluation clusters: per far pair the direction and h_n (tab [npair][4 + 2L] doubles),
 // then z[r][ev][p] = sum over the pairs of mu * w * tv
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
