@@ -192,6 +192,68 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
  * last assembly (far + near + singular)  [7] bytes of the stored operator. */
 int ncg_timings(ncg_problem *p, double *t, int32_t reset);
 
+/* ---------------------------------------------------------------------------------
+ * Serving runtime.  No counterpart in the reference, whose parallelism is one NumCalc
+ * process per (source, frequency range) started by manage_numcalc
+ * (mesh2hrtf/NumCalc/manage_numcalc.py:376-430, RAM admission from Memory.txt at
+ * :236-290).  Here one process serves jobs (mesh, band of frequencies, load cases)
+ * on one GPU with continuous batching: `lanes` worker threads each drive their own
+ * problem handles and stream, take the next frequency step of any queued job as soon
+ * as they are free and the step's device memory fits the budget, and keep geometry,
+ * FMM trees and operators resident per mesh across frequencies.  Steps of different
+ * jobs overlap on the device.  Results do not depend on what else is running.
+ * --------------------------------------------------------------------------------- */
+typedef struct ncg_server ncg_server;
+
+typedef struct {
+    int32_t device;                     /* HIP device                                        */
+    int32_t lanes;                      /* steps in flight (threads + streams); 0 -> 4       */
+    double memory_budget_bytes;         /* device memory resident problems may hold;
+                                           0 -> 90% of the free memory at creation          */
+    int32_t problems_per_lane;          /* meshes a lane keeps resident (LRU); 0 -> 2         */
+} ncg_server_config;
+
+typedef struct {
+    int32_t mesh;                       /* id from ncg_server_add_mesh                        */
+    int32_t n_freq;
+    const ncg_frequency *freq;          /* [n_freq], equal n_rhs; copied at submit           */
+    int32_t precision;                  /* NCG_FP64 or NCG_MIXED                              */
+    ncg_solver_options solver;
+    double *solution;                   /* [n_freq][n_rhs][n] complex out, or NULL            */
+    double *phi;                        /* [n_freq][n_rhs][m] complex out, or NULL            */
+    double *grad;                       /* [n_freq][n_rhs][m][3] complex out, or NULL         */
+} ncg_job;                              /* output buffers must stay valid until the job ends */
+
+typedef struct {
+    int32_t state;                      /* 0 queued, 1 running, 2 done, 3 failed             */
+    int32_t error_code;
+    int32_t freq_done;
+    int32_t iterations;                 /* Krylov iterations summed over steps and load cases */
+    double latency_ms;                  /* submit -> last step done                          */
+    double queue_ms;                    /* submit -> first step started                      */
+    char error[256];
+} ncg_job_status;
+
+typedef struct {
+    int64_t jobs_done, jobs_failed, steps_done, solves_done;   /* solves = steps x load cases */
+    double p50_ms, p99_ms, mean_ms;     /* job latency over finished jobs                    */
+    double wall_s;                      /* since creation                                    */
+    int64_t resident_bytes, peak_resident_bytes;
+    int32_t lanes, queued_steps;
+} ncg_server_info;
+
+int ncg_server_create(const ncg_server_config *cfg, ncg_server **out);
+/* Registers a mesh (the geometry is copied); problems are created on the lanes lazily. */
+int ncg_server_add_mesh(ncg_server *s, const ncg_geometry *geometry, int32_t *mesh_id);
+int ncg_server_submit(ncg_server *s, const ncg_job *job, int64_t *job_id);
+/* Waits until the job is done or failed, at most timeout_s seconds (< 0: no limit);
+ * returns NCG_E_STATE if it is still running at the limit. */
+int ncg_server_wait(ncg_server *s, int64_t job_id, double timeout_s, ncg_job_status *status);
+int ncg_server_query(ncg_server *s, int64_t job_id, ncg_job_status *status);
+int ncg_server_stats(ncg_server *s, ncg_server_info *stats);
+/* Finishes the queued work, stops the lanes, frees every problem. */
+int ncg_server_destroy(ncg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "ncg_fmm.hpp"
+#include "ncg_internal.hpp"
 #include "ncg_kernels.hpp"
 
 #include <array>
@@ -113,15 +114,26 @@ struct Pool {
 struct Grow {
     void *p = nullptr;
     size_t cap = 0;
+    // grows by at least half of the current capacity, so a band of rising frequencies
+    // reallocates (and synchronises the device in hipFree) only a few times
     int ensure(size_t bytes)
     {
         if (bytes <= cap && p) return 0;
+        const size_t want = std::max<size_t>(std::max<size_t>(bytes, 16), cap + cap / 2);
         hipFree(p);
         p = nullptr;
         cap = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
-        if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipMalloc (FMM): ") + hipGetErrorString(e));
-        cap = std::max<size_t>(bytes, 16);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            if (want > bytes && hipMalloc(&p, std::max<size_t>(bytes, 16)) == hipSuccess) {
+                cap = std::max<size_t>(bytes, 16);
+                return 0;
+            }
+            return fail(NCG_E_MEMORY, std::string("hipMalloc (FMM): ") + hipGetErrorString(e));
+        }
+        cap = want;
         return 0;
     }
     void release()
@@ -1670,3 +1682,52 @@ int ncg_timings(ncg_problem *p, double *t, int32_t reset)
 }
 
 }  // extern "C"
+
+// ==================================================================================== internal
+namespace ncgi {
+
+int fail(int code, const std::string &msg) { return ::fail(code, msg); }
+
+void planner_mesh(const ncg_geometry *g, ncgf::Mesh &M, ncgf::Options &O)
+{
+    build_fmm_mesh(g, M);
+    O = fmm_options(g);
+}
+
+size_t estimate_bytes(const ncgf::Mesh &M, const ncgf::Options &O, int m_eval, double k, int precision, int nrhs,
+                      int restart)
+{
+    ncgf::Plan P;
+    if (!ncgf::make_plan(M, O, k, P)) return 0;
+    const double n = M.n, nr = nrhs, m = m_eval, se = precision == NCG_MIXED ? 8.0 : 16.0;
+    double b = n * 180.0 + m * 24.0;                                   // geometry
+    b += 16.0 * nr * n * (4 + 10 + std::max(restart, 2) + 1);          // vectors, work, GMRES basis
+    b += 64.0 * nr * m;                                                // field outputs
+    if (P.method == 0) {
+        b += n * n * se + n * 12.0 * 8.0;                              // dense operator, near pairs
+    } else {
+        const ncgf::Level &F = P.lev.back();
+        double nnz = 0.0;
+        for (const ncgf::Cluster &c : F.cl) {
+            double row = 0.0;
+            for (int j : c.near) row += (double)F.cl[j].elems.size();
+            nnz += row * (double)c.elems.size();
+        }
+        b += nnz * (se + 8.0) + n * 16.0;                              // near CSR
+        b += n * 64.0 * 16.0;                                          // block-Jacobi blocks (bound)
+        for (const ncgf::Level &L : P.lev) {
+            double fan = 0.0;
+            for (const ncgf::Cluster &c : L.cl) fan += (double)c.fan.size();
+            const double sf = (precision == NCG_MIXED && L.L > O.min_terms) ? 8.0 : 16.0;
+            const double ncl = (double)L.cl.size(), pts = (double)L.P;
+            b += 1.5 * ((2.0 * n + fan) * pts * sf + 3.0 * 16.0 * nr * ncl * pts);   // T, S, D, work (grow slack)
+            b += (4.0 * n + 3.0 * fan) * 4.0 + 32.0 * pts;
+        }
+        const ncgf::Level &L0 = P.lev.front();
+        const double npair = (double)P.ev.size() * (double)L0.cl.size();
+        b += 1.5 * (npair * (4.0 + 2.0 * L0.L) * 8.0 + 16.0 * nr * (double)P.ev.size() * L0.P) + 64.0 * nr * m;
+    }
+    return (size_t)(b * 1.05) + ((size_t)64 << 20);
+}
+
+}  // namespace ncgi

@@ -1,0 +1,488 @@
+// Serving runtime of libncg (ncg_server_* in include/ncg.h).
+//
+// Jobs are (mesh, band of frequencies, load cases).  Each frequency of a job is one
+// step: assemble, solve all load cases, evaluate the grid.  Steps wait in one queue in
+// arrival order; `lanes` worker threads take the first step whose device memory fits
+// the budget as soon as they are free, so steps of different jobs run side by side on
+// the device, each on the stream of its problem handle.  Problem handles live in a
+// pool shared by the lanes: a step reuses an idle problem of its mesh (geometry,
+// cluster tree, structure and grow-only operator buffers stay resident across
+// frequencies) or creates one, evicting idle problems of other meshes, least recently
+// used first, when the budget requires it.  A step's footprint is estimated from the
+// plan of the job's highest frequency (ncgi::estimate_bytes).
+//
+// The reference's counterpart is a pool of NumCalc processes started by
+// manage_numcalc (mesh2hrtf/NumCalc/manage_numcalc.py:376-430) and admitted by their
+// Memory.txt RAM estimates (:236-290).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ncg.h"
+#include "ncg_internal.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_between(clk::time_point a, clk::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// a registered mesh: owned copies of the geometry arrays and the planner mesh used
+// for footprint estimates
+struct MeshRec {
+    ncg_geometry g;
+    std::vector<int32_t> nv, kind, node, group;
+    std::vector<double> xyz, eval;
+    ncgf::Mesh pm;
+    ncgf::Options po;
+};
+
+// owned copy of one ncg_frequency
+struct FreqCopy {
+    ncg_frequency f;
+    std::vector<double> bc, adm, ps, pw;
+};
+
+struct Job {
+    int64_t id = 0;
+    int mesh = 0, precision = NCG_MIXED, nrhs = 1, n = 0, m = 0;
+    ncg_solver_options solver;
+    std::vector<FreqCopy> freq;
+    double *solution = nullptr, *phi = nullptr, *grad = nullptr;
+    size_t bytes = 0;   // device footprint of one step (highest frequency)
+    clk::time_point t_submit, t_first, t_done;
+    bool started = false;
+    int freq_done = 0, state = 0, error_code = 0;
+    long long iterations = 0;
+    std::string error;
+};
+
+struct Step {
+    Job *job;
+    int fi;
+};
+
+// a problem handle in the shared pool
+struct Res {
+    int mesh = -1;
+    ncg_problem *p = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+    unsigned long long last = 0;
+};
+
+}  // namespace
+
+struct ncg_server {
+    ncg_server_config cfg{};
+    double budget = 0.0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<std::unique_ptr<MeshRec>> meshes;
+    std::map<int64_t, std::unique_ptr<Job>> jobs;
+    std::deque<Step> queue;
+    std::vector<std::unique_ptr<Res>> pool;
+    std::vector<std::thread> lanes;
+    int64_t next_id = 1;
+    unsigned long long tick = 0;
+    bool stop = false;
+    int running = 0;
+    size_t resident = 0, peak = 0;
+    std::vector<double> latencies;
+    int64_t jobs_done = 0, jobs_failed = 0, steps_done = 0, solves_done = 0;
+    clk::time_point t0;
+};
+
+namespace {
+
+int sfail(int code, const std::string &msg) { return ncgi::fail(code, msg); }
+
+void finish_job(ncg_server *S, Job *J, int code, const std::string &msg)
+{
+    if (J->state >= 2) return;
+    J->t_done = clk::now();
+    if (code) {
+        J->state = 3;
+        J->error_code = code;
+        J->error = msg;
+        S->jobs_failed++;
+        // drop its remaining steps
+        S->queue.erase(std::remove_if(S->queue.begin(), S->queue.end(), [&](const Step &s) { return s.job == J; }),
+                       S->queue.end());
+    } else {
+        J->state = 2;
+        S->jobs_done++;
+    }
+    S->latencies.push_back(ms_between(J->t_submit, J->t_done));
+    S->cv_done.notify_all();
+}
+
+// Choice of the next step for a free lane (lock held).  Returns the queue index, the
+// pool entry to reuse (or null: create) and the idle entries to evict, or -1.
+int choose(ncg_server *S, Res **reuse, std::vector<Res *> &evict, size_t &need)
+{
+    const size_t cap = (size_t)S->cfg.lanes * (size_t)S->cfg.problems_per_lane;
+    const size_t scan = std::min<size_t>(S->queue.size(), 64);
+    for (size_t q = 0; q < scan; ++q) {
+        const Step &st = S->queue[q];
+        const int mesh = st.job->mesh;
+        Res *r = nullptr;
+        for (auto &e : S->pool)
+            if (!e->busy && e->mesh == mesh && (!r || e->last > r->last)) r = e.get();
+        need = r ? (st.job->bytes > r->bytes ? st.job->bytes - r->bytes : 0) : st.job->bytes;
+        // idle entries of other meshes, least recently used first
+        std::vector<Res *> idle;
+        for (auto &e : S->pool)
+            if (!e->busy && e.get() != r) idle.push_back(e.get());
+        std::sort(idle.begin(), idle.end(), [](const Res *a, const Res *b) { return a->last < b->last; });
+        evict.clear();
+        size_t res = S->resident;
+        size_t count = S->pool.size();
+        size_t k = 0;
+        while ((double)(res + need) > S->budget || (!r && count >= cap)) {
+            if (k == idle.size()) break;
+            res -= idle[k]->bytes;
+            evict.push_back(idle[k]);
+            --count;
+            ++k;
+        }
+        if ((double)(res + need) <= S->budget && (r || count < cap)) {
+            *reuse = r;
+            return (int)q;
+        }
+    }
+    return -1;
+}
+
+void run_lane(ncg_server *S)
+{
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        Res *r = nullptr;
+        std::vector<Res *> evict;
+        size_t need = 0;
+        int q = -1;
+        for (;;) {
+            if (S->queue.empty()) {
+                if (S->stop) return;
+                S->cv_work.wait(lk);
+                continue;
+            }
+            q = choose(S, &r, evict, need);
+            if (q >= 0) break;
+            if (S->running == 0) {
+                // nothing runs that could free memory: the head step cannot fit at all
+                Job *J = S->queue.front().job;
+                finish_job(S, J, NCG_E_MEMORY,
+                           "a step of this job needs " + std::to_string(J->bytes >> 20) + " MiB, beyond the " +
+                               std::to_string((long long)S->budget >> 20) + " MiB device-memory budget");
+                continue;
+            }
+            S->cv_work.wait(lk);
+        }
+        const Step st = S->queue[q];
+        S->queue.erase(S->queue.begin() + q);
+        Job *J = st.job;
+        // reserve memory and pool slots, unlink the evicted entries
+        for (Res *e : evict) {
+            S->resident -= e->bytes;
+            for (size_t i = 0; i < S->pool.size(); ++i)
+                if (S->pool[i].get() == e) {
+                    S->pool[i].release();
+                    S->pool.erase(S->pool.begin() + i);
+                    break;
+                }
+        }
+        if (!r) {
+            S->pool.push_back(std::unique_ptr<Res>(new Res()));
+            r = S->pool.back().get();
+            r->mesh = J->mesh;
+        }
+        r->busy = true;
+        S->resident += need;
+        r->bytes += need;
+        S->peak = std::max(S->peak, S->resident);
+        S->running++;
+        if (!J->started) {
+            J->started = true;
+            J->t_first = clk::now();
+            J->state = 1;
+        }
+        const MeshRec *M = S->meshes[J->mesh].get();
+        lk.unlock();
+
+        for (Res *e : evict) {
+            ncg_problem_destroy(e->p);
+            delete e;
+        }
+        int rc = 0;
+        std::string msg;
+        long long its = 0;
+        if (!r->p) {
+            rc = ncg_problem_create(&M->g, S->cfg.device, &r->p);
+            if (rc) msg = ncg_last_error();
+        }
+        if (!rc) {
+            rc = ncg_assemble(r->p, &J->freq[st.fi].f, J->precision);
+            if (rc) msg = ncg_last_error();
+        }
+        if (!rc) {
+            ncg_solve_stats ss;
+            rc = ncg_solve(r->p, &J->solver, &ss);
+            if (rc) msg = ncg_last_error();
+            for (int c = 0; c < J->nrhs && c < 64; ++c) its += ss.iterations[c];
+        }
+        const size_t vec = 2 * (size_t)J->nrhs;
+        if (!rc && J->solution) {
+            rc = ncg_get_solution(r->p, J->solution + vec * J->n * st.fi);
+            if (rc) msg = ncg_last_error();
+        }
+        if (!rc && J->m > 0 && (J->phi || J->grad)) {
+            std::vector<double> tmp;
+            double *ph = J->phi ? J->phi + vec * J->m * st.fi : nullptr;
+            if (!ph) {
+                tmp.resize(vec * J->m);
+                ph = tmp.data();
+            }
+            rc = ncg_field(r->p, ph, J->grad ? J->grad + 3 * vec * J->m * st.fi : nullptr);
+            if (rc) msg = ncg_last_error();
+        }
+
+        lk.lock();
+        S->running--;
+        r->busy = false;
+        r->last = ++S->tick;
+        if (rc && !r->p) {
+            // the problem could not be created: give its reservation back
+            S->resident -= r->bytes;
+            for (size_t i = 0; i < S->pool.size(); ++i)
+                if (S->pool[i].get() == r) {
+                    S->pool.erase(S->pool.begin() + i);
+                    break;
+                }
+        }
+        S->steps_done++;
+        S->solves_done += J->nrhs;
+        J->iterations += its;
+        if (rc) {
+            finish_job(S, J, rc, msg);
+        } else if (J->state < 2 && ++J->freq_done == (int)J->freq.size()) {
+            finish_job(S, J, 0, "");
+        }
+        S->cv_work.notify_all();
+    }
+}
+
+void fill_status(const Job *J, ncg_job_status *st)
+{
+    memset(st, 0, sizeof(*st));
+    st->state = J->state;
+    st->error_code = J->error_code;
+    st->freq_done = J->freq_done;
+    st->iterations = (int32_t)std::min<long long>(J->iterations, 0x7fffffff);
+    if (J->state >= 2) st->latency_ms = ms_between(J->t_submit, J->t_done);
+    if (J->started) st->queue_ms = ms_between(J->t_submit, J->t_first);
+    snprintf(st->error, sizeof(st->error), "%s", J->error.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ncg_server_create(const ncg_server_config *cfg, ncg_server **out)
+{
+    if (!out) return sfail(NCG_E_ARG, "ncg_server_create: null argument");
+    *out = nullptr;
+    ncg_server_config c{};
+    if (cfg) c = *cfg;
+    if (c.lanes <= 0) c.lanes = 4;
+    if (c.lanes > 16) return sfail(NCG_E_ARG, "ncg_server_create: at most 16 lanes");
+    if (c.problems_per_lane <= 0) c.problems_per_lane = 2;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sfail(NCG_E_DEVICE, "no HIP device available");
+    if (c.device < 0 || c.device >= ndev) return sfail(NCG_E_ARG, "device index out of range");
+    if (hipSetDevice(c.device) != hipSuccess) return sfail(NCG_E_DEVICE, "hipSetDevice failed");
+    double budget = c.memory_budget_bytes;
+    if (budget <= 0.0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return sfail(NCG_E_DEVICE, "hipMemGetInfo failed");
+        budget = 0.9 * (double)fr;
+    }
+    ncg_server *S = new ncg_server();
+    S->cfg = c;
+    S->budget = budget;
+    S->t0 = clk::now();
+    for (int i = 0; i < c.lanes; ++i) S->lanes.emplace_back(run_lane, S);
+    *out = S;
+    return 0;
+}
+
+int ncg_server_add_mesh(ncg_server *S, const ncg_geometry *g, int32_t *mesh_id)
+{
+    if (!S || !g || !mesh_id) return sfail(NCG_E_ARG, "ncg_server_add_mesh: null argument");
+    const int n = g->n_elements, m = g->n_eval > 0 ? g->n_eval : 0;
+    if (n <= 0 || !g->nodes_per_element || !g->node_xyz || !g->bc_kind || (m > 0 && !g->eval_xyz))
+        return sfail(NCG_E_ARG, "ncg_server_add_mesh: missing geometry");
+    std::unique_ptr<MeshRec> M(new MeshRec());
+    M->nv.assign(g->nodes_per_element, g->nodes_per_element + n);
+    M->kind.assign(g->bc_kind, g->bc_kind + n);
+    M->xyz.assign(g->node_xyz, g->node_xyz + 12 * (size_t)n);
+    if (m) M->eval.assign(g->eval_xyz, g->eval_xyz + 3 * (size_t)m);
+    if (g->node_index) M->node.assign(g->node_index, g->node_index + 4 * (size_t)n);
+    if (g->element_group) M->group.assign(g->element_group, g->element_group + n);
+    M->g = *g;
+    M->g.nodes_per_element = M->nv.data();
+    M->g.bc_kind = M->kind.data();
+    M->g.node_xyz = M->xyz.data();
+    M->g.eval_xyz = m ? M->eval.data() : nullptr;
+    M->g.node_index = g->node_index ? M->node.data() : nullptr;
+    M->g.element_group = g->element_group ? M->group.data() : nullptr;
+    ncgi::planner_mesh(&M->g, M->pm, M->po);
+    std::lock_guard<std::mutex> lk(S->mu);
+    *mesh_id = (int32_t)S->meshes.size();
+    S->meshes.push_back(std::move(M));
+    return 0;
+}
+
+int ncg_server_submit(ncg_server *S, const ncg_job *job, int64_t *job_id)
+{
+    if (!S || !job || !job_id) return sfail(NCG_E_ARG, "ncg_server_submit: null argument");
+    if (job->n_freq <= 0 || !job->freq) return sfail(NCG_E_ARG, "ncg_server_submit: no frequencies");
+    if (job->precision != NCG_FP64 && job->precision != NCG_MIXED)
+        return sfail(NCG_E_ARG, "ncg_server_submit: precision must be NCG_FP64 or NCG_MIXED");
+    const MeshRec *M;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (job->mesh < 0 || job->mesh >= (int)S->meshes.size()) return sfail(NCG_E_ARG, "ncg_server_submit: unknown mesh");
+        M = S->meshes[job->mesh].get();
+    }
+    const int n = M->g.n_elements, m = M->g.n_eval > 0 ? M->g.n_eval : 0;
+    const int nr = job->freq[0].n_rhs;
+    if (nr <= 0 || nr > 64) return sfail(NCG_E_ARG, "ncg_server_submit: n_rhs must be 1..64");
+    std::unique_ptr<Job> J(new Job());
+    J->mesh = job->mesh;
+    J->precision = job->precision;
+    J->solver = job->solver;
+    J->nrhs = nr;
+    J->n = n;
+    J->m = m;
+    J->solution = job->solution;
+    J->phi = job->phi;
+    J->grad = job->grad;
+    double kmax = 0.0;
+    J->freq.resize(job->n_freq);
+    for (int i = 0; i < job->n_freq; ++i) {
+        const ncg_frequency &f = job->freq[i];
+        if (f.n_rhs != nr) return sfail(NCG_E_ARG, "ncg_server_submit: every frequency needs the same n_rhs");
+        if (!(f.k > 0.0)) return sfail(NCG_E_ARG, "ncg_server_submit: wave numbers must be positive");
+        FreqCopy &c = J->freq[i];
+        c.f = f;
+        if (f.bc_value) c.bc.assign(f.bc_value, f.bc_value + 2 * (size_t)nr * n);
+        if (f.admittance) c.adm.assign(f.admittance, f.admittance + 2 * (size_t)n);
+        if (f.n_point_sources > 0) c.ps.assign(f.point_sources, f.point_sources + 6 * (size_t)f.n_point_sources);
+        if (f.n_plane_waves > 0) c.pw.assign(f.plane_waves, f.plane_waves + 6 * (size_t)f.n_plane_waves);
+        c.f.bc_value = f.bc_value ? c.bc.data() : nullptr;
+        c.f.admittance = f.admittance ? c.adm.data() : nullptr;
+        c.f.point_sources = f.n_point_sources > 0 ? c.ps.data() : nullptr;
+        c.f.plane_waves = f.n_plane_waves > 0 ? c.pw.data() : nullptr;
+        kmax = std::max(kmax, f.k);
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    J->bytes = ncgi::estimate_bytes(M->pm, M->po, m, kmax, job->precision, nr, job->solver.restart > 0 ? job->solver.restart : 50);
+    if (J->bytes == 0) return sfail(NCG_E_ARG, "ncg_server_submit: no FMM plan at the highest frequency");
+    J->id = S->next_id++;
+    J->t_submit = clk::now();
+    Job *jp = J.get();
+    S->jobs[J->id] = std::move(J);
+    for (int i = 0; i < job->n_freq; ++i) S->queue.push_back(Step{jp, i});
+    *job_id = jp->id;
+    S->cv_work.notify_all();
+    return 0;
+}
+
+int ncg_server_query(ncg_server *S, int64_t job_id, ncg_job_status *st)
+{
+    if (!S || !st) return sfail(NCG_E_ARG, "ncg_server_query: null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return sfail(NCG_E_ARG, "ncg_server_query: unknown job");
+    fill_status(it->second.get(), st);
+    return 0;
+}
+
+int ncg_server_wait(ncg_server *S, int64_t job_id, double timeout_s, ncg_job_status *st)
+{
+    if (!S) return sfail(NCG_E_ARG, "ncg_server_wait: null argument");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return sfail(NCG_E_ARG, "ncg_server_wait: unknown job");
+    Job *J = it->second.get();
+    auto done = [&] { return J->state >= 2; };
+    if (timeout_s < 0) S->cv_done.wait(lk, done);
+    else S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+    if (st) fill_status(J, st);
+    if (!done()) return sfail(NCG_E_STATE, "ncg_server_wait: job still running");
+    return J->state == 3 ? sfail(J->error_code, J->error) : 0;
+}
+
+int ncg_server_stats(ncg_server *S, ncg_server_info *st)
+{
+    if (!S || !st) return sfail(NCG_E_ARG, "ncg_server_stats: null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    memset(st, 0, sizeof(*st));
+    st->jobs_done = S->jobs_done;
+    st->jobs_failed = S->jobs_failed;
+    st->steps_done = S->steps_done;
+    st->solves_done = S->solves_done;
+    std::vector<double> l = S->latencies;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double q) {
+            const double pos = q * (double)(l.size() - 1);
+            const size_t i = (size_t)pos;
+            const double f = pos - (double)i;
+            return i + 1 < l.size() ? l[i] * (1.0 - f) + l[i + 1] * f : l[i];
+        };
+        st->p50_ms = pct(0.5);
+        st->p99_ms = pct(0.99);
+        double s = 0.0;
+        for (double v : l) s += v;
+        st->mean_ms = s / (double)l.size();
+    }
+    st->wall_s = ms_between(S->t0, clk::now()) / 1000.0;
+    st->resident_bytes = (int64_t)S->resident;
+    st->peak_resident_bytes = (int64_t)S->peak;
+    st->lanes = S->cfg.lanes;
+    st->queued_steps = (int32_t)S->queue.size();
+    return 0;
+}
+
+int ncg_server_destroy(ncg_server *S)
+{
+    if (!S) return 0;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv_work.notify_all();
+    }
+    for (std::thread &t : S->lanes) t.join();
+    for (auto &e : S->pool)
+        if (e->p) ncg_problem_destroy(e->p);
+    delete S;
+    return 0;
+}
+
+}  // extern "C"

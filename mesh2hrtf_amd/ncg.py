@@ -87,7 +87,32 @@ class _Stats(ctypes.Structure):
 # every entry point include/ncg.h declares
 EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_create", "ncg_problem_destroy",
            "ncg_assemble", "ncg_solve", "ncg_problem_fmm_info", "ncg_get_solution", "ncg_field", "ncg_get_system",
-           "ncg_apply", "ncg_timings")
+           "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
+           "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy")
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("memory_budget_bytes", ctypes.c_double),
+                ("problems_per_lane", ctypes.c_int32)]
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_int32), ("n_freq", ctypes.c_int32), ("freq", ctypes.c_void_p),
+                ("precision", ctypes.c_int32), ("solver", _Options), ("solution", ctypes.c_void_p),
+                ("phi", ctypes.c_void_p), ("grad", ctypes.c_void_p)]
+
+
+class _JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("error_code", ctypes.c_int32), ("freq_done", ctypes.c_int32),
+                ("iterations", ctypes.c_int32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("error", ctypes.c_char * 256)]
+
+
+class _ServerInfo(ctypes.Structure):
+    _fields_ = [("jobs_done", ctypes.c_int64), ("jobs_failed", ctypes.c_int64), ("steps_done", ctypes.c_int64),
+                ("solves_done", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("resident_bytes", ctypes.c_int64),
+                ("peak_resident_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("queued_steps", ctypes.c_int32)]
 
 _lib = None
 
@@ -241,3 +266,100 @@ class Problem:
         keys = ("ms_assemble", "ms_solve", "ms_field", "far_kernel_ms", "matvec_kernel_ms", "matvec_calls",
                 "far_evaluations", "operator_bytes")
         return dict(zip(keys, list(t)))
+
+
+def _status_dict(st):
+    return dict(state=("queued", "running", "done", "failed")[st.state], error_code=st.error_code,
+                freq_done=st.freq_done, iterations=st.iterations, latency_ms=st.latency_ms, queue_ms=st.queue_ms,
+                error=st.error.decode())
+
+
+class ServerJob:
+    """A submitted job: its id and the output arrays the server writes into."""
+
+    def __init__(self, server, job_id, solution, phi, grad):
+        self.server, self.id = server, job_id
+        self.solution, self.phi, self.grad = solution, phi, grad
+
+    def wait(self, timeout=-1.0):
+        return self.server.wait(self, timeout)
+
+
+class Server:
+    """Serving runtime (ncg_server_*): jobs of (mesh, frequencies, load cases) on one GPU
+    with continuous batching over `lanes` concurrent steps and resident problems."""
+
+    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2):
+        self.lib = load()
+        self._h = ctypes.c_void_p()
+        cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane))
+        _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        self._meshes = {}
+
+    def add_mesh(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, node_index=None, group=None, method_bem=4,
+                 min_terms=0, cluster_edge=0.0):
+        ga = _GeometryArrays(nv, xyz, kind, eval_xyz, h, interior, node_index, group, method_bem, min_terms,
+                             cluster_edge)
+        mid = ctypes.c_int32()
+        _check(self.lib.ncg_server_add_mesh(self._h, ctypes.byref(ga.g), ctypes.byref(mid)))
+        self._meshes[mid.value] = (ga.n, ga.m)
+        return mid.value
+
+    def submit(self, mesh, freqs, precision=MIXED, solver=GMRES, precond=BLOCK, max_iterations=500, tolerance=1e-9,
+               restart=50, refinement_steps=8, refinement_tolerance=1e-5, ilu_scanning_degree=0, solution=True,
+               field=True, gradient=False):
+        """freqs: list of dicts with k and optionally bc ((nrhs, n) complex), adm (n complex),
+        point_sources / plane_waves (rows x y z re im rhs), nrhs."""
+        n, m = self._meshes[mesh]
+        keep = []
+        arr = (_Frequency * len(freqs))()
+        nrhs = None
+        for i, f in enumerate(freqs):
+            nr = int(f.get("nrhs", 1 if f.get("bc") is None else np.atleast_2d(f["bc"]).shape[0]))
+            nrhs = nr if nrhs is None else nrhs
+            bc = f.get("bc")
+            bc = None if bc is None else np.ascontiguousarray(np.atleast_2d(bc), np.complex128)
+            adm = f.get("adm")
+            adm = None if adm is None else np.ascontiguousarray(adm, np.complex128)
+            ps = np.ascontiguousarray(np.asarray(f.get("point_sources", ()), np.float64).reshape(-1, 6))
+            pw = np.ascontiguousarray(np.asarray(f.get("plane_waves", ()), np.float64).reshape(-1, 6))
+            keep += [bc, adm, ps, pw]
+            arr[i] = _Frequency(float(f["k"]), nr, _ptr(bc), _ptr(adm), len(ps), _ptr(ps), len(pw), _ptr(pw))
+        nf = len(freqs)
+        x = np.zeros((nf, nrhs, n), np.complex128) if solution else None
+        phi = np.zeros((nf, nrhs, m), np.complex128) if field and m else None
+        grad = np.zeros((nf, nrhs, m, 3), np.complex128) if gradient and m else None
+        o = _Options(solver, precond, max_iterations, tolerance, restart, refinement_steps, refinement_tolerance,
+                     ilu_scanning_degree)
+        job = _Job(int(mesh), nf, ctypes.cast(arr, ctypes.c_void_p), int(precision), o, _ptr(x), _ptr(phi), _ptr(grad))
+        jid = ctypes.c_int64()
+        _check(self.lib.ncg_server_submit(self._h, ctypes.byref(job), ctypes.byref(jid)))
+        return ServerJob(self, jid.value, x, phi, grad)
+
+    def wait(self, job, timeout=-1.0):
+        st = _JobStatus()
+        rc = self.lib.ncg_server_wait(self._h, ctypes.c_int64(job.id), ctypes.c_double(timeout), ctypes.byref(st))
+        if rc not in (0, -5) and st.state != 3:   # a failed job reports through its status
+            _check(rc)
+        return _status_dict(st)
+
+    def query(self, job):
+        st = _JobStatus()
+        _check(self.lib.ncg_server_query(self._h, ctypes.c_int64(job.id), ctypes.byref(st)))
+        return _status_dict(st)
+
+    def stats(self):
+        s = _ServerInfo()
+        _check(self.lib.ncg_server_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _ServerInfo._fields_}
+
+    def close(self):
+        if self._h:
+            self.lib.ncg_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
