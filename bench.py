@@ -1,23 +1,29 @@
-"""Benchmark: frequency-solves/s of the Burton–Miller BEM solve on one MI355X.
+"""Benchmark: frequency-solves/s of the Burton-Miller BEM solve on one MI355X.
 
 Workload (the configuration the metric is quoted on): seeded head-like mesh, ellipsoid
 0.075/0.09/0.11 m radially perturbed by a real spherical-harmonic sum (degree <= 8, RMS
 3 mm, seed 0), geodesic frequency 32 -> 20480 triangles; two reciprocal point sources
 1 mm off the surface at the +-y ear positions; 1730-point Fibonacci evaluation grid at
-1.2 m; frequencies 200..16000 Hz step 200 Hz.  A step is one frequency: dense operator
-assembly, the Krylov solve of both sources (they share the operator) and the field at the
-evaluation grid.  Steps walk the 80 frequencies with an even stride.
+1.2 m; frequencies 200..16000 Hz step 200 Hz, method as NumCalc chooses it for
+methodBEM 4 (SL-FMM below lambda/D = 1, two-level ML-FMM above).  A step is one frequency:
+operator assembly, the solve of both sources and the field at the evaluation grid.
 
-value      = frequency-solves/s of the mixed-precision form (FP32-complex bulk quadrature,
-             operator and inner matvecs; FP64 singular/near integrals, Krylov dots and the
-             refinement residual), counted only if every surface and evaluation-grid
-             pressure is within rel-L2 1e-3 of the FP64 plain method on the same frequency.
-plain      = the same steps with the FP64 plain method (the GPU drop-in).
-roofline   = the dominant kernel (the operator matvec, HBM bound): algorithmic bytes per
-             launch (operator once + B vectors in and out) / mean launch time (HIP events).
+value        = frequency-solves/s (one solve = one frequency x one source) of the served
+               mixed-precision form: the K steps are submitted at once to the serving
+               runtime (ncg.Server: `lanes` steps in flight on their own streams, problems
+               resident across frequencies), FP32-complex bulk operator with FP64
+               near/singular integrals, FP64 Krylov dots and refinement residual, GMRES with
+               the block-Jacobi preconditioner.  Only solves whose surface and grid pressures
+               are within rel-L2 1e-3 of the FP64 plain method count.
+plain_fp64   = the FP64 plain method, one step after the other as the drop-in NumCalc runs
+               them: NumCalc's method choice, CGS with its ILU preconditioner (IPRECOND 0,
+               what mesh2input writes), tolerance 1e-9.
+p50/p99      = per-request latency of single-frequency requests (both sources) arriving as a
+               Poisson stream at 70% of the measured throughput.
+roofline     = the dominant kernel of the served run (HIP events around each of its launches
+               on the problem streams): algorithmic bytes per launch / mean launch time.
 cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) on one
-             frequency of one source of the same mesh, ML-FMM as the configuration asks,
-             on one host core.
+               frequency of one source of the same mesh, on one host core.
 """
 import argparse
 import json
@@ -38,33 +44,15 @@ from mesh2hrtf_amd import ncg, project as P  # noqa: E402
 METRIC = "frequency-solves/sec at rel-L2≤1e-3 vs FP64 oracle; p50/p99 solve latency"
 C, RHO = 343.0, 1.1839
 HBM_PEAK = 8.0e12
+SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50, max_iterations=500,
+              refinement_steps=8, refinement_tolerance=1e-5)
 
 
-def setup(freq_nu):
-    mesh = P.config2_mesh(freq_nu)
-    grid = P.fibonacci_grid(1730, 1.2)
-    src = P.ear_sources(mesh)
-    n = len(mesh.elems)
-    xyz = mesh.nodes[mesh.elems]
-    prob = ncg.Problem(np.full(n, 3), xyz, np.zeros(n, np.int32), grid.nodes, device=0)
-    return mesh, grid, src, prob
-
-
-def step(prob, src, f, precision):
+def freq_spec(src, f):
     omega = 2 * np.pi * f
-    k = omega / C
-    rp = RHO * omega
-    amp = 1.0 / (1j * rp)
+    amp = 1.0 / (1j * RHO * omega)
     ps = [[s[0], s[1], s[2], amp.real, amp.imag, r] for r, s in enumerate(src)]
-    prob.assemble(k, nrhs=len(src), point_sources=ps, precision=precision)
-    if precision == ncg.FP64:
-        st = prob.solve(solver=ncg.CGS, precond=ncg.NONE, tolerance=1e-9)
-    else:
-        st = prob.solve(solver=ncg.GMRES, restart=50, refinement_steps=8, refinement_tolerance=1e-7,
-                        max_iterations=500)
-    x = prob.solution()
-    phi, _ = prob.field(gradient=False)
-    return x, phi, st
+    return dict(k=omega / C, nrhs=len(src), point_sources=ps)
 
 
 def cpu_baseline(mesh, grid, src, f, timeout):
@@ -82,20 +70,32 @@ def cpu_baseline(mesh, grid, src, f, timeout):
         if r.returncode != 0:
             return None
         return dict(value=1.0 / wall, unit="frequency-solves/s", cores=1, kind="reference",
-                    sample="reference NumCalc (ML-FMM, method 4) on the same 20480-element mesh, one source, "
-                           "one frequency (%.0f Hz) incl. the 1730-point grid: %.1f s wall on one core" % (f, wall))
+                    sample="reference NumCalc (ML-FMM, methodBEM 4, ILU-CGS) on the same 20480-element mesh, one "
+                           "source, one frequency (%.0f Hz) incl. the 1730-point grid: %.1f s wall on one core"
+                           % (f, wall))
     except subprocess.TimeoutExpired:
         return None
     finally:
         shutil.rmtree(d, ignore_errors=True)
 
 
+def pick(freqs, count, offset=0):
+    """count frequencies spread evenly over the sweep (all of them, cyclically, if count >= len)."""
+    if count >= len(freqs):
+        return [freqs[(offset + i) % len(freqs)] for i in range(count)]
+    stride = len(freqs) / count
+    return [freqs[int(offset + i * stride) % len(freqs)] for i in range(count)]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=80, help="frequencies in the timed region (80 = the sweep)")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--nu", type=int, default=32, help="geodesic frequency of the mesh (32 -> 20480 triangles)")
+    ap.add_argument("--plain-steps", type=int, default=-1, help="frequencies the FP64 plain method runs (-1: all)")
+    ap.add_argument("--poisson", type=int, default=-1, help="requests of the latency run (-1: steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     a = ap.parse_args()
@@ -103,95 +103,135 @@ def main():
         print("bench.py measures one GPU: the solve shards by independent (frequency, source) jobs with no "
               "communication, so only --gpus 1 is run here", file=sys.stderr)
         sys.exit(2)
+
     freqs = P.config2_freqs()
-    t_setup = time.perf_counter()
-    mesh, grid, src, prob = setup(a.nu)
-    t_setup = time.perf_counter() - t_setup
-    n = len(mesh.elems)
-    total = a.warmup + a.steps
-    stride = max(1, len(freqs) // total)
-    idx = [(i * stride + stride // 2) % len(freqs) for i in range(total)]
-    warm, timed = idx[:a.warmup], idx[a.warmup:]
-    print("# n=%d eval=%d sources=%d setup %.2fs; timed frequencies %s" % (
-        n, len(grid.nodes), len(src), t_setup, [freqs[i] for i in timed]), file=sys.stderr, flush=True)
+    mesh = P.config2_mesh(a.nu)
+    grid = P.fibonacci_grid(1730, 1.2)
+    src = P.ear_sources(mesh)
+    n, m, nr = len(mesh.elems), len(grid.nodes), len(src)
+    geo = dict(nv=np.full(n, 3), xyz=mesh.nodes[mesh.elems], kind=np.zeros(n, np.int32), eval_xyz=grid.nodes,
+               node_index=mesh.elems, method_bem=4)
+    timed = pick(freqs, a.steps)
+    print("# n=%d eval=%d sources=%d lanes=%d; %d timed frequencies" % (n, m, nr, a.lanes, len(timed)),
+          file=sys.stderr, flush=True)
 
-    out = {}
-    for name, prec in (("mixed", ncg.MIXED), ("fp64", ncg.FP64)):
-        for i in warm:
-            step(prob, src, freqs[i], prec)
-        prob.timings(reset=True)
-        lat, res, mv_ms, far_ms, asm_ms, sol_ms, fld_ms = [], [], [], [], [], [], []
-        t0 = time.perf_counter()
-        for i in timed:
-            ts = time.perf_counter()
-            x, phi, st = step(prob, src, freqs[i], prec)
-            lat.append(time.perf_counter() - ts)
-            tm = prob.timings()
-            asm_ms.append(tm["ms_assemble"])
-            sol_ms.append(tm["ms_solve"])
-            fld_ms.append(tm["ms_field"])
-            res.append((x, phi, st["iterations"]))
-            print("# %s f=%5.0f  %.1f ms  (assemble %.1f, solve %.1f [%s it], field %.1f)" % (
-                name, freqs[i], lat[-1] * 1e3, tm["ms_assemble"], tm["ms_solve"], st["iterations"], tm["ms_field"]),
-                file=sys.stderr, flush=True)
-        elapsed = time.perf_counter() - t0
-        tm = prob.timings()
-        out[name] = dict(elapsed=elapsed, lat=lat, res=res, matvec_ms=tm["matvec_kernel_ms"],
-                         matvec_calls=tm["matvec_calls"], far_ms=tm["far_kernel_ms"],
-                         far_evals=tm["far_evaluations"], asm=np.mean(asm_ms), sol=np.mean(sol_ms),
-                         fld=np.mean(fld_ms))
+    # ---- served mixed form: all K steps submitted at once -------------------------------
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True)
+    mid = srv.add_mesh(**geo)
+    warm = [srv.submit(mid, [freq_spec(src, f)], **SERVED) for f in pick(freqs, a.warmup, offset=1)]
+    for j in warm:
+        j.wait()
+    srv.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    jobs = [srv.submit(mid, [freq_spec(src, f)], **SERVED) for f in timed]
+    sts = [j.wait() for j in jobs]
+    elapsed = time.perf_counter() - t0
+    kst = srv.kernel_stats()
+    failed = [s for s in sts if s["state"] != "done"]
+    if failed:
+        print("# served steps failed: %s" % failed[:3], file=sys.stderr, flush=True)
+    batch_lat = [s["latency_ms"] for s in sts]
+    print("# served: %d steps in %.2f s (%.1f ms/step), batch latency p50 %.0f ms p99 %.0f ms" % (
+        len(timed), elapsed, elapsed / len(timed) * 1e3, np.percentile(batch_lat, 50), np.percentile(batch_lat, 99)),
+        file=sys.stderr, flush=True)
 
-    # accuracy of the mixed form against the FP64 plain method, per frequency
-    err_s, err_e = [], []
-    for (x1, p1, _), (x0, p0, _) in zip(out["mixed"]["res"], out["fp64"]["res"]):
-        err_s.append(max(np.linalg.norm(x1[r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(len(src))))
-        err_e.append(max(np.linalg.norm(p1[r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(len(src))))
-    ok = max(err_s + err_e) <= 1e-3
-    chosen = "mixed" if ok else "fp64"
-    o = out[chosen]
-    solves = a.steps * len(src)
-    value = solves / o["elapsed"]
-    nrhs = len(src)
-    if chosen == "mixed":
-        bytes_launch = n * n * 8 + 2 * nrhs * n * 16
-    else:
-        bytes_launch = n * n * 16 + 2 * nrhs * n * 16
-    achieved = bytes_launch / (o["matvec_ms"] * 1e-3) if o["matvec_ms"] > 0 else 0.0
+    # ---- latency: Poisson arrivals at 70% of the measured step rate ---------------------
+    npois = a.steps if a.poisson < 0 else a.poisson
+    rate = 0.7 * len(timed) / elapsed
+    rng = np.random.default_rng(1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, npois))
+    pfreqs = [freqs[i] for i in rng.integers(0, len(freqs), npois)]
+    pjobs = []
+    t0p = time.perf_counter()
+    for t_arr, f in zip(arrivals, pfreqs):
+        dt = t_arr - (time.perf_counter() - t0p)
+        if dt > 0:
+            time.sleep(dt)
+        pjobs.append(srv.submit(mid, [freq_spec(src, f)], **SERVED))
+    pl = [j.wait()["latency_ms"] for j in pjobs]
+    print("# poisson %.1f req/s: latency p50 %.1f ms p99 %.1f ms" % (rate, np.percentile(pl, 50), np.percentile(pl, 99)),
+          file=sys.stderr, flush=True)
+    info = srv.stats()
+    srv.close()
+
+    # ---- FP64 plain method (the drop-in), one step after the other ----------------------
+    nplain = len(timed) if a.plain_steps < 0 else min(a.plain_steps, len(timed))
+    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
+                       method_bem=4)
+    sp = freq_spec(src, timed[0])
+    prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
+    prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+    err_s, err_e, plain_lat = [], [], []
+    t0 = time.perf_counter()
+    for i in range(nplain):
+        f = timed[i]
+        ts = time.perf_counter()
+        sp = freq_spec(src, f)
+        prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
+        prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+        x0 = prob.solution()
+        p0, _ = prob.field(gradient=False)
+        plain_lat.append((time.perf_counter() - ts) * 1e3)
+        x1, p1 = jobs[i].solution[0], jobs[i].phi[0]
+        err_s.append(max(np.linalg.norm(x1[r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(nr)))
+        err_e.append(max(np.linalg.norm(p1[r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(nr)))
+        print("# plain f=%5.0f %.1f ms  mixed rel-L2 surface %.1e grid %.1e" % (f, plain_lat[-1], err_s[-1], err_e[-1]),
+              file=sys.stderr, flush=True)
+    plain_elapsed = time.perf_counter() - t0
+    prob.close()
+
+    ok = [s["state"] == "done" and (i >= nplain or (err_s[i] <= 1e-3 and err_e[i] <= 1e-3))
+          for i, s in enumerate(sts)]
+    valid = int(sum(bool(v) for v in ok)) * nr
+    value = valid / elapsed
+
+    # roofline of the dominant kernel of the served run
+    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
+    roof = None
+    if top:
+        s = kst[top]
+        achieved = s["bytes"] / s["calls"] / (s["total_ms"] / s["calls"] * 1e-3)
+        roof = {"bound": "hbm", "kernel": top, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": s["bytes"] / s["calls"],
+                "mean_launch_ms": s["total_ms"] / s["calls"], "launches": s["calls"],
+                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
     cpu = None if a.no_cpu else cpu_baseline(mesh, grid, src, 8000.0, a.cpu_timeout)
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "frequency-solves/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": len(timed),
         "warmup": a.warmup,
-        "ms_per_step": o["elapsed"] / a.steps * 1e3,
+        "ms_per_step": elapsed / len(timed) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if chosen == "mixed" else "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "head-like mesh %d triangles (seed 0), 2 ear point sources, 1730-point grid at 1.2 m, "
-                               "frequencies 200-16000 Hz; dense Burton-Miller BEM" % n,
-                   "form": chosen, "frequencies": [freqs[i] for i in idx[a.warmup:]]},
-        "p50_ms": float(np.percentile(o["lat"], 50) * 1e3),
-        "p99_ms": float(np.percentile(o["lat"], 99) * 1e3),
-        "plain_fp64": {"value": solves / out["fp64"]["elapsed"],
-                       "ms_per_step": out["fp64"]["elapsed"] / a.steps * 1e3,
-                       "p50_ms": float(np.percentile(out["fp64"]["lat"], 50) * 1e3),
-                       "p99_ms": float(np.percentile(out["fp64"]["lat"], 99) * 1e3)},
-        "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)), "eval_grid_max": float(max(err_e))},
-        "phase_ms": {k: {"assemble": float(out[k]["asm"]), "solve": float(out[k]["sol"]), "field": float(out[k]["fld"])}
-                     for k in out},
-        "roofline": {"bound": "hbm", "kernel": "k_matvec (%s operator, %d vectors)" % (
-                         "FP32" if chosen == "mixed" else "FP64", nrhs),
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_launch": bytes_launch, "mean_launch_ms": o["matvec_ms"],
-                     "launches": o["matvec_calls"]},
+                               "frequencies 200-16000 Hz step 200; NumCalc methodBEM 4 (SL/ML-FMM)" % n,
+                   "form": "served mixed precision, %d lanes, GMRES + block-Jacobi" % a.lanes,
+                   "frequencies": len(timed)},
+        "p50_ms": float(np.percentile(pl, 50)),
+        "p99_ms": float(np.percentile(pl, 99)),
+        "latency": {"poisson_rate_per_s": rate, "requests": npois,
+                    "batch_p50_ms": float(np.percentile(batch_lat, 50)),
+                    "batch_p99_ms": float(np.percentile(batch_lat, 99))},
+        "valid_solves": valid,
+        "solves": len(timed) * nr,
+        "plain_fp64": {"value": nplain * nr / plain_elapsed, "ms_per_step": plain_elapsed / max(nplain, 1) * 1e3,
+                       "p50_ms": float(np.percentile(plain_lat, 50)) if plain_lat else None,
+                       "p99_ms": float(np.percentile(plain_lat, 99)) if plain_lat else None,
+                       "form": "NumCalc's method choice, CGS + ILU, tol 1e-9, one step at a time"},
+        "speedup_vs_plain": value / (nplain * nr / plain_elapsed) if nplain else None,
+        "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
+                                 "eval_grid_max": float(max(err_e)) if err_e else None},
+        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done")},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
 
 
 if __name__ == "__main__":

@@ -192,6 +192,20 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
  * last assembly (far + near + singular)  [7] bytes of the stored operator. */
 int ncg_timings(ncg_problem *p, double *t, int32_t reset);
 
+/* Per-kernel timing of the operator apply (near-field, T, D, S or dense matvec
+ * launches): HIP events around every launch on the problem's stream while enabled, and
+ * the algorithmic bytes each launch must move (stored operator once per 4 load cases,
+ * vectors in and out).  No counterpart in the reference; for measurement.  Stats are
+ * accumulated since the last reset; count receives the number of kernels with launches. */
+typedef struct {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double bytes;
+} ncg_kernel_stat;
+int ncg_set_kernel_timing(ncg_problem *p, int32_t on);
+int ncg_kernel_stats(ncg_problem *p, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset);
+
 /* ---------------------------------------------------------------------------------
  * Serving runtime.  No counterpart in the reference, whose parallelism is one NumCalc
  * process per (source, frequency range) started by manage_numcalc
@@ -211,6 +225,7 @@ typedef struct {
     double memory_budget_bytes;         /* device memory resident problems may hold;
                                            0 -> 90% of the free memory at creation          */
     int32_t problems_per_lane;          /* meshes a lane keeps resident (LRU); 0 -> 2         */
+    int32_t kernel_timing;              /* 1: per-kernel timing of every step (ncg_kernel_stats) */
 } ncg_server_config;
 
 typedef struct {
@@ -251,6 +266,9 @@ int ncg_server_submit(ncg_server *s, const ncg_job *job, int64_t *job_id);
 int ncg_server_wait(ncg_server *s, int64_t job_id, double timeout_s, ncg_job_status *status);
 int ncg_server_query(ncg_server *s, int64_t job_id, ncg_job_status *status);
 int ncg_server_stats(ncg_server *s, ncg_server_info *stats);
+/* Per-kernel timing summed over the steps finished since the last reset (requires
+ * kernel_timing in the configuration). */
+int ncg_server_kernel_stats(ncg_server *s, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset);
 /* Finishes the queued work, stops the lanes, frees every problem. */
 int ncg_server_destroy(ncg_server *s);
 

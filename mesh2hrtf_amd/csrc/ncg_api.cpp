@@ -155,6 +155,7 @@ struct DevLevel {
     int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
     int P = 0, L = 0;
     int f32 = 0;                      // T, D, S stored in FP32 (mixed form, regular levels only)
+    double dmin = 1e300;              // shortest centre distance of an interaction pair
     Grow u, w, T, D, S, tw, sw, tv;   // tw, sw, tv: [nrhs][ncl*P]
     void release()
     {
@@ -173,6 +174,7 @@ struct DevFmm {
     int *elem_cl_fine = nullptr;
     unsigned char *cl_near = nullptr;
     int ncl_fine = 0;
+    long long *adj_pos = nullptr;   // CSR position of each subdivided pair (or -1)
     Grow near_val, xs;            // xs: scaled copy of the input vector
     // block-Jacobi preconditioner: block layout (structure) and inverses (values)
     int *self_off = nullptr, *elem_blk = nullptr, *blk_cl = nullptr, *blk_b0 = nullptr, *blk_m = nullptr;
@@ -226,6 +228,13 @@ struct Fnv {
     }
 };
 
+// one timed launch: kernel id, event pair slot, algorithmic bytes
+struct KernelRec {
+    int id;
+    size_t slot;
+    double bytes;
+};
+
 // device state of the incomplete LU preconditioner (pattern CSR and level schedules)
 struct DevIlu {
     int nnz = 0, nlevL = 0, nlevU = 0;
@@ -262,6 +271,7 @@ struct ncg_problem {
     double *X = nullptr, *cen = nullptr, *nrm = nullptr, *area = nullptr, *jac = nullptr, *eval = nullptr;
     int8_t *nv = nullptr, *kind = nullptr;
     std::vector<int8_t> kind_h;
+    int all_velocity = 0;   // every element has a velocity boundary condition
     long long nnear = 0;
     int *near_rows = nullptr, *near_cols = nullptr;
     // frequency data
@@ -283,6 +293,7 @@ struct ncg_problem {
     std::vector<double *> work;
     size_t work_cap = 0;
     double *scal = nullptr, *partial = nullptr, *dotout = nullptr, *dvec = nullptr;
+    size_t partial_cap = 0;   // complex entries of partial
     int *active = nullptr;
     double *basis = nullptr;
     size_t basis_cap = 0;
@@ -302,6 +313,12 @@ struct ncg_problem {
     size_t nrhs_fmm = 0;
     DevIlu ilu;
     Grow out_phi, out_grad;   // evaluation-grid results before the copy to the host
+    // per-kernel timing of the operator apply (off unless requested)
+    int ktime = 0;
+    std::vector<EventPair> kpool;
+    std::vector<KernelRec> kpend;
+    double k_ms[16] = {0}, k_bytes[16] = {0};
+    long long k_calls[16] = {0};
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -358,6 +375,23 @@ double elapsed(const EventPair &ev)
 }
 
 // matvec with per-call timing (the Krylov loops' dominant kernel)
+// Per-kernel timing of the operator apply (enabled by ncg_set_kernel_timing): HIP events
+// around each launch on the problem's stream, with the algorithmic bytes of the launch.
+const char *const kKernelNames[] = {"k_near_spmv<f64>",   "k_near_spmv<f32>",   "k_fmm_apply_T<f64>",
+                                    "k_fmm_apply_T<f32>", "k_fmm_apply_D<f64>", "k_fmm_apply_D<f32>",
+                                    "k_fmm_apply_S<f64>", "k_fmm_apply_S<f32>", "k_matvec<f64>",
+                                    "k_matvec<f32>"};
+constexpr int kNumKernels = 10;
+
+struct KernelScope {
+    ncg_problem *p;
+    int id;
+    double bytes;
+    EventPair *ev = nullptr;
+    KernelScope(ncg_problem *p_, int id_, double bytes_);
+    ~KernelScope();
+};
+
 void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
 {
     if (p->mv_used == p->ev_mv.size()) {
@@ -368,8 +402,11 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
     }
     EventPair &e = p->ev_mv[p->mv_used++];
     record(p, e.a);
+    const double n = p->n, nr = p->nrhs, chunks = std::ceil(nr / 4.0);
     if (p->plan.method == 0) {
-        ncgk::matvec(p->stream, p->A, p->precision == NCG_MIXED, acc64, p->n, p->nrhs, x, d, y);
+        const int mixed = p->precision == NCG_MIXED;
+        KernelScope ks(p, 8 + mixed, n * n * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
+        ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);
     } else {
         // y = Near x + sum over levels S (D (T x))  (src/NC_CommonFunctions.cpp:1247-1316)
         const int mixed = p->precision == NCG_MIXED;
@@ -379,11 +416,26 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
             ncgk::scale_vec(p->stream, p->n, p->nrhs, d, p->fmm.xs.p);
             xin = p->fmm.xs.d();
         }
-        ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val.p, mixed, xin, y);
+        {
+            const double nnz = (double)p->fmm.nnz;
+            KernelScope ks(p, mixed, chunks * nnz * ((mixed ? 8.0 : 16.0) + 4.0) + (n + 1) * 8.0 + 2.0 * nr * n * 16.0);
+            ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val.p, mixed, xin,
+                            y);
+        }
         for (DevLevel &L : p->fmm.lev) {
-            ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, xin, L.tw.p);
-            ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, L.tw.p, L.sw.p);
-            ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, L.f32, L.sw.p, y);
+            const double sf = L.f32 ? 8.0 : 16.0, P = L.P, ncl = L.ncl, nfan = L.nfan;
+            {
+                KernelScope ks(p, 2 + L.f32, chunks * (n * P * sf + n * 4.0) + nr * n * 16.0 + nr * ncl * P * 16.0);
+                ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, xin, L.tw.p);
+            }
+            {
+                KernelScope ks(p, 4 + L.f32, chunks * (nfan * P * sf + nfan * 4.0) + 2.0 * nr * ncl * P * 16.0);
+                ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, L.tw.p, L.sw.p);
+            }
+            {
+                KernelScope ks(p, 6 + L.f32, chunks * (n * P * sf + n * 4.0) + nr * ncl * P * 16.0 + 2.0 * nr * n * 16.0);
+                ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, L.f32, L.sw.p, y);
+            }
         }
     }
     record(p, e.b);
@@ -397,6 +449,33 @@ void harvest_mv(ncg_problem *p)
         p->mv_calls++;
     }
     p->mv_used = 0;
+    for (size_t i = 0; i < p->kpend.size(); ++i) {
+        const KernelRec &r = p->kpend[i];
+        p->k_ms[r.id] += elapsed(p->kpool[r.slot]);
+        p->k_calls[r.id]++;
+        p->k_bytes[r.id] += r.bytes;
+    }
+    p->kpend.clear();
+}
+
+KernelScope::KernelScope(ncg_problem *p_, int id_, double bytes_) : p(p_), id(id_), bytes(bytes_)
+{
+    if (!p->ktime) return;
+    const size_t slot = p->kpend.size();
+    if (slot == p->kpool.size()) {
+        EventPair e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        p->kpool.push_back(e);
+    }
+    p->kpend.push_back(KernelRec{id, slot, bytes});
+    ev = &p->kpool[slot];
+    record(p, ev->a);
+}
+
+KernelScope::~KernelScope()
+{
+    if (ev) record(p, ev->b);
 }
 
 int check_err(ncg_problem *p)
@@ -603,6 +682,9 @@ int fmm_structure(ncg_problem *p)
             for (int j : L.cl[c].fan) {
                 fan_idx.push_back(j);
                 fan_row.push_back(c);
+                const double dx = L.cl[c].cen[0] - L.cl[j].cen[0], dy = L.cl[c].cen[1] - L.cl[j].cen[1],
+                             dz = L.cl[c].cen[2] - L.cl[j].cen[2];
+                D.dmin = std::min(D.dmin, sqrt(dx * dx + dy * dy + dz * dz));
             }
             fan_ptr[c + 1] = (int)fan_idx.size();
         }
@@ -672,6 +754,21 @@ int fmm_structure(ncg_problem *p)
         return rc;
     ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, F.near_row, F.near_col,
                    F.diag_pos);
+    // positions of the subdivided pairs (the dense method's near-pair table) in the CSR
+    std::vector<int> nn_off(nn_idx.size(), 0);
+    for (int c = 0; c < ncf; ++c) {
+        int o = 0;
+        for (int t = nn_ptr[c]; t < nn_ptr[c + 1]; ++t) {
+            nn_off[t] = o;
+            o += (int)FL.cl[nn_idx[t]].elems.size();
+        }
+    }
+    int *d_nn_off;
+    if ((rc = ensure_near_table(p)) || (rc = upload(F.st, s, &d_nn_off, nn_off)) ||
+        (rc = F.st.alloc(&F.adj_pos, (size_t)p->nnear)))
+        return rc;
+    ncgk::adj_positions(s, p->nnear, p->near_rows, p->near_cols, F.elem_cl_fine, DF.elem_pos, DF.cl_ptr, d_nn_ptr,
+                        d_nn_idx, d_nn_off, F.near_ptr, F.adj_pos);
     F.sig = plan_signature(p->plan);
     return 0;
 }
@@ -700,10 +797,10 @@ int assemble_fmm(ncg_problem *p)
         DevLevel &D = F.lev[l];
         D.P = L.P;
         D.L = L.L;
-        // A level whose expansion length sits at the minimum (small k R) sums large
-        // translation values against O(1) aggregation and disaggregation entries with
-        // cancellation: its T, D and S stay FP64 in the mixed form.
-        D.f32 = mixed && L.L > p->fo.min_terms;
+        // When k times the shortest interaction distance is small, the translations sum
+        // large Hankel values against O(1) aggregation and disaggregation entries with
+        // cancellation: such a level keeps T, D and S in FP64 in the mixed form.
+        D.f32 = mixed && p->k * D.dmin >= 1.0;
         const size_t sf = D.f32 ? 8 : 16;
         if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(sf * (size_t)n * D.P)) ||
             (rc = D.D.ensure(sf * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(sf * (size_t)n * D.P)) ||
@@ -716,6 +813,7 @@ int assemble_fmm(ncg_problem *p)
     // values
     const ncgk::DevElements E = p->elements();
     ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val.p, mixed, p->err);
+    ncgk::fmm_near_adj(s, E, b, p->adm, p->near_rows, p->near_cols, F.adj_pos, p->nnear, F.near_val.p, mixed, p->err);
     if (p->nsrc)
         ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near,
                           F.ncl_fine);
@@ -801,11 +899,17 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
         (rc = F.fgrad.ensure(48 * (size_t)nr * m)))
         return rc;
     const ncgk::BMParams b = p->bm();
-    ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
-                       D0.tv.d(), D0.f32);
+    if (p->all_velocity && p->nsrc == 0) {
+        // velocity elements with no prescribed velocity: the signature of the solution is
+        // the level-0 aggregation T applied to the potential (same sum as below)
+        ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, pot, D0.tv.d());
+    } else {
+        ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
+                           D0.tv.d(), D0.f32);
+    }
     ncgk::fmm_eval_hankel(s, D0.L, p->k, p->h, F.ecen, D0.ccen, F.pair_ev, F.pair_cl, F.npair, F.tab.d());
     ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
-                     nr, F.z.p);
+                     nr, F.z.p, D0.f32);
     ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
     ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad,
                 p->precision == NCG_MIXED, p->err, F.pt_ev, F.el_ptr, F.el_idx, F.fphi.p, F.fgrad.p);
@@ -1137,9 +1241,9 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
             // projections h1, h2 (two Gram-Schmidt passes) and the norm, all on the device:
             // dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3]
             cd *h1 = (cd *)p->dotout, *h2 = h1 + (size_t)nr * (k + 1), *hn = h2 + (size_t)nr * (k + 1);
-            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h1);
+            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h1, p->partial, p->partial_cap);
             ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h1, W, 0);
-            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h2);
+            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h2, p->partial, p->partial_cap);
             ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h2, W, 0);
             {
                 const void *u[1] = {W}, *v[1] = {W};
@@ -1339,6 +1443,7 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     p->h = g->harmonic_time_factor == -1.0 ? -1.0 : 1.0;
     p->interior = g->interior ? 1 : 0;
     p->kind_h = kh;
+    p->all_velocity = std::all_of(kh.begin(), kh.end(), [](int8_t k) { return k == NCG_BC_VELOCITY; });
     auto bail = [&](int rc) {
         ncg_problem_destroy(p);
         return rc;
@@ -1364,7 +1469,10 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
         (rc = dalloc(&p->dvec, n)) || (rc = dalloc(&p->active, 64)))
         return bail(rc);
     const int nb = ncgk::dot_blocks(n);
-    if ((rc = dalloc(&p->partial, (size_t)2 * nb * 3 * 64)) || (rc = dalloc(&p->dotout, (size_t)2 * 3 * 64 * 256)) ||
+    // partial sums of the dot kernels: 3 dots x 64 load cases per block, and at least
+    // 256 slices for 8 basis vectors of 2 load cases in GMRES projections
+    p->partial_cap = std::max<size_t>((size_t)nb * 3 * 64, 4096);
+    if ((rc = dalloc(&p->partial, 2 * p->partial_cap)) || (rc = dalloc(&p->dotout, (size_t)2 * 3 * 64 * 256)) ||
         (rc = dalloc(&p->scal, (size_t)2 * 8 * 64 * 2)))
         return bail(rc);
     hipStream_t s = p->stream;
@@ -1401,6 +1509,7 @@ int ncg_problem_destroy(ncg_problem *p)
         if (e->b) hipEventDestroy(e->b);
     }
     for (EventPair &e : p->ev_mv) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    for (EventPair &e : p->kpool) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
     return 0;
@@ -1731,3 +1840,41 @@ size_t estimate_bytes(const ncgf::Mesh &M, const ncgf::Options &O, int m_eval, d
 }
 
 }  // namespace ncgi
+
+// ==================================================================================== kernel timing
+extern "C" {
+
+int ncg_set_kernel_timing(ncg_problem *p, int32_t on)
+{
+    if (!p) return fail(NCG_E_ARG, "ncg_set_kernel_timing: null argument");
+    p->ktime = on ? 1 : 0;
+    return 0;
+}
+
+int ncg_kernel_stats(ncg_problem *p, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset)
+{
+    if (!p || !count || (cap > 0 && !out)) return fail(NCG_E_ARG, "ncg_kernel_stats: null argument");
+    NCG_HIP(hipSetDevice(p->device));
+    harvest_mv(p);
+    int c = 0;
+    for (int id = 0; id < kNumKernels; ++id) {
+        if (p->k_calls[id] == 0) continue;
+        if (c < cap) {
+            memset(&out[c], 0, sizeof(out[c]));
+            snprintf(out[c].name, sizeof(out[c].name), "%s", kKernelNames[id]);
+            out[c].launches = p->k_calls[id];
+            out[c].total_ms = p->k_ms[id];
+            out[c].bytes = p->k_bytes[id];
+        }
+        ++c;
+    }
+    *count = c;
+    if (reset)
+        for (int id = 0; id < kNumKernels; ++id) {
+            p->k_calls[id] = 0;
+            p->k_ms[id] = p->k_bytes[id] = 0.0;
+        }
+    return 0;
+}
+
+}  // extern "C"

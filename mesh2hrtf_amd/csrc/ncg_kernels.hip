@@ -479,16 +479,18 @@ __global__ void k_row_scaling_csr(int n, const long long *ptr, const cx<S> *val,
     if (lane == 0) d[i] = sqrt((double)(ptr[i + 1] - ptr[i]) / s);
 }
 
-// GMRES: h[r][k] = <V_k, w_r> for k < kcount (Hermitian), one block per (k, r) pair.
-// V layout: [basis k][rhs r][n]; w: [r][n].
-__global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, const cx<double> *V, const cx<double> *w,
-                                                  cx<double> *h)
+// GMRES: h[r][k] = <V_k, w_r> for k < kcount (Hermitian).  One block per (k, r, slice of
+// n) writes a partial sum; k_multidot_sum adds the slices in a fixed order.
+// V layout: [basis k][rhs r][n]; w: [r][n]; part: [(k * nrhs + r) * nslice + slice].
+__global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, int slice, const cx<double> *V,
+                                                  const cx<double> *w, cx<double> *part)
 {
     __shared__ double sre[256], sim[256];
-    const int k = blockIdx.x, r = blockIdx.y;
+    const int k = blockIdx.x, r = blockIdx.y, sl = blockIdx.z, nsl = gridDim.z;
     const cx<double> *v = V + ((size_t)k * nrhs + r) * n, *ww = w + (size_t)r * n;
+    const int i0 = sl * slice, i1 = min(n, i0 + slice);
     double re = 0.0, im = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
         const cx<double> a = v[i], b = ww[i];
         re += a.re * b.re + a.im * b.im;
         im += a.re * b.im - a.im * b.re;
@@ -503,7 +505,17 @@ __global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, c
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) h[(size_t)r * kcount + k] = mk(sre[0], sim[0]);
+    if (threadIdx.x == 0) part[((size_t)k * nrhs + r) * nsl + sl] = mk(sre[0], sim[0]);
+}
+
+__global__ void k_multidot_sum(int nrhs, int kcount, int nsl, const cx<double> *part, cx<double> *h)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrhs * kcount) return;
+    const int k = t / nrhs, r = t % nrhs;
+    cx<double> s = mk(0.0, 0.0);
+    for (int q = 0; q < nsl; ++q) s += part[(size_t)t * nsl + q];
+    h[(size_t)r * kcount + k] = s;
 }
 
 // w[r] -= sum_k V_k[r] h[r][k]
@@ -1121,9 +1133,54 @@ __global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, co
     Sv[(size_t)e * P + p] = cx<S>{S(v.re), S(v.im)};
 }
 
-// near-field entries of the FMM operator: one thread per off-diagonal nonzero.  Pairs
-// NumCalc subdivides are integrated in FP64; the regular rule of the other near pairs
-// runs in T (FP32 in the mixed form, with the geometry differences formed in FP64).
+// CSR position of every subdivided pair (i, j) of the dense near-pair table inside the
+// FMM near field: the near-list slot of j's cluster in i's cluster, its element offset
+// there, and j's place in its cluster; -1 when the clusters are far from each other.
+__global__ void k_adj_positions(long long npairs, const int *rows, const int *cols, const int *elem_cl,
+                                const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx,
+                                const int *nn_off, const long long *near_ptr, long long *pos)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs) return;
+    const int i = rows[q], j = cols[q];
+    const int ci = elem_cl[i], cj = elem_cl[j];
+    long long r = -1;
+    for (int t = nn_ptr[ci]; t < nn_ptr[ci + 1]; ++t)
+        if (nn_idx[t] == cj) {
+            r = near_ptr[i] + nn_off[t] + (elem_pos[j] - cl_ptr[cj]);
+            break;
+        }
+    pos[q] = r;
+}
+
+// near-field entries NumCalc subdivides: adaptive quadrature in FP64, one thread per pair
+template <typename S>
+__global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, const double *adm, const int *rows,
+                                                      const int *cols, const long long *pos, long long npairs,
+                                                      cx<S> *vals, int *err)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs || pos[q] < 0) return;
+    const int i = rows[q], j = cols[q];
+    const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
+    const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
+    LocalElem L;
+    load_local(E, j, x, L);
+    BMAcc<double> acc;
+    acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
+    const int e = visit_adaptive(L, [&](double s, double t, double w) {
+        double q2[3], ny[3], J;
+        point_on(L, s, t, q2, ny, J);
+        bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * J, bm.hk, bm.k2);
+    });
+    if (e) atomicOr(err, e);
+    const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+    vals[pos[q]] = cx<S>{S(a.re), S(a.im)};
+}
+
+// near-field entries of the FMM operator with the regular rule: one thread per
+// off-diagonal nonzero; subdivided pairs are left to k_fmm_near_adj.  T is the
+// arithmetic (FP32 in the mixed form, with the geometry differences formed in FP64).
 template <typename S, typename T>
 __global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *rows,
                                                   const int *cols, long long nnz, cx<S> *vals, int *err)
@@ -1137,19 +1194,7 @@ __global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const 
     LocalElem L;
     load_local(E, j, x, L);
     const double ratdis = ratio_to_centre(E.cen[j], E.cen[E.n + j], E.cen[2 * E.n + j], x, L.area);
-    if (ratdis < kTolf) {
-        BMAcc<double> acc;
-        acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
-        const int e = visit_adaptive(L, [&](double s, double t, double w) {
-            double q2[3], ny[3], J;
-            point_on(L, s, t, q2, ny, J);
-            bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * J, bm.hk, bm.k2);
-        });
-        if (e) atomicOr(err, e);
-        const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
-        vals[q] = cx<S>{S(a.re), S(a.im)};
-        return;
-    }
+    if (ratdis < kTolf) return;
     BMAcc<T> acc;
     acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
     const T nxx = T(nx[0]), nxy = T(nx[1]), nxz = T(nx[2]), hk = T(bm.hk), k2 = T(bm.k2);
@@ -1551,48 +1596,64 @@ __global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, doubl
 }
 
 // Second pass: one block per evaluation cluster, threads over sphere points, summing
-// its far pairs in registers (pairs of one evaluation cluster are contiguous)
-template <int NR>
+// its far pairs in registers (pairs of one evaluation cluster are contiguous and are
+// staged through LDS in batches).  T is the arithmetic of the series (FP32 in the mixed
+// form at regular levels); the sums over pairs are FP64.
+template <int NR, typename T>
 __global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, double h, const int *ev_ptr,
                                                     const int *pair_cl, const double *tab, const double *u,
                                                     const double *wsph, const cx<double> *tv, int ncl,
                                                     cx<double> *z)
 {
-    __shared__ double sh[4 + 2 * kMaxTerms];
-    const int a = blockIdx.x;
+    constexpr int QB = 16;
+    __shared__ T sh[QB][4 + 2 * kMaxTerms];
+    __shared__ T ca[kMaxTerms], cb[kMaxTerms], cn[kMaxTerms];
+    __shared__ int scl[QB];
+    const int a = blockIdx.x, w = 4 + 2 * L;
     const size_t ldt = (size_t)ncl * P, ldz = (size_t)nev * P;
+    const T hh = T(h);
+    // Legendre recurrence P_n = ((2n-1) c P_{n-1} - (n-1) P_{n-2}) / n as P_n = ca c P_{n-1} - cb P_{n-2}
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+        ca[t] = t >= 2 ? T((double)(2 * t - 1) / (double)t) : T(0);
+        cb[t] = t >= 2 ? T((double)(t - 1) / (double)t) : T(0);
+        cn[t] = T(2 * t + 1);
+    }
     for (int p0 = 0; p0 < P; p0 += blockDim.x) {
         const int p = p0 + threadIdx.x;
         const bool on = p < P;
-        double su[3] = {0.0, 0.0, 0.0};
-        if (on) { su[0] = u[3 * p]; su[1] = u[3 * p + 1]; su[2] = u[3 * p + 2]; }
+        T su[3] = {T(0), T(0), T(0)};
+        T wp = T(0);
+        if (on) { su[0] = T(u[3 * p]); su[1] = T(u[3 * p + 1]); su[2] = T(u[3 * p + 2]); wp = T(wsph[p]); }
         cx<double> acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[r] = mk(0.0, 0.0);
-        for (int q = ev_ptr[a]; q < ev_ptr[a + 1]; ++q) {
+        const int qend = ev_ptr[a + 1];
+        for (int q0 = ev_ptr[a]; q0 < qend; q0 += QB) {
+            const int nb = min(QB, qend - q0);
             __syncthreads();
-            for (int t = threadIdx.x; t < 4 + 2 * L; t += blockDim.x) sh[t] = tab[(size_t)q * (4 + 2 * L) + t];
+            for (int t = threadIdx.x; t < nb * w; t += blockDim.x) sh[t / w][t % w] = T(tab[(size_t)q0 * w + t]);
+            for (int t = threadIdx.x; t < nb; t += blockDim.x) scl[t] = pair_cl[q0 + t];
             __syncthreads();
             if (!on) continue;
-            const double cc = sh[0] * su[0] + sh[1] * su[1] + sh[2] * su[2];
-            cx<double> mu = mk(0.0, 0.0), zi = mk(1.0, 0.0);
-            double pm1 = 1.0, pm0 = cc;
-            for (int nn = 0; nn < L; ++nn) {
-                double pn;
-                if (nn == 0) pn = 1.0;
-                else if (nn == 1) pn = cc;
-                else {
-                    pn = (pm0 * cc * (double)(2 * nn - 1) - pm1 * (double)(nn - 1)) / (double)nn;
+            for (int qq = 0; qq < nb; ++qq) {
+                const T *s = sh[qq];
+                const T cc = s[0] * su[0] + s[1] * su[1] + s[2] * su[2];
+                cx<T> mu = mk(s[4], s[5]);   // n = 0
+                cx<T> zi = mk(T(0), hh);     // (i h)^1
+                T pm1 = T(1), pm0 = cc;
+                if (L > 1) mu += zi * mk(s[6], s[7]) * (cc * cn[1]);
+                for (int nn = 2; nn < L; ++nn) {
+                    const T pn = ca[nn] * cc * pm0 - cb[nn] * pm1;
                     pm1 = pm0;
                     pm0 = pn;
+                    zi = mk(-zi.im * hh, zi.re * hh);
+                    mu += zi * mk(s[4 + 2 * nn], s[5 + 2 * nn]) * (pn * cn[nn]);
                 }
-                mu += zi * mk(sh[4 + 2 * nn], sh[5 + 2 * nn]) * (pn * (double)(2 * nn + 1));
-                zi = mk(-zi.im * h, zi.re * h);
-            }
-            mu = mu * wsph[p];
-            const int c = pair_cl[q];
+                const cx<double> md = mk(double(mu.re * wp), double(mu.im * wp));
+                const int c = scl[qq];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += mu * tv[r * ldt + (size_t)c * P + p];
+                for (int r = 0; r < NR; ++r) acc[r] += md * tv[r * ldt + (size_t)c * P + p];
+            }
         }
         if (on)
 #pragma unroll
@@ -1805,10 +1866,17 @@ void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val
     if (mixed) k_row_scaling_csr<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<float> *)val, d);
     else k_row_scaling_csr<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<double> *)val, d);
 }
-void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h)
+void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h, void *part,
+              size_t part_cap)
 {
-    k_multidot<<<dim3(kcount, nrhs), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V, (const cx<double> *)w,
-                                                  (cx<double> *)h);
+    // slices of about 4096 entries, as many as the partial buffer holds
+    int nsl = std::max(1, (n + 4095) / 4096);
+    nsl = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsl, part_cap / ((size_t)kcount * nrhs)));
+    const int slice = (n + nsl - 1) / nsl;
+    k_multidot<<<dim3(kcount, nrhs, nsl), 256, 0, s>>>(n, nrhs, kcount, slice, (const cx<double> *)V,
+                                                       (const cx<double> *)w, (cx<double> *)part);
+    k_multidot_sum<<<blocks_for((size_t)nrhs * kcount, 256), 256, 0, s>>>(nrhs, kcount, nsl, (const cx<double> *)part,
+                                                                           (cx<double> *)h);
 }
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add)
 {
@@ -1926,6 +1994,23 @@ void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const doub
     const unsigned g = blocks_for(nnz, 128);
     if (mixed) k_fmm_near<float, float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
     else k_fmm_near<double, double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
+}
+void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
+                   const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
+                   const long long *near_ptr, long long *pos)
+{
+    if (npairs == 0) return;
+    k_adj_positions<<<blocks_for(npairs, 256), 256, 0, s>>>(npairs, rows, cols, elem_cl, elem_pos, cl_ptr, nn_ptr,
+                                                            nn_idx, nn_off, near_ptr, pos);
+}
+void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
+                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err)
+{
+    if (npairs == 0) return;
+    Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
+    const unsigned g = blocks_for(npairs, 128);
+    if (mixed) k_fmm_near_adj<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<float> *)vals, err);
+    else k_fmm_near_adj<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<double> *)vals, err);
 }
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag)
@@ -2074,17 +2159,27 @@ void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ece
     k_fmm_pair_hankel<<<blocks_for(npair, 128), 128, 0, s>>>(npair, L, k, h, ecen, ccen, pair_ev, pair_cl, tab);
 }
 void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
-                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z)
+                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
+                int fp32)
 {
     if (nev == 0) return;
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const cx<double> *t = (const cx<double> *)tv + (size_t)r0 * ncl * P;
         cx<double> *zz = (cx<double> *)z + (size_t)r0 * nev * P;
-        switch (nr) {
-        case 1: k_fmm_eval_z<1><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-        case 2: k_fmm_eval_z<2><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-        case 3: k_fmm_eval_z<3><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-        default: k_fmm_eval_z<4><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        if (fp32) {
+            switch (nr) {
+            case 1: k_fmm_eval_z<1, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 2: k_fmm_eval_z<2, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 3: k_fmm_eval_z<3, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            default: k_fmm_eval_z<4, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            }
+        } else {
+            switch (nr) {
+            case 1: k_fmm_eval_z<1, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 2: k_fmm_eval_z<2, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 3: k_fmm_eval_z<3, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            default: k_fmm_eval_z<4, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            }
         }
     });
 }

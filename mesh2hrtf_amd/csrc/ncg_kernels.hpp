@@ -62,7 +62,9 @@ void axpby(hipStream_t s, int n, int nrhs, const void *coef, const void *x, size
 void scale_vec(hipStream_t s, int n, int nrhs, const double *d, void *x);
 void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d);
 void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val, int mixed, double *d);
-void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h);
+// h[r][k] = <V_k, w_r>; part: scratch of part_cap complex values for slice partials
+void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h, void *part,
+              size_t part_cap);
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add);
 // incomplete LU of the row-scaled operator (dense A when ptr == nullptr, else the
 // near-field CSR ptr/ncol/A); pattern CSR rp/col/val, level schedules from the host
@@ -95,6 +97,12 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
            const double *u, const double *w, void *Sv, int mixed);
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
               long long nnz, void *vals, int mixed, int *err);
+// subdivided pairs of the near field: CSR positions (once per structure) and values
+void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
+                   const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
+                   const long long *near_ptr, long long *pos);
+void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
+                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err);
 // block-Jacobi preconditioner: blocks of at most 64 consecutive elements of a finest
 // cluster, gathered from the near-field CSR and inverted in place (complex FP64)
 void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
@@ -122,7 +130,8 @@ void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P,
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
                      const int *pair_ev, const int *pair_cl, int npair, double *tab);
 void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
-                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z);
+                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
+                int fp32);
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad);
 

@@ -104,6 +104,7 @@ struct ncg_server {
     std::vector<double> latencies;
     int64_t jobs_done = 0, jobs_failed = 0, steps_done = 0, solves_done = 0;
     clk::time_point t0;
+    std::map<std::string, ncg_kernel_stat> kstats;   // per-kernel timing of finished steps
 };
 
 namespace {
@@ -234,6 +235,7 @@ void run_lane(ncg_server *S)
         if (!r->p) {
             rc = ncg_problem_create(&M->g, S->cfg.device, &r->p);
             if (rc) msg = ncg_last_error();
+            else if (S->cfg.kernel_timing) ncg_set_kernel_timing(r->p, 1);
         }
         if (!rc) {
             rc = ncg_assemble(r->p, &J->freq[st.fi].f, J->precision);
@@ -261,7 +263,18 @@ void run_lane(ncg_server *S)
             if (rc) msg = ncg_last_error();
         }
 
+        ncg_kernel_stat ks[16];
+        int32_t nks = 0;
+        if (S->cfg.kernel_timing && r->p && ncg_kernel_stats(r->p, ks, 16, &nks, 1) != 0) nks = 0;
+
         lk.lock();
+        for (int i = 0; i < nks && i < 16; ++i) {
+            ncg_kernel_stat &a = S->kstats[ks[i].name];
+            if (a.launches == 0) snprintf(a.name, sizeof(a.name), "%s", ks[i].name);
+            a.launches += ks[i].launches;
+            a.total_ms += ks[i].total_ms;
+            a.bytes += ks[i].bytes;
+        }
         S->running--;
         r->busy = false;
         r->last = ++S->tick;
@@ -467,6 +480,20 @@ int ncg_server_stats(ncg_server *S, ncg_server_info *st)
     st->peak_resident_bytes = (int64_t)S->peak;
     st->lanes = S->cfg.lanes;
     st->queued_steps = (int32_t)S->queue.size();
+    return 0;
+}
+
+int ncg_server_kernel_stats(ncg_server *S, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset)
+{
+    if (!S || !count || (cap > 0 && !out)) return sfail(NCG_E_ARG, "ncg_server_kernel_stats: null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    int c = 0;
+    for (auto &kv : S->kstats) {
+        if (c < cap) out[c] = kv.second;
+        ++c;
+    }
+    *count = c;
+    if (reset) S->kstats.clear();
     return 0;
 }
 

@@ -88,12 +88,23 @@ class _Stats(ctypes.Structure):
 EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_create", "ncg_problem_destroy",
            "ncg_assemble", "ncg_solve", "ncg_problem_fmm_info", "ncg_get_solution", "ncg_field", "ncg_get_system",
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
-           "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy")
+           "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
+           "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats")
+
+
+class _KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
+def _kernel_dict(arr, count):
+    return {arr[i].name.decode(): dict(calls=arr[i].launches, total_ms=arr[i].total_ms, bytes=arr[i].bytes)
+            for i in range(min(count, len(arr)))}
 
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("memory_budget_bytes", ctypes.c_double),
-                ("problems_per_lane", ctypes.c_int32)]
+                ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32)]
 
 
 class _Job(ctypes.Structure):
@@ -260,6 +271,15 @@ class Problem:
         _check(self.lib.ncg_apply(self._h, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data), len(x)))
         return y
 
+    def kernel_timing(self, on=True):
+        _check(self.lib.ncg_set_kernel_timing(self._h, int(on)))
+
+    def kernel_stats(self, reset=False):
+        arr = (_KernelStat * 16)()
+        cnt = ctypes.c_int32()
+        _check(self.lib.ncg_kernel_stats(self._h, arr, 16, ctypes.byref(cnt), int(reset)))
+        return _kernel_dict(arr, cnt.value)
+
     def timings(self, reset=False):
         t = (ctypes.c_double * 8)()
         _check(self.lib.ncg_timings(self._h, t, int(reset)))
@@ -289,10 +309,10 @@ class Server:
     """Serving runtime (ncg_server_*): jobs of (mesh, frequencies, load cases) on one GPU
     with continuous batching over `lanes` concurrent steps and resident problems."""
 
-    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2):
+    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False):
         self.lib = load()
         self._h = ctypes.c_void_p()
-        cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane))
+        cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing))
         _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._meshes = {}
 
@@ -347,6 +367,12 @@ class Server:
         st = _JobStatus()
         _check(self.lib.ncg_server_query(self._h, ctypes.c_int64(job.id), ctypes.byref(st)))
         return _status_dict(st)
+
+    def kernel_stats(self, reset=False):
+        arr = (_KernelStat * 16)()
+        cnt = ctypes.c_int32()
+        _check(self.lib.ncg_server_kernel_stats(self._h, arr, 16, ctypes.byref(cnt), int(reset)))
+        return _kernel_dict(arr, cnt.value)
 
     def stats(self):
         s = _ServerInfo()
