@@ -1419,79 +1419,108 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, const long long *ptr, 
     }
 }
 
-// t[c, p] = sum over the cluster's elements of T * x; thread per (cluster, sphere point),
-// the element loop unrolled by 4 for independent loads
+// t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
+// of 64 sphere points); its 4 waves take every 4th element of the cluster and their
+// partial sums meet in LDS, so a cluster's ~100 elements are read by 256 lanes at once.
 template <typename S, int NR>
 __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
                                                      const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)ncl * P) return;
-    const int c = (int)(t / P), p = (int)(t % P);
+    __shared__ double red[3][64][2 * NR];
+    const int tiles = (P + 63) / 64;
+    const int c = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = tile * 64 + lane;
+    const bool on = p < P;
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
     const int k1 = cl_ptr[c + 1];
-    int k = cl_ptr[c];
-    for (; k + 4 <= k1; k += 4) {
-        cx<S> a[4];
-        int e[4];
+    int k = cl_ptr[c] + wv;
+    if (on) {
+        for (; k + 4 < k1; k += 8) {
+            const cx<S> a0 = Tv[(size_t)k * P + p], a1 = Tv[(size_t)(k + 4) * P + p];
+            const int e0 = cl_elem[k], e1 = cl_elem[k + 4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a[u] = Tv[(size_t)(k + u) * P + p];
-            e[u] = cl_elem[k + u];
+            for (int r = 0; r < NR; ++r) {
+                cmac(re[r], im[r], double(a0.re), double(a0.im), x[(size_t)r * n + e0]);
+                cmac(re[r], im[r], double(a1.re), double(a1.im), x[(size_t)r * n + e1]);
+            }
         }
+        for (; k < k1; k += 4) {
+            const cx<S> a = Tv[(size_t)k * P + p];
+            const int e = cl_elem[k];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a[u].re), double(a[u].im), x[(size_t)r * n + e[u]]);
+            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + e]);
+        }
     }
-    for (; k < k1; ++k) {
-        const cx<S> a = Tv[(size_t)k * P + p];
-        const int e = cl_elem[k];
+    if (wv > 0)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + e]);
+        for (int r = 0; r < NR; ++r) {
+            red[wv - 1][lane][2 * r] = re[r];
+            red[wv - 1][lane][2 * r + 1] = im[r];
+        }
+    __syncthreads();
+    if (wv == 0 && on) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
+            const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
+            tw[(size_t)r * ncl * P + (size_t)c * P + p] = mk(sr, si);
+        }
     }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) tw[(size_t)r * ncl * P + t] = mk(re[r], im[r]);
 }
 
-// s[i, p] = sum over the interaction list of D * t (D is diagonal in p)
+// s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
+// per (cluster, tile of 64 points), the 4 waves splitting the interaction list
 template <typename S, int NR>
 __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
                                                      const cx<S> *Dv, const cx<double> *tw, cx<double> *sw)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)ncl * P) return;
-    const int i = (int)(t / P), p = (int)(t % P);
+    __shared__ double red[3][64][2 * NR];
+    const int tiles = (P + 63) / 64;
+    const int i = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = tile * 64 + lane;
+    const bool on = p < P;
     const size_t ld = (size_t)ncl * P;
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
     const int q1 = fan_ptr[i + 1];
-    int q = fan_ptr[i];
-    for (; q + 4 <= q1; q += 4) {
-        cx<S> a[4];
-        int j[4];
+    int q = fan_ptr[i] + wv;
+    if (on) {
+        for (; q + 4 < q1; q += 8) {
+            const cx<S> a0 = Dv[(size_t)q * P + p], a1 = Dv[(size_t)(q + 4) * P + p];
+            const int j0 = fan_idx[q], j1 = fan_idx[q + 4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a[u] = Dv[(size_t)(q + u) * P + p];
-            j[u] = fan_idx[q + u];
+            for (int r = 0; r < NR; ++r) {
+                cmac(re[r], im[r], double(a0.re), double(a0.im), tw[r * ld + (size_t)j0 * P + p]);
+                cmac(re[r], im[r], double(a1.re), double(a1.im), tw[r * ld + (size_t)j1 * P + p]);
+            }
         }
+        for (; q < q1; q += 4) {
+            const cx<S> a = Dv[(size_t)q * P + p];
+            const int j = fan_idx[q];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                cmac(re[r], im[r], double(a[u].re), double(a[u].im), tw[r * ld + (size_t)j[u] * P + p]);
+            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), tw[r * ld + (size_t)j * P + p]);
+        }
     }
-    for (; q < q1; ++q) {
-        const cx<S> a = Dv[(size_t)q * P + p];
-        const int j = fan_idx[q];
+    if (wv > 0)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), tw[r * ld + (size_t)j * P + p]);
+        for (int r = 0; r < NR; ++r) {
+            red[wv - 1][lane][2 * r] = re[r];
+            red[wv - 1][lane][2 * r + 1] = im[r];
+        }
+    __syncthreads();
+    if (wv == 0 && on) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
+            const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
+            sw[r * ld + (size_t)i * P + p] = mk(sr, si);
+        }
     }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) sw[r * ld + t] = mk(re[r], im[r]);
 }
 
 // y[e] += sum_p S[e, p] s[cluster(e), p]; one wave per element
@@ -2090,7 +2119,7 @@ template <typename S>
 static void apply_T_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem,
                       const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
 {
-    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
         NCG_NR_DISPATCH(nr, k_fmm_apply_T, S, g, n, ncl, P, cl_ptr, cl_elem, Tv, x + (size_t)r0 * n,
                         tw + (size_t)r0 * ncl * P);
@@ -2108,7 +2137,7 @@ template <typename S>
 static void apply_D_t(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const cx<S> *Dv,
                       const cx<double> *tw, cx<double> *sw)
 {
-    const unsigned g = blocks_for((size_t)ncl * P, 256);
+    const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
         NCG_NR_DISPATCH(nr, k_fmm_apply_D, S, g, ncl, P, fan_ptr, fan_idx, Dv, tw + (size_t)r0 * ncl * P,
                         sw + (size_t)r0 * ncl * P);
