@@ -153,10 +153,21 @@ struct DevLevel {
     double *ccen = nullptr;
     int *cl_ptr = nullptr, *cl_elem = nullptr, *elem_cl = nullptr, *elem_pos = nullptr;
     int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
+    int nch = 0;                      // row chunks: <= 16 elements of one cluster each
+    int *ch_cl = nullptr, *ch_r0 = nullptr, *ch_m = nullptr;
     int P = 0, L = 0;
-    int f32 = 0;                      // T, D, S stored in FP32 (mixed form, regular levels only)
+    int f32 = 0;                      // T, D, S (and t, s) stored in FP32 (mixed form, regular levels only)
     double dmin = 1e300;              // shortest centre distance of an interaction pair
     Grow u, w, T, D, S, tw, sw, tv;   // tw, sw, tv: [nrhs][ncl*P]
+    ncgk::RowChunks chunks() const
+    {
+        ncgk::RowChunks r;
+        r.n = nch;
+        r.cl = ch_cl;
+        r.r0 = ch_r0;
+        r.m = ch_m;
+        return r;
+    }
     void release()
     {
         for (Grow *g : {&u, &w, &T, &D, &S, &tw, &sw, &tv}) g->release();
@@ -294,6 +305,7 @@ struct ncg_problem {
     size_t work_cap = 0;
     double *scal = nullptr, *partial = nullptr, *dotout = nullptr, *dvec = nullptr;
     size_t partial_cap = 0;   // complex entries of partial
+    double *npart = nullptr;  // [64 load cases][256 blocks] partial norms of the Gram-Schmidt passes
     int *active = nullptr;
     double *basis = nullptr;
     size_t basis_cap = 0;
@@ -418,23 +430,28 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         }
         {
             const double nnz = (double)p->fmm.nnz;
-            KernelScope ks(p, mixed, chunks * nnz * ((mixed ? 8.0 : 16.0) + 4.0) + (n + 1) * 8.0 + 2.0 * nr * n * 16.0);
-            ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_ptr, p->fmm.near_col, p->fmm.near_val.p, mixed, xin,
-                            y);
+            const DevLevel &DF = p->fmm.lev.back();
+            KernelScope ks(p, mixed, chunks * nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
+            ncgk::near_spmv(p->stream, p->n, p->nrhs, DF.chunks(), DF.cl_ptr, DF.cl_elem, p->fmm.near_ptr,
+                            p->fmm.near_col, p->fmm.near_val.p, mixed, xin, y);
         }
         for (DevLevel &L : p->fmm.lev) {
+            // algorithmic bytes: the operator once per pass of at most 4 load cases, the
+            // vectors once (t and s in the level's storage precision)
             const double sf = L.f32 ? 8.0 : 16.0, P = L.P, ncl = L.ncl, nfan = L.nfan;
             {
-                KernelScope ks(p, 2 + L.f32, chunks * (n * P * sf + n * 4.0) + nr * n * 16.0 + nr * ncl * P * 16.0);
-                ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, xin, L.tw.p);
+                KernelScope ks(p, 2 + L.f32, chunks * n * P * sf + nr * n * 16.0 + nr * ncl * P * sf);
+                ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, 0, xin,
+                                  L.tw.p);
             }
             {
-                KernelScope ks(p, 4 + L.f32, chunks * (nfan * P * sf + nfan * 4.0) + 2.0 * nr * ncl * P * 16.0);
-                ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, L.tw.p, L.sw.p);
+                KernelScope ks(p, 4 + L.f32, chunks * nfan * P * sf + 2.0 * nr * ncl * P * sf);
+                ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, 0, L.tw.p,
+                                  L.sw.p);
             }
             {
-                KernelScope ks(p, 6 + L.f32, chunks * (n * P * sf + n * 4.0) + nr * ncl * P * 16.0 + 2.0 * nr * n * 16.0);
-                ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.elem_cl, L.S.p, L.f32, L.sw.p, y);
+                KernelScope ks(p, 6 + L.f32, chunks * n * P * sf + nr * ncl * P * sf + 2.0 * nr * n * 16.0);
+                ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.chunks(), L.cl_elem, L.S.p, L.f32, L.sw.p, y);
             }
         }
     }
@@ -689,7 +706,20 @@ int fmm_structure(ncg_problem *p)
             fan_ptr[c + 1] = (int)fan_idx.size();
         }
         D.nfan = (int)fan_idx.size();
-        if ((rc = upload(F.st, s, &D.ccen, cc)) || (rc = upload(F.st, s, &D.cl_ptr, cl_ptr)) ||
+        std::vector<int> ch_cl, ch_r0, ch_m;
+        for (int c = 0; c < D.ncl; ++c) {
+            const int m = cl_ptr[c + 1] - cl_ptr[c], nch = (m + ncgk::kChunkRows - 1) / ncgk::kChunkRows;
+            for (int q = 0, r0 = cl_ptr[c]; q < nch; ++q) {
+                const int mq = m / nch + (q < m % nch ? 1 : 0);
+                ch_cl.push_back(c);
+                ch_r0.push_back(r0);
+                ch_m.push_back(mq);
+                r0 += mq;
+            }
+        }
+        D.nch = (int)ch_cl.size();
+        if ((rc = upload(F.st, s, &D.ccen, cc)) || (rc = upload(F.st, s, &D.ch_cl, ch_cl)) ||
+            (rc = upload(F.st, s, &D.ch_r0, ch_r0)) || (rc = upload(F.st, s, &D.ch_m, ch_m)) || (rc = upload(F.st, s, &D.cl_ptr, cl_ptr)) ||
             (rc = upload(F.st, s, &D.cl_elem, cl_elem)) || (rc = upload(F.st, s, &D.elem_cl, elem_cl)) ||
             (rc = upload(F.st, s, &D.elem_pos, elem_pos)) || (rc = upload(F.st, s, &D.fan_ptr, fan_ptr)) ||
             (rc = upload(F.st, s, &D.fan_idx, fan_idx)) || (rc = upload(F.st, s, &D.fan_row, fan_row)))
@@ -826,8 +856,8 @@ int assemble_fmm(ncg_problem *p)
         if (p->nsrc) {
             // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
             ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u.d(), p->bcval, nr, D.tv.d());
-            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, D.f32, D.tv.d(), D.sw.d());
-            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.elem_cl, D.S.p, D.f32, D.sw.d(), p->rhs);
+            ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, D.f32, 1, D.tv.d(), D.sw.p);
+            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.chunks(), D.cl_elem, D.S.p, D.f32, D.sw.p, p->rhs);
         }
     }
     return 0;
@@ -902,7 +932,8 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
     if (p->all_velocity && p->nsrc == 0) {
         // velocity elements with no prescribed velocity: the signature of the solution is
         // the level-0 aggregation T applied to the potential (same sum as below)
-        ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, pot, D0.tv.d());
+        ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, 1, pot,
+                          D0.tv.d());
     } else {
         ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
                            D0.tv.d(), D0.f32);
@@ -1049,8 +1080,7 @@ void bj_apply_vec(ncg_problem *p, const double *x, double *y)
 {
     const DevFmm &F = p->fmm;
     const DevLevel &D = F.lev.back();
-    ncgk::bj_apply(p->stream, p->n, p->nrhs, F.elem_blk, D.elem_pos, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, x,
-                   y);
+    ncgk::bj_apply(p->stream, p->n, p->nrhs, F.nblk, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, x, y);
 }
 
 // ---------------------------------------------------------------------------------- CGS
@@ -1241,15 +1271,9 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
             // projections h1, h2 (two Gram-Schmidt passes) and the norm, all on the device:
             // dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3]
             cd *h1 = (cd *)p->dotout, *h2 = h1 + (size_t)nr * (k + 1), *hn = h2 + (size_t)nr * (k + 1);
-            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h1, p->partial, p->partial_cap);
-            ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h1, W, 0);
-            ncgk::multidot(p->stream, n, nr, k + 1, V, W, h2, p->partial, p->partial_cap);
-            ncgk::multiaxpy(p->stream, n, nr, k + 1, V, h2, W, 0);
-            {
-                const void *u[1] = {W}, *v[1] = {W};
-                ncgk::dots(p->stream, n, nr, 1, u, v, 1, p->partial, hn);
-            }
-            ncgk::scale_norm(p->stream, n, nr, hn, W, V + 2 * (size_t)(k + 1) * vlen);
+            ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h1, p->partial, p->partial_cap, nullptr);
+            ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h2, p->partial, p->partial_cap, p->npart);
+            ncgk::gs_normalize(p->stream, n, nr, p->npart, W, V + 2 * (size_t)(k + 1) * vlen, hn);
             const size_t ncol = 2 * (size_t)nr * (k + 1) + 3 * (size_t)nr;
             hcol.resize(ncol);
             NCG_HIP(hipMemcpyAsync(hcol.data(), h1, sizeof(cd) * ncol, hipMemcpyDeviceToHost, p->stream));
@@ -1473,7 +1497,7 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     // 256 slices for 8 basis vectors of 2 load cases in GMRES projections
     p->partial_cap = std::max<size_t>((size_t)nb * 3 * 64, 4096);
     if ((rc = dalloc(&p->partial, 2 * p->partial_cap)) || (rc = dalloc(&p->dotout, (size_t)2 * 3 * 64 * 256)) ||
-        (rc = dalloc(&p->scal, (size_t)2 * 8 * 64 * 2)))
+        (rc = dalloc(&p->scal, (size_t)2 * 8 * 64 * 2)) || (rc = dalloc(&p->npart, (size_t)64 * 256)))
         return bail(rc);
     hipStream_t s = p->stream;
     hipMemcpyAsync(p->X, Xh.data(), sizeof(double) * Xh.size(), hipMemcpyHostToDevice, s);
@@ -1497,7 +1521,7 @@ int ncg_problem_destroy(ncg_problem *p)
                     (void *)p->nv, (void *)p->kind, (void *)p->near_rows, (void *)p->near_cols, p->A, (void *)p->adm,
                     (void *)p->bcval, (void *)p->rhs, (void *)p->rhs_src, (void *)p->x, (void *)p->ps, (void *)p->pw,
                     (void *)p->src, (void *)p->err, (void *)p->evals, (void *)p->scal, (void *)p->partial,
-                    (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis})
+                    (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis, (void *)p->npart})
         if (q) hipFree(q);
     for (double *w : p->work) hipFree(w);
     p->fmm.release();

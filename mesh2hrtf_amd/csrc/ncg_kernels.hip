@@ -348,14 +348,28 @@ __global__ void __launch_bounds__(256) k_dots(int n, int nrhs, int ndots, const 
         }
 }
 
-// out[r][q] = sum over blocks of partial; one block.
-__global__ void k_reduce_partials(int nblocks, int nrhs, int ndots, const cx<double> *partial, cx<double> *out)
+// out[r][q] = sum over blocks of partial; one block of 256 threads, nblocks <= 256.  Each
+// sum is a fixed tree over the block partials (deterministic).
+__global__ void __launch_bounds__(256) k_reduce_partials(int nblocks, int nrhs, int ndots, const cx<double> *partial,
+                                                         cx<double> *out)
 {
-    for (int t = threadIdx.x; t < nrhs * ndots; t += blockDim.x) {
-        const int r = t / ndots, q = t % ndots;
-        cx<double> s = mk(0.0, 0.0);
-        for (int b = 0; b < nblocks; ++b) s += partial[((size_t)b * nrhs + r) * 3 + q];
-        out[r * 3 + q] = s;
+    __shared__ double sre[256], sim[256];
+    const int t = threadIdx.x;
+    for (int rq = 0; rq < nrhs * ndots; ++rq) {
+        const int r = rq / ndots, q = rq % ndots;
+        const cx<double> v = t < nblocks ? partial[((size_t)t * nrhs + r) * 3 + q] : mk(0.0, 0.0);
+        sre[t] = v.re;
+        sim[t] = v.im;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) {
+                sre[t] += sre[t + s];
+                sim[t] += sim[t + s];
+            }
+            __syncthreads();
+        }
+        if (t == 0) out[r * 3 + q] = mk(sre[0], sim[0]);
+        __syncthreads();
     }
 }
 
@@ -528,6 +542,61 @@ __global__ void k_multiaxpy(int n, int nrhs, int kcount, const cx<double> *V, co
     cx<double> s = w[t];
     for (int k = 0; k < kcount; ++k) s -= V[((size_t)k * nrhs + r) * n + i] * h[(size_t)r * kcount + k];
     w[t] = s;
+}
+
+// One Gram-Schmidt pass of GMRES after k_multidot: h[r][k] = sum over the slices of
+// part (fixed order), w[r] -= sum_k V_k[r] h[r][k].  Block (x, r); block x = 0 stores
+// h[r][:] for the host.  With npart, block x also writes its partial sum of |w_new|^2
+// to npart[r][x] (gridDim.x <= 256).
+constexpr int kGsMaxBasis = 256;
+__global__ void __launch_bounds__(256) k_gs_update(int n, int nrhs, int kcount, int nsl, const cx<double> *part,
+                                                   const cx<double> *V, cx<double> *w, cx<double> *hout, double *npart)
+{
+    __shared__ cx<double> hs[kGsMaxBasis];
+    __shared__ double red[256];
+    const int r = blockIdx.y, t = threadIdx.x;
+    for (int k = t; k < kcount; k += 256) {
+        cx<double> s = mk(0.0, 0.0);
+        for (int q = 0; q < nsl; ++q) s += part[((size_t)k * nrhs + r) * nsl + q];
+        hs[k] = s;
+        if (blockIdx.x == 0) hout[(size_t)r * kcount + k] = s;
+    }
+    __syncthreads();
+    double nn = 0.0;
+    for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) {
+        cx<double> s = w[(size_t)r * n + i];
+        for (int k = 0; k < kcount; ++k) s -= V[((size_t)k * nrhs + r) * n + i] * hs[k];
+        w[(size_t)r * n + i] = s;
+        nn += s.re * s.re + s.im * s.im;
+    }
+    if (!npart) return;
+    red[t] = nn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) npart[(size_t)r * gridDim.x + blockIdx.x] = red[0];
+}
+
+// v[r] = w[r] / sqrt(sum_x npart[r][x]) after k_gs_update; block x = 0 stores the
+// squared norm as nout[r][0] (the layout of a dots() result).
+__global__ void __launch_bounds__(256) k_gs_normalize(int n, int nb, const double *npart, const cx<double> *w,
+                                                      cx<double> *v, cx<double> *nout)
+{
+    __shared__ double red[256];
+    const int r = blockIdx.y, t = threadIdx.x;
+    red[t] = t < nb ? npart[(size_t)r * nb + t] : 0.0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double nrm2 = red[0];
+    if (blockIdx.x == 0 && t == 0) nout[3 * r] = mk(nrm2, 0.0);
+    const double sc = sqrt(nrm2);
+    const double inv = sc > 1e-300 ? 1.0 / sc : 0.0;
+    for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) v[(size_t)r * n + i] = w[(size_t)r * n + i] * inv;
 }
 
 // x[r] += sum_k V_k[r] y[r][k]
@@ -1281,6 +1350,7 @@ __global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_blk, c
 
 // In-place Gauss-Jordan inversion with partial (row) pivoting, one work-group per block,
 // the block held in LDS; the row interchanges are undone as column interchanges at the end.
+// The inverse is written back column-major.
 __global__ void __launch_bounds__(256) k_bj_invert(const int *blk_m, const long long *boff, cx<double> *blk, int *err)
 {
     __shared__ cx<double> A[kBlk * kBlk];
@@ -1344,36 +1414,60 @@ __global__ void __launch_bounds__(256) k_bj_invert(const int *blk_m, const long 
             }
         __syncthreads();
     }
-    for (int t = tid; t < m * m; t += blockDim.x) G[t] = A[(t / m) * kBlk + t % m];
+    // column-major: G[j * m + i] = inverse[i][j]
+    for (int t = tid; t < m * m; t += blockDim.x) G[t] = A[(t % m) * kBlk + t / m];
 }
 
-// y = M^-1 x: one wave per row
+// y = M^-1 x: one work-group per block; lane a is row a, the 4 waves split the columns
+// (four per step) of the column-major inverse and meet in LDS
 template <int NR>
-__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *elem_blk, const int *elem_pos, const int *blk_b0,
-                                                  const int *blk_m, const int *cl_elem, const long long *boff,
-                                                  const cx<double> *minv, const cx<double> *x, cx<double> *y)
+__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *blk_b0, const int *blk_m, const int *cl_elem,
+                                                  const long long *boff, const cx<double> *minv, const cx<double> *x,
+                                                  cx<double> *y)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    const int b = elem_blk[i], b0 = blk_b0[b], m = blk_m[b], a = elem_pos[i] - b0;
-    const cx<double> *row = minv + boff[b] + (size_t)a * m;
+    __shared__ cx<double> xs[NR][kBlk];
+    __shared__ double red[3][kBlk][2 * NR];
+    const int b = blockIdx.x, b0 = blk_b0[b], m = blk_m[b];
+    const int a = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int t = threadIdx.x; t < NR * m; t += 256) {
+        const int r = t / m, j = t - r * m;
+        xs[r][j] = x[(size_t)r * n + cl_elem[b0 + j]];
+    }
+    __syncthreads();
+    const cx<double> *M = minv + boff[b];
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
-    for (int j = lane; j < m; j += 64) {
-        const cx<double> v = row[j];
-        const int e = cl_elem[b0 + j];
+    if (a < m) {
+        int j = wv;
+        for (; j + 12 < m; j += 16) {
+            cx<double> v[4];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, x[(size_t)r * n + e]);
-    }
+            for (int q = 0; q < 4; ++q) v[q] = M[(size_t)(j + 4 * q) * m + a];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        for (int o = 32; o > 0; o >>= 1) {
-            re[r] += __shfl_xor(re[r], o);
-            im[r] += __shfl_xor(im[r], o);
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v[q].re, v[q].im, xs[r][j + 4 * q]);
         }
-        if (lane == 0) y[(size_t)r * n + i] = mk(re[r], im[r]);
+        for (; j < m; j += 4) {
+            const cx<double> v = M[(size_t)j * m + a];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, xs[r][j]);
+        }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            red[wv - 1][a][2 * r] = re[r];
+            red[wv - 1][a][2 * r + 1] = im[r];
+        }
+    __syncthreads();
+    if (wv == 0 && a < m) {
+        const int e = cl_elem[b0 + a];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            y[(size_t)r * n + e] = mk(re[r] + red[0][a][2 * r] + red[1][a][2 * r] + red[2][a][2 * r],
+                                      im[r] + red[0][a][2 * r + 1] + red[1][a][2 * r + 1] + red[2][a][2 * r + 1]);
     }
 }
 
@@ -1391,40 +1485,90 @@ __global__ void k_scale_norm(int n, int nrhs, const cx<double> *dts, const cx<do
 // Each kernel handles NR (1..4) load cases per pass so the operator is read once for
 // all of them; vectors are [r][...] with the given strides.
 
-// near field: one wave per row, lanes over the row's nonzeros
-template <typename S, int NR>
-__global__ void __launch_bounds__(256) k_near_spmv(int n, const long long *ptr, const int *col, const cx<S> *val,
-                                                   const cx<double> *x, cx<double> *y)
+// Row chunks: the kernels below give each work-group a chunk of at most 16 rows that
+// share one cluster (ch_cl, ch_r0 = first position in the cluster's element list, ch_m
+// = rows).  Each of the 4 waves owns rows a = wave + 4 j and streams their values with
+// four independent loads per lane per step; the vector the rows multiply is staged in
+// LDS once per work-group.
+static_assert(ncgk::kChunkRows == 16, "4 waves x 4 rows per chunk");
+
+// sum over the wave of re/im for four rows and NR load cases, then row a gets lane 0's value
+template <int NR>
+__device__ __forceinline__ void wave_sum4(double (&re)[4][NR], double (&im)[4][NR])
 {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    double re[NR], im[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
-    const long long b = ptr[i], e = ptr[i + 1];
-    for (long long k = b + lane; k < e; k += 64) {
-        const cx<S> a = val[k];
-        const int j = col[k];
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + j]);
+        for (int r = 0; r < NR; ++r)
+            for (int o = 32; o > 0; o >>= 1) {
+                re[j][r] += __shfl_xor(re[j][r], o);
+                im[j][r] += __shfl_xor(im[j][r], o);
+            }
+}
+
+// near field: all rows of a finest cluster hold the same columns (the elements of its
+// near clusters), so a chunk gathers x over that column list into LDS (in segments of
+// seg columns) and the values are the only per-nonzero traffic.
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch_cl, const int *ch_r0, const int *ch_m,
+                                                   const int *cl_ptr, const int *cl_elem, const long long *ptr,
+                                                   const int *col, const cx<S> *val, const cx<double> *x, cx<double> *y)
+{
+    extern __shared__ double xs_sh[];   // [NR][seg] complex
+    cx<double> *xs = (cx<double> *)xs_sh;
+    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long cb = ptr[cl_elem[cl_ptr[c]]];
+    const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
+    int row[4];
+    const cx<S> *vr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int a = wv + 4 * j;
+        row[j] = a < m ? cl_elem[r0 + a] : -1;
+        vr[j] = val + (row[j] >= 0 ? ptr[row[j]] : cb);
     }
+    double re[4][NR], im[4][NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        for (int o = 32; o > 0; o >>= 1) {
-            re[r] += __shfl_xor(re[r], o);
-            im[r] += __shfl_xor(im[r], o);
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
+    for (int s0 = 0; s0 < w; s0 += seg) {
+        const int sl = min(seg, w - s0);
+        __syncthreads();
+        for (int u = threadIdx.x; u < sl; u += 256) {
+            const int e = col[cb + s0 + u];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) xs[r * seg + u] = x[(size_t)r * n + e];
         }
-        if (lane == 0) y[(size_t)r * n + i] = mk(re[r], im[r]);
+        __syncthreads();
+        for (int u = lane; u < sl; u += 64) {
+            cx<S> v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = vr[j][s0 + u];   // unused slots read the first row
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const cx<double> xv = xs[r * seg + u];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[j].re), double(v[j].im), xv);
+            }
+        }
     }
+    wave_sum4<NR>(re, im);
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (row[j] >= 0)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) y[(size_t)r * n + row[j]] = mk(re[j][r], im[j][r]);
 }
 
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
-// of 64 sphere points); its 4 waves take every 4th element of the cluster and their
-// partial sums meet in LDS, so a cluster's ~100 elements are read by 256 lanes at once.
-template <typename S, int NR>
+// of 64 sphere points); its 4 waves take every 4th element of the cluster, four elements
+// per step, and their partial sums meet in LDS.  W is the storage type of t.
+template <typename S, typename W, int NR>
 __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
-                                                     const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
+                                                     const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
 {
     __shared__ double red[3][64][2 * NR];
     const int tiles = (P + 63) / 64;
@@ -1438,14 +1582,19 @@ __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, cons
     const int k1 = cl_ptr[c + 1];
     int k = cl_ptr[c] + wv;
     if (on) {
-        for (; k + 4 < k1; k += 8) {
-            const cx<S> a0 = Tv[(size_t)k * P + p], a1 = Tv[(size_t)(k + 4) * P + p];
-            const int e0 = cl_elem[k], e1 = cl_elem[k + 4];
+        for (; k + 12 < k1; k += 16) {
+            cx<S> a[4];
+            int e[4];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                cmac(re[r], im[r], double(a0.re), double(a0.im), x[(size_t)r * n + e0]);
-                cmac(re[r], im[r], double(a1.re), double(a1.im), x[(size_t)r * n + e1]);
+            for (int j = 0; j < 4; ++j) {
+                a[j] = Tv[(size_t)(k + 4 * j) * P + p];
+                e[j] = cl_elem[k + 4 * j];
             }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), x[(size_t)r * n + e[j]]);
         }
         for (; k < k1; k += 4) {
             const cx<S> a = Tv[(size_t)k * P + p];
@@ -1466,16 +1615,17 @@ __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, cons
         for (int r = 0; r < NR; ++r) {
             const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
             const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
-            tw[(size_t)r * ncl * P + (size_t)c * P + p] = mk(sr, si);
+            tw[(size_t)r * ncl * P + (size_t)c * P + p] = cx<W>{W(sr), W(si)};
         }
     }
 }
 
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
-// per (cluster, tile of 64 points), the 4 waves splitting the interaction list
-template <typename S, int NR>
+// per (cluster, tile of 64 points), the 4 waves splitting the interaction list four
+// entries per step.  WI / WO are the storage types of t and s.
+template <typename S, typename WI, typename WO, int NR>
 __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
-                                                     const cx<S> *Dv, const cx<double> *tw, cx<double> *sw)
+                                                     const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
 {
     __shared__ double red[3][64][2 * NR];
     const int tiles = (P + 63) / 64;
@@ -1490,20 +1640,30 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
     const int q1 = fan_ptr[i + 1];
     int q = fan_ptr[i] + wv;
     if (on) {
-        for (; q + 4 < q1; q += 8) {
-            const cx<S> a0 = Dv[(size_t)q * P + p], a1 = Dv[(size_t)(q + 4) * P + p];
-            const int j0 = fan_idx[q], j1 = fan_idx[q + 4];
+        for (; q + 12 < q1; q += 16) {
+            cx<S> a[4];
+            int jj[4];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                cmac(re[r], im[r], double(a0.re), double(a0.im), tw[r * ld + (size_t)j0 * P + p]);
-                cmac(re[r], im[r], double(a1.re), double(a1.im), tw[r * ld + (size_t)j1 * P + p]);
+            for (int j = 0; j < 4; ++j) {
+                a[j] = Dv[(size_t)(q + 4 * j) * P + p];
+                jj[j] = fan_idx[q + 4 * j];
             }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const cx<WI> t = tw[r * ld + (size_t)jj[j] * P + p];
+                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), mk(double(t.re), double(t.im)));
+                }
         }
         for (; q < q1; q += 4) {
             const cx<S> a = Dv[(size_t)q * P + p];
             const int j = fan_idx[q];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), tw[r * ld + (size_t)j * P + p]);
+            for (int r = 0; r < NR; ++r) {
+                const cx<WI> t = tw[r * ld + (size_t)j * P + p];
+                cmac(re[r], im[r], double(a.re), double(a.im), mk(double(t.re), double(t.im)));
+            }
         }
     }
     if (wv > 0)
@@ -1518,37 +1678,64 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
         for (int r = 0; r < NR; ++r) {
             const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
             const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
-            sw[r * ld + (size_t)i * P + p] = mk(sr, si);
+            sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(sr), WO(si)};
         }
     }
 }
 
-// y[e] += sum_p S[e, p] s[cluster(e), p]; one wave per element
-template <typename S, int NR>
-__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, const int *elem_cl, const cx<S> *Sv,
-                                                     const cx<double> *sw, cx<double> *y)
+// y[e] += sum_p S[e, p] s[cluster(e), p] over a row chunk of one cluster, s[c, :] staged
+// in LDS ([NR][seg] complex FP64, seg points at a time)
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int seg, const int *ch_cl, const int *ch_r0,
+                                                     const int *ch_m, const int *cl_elem, const cx<S> *Sv,
+                                                     const cx<W> *sw, cx<double> *y)
 {
-    const int lane = threadIdx.x & 63;
-    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (e >= n) return;
-    const int c = elem_cl[e];
+    extern __shared__ double sv_sh[];
+    cx<double> *sv = (cx<double> *)sv_sh;
+    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const size_t ld = (size_t)ncl * P;
-    double re[NR], im[NR];
+    int row[4];
+    const cx<S> *sr[4];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
-    for (int p = lane; p < P; p += 64) {
-        const cx<S> a = Sv[(size_t)e * P + p];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), sw[r * ld + (size_t)c * P + p]);
+    for (int j = 0; j < 4; ++j) {
+        const int a = wv + 4 * j;
+        row[j] = a < m ? cl_elem[r0 + a] : -1;
+        sr[j] = Sv + (size_t)(row[j] >= 0 ? row[j] : 0) * P;
     }
+    double re[4][NR], im[4][NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        for (int o = 32; o > 0; o >>= 1) {
-            re[r] += __shfl_xor(re[r], o);
-            im[r] += __shfl_xor(im[r], o);
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
+    for (int p0 = 0; p0 < P; p0 += seg) {
+        const int sl = min(seg, P - p0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < NR * sl; t += 256) {
+            const int r = t / sl, u = t - r * sl;
+            const cx<W> v = sw[r * ld + (size_t)c * P + p0 + u];
+            sv[r * seg + u] = mk(double(v.re), double(v.im));
         }
-        if (lane == 0) y[(size_t)r * n + e] += mk(re[r], im[r]);
+        __syncthreads();
+        for (int u = lane; u < sl; u += 64) {
+            cx<S> v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = sr[j][p0 + u];   // unused slots read row 0
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const cx<double> s = sv[r * seg + u];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[j].re), double(v[j].im), s);
+            }
+        }
     }
+    wave_sum4<NR>(re, im);
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (row[j] >= 0)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) y[(size_t)r * n + row[j]] += mk(re[j][r], im[j][r]);
 }
 
 // ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
@@ -1895,17 +2082,37 @@ void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val
     if (mixed) k_row_scaling_csr<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<float> *)val, d);
     else k_row_scaling_csr<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<double> *)val, d);
 }
+// slices of k_multidot: about 4096 entries each, as many as the partial buffer holds
+static int multidot_slices(int n, int nrhs, int kcount, size_t part_cap)
+{
+    const int nsl = std::max(1, (n + 4095) / 4096);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nsl, part_cap / ((size_t)kcount * nrhs)));
+}
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h, void *part,
               size_t part_cap)
 {
-    // slices of about 4096 entries, as many as the partial buffer holds
-    int nsl = std::max(1, (n + 4095) / 4096);
-    nsl = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsl, part_cap / ((size_t)kcount * nrhs)));
+    const int nsl = multidot_slices(n, nrhs, kcount, part_cap);
     const int slice = (n + nsl - 1) / nsl;
     k_multidot<<<dim3(kcount, nrhs, nsl), 256, 0, s>>>(n, nrhs, kcount, slice, (const cx<double> *)V,
                                                        (const cx<double> *)w, (cx<double> *)part);
     k_multidot_sum<<<blocks_for((size_t)nrhs * kcount, 256), 256, 0, s>>>(nrhs, kcount, nsl, (const cx<double> *)part,
                                                                            (cx<double> *)h);
+}
+static unsigned gs_blocks(int n) { return std::min<unsigned>(256u, blocks_for(n, 256)); }
+void gs_pass(hipStream_t s, int n, int nrhs, int kcount, const void *V, void *w, void *h, void *part, size_t part_cap,
+             double *npart)
+{
+    const int nsl = multidot_slices(n, nrhs, kcount, part_cap);
+    const int slice = (n + nsl - 1) / nsl;
+    k_multidot<<<dim3(kcount, nrhs, nsl), 256, 0, s>>>(n, nrhs, kcount, slice, (const cx<double> *)V,
+                                                       (const cx<double> *)w, (cx<double> *)part);
+    k_gs_update<<<dim3(gs_blocks(n), nrhs), 256, 0, s>>>(n, nrhs, kcount, nsl, (const cx<double> *)part,
+                                                         (const cx<double> *)V, (cx<double> *)w, (cx<double> *)h, npart);
+}
+void gs_normalize(hipStream_t s, int n, int nrhs, const double *npart, const void *w, void *v, void *nout)
+{
+    k_gs_normalize<<<dim3(gs_blocks(n), nrhs), 256, 0, s>>>(n, (int)gs_blocks(n), npart, (const cx<double> *)w,
+                                                            (cx<double> *)v, (cx<double> *)nout);
 }
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add)
 {
@@ -2070,19 +2277,18 @@ void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *el
                                                              (cx<double> *)blk);
     k_bj_invert<<<nblk, 256, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
 }
-void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_blk, const int *elem_pos, const int *blk_b0,
-              const int *blk_m, const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y)
+void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
+              const long long *boff, const void *minv, const void *x, void *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
         const cx<double> *M = (const cx<double> *)minv;
-        const unsigned g = blocks_for(n, 4);
         switch (nr) {
-        case 1: k_bj_apply<1><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 2: k_bj_apply<2><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 3: k_bj_apply<3><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        default: k_bj_apply<4><<<g, 256, 0, s>>>(n, elem_blk, elem_pos, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 1: k_bj_apply<1><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 2: k_bj_apply<2><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 3: k_bj_apply<3><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        default: k_bj_apply<4><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
         }
     });
 }
@@ -2092,81 +2298,95 @@ void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, 
                                                                     (const cx<double> *)w, (cx<double> *)v);
 }
 
-#define NCG_NR_DISPATCH(NR, KERNEL, T, GRID, ...)                                       \
+#define NCG_TARGS(...) __VA_ARGS__
+#define NCG_NR_DISPATCH_SH(NR, KERNEL, TARGS, GRID, SH, ...)                            \
     switch (NR) {                                                                       \
-    case 1: KERNEL<T, 1><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
-    case 2: KERNEL<T, 2><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
-    case 3: KERNEL<T, 3><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                      \
-    default: KERNEL<T, 4><<<GRID, 256, 0, s>>>(__VA_ARGS__); break;                     \
+    case 1: KERNEL<TARGS, 1><<<GRID, 256, (SH) * 1, s>>>(__VA_ARGS__); break;           \
+    case 2: KERNEL<TARGS, 2><<<GRID, 256, (SH) * 2, s>>>(__VA_ARGS__); break;           \
+    case 3: KERNEL<TARGS, 3><<<GRID, 256, (SH) * 3, s>>>(__VA_ARGS__); break;           \
+    default: KERNEL<TARGS, 4><<<GRID, 256, (SH) * 4, s>>>(__VA_ARGS__); break;          \
     }
 
+// LDS budgets of one chunk: seg columns (points) x NR load cases x 16 bytes
+constexpr int kNearLdsBytes = 32768, kSLdsBytes = 32768;
+
 template <typename S>
-static void near_spmv_t(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const cx<S> *val,
-                        const cx<double> *x, cx<double> *y)
+static void near_spmv_t(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
+                        const long long *ptr, const int *col, const cx<S> *val, const cx<double> *x, cx<double> *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH(nr, k_near_spmv, S, blocks_for(n, 4), n, ptr, col, val, x + (size_t)r0 * n,
-                        y + (size_t)r0 * n);
+        const int seg = kNearLdsBytes / (16 * nr);
+        NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, rc.n, 16 * seg, n, seg, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+                           val, x + (size_t)r0 * n, y + (size_t)r0 * n);
     });
 }
-void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
-               const void *x, void *y)
+void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
+               const long long *ptr, const int *col, const void *val, int mixed, const void *x, void *y)
 {
-    if (mixed) near_spmv_t(s, n, nrhs, ptr, col, (const cx<float> *)val, (const cx<double> *)x, (cx<double> *)y);
-    else near_spmv_t(s, n, nrhs, ptr, col, (const cx<double> *)val, (const cx<double> *)x, (cx<double> *)y);
+    if (mixed)
+        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, (const cx<float> *)val, (const cx<double> *)x,
+                    (cx<double> *)y);
+    else
+        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, (const cx<double> *)val, (const cx<double> *)x,
+                    (cx<double> *)y);
 }
-template <typename S>
+template <typename S, typename W>
 static void apply_T_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem,
-                      const cx<S> *Tv, const cx<double> *x, cx<double> *tw)
+                      const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
 {
     const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH(nr, k_fmm_apply_T, S, g, n, ncl, P, cl_ptr, cl_elem, Tv, x + (size_t)r0 * n,
-                        tw + (size_t)r0 * ncl * P);
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_T, NCG_TARGS(S, W), g, 0, n, ncl, P, cl_ptr, cl_elem, Tv,
+                           x + (size_t)r0 * n, tw + (size_t)r0 * ncl * P);
     });
 }
 void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
-                 int mixed, const void *x, void *tw)
+                 int f32, int tw_f64, const void *x, void *tw)
 {
-    if (mixed)
+    if (f32 && tw_f64)
         apply_T_t(s, n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<float> *)Tv, (const cx<double> *)x, (cx<double> *)tw);
+    else if (f32)
+        apply_T_t(s, n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<float> *)Tv, (const cx<double> *)x, (cx<float> *)tw);
     else
         apply_T_t(s, n, ncl, P, nrhs, cl_ptr, cl_elem, (const cx<double> *)Tv, (const cx<double> *)x, (cx<double> *)tw);
 }
-template <typename S>
+template <typename S, typename WI, typename WO>
 static void apply_D_t(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const cx<S> *Dv,
-                      const cx<double> *tw, cx<double> *sw)
+                      const cx<WI> *tw, cx<WO> *sw)
 {
     const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH(nr, k_fmm_apply_D, S, g, ncl, P, fan_ptr, fan_idx, Dv, tw + (size_t)r0 * ncl * P,
-                        sw + (size_t)r0 * ncl * P);
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_D, NCG_TARGS(S, WI, WO), g, 0, ncl, P, fan_ptr, fan_idx, Dv,
+                           tw + (size_t)r0 * ncl * P, sw + (size_t)r0 * ncl * P);
     });
 }
 void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
-                 int mixed, const void *tw, void *sw)
+                 int f32, int tw_f64, const void *tw, void *sw)
 {
-    if (mixed)
-        apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv, (const cx<double> *)tw, (cx<double> *)sw);
+    if (f32 && tw_f64)
+        apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv, (const cx<double> *)tw, (cx<float> *)sw);
+    else if (f32)
+        apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<float> *)Dv, (const cx<float> *)tw, (cx<float> *)sw);
     else
         apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<double> *)Dv, (const cx<double> *)tw, (cx<double> *)sw);
 }
-template <typename S>
-static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const cx<S> *Sv,
-                      const cx<double> *sw, cx<double> *y)
+template <typename S, typename W>
+static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
+                      const cx<S> *Sv, const cx<W> *sw, cx<double> *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH(nr, k_fmm_apply_S, S, blocks_for(n, 4), n, ncl, P, elem_cl, Sv, sw + (size_t)r0 * ncl * P,
-                        y + (size_t)r0 * n);
+        const int seg = std::min(P, kSLdsBytes / (16 * nr));
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_S, NCG_TARGS(S, W), rc.n, 16 * seg, n, ncl, P, seg, rc.cl, rc.r0, rc.m,
+                           cl_elem, Sv, sw + (size_t)r0 * ncl * P, y + (size_t)r0 * n);
     });
 }
-void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const void *Sv, int mixed,
-                 const void *sw, void *y)
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
+                 const void *Sv, int f32, const void *sw, void *y)
 {
-    if (mixed)
-        apply_S_t(s, n, ncl, P, nrhs, elem_cl, (const cx<float> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
+    if (f32)
+        apply_S_t(s, n, ncl, P, nrhs, rc, cl_elem, (const cx<float> *)Sv, (const cx<float> *)sw, (cx<double> *)y);
     else
-        apply_S_t(s, n, ncl, P, nrhs, elem_cl, (const cx<double> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
+        apply_S_t(s, n, ncl, P, nrhs, rc, cl_elem, (const cx<double> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
 }
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,

@@ -65,6 +65,12 @@ void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val
 // h[r][k] = <V_k, w_r>; part: scratch of part_cap complex values for slice partials
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h, void *part,
               size_t part_cap);
+// one classical Gram-Schmidt pass of GMRES: h[r][k] = <V_k[r], w[r]>, w -= V h (h to the
+// device buffer h, [r][kcount]); with npart also the partial sums of |w|^2 per block
+void gs_pass(hipStream_t s, int n, int nrhs, int kcount, const void *V, void *w, void *h, void *part, size_t part_cap,
+             double *npart);
+// v = w / ||w|| from gs_pass's partial sums; the squared norm also goes to nout[r][0]
+void gs_normalize(hipStream_t s, int n, int nrhs, const double *npart, const void *w, void *v, void *nout);
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add);
 // incomplete LU of the row-scaled operator (dense A when ptr == nullptr, else the
 // near-field CSR ptr/ncol/A); pattern CSR rp/col/val, level schedules from the host
@@ -108,20 +114,31 @@ void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const 
 void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
               const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
               const void *nval, int mixed, const long long *boff, void *blk, int *err);
-void bj_apply(hipStream_t s, int n, int nrhs, const int *elem_blk, const int *elem_pos, const int *blk_b0,
-              const int *blk_m, const int *cl_elem, const long long *boff, const void *minv, const void *x, void *y);
+void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
+              const long long *boff, const void *minv, const void *x, void *y);
 // v = w / ||w|| per load case, the norm read from a dots() result
 void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v);
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag);
-void near_spmv(hipStream_t s, int n, int nrhs, const long long *ptr, const int *col, const void *val, int mixed,
-               const void *x, void *y);
+// Row chunks of the FMM kernels: chunk q holds m[q] <= kChunkRows rows of cluster cl[q], starting
+// at position r0[q] of the level's element list (cl_elem).
+constexpr int kChunkRows = 16;
+struct RowChunks {
+    int n = 0;
+    const int *cl = nullptr, *r0 = nullptr, *m = nullptr;
+};
+// near field of the finest level (CSR whose rows in one cluster share their columns)
+void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
+               const long long *ptr, const int *col, const void *val, int mixed, const void *x, void *y);
+// far field of one level: f32 selects FP32 T, D, S and FP32 storage of t and s
+// tw_f64: t is FP64 whatever f32 says (signatures for the evaluation grid and the
+// right-hand side's sources)
 void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem, const void *Tv,
-                 int mixed, const void *x, void *tw);
+                 int f32, int tw_f64, const void *x, void *tw);
 void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
-                 int mixed, const void *tw, void *sw);
-void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const int *elem_cl, const void *Sv, int mixed,
-                 const void *sw, void *y);
+                 int f32, int tw_f64, const void *tw, void *sw);
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
+                 const void *Sv, int f32, const void *sw, void *y);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
                   void *tv, int fp32);
