@@ -1506,6 +1506,35 @@ __device__ __forceinline__ void wave_sum4(double (&re)[4][NR], double (&im)[4][N
             }
 }
 
+// re/im[j][r] += this lane's share (u = lane + 64 t < sl) of sum_u v_j[u] * xs[r][u] for
+// four rows v_j and NR vectors in LDS (row stride seg).  U steps per batch with every
+// load issued first; indices past sl are clamped and weighted by zero.
+template <typename S, int NR, int U>
+__device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, const cx<double> *xs, int seg, int lane,
+                                          double (&re)[4][NR], double (&im)[4][NR])
+{
+    for (int u0 = lane; u0 < sl; u0 += 64 * U) {
+        cx<S> v[U][4];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int u = min(u0 + 64 * q, sl - 1);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[q][j] = vr[j][u];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int u = u0 + 64 * q;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                cx<double> xv = xs[r * seg + min(u, sl - 1)];
+                if (u >= sl) xv = mk(0.0, 0.0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[q][j].re), double(v[q][j].im), xv);
+            }
+        }
+    }
+}
+
 // near field: all rows of a finest cluster hold the same columns (the elements of its
 // near clusters), so a chunk gathers x over that column list into LDS (in segments of
 // seg columns) and the values are the only per-nonzero traffic.
@@ -1517,7 +1546,8 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch
     extern __shared__ double xs_sh[];   // [NR][seg] complex
     cx<double> *xs = (cx<double> *)xs_sh;
     const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long cb = ptr[cl_elem[cl_ptr[c]]];
     const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
     int row[4];
@@ -1542,17 +1572,9 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch
             for (int r = 0; r < NR; ++r) xs[r * seg + u] = x[(size_t)r * n + e];
         }
         __syncthreads();
-        for (int u = lane; u < sl; u += 64) {
-            cx<S> v[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = vr[j][s0 + u];   // unused slots read the first row
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const cx<double> xv = xs[r * seg + u];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[j].re), double(v[j].im), xv);
-            }
-        }
+        // unused row slots read the first row of the cluster and are not stored
+        const cx<S> *vs[4] = {vr[0] + s0, vr[1] + s0, vr[2] + s0, vr[3] + s0};
+        rows4_dot<S, NR, 2>(vs, sl, xs, seg, lane, re, im);
     }
     wave_sum4<NR>(re, im);
     if (lane == 0)
@@ -1573,7 +1595,8 @@ __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, cons
     __shared__ double red[3][64][2 * NR];
     const int tiles = (P + 63) / 64;
     const int c = blockIdx.x / tiles, tile = blockIdx.x % tiles;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
     double re[NR], im[NR];
@@ -1630,7 +1653,8 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
     __shared__ double red[3][64][2 * NR];
     const int tiles = (P + 63) / 64;
     const int i = blockIdx.x / tiles, tile = blockIdx.x % tiles;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
     const size_t ld = (size_t)ncl * P;
@@ -1693,7 +1717,8 @@ __global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int 
     extern __shared__ double sv_sh[];
     cx<double> *sv = (cx<double> *)sv_sh;
     const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t ld = (size_t)ncl * P;
     int row[4];
     const cx<S> *sr[4];
@@ -1717,17 +1742,9 @@ __global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int 
             sv[r * seg + u] = mk(double(v.re), double(v.im));
         }
         __syncthreads();
-        for (int u = lane; u < sl; u += 64) {
-            cx<S> v[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = sr[j][p0 + u];   // unused slots read row 0
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const cx<double> s = sv[r * seg + u];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[j].re), double(v[j].im), s);
-            }
-        }
+        // unused row slots read row 0 and are not stored
+        const cx<S> *ss[4] = {sr[0] + p0, sr[1] + p0, sr[2] + p0, sr[3] + p0};
+        rows4_dot<S, NR, 4>(ss, sl, sv, seg, lane, re, im);
     }
     wave_sum4<NR>(re, im);
     if (lane == 0)
