@@ -1485,6 +1485,35 @@ __global__ void k_scale_norm(int n, int nrhs, const cx<double> *dts, const cx<do
 // Each kernel handles NR (1..4) load cases per pass so the operator is read once for
 // all of them; vectors are [r][...] with the given strides.
 
+// The T and D applies sum over a cluster's elements or interaction list per sphere
+// point; they are latency-bound at these sizes (a few tens of MB per launch), so their
+// work-groups have 8 waves that split the sum and meet in LDS.
+constexpr int kSumWaves = 8;
+
+// sum over the NW waves of a work-group of each lane's re/im, by halves in LDS (red holds
+// [NW/2][64][2 NR] doubles); the result is left in wave 0
+template <int NR, int NW>
+__device__ __forceinline__ void waves_sum(double (&re)[NR], double (&im)[NR], double *red, int wv, int lane)
+{
+#pragma unroll
+    for (int h = NW / 2; h >= 1; h >>= 1) {
+        if (wv >= h && wv < 2 * h)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                red[((wv - h) * 64 + lane) * 2 * NR + 2 * r] = re[r];
+                red[((wv - h) * 64 + lane) * 2 * NR + 2 * r + 1] = im[r];
+            }
+        __syncthreads();
+        if (wv < h)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                re[r] += red[(wv * 64 + lane) * 2 * NR + 2 * r];
+                im[r] += red[(wv * 64 + lane) * 2 * NR + 2 * r + 1];
+            }
+        __syncthreads();
+    }
+}
+
 // Row chunks: the kernels below give each work-group a chunk of at most 16 rows that
 // share one cluster (ch_cl, ch_r0 = first position in the cluster's element list, ch_m
 // = rows).  Each of the 4 waves owns rows a = wave + 4 j and streams their values with
@@ -1586,16 +1615,17 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch
 }
 
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
-// of 64 sphere points); its 4 waves take every 4th element of the cluster, four elements
+// of 64 sphere points); its waves take every kSumWaves-th element of the cluster, four
 // per step, and their partial sums meet in LDS.  W is the storage type of t.
 template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
-                                                     const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr,
+                                                                const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
+                                                                cx<W> *tw)
 {
-    __shared__ double red[3][64][2 * NR];
+    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
+    constexpr int NW = kSumWaves;
     const int tiles = (P + 63) / 64;
     const int c = blockIdx.x / tiles, tile = blockIdx.x % tiles;
-    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
@@ -1605,13 +1635,13 @@ __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, cons
     const int k1 = cl_ptr[c + 1];
     int k = cl_ptr[c] + wv;
     if (on) {
-        for (; k + 12 < k1; k += 16) {
+        for (; k + 3 * NW < k1; k += 4 * NW) {
             cx<S> a[4];
             int e[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                a[j] = Tv[(size_t)(k + 4 * j) * P + p];
-                e[j] = cl_elem[k + 4 * j];
+                a[j] = Tv[(size_t)(k + NW * j) * P + p];
+                e[j] = cl_elem[k + NW * j];
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -1619,41 +1649,30 @@ __global__ void __launch_bounds__(256) k_fmm_apply_T(int n, int ncl, int P, cons
                 for (int r = 0; r < NR; ++r)
                     cmac(re[r], im[r], double(a[j].re), double(a[j].im), x[(size_t)r * n + e[j]]);
         }
-        for (; k < k1; k += 4) {
+        for (; k < k1; k += NW) {
             const cx<S> a = Tv[(size_t)k * P + p];
             const int e = cl_elem[k];
 #pragma unroll
             for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + e]);
         }
     }
-    if (wv > 0)
+    waves_sum<NR, NW>(re, im, red, wv, lane);
+    if (wv == 0 && on)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            red[wv - 1][lane][2 * r] = re[r];
-            red[wv - 1][lane][2 * r + 1] = im[r];
-        }
-    __syncthreads();
-    if (wv == 0 && on) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
-            const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
-            tw[(size_t)r * ncl * P + (size_t)c * P + p] = cx<W>{W(sr), W(si)};
-        }
-    }
+        for (int r = 0; r < NR; ++r) tw[(size_t)r * ncl * P + (size_t)c * P + p] = cx<W>{W(re[r]), W(im[r])};
 }
 
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
-// per (cluster, tile of 64 points), the 4 waves splitting the interaction list four
+// per (cluster, tile of 64 points), the waves splitting the interaction list four
 // entries per step.  WI / WO are the storage types of t and s.
 template <typename S, typename WI, typename WO, int NR>
-__global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
-                                                     const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
+                                                                const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
 {
-    __shared__ double red[3][64][2 * NR];
+    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
+    constexpr int NW = kSumWaves;
     const int tiles = (P + 63) / 64;
     const int i = blockIdx.x / tiles, tile = blockIdx.x % tiles;
-    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
@@ -1664,13 +1683,13 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
     const int q1 = fan_ptr[i + 1];
     int q = fan_ptr[i] + wv;
     if (on) {
-        for (; q + 12 < q1; q += 16) {
+        for (; q + 3 * NW < q1; q += 4 * NW) {
             cx<S> a[4];
             int jj[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                a[j] = Dv[(size_t)(q + 4 * j) * P + p];
-                jj[j] = fan_idx[q + 4 * j];
+                a[j] = Dv[(size_t)(q + NW * j) * P + p];
+                jj[j] = fan_idx[q + NW * j];
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -1680,7 +1699,7 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
                     cmac(re[r], im[r], double(a[j].re), double(a[j].im), mk(double(t.re), double(t.im)));
                 }
         }
-        for (; q < q1; q += 4) {
+        for (; q < q1; q += NW) {
             const cx<S> a = Dv[(size_t)q * P + p];
             const int j = fan_idx[q];
 #pragma unroll
@@ -1690,21 +1709,10 @@ __global__ void __launch_bounds__(256) k_fmm_apply_D(int ncl, int P, const int *
             }
         }
     }
-    if (wv > 0)
+    waves_sum<NR, NW>(re, im, red, wv, lane);
+    if (wv == 0 && on)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            red[wv - 1][lane][2 * r] = re[r];
-            red[wv - 1][lane][2 * r + 1] = im[r];
-        }
-    __syncthreads();
-    if (wv == 0 && on) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const double sr = re[r] + red[0][lane][2 * r] + red[1][lane][2 * r] + red[2][lane][2 * r];
-            const double si = im[r] + red[0][lane][2 * r + 1] + red[1][lane][2 * r + 1] + red[2][lane][2 * r + 1];
-            sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(sr), WO(si)};
-        }
-    }
+        for (int r = 0; r < NR; ++r) sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(re[r]), WO(im[r])};
 }
 
 // y[e] += sum_p S[e, p] s[cluster(e), p] over a row chunk of one cluster, s[c, :] staged
@@ -2316,12 +2324,12 @@ void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, 
 }
 
 #define NCG_TARGS(...) __VA_ARGS__
-#define NCG_NR_DISPATCH_SH(NR, KERNEL, TARGS, GRID, SH, ...)                            \
+#define NCG_NR_DISPATCH_SH(NR, KERNEL, TARGS, GRID, BLOCK, SH, ...)                     \
     switch (NR) {                                                                       \
-    case 1: KERNEL<TARGS, 1><<<GRID, 256, (SH) * 1, s>>>(__VA_ARGS__); break;           \
-    case 2: KERNEL<TARGS, 2><<<GRID, 256, (SH) * 2, s>>>(__VA_ARGS__); break;           \
-    case 3: KERNEL<TARGS, 3><<<GRID, 256, (SH) * 3, s>>>(__VA_ARGS__); break;           \
-    default: KERNEL<TARGS, 4><<<GRID, 256, (SH) * 4, s>>>(__VA_ARGS__); break;          \
+    case 1: KERNEL<TARGS, 1><<<GRID, BLOCK, (SH) * 1, s>>>(__VA_ARGS__); break;         \
+    case 2: KERNEL<TARGS, 2><<<GRID, BLOCK, (SH) * 2, s>>>(__VA_ARGS__); break;         \
+    case 3: KERNEL<TARGS, 3><<<GRID, BLOCK, (SH) * 3, s>>>(__VA_ARGS__); break;         \
+    default: KERNEL<TARGS, 4><<<GRID, BLOCK, (SH) * 4, s>>>(__VA_ARGS__); break;        \
     }
 
 // LDS budgets of one chunk: seg columns (points) x NR load cases x 16 bytes
@@ -2333,7 +2341,7 @@ static void near_spmv_t(hipStream_t s, int n, int nrhs, const RowChunks &rc, con
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const int seg = kNearLdsBytes / (16 * nr);
-        NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, rc.n, 16 * seg, n, seg, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+        NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, rc.n, 256, 16 * seg, n, seg, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                            val, x + (size_t)r0 * n, y + (size_t)r0 * n);
     });
 }
@@ -2353,7 +2361,7 @@ static void apply_T_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int 
 {
     const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_T, NCG_TARGS(S, W), g, 0, n, ncl, P, cl_ptr, cl_elem, Tv,
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_T, NCG_TARGS(S, W), g, 64 * kSumWaves, 0, n, ncl, P, cl_ptr, cl_elem, Tv,
                            x + (size_t)r0 * n, tw + (size_t)r0 * ncl * P);
     });
 }
@@ -2373,7 +2381,7 @@ static void apply_D_t(hipStream_t s, int ncl, int P, int nrhs, const int *fan_pt
 {
     const unsigned g = (unsigned)ncl * (unsigned)((P + 63) / 64);
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_D, NCG_TARGS(S, WI, WO), g, 0, ncl, P, fan_ptr, fan_idx, Dv,
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_D, NCG_TARGS(S, WI, WO), g, 64 * kSumWaves, 0, ncl, P, fan_ptr, fan_idx, Dv,
                            tw + (size_t)r0 * ncl * P, sw + (size_t)r0 * ncl * P);
     });
 }
@@ -2393,7 +2401,7 @@ static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const RowC
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const int seg = std::min(P, kSLdsBytes / (16 * nr));
-        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_S, NCG_TARGS(S, W), rc.n, 16 * seg, n, ncl, P, seg, rc.cl, rc.r0, rc.m,
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_S, NCG_TARGS(S, W), rc.n, 256, 16 * seg, n, ncl, P, seg, rc.cl, rc.r0, rc.m,
                            cl_elem, Sv, sw + (size_t)r0 * ncl * P, y + (size_t)r0 * n);
     });
 }
