@@ -9,19 +9,25 @@ methodBEM 4 (SL-FMM below lambda/D = 1, two-level ML-FMM above).  A step is one 
 operator assembly, the solve of both sources and the field at the evaluation grid.
 
 value        = frequency-solves/s (one solve = one frequency x one source) of the served
-               mixed-precision form: the K steps are submitted at once to the serving
+               mixed-precision form: K steps (by default three passes over the sweep) are
+               submitted at once to the serving
                runtime (ncg.Server: `lanes` steps in flight on their own streams, problems
                resident across frequencies), FP32-complex bulk operator with FP64
                near/singular integrals, FP64 Krylov dots and refinement residual, GMRES with
                the block-Jacobi preconditioner.  Only solves whose surface and grid pressures
-               are within rel-L2 1e-3 of the FP64 plain method count.
-plain_fp64   = the FP64 plain method, one step after the other as the drop-in NumCalc runs
-               them: NumCalc's method choice, CGS with its ILU preconditioner (IPRECOND 0,
-               what mesh2input writes), tolerance 1e-9.
+               are within rel-L2 1e-3 of the FP64 plain method at the same frequency count.
+plain_fp64   = the FP64 plain method over the sweep's frequencies, one step after the other
+               as the drop-in NumCalc runs them: NumCalc's method choice, CGS with its ILU
+               preconditioner (IPRECOND 0, what mesh2input writes), tolerance 1e-9.
 p50/p99      = per-request latency of single-frequency requests (both sources) arriving as a
                Poisson stream at 70% of the measured throughput.
-roofline     = the dominant kernel of the served run (HIP events around each of its launches
-               on the problem streams): algorithmic bytes per launch / mean launch time.
+roofline     = the kernel with the largest share of the served run's timed kernels, timed
+               with HIP events on one problem of the same mesh at 8000 Hz with nothing else
+               on the GPU: algorithmic bytes per launch / mean launch time.  `served` holds
+               the same kernel's figures in the served run, where the lanes share the GPU.
+               `traffic` is the measured HBM bytes per launch of that single-problem timing
+               from the committed counter profile (rocprofv3 FETCH_SIZE and WRITE_SIZE,
+               calibrated on a copy kernel).
 cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) on one
                frequency of one source of the same mesh, on one host core.
 """
@@ -44,6 +50,7 @@ from mesh2hrtf_amd import ncg, project as P  # noqa: E402
 METRIC = "frequency-solves/sec at rel-L2≤1e-3 vs FP64 oracle; p50/p99 solve latency"
 C, RHO = 343.0, 1.1839
 HBM_PEAK = 8.0e12
+TRAFFIC_PROFILE = "k_near_spmv_head20480.txt"
 SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50, max_iterations=500,
               refinement_steps=8, refinement_tolerance=1e-5)
 
@@ -79,8 +86,42 @@ def cpu_baseline(mesh, grid, src, f, timeout):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def isolated_kernels(geo, src, f, launches=20):
+    """per-kernel launch times of the operator apply on one problem with nothing else running"""
+    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], None, node_index=geo["node_index"], method_bem=4)
+    try:
+        sp = freq_spec(src, f)
+        prob.assemble(sp["k"], nrhs=sp["nrhs"], point_sources=sp["point_sources"], precision=ncg.MIXED)
+        x = np.random.default_rng(0).standard_normal((sp["nrhs"], prob.n)) + 0j
+        prob.apply(x)
+        prob.kernel_timing(True)
+        prob.kernel_stats(reset=True)
+        for _ in range(launches):
+            prob.apply(x)
+        return prob.kernel_stats()
+    finally:
+        prob.close()
+
+
+def profiled_traffic(kernel):
+    """measured HBM bytes per launch of `kernel` in the 8000 Hz single-problem timing, from
+    the committed rocprofv3 counter summary of this benchmark (a line 'traffic <kernel>
+    <bytes>'), or (None, None)"""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        for line in fh:
+            w = line.split()
+            if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
+                return float(w[2]), "profiles/" + TRAFFIC_PROFILE
+    return None, None
+
+
 def pick(freqs, count, offset=0):
     """count frequencies spread evenly over the sweep (all of them, cyclically, if count >= len)."""
+    if count <= 0:
+        return []
     if count >= len(freqs):
         return [freqs[(offset + i) % len(freqs)] for i in range(count)]
     stride = len(freqs) / count
@@ -90,13 +131,16 @@ def pick(freqs, count, offset=0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=80, help="frequencies in the timed region (80 = the sweep)")
+    ap.add_argument("--steps", type=int, default=240,
+                    help="frequency steps in the timed region (the 80-frequency sweep, cyclically)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--nu", type=int, default=32, help="geodesic frequency of the mesh (32 -> 20480 triangles)")
-    ap.add_argument("--plain-steps", type=int, default=-1, help="frequencies the FP64 plain method runs (-1: all)")
+    ap.add_argument("--plain-steps", type=int, default=-1,
+                    help="frequencies the FP64 plain method runs (-1: every frequency of the timed steps)")
     ap.add_argument("--poisson", type=int, default=-1, help="requests of the latency run (-1: steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -148,23 +192,23 @@ def main():
         if dt > 0:
             time.sleep(dt)
         pjobs.append(srv.submit(mid, [freq_spec(src, f)], **SERVED))
-    pl = [j.wait()["latency_ms"] for j in pjobs]
+    pl = [j.wait()["latency_ms"] for j in pjobs] or [float("nan")]
     print("# poisson %.1f req/s: latency p50 %.1f ms p99 %.1f ms" % (rate, np.percentile(pl, 50), np.percentile(pl, 99)),
           file=sys.stderr, flush=True)
     info = srv.stats()
     srv.close()
 
     # ---- FP64 plain method (the drop-in), one step after the other ----------------------
-    nplain = len(timed) if a.plain_steps < 0 else min(a.plain_steps, len(timed))
+    uniq = sorted(set(timed))
+    nplain = len(uniq) if a.plain_steps < 0 else min(a.plain_steps, len(uniq))
     prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
                        method_bem=4)
-    sp = freq_spec(src, timed[0])
+    sp = freq_spec(src, uniq[0])
     prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
     prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
-    err_s, err_e, plain_lat = [], [], []
+    ref, plain_lat = {}, []
     t0 = time.perf_counter()
-    for i in range(nplain):
-        f = timed[i]
+    for f in uniq[:nplain]:
         ts = time.perf_counter()
         sp = freq_spec(src, f)
         prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
@@ -172,29 +216,50 @@ def main():
         x0 = prob.solution()
         p0, _ = prob.field(gradient=False)
         plain_lat.append((time.perf_counter() - ts) * 1e3)
-        x1, p1 = jobs[i].solution[0], jobs[i].phi[0]
-        err_s.append(max(np.linalg.norm(x1[r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(nr)))
-        err_e.append(max(np.linalg.norm(p1[r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(nr)))
-        print("# plain f=%5.0f %.1f ms  mixed rel-L2 surface %.1e grid %.1e" % (f, plain_lat[-1], err_s[-1], err_e[-1]),
-              file=sys.stderr, flush=True)
+        ref[f] = (x0, p0)
+        print("# plain f=%5.0f %.1f ms" % (f, plain_lat[-1]), file=sys.stderr, flush=True)
     plain_elapsed = time.perf_counter() - t0
     prob.close()
 
-    ok = [s["state"] == "done" and (i >= nplain or (err_s[i] <= 1e-3 and err_e[i] <= 1e-3))
-          for i, s in enumerate(sts)]
+    # every timed step against the plain solution at its frequency
+    err_s, err_e, ok = [], [], []
+    for f, j, s in zip(timed, jobs, sts):
+        good = s["state"] == "done"
+        if good and f in ref:
+            x0, p0 = ref[f]
+            es = max(np.linalg.norm(j.solution[0][r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(nr))
+            ee = max(np.linalg.norm(j.phi[0][r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(nr))
+            err_s.append(es)
+            err_e.append(ee)
+            good = es <= 1e-3 and ee <= 1e-3
+        ok.append(good)
     valid = int(sum(bool(v) for v in ok)) * nr
     value = valid / elapsed
 
-    # roofline of the dominant kernel of the served run
+    # the dominant kernel again on one problem with nothing else on the GPU
+    iso = {} if a.no_isolated else isolated_kernels(geo, src, 8000.0)
+
+    # roofline of the dominant kernel of the served run, timed on one problem with nothing
+    # else on the GPU (launch times under the lanes' sharing are kept as `served`)
     top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
     roof = None
     if top:
         s = kst[top]
-        achieved = s["bytes"] / s["calls"] / (s["total_ms"] / s["calls"] * 1e-3)
-        roof = {"bound": "hbm", "kernel": top, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": s["bytes"] / s["calls"],
-                "mean_launch_ms": s["total_ms"] / s["calls"], "launches": s["calls"],
-                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
+        served = {"achieved": s["bytes"] / s["total_ms"] * 1e3 / 1e9, "frac": s["bytes"] / s["total_ms"] * 1e3 / HBM_PEAK,
+                  "bytes_per_launch": s["bytes"] / s["calls"], "mean_launch_ms": s["total_ms"] / s["calls"],
+                  "launches": s["calls"], "timing": "HIP events around each launch on the problem streams, %d lanes "
+                                                    "sharing the GPU" % a.lanes,
+                  "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
+        roof = {"bound": "hbm", "kernel": top, "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": None,
+                "traffic": None, "served": served}
+        if top in iso:
+            d = iso[top]
+            ach = d["bytes"] / d["total_ms"] * 1e3
+            roof.update({"achieved": ach / 1e9, "frac": ach / HBM_PEAK, "bytes_per_launch": d["bytes"] / d["calls"],
+                         "mean_launch_ms": d["total_ms"] / d["calls"], "launches": d["calls"],
+                         "workload": "one problem of the same mesh at 8000 Hz (two-level ML-FMM), 2 load cases, "
+                                     "nothing else on the GPU; HIP events around each launch"})
+            roof["traffic"], roof["traffic_source"] = profiled_traffic(top)
     cpu = None if a.no_cpu else cpu_baseline(mesh, grid, src, 8000.0, a.cpu_timeout)
     line = {
         "metric": METRIC,
@@ -226,7 +291,8 @@ def main():
                        "form": "NumCalc's method choice, CGS + ILU, tol 1e-9, one step at a time"},
         "speedup_vs_plain": value / (nplain * nr / plain_elapsed) if nplain else None,
         "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
-                                 "eval_grid_max": float(max(err_e)) if err_e else None},
+                                 "eval_grid_max": float(max(err_e)) if err_e else None,
+                                 "steps_checked": len(err_s)},
         "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done")},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -252,6 +252,11 @@ struct DevIlu {
     int *rp = nullptr, *col = nullptr, *diag = nullptr;
     int *levL_ptr = nullptr, *levL_rows = nullptr, *levU_ptr = nullptr, *levU_rows = nullptr;
     double *val = nullptr;
+    // level-ordered factors for the triangular solves: offsets, columns, CSR positions
+    int *Loff = nullptr, *Lc = nullptr, *Lmap = nullptr, *Uoff = nullptr, *Uc = nullptr, *Umap = nullptr,
+        *Udmap = nullptr;
+    double *Lv = nullptr, *Uv = nullptr, *Ud = nullptr;
+    int nL = 0, nU = 0;   // strictly lower / upper entries
     std::vector<void *> owned;
     void release()
     {
@@ -958,6 +963,15 @@ int dotv(ncg_problem *p, int ndots, const double *const *u, const double *const 
     return 0;
 }
 
+// host vector to a new device array owned by the ILU (asynchronous on s)
+int upload_ilu(DevIlu &I, hipStream_t s, int **dst, const std::vector<int> &v)
+{
+    int rc = I.alloc(dst, v.size());
+    if (rc) return rc;
+    if (!v.empty()) NCG_HIP(hipMemcpyAsync(*dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
 // ---------------------------------------------------------------------------------- ILU
 // Incomplete LU of NC_IncompleteLUDecomposition (src/NC_CommonFunctions.cpp:1356-1659):
 // rows scaled by d_i = sqrt(n_i / sum |a_ij|^2) over the dense row or the near-field row,
@@ -1045,16 +1059,63 @@ int ilu_setup(ncg_problem *p, int degree)
     NCG_HIP(hipMemcpyAsync(I.levU_rows, rowsU.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     I.nlevL = nL;
     I.nlevU = nU;
+    // the factors in the order the solves visit them
+    std::vector<int> Loff(n + 1, 0), Uoff(n + 1, 0), Lc, Lmap, Uc, Umap, Udmap(n);
+    Lc.reserve(tot);
+    Lmap.reserve(tot);
+    for (int u = 0; u < n; ++u) {
+        const int i = rowsL[u];
+        for (int q = rp[i]; q < diag[i]; ++q) {
+            Lc.push_back(col[q]);
+            Lmap.push_back(q);
+        }
+        Loff[u + 1] = (int)Lc.size();
+    }
+    for (int u = 0; u < n; ++u) {
+        const int i = rowsU[u];
+        for (int q = diag[i] + 1; q < rp[i + 1]; ++q) {
+            Uc.push_back(col[q]);
+            Umap.push_back(q);
+        }
+        Uoff[u + 1] = (int)Uc.size();
+        Udmap[u] = diag[i];
+    }
+    I.nL = (int)Lc.size();
+    I.nU = (int)Uc.size();
+    if ((rc = upload_ilu(I, s, &I.Loff, Loff)) || (rc = upload_ilu(I, s, &I.Lc, Lc)) ||
+        (rc = upload_ilu(I, s, &I.Lmap, Lmap)) || (rc = upload_ilu(I, s, &I.Uoff, Uoff)) ||
+        (rc = upload_ilu(I, s, &I.Uc, Uc)) || (rc = upload_ilu(I, s, &I.Umap, Umap)) ||
+        (rc = upload_ilu(I, s, &I.Udmap, Udmap)) || (rc = I.alloc(&I.Lv, 2 * (size_t)I.nL)) ||
+        (rc = I.alloc(&I.Uv, 2 * (size_t)I.nU)) || (rc = I.alloc(&I.Ud, 2 * (size_t)n)))
+        return rc;
     ncgk::ilu_factor(s, nL, I.levL_ptr, I.levL_rows, I.rp, I.col, I.diag, I.val, p->err);
+    ncgk::ilu_pack(s, I.nL, I.Lmap, I.val, I.Lv);
+    ncgk::ilu_pack(s, I.nU, I.Umap, I.val, I.Uv);
+    ncgk::ilu_pack(s, n, I.Udmap, I.val, I.Ud);
     NCG_HIP(hipGetLastError());
+    // the host vectors go out of scope here: the uploads must have completed
+    NCG_HIP(hipStreamSynchronize(s));
     return check_err(p);
 }
 
 void ilu_apply(ncg_problem *p, const double *x, double *y)
 {
-    DevIlu &I = p->ilu;
-    ncgk::ilu_solve(p->stream, p->n, p->nrhs, I.nlevL, I.levL_ptr, I.levL_rows, I.nlevU, I.levU_ptr, I.levU_rows, I.rp,
-                    I.col, I.diag, I.val, x, y);
+    const DevIlu &I = p->ilu;
+    ncgk::IluSolveArgs a;
+    a.nlevL = I.nlevL;
+    a.levL_ptr = I.levL_ptr;
+    a.Lrow = I.levL_rows;
+    a.Loff = I.Loff;
+    a.Lc = I.Lc;
+    a.Lv = I.Lv;
+    a.nlevU = I.nlevU;
+    a.levU_ptr = I.levU_ptr;
+    a.Urow = I.levU_rows;
+    a.Uoff = I.Uoff;
+    a.Uc = I.Uc;
+    a.Uv = I.Uv;
+    a.Ud = I.Ud;
+    ncgk::ilu_solve(p->stream, p->n, p->nrhs, a, x, y);
 }
 
 // ---------------------------------------------------------------------------------- block-Jacobi

@@ -796,33 +796,66 @@ __global__ void __launch_bounds__(1024) k_ilu_factor(int nlev, const int *lev_pt
     }
 }
 
+// Level-ordered copies of the factors for the triangular solves: position u of the L
+// (U) schedule is row Lrow[u] (Urow[u]); its strictly lower (upper) entries are
+// Lc/Lv[Loff[u] .. Loff[u+1]) (Uc/Uv), in the CSR order; Ud[u] is U's diagonal.
+__global__ void k_ilu_pack(int cnt, const int *map, const cx<double> *val, cx<double> *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < cnt) out[t] = val[map[t]];
+}
+
+// one triangular row: s -= sum_k v[k] y[c[k]] over [k0, k1) in increasing k, eight entries
+// per step with every load issued first (the indices past k1 are clamped and skipped)
+__device__ __forceinline__ cx<double> ilu_row(cx<double> s, int k0, int k1, const int *c, const cx<double> *v,
+                                              const cx<double> *yr)
+{
+    constexpr int B = 8;
+    for (int k = k0; k < k1; k += B) {
+        int cc[B];
+        cx<double> vv[B], yy[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int kk = min(k + q, k1 - 1);
+            cc[q] = c[kk];
+            vv[q] = v[kk];
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) yy[q] = yr[cc[q]];
+#pragma unroll
+        for (int q = 0; q < B; ++q)
+            if (k + q < k1) s -= vv[q] * yy[q];
+    }
+    return s;
+}
+
 // y = (L U)^-1 x for every load case: forward over the L levels, backward over the U
-// levels; one work-group (x and y may be the same vector)
-__global__ void __launch_bounds__(1024) k_ilu_solve(int n, int nrhs, int nlevL, const int *levL_ptr,
-                                                    const int *levL_rows, int nlevU, const int *levU_ptr,
-                                                    const int *levU_rows, const int *rp, const int *col,
-                                                    const int *diag, const cx<double> *val, const cx<double> *x,
-                                                    cx<double> *y)
+// levels; one work-group, a thread per (row of the level, load case).  The sums run in
+// the CSR order of each row, as the factorization stored them.  x and y may be the same
+// vector.
+__global__ void __launch_bounds__(1024) k_ilu_solve(int n, int nrhs, int nlevL, const int *levL_ptr, const int *Lrow,
+                                                    const int *Loff, const int *Lc, const cx<double> *Lv, int nlevU,
+                                                    const int *levU_ptr, const int *Urow, const int *Uoff,
+                                                    const int *Uc, const cx<double> *Uv, const cx<double> *Ud,
+                                                    const cx<double> *x, cx<double> *y)
 {
     for (int l = 0; l < nlevL; ++l) {
         const int b = levL_ptr[l], w = levL_ptr[l + 1] - b;
-        for (int u = threadIdx.x; u < w * nrhs; u += blockDim.x) {
-            const int r = u / w, i = levL_rows[b + u % w];
-            const cx<double> *yr = y + (size_t)r * n;
-            cx<double> s = x[(size_t)r * n + i];
-            for (int p = rp[i]; p < diag[i]; ++p) s -= val[p] * yr[col[p]];
-            y[(size_t)r * n + i] = s;
+        for (int t = threadIdx.x; t < w * nrhs; t += blockDim.x) {
+            const int r = t / w, u = b + t % w;
+            const int i = Lrow[u];
+            cx<double> *yr = y + (size_t)r * n;
+            yr[i] = ilu_row(x[(size_t)r * n + i], Loff[u], Loff[u + 1], Lc, Lv, yr);
         }
         __syncthreads();
     }
     for (int l = 0; l < nlevU; ++l) {
         const int b = levU_ptr[l], w = levU_ptr[l + 1] - b;
-        for (int u = threadIdx.x; u < w * nrhs; u += blockDim.x) {
-            const int r = u / w, i = levU_rows[b + u % w];
-            const cx<double> *yr = y + (size_t)r * n;
-            cx<double> s = yr[i];
-            for (int p = diag[i] + 1; p < rp[i + 1]; ++p) s -= val[p] * yr[col[p]];
-            y[(size_t)r * n + i] = cdiv(s, val[diag[i]]);
+        for (int t = threadIdx.x; t < w * nrhs; t += blockDim.x) {
+            const int r = t / w, u = b + t % w;
+            const int i = Urow[u];
+            cx<double> *yr = y + (size_t)r * n;
+            yr[i] = cdiv(ilu_row(yr[i], Uoff[u], Uoff[u + 1], Uc, Uv, yr), Ud[u]);
         }
         __syncthreads();
     }
@@ -2171,12 +2204,16 @@ void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows
 {
     k_ilu_factor<<<1, 1024, 0, s>>>(nlev, lev_ptr, lev_rows, rp, col, diag, (cx<double> *)val, err);
 }
-void ilu_solve(hipStream_t s, int n, int nrhs, int nlevL, const int *levL_ptr, const int *levL_rows, int nlevU,
-               const int *levU_ptr, const int *levU_rows, const int *rp, const int *col, const int *diag,
-               const void *val, const void *x, void *y)
+void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out)
 {
-    k_ilu_solve<<<1, 1024, 0, s>>>(n, nrhs, nlevL, levL_ptr, levL_rows, nlevU, levU_ptr, levU_rows, rp, col, diag,
-                                   (const cx<double> *)val, (const cx<double> *)x, (cx<double> *)y);
+    if (cnt > 0)
+        k_ilu_pack<<<blocks_for(cnt, 256), 256, 0, s>>>(cnt, map, (const cx<double> *)val, (cx<double> *)out);
+}
+void ilu_solve(hipStream_t s, int n, int nrhs, const IluSolveArgs &a, const void *x, void *y)
+{
+    k_ilu_solve<<<1, 1024, 0, s>>>(n, nrhs, a.nlevL, a.levL_ptr, a.Lrow, a.Loff, a.Lc, (const cx<double> *)a.Lv, a.nlevU,
+                                   a.levU_ptr, a.Urow, a.Uoff, a.Uc, (const cx<double> *)a.Uv,
+                                   (const cx<double> *)a.Ud, (const cx<double> *)x, (cx<double> *)y);
 }
 void lu_factor(hipStream_t s, void *A, int n)
 {

@@ -80,9 +80,15 @@ void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const i
               const int *rp, int *col, void *val);
 void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
                 const int *diag, void *val, int *err);
-void ilu_solve(hipStream_t s, int n, int nrhs, int nlevL, const int *levL_ptr, const int *levL_rows, int nlevU,
-               const int *levU_ptr, const int *levU_rows, const int *rp, const int *col, const int *diag,
-               const void *val, const void *x, void *y);
+// level-ordered triangular factors (values gathered by ilu_pack after the factorization)
+struct IluSolveArgs {
+    int nlevL = 0, nlevU = 0;
+    const int *levL_ptr = nullptr, *Lrow = nullptr, *Loff = nullptr, *Lc = nullptr;
+    const int *levU_ptr = nullptr, *Urow = nullptr, *Uoff = nullptr, *Uc = nullptr;
+    const void *Lv = nullptr, *Uv = nullptr, *Ud = nullptr;
+};
+void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out);
+void ilu_solve(hipStream_t s, int n, int nrhs, const IluSolveArgs &a, const void *x, void *y);
 void lu_factor(hipStream_t s, void *A, int n);
 void lu_solve(hipStream_t s, const void *A, int n, int nrhs, void *b);
 void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double *adm, double tau, const void *bcval,
