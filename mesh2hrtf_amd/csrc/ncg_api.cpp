@@ -256,6 +256,9 @@ struct DevIlu {
     int *Loff = nullptr, *Lc = nullptr, *Lmap = nullptr, *Uoff = nullptr, *Uc = nullptr, *Umap = nullptr,
         *Udmap = nullptr;
     double *Lv = nullptr, *Uv = nullptr, *Ud = nullptr;
+    // the first kIluB entries of every position again, padded: [n][kIluB]
+    int *Lcd = nullptr, *Lmapd = nullptr, *Ucd = nullptr, *Umapd = nullptr;
+    double *Lvd = nullptr, *Uvd = nullptr;
     int nL = 0, nU = 0;   // strictly lower / upper entries
     std::vector<void *> owned;
     void release()
@@ -1082,6 +1085,22 @@ int ilu_setup(ncg_problem *p, int degree)
     }
     I.nL = (int)Lc.size();
     I.nU = (int)Uc.size();
+    constexpr int B = ncgk::kIluB;
+    std::vector<int> Lcd((size_t)n * B, 0), Lmapd((size_t)n * B, -1), Ucd((size_t)n * B, 0), Umapd((size_t)n * B, -1);
+    for (int u = 0; u < n; ++u) {
+        for (int q = 0; q < B && Loff[u] + q < Loff[u + 1]; ++q) {
+            Lcd[(size_t)u * B + q] = Lc[Loff[u] + q];
+            Lmapd[(size_t)u * B + q] = Lmap[Loff[u] + q];
+        }
+        for (int q = 0; q < B && Uoff[u] + q < Uoff[u + 1]; ++q) {
+            Ucd[(size_t)u * B + q] = Uc[Uoff[u] + q];
+            Umapd[(size_t)u * B + q] = Umap[Uoff[u] + q];
+        }
+    }
+    if ((rc = upload_ilu(I, s, &I.Lcd, Lcd)) || (rc = upload_ilu(I, s, &I.Lmapd, Lmapd)) ||
+        (rc = upload_ilu(I, s, &I.Ucd, Ucd)) || (rc = upload_ilu(I, s, &I.Umapd, Umapd)) ||
+        (rc = I.alloc(&I.Lvd, 2 * (size_t)n * B)) || (rc = I.alloc(&I.Uvd, 2 * (size_t)n * B)))
+        return rc;
     if ((rc = upload_ilu(I, s, &I.Loff, Loff)) || (rc = upload_ilu(I, s, &I.Lc, Lc)) ||
         (rc = upload_ilu(I, s, &I.Lmap, Lmap)) || (rc = upload_ilu(I, s, &I.Uoff, Uoff)) ||
         (rc = upload_ilu(I, s, &I.Uc, Uc)) || (rc = upload_ilu(I, s, &I.Umap, Umap)) ||
@@ -1092,6 +1111,8 @@ int ilu_setup(ncg_problem *p, int degree)
     ncgk::ilu_pack(s, I.nL, I.Lmap, I.val, I.Lv);
     ncgk::ilu_pack(s, I.nU, I.Umap, I.val, I.Uv);
     ncgk::ilu_pack(s, n, I.Udmap, I.val, I.Ud);
+    ncgk::ilu_pack(s, n * B, I.Lmapd, I.val, I.Lvd);
+    ncgk::ilu_pack(s, n * B, I.Umapd, I.val, I.Uvd);
     NCG_HIP(hipGetLastError());
     // the host vectors go out of scope here: the uploads must have completed
     NCG_HIP(hipStreamSynchronize(s));
@@ -1101,21 +1122,25 @@ int ilu_setup(ncg_problem *p, int degree)
 void ilu_apply(ncg_problem *p, const double *x, double *y)
 {
     const DevIlu &I = p->ilu;
-    ncgk::IluSolveArgs a;
-    a.nlevL = I.nlevL;
-    a.levL_ptr = I.levL_ptr;
-    a.Lrow = I.levL_rows;
-    a.Loff = I.Loff;
-    a.Lc = I.Lc;
-    a.Lv = I.Lv;
-    a.nlevU = I.nlevU;
-    a.levU_ptr = I.levU_ptr;
-    a.Urow = I.levU_rows;
-    a.Uoff = I.Uoff;
-    a.Uc = I.Uc;
-    a.Uv = I.Uv;
-    a.Ud = I.Ud;
-    ncgk::ilu_solve(p->stream, p->n, p->nrhs, a, x, y);
+    ncgk::IluSweep L, U;
+    L.nlev = I.nlevL;
+    L.lev_ptr = I.levL_ptr;
+    L.row = I.levL_rows;
+    L.off = I.Loff;
+    L.c = I.Lc;
+    L.v = I.Lv;
+    L.cd = I.Lcd;
+    L.vd = I.Lvd;
+    U.nlev = I.nlevU;
+    U.lev_ptr = I.levU_ptr;
+    U.row = I.levU_rows;
+    U.off = I.Uoff;
+    U.c = I.Uc;
+    U.v = I.Uv;
+    U.cd = I.Ucd;
+    U.vd = I.Uvd;
+    U.d = I.Ud;
+    ncgk::ilu_solve(p->stream, p->n, p->nrhs, L, U, x, y);
 }
 
 // ---------------------------------------------------------------------------------- block-Jacobi

@@ -796,13 +796,14 @@ __global__ void __launch_bounds__(1024) k_ilu_factor(int nlev, const int *lev_pt
     }
 }
 
-// Level-ordered copies of the factors for the triangular solves: position u of the L
-// (U) schedule is row Lrow[u] (Urow[u]); its strictly lower (upper) entries are
-// Lc/Lv[Loff[u] .. Loff[u+1]) (Uc/Uv), in the CSR order; Ud[u] is U's diagonal.
+// Level-ordered copies of the factors for the triangular solves: position u of a sweep's
+// schedule is row row[u]; its strictly lower (upper) entries are c/v[off[u] .. off[u+1])
+// in the CSR order, the first kIluB of them also in cd/vd[u][kIluB] (padded with column
+// 0 and value 0); d[u] is U's diagonal.  Map entries -1 pack as zero.
 __global__ void k_ilu_pack(int cnt, const int *map, const cx<double> *val, cx<double> *out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < cnt) out[t] = val[map[t]];
+    if (t < cnt) out[t] = map[t] >= 0 ? val[map[t]] : mk(0.0, 0.0);
 }
 
 // one triangular row: s -= sum_k v[k] y[c[k]] over [k0, k1) in increasing k, eight entries
@@ -829,36 +830,107 @@ __device__ __forceinline__ cx<double> ilu_row(cx<double> s, int k0, int k1, cons
     return s;
 }
 
-// y = (L U)^-1 x for every load case: forward over the L levels, backward over the U
-// levels; one work-group, a thread per (row of the level, load case).  The sums run in
-// the CSR order of each row, as the factorization stored them.  x and y may be the same
-// vector.
-__global__ void __launch_bounds__(1024) k_ilu_solve(int n, int nrhs, int nlevL, const int *levL_ptr, const int *Lrow,
-                                                    const int *Loff, const int *Lc, const cx<double> *Lv, int nlevU,
-                                                    const int *levU_ptr, const int *Urow, const int *Uoff,
-                                                    const int *Uc, const cx<double> *Uv, const cx<double> *Ud,
-                                                    const cx<double> *x, cx<double> *y)
+// One row of a triangular sweep with everything that does not depend on the solution
+// loaded ahead: entry range, the first kIluB columns and values (from the padded copies
+// cd/vd, [position][kIluB]), the right-hand side (forward sweep), the diagonal (backward).
+constexpr int kIluB = ncgk::kIluB;
+struct IluItem {
+    int i, k0, k1;
+    bool on;
+    int cc[kIluB];
+    cx<double> vv[kIluB];
+    cx<double> xv, dv;
+};
+
+// lane's row i of the level [b, b + w) (w <= 64, i already loaded); every load here
+// depends only on the position and i.  xr (forward) or d (backward) may be null.
+__device__ __forceinline__ void ilu_item_load(IluItem &it, int lane, int b, int w, int i, const int *off,
+                                              const int *cd, const cx<double> *vd, const cx<double> *xr,
+                                              const cx<double> *d)
 {
-    for (int l = 0; l < nlevL; ++l) {
-        const int b = levL_ptr[l], w = levL_ptr[l + 1] - b;
-        for (int t = threadIdx.x; t < w * nrhs; t += blockDim.x) {
-            const int r = t / w, u = b + t % w;
-            const int i = Lrow[u];
-            cx<double> *yr = y + (size_t)r * n;
-            yr[i] = ilu_row(x[(size_t)r * n + i], Loff[u], Loff[u + 1], Lc, Lv, yr);
-        }
-        __syncthreads();
+    it.on = lane < w;
+    const int u = b + (it.on ? lane : 0);
+    it.i = i;
+    it.k0 = off[u];
+    it.k1 = off[u + 1];
+#pragma unroll
+    for (int q = 0; q < kIluB; ++q) {
+        it.cc[q] = cd[(size_t)u * kIluB + q];
+        it.vv[q] = vd[(size_t)u * kIluB + q];
     }
-    for (int l = 0; l < nlevU; ++l) {
-        const int b = levU_ptr[l], w = levU_ptr[l + 1] - b;
-        for (int t = threadIdx.x; t < w * nrhs; t += blockDim.x) {
-            const int r = t / w, u = b + t % w;
-            const int i = Urow[u];
-            cx<double> *yr = y + (size_t)r * n;
-            yr[i] = cdiv(ilu_row(yr[i], Uoff[u], Uoff[u + 1], Uc, Uv, yr), Ud[u]);
+    it.xv = xr ? xr[i] : mk(0.0, 0.0);
+    it.dv = d ? d[u] : mk(1.0, 0.0);
+}
+
+// s minus the item's row sum, in the entries' order
+__device__ __forceinline__ cx<double> ilu_item_sum(const IluItem &it, cx<double> s, const int *c, const cx<double> *v,
+                                                   const cx<double> *yr)
+{
+    cx<double> yy[kIluB];
+#pragma unroll
+    for (int q = 0; q < kIluB; ++q) yy[q] = yr[it.cc[q]];
+#pragma unroll
+    for (int q = 0; q < kIluB; ++q)
+        if (it.k0 + q < it.k1) s -= it.vv[q] * yy[q];
+    if (it.k1 - it.k0 > kIluB) s = ilu_row(s, it.k0 + kIluB, it.k1, c, v, yr);
+    return s;
+}
+
+// One triangular sweep of one load case by one wave.  A wave's memory operations complete
+// in program order and its lanes share one vector L1, so a row stored at one level is
+// seen by every lane at the next with no barrier (a wavefront-scope fence keeps the
+// compiler from moving loads above the stores).  The rows of level l + 2 and everything
+// else of level l + 1 are loaded while level l is computed, so a level costs one round of
+// solution gathers.  Levels wider than the wave take a plain loop.
+template <bool UPPER>
+__device__ __forceinline__ void ilu_sweep(int nlev, const int *lev_ptr, const int *row, const int *off, const int *c,
+                                          const cx<double> *v, const int *cd, const cx<double> *vd,
+                                          const cx<double> *d, const cx<double> *xr, cx<double> *yr)
+{
+    const int lane = threadIdx.x;
+    auto width = [&](int l) { return lev_ptr[l + 1] - lev_ptr[l]; };
+    auto row_of = [&](int l) { return l < nlev && lane < width(l) ? row[lev_ptr[l] + lane] : 0; };
+    IluItem cur, nxt;
+    cur.on = false;
+    int rnext = row_of(1);
+    if (nlev > 0 && width(0) <= 64)
+        ilu_item_load(cur, lane, lev_ptr[0], width(0), row_of(0), off, cd, vd, UPPER ? nullptr : xr, UPPER ? d : nullptr);
+    for (int l = 0; l < nlev; ++l) {
+        const int b = lev_ptr[l], w = lev_ptr[l + 1] - b;
+        nxt.on = false;
+        const int rafter = row_of(l + 2);
+        if (l + 1 < nlev && width(l + 1) <= 64)
+            ilu_item_load(nxt, lane, lev_ptr[l + 1], width(l + 1), rnext, off, cd, vd, UPPER ? nullptr : xr,
+                          UPPER ? d : nullptr);
+        if (w <= 64) {
+            if (cur.on) {
+                if (UPPER) yr[cur.i] = cdiv(ilu_item_sum(cur, yr[cur.i], c, v, yr), cur.dv);
+                else yr[cur.i] = ilu_item_sum(cur, cur.xv, c, v, yr);
+            }
+        } else {
+            for (int t = lane; t < w; t += 64) {
+                const int u = b + t, i = row[u];
+                if (UPPER) yr[i] = cdiv(ilu_row(yr[i], off[u], off[u + 1], c, v, yr), d[u]);
+                else yr[i] = ilu_row(xr[i], off[u], off[u + 1], c, v, yr);
+            }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        cur = nxt;
+        rnext = rafter;
     }
+}
+
+// y = (L U)^-1 x: one wave per load case (block r of 64 threads), forward over the L
+// levels, backward over the U levels.  The sums run in the CSR order of each row, as the
+// factorization stored them.  x and y may be the same vector (a row's right-hand side is
+// read before the row is written).
+__global__ void __launch_bounds__(64) k_ilu_solve(int n, ncgk::IluSweep L, ncgk::IluSweep U, const cx<double> *x, cx<double> *y)
+{
+    const size_t r = blockIdx.x;
+    ilu_sweep<false>(L.nlev, L.lev_ptr, L.row, L.off, L.c, (const cx<double> *)L.v, L.cd, (const cx<double> *)L.vd,
+                     nullptr, x + r * n, y + r * n);
+    ilu_sweep<true>(U.nlev, U.lev_ptr, U.row, U.off, U.c, (const cx<double> *)U.v, U.cd, (const cx<double> *)U.vd,
+                    (const cx<double> *)U.d, nullptr, y + r * n);
 }
 
 // ======================================================================================
@@ -2209,11 +2281,9 @@ void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out
     if (cnt > 0)
         k_ilu_pack<<<blocks_for(cnt, 256), 256, 0, s>>>(cnt, map, (const cx<double> *)val, (cx<double> *)out);
 }
-void ilu_solve(hipStream_t s, int n, int nrhs, const IluSolveArgs &a, const void *x, void *y)
+void ilu_solve(hipStream_t s, int n, int nrhs, const IluSweep &L, const IluSweep &U, const void *x, void *y)
 {
-    k_ilu_solve<<<1, 1024, 0, s>>>(n, nrhs, a.nlevL, a.levL_ptr, a.Lrow, a.Loff, a.Lc, (const cx<double> *)a.Lv, a.nlevU,
-                                   a.levU_ptr, a.Urow, a.Uoff, a.Uc, (const cx<double> *)a.Uv,
-                                   (const cx<double> *)a.Ud, (const cx<double> *)x, (cx<double> *)y);
+    k_ilu_solve<<<nrhs, 64, 0, s>>>(n, L, U, (const cx<double> *)x, (cx<double> *)y);
 }
 void lu_factor(hipStream_t s, void *A, int n)
 {

@@ -80,15 +80,18 @@ void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const i
               const int *rp, int *col, void *val);
 void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
                 const int *diag, void *val, int *err);
-// level-ordered triangular factors (values gathered by ilu_pack after the factorization)
-struct IluSolveArgs {
-    int nlevL = 0, nlevU = 0;
-    const int *levL_ptr = nullptr, *Lrow = nullptr, *Loff = nullptr, *Lc = nullptr;
-    const int *levU_ptr = nullptr, *Urow = nullptr, *Uoff = nullptr, *Uc = nullptr;
-    const void *Lv = nullptr, *Uv = nullptr, *Ud = nullptr;
+// one level-scheduled triangular sweep of the incomplete LU: position u of the schedule is
+// row row[u] with entries c/v[off[u] .. off[u+1]) (CSR order), the first kIluB of them
+// also padded in cd/vd[u * kIluB + q]; d = U's diagonal per position (backward sweep)
+constexpr int kIluB = 8;
+struct IluSweep {
+    int nlev = 0;
+    const int *lev_ptr = nullptr, *row = nullptr, *off = nullptr, *c = nullptr, *cd = nullptr;
+    const void *v = nullptr, *vd = nullptr, *d = nullptr;
 };
+// out[t] = val[map[t]] (0 where map[t] < 0)
 void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out);
-void ilu_solve(hipStream_t s, int n, int nrhs, const IluSolveArgs &a, const void *x, void *y);
+void ilu_solve(hipStream_t s, int n, int nrhs, const IluSweep &L, const IluSweep &U, const void *x, void *y);
 void lu_factor(hipStream_t s, void *A, int n);
 void lu_solve(hipStream_t s, const void *A, int n, int nrhs, void *b);
 void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double *adm, double tau, const void *bcval,
