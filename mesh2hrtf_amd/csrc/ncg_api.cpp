@@ -181,7 +181,8 @@ struct DevFmm {
     std::vector<DevLevel> lev;
     long long nnz = 0;
     long long *near_ptr = nullptr, *diag_pos = nullptr;
-    int *near_row = nullptr, *near_col = nullptr;
+    int *near_col = nullptr;
+    int near_wmax = 0;                 // longest near-field row
     int *elem_cl_fine = nullptr;
     unsigned char *cl_near = nullptr;
     int ncl_fine = 0;
@@ -781,16 +782,17 @@ int fmm_structure(ncg_problem *p)
     F.bj_total = boff.back();
     F.nnz = ptr[n];
     F.ncl_fine = ncf;
+    F.near_wmax = rowlen.empty() ? 0 : (int)*std::max_element(rowlen.begin(), rowlen.end());
     int *d_nn_ptr, *d_nn_idx;
     if ((rc = upload(F.st, s, &F.near_ptr, ptr)) || (rc = F.st.alloc(&F.diag_pos, n)) ||
-        (rc = F.st.alloc(&F.near_row, F.nnz)) || (rc = F.st.alloc(&F.near_col, F.nnz)) ||
+        (rc = F.st.alloc(&F.near_col, F.nnz)) ||
         (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
         (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)) ||
         (rc = upload(F.st, s, &F.self_off, self_off)) || (rc = upload(F.st, s, &F.boff, boff)) ||
         (rc = upload(F.st, s, &F.elem_blk, elem_blk)) || (rc = upload(F.st, s, &F.blk_cl, blk_cl)) ||
         (rc = upload(F.st, s, &F.blk_b0, blk_b0)) || (rc = upload(F.st, s, &F.blk_m, blk_m)))
         return rc;
-    ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, F.near_row, F.near_col,
+    ncgk::near_csr(s, n, F.elem_cl_fine, DF.cl_ptr, DF.cl_elem, d_nn_ptr, d_nn_idx, F.near_ptr, nullptr, F.near_col,
                    F.diag_pos);
     // positions of the subdivided pairs (the dense method's near-pair table) in the CSR
     std::vector<int> nn_off(nn_idx.size(), 0);
@@ -850,7 +852,9 @@ int assemble_fmm(ncg_problem *p)
     F.bj_ready = 0;
     // values
     const ncgk::DevElements E = p->elements();
-    ncgk::fmm_near(s, E, b, p->adm, F.near_row, F.near_col, F.nnz, F.near_val.p, mixed, p->err);
+    const DevLevel &DF = F.lev.back();
+    ncgk::fmm_near(s, E, b, p->adm, DF.chunks(), DF.cl_ptr, DF.cl_elem, F.near_ptr, F.near_col, F.near_wmax,
+                   F.near_val.p, mixed);
     ncgk::fmm_near_adj(s, E, b, p->adm, p->near_rows, p->near_cols, F.adj_pos, p->nnear, F.near_val.p, mixed, p->err);
     if (p->nsrc)
         ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near,
@@ -1289,9 +1293,10 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 // Givens rotations and the convergence test.  Optional right preconditioning by the
 // block-Jacobi inverse (A M^-1 u = b, x = M^-1 u).  acc64 selects the matvec
 // accumulation (FP32 operator: 0 = FP32 inner solve).  Solves for the correction in 'x'
-// starting from its current content.
+// starting from its current content; x_zero says that content is zero, so the first
+// residual is b without a matvec.
 int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, double tol, int acc64, int bj,
-          std::vector<int> &iters, std::vector<double> &relres)
+          bool x_zero, std::vector<int> &iters, std::vector<double> &relres)
 {
     const int n = p->n, nr = p->nrhs;
     const size_t vlen = (size_t)nr * n;
@@ -1319,13 +1324,17 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
     std::vector<int> done(nr, 0);
     std::vector<cd> hcol;
     int total = 0;
+    bool first = true;
     while (total < maxit) {
         // r = b - A x  -> W
-        mv(p, acc64, x, nullptr, T);
         NCG_HIP(hipMemcpyAsync(W, b, sizeof(cd) * vlen, hipMemcpyDeviceToDevice, p->stream));
-        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
-        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-        ncgk::axpby(p->stream, n, nr, p->scal, T, n, W, n);
+        if (!(first && x_zero)) {
+            mv(p, acc64, x, nullptr, T);
+            for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
+            NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
+            ncgk::axpby(p->stream, n, nr, p->scal, T, n, W, n);
+        }
+        first = false;
         {
             const double *u[1] = {W}, *v[1] = {W};
             if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
@@ -1433,7 +1442,8 @@ int solve_gmres(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st
     NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
     std::vector<int> it;
     std::vector<double> rr;
-    rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance, 1, bj, it, rr);
+    rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance, 1, bj, true, it,
+               rr);
     if (rc) return rc;
     harvest_mv(p);
     int all = 1;
@@ -1483,7 +1493,7 @@ int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st
         std::vector<int> it;
         std::vector<double> rr;
         const double inner = std::max(1e-5, target * 0.5);
-        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, bj, it, rr)))
+        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, bj, true, it, rr)))
             return rc;
         for (int r = 0; r < nr; ++r) total[r] += it[r];
         // x += d ; r = b - A x (FP64 accumulation)

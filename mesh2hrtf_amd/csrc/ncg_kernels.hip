@@ -1352,51 +1352,74 @@ __global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, co
     vals[pos[q]] = cx<S>{S(a.re), S(a.im)};
 }
 
-// near-field entries of the FMM operator with the regular rule: one thread per
-// off-diagonal nonzero; subdivided pairs are left to k_fmm_near_adj.  T is the
-// arithmetic (FP32 in the mixed form, with the geometry differences formed in FP64).
+// Regular near-field entries (NC_ComputeEntriesNforFMM, src/NC_EquationSystem.cpp:1243-1405,
+// without the subdivided pairs, which k_fmm_near_adj writes).  The rows of a finest
+// cluster share their column list, so a work-group takes a row chunk of one cluster
+// (ch_*, at most 16 rows) and a segment of 256 columns: each thread holds one column
+// element's geometry in registers and walks the chunk's rows, whose collocation data is
+// wave-uniform; a row's entries are stored coalesced.  T is the arithmetic of the
+// quadrature (FP32 in the mixed form, after the geometry differences are formed in FP64).
 template <typename S, typename T>
-__global__ void __launch_bounds__(128) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *rows,
-                                                  const int *cols, long long nnz, cx<S> *vals, int *err)
+__global__ void __launch_bounds__(256) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *ch_cl,
+                                                  const int *ch_r0, const int *ch_m, const int *cl_ptr,
+                                                  const int *cl_elem, const long long *ptr, const int *col, cx<S> *vals)
 {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nnz) return;
-    const int i = rows[q], j = cols[q];
-    if (i == j) return;   // diagonal: k_assemble_diag
-    const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
-    const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
-    LocalElem L;
-    load_local(E, j, x, L);
-    const double ratdis = ratio_to_centre(E.cen[j], E.cen[E.n + j], E.cen[2 * E.n + j], x, L.area);
-    if (ratdis < kTolf) return;
-    BMAcc<T> acc;
-    acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
-    const T nxx = T(nx[0]), nxy = T(nx[1]), nxz = T(nx[2]), hk = T(bm.hk), k2 = T(bm.k2);
-    const int m = gauss_order(ratdis);
-    if (L.nv == 3) {
-        // x(s,t) - x_i = p2 + s (p0 - p2) + t (p1 - p2), nodes already relative to x_i
-        const T e1x = T(L.p[0][0] - L.p[2][0]), e1y = T(L.p[0][1] - L.p[2][1]), e1z = T(L.p[0][2] - L.p[2][2]);
-        const T e2x = T(L.p[1][0] - L.p[2][0]), e2y = T(L.p[1][1] - L.p[2][1]), e2z = T(L.p[1][2] - L.p[2][2]);
-        const T px = T(L.p[2][0]), py = T(L.p[2][1]), pz = T(L.p[2][2]);
-        const T nyx = T(E.nrm[j]), nyy = T(E.nrm[E.n + j]), nyz = T(E.nrm[2 * E.n + j]), jac = T(E.jac[j]);
-        const int off = tri_rule_offset(m), cnt = tri_rule_count(m);
-        for (int g = 0; g < cnt; ++g) {
-            const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), w = T(c_tri[off + g][2]);
-            bm_point<T>(acc, px + s * e1x + t * e2x, py + s * e1y + t * e2y, pz + s * e1z + t * e2z, nyx, nyy, nyz, nxx,
-                        nxy, nxz, w * jac, hk, k2);
-        }
-    } else {
-        const int base = m * (m - 1) / 2;
-        for (int a = 0; a < m; ++a)
-            for (int b = 0; b < m; ++b) {
-                double q2[3], ny[3], J;
-                point_on(L, c_gl_x[base + a], c_gl_x[base + b], q2, ny, J);
-                bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
-                            T(c_gl_w[base + a] * c_gl_w[base + b] * J), hk, k2);
+    const int n = E.n;
+    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
+    const long long cb = ptr[cl_elem[cl_ptr[c]]];
+    const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
+    const int u = blockIdx.y * 256 + threadIdx.x;
+    if (u >= w) return;
+    const int j = col[cb + u];
+    const int nv = E.nv[j];
+    const int kind = E.kind[j];
+    const cx<T> admj = {T(adm[2 * j]), T(adm[2 * j + 1])};
+    const double cj[3] = {E.cen[j], E.cen[n + j], E.cen[2 * n + j]};
+    const double areaj = E.area[j];
+    const T nyx = T(E.nrm[j]), nyy = T(E.nrm[n + j]), nyz = T(E.nrm[2 * n + j]);
+    const T jacj = T(E.jac[j]);
+    double P[4][3];
+    for (int a = 0; a < nv; ++a)
+        for (int d = 0; d < 3; ++d) P[a][d] = E.X[(3 * a + d) * n + j];
+    // triangle: x(s,t) = x2 + s (x0 - x2) + t (x1 - x2)
+    const T e1x = T(P[0][0] - P[2][0]), e1y = T(P[0][1] - P[2][1]), e1z = T(P[0][2] - P[2][2]);
+    const T e2x = T(P[1][0] - P[2][0]), e2y = T(P[1][1] - P[2][1]), e2z = T(P[1][2] - P[2][2]);
+    const T hk = T(bm.hk), k2 = T(bm.k2), tau = T(bm.tau), bim = T(bm.beta_im);
+    for (int a = 0; a < m; ++a) {
+        const int i = cl_elem[r0 + a];
+        if (i == j) continue;   // diagonal: k_assemble_diag
+        const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
+        const double ratdis = ratio_to_centre(cj[0], cj[1], cj[2], x, areaj);
+        if (ratdis < kTolf) continue;   // subdivided pair: k_fmm_near_adj
+        const T nxx = T(E.nrm[i]), nxy = T(E.nrm[n + i]), nxz = T(E.nrm[2 * n + i]);
+        const int mg = gauss_order(ratdis);
+        BMAcc<T> acc;
+        acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
+        if (nv == 3) {
+            const T px = T(P[2][0] - x[0]), py = T(P[2][1] - x[1]), pz = T(P[2][2] - x[2]);
+            const int off = tri_rule_offset(mg), cnt = tri_rule_count(mg);
+            for (int g = 0; g < cnt; ++g) {
+                const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), wg = T(c_tri[off + g][2]);
+                bm_point<T>(acc, px + s * e1x + t * e2x, py + s * e1y + t * e2y, pz + s * e1z + t * e2z, nyx, nyy, nyz,
+                            nxx, nxy, nxz, wg * jacj, hk, k2);
             }
+        } else {
+            LocalElem L;
+            L.nv = 4;
+            for (int b = 0; b < 4; ++b)
+                for (int d = 0; d < 3; ++d) L.p[b][d] = P[b][d] - x[d];
+            const int base = mg * (mg - 1) / 2;
+            for (int s2 = 0; s2 < mg; ++s2)
+                for (int t2 = 0; t2 < mg; ++t2) {
+                    double q2[3], ny[3], J;
+                    point_on(L, c_gl_x[base + s2], c_gl_x[base + t2], q2, ny, J);
+                    bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
+                                T(c_gl_w[base + s2] * c_gl_w[base + t2] * J), hk, k2);
+                }
+        }
+        const cx<T> e = bm_entry<T>(acc, kind, admj, tau, bim);
+        vals[ptr[i] + u] = cx<S>{S(e.re), S(e.im)};
     }
-    const cx<T> a = bm_entry<T>(acc, E.kind[j], cx<T>{T(adm[2 * j]), T(adm[2 * j + 1])}, T(bm.tau), T(bm.beta_im));
-    vals[q] = cx<S>{S(a.re), S(a.im)};
 }
 
 // near-field CSR of the finest level: row i holds the elements of the near clusters of
@@ -1416,7 +1439,7 @@ __global__ void __launch_bounds__(256) k_near_csr(int n, const int *elem_cl, con
         const int b = cl_ptr[j], sz = cl_ptr[j + 1] - b;
         for (int u = lane; u < sz; u += 64) {
             const int e = cl_elem[b + u];
-            rows[q + u] = i;
+            if (rows) rows[q + u] = i;
             cols[q + u] = e;
             if (e == i) diag[i] = q + u;
         }
@@ -2354,14 +2377,19 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
     if (mixed) k_fmm_S<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<float> *)Sv);
     else k_fmm_S<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<double> *)Sv);
 }
-void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
-              long long nnz, void *vals, int mixed, int *err)
+void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
+              const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
+              int mixed)
 {
-    if (nnz == 0) return;
+    if (rc.n == 0 || wmax == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    const unsigned g = blocks_for(nnz, 128);
-    if (mixed) k_fmm_near<float, float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<float> *)vals, err);
-    else k_fmm_near<double, double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, nnz, (cx<double> *)vals, err);
+    const dim3 g(rc.n, (wmax + 255) / 256);
+    if (mixed)
+        k_fmm_near<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+                                                   (cx<float> *)vals);
+    else
+        k_fmm_near<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+                                                     (cx<double> *)vals);
 }
 void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
                    const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,

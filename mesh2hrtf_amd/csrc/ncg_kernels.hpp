@@ -110,8 +110,6 @@ void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, 
                  const int *cl_ptr, const int *cl_elem, const double *u, const void *bcval, int nrhs, void *tv);
 void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
            const double *u, const double *w, void *Sv, int mixed);
-void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows, const int *cols,
-              long long nnz, void *vals, int mixed, int *err);
 // subdivided pairs of the near field: CSR positions (once per structure) and values
 void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
                    const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
@@ -148,6 +146,11 @@ void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, co
                  int f32, int tw_f64, const void *tw, void *sw);
 void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
                  const void *Sv, int f32, const void *sw, void *y);
+// regular near-field entries by (row chunk, 256 columns); the subdivided pairs are left to
+// fmm_near_adj and the diagonal to assemble_diag.  wmax = longest row.
+void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
+              const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
+              int mixed);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
                   void *tv, int fp32);
