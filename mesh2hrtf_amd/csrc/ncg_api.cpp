@@ -193,7 +193,7 @@ struct DevFmm {
     long long *boff = nullptr, bj_total = 0;
     int nblk = 0;
     Grow bj_blk;
-    int bj_ready = 0;
+    int bj_ready = 0, bj_f32 = 0;   // bj_f32: inverses in FP32 (mixed form)
     // evaluation-grid pairs (rebuilt when the pair lists change)
     unsigned long long ev_sig = 0, ev_struct = 0;
     int ev_pure = 0;
@@ -1157,9 +1157,10 @@ int bj_prepare(ncg_problem *p)
     if (F.bj_ready) return 0;
     const DevLevel &D = F.lev.back();
     int rc;
-    if ((rc = F.bj_blk.ensure(16 * (size_t)F.bj_total))) return rc;
+    F.bj_f32 = p->precision == NCG_MIXED;
+    if ((rc = F.bj_blk.ensure((F.bj_f32 ? 8 : 16) * (size_t)F.bj_total))) return rc;
     ncgk::bj_setup(p->stream, p->n, F.nblk, F.elem_blk, D.elem_pos, F.blk_cl, F.blk_b0, F.blk_m, D.cl_ptr, F.near_ptr,
-                   F.self_off, F.near_val.p, p->precision == NCG_MIXED, F.boff, F.bj_blk.p, p->err);
+                   F.self_off, F.near_val.p, F.bj_f32, F.boff, F.bj_blk.p, p->err);
     NCG_HIP(hipGetLastError());
     if ((rc = check_err(p))) return rc;
     F.bj_ready = 1;
@@ -1170,7 +1171,8 @@ void bj_apply_vec(ncg_problem *p, const double *x, double *y)
 {
     const DevFmm &F = p->fmm;
     const DevLevel &D = F.lev.back();
-    ncgk::bj_apply(p->stream, p->n, p->nrhs, F.nblk, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, x, y);
+    ncgk::bj_apply(p->stream, p->n, p->nrhs, F.nblk, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, F.bj_f32, x,
+                   y);
 }
 
 // ---------------------------------------------------------------------------------- CGS

@@ -1461,96 +1461,111 @@ __device__ __forceinline__ void cmac(double &re, double &im, double ar, double a
 // blk_b0[b] - cl_ptr[c] further.  Inverses are stored row-major m x m at boff[b].
 constexpr int kBlk = 64;
 
-template <typename S>
+template <typename S, typename TB>
 __global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_blk, const int *elem_pos, const int *blk_cl,
                                                    const int *blk_b0, const int *blk_m, const int *cl_ptr,
                                                    const long long *near_ptr, const int *self_off, const cx<S> *nval,
-                                                   const long long *boff, cx<double> *blk)
+                                                   const long long *boff, cx<TB> *blk)
 {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
     const int b = elem_blk[i], c = blk_cl[b], m = blk_m[b], a = elem_pos[i] - blk_b0[b];
     const cx<S> *src = nval + near_ptr[i] + self_off[c] + (blk_b0[b] - cl_ptr[c]);
-    cx<double> *dst = blk + boff[b] + (size_t)a * m;
-    for (int j = lane; j < m; j += 64) dst[j] = mk(double(src[j].re), double(src[j].im));
+    cx<TB> *dst = blk + boff[b] + (size_t)a * m;
+    for (int j = lane; j < m; j += 64) dst[j] = cx<TB>{TB(src[j].re), TB(src[j].im)};
 }
 
-// In-place Gauss-Jordan inversion with partial (row) pivoting, one work-group per block,
-// the block held in LDS; the row interchanges are undone as column interchanges at the end.
-// The inverse is written back column-major.
-__global__ void __launch_bounds__(256) k_bj_invert(const int *blk_m, const long long *boff, cx<double> *blk, int *err)
+// In-place Gauss-Jordan inversion with partial (row) pivoting of one block (m <= 64) by
+// one wave, the block in LDS: during the elimination lane j owns column j, so a row swap,
+// the scaling of the pivot row and the update of every row are lane-parallel and need no
+// barrier (a wave's LDS operations complete in order).  The row interchanges are undone
+// as column interchanges at the end, lane = row, and the inverse is written back
+// column-major.  TB is the arithmetic and storage type (FP32 in the mixed form).
+template <typename TB>
+__global__ void __launch_bounds__(64) k_bj_invert(const int *blk_m, const long long *boff, cx<TB> *blk, int *err)
 {
-    __shared__ cx<double> A[kBlk * kBlk];
-    __shared__ cx<double> fac[kBlk];
+    __shared__ cx<TB> A[kBlk * kBlk];
     __shared__ int perm[kBlk];
-    __shared__ int piv;
-    const int b = blockIdx.x, m = blk_m[b], tid = threadIdx.x;
-    cx<double> *G = blk + boff[b];
-    for (int t = tid; t < m * m; t += blockDim.x) A[(t / m) * kBlk + t % m] = G[t];
-    __syncthreads();
+    const int b = blockIdx.x, m = blk_m[b], lane = threadIdx.x;
+    cx<TB> *G = blk + boff[b];
+    // row-major m x m from the gather
+    if (lane < m)
+        for (int i = 0; i < m; ++i) A[i * kBlk + lane] = G[(size_t)i * m + lane];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    bool singular = false;
     for (int k = 0; k < m; ++k) {
-        if (tid < 64) {
-            const int i = k + tid;
-            double v = -1.0;
-            int idx = i;
-            if (i < m) {
-                const cx<double> a = A[i * kBlk + k];
-                v = a.re * a.re + a.im * a.im;
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const double v2 = __shfl_xor(v, o);
-                const int i2 = __shfl_xor(idx, o);
-                if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
-            }
-            if (tid == 0) {
-                piv = idx;
-                perm[k] = idx;
-                if (!(v > 0.0)) atomicOr(err, 16);
+        // pivot: the largest |A[i][k]|, i >= k (first on ties)
+        TB v = TB(-1);
+        int idx = lane;
+        if (lane >= k && lane < m) {
+            const cx<TB> a = A[lane * kBlk + k];
+            v = a.re * a.re + a.im * a.im;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const TB v2 = __shfl_xor(v, o);
+            const int i2 = __shfl_xor(idx, o);
+            if (v2 > v || (v2 == v && i2 < idx)) {
+                v = v2;
+                idx = i2;
             }
         }
-        __syncthreads();
-        const int pr = piv;
-        if (pr != k)
-            for (int j = tid; j < m; j += blockDim.x) {
-                const cx<double> t0 = A[k * kBlk + j];
-                A[k * kBlk + j] = A[pr * kBlk + j];
-                A[pr * kBlk + j] = t0;
+        const int pr = __builtin_amdgcn_readfirstlane(idx);
+        if (!(v > TB(0))) singular = true;
+        if (lane == 0) perm[k] = pr;
+        cx<TB> rk = mk(TB(0), TB(0));
+        if (lane < m) {
+            rk = A[pr * kBlk + lane];
+            if (pr != k) {
+                A[pr * kBlk + lane] = A[k * kBlk + lane];
             }
-        __syncthreads();
-        const cx<double> pv = A[k * kBlk + k];
-        const cx<double> inv = cdiv(mk(1.0, 0.0), pv);
-        __syncthreads();
-        for (int j = tid; j < m; j += blockDim.x) A[k * kBlk + j] = j == k ? inv : A[k * kBlk + j] * inv;
-        for (int i = tid; i < m; i += blockDim.x) fac[i] = i == k ? mk(0.0, 0.0) : A[i * kBlk + k];
-        __syncthreads();
-        for (int t = tid; t < m * m; t += blockDim.x) {
-            const int i = t / m, j = t % m;
-            if (i == k) continue;
-            const cx<double> base = j == k ? mk(0.0, 0.0) : A[i * kBlk + j];
-            A[i * kBlk + j] = base - fac[i] * A[k * kBlk + j];
         }
-        __syncthreads();
+        const cx<TB> pv = {__shfl(rk.re, k), __shfl(rk.im, k)};   // the pivot, from lane k's column
+        const cx<TB> inv = cdiv(mk(TB(1), TB(0)), pv);
+        rk = lane == k ? inv : rk * inv;
+        if (lane < m) A[k * kBlk + lane] = rk;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        // every other row i: A[i][j] -= A[i][k] * rk_j (column k: 0 - A[i][k] * inv).  Sixteen
+        // rows per batch, their column-k entries read by every lane before any is written.
+        constexpr int RB = 16;
+        for (int i0 = 0; i0 < m; i0 += RB) {
+            cx<TB> f[RB], base[RB];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int i = min(i0 + q, m - 1);
+                f[q] = A[i * kBlk + k];
+                base[q] = A[i * kBlk + min(lane, m - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int i = i0 + q;
+                if (i < m && i != k && lane < m)
+                    A[i * kBlk + lane] = (lane == k ? mk(TB(0), TB(0)) : base[q]) - f[q] * rk;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
+    if (singular && lane == 0) atomicOr(err, 16);
+    // undo the row interchanges as column interchanges, lane = row
     for (int k = m - 1; k >= 0; --k) {
         const int pk = perm[k];
-        if (pk != k)
-            for (int i = tid; i < m; i += blockDim.x) {
-                const cx<double> t0 = A[i * kBlk + k];
-                A[i * kBlk + k] = A[i * kBlk + pk];
-                A[i * kBlk + pk] = t0;
-            }
-        __syncthreads();
+        if (pk != k && lane < m) {
+            const cx<TB> t0 = A[lane * kBlk + k];
+            A[lane * kBlk + k] = A[lane * kBlk + pk];
+            A[lane * kBlk + pk] = t0;
+        }
     }
-    // column-major: G[j * m + i] = inverse[i][j]
-    for (int t = tid; t < m * m; t += blockDim.x) G[t] = A[(t % m) * kBlk + t / m];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // column-major: G[j * m + i] = inverse[i][j], lane = i
+    if (lane < m)
+        for (int j = 0; j < m; ++j) G[(size_t)j * m + lane] = A[lane * kBlk + j];
 }
 
 // y = M^-1 x: one work-group per block; lane a is row a, the 4 waves split the columns
 // (four per step) of the column-major inverse and meet in LDS
-template <int NR>
+template <typename TB, int NR>
 __global__ void __launch_bounds__(256) k_bj_apply(int n, const int *blk_b0, const int *blk_m, const int *cl_elem,
-                                                  const long long *boff, const cx<double> *minv, const cx<double> *x,
+                                                  const long long *boff, const cx<TB> *minv, const cx<double> *x,
                                                   cx<double> *y)
 {
     __shared__ cx<double> xs[NR][kBlk];
@@ -1562,25 +1577,25 @@ __global__ void __launch_bounds__(256) k_bj_apply(int n, const int *blk_b0, cons
         xs[r][j] = x[(size_t)r * n + cl_elem[b0 + j]];
     }
     __syncthreads();
-    const cx<double> *M = minv + boff[b];
+    const cx<TB> *M = minv + boff[b];
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
     if (a < m) {
         int j = wv;
         for (; j + 12 < m; j += 16) {
-            cx<double> v[4];
+            cx<TB> v[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = M[(size_t)(j + 4 * q) * m + a];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v[q].re, v[q].im, xs[r][j + 4 * q]);
+                for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(v[q].re), double(v[q].im), xs[r][j + 4 * q]);
         }
         for (; j < m; j += 4) {
-            const cx<double> v = M[(size_t)j * m + a];
+            const cx<TB> v = M[(size_t)j * m + a];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], v.re, v.im, xs[r][j]);
+            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(v.re), double(v.im), xs[r][j]);
         }
     }
     if (wv > 0)
@@ -2428,29 +2443,38 @@ void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *el
               const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
               const void *nval, int mixed, const long long *boff, void *blk, int *err)
 {
-    if (mixed)
-        k_bj_gather<float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr, near_ptr,
-                                                            self_off, (const cx<float> *)nval, boff, (cx<double> *)blk);
-    else
-        k_bj_gather<double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
-                                                             near_ptr, self_off, (const cx<double> *)nval, boff,
-                                                             (cx<double> *)blk);
-    k_bj_invert<<<nblk, 256, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
+    if (mixed) {
+        k_bj_gather<float, float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
+                                                                   near_ptr, self_off, (const cx<float> *)nval, boff,
+                                                                   (cx<float> *)blk);
+        k_bj_invert<float><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<float> *)blk, err);
+    } else {
+        k_bj_gather<double, double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m,
+                                                                     cl_ptr, near_ptr, self_off,
+                                                                     (const cx<double> *)nval, boff, (cx<double> *)blk);
+        k_bj_invert<double><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
+    }
 }
-void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
-              const long long *boff, const void *minv, const void *x, void *y)
+template <typename TB>
+static void bj_apply_t(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m,
+                       const int *cl_elem, const long long *boff, const cx<TB> *M, const void *x, void *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
-        const cx<double> *M = (const cx<double> *)minv;
         switch (nr) {
-        case 1: k_bj_apply<1><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 2: k_bj_apply<2><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 3: k_bj_apply<3><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        default: k_bj_apply<4><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 1: k_bj_apply<TB, 1><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 2: k_bj_apply<TB, 2><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 3: k_bj_apply<TB, 3><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        default: k_bj_apply<TB, 4><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
         }
     });
+}
+void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
+              const long long *boff, const void *minv, int mixed, const void *x, void *y)
+{
+    if (mixed) bj_apply_t(s, n, nrhs, nblk, blk_b0, blk_m, cl_elem, boff, (const cx<float> *)minv, x, y);
+    else bj_apply_t(s, n, nrhs, nblk, blk_b0, blk_m, cl_elem, boff, (const cx<double> *)minv, x, y);
 }
 void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v)
 {

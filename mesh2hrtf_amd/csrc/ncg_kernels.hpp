@@ -121,8 +121,9 @@ void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const 
 void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
               const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
               const void *nval, int mixed, const long long *boff, void *blk, int *err);
+// mixed: the blocks are inverted and stored in FP32 (FP64 otherwise)
 void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
-              const long long *boff, const void *minv, const void *x, void *y);
+              const long long *boff, const void *minv, int mixed, const void *x, void *y);
 // v = w / ||w|| per load case, the norm read from a dots() result
 void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v);
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
