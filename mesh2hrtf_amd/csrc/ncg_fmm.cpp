@@ -175,27 +175,107 @@ void finish_level(const Mesh &M, Level &L)
     L.ravg /= (double)L.cl.size();
 }
 
-// near/far lists of one level (src/NC_Addresses.cpp:1046-1083)
-void lists(Level &L, double min_dist)
+// Near lists, interaction lists and the cancelled-pair counts of one level
+// (src/NC_Addresses.cpp:1046-1083; interaction lists NC_AllocateDmtxMLFMM,
+// src/NC_Addresses.cpp:1699-1767).  Clusters i, j are far when |c_i - c_j| > (2/sqrt 3)
+// (R_i + R_j) and > min_dist; geometrically far pairs closer than min_dist are cancelled
+// and stay near.  Only pairs closer than h = max(2 (2/sqrt 3) R_max, min_dist) (with a
+// margin) can be near or cancelled, so each cluster tests the clusters in the 27 grid
+// cells of side h around its centre; every other pair is far.  The interaction list of
+// a cluster is its far clusters (level 0 and single level) or its far clusters whose
+// fathers are near its father (the children of the father's near clusters, ascending).
+void lists(Level &L, double min_dist, const Level *F)
 {
     const int nc = (int)L.cl.size();
     const double fac = 2.0 / sqrt(3.0);
-    std::vector<char> far((size_t)nc * nc, 0);
-    L.n_canceled = L.n_far_geom = 0;
-    for (int i = 0; i < nc; ++i)
-        for (int j = 0; j < i; ++j) {
-            const double d = dist(L.cl[i].cen, L.cl[j].cen);
-            const double w = L.cl[i].radius + L.cl[j].radius;
-            if (d > fac * w) {
-                if (d > min_dist) far[(size_t)i * nc + j] = far[(size_t)j * nc + i] = 1;
-                else L.n_canceled++;
-                L.n_far_geom++;
-            }
+    double rmax = 0.0, mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (const Cluster &c : L.cl) {
+        rmax = std::max(rmax, c.radius);
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = std::min(mn[d], c.cen[d]);
+            mx[d] = std::max(mx[d], c.cen[d]);
         }
+    }
+    const double h = 1.001 * std::max(2.0 * fac * rmax, min_dist) + 1e-300;
+    long long dims[3], cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        dims[d] = std::min<long long>((long long)((mx[d] - mn[d]) / h) + 1, 1 << 20);
+        cells *= dims[d];
+    }
+    const bool grid = cells <= 8LL * nc + 64;
+    auto cell_of = [&](const double *c, int d) {
+        return grid ? std::min<long long>((long long)((c[d] - mn[d]) / h), dims[d] - 1) : 0;
+    };
+    // clusters bucketed by cell (one cell holding every cluster without the grid)
+    std::vector<int> start((size_t)(grid ? cells : 1) + 1, 0), item(nc);
+    std::vector<long long> cid(nc);
     for (int i = 0; i < nc; ++i) {
-        L.cl[i].near.clear();
-        L.cl[i].far.clear();
-        for (int j = 0; j < nc; ++j) (far[(size_t)i * nc + j] ? L.cl[i].far : L.cl[i].near).push_back(j);
+        const double *c = L.cl[i].cen;
+        cid[i] = grid ? (cell_of(c, 0) * dims[1] + cell_of(c, 1)) * dims[2] + cell_of(c, 2) : 0;
+        start[cid[i] + 1]++;
+    }
+    for (size_t q = 1; q < start.size(); ++q) start[q] += start[q - 1];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < nc; ++i) item[fill[cid[i]]++] = i;
+    }
+    long long geom_near = 0, canceled = 0;   // pairs j < i
+    for (int i = 0; i < nc; ++i) {
+        std::vector<int> &nr = L.cl[i].near;
+        nr.clear();
+        const double *ci = L.cl[i].cen;
+        long long lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            const long long k = cell_of(ci, d);
+            lo[d] = grid ? std::max(0LL, k - 1) : 0;
+            hi[d] = grid ? std::min(dims[d] - 1, k + 1) : 0;
+        }
+        for (long long a = lo[0]; a <= hi[0]; ++a)
+            for (long long b = lo[1]; b <= hi[1]; ++b)
+                for (long long c = lo[2]; c <= hi[2]; ++c) {
+                    const long long q = grid ? (a * dims[1] + b) * dims[2] + c : 0;
+                    for (int t = start[q]; t < start[q + 1]; ++t) {
+                        const int j = item[t];
+                        if (j == i) {
+                            nr.push_back(j);
+                            continue;
+                        }
+                        const double dd = dist(ci, L.cl[j].cen);
+                        const double w = L.cl[i].radius + L.cl[j].radius;
+                        const bool far_geom = dd > fac * w;
+                        const bool far = far_geom && dd > min_dist;
+                        if (!far) nr.push_back(j);
+                        if (j < i) {
+                            if (!far_geom) geom_near++;
+                            else if (!far) canceled++;
+                        }
+                    }
+                }
+        std::sort(nr.begin(), nr.end());
+    }
+    L.n_far_geom = (long long)nc * (nc - 1) / 2 - geom_near;
+    L.n_canceled = canceled;
+    // interaction lists
+    std::vector<int> mark(nc, -1);
+    std::vector<int> child_begin, child_end;
+    if (F) {
+        child_begin.assign(F->cl.size(), 0);
+        child_end.assign(F->cl.size(), 0);
+        for (int j = nc - 1; j >= 0; --j) child_begin[L.cl[j].father] = j;
+        for (int j = 0; j < nc; ++j) child_end[L.cl[j].father] = j + 1;
+    }
+    for (int i = 0; i < nc; ++i) {
+        Cluster &c = L.cl[i];
+        for (int j : c.near) mark[j] = i;
+        c.fan.clear();
+        if (!F) {
+            for (int j = 0; j < nc; ++j)
+                if (mark[j] != i) c.fan.push_back(j);
+        } else {
+            for (int f : F->cl[c.father].near)
+                for (int j = child_begin[f]; j < child_end[f]; ++j)
+                    if (mark[j] != i) c.fan.push_back(j);
+        }
     }
 }
 
@@ -485,23 +565,8 @@ bool make_plan(const Mesh &M, const Options &O, double k, Plan &P)
             tree = &M.trees.back();
         }
         P.lev = tree->lev;
-        for (Level &L : P.lev) lists(L, P.min_dist);
-        if (method == 3) {
-            // interaction lists (NC_AllocateDmtxMLFMM, src/NC_Addresses.cpp:1699-1767)
-            for (int lv = 0; lv < nlev; ++lv) {
-                Level &L = P.lev[lv];
-                for (Cluster &c : L.cl) {
-                    c.fan.clear();
-                    if (lv == 0) { c.fan = c.far; continue; }
-                    const Level &F = P.lev[lv - 1];
-                    const std::vector<int> &fn = F.cl[c.father].near;
-                    for (int j : c.far)
-                        if (std::find(fn.begin(), fn.end(), L.cl[j].father) != fn.end()) c.fan.push_back(j);
-                }
-            }
-        }
-        if (method == 1)
-            for (Cluster &c : P.lev[0].cl) c.fan = c.far;
+        for (size_t lv = 0; lv < P.lev.size(); ++lv)
+            lists(P.lev[lv], P.min_dist, method == 3 && lv > 0 ? &P.lev[lv - 1] : nullptr);
         // canceled-pair ratios (src/NC_Addresses.cpp:224-283)
         P.cancel_ratio.clear();
         bool fallback = false;
@@ -528,8 +593,25 @@ bool make_plan(const Mesh &M, const Options &O, double k, Plan &P)
         std::string err;
         if (!expansion(L, k, O.min_terms, err)) { P.error = err; return false; }
     }
+    // the evaluation clusters depend on the mesh and the cluster edge only
+    Tree *tree = nullptr;
+    for (Tree &t : M.trees)
+        if (t.method == P.method && t.nlev == (int)P.lev.size()) tree = &t;
+    if (tree && tree->ev_ready && tree->ev_edge == P.clus_edge) {
+        P.ev = tree->ev;
+        P.ev_rmax = tree->ev_rmax;
+        P.ev_rmin = tree->ev_rmin;
+        return true;
+    }
     std::string err;
     if (!eval_clusters(M, P, err)) { P.error = err; return false; }
+    if (tree) {
+        tree->ev = P.ev;
+        tree->ev_rmax = P.ev_rmax;
+        tree->ev_rmin = P.ev_rmin;
+        tree->ev_edge = P.clus_edge;
+        tree->ev_ready = true;
+    }
     return true;
 }
 

@@ -17,9 +17,8 @@ struct Cluster {
     double cen[3] = {0, 0, 0};
     double radius = 0.0;
     int father = -1;
-    std::vector<int> near;      // clusters of this level not in the far field (incl. itself)
-    std::vector<int> far;       // far clusters
-    std::vector<int> fan;       // interaction list (far here, fathers near)
+    std::vector<int> near;      // clusters of this level not in the far field (incl. itself), ascending
+    std::vector<int> fan;       // interaction list (far here, fathers near), ascending
 };
 
 struct Level {
@@ -38,10 +37,14 @@ struct EvalCluster {
 };
 
 // Cluster tree of one (method, number of levels): clusters with their elements, centres,
-// radii and fathers, before the frequency-dependent near/far split.
+// radii and fathers, before the frequency-dependent near/far split, and the evaluation
+// clusters once computed (they depend on the cluster edge only).
 struct Tree {
     int method = -1, nlev = 0;
     std::vector<Level> lev;
+    bool ev_ready = false;
+    double ev_edge = 0.0, ev_rmax = 0.0, ev_rmin = 0.0;
+    std::vector<EvalCluster> ev;
 };
 
 // Frequency-independent mesh data the plan needs (boundary elements only).
