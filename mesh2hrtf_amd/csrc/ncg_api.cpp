@@ -224,14 +224,14 @@ struct DevFmm {
 };
 
 // FNV-1a over integer sequences: structure signatures of plans and pair lists
+// 64-bit digest for change detection: FNV-1a offset, one multiply-xorshift step per word
 struct Fnv {
     unsigned long long h = 1469598103934665603ull;
     void add(long long v)
     {
-        for (int b = 0; b < 8; ++b) {
-            h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
-            h *= 1099511628211ull;
-        }
+        h ^= (unsigned long long)v;
+        h *= 0x100000001B3ull;
+        h ^= h >> 29;
     }
     void add(const std::vector<int> &v)
     {
@@ -666,20 +666,22 @@ int upload_grow(Grow &g, hipStream_t s, const std::vector<T> &src)
 }
 
 // structure signature of a plan: method, clusters, interaction lists, finest near lists
+// Within one problem the cluster tree is fixed by (method, levels, cluster edge), and the
+// near and interaction lists then follow from the set of cancelled pairs of each level.
 unsigned long long plan_signature(const ncgf::Plan &P)
 {
     Fnv f;
     f.add((long long)P.method);
     f.add((long long)P.lev.size());
+    long long edge;
+    memcpy(&edge, &P.clus_edge, sizeof edge);
+    f.add(edge);
     for (const ncgf::Level &L : P.lev) {
         f.add((long long)L.cl.size());
-        for (const ncgf::Cluster &c : L.cl) {
-            f.add(c.elems);
-            f.add(c.fan);
-        }
+        f.add(L.n_far_geom);
+        f.add(L.n_canceled);
+        f.add((long long)L.cancel_hash);
     }
-    if (!P.lev.empty())
-        for (const ncgf::Cluster &c : P.lev.back().cl) f.add(c.near);
     return f.h;
 }
 

@@ -220,6 +220,7 @@ void lists(Level &L, double min_dist, const Level *F)
         for (int i = 0; i < nc; ++i) item[fill[cid[i]]++] = i;
     }
     long long geom_near = 0, canceled = 0;   // pairs j < i
+    unsigned long long cancel_hash = 0;
     for (int i = 0; i < nc; ++i) {
         std::vector<int> &nr = L.cl[i].near;
         nr.clear();
@@ -247,7 +248,13 @@ void lists(Level &L, double min_dist, const Level *F)
                         if (!far) nr.push_back(j);
                         if (j < i) {
                             if (!far_geom) geom_near++;
-                            else if (!far) canceled++;
+                            else if (!far) {
+                                canceled++;
+                                // order-independent digest of the cancelled pairs
+                                unsigned long long v = ((unsigned long long)i << 32) ^ (unsigned long long)j;
+                                v *= 0x9E3779B97F4A7C15ull;
+                                cancel_hash += v ^ (v >> 31);
+                            }
                         }
                     }
                 }
@@ -255,6 +262,7 @@ void lists(Level &L, double min_dist, const Level *F)
     }
     L.n_far_geom = (long long)nc * (nc - 1) / 2 - geom_near;
     L.n_canceled = canceled;
+    L.cancel_hash = cancel_hash;
     // interaction lists
     std::vector<int> mark(nc, -1);
     std::vector<int> child_begin, child_end;

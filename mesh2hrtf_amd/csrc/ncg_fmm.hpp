@@ -28,6 +28,9 @@ struct Level {
     std::vector<double> u;      // [P][3] unit vectors
     std::vector<double> w;      // [P] weights
     long long n_canceled = 0, n_far_geom = 0;
+    // digest of the cancelled pairs: with the tree fixed, the near and interaction lists
+    // are a function of this set
+    unsigned long long cancel_hash = 0;
 };
 
 struct EvalCluster {

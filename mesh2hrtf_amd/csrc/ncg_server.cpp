@@ -48,6 +48,7 @@ struct MeshRec {
     std::vector<double> xyz, eval;
     ncgf::Mesh pm;
     ncgf::Options po;
+    std::mutex pm_mu;   // pm caches cluster trees: one planner at a time
 };
 
 // owned copy of one ncg_frequency
@@ -376,7 +377,7 @@ int ncg_server_submit(ncg_server *S, const ncg_job *job, int64_t *job_id)
     if (job->n_freq <= 0 || !job->freq) return sfail(NCG_E_ARG, "ncg_server_submit: no frequencies");
     if (job->precision != NCG_FP64 && job->precision != NCG_MIXED)
         return sfail(NCG_E_ARG, "ncg_server_submit: precision must be NCG_FP64 or NCG_MIXED");
-    const MeshRec *M;
+    MeshRec *M;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if (job->mesh < 0 || job->mesh >= (int)S->meshes.size()) return sfail(NCG_E_ARG, "ncg_server_submit: unknown mesh");
@@ -413,9 +414,14 @@ int ncg_server_submit(ncg_server *S, const ncg_job *job, int64_t *job_id)
         c.f.plane_waves = f.n_plane_waves > 0 ? c.pw.data() : nullptr;
         kmax = std::max(kmax, f.k);
     }
-    std::lock_guard<std::mutex> lk(S->mu);
-    J->bytes = ncgi::estimate_bytes(M->pm, M->po, m, kmax, job->precision, nr, job->solver.restart > 0 ? job->solver.restart : 50);
+    {
+        // the footprint estimate plans the highest frequency; lanes keep running meanwhile
+        std::lock_guard<std::mutex> lp(M->pm_mu);
+        J->bytes = ncgi::estimate_bytes(M->pm, M->po, m, kmax, job->precision, nr,
+                                        job->solver.restart > 0 ? job->solver.restart : 50);
+    }
     if (J->bytes == 0) return sfail(NCG_E_ARG, "ncg_server_submit: no FMM plan at the highest frequency");
+    std::lock_guard<std::mutex> lk(S->mu);
     J->id = S->next_id++;
     J->t_submit = clk::now();
     Job *jp = J.get();
