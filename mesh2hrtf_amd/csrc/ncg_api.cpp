@@ -204,6 +204,7 @@ struct DevFmm {
     double *ecen = nullptr;
     int npair = 0, nev = 0;
     Grow tab, z, fphi, fgrad;
+    Grow dcoef;                   // translation series coefficients while D is built
     void release_structure()
     {
         for (DevLevel &L : lev) L.release();
@@ -219,7 +220,7 @@ struct DevFmm {
     void release()
     {
         release_structure();
-        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad, &bj_blk}) g->release();
+        for (Grow *g : {&xs, &tab, &z, &fphi, &fgrad, &bj_blk, &dcoef}) g->release();
     }
 };
 
@@ -864,7 +865,9 @@ int assemble_fmm(ncg_problem *p)
     ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0,
                         F.near_val.p, mixed, p->rhs, F.diag_pos);
     for (DevLevel &D : F.lev) {
-        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32);
+        if ((rc = F.dcoef.ensure(sizeof(double) * (size_t)std::max(D.nfan, 1) * (3 + 2 * D.L)))) return rc;
+        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32,
+                    F.dcoef.d());
         ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, D.f32);
         ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u.d(), D.w.d(), D.S.p, D.f32);
         if (p->nsrc) {

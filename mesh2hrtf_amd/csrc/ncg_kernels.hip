@@ -1164,25 +1164,63 @@ __device__ __forceinline__ cx<double> translation(int L, double c, double h, con
     return mu;
 }
 
-// D values of one level: one block per interaction q = (cluster i, far cluster j)
-template <typename S>
-__global__ void __launch_bounds__(128) k_fmm_D(int L, int P, double k, double h, const double *ccen, int ncl,
-                                               const int *fan_row, const int *fan_idx, const double *u, cx<S> *Dv)
+// D values of one level (NC_BuildD*FMM, src/NC_EquationSystem.cpp:488-547, 838-900):
+// D_q(s_p) = sum_{n<L} (i h)^n (2n + 1) h_n(k |C_i - C_j|) P_n(d_q . s_p) per interaction
+// q = (cluster i, far cluster j).  First pass, a thread per interaction: the direction
+// d_q and the series coefficients a_n = (i h)^n (2n + 1) h_n into coef[q][3 + 2L].
+__global__ void __launch_bounds__(128) k_fmm_D_coef(int nfan, int L, double k, double h, const double *ccen,
+                                                    const int *fan_row, const int *fan_idx, double *coef)
 {
-    __shared__ double hre[kMaxTerms], him[kMaxTerms], dhat[3];
-    const int q = blockIdx.x;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nfan) return;
     const int i = fan_row[q], j = fan_idx[q];
-    if (threadIdx.x == 0) {
-        double d[3] = {ccen[3 * i] - ccen[3 * j], ccen[3 * i + 1] - ccen[3 * j + 1], ccen[3 * i + 2] - ccen[3 * j + 2]};
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        dhat[0] = d[0] / r; dhat[1] = d[1] / r; dhat[2] = d[2] / r;
-        spherical_hankel(L, r * k, h, hre, him);
+    const double d[3] = {ccen[3 * i] - ccen[3 * j], ccen[3 * i + 1] - ccen[3 * j + 1], ccen[3 * i + 2] - ccen[3 * j + 2]};
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double hre[kMaxTerms], him[kMaxTerms];
+    spherical_hankel(L, r * k, h, hre, him);
+    double *o = coef + (size_t)q * (3 + 2 * L);
+    o[0] = d[0] / r;
+    o[1] = d[1] / r;
+    o[2] = d[2] / r;
+    cx<double> zi = mk(1.0, 0.0);
+    for (int n = 0; n < L; ++n) {
+        const cx<double> an = zi * mk(hre[n], him[n]) * double(2 * n + 1);
+        o[3 + 2 * n] = an.re;
+        o[4 + 2 * n] = an.im;
+        zi = mk(-zi.im * h, zi.re * h);
+    }
+}
+
+// Second pass: one block per interaction, threads over the sphere points, the series
+// summed in FP64 with the Legendre recurrence P_n = ((2n-1) c P_{n-1} - (n-1) P_{n-2}) / n
+// (the two ratios tabulated per block); stored as S.
+template <typename S>
+__global__ void __launch_bounds__(128) k_fmm_D(int L, int P, const double *coef, const double *u, cx<S> *Dv)
+{
+    __shared__ double cf[3 + 2 * kMaxTerms], ca[kMaxTerms], cb[kMaxTerms];
+    const int q = blockIdx.x, w = 3 + 2 * L;
+    for (int t = threadIdx.x; t < w; t += blockDim.x) cf[t] = coef[(size_t)q * w + t];
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+        ca[t] = t >= 2 ? (double)(2 * t - 1) / (double)t : 0.0;
+        cb[t] = t >= 2 ? (double)(t - 1) / (double)t : 0.0;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        const double c = dhat[0] * u[3 * p] + dhat[1] * u[3 * p + 1] + dhat[2] * u[3 * p + 2];
-        const cx<double> mu = translation(L, c, h, hre, him);
-        Dv[(size_t)q * P + p] = cx<S>{S(mu.re), S(mu.im)};
+        const double c = cf[0] * u[3 * p] + cf[1] * u[3 * p + 1] + cf[2] * u[3 * p + 2];
+        double mre = cf[3], mim = cf[4];
+        double p0 = 1.0, p1 = c;
+        if (L > 1) {
+            mre += cf[5] * c;
+            mim += cf[6] * c;
+        }
+        for (int n = 2; n < L; ++n) {
+            const double pn = ca[n] * c * p1 - cb[n] * p0;
+            p0 = p1;
+            p1 = pn;
+            mre += cf[3 + 2 * n] * pn;
+            mim += cf[4 + 2 * n] * pn;
+        }
+        Dv[(size_t)q * P + p] = cx<S>{S(mre), S(mim)};
     }
 }
 
@@ -1973,9 +2011,13 @@ __global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, doubl
     o[1] = d[1] / r;
     o[2] = d[2] / r;
     o[3] = 0.0;
+    // series coefficients a_n = (i h)^n (2n + 1) h_n(k r)
+    cx<double> zi = mk(1.0, 0.0);
     for (int n = 0; n < L; ++n) {
-        o[4 + 2 * n] = hre[n];
-        o[5 + 2 * n] = him[n];
+        const cx<double> an = zi * mk(hre[n], him[n]) * double(2 * n + 1);
+        o[4 + 2 * n] = an.re;
+        o[5 + 2 * n] = an.im;
+        zi = mk(-zi.im * h, zi.re * h);
     }
 }
 
@@ -1991,16 +2033,14 @@ __global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, doubl
 {
     constexpr int QB = 16;
     __shared__ T sh[QB][4 + 2 * kMaxTerms];
-    __shared__ T ca[kMaxTerms], cb[kMaxTerms], cn[kMaxTerms];
+    __shared__ T ca[kMaxTerms], cb[kMaxTerms];
     __shared__ int scl[QB];
     const int a = blockIdx.x, w = 4 + 2 * L;
     const size_t ldt = (size_t)ncl * P, ldz = (size_t)nev * P;
-    const T hh = T(h);
     // Legendre recurrence P_n = ((2n-1) c P_{n-1} - (n-1) P_{n-2}) / n as P_n = ca c P_{n-1} - cb P_{n-2}
     for (int t = threadIdx.x; t < L; t += blockDim.x) {
         ca[t] = t >= 2 ? T((double)(2 * t - 1) / (double)t) : T(0);
         cb[t] = t >= 2 ? T((double)(t - 1) / (double)t) : T(0);
-        cn[t] = T(2 * t + 1);
     }
     for (int p0 = 0; p0 < P; p0 += blockDim.x) {
         const int p = p0 + threadIdx.x;
@@ -2021,17 +2061,20 @@ __global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, doubl
             if (!on) continue;
             for (int qq = 0; qq < nb; ++qq) {
                 const T *s = sh[qq];
+                // mu = sum_n a_n P_n(cc), a_n = (i h)^n (2n + 1) h_n from the table
                 const T cc = s[0] * su[0] + s[1] * su[1] + s[2] * su[2];
                 cx<T> mu = mk(s[4], s[5]);   // n = 0
-                cx<T> zi = mk(T(0), hh);     // (i h)^1
                 T pm1 = T(1), pm0 = cc;
-                if (L > 1) mu += zi * mk(s[6], s[7]) * (cc * cn[1]);
+                if (L > 1) {
+                    mu.re += s[6] * cc;
+                    mu.im += s[7] * cc;
+                }
                 for (int nn = 2; nn < L; ++nn) {
                     const T pn = ca[nn] * cc * pm0 - cb[nn] * pm1;
                     pm1 = pm0;
                     pm0 = pn;
-                    zi = mk(-zi.im * hh, zi.re * hh);
-                    mu += zi * mk(s[4 + 2 * nn], s[5 + 2 * nn]) * (pn * cn[nn]);
+                    mu.re += s[4 + 2 * nn] * pn;
+                    mu.im += s[5 + 2 * nn] * pn;
                 }
                 const cx<double> md = mk(double(mu.re * wp), double(mu.im * wp));
                 const int c = scl[qq];
@@ -2363,11 +2406,12 @@ void field(hipStream_t s, const DevElements &D, const BMParams &b, const double 
 
 // ---- FMM launchers -----------------------------------------------------------------
 void fmm_D(hipStream_t s, int L, int P, double k, double h, const double *ccen, int ncl, const int *fan_row,
-           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed)
+           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed, double *coef)
 {
     if (nfan == 0) return;
-    if (mixed) k_fmm_D<float><<<nfan, 128, 0, s>>>(L, P, k, h, ccen, ncl, fan_row, fan_idx, u, (cx<float> *)Dv);
-    else k_fmm_D<double><<<nfan, 128, 0, s>>>(L, P, k, h, ccen, ncl, fan_row, fan_idx, u, (cx<double> *)Dv);
+    k_fmm_D_coef<<<blocks_for(nfan, 128), 128, 0, s>>>(nfan, L, k, h, ccen, fan_row, fan_idx, coef);
+    if (mixed) k_fmm_D<float><<<nfan, 128, 0, s>>>(L, P, coef, u, (cx<float> *)Dv);
+    else k_fmm_D<double><<<nfan, 128, 0, s>>>(L, P, coef, u, (cx<double> *)Dv);
 }
 void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, int P, const double *ccen,
            const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed)

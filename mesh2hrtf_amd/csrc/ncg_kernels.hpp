@@ -102,8 +102,9 @@ void field(hipStream_t s, const DevElements &D, const BMParams &b, const double 
            const void *fphi = nullptr, const void *fgrad = nullptr);
 
 // fast multipole operators (layouts in ncg_kernels.hip)
+// coef: scratch of nfan * (3 + 2 L) doubles (series coefficients per interaction)
 void fmm_D(hipStream_t s, int L, int P, double k, double h, const double *ccen, int ncl, const int *fan_row,
-           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed);
+           const int *fan_idx, int nfan, const double *u, void *Dv, int mixed, double *coef);
 void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, int P, const double *ccen,
            const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed);
 void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
