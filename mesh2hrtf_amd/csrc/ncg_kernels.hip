@@ -1110,6 +1110,8 @@ constexpr int kMaxTerms = 80;
 // h_n = j_n + i h y_n, n < N (NC_SphericalHankel, src/NC_3dFunctions.cpp:821-886): y_n by
 // upward recurrence, j_n by downward recurrence from the continued fraction of
 // j_N/j_{N-1} (modified Lentz), normalised with j_0 = sin x / x (or j_0' when j_0 ~ 0).
+// When x > N + 10 every order is well below the argument and j_n also comes from the
+// upward recurrence (the continued fraction would take O(x) steps there).
 __device__ void spherical_hankel(int N, double x, double h, double *hre, double *him)
 {
     double gg[kMaxTerms], dg[kMaxTerms];
@@ -1117,6 +1119,13 @@ __device__ void spherical_hankel(int N, double x, double h, double *hre, double 
     him[0] = -cxs / x;
     if (N > 1) him[1] = -(cxs / x + sx) / x;
     for (int i = 2; i < N; ++i) him[i] = him[i - 1] * (double)(2 * i - 1) / x - him[i - 2];
+    if (x > (double)N + 10.0) {
+        hre[0] = sx / x;
+        if (N > 1) hre[1] = (sx / x - cxs) / x;
+        for (int i = 2; i < N; ++i) hre[i] = hre[i - 1] * (double)(2 * i - 1) / x - hre[i - 2];
+        for (int i = 0; i < N; ++i) him[i] = h * him[i];
+        return;
+    }
     const double nu = (double)(N - 1);
     double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
     for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
@@ -1261,9 +1270,10 @@ __global__ void __launch_bounds__(256) k_fmm_T(Elements E, BMConst bm, const dou
         point_on(L, s, tt, q, ny, J);   // q = y - C
         const double wga = w * J;
         const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
-        double sn, cs;
-        sincos(dw, &sn, &cs);
-        cx<double> z = mk(cs, sn);
+        // the phase in the storage precision (FP32 in the mixed form), formed in FP64
+        S sn, cs;
+        sincos_t(S(dw), &sn, &cs);
+        cx<double> z = mk(double(cs), double(sn));
         if (kind == 1) {
             z = z * wga;
             z = mk(z.im, -z.re);   // times -i
@@ -1335,9 +1345,10 @@ __global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, co
     const double d[3] = {E.cen[e] - ccen[3 * c], E.cen[E.n + e] - ccen[3 * c + 1], E.cen[2 * E.n + e] - ccen[3 * c + 2]};
     const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
     const double d0 = (d[0] * su[0] + d[1] * su[1] + d[2] * su[2]) * bm.hk;
-    double sn, cs;
-    sincos(d0, &sn, &cs);
-    const cx<double> z0 = mk(cs, sn) * wsph[p];
+    // the phase in the storage precision (FP32 in the mixed form), formed in FP64
+    S sn, cs;
+    sincos_t(S(d0), &sn, &cs);
+    const cx<double> z0 = mk(double(cs), double(sn)) * wsph[p];
     const double dn = (su[0] * E.nrm[e] + su[1] * E.nrm[E.n + e] + su[2] * E.nrm[2 * E.n + e]) * bm.hk;
     const cx<double> zd = mk(-z0.im * dn, z0.re * dn);
     const cx<double> beta_tau = mk(0.0, bm.beta_im * bm.tau);
