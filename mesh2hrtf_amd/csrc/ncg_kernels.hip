@@ -1245,31 +1245,43 @@ __device__ __forceinline__ int fmm_gauss(int nv, int g, double &s, double &t, do
     return 4;
 }
 
-// T values of one level: thread per (element, sphere point)
-// (NC_ComputeEntriesTforFMM, src/NC_EquationSystem.cpp:1150-1240)
+// T values of one level (NC_ComputeEntriesTforFMM, src/NC_EquationSystem.cpp:1150-1240).
+// A work-group takes 4 elements x 64 sphere points: the 4 aggregation Gauss points of
+// each element (point relative to the cluster centre, normal, weight x Jacobian) are
+// formed once in FP64 into LDS, then each thread forms one (element, point) entry.
 template <typename S>
 __global__ void __launch_bounds__(256) k_fmm_T(Elements E, BMConst bm, const double *adm, int P, const double *ccen,
                                                const int *elem_cl, const int *elem_pos, const double *u, cx<S> *Tv)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)E.n * P) return;
-    const int e = (int)(t / P), p = (int)(t % P);
-    const int c = elem_cl[e];
-    const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
+    __shared__ double gp[4][4][7];   // [element][gauss] q(3), ny(3), w*J
+    const int le = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + le, p = blockIdx.y * 64 + lane;
+    if (lane < 4 && e < E.n) {
+        const int c = elem_cl[e];
+        const double C[3] = {ccen[3 * c], ccen[3 * c + 1], ccen[3 * c + 2]};
+        LocalElem L;
+        load_local(E, e, C, L);   // nodes relative to the cluster centre
+        double s, tt, w;
+        fmm_gauss(L.nv, lane, s, tt, w);
+        double q[3], ny[3], J;
+        point_on(L, s, tt, q, ny, J);
+        for (int d = 0; d < 3; ++d) {
+            gp[le][lane][d] = q[d];
+            gp[le][lane][3 + d] = ny[d];
+        }
+        gp[le][lane][6] = w * J;
+    }
+    __syncthreads();
+    if (e >= E.n || p >= P) return;
     const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
-    LocalElem L;
-    load_local(E, e, C, L);   // nodes relative to the cluster centre
     const int kind = E.kind[e];
     const cx<double> ad = mk(adm[2 * e], adm[2 * e + 1]);
     const bool has_adm = kind == 2 && (ad.re != 0.0 || ad.im != 0.0);
     cx<double> acc = mk(0.0, 0.0);
     for (int g = 0; g < 4; ++g) {
-        double s, tt, w;
-        fmm_gauss(L.nv, g, s, tt, w);
-        double q[3], ny[3], J;
-        point_on(L, s, tt, q, ny, J);   // q = y - C
-        const double wga = w * J;
-        const double dw = -(q[0] * su[0] + q[1] * su[1] + q[2] * su[2]) * bm.hk;
+        const double *G = gp[le][g];
+        const double wga = G[6];
+        const double dw = -(G[0] * su[0] + G[1] * su[1] + G[2] * su[2]) * bm.hk;
         // the phase in the storage precision (FP32 in the mixed form), formed in FP64
         S sn, cs;
         sincos_t(S(dw), &sn, &cs);
@@ -1278,7 +1290,7 @@ __global__ void __launch_bounds__(256) k_fmm_T(Elements E, BMConst bm, const dou
             z = z * wga;
             z = mk(z.im, -z.re);   // times -i
         } else {
-            const double d1 = (ny[0] * su[0] + ny[1] * su[1] + ny[2] * su[2]) * bm.k * bm.tau;
+            const double d1 = (G[3] * su[0] + G[4] * su[1] + G[5] * su[2]) * bm.k * bm.tau;
             if (!has_adm) z = z * (d1 * wga);
             else z = z * (mk(d1 - bm.hk / bm.k * ad.im, bm.hk / bm.k * ad.re) * wga);
         }
@@ -2428,7 +2440,7 @@ void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double 
            const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    const unsigned g = blocks_for((size_t)D.n * P, 256);
+    const dim3 g(blocks_for(D.n, 4), (P + 63) / 64);
     if (mixed) k_fmm_T<float><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<float> *)Tv);
     else k_fmm_T<double><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<double> *)Tv);
 }
