@@ -155,6 +155,8 @@ struct DevLevel {
     int *fan_ptr = nullptr, *fan_idx = nullptr, *fan_row = nullptr;
     int nch = 0;                      // row chunks: <= 16 elements of one cluster each
     int *ch_cl = nullptr, *ch_r0 = nullptr, *ch_m = nullptr;
+    int ntl = 0;                      // tiles: <= 64 elements of one cluster each (local first)
+    int *tl_cl = nullptr, *tl_a0 = nullptr, *tl_m = nullptr;
     int P = 0, L = 0;
     int f32 = 0;                      // T, D, S (and t, s) stored in FP32 (mixed form, regular levels only)
     double dmin = 1e300;              // shortest centre distance of an interaction pair
@@ -166,6 +168,15 @@ struct DevLevel {
         r.cl = ch_cl;
         r.r0 = ch_r0;
         r.m = ch_m;
+        return r;
+    }
+    ncgk::RowChunks tiles() const
+    {
+        ncgk::RowChunks r;
+        r.n = ntl;
+        r.cl = tl_cl;
+        r.r0 = tl_a0;
+        r.m = tl_m;
         return r;
     }
     void release()
@@ -461,7 +472,8 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
             }
             {
                 KernelScope ks(p, 6 + L.f32, chunks * n * P * sf + nr * ncl * P * sf + 2.0 * nr * n * 16.0);
-                ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.chunks(), L.cl_elem, L.S.p, L.f32, L.sw.p, y);
+                ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.tiles(), L.cl_ptr, L.cl_elem, L.S.p, L.f32,
+                                  L.sw.p, y);
             }
         }
     }
@@ -730,7 +742,21 @@ int fmm_structure(ncg_problem *p)
             }
         }
         D.nch = (int)ch_cl.size();
+        std::vector<int> tl_cl, tl_a0, tl_m;
+        for (int c = 0; c < D.ncl; ++c) {
+            const int m = cl_ptr[c + 1] - cl_ptr[c], nt = (m + 63) / 64;
+            for (int q = 0, a0 = 0; q < nt; ++q) {
+                const int mq = m / nt + (q < m % nt ? 1 : 0);
+                tl_cl.push_back(c);
+                tl_a0.push_back(a0);
+                tl_m.push_back(mq);
+                a0 += mq;
+            }
+        }
+        D.ntl = (int)tl_cl.size();
         if ((rc = upload(F.st, s, &D.ccen, cc)) || (rc = upload(F.st, s, &D.ch_cl, ch_cl)) ||
+            (rc = upload(F.st, s, &D.tl_cl, tl_cl)) || (rc = upload(F.st, s, &D.tl_a0, tl_a0)) ||
+            (rc = upload(F.st, s, &D.tl_m, tl_m)) ||
             (rc = upload(F.st, s, &D.ch_r0, ch_r0)) || (rc = upload(F.st, s, &D.ch_m, ch_m)) || (rc = upload(F.st, s, &D.cl_ptr, cl_ptr)) ||
             (rc = upload(F.st, s, &D.cl_elem, cl_elem)) || (rc = upload(F.st, s, &D.elem_cl, elem_cl)) ||
             (rc = upload(F.st, s, &D.elem_pos, elem_pos)) || (rc = upload(F.st, s, &D.fan_ptr, fan_ptr)) ||
@@ -869,12 +895,12 @@ int assemble_fmm(ncg_problem *p)
         ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32,
                     F.dcoef.d());
         ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, D.f32);
-        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.u.d(), D.w.d(), D.S.p, D.f32);
+        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.cl_ptr, D.cl_elem, D.u.d(), D.w.d(), D.S.p, D.f32);
         if (p->nsrc) {
             // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
             ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u.d(), p->bcval, nr, D.tv.d());
             ncgk::fmm_apply_D(s, D.ncl, D.P, nr, D.fan_ptr, D.fan_idx, D.D.p, D.f32, 1, D.tv.d(), D.sw.p);
-            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.chunks(), D.cl_elem, D.S.p, D.f32, D.sw.p, p->rhs);
+            ncgk::fmm_apply_S(s, n, D.ncl, D.P, nr, D.tiles(), D.cl_ptr, D.cl_elem, D.S.p, D.f32, D.sw.p, p->rhs);
         }
     }
     return 0;

@@ -1102,7 +1102,8 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
 // T, a diagonal translation D between interacting clusters and a disaggregation S;
 // src/NC_EquationSystem.cpp:305-1240, src/NC_CommonFunctions.cpp:1247-1316).
 // Layouts (p, the unit-sphere point, fastest so that lanes coalesce):
-//   T[(cl_ptr[c] + local element) * P + p]   S[e * P + p]   D[q * P + p], q = interaction
+//   T[(cl_ptr[c] + local element) * P + p]   S[cl_ptr[c] * P + p * m_c + local element]
+//   D[q * P + p], q = interaction
 //   t, s work vectors [nrhs][ncl * P]
 // ======================================================================================
 constexpr int kMaxTerms = 80;
@@ -1344,16 +1345,20 @@ __global__ void __launch_bounds__(256) k_fmm_Tvec_bc(Elements E, BMConst bm, int
     }
 }
 
-// S values of one level: thread per (element, sphere point)
-// (NC_BuildSforSLFMM, src/NC_EquationSystem.cpp:550-609)
+// S values of one level (NC_BuildS*FMM, src/NC_EquationSystem.cpp:550-609, 903-977),
+// stored per cluster column-major: the block of cluster c starts at cl_ptr[c] * P and
+// holds [point p][local element a] with a running fastest (thread per stored entry).
 template <typename S>
 __global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, const double *ccen, const int *elem_cl,
-                                               const double *u, const double *wsph, cx<S> *Sv)
+                                               const int *cl_ptr, const int *cl_elem, const double *u,
+                                               const double *wsph, cx<S> *Sv)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)E.n * P) return;
-    const int e = (int)(t / P), p = (int)(t % P);
-    const int c = elem_cl[e];
+    const int c = elem_cl[cl_elem[t / P]];
+    const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
+    const size_t off = t - (size_t)b * P;
+    const int p = (int)(off / mc), e = cl_elem[b + (int)(off % mc)];
     const double d[3] = {E.cen[e] - ccen[3 * c], E.cen[E.n + e] - ccen[3 * c + 1], E.cen[2 * E.n + e] - ccen[3 * c + 2]};
     const double su[3] = {u[3 * p], u[3 * p + 1], u[3 * p + 2]};
     const double d0 = (d[0] * su[0] + d[1] * su[1] + d[2] * su[2]) * bm.hk;
@@ -1365,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_fmm_S(Elements E, BMConst bm, int P, co
     const cx<double> zd = mk(-z0.im * dn, z0.re * dn);
     const cx<double> beta_tau = mk(0.0, bm.beta_im * bm.tau);
     const cx<double> v = z0 + zd * beta_tau;
-    Sv[(size_t)e * P + p] = cx<S>{S(v.re), S(v.im)};
+    Sv[t] = cx<S>{S(v.re), S(v.im)};
 }
 
 // CSR position of every subdivided pair (i, j) of the dense near-pair table inside the
@@ -1919,52 +1924,58 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, 
         for (int r = 0; r < NR; ++r) sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(re[r]), WO(im[r])};
 }
 
-// y[e] += sum_p S[e, p] s[cluster(e), p] over a row chunk of one cluster, s[c, :] staged
-// in LDS ([NR][seg] complex FP64, seg points at a time)
+// y[e] += sum_p S[e, p] s[cluster(e), p].  One work-group of 8 waves per tile of at most
+// 64 elements of one cluster (tl_*: cluster, first local element, elements): lane a is
+// element a0 + a of the cluster's column-major S block, so every load is a contiguous run
+// of the block; the waves split the points, four per step, s[c, p] is wave-uniform, and
+// the partial sums meet in an LDS tree.
 template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(256) k_fmm_apply_S(int n, int ncl, int P, int seg, const int *ch_cl, const int *ch_r0,
-                                                     const int *ch_m, const int *cl_elem, const cx<S> *Sv,
-                                                     const cx<W> *sw, cx<double> *y)
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, int P, const int *tl_cl,
+                                                                const int *tl_a0, const int *tl_m, const int *cl_ptr,
+                                                                const int *cl_elem, const cx<S> *Sv, const cx<W> *sw,
+                                                                cx<double> *y)
 {
-    extern __shared__ double sv_sh[];
-    cx<double> *sv = (cx<double> *)sv_sh;
-    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
-    // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
+    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
+    constexpr int NW = kSumWaves;
+    const int q = blockIdx.x, c = tl_cl[q], a0 = tl_a0[q], m = tl_m[q];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
+    const bool on = lane < m;
+    const cx<S> *blk = Sv + (size_t)b * P + a0 + (on ? lane : 0);
     const size_t ld = (size_t)ncl * P;
-    int row[4];
-    const cx<S> *sr[4];
+    const cx<W> *sc = sw + (size_t)c * P;
+    double re[NR], im[NR];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int a = wv + 4 * j;
-        row[j] = a < m ? cl_elem[r0 + a] : -1;
-        sr[j] = Sv + (size_t)(row[j] >= 0 ? row[j] : 0) * P;
-    }
-    double re[4][NR], im[4][NR];
+    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    if (on) {
+        int p = wv;
+        for (; p + 3 * NW < P; p += 4 * NW) {
+            cx<S> a[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 4; ++j) a[j] = blk[(size_t)(p + NW * j) * mc];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
-    for (int p0 = 0; p0 < P; p0 += seg) {
-        const int sl = min(seg, P - p0);
-        __syncthreads();
-        for (int t = threadIdx.x; t < NR * sl; t += 256) {
-            const int r = t / sl, u = t - r * sl;
-            const cx<W> v = sw[r * ld + (size_t)c * P + p0 + u];
-            sv[r * seg + u] = mk(double(v.re), double(v.im));
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const cx<W> t = sc[r * ld + p + NW * j];
+                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), mk(double(t.re), double(t.im)));
+                }
         }
-        __syncthreads();
-        // unused row slots read row 0 and are not stored
-        const cx<S> *ss[4] = {sr[0] + p0, sr[1] + p0, sr[2] + p0, sr[3] + p0};
-        rows4_dot<S, NR, 4>(ss, sl, sv, seg, lane, re, im);
+        for (; p < P; p += NW) {
+            const cx<S> a = blk[(size_t)p * mc];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const cx<W> t = sc[r * ld + p];
+                cmac(re[r], im[r], double(a.re), double(a.im), mk(double(t.re), double(t.im)));
+            }
+        }
     }
-    wave_sum4<NR>(re, im);
-    if (lane == 0)
+    waves_sum<NR, NW>(re, im, red, wv, lane);
+    if (wv == 0 && on) {
+        const int e = cl_elem[b + a0 + lane];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (row[j] >= 0)
-#pragma unroll
-                for (int r = 0; r < NR; ++r) y[(size_t)r * n + row[j]] += mk(re[j][r], im[j][r]);
+        for (int r = 0; r < NR; ++r) y[(size_t)r * n + e] += mk(re[r], im[r]);
+    }
 }
 
 // ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
@@ -2452,12 +2463,14 @@ void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, 
                                                                    (const cx<double> *)bcval, nrhs, (cx<double> *)tv);
 }
 void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
-           const double *u, const double *w, void *Sv, int mixed)
+           const int *cl_ptr, const int *cl_elem, const double *u, const double *w, void *Sv, int mixed)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for((size_t)D.n * P, 256);
-    if (mixed) k_fmm_S<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<float> *)Sv);
-    else k_fmm_S<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, u, w, (cx<double> *)Sv);
+    if (mixed)
+        k_fmm_S<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<float> *)Sv);
+    else
+        k_fmm_S<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<double> *)Sv);
 }
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
@@ -2558,8 +2571,8 @@ void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, 
     default: KERNEL<TARGS, 4><<<GRID, BLOCK, (SH) * 4, s>>>(__VA_ARGS__); break;        \
     }
 
-// LDS budgets of one chunk: seg columns (points) x NR load cases x 16 bytes
-constexpr int kNearLdsBytes = 32768, kSLdsBytes = 32768;
+// LDS budget of one near-field chunk: seg columns x NR load cases x 16 bytes
+constexpr int kNearLdsBytes = 32768;
 
 template <typename S>
 static void near_spmv_t(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
@@ -2622,22 +2635,23 @@ void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, co
         apply_D_t(s, ncl, P, nrhs, fan_ptr, fan_idx, (const cx<double> *)Dv, (const cx<double> *)tw, (cx<double> *)sw);
 }
 template <typename S, typename W>
-static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
-                      const cx<S> *Sv, const cx<W> *sw, cx<double> *y)
+static void apply_S_t(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &tl, const int *cl_ptr,
+                      const int *cl_elem, const cx<S> *Sv, const cx<W> *sw, cx<double> *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        const int seg = std::min(P, kSLdsBytes / (16 * nr));
-        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_S, NCG_TARGS(S, W), rc.n, 256, 16 * seg, n, ncl, P, seg, rc.cl, rc.r0, rc.m,
-                           cl_elem, Sv, sw + (size_t)r0 * ncl * P, y + (size_t)r0 * n);
+        NCG_NR_DISPATCH_SH(nr, k_fmm_apply_S, NCG_TARGS(S, W), tl.n, 64 * kSumWaves, 0, n, ncl, P, tl.cl, tl.r0, tl.m,
+                           cl_ptr, cl_elem, Sv, sw + (size_t)r0 * ncl * P, y + (size_t)r0 * n);
     });
 }
-void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
-                 const void *Sv, int f32, const void *sw, void *y)
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &tl, const int *cl_ptr,
+                 const int *cl_elem, const void *Sv, int f32, const void *sw, void *y)
 {
     if (f32)
-        apply_S_t(s, n, ncl, P, nrhs, rc, cl_elem, (const cx<float> *)Sv, (const cx<float> *)sw, (cx<double> *)y);
+        apply_S_t(s, n, ncl, P, nrhs, tl, cl_ptr, cl_elem, (const cx<float> *)Sv, (const cx<float> *)sw,
+                  (cx<double> *)y);
     else
-        apply_S_t(s, n, ncl, P, nrhs, rc, cl_elem, (const cx<double> *)Sv, (const cx<double> *)sw, (cx<double> *)y);
+        apply_S_t(s, n, ncl, P, nrhs, tl, cl_ptr, cl_elem, (const cx<double> *)Sv, (const cx<double> *)sw,
+                  (cx<double> *)y);
 }
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,

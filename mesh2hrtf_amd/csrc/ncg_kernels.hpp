@@ -109,8 +109,9 @@ void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double 
            const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed);
 void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                  const int *cl_ptr, const int *cl_elem, const double *u, const void *bcval, int nrhs, void *tv);
+// S stored per cluster column-major ([point][local element] from cl_ptr[c] * P)
 void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
-           const double *u, const double *w, void *Sv, int mixed);
+           const int *cl_ptr, const int *cl_elem, const double *u, const double *w, void *Sv, int mixed);
 // subdivided pairs of the near field: CSR positions (once per structure) and values
 void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
                    const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
@@ -146,8 +147,9 @@ void fmm_apply_T(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_p
                  int f32, int tw_f64, const void *x, void *tw);
 void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, const int *fan_idx, const void *Dv,
                  int f32, int tw_f64, const void *tw, void *sw);
-void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &rc, const int *cl_elem,
-                 const void *Sv, int f32, const void *sw, void *y);
+// tl: tiles of at most 64 elements of one cluster (RowChunks with m <= 64)
+void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &tl, const int *cl_ptr,
+                 const int *cl_elem, const void *Sv, int f32, const void *sw, void *y);
 // regular near-field entries by (row chunk, 256 columns); the subdivided pairs are left to
 // fmm_near_adj and the diagonal to assemble_diag.  wmax = longest row.
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
