@@ -454,7 +454,7 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
             const DevLevel &DF = p->fmm.lev.back();
             KernelScope ks(p, mixed, chunks * nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
             ncgk::near_spmv(p->stream, p->n, p->nrhs, DF.chunks(), DF.cl_ptr, DF.cl_elem, p->fmm.near_ptr,
-                            p->fmm.near_col, p->fmm.near_val.p, mixed, xin, y);
+                            p->fmm.near_col, p->fmm.near_wmax, p->fmm.near_val.p, mixed, xin, y);
         }
         for (DevLevel &L : p->fmm.lev) {
             // algorithmic bytes: the operator once per pass of at most 4 load cases, the

@@ -504,10 +504,21 @@ __global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, i
     const cx<double> *v = V + ((size_t)k * nrhs + r) * n, *ww = w + (size_t)r * n;
     const int i0 = sl * slice, i1 = min(n, i0 + slice);
     double re = 0.0, im = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
-        const cx<double> a = v[i], b = ww[i];
-        re += a.re * b.re + a.im * b.im;
-        im += a.re * b.im - a.im * b.re;
+    // four elements per thread and step, every load first (past the slice: weight zero)
+    for (int i = i0 + threadIdx.x; i < i1; i += 1024) {
+        cx<double> a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ii = min(i + 256 * q, i1 - 1);
+            a[q] = v[ii];
+            b[q] = ww[ii];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i + 256 * q < i1) {
+                re += a[q].re * b[q].re + a[q].im * b[q].im;
+                im += a[q].re * b[q].im - a[q].im * b[q].re;
+            }
     }
     sre[threadIdx.x] = re;
     sim[threadIdx.x] = im;
@@ -2327,10 +2338,11 @@ void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val
     if (mixed) k_row_scaling_csr<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<float> *)val, d);
     else k_row_scaling_csr<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<double> *)val, d);
 }
-// slices of k_multidot: about 4096 entries each, as many as the partial buffer holds
+// slices of k_multidot: about 1024 entries each (one load step of four per thread), as
+// many as the partial buffer holds
 static int multidot_slices(int n, int nrhs, int kcount, size_t part_cap)
 {
-    const int nsl = std::max(1, (n + 4095) / 4096);
+    const int nsl = std::max(1, (n + 1023) / 1024);
     return (int)std::max<size_t>(1, std::min<size_t>((size_t)nsl, part_cap / ((size_t)kcount * nrhs)));
 }
 void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *w, void *h, void *part,
@@ -2576,22 +2588,24 @@ constexpr int kNearLdsBytes = 32768;
 
 template <typename S>
 static void near_spmv_t(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-                        const long long *ptr, const int *col, const cx<S> *val, const cx<double> *x, cx<double> *y)
+                        const long long *ptr, const int *col, int wmax, const cx<S> *val, const cx<double> *x,
+                        cx<double> *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        const int seg = kNearLdsBytes / (16 * nr);
+        // LDS for the longest row (in 64-column steps), at most the budget: more chunks per CU
+        const int seg = std::max(64, std::min(kNearLdsBytes / (16 * nr), (wmax + 63) / 64 * 64));
         NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, rc.n, 256, 16 * seg, n, seg, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                            val, x + (size_t)r0 * n, y + (size_t)r0 * n);
     });
 }
 void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-               const long long *ptr, const int *col, const void *val, int mixed, const void *x, void *y)
+               const long long *ptr, const int *col, int wmax, const void *val, int mixed, const void *x, void *y)
 {
     if (mixed)
-        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, (const cx<float> *)val, (const cx<double> *)x,
+        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, wmax, (const cx<float> *)val, (const cx<double> *)x,
                     (cx<double> *)y);
     else
-        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, (const cx<double> *)val, (const cx<double> *)x,
+        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, wmax, (const cx<double> *)val, (const cx<double> *)x,
                     (cx<double> *)y);
 }
 template <typename S, typename W>

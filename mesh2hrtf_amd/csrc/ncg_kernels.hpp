@@ -139,7 +139,7 @@ struct RowChunks {
 };
 // near field of the finest level (CSR whose rows in one cluster share their columns)
 void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-               const long long *ptr, const int *col, const void *val, int mixed, const void *x, void *y);
+               const long long *ptr, const int *col, int wmax, const void *val, int mixed, const void *x, void *y);
 // far field of one level: f32 selects FP32 T, D, S and FP32 storage of t and s
 // tw_f64: t is FP64 whatever f32 says (signatures for the evaluation grid and the
 // right-hand side's sources)
