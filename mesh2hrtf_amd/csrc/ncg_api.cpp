@@ -327,6 +327,12 @@ struct ncg_problem {
     double *scal = nullptr, *partial = nullptr, *dotout = nullptr, *dvec = nullptr;
     size_t partial_cap = 0;   // complex entries of partial
     double *npart = nullptr;  // [64 load cases][256 blocks] partial norms of the Gram-Schmidt passes
+    // block-Jacobi of a dense operator: blocks of at most 64 consecutive unknowns
+    Pool dbj;
+    int dbj_nblk = 0, dbj_ready = 0, dbj_f32 = 0;
+    int *dbj_elem_blk = nullptr, *dbj_b0 = nullptr, *dbj_m = nullptr, *dbj_ident = nullptr;
+    long long *dbj_boff = nullptr, dbj_total = 0;
+    Grow dbj_blk;
     int *active = nullptr;
     double *basis = nullptr;
     size_t basis_cap = 0;
@@ -1184,7 +1190,40 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
 int bj_prepare(ncg_problem *p)
 {
     DevFmm &F = p->fmm;
-    if (p->plan.method == 0) return fail(NCG_E_ARG, "the block-Jacobi preconditioner needs an FMM plan");
+    if (p->plan.method == 0) {
+        // dense operator: diagonal blocks of at most 64 consecutive unknowns, sizes balanced
+        int rc;
+        if (p->dbj_ready) return 0;
+        const int n = p->n;
+        if (!p->dbj_b0) {
+            const int nb = (n + 63) / 64;
+            std::vector<int> eb(n), b0(nb), m(nb), id(n);
+            std::vector<long long> boff(nb + 1, 0);
+            for (int b = 0, s0 = 0; b < nb; ++b) {
+                m[b] = n / nb + (b < n % nb ? 1 : 0);
+                b0[b] = s0;
+                for (int i = s0; i < s0 + m[b]; ++i) eb[i] = b;
+                boff[b + 1] = boff[b] + (long long)m[b] * m[b];
+                s0 += m[b];
+            }
+            for (int i = 0; i < n; ++i) id[i] = i;
+            if ((rc = upload(p->dbj, p->stream, &p->dbj_elem_blk, eb)) || (rc = upload(p->dbj, p->stream, &p->dbj_b0, b0)) ||
+                (rc = upload(p->dbj, p->stream, &p->dbj_m, m)) || (rc = upload(p->dbj, p->stream, &p->dbj_ident, id)) ||
+                (rc = upload(p->dbj, p->stream, &p->dbj_boff, boff)))
+                return rc;
+            p->dbj_nblk = nb;
+            p->dbj_total = boff[nb];
+            NCG_HIP(hipStreamSynchronize(p->stream));
+        }
+        p->dbj_f32 = p->precision == NCG_MIXED;
+        if ((rc = p->dbj_blk.ensure((p->dbj_f32 ? 8 : 16) * (size_t)p->dbj_total))) return rc;
+        ncgk::bj_setup_dense(p->stream, n, p->dbj_nblk, p->dbj_elem_blk, p->dbj_b0, p->dbj_m, p->A, p->dbj_f32,
+                             p->dbj_boff, p->dbj_blk.p, p->err);
+        NCG_HIP(hipGetLastError());
+        if ((rc = check_err(p))) return rc;
+        p->dbj_ready = 1;
+        return 0;
+    }
     if (F.bj_ready) return 0;
     const DevLevel &D = F.lev.back();
     int rc;
@@ -1200,6 +1239,11 @@ int bj_prepare(ncg_problem *p)
 
 void bj_apply_vec(ncg_problem *p, const double *x, double *y)
 {
+    if (p->plan.method == 0) {
+        ncgk::bj_apply(p->stream, p->n, p->nrhs, p->dbj_nblk, p->dbj_b0, p->dbj_m, p->dbj_ident, p->dbj_boff,
+                       p->dbj_blk.p, p->dbj_f32, x, y);
+        return;
+    }
     const DevFmm &F = p->fmm;
     const DevLevel &D = F.lev.back();
     ncgk::bj_apply(p->stream, p->n, p->nrhs, F.nblk, F.blk_b0, F.blk_m, D.cl_elem, F.boff, F.bj_blk.p, F.bj_f32, x,
@@ -1655,6 +1699,8 @@ int ncg_problem_destroy(ncg_problem *p)
     for (double *w : p->work) hipFree(w);
     p->fmm.release();
     p->ilu.release();
+    p->dbj.release();
+    p->dbj_blk.release();
     p->out_phi.release();
     p->out_grad.release();
     for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {
@@ -1757,6 +1803,7 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
     if ((rc = ensure_near_table(p))) return rc;
     record(p, p->ev_asm.a);
     record(p, p->ev_far.a);
+    p->dbj_ready = 0;
     ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->evals);
     record(p, p->ev_far.b);
     ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err);

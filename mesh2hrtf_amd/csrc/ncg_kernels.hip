@@ -1553,6 +1553,21 @@ __global__ void __launch_bounds__(256) k_bj_gather(int n, const int *elem_blk, c
     for (int j = lane; j < m; j += 64) dst[j] = cx<TB>{TB(src[j].re), TB(src[j].im)};
 }
 
+// the same blocks for a dense operator: block b holds the unknowns [blk_b0[b], + blk_m[b])
+// and row i of it is copied from A's row i (one wave per row)
+template <typename S, typename TB>
+__global__ void __launch_bounds__(256) k_bj_gather_dense(int n, const int *elem_blk, const int *blk_b0, const int *blk_m,
+                                                         const cx<S> *A, const long long *boff, cx<TB> *blk)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int b = elem_blk[i], b0 = blk_b0[b], m = blk_m[b], a = i - b0;
+    const cx<S> *src = A + (size_t)i * n + b0;
+    cx<TB> *dst = blk + boff[b] + (size_t)a * m;
+    for (int j = lane; j < m; j += 64) dst[j] = cx<TB>{TB(src[j].re), TB(src[j].im)};
+}
+
 // In-place Gauss-Jordan inversion with partial (row) pivoting of one block (m <= 64) by
 // one wave, the block in LDS: during the elimination lane j owns column j, so a row swap,
 // the scaling of the pivot row and the update of every row are lane-parallel and need no
@@ -2544,6 +2559,20 @@ void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *el
         k_bj_gather<double, double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m,
                                                                      cl_ptr, near_ptr, self_off,
                                                                      (const cx<double> *)nval, boff, (cx<double> *)blk);
+        k_bj_invert<double><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
+    }
+}
+void bj_setup_dense(hipStream_t s, int n, int nblk, const int *elem_blk, const int *blk_b0, const int *blk_m,
+                    const void *A, int mixed, const long long *boff, void *blk, int *err)
+{
+    if (mixed) {
+        k_bj_gather_dense<float, float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, blk_b0, blk_m, (const cx<float> *)A,
+                                                                         boff, (cx<float> *)blk);
+        k_bj_invert<float><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<float> *)blk, err);
+    } else {
+        k_bj_gather_dense<double, double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, blk_b0, blk_m,
+                                                                           (const cx<double> *)A, boff,
+                                                                           (cx<double> *)blk);
         k_bj_invert<double><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
     }
 }

@@ -123,6 +123,9 @@ void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const 
 void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
               const int *blk_b0, const int *blk_m, const int *cl_ptr, const long long *near_ptr, const int *self_off,
               const void *nval, int mixed, const long long *boff, void *blk, int *err);
+// the same for a dense operator A (n x n row-major): blocks of consecutive unknowns
+void bj_setup_dense(hipStream_t s, int n, int nblk, const int *elem_blk, const int *blk_b0, const int *blk_m,
+                    const void *A, int mixed, const long long *boff, void *blk, int *err);
 // mixed: the blocks are inverted and stored in FP32 (FP64 otherwise)
 void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
               const long long *boff, const void *minv, int mixed, const void *x, void *y);

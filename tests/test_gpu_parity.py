@@ -188,6 +188,25 @@ def test_mixed_precision_within_tolerance(ncg):
         assert R.rel_l2(p32[0], p64[0]) < 1e-4
 
 
+def test_dense_block_jacobi(ncg):
+    """The served form's block-Jacobi preconditioner on a dense (TBEM) operator: blocks of
+    consecutive unknowns; GMRES with it reaches the FP64 CGS solution, in both precisions."""
+    case = R.load_case("head8")
+    f = case["spec"]["freqs"][-1]
+    pa = R.problem_arrays(case, f)
+    prob = gpu_problem(ncg, case, pa)
+    assemble(prob, pa, precision=ncg.FP64)
+    prob.solve(solver=ncg.CGS, tolerance=1e-10)
+    x64 = prob.solution()[0]
+    st = prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, tolerance=1e-10)
+    assert st["converged"]
+    assert R.rel_l2(prob.solution()[0], x64) < 1e-8
+    assemble(prob, pa, precision=ncg.MIXED)
+    st = prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_tolerance=1e-7)
+    assert st["converged"]
+    assert R.rel_l2(prob.solution()[0], x64) < 1e-4
+
+
 def test_apply_matches_system(ncg):
     case = R.load_case("quad")
     pa = R.problem_arrays(case, 2100.0)
