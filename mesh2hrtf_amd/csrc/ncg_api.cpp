@@ -333,6 +333,8 @@ struct ncg_problem {
     int *dbj_elem_blk = nullptr, *dbj_b0 = nullptr, *dbj_m = nullptr, *dbj_ident = nullptr;
     long long *dbj_boff = nullptr, dbj_total = 0;
     Grow dbj_blk;
+    // dense FP32 operator on the matrix cores: X^ operand and column-split partial sums
+    Grow mm_xh, mm_part;
     int *active = nullptr;
     double *basis = nullptr;
     size_t basis_cap = 0;
@@ -445,7 +447,14 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
     if (p->plan.method == 0) {
         const int mixed = p->precision == NCG_MIXED;
         KernelScope ks(p, 8 + mixed, n * n * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
-        ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);
+        // more than 4 load cases: the FP32 operator goes through the matrix cores, up to 16
+        // load cases per read of A (the vector kernel reads A once per 4)
+        size_t xf = 0, pf = 0;
+        if (mixed && !acc64 && !d && p->nrhs > 4) ncgk::matvec_mfma_scratch(p->n, p->nrhs, &xf, &pf);
+        if (xf && !p->mm_xh.ensure(sizeof(float) * xf) && !p->mm_part.ensure(sizeof(float) * pf))
+            ncgk::matvec_mfma(p->stream, p->A, p->n, p->nrhs, x, y, (float *)p->mm_xh.p, (float *)p->mm_part.p);
+        else
+            ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);
     } else {
         // y = Near x + sum over levels S (D (T x))  (src/NC_CommonFunctions.cpp:1247-1316)
         const int mixed = p->precision == NCG_MIXED;
@@ -1701,6 +1710,8 @@ int ncg_problem_destroy(ncg_problem *p)
     p->ilu.release();
     p->dbj.release();
     p->dbj_blk.release();
+    p->mm_xh.release();
+    p->mm_part.release();
     p->out_phi.release();
     p->out_grad.release();
     for (EventPair *e : {&p->ev_asm, &p->ev_far, &p->ev_solve, &p->ev_field}) {

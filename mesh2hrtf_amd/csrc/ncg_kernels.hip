@@ -2292,6 +2292,120 @@ void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs,
     }
 }
 
+// ---- dense operator applied to many load cases on the matrix cores --------------------
+// Y = A X for up to 16 load cases per pass is the real GEMM Y^ = A^ X^ with A^ = A read as
+// an n x 2n real matrix (re/im interleaved, as stored) and X^ (2n x 32):
+//   X^[2j][c] = Re x_c[j], X^[2j+1][c] = -Im x_c[j]        (c < 16: real parts of Y)
+//   X^[2j][16+c] = Im x_c[j], X^[2j+1][16+c] = Re x_c[j]   (imaginary parts of Y)
+// One wave owns a 32-row tile and all 32 columns in the 16 accumulators of
+// v_mfma_f32_32x32x2f32 (D lane l, register v: row 8 (v / 4) + 4 (l / 32) + v % 4,
+// column l % 32).  A work-group of 4 waves covers 128 rows; the columns of A are split
+// over gridDim.y work-groups (partial sums [split][32][n], added by k_mfma_reduce).
+constexpr int kMfmaRows = 128, kMfmaCols = 32;   // rows per work-group, complex columns per LDS chunk
+
+__global__ void __launch_bounds__(256) k_xhat(int n, int nr, const cx<double> *x, float *xh)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 32) return;
+    const int j = t / 32, c = t % 32, r = c % 16;
+    float v0 = 0.f, v1 = 0.f;
+    if (r < nr) {
+        const cx<double> a = x[(size_t)r * n + j];
+        if (c < 16) { v0 = float(a.re); v1 = float(-a.im); }
+        else { v0 = float(a.im); v1 = float(a.re); }
+    }
+    xh[(size_t)(2 * j) * 32 + c] = v0;
+    xh[(size_t)(2 * j + 1) * 32 + c] = v1;
+}
+
+typedef float ncg_floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(256) k_matvec_mfma(const cx<float> *__restrict__ A, int n, int chunks_per_split,
+                                                     const float *__restrict__ xh, float *__restrict__ part)
+{
+    __shared__ float As[kMfmaRows][2 * kMfmaCols + 1];
+    __shared__ float Xs[2 * kMfmaCols][32];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int r0 = blockIdx.x * kMfmaRows;
+    const int nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    const int c0 = blockIdx.y * chunks_per_split, c1 = min(nchunks, c0 + chunks_per_split);
+    ncg_floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    for (int ch = c0; ch < c1; ++ch) {
+        const int j0 = ch * kMfmaCols;
+        __syncthreads();
+        // A tile: 128 rows x 32 complex columns, one complex (float2) per load
+        for (int q = t; q < kMfmaRows * kMfmaCols; q += 256) {
+            const int rr = q / kMfmaCols, jj = q % kMfmaCols, row = r0 + rr, col = j0 + jj;
+            cx<float> a = {0.f, 0.f};
+            if (row < n && col < n) a = A[(size_t)row * n + col];
+            As[rr][2 * jj] = a.re;
+            As[rr][2 * jj + 1] = a.im;
+        }
+        // X^ tile: 64 real rows x 32 columns
+        for (int q = t; q < 2 * kMfmaCols * 32; q += 256) {
+            const int kk = q / 32, c = q % 32, k = 2 * j0 + kk;
+            Xs[kk][c] = k < 2 * n ? xh[(size_t)k * 32 + c] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ks = 0; ks < kMfmaCols; ++ks) {
+            const float a = As[wv * 32 + (lane & 31)][2 * ks + (lane >> 5)];
+            const float b = Xs[2 * ks + (lane >> 5)][lane & 31];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+    }
+    // partial sums of this column split: part[split][column][row]
+    float *out = part + (size_t)blockIdx.y * 32 * n;
+    const int col = lane & 31;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = r0 + wv * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+        if (row < n) out[(size_t)col * n + row] = acc[v];
+    }
+}
+
+// y[r][i] = sum over the splits of the partial real and imaginary parts, r < nr
+__global__ void __launch_bounds__(256) k_mfma_reduce(int n, int nr, int splits, const float *part, cx<double> *y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nr) return;
+    const int r = t / n, i = t % n;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < splits; ++s) {
+        re += part[((size_t)s * 32 + r) * n + i];
+        im += part[((size_t)s * 32 + 16 + r) * n + i];
+    }
+    y[(size_t)r * n + i] = mk(re, im);
+}
+
+int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats)
+{
+    const int tiles = (n + kMfmaRows - 1) / kMfmaRows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    // enough work-groups to cover the device twice, each at least 4 column chunks
+    int splits = std::max(1, std::min(std::min(16, (512 + tiles - 1) / tiles), (nchunks + 3) / 4));
+    *xh_floats = (size_t)2 * n * 32;
+    *part_floats = (size_t)splits * 32 * n;
+    (void)nrhs;
+    return splits;
+}
+
+void matvec_mfma(hipStream_t s, const void *A, int n, int nrhs, const void *x, void *y, float *xh, float *part)
+{
+    size_t xf, pf;
+    const int splits = matvec_mfma_scratch(n, nrhs, &xf, &pf);
+    const int tiles = (n + kMfmaRows - 1) / kMfmaRows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    const int per = (nchunks + splits - 1) / splits;
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {
+        const int nr = std::min(16, nrhs - r0);
+        k_xhat<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x + (size_t)r0 * n, xh);
+        k_matvec_mfma<<<dim3(tiles, splits), 256, 0, s>>>((const cx<float> *)A, n, per, xh, part);
+        k_mfma_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, splits, part,
+                                                                      (cx<double> *)y + (size_t)r0 * n);
+    }
+}
+
 int dot_blocks(int n) { return std::min(kDotBlocks, (int)blocks_for(n, 256)); }
 
 void dots(hipStream_t s, int n, int nrhs, int ndots, const void *const *u, const void *const *v, int conj_mask,

@@ -45,6 +45,10 @@ void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
                    void *A, int mixed, void *rhs, const long long *diag_index = nullptr);
 
+// FP32 dense operator applied to many load cases on the matrix cores (FP32 accumulation,
+// up to 16 load cases per pass); scratch sizes from matvec_mfma_scratch
+int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats);
+void matvec_mfma(hipStream_t s, const void *A, int n, int nrhs, const void *x, void *y, float *xh, float *part);
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d,
             void *y);
 int dot_blocks(int n);
