@@ -67,6 +67,28 @@ void warning(std::ofstream &log, const std::string &msg, const std::string &extr
 void read_project(std::ofstream &log, Project &P, bool check_normals);
 void write_parameters(std::ofstream &log, const Project &P);
 
+// Symmetry planes (src/NC_Input.cpp:459-495, NC_ReflectElement src/NC_3dFunctions.cpp:357-452).
+// NumCalc adds, for every element, the integrals over its mirror images with the factor -1
+// per antisymmetric plane.  This build unfolds the mesh instead: image r (bit b set =
+// mirrored in the b-th plane) of boundary element q becomes unknown r n + q.  The unfolded
+// problem with mirrored boundary values and sources has the symmetric solution
+// x[r n + q] = sign(r) x[q]; its first n unknowns are NumCalc's.
+struct Symmetry {
+    std::vector<int> dims;   // axes that carry a plane, ascending
+    int nref = 1;            // 2^planes images, the identity first
+    std::vector<double> coord, type;
+    explicit Symmetry(const Project &P);
+    double sign(int r) const;                  // -1 per antisymmetric plane mirrored in r
+    void mirror(int r, double *x) const;       // point x -> its image r
+};
+
+// element arrays of the ABI geometry for the boundary elements be[] and their images:
+// nodes mirrored, nodes 0<->1 (and 2<->3 of a quad) swapped after an odd number of
+// mirrors so the normal keeps pointing outwards, as NumCalc reorders them; a node on a
+// plane keeps its id across that plane
+void unfold_mesh(const Project &P, const std::vector<int> &be, const Symmetry &S, std::vector<int> &nv,
+                 std::vector<int> &kind, std::vector<int> &nodes, std::vector<int> &groups, std::vector<double> &xyz);
+
 // per-frequency scaling of NC_UpdateFreqCurves (src/NC_EquationSystem.cpp:1815-2009)
 double curve_factor(std::ofstream &log, const Project &P, int curve_id, double freq);
 

@@ -196,7 +196,21 @@ void read_mesh_files(std::ofstream &log, FILE *in, Project &P, bool check_normal
         for (int d = 0; d < 3; ++d) P.sym_planes[d] = to_int(tok[d]);
         next(in, tok);
         for (int d = 0; d < 3; ++d) P.sym_coord[d] = to_double(tok[d]);
-        fatal(log, "Symmetry planes are not supported by this NumCalc build!");
+        // plane types 0 none, 1 symmetric, 2 antisymmetric, 3 and 4 read as +-2 (only the
+        // sign is used by the solve: negative = antisymmetric)
+        int cnt = 0;
+        for (int d = 0; d < 3; ++d) {
+            int &t = P.sym_planes[d];
+            if (t) cnt++;
+            if (t == 2) t = -1;
+            else if (t == 3) t = 2;
+            else if (t == 4) t = -2;
+            else if (t != 0 && t != 1)
+                fatal(log, "listSymmetryPlanes[i] must be equal to 0, 1, 2, 3, or 4!",
+                      "i = " + str(d) + "\nlistSymmetryPlanes[i] = " + str(t));
+        }
+        if (cnt != P.nsym)
+            fatal(log, "j must equal to numSymmetricPlanes_", "j = " + str(cnt) + "\nnumSymmetricPlanes_ = " + str(P.nsym));
     }
     // external node numbers -> internal indices (src/NC_Input.cpp:497-511)
     std::unordered_map<int, int> idx;
@@ -261,7 +275,8 @@ void read_mesh_files(std::ofstream &log, FILE *in, Project &P, bool check_normal
         }
     }
     if (P.n_be_nodes == 0) fatal(log, "Number of boundary elements is zero!");
-    // BE bounding-box diagonal (src/NC_Input.cpp:659-703, no symmetry planes)
+    // BE bounding-box diagonal (src/NC_Input.cpp:659-703); across a symmetry plane the box
+    // is the one of the mesh and its mirror image, and the mesh must lie on one side
     double mx[3] = {-9.9e20, -9.9e20, -9.9e20}, mn[3] = {9.9e20, 9.9e20, 9.9e20};
     for (int k = 0; k < P.nnode; ++k) {
         if (P.node_class[k] < 0) continue;
@@ -270,8 +285,23 @@ void read_mesh_files(std::ofstream &log, FILE *in, Project &P, bool check_normal
             mn[d] = std::min(mn[d], P.node_xyz[3 * (size_t)k + d]);
         }
     }
+    const double dtl = sqrt(P.avg_area) / 100.0;
     P.diameter = 0.0;
-    for (int d = 0; d < 3; ++d) P.diameter += (mx[d] - mn[d]) * (mx[d] - mn[d]);
+    for (int d = 0; d < 3; ++d) {
+        double ext = mx[d] - mn[d];
+        if (P.nsym > 0 && P.sym_planes[d] != 0) {
+            const double c = P.sym_coord[d];
+            if (mx[d] >= c - dtl && mn[d] >= c - dtl) ext = 2.0 * (mx[d] - c);
+            else if (mx[d] <= c + dtl && mn[d] <= c + dtl) ext = 2.0 * (c - mn[d]);
+            else {
+                warning(log, "", "Number of the symmetric plane = " + str(d));
+                fatal(log, "All BE nodes must be located on the same side of a symmetric plane!",
+                      "crdmaxmin[j][0] - listSymmetryPlanesCoordinates[j] = " + std::to_string(mx[d] - c) +
+                          "\ncrdmaxmin[j][1] - listSymmetryPlanesCoordinates[j] =  = " + std::to_string(mn[d] - c));
+            }
+        }
+        P.diameter += ext * ext;
+    }
     P.diameter = sqrt(P.diameter);
     // element groups (src/NC_Input.cpp:705-736)
     for (int e = 0; e < P.nelem; ++e) {
@@ -556,6 +586,62 @@ void read_project(std::ofstream &log, Project &P, bool check_normals)
 }
 
 // NC_WriteParameters (src/NC_Input.cpp:1409-1593)
+Symmetry::Symmetry(const Project &P)
+{
+    coord.assign(P.sym_coord, P.sym_coord + 3);
+    type.assign(3, 0.0);
+    for (int d = 0; d < 3; ++d)
+        if (P.nsym > 0 && P.sym_planes[d] != 0) {
+            dims.push_back(d);
+            type[d] = P.sym_planes[d];
+        }
+    nref = 1 << dims.size();
+}
+
+double Symmetry::sign(int r) const
+{
+    double s = 1.0;
+    for (size_t b = 0; b < dims.size(); ++b)
+        if ((r >> b & 1) && type[dims[b]] < 0) s = -s;
+    return s;
+}
+
+void Symmetry::mirror(int r, double *x) const
+{
+    for (size_t b = 0; b < dims.size(); ++b)
+        if (r >> b & 1) x[dims[b]] = 2.0 * coord[dims[b]] - x[dims[b]];
+}
+
+void unfold_mesh(const Project &P, const std::vector<int> &be, const Symmetry &S, std::vector<int> &nv,
+                 std::vector<int> &kind, std::vector<int> &nodes, std::vector<int> &groups, std::vector<double> &xyz)
+{
+    const int n = (int)be.size(), N = n * S.nref;
+    nv.assign(N, 0);
+    kind.assign(N, 0);
+    groups.assign(N, 0);
+    nodes.assign(4 * (size_t)N, -1);
+    xyz.assign(12 * (size_t)N, 0.0);
+    for (int r = 0; r < S.nref; ++r)
+        for (int q = 0; q < n; ++q) {
+            const int e = be[q], Q = r * n + q, m = P.elem_nv[e];
+            const bool odd = __builtin_popcount(r) & 1;
+            nv[Q] = m;
+            kind[Q] = P.ibval[e];
+            groups[Q] = P.elem_group[e];
+            for (int a = 0; a < m; ++a) {
+                const int src = odd && (a < 2 || m == 4) ? (a ^ 1) : a;
+                const int v = P.elem_conn[4 * (size_t)e + src];
+                double x[3] = {P.node_xyz[3 * (size_t)v], P.node_xyz[3 * (size_t)v + 1], P.node_xyz[3 * (size_t)v + 2]};
+                int moved = 0;
+                for (size_t b = 0; b < S.dims.size(); ++b)
+                    if ((r >> b & 1) && 2.0 * S.coord[S.dims[b]] - x[S.dims[b]] != x[S.dims[b]]) moved |= 1 << b;
+                S.mirror(r, x);
+                nodes[4 * (size_t)Q + a] = v + P.nnode * moved;
+                for (int d = 0; d < 3; ++d) xyz[12 * (size_t)Q + 3 * a + d] = x[d];
+            }
+        }
+}
+
 void write_parameters(std::ofstream &log, const Project &Pc)
 {
     Project &P = const_cast<Project &>(Pc);

@@ -259,19 +259,13 @@ int main(int argc, char **argv)
             for (int d = 0; d < 3; ++d) evalp.push_back(P.node_xyz[3 * (size_t)k + d]);
     const int m = (int)evalp.size() / 3;
 
+    // boundary elements unfolded across the symmetry planes (nc::Symmetry)
+    const nc::Symmetry sym(P);
+    const int nref = sym.nref, N = n * nref;
     ncg_problem *prob = nullptr;
-    std::vector<int> nvv(n), kind(n), nodes(4 * (size_t)n, -1), groups(n);
-    std::vector<double> xyz(12 * (size_t)n, 0.0);
-    for (int q = 0; q < n; ++q) {
-        const int e = be[q];
-        nvv[q] = P.elem_nv[e];
-        kind[q] = P.ibval[e];
-        groups[q] = P.elem_group[e];
-        for (int a = 0; a < nvv[q]; ++a) {
-            nodes[4 * (size_t)q + a] = P.elem_conn[4 * (size_t)e + a];
-            for (int d = 0; d < 3; ++d) xyz[12 * (size_t)q + 3 * a + d] = P.node_xyz[3 * (size_t)P.elem_conn[4 * (size_t)e + a] + d];
-        }
-    }
+    std::vector<int> nvv, kind, nodes, groups;
+    std::vector<double> xyz;
+    nc::unfold_mesh(P, be, sym, nvv, kind, nodes, groups, xyz);
     ncg_geometry g;
     memset(&g, 0, sizeof(g));
     g.node_index = nodes.data();
@@ -279,7 +273,7 @@ int main(int argc, char **argv)
     g.method.method_bem = P.method_bem;
     g.method.min_expansion_terms = P.min_terms;
     g.method.cluster_edge = P.clus_edge;
-    g.n_elements = n;
+    g.n_elements = N;
     g.nodes_per_element = nvv.data();
     g.node_xyz = xyz.data();
     g.bc_kind = kind.data();
@@ -321,13 +315,13 @@ int main(int argc, char **argv)
             const int rc = ncg_fmm_plan(&g, k, &fi);
             if (rc) gpu_fatal(log, "ncg_fmm_plan", rc);
             log_plan(fi, step == istart);
-            const double gb = ram_estimate(fi, n);
+            const double gb = ram_estimate(fi, N);
             std::cout << "RAM Estimation: " << gb << " GByte\n";
             estim << step + 1 << " " << freq << " " << gb << "\n";
             continue;
         }
         // per-frequency values (NC_UpdateFreqCurves, src/NC_EquationSystem.cpp:1815-1961)
-        std::vector<double> bcv(2 * (size_t)n), adm(2 * (size_t)n, 0.0);
+        std::vector<double> bcv(2 * (size_t)N), adm(2 * (size_t)N, 0.0);
         for (int q = 0; q < n; ++q) {
             const int e = be[q];
             cplx v = P.bc0[e];
@@ -344,29 +338,51 @@ int main(int argc, char **argv)
                 adm[2 * q + 1] = a.imag();
             }
         }
+        // mirror images: prescribed velocity and pressure times the image's sign, the same
+        // admittance
+        for (int r = 1; r < nref; ++r)
+            for (int q = 0; q < n; ++q) {
+                const size_t Q = (size_t)r * n + q;
+                bcv[2 * Q] = sym.sign(r) * bcv[2 * q];
+                bcv[2 * Q + 1] = sym.sign(r) * bcv[2 * q + 1];
+                adm[2 * Q] = adm[2 * q];
+                adm[2 * Q + 1] = adm[2 * q + 1];
+            }
         auto scaled = [&](const nc::Source &s) {
             cplx z = s.amp0;
             if (s.curve[0] >= 0) z = cplx(z.real() * nc::curve_factor(log, P, s.curve[0], freq), z.imag());
             if (s.curve[1] >= 0) z = cplx(z.real(), z.imag() * nc::curve_factor(log, P, s.curve[1], freq));
             return z / cplx(0.0, rpfact);
         };
+        // sources and their mirror images (NC_IncidentWaveRHS src/NC_EquationSystem.cpp:1735-1810):
+        // an image carries the sign of its mirrors; with symmetry planes the plane-wave phase
+        // is taken relative to the planes, exp(i h k d_r . (x - c)) with c the plane coordinates
         std::vector<double> ps, pw;
-        for (const nc::Source &s : P.ps) {
-            const cplx z = scaled(s);
-            ps.insert(ps.end(), {s.xyz[0], s.xyz[1], s.xyz[2], z.real(), z.imag(), 0.0});
-        }
-        for (const nc::Source &s : P.pw) {
-            const cplx z = scaled(s);
-            pw.insert(pw.end(), {s.xyz[0], s.xyz[1], s.xyz[2], z.real(), z.imag(), 0.0});
-        }
+        for (const nc::Source &s : P.ps)
+            for (int r = 0; r < nref; ++r) {
+                const cplx z = scaled(s) * sym.sign(r);
+                double x[3] = {s.xyz[0], s.xyz[1], s.xyz[2]};
+                sym.mirror(r, x);
+                ps.insert(ps.end(), {x[0], x[1], x[2], z.real(), z.imag(), 0.0});
+            }
+        for (const nc::Source &s : P.pw)
+            for (int r = 0; r < nref; ++r) {
+                double dir[3] = {s.xyz[0], s.xyz[1], s.xyz[2]}, dc = 0.0;
+                for (size_t b = 0; b < sym.dims.size(); ++b) {
+                    if (r >> b & 1) dir[sym.dims[b]] = -dir[sym.dims[b]];
+                    dc += dir[sym.dims[b]] * P.sym_coord[sym.dims[b]];
+                }
+                const cplx z = scaled(s) * sym.sign(r) * std::exp(cplx(0.0, -P.h * k * dc));
+                pw.insert(pw.end(), {dir[0], dir[1], dir[2], z.real(), z.imag(), 0.0});
+            }
         ncg_frequency fr;
         fr.k = k;
         fr.n_rhs = 1;
         fr.bc_value = bcv.data();
         fr.admittance = adm.data();
-        fr.n_point_sources = (int)P.ps.size();
+        fr.n_point_sources = (int)ps.size() / 6;
         fr.point_sources = ps.empty() ? nullptr : ps.data();
-        fr.n_plane_waves = (int)P.pw.size();
+        fr.n_plane_waves = (int)pw.size() / 6;
         fr.plane_waves = pw.empty() ? nullptr : pw.data();
         int rc = ncg_assemble(prob, &fr, precision);
         if (rc) gpu_fatal(log, "ncg_assemble", rc);
@@ -407,7 +423,7 @@ int main(int argc, char **argv)
         } else if (rc) {
             gpu_fatal(log, "ncg_solve (direct)", rc);
         }
-        std::vector<cplx> x(n);
+        std::vector<cplx> x(N);
         rc = ncg_get_solution(prob, (double *)x.data());
         if (rc) gpu_fatal(log, "ncg_get_solution", rc);
         const time_t tsol = time(nullptr);

@@ -206,6 +206,9 @@ class Project:
     c: float = 343.0
     rho: float = 1.1839
     title: str = "synthetic project"
+    # symmetry planes of NC.inp (src/NC_Input.cpp:459-495): per axis x, y, z the plane type
+    # (0 none, 1 symmetric, 2 antisymmetric, 3 and 4 as NumCalc reads them) and its coordinate
+    symmetry: tuple | None = None
 
 
 def _fmt_freq_curve(freqs):
@@ -240,6 +243,7 @@ def write_project(folder: str, prj: Project) -> str:
         n_nodes += len(prj.eval_grid.nodes)
         n_elems += len(prj.eval_grid.elems)
         groups = 2
+    nsym = 0 if prj.symmetry is None else sum(1 for t in prj.symmetry[0] if t)
     bnd = prj.boundary or [Boundary(int(prj.mesh.elem_ids[0]), int(prj.mesh.elem_ids[-1]),
                                     "VELO", 0.0, 0.0)]
     L = ["##", "## synthetic NumCalc project", "##", "Mesh2HRTF 1.0.0", "##", prj.title, "##",
@@ -247,7 +251,7 @@ def write_project(folder: str, prj: Project) -> str:
          "1 %d 0.000001 0.00e+00 1 0 0" % len(prj.freqs), "##", "## Load Frequency Curve"]
     L += _fmt_freq_curve(prj.freqs)
     L += ["##", "## 1. Main Parameters I",
-          "%d %d %d 0 0 2 1 %d %d" % (groups, n_elems, n_nodes, prj.method_bem, prj.method_solver),
+          "%d %d %d 0 %d 2 1 %d %d" % (groups, n_elems, n_nodes, nsym, prj.method_bem, prj.method_solver),
           "##", "## 2. Main Parameters II",
           "%d %d 0 0.0000e+00 %d 0 0" % (len(prj.plane_waves), len(prj.point_sources), prj.precond),
           "##", "## 3. Main Parameters III", "0 0 0 0", "##", "## 4. Main Parameters IV",
@@ -258,7 +262,10 @@ def write_project(folder: str, prj: Project) -> str:
     L += ["##", "ELEMENTS", "ObjectMesh/Elements.txt"]
     if prj.eval_grid is not None:
         L.append("EvalGrid/Elements.txt")
-    L += ["##", "BOUNDARY"]
+    L.append("##")
+    if nsym:
+        L += ["SYMMETRY", "%d %d %d" % tuple(prj.symmetry[0]), "%.17g %.17g %.17g" % tuple(prj.symmetry[1]), "##"]
+    L.append("BOUNDARY")
     for b in bnd:
         L.append("ELEM %d TO %d %s %.17g -1 %.17g -1" % (b.lo, b.hi, b.kind, b.re, b.im))
     L += ["RETU", "##"]

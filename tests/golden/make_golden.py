@@ -68,6 +68,24 @@ def cube_sphere(n, radius):
     return P.Mesh(nodes, np.asarray(out))
 
 
+def cut_mesh(mesh, planes, shift=(0.0, 0.0, 0.0), triangles=False):
+    """The part of ``mesh`` (moved by ``shift``) on the + side of every plane (axis,
+    coordinate): elements whose centroid lies beyond each plane.  Quads are split into two
+    triangles when ``triangles``."""
+    nodes = mesh.nodes + np.asarray(shift)
+    cen = nodes[mesh.elems].mean(axis=1)
+    keep = np.ones(len(mesh.elems), bool)
+    for axis, c in planes:
+        keep &= cen[:, axis] > c
+    el = mesh.elems[keep]
+    if triangles:
+        el = np.concatenate([el[:, [0, 1, 2]], el[:, [0, 2, 3]]])
+    used = np.unique(el)
+    remap = -np.ones(len(nodes), np.int64)
+    remap[used] = np.arange(len(used))
+    return P.Mesh(nodes[used], remap[el])
+
+
 def cases():
     ico2 = P.icosphere(2, 0.1)
     ne = len(ico2.elems)
@@ -130,6 +148,42 @@ def cases():
         mesh=head12, freqs=[5000.0], eval_grid=P.fibonacci_grid(80, 1.0), method_bem=4,
         boundary=[P.Boundary(0, n12 - 1, "VELO", 0.0, 0.0), P.Boundary(100, 103, "VELO", 0.1, 0.0)],
         title="ML-FMM head-like mesh vibrating patch"))
+    # symmetry planes (NC.inp SYMMETRY block): mesh parts on one side of the planes
+    cs6, cs8 = cube_sphere(6, 0.1), cube_sphere(8, 0.1)
+    half = cut_mesh(cs6, [(2, 0.03)], shift=(0.0, 0.0, 0.03))
+    nh = len(half.elems)
+    C["sym_half"] = dict(prj=P.Project(
+        mesh=half, freqs=[800.0, 2500.0], eval_grid=P.fibonacci_grid(60, 0.6), symmetry=((0, 0, 1), (0.0, 0.0, 0.03)),
+        boundary=[P.Boundary(0, nh - 1, "VELO", 0.0, 0.0), P.Boundary(5, 9, "VELO", 0.1, 0.02),
+                  P.Boundary(40, 60, "ADMI", 0.4, 0.1)],
+        point_sources=[(0.02, 0.01, 0.3, 1.0, 0.0)], plane_waves=[(0.6, 0.0, 0.8, 0.5, 0.2)],
+        title="half cube-sphere, symmetric plane z=0.03"))
+    quarter = cut_mesh(cs8, [(0, 0.0), (1, 0.0)], triangles=True)
+    nq = len(quarter.elems)
+    C["sym_quarter_anti"] = dict(prj=P.Project(
+        mesh=quarter, freqs=[1800.0], eval_grid=P.fibonacci_grid(60, 0.7), symmetry=((2, 1, 0), (0.0, 0.0, 0.0)),
+        boundary=[P.Boundary(0, nq - 1, "VELO", 0.0, 0.0), P.Boundary(10, 14, "VELO", 0.05, -0.02)],
+        point_sources=[(0.05, 0.12, 0.1, 1.0, 0.3)], plane_waves=[(0.0, 0.6, -0.8, 1.0, 0.0)],
+        title="quarter sphere (triangles), antisymmetric x=0, symmetric y=0"))
+    octant = cut_mesh(cs8, [(0, 0.0), (1, 0.0), (2, 0.0)])
+    no = len(octant.elems)
+    C["sym_octant"] = dict(prj=P.Project(
+        mesh=octant, freqs=[2200.0], eval_grid=P.fibonacci_grid(50, 0.8), symmetry=((1, 4, 3), (0.0, 0.0, 0.0)),
+        boundary=[P.Boundary(0, no - 1, "VELO", 0.0, 0.0), P.Boundary(3, 4, "VELO", 0.1, 0.0),
+                  P.Boundary(8, 12, "IMPE", 3.0, 1.0)],
+        point_sources=[(0.1, 0.2, 0.05, 1.0, 0.0)], title="octant, planes x sym, y antisym (4), z sym (3)"))
+    cs12, cs16 = cube_sphere(12, 0.1), cube_sphere(16, 0.1)
+    C["sym_fmm_sl"] = dict(prj=P.Project(
+        mesh=cut_mesh(cs12, [(2, 0.0)]), freqs=[2000.0], eval_grid=P.fibonacci_grid(60, 0.6), method_bem=1,
+        symmetry=((0, 0, 1), (0.0, 0.0, 0.0)), point_sources=[(0.03, 0.0, 0.2, 1.0, 0.0)],
+        title="SL-FMM half sphere, symmetric z=0"))
+    ml = cut_mesh(cs16, [(1, 0.0)])
+    nm = len(ml.elems)
+    C["sym_fmm_ml"] = dict(prj=P.Project(
+        mesh=ml, freqs=[4000.0], eval_grid=P.fibonacci_grid(60, 0.9), method_bem=4,
+        symmetry=((0, 2, 0), (0.0, 0.0, 0.0)),
+        boundary=[P.Boundary(0, nm - 1, "VELO", 0.0, 0.0), P.Boundary(30, 35, "VELO", 0.1, 0.0)],
+        plane_waves=[(0.0, 0.6, 0.8, 1.0, 0.0)], title="ML-FMM half sphere, antisymmetric y=0"))
     return C
 
 
@@ -139,7 +193,8 @@ def spec_of(prj):
                 point_sources=[list(map(float, s)) for s in prj.point_sources],
                 plane_waves=[list(map(float, s)) for s in prj.plane_waves],
                 method_bem=prj.method_bem, method_solver=prj.method_solver,
-                precond=prj.precond, c=prj.c, rho=prj.rho)
+                precond=prj.precond, c=prj.c, rho=prj.rho,
+                symmetry=None if prj.symmetry is None else [list(prj.symmetry[0]), list(prj.symmetry[1])])
 
 
 def run_case(name, prj):

@@ -88,6 +88,61 @@ def problem_arrays(case, freq, h=1.0):
                 k=k, h=h, rpfact=rpfact)
 
 
+def unfold_symmetry(pa, spec):
+    """Problem arrays of the mesh unfolded across the NC.inp symmetry planes.
+
+    NumCalc integrates over the mirror images of every element, with the factor -1 per
+    antisymmetric plane (NC_ReflectElement src/NC_3dFunctions.cpp:357-452), and mirrors
+    the sources (src/NC_EquationSystem.cpp:1735-1810; plane-wave phase taken relative to
+    the planes).  The same problem with the images as elements of their own: image r of
+    element q is element r n + q (nodes mirrored; nodes 0<->1 and, for quads, 2<->3
+    swapped after an odd number of mirrors), its prescribed velocity or pressure times the
+    image's sign; its solution restricted to the first n elements is NumCalc's.
+    """
+    sym = spec.get("symmetry")
+    if not sym:
+        return pa
+    types = [{0: 0, 1: 1, 2: -1, 3: 2, 4: -2}[int(t)] for t in sym[0]]
+    c = np.asarray(sym[1], np.float64)
+    dims = [d for d in range(3) if types[d]]
+    cs = np.zeros(3)
+    cs[dims] = c[dims]
+    xyz, nv, kind, bc, adm, ps, pw = [], [], [], [], [], [], []
+    for r in range(1 << len(dims)):
+        refl = [dims[b] for b in range(len(dims)) if r >> b & 1]
+        sgn = float(np.prod([-1.0 if types[d] < 0 else 1.0 for d in refl]))
+        x = pa["xyz"].copy()
+        for d in refl:
+            x[:, :, d] = 2.0 * c[d] - x[:, :, d]
+        if len(refl) % 2:
+            x[:, [0, 1]] = x[:, [1, 0]]
+            q4 = np.where(pa["nv"] == 4)[0]
+            x[np.ix_(q4, [2, 3])] = x[np.ix_(q4, [3, 2])]
+        xyz.append(x)
+        nv.append(pa["nv"])
+        kind.append(pa["kind"])
+        bc.append(pa["bc"] * sgn)
+        adm.append(pa["adm"])
+        for s in pa["ps"]:
+            q = s.copy()
+            for d in refl:
+                q[d] = 2.0 * c[d] - q[d]
+            q[3:5] *= sgn
+            ps.append(q)
+        for s in pa["pw"]:
+            q = s.copy()
+            for d in refl:
+                q[d] = -q[d]
+            z = complex(q[3], q[4]) * sgn * np.exp(-1j * pa["h"] * pa["k"] * (q[:3] @ cs))
+            q[3:5] = z.real, z.imag
+            pw.append(q)
+    out = dict(pa)
+    out.update(n=pa["n"] << len(dims), xyz=np.concatenate(xyz), nv=np.concatenate(nv), kind=np.concatenate(kind),
+               bc=np.concatenate(bc), adm=np.concatenate(adm), ps=np.asarray(ps).reshape(-1, 5),
+               pw=np.asarray(pw).reshape(-1, 5))
+    return out
+
+
 def surface_outputs(pa, sol):
     """pBoundary and vBoundary values from the solved unknowns."""
     kind = pa["kind"]

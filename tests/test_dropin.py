@@ -39,7 +39,8 @@ def project_from_case(folder, name, steps=None):
                     eval_grid=grid, boundary=[P.Boundary(*b) for b in spec["boundary"]],
                     point_sources=[tuple(s) for s in spec["point_sources"]],
                     plane_waves=[tuple(s) for s in spec["plane_waves"]], method_solver=spec["method_solver"],
-                    method_bem=spec["method_bem"], precond=spec["precond"], c=spec["c"], rho=spec["rho"])
+                    method_bem=spec["method_bem"], precond=spec["precond"], c=spec["c"], rho=spec["rho"],
+                    symmetry=spec.get("symmetry"))
     P.write_project(folder, prj)
     return c
 
@@ -123,15 +124,56 @@ def test_estimate_ram_matches_reference(tmp_path, name):
                  skip_nonzeros=True)
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch", "fmm_sl_point",
-                                  "fmm_ml_head"])
-def test_dropin_matches_reference_be_out(tmp_path, name):
-    """be.out of every step (rel-L2 1e-6 dense, 1e-5 FMM: the FMM solve differs from the
-    reference's by CGS iteration-path rounding) and the NC.out analysis tables."""
+def test_symmetry_input_errors(tmp_path):
+    """SYMMETRY block checks of src/NC_Input.cpp:459-495, 678-699 (before any device use)."""
+    _need_binary()
+    case = project_from_case(str(tmp_path), "sym_half")
+    txt = open(tmp_path / "NC.inp").read()
+    open(tmp_path / "NC.inp", "w").write(txt.replace("SYMMETRY\n0 0 1\n", "SYMMETRY\n0 0 5\n"))
+    r = subprocess.run([NUMCALC, "-estimate_ram"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0 and "must be equal to 0, 1, 2, 3, or 4" in r.stdout
+    open(tmp_path / "NC.inp", "w").write(txt.replace("SYMMETRY\n0 0 1\n", "SYMMETRY\n1 0 1\n"))
+    r = subprocess.run([NUMCALC, "-estimate_ram"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0 and "j must equal to numSymmetricPlanes_" in r.stdout
+    # a plane through the mesh: x = 0 cuts the half sphere
+    open(tmp_path / "NC.inp", "w").write(txt.replace("SYMMETRY\n0 0 1\n", "SYMMETRY\n1 0 0\n"))
+    r = subprocess.run([NUMCALC, "-estimate_ram"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0 and "All BE nodes must be located on the same side of a symmetric plane!" in r.stdout
+    del case
+
+
+@pytest.mark.parametrize("name", ["sym_half", "sym_fmm_ml"])
+def test_symmetry_estimate_ram_runs(tmp_path, name):
+    """-estimate_ram on a symmetric project plans the mesh unfolded across its planes."""
     _need_binary()
     case = project_from_case(str(tmp_path), name)
-    tol = 1e-5 if name.startswith("fmm_") else 1e-6
+    r = subprocess.run([NUMCALC, "-estimate_ram"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    lines = open(tmp_path / "Memory.txt").read().splitlines()
+    assert len(lines) == len(case["spec"]["freqs"])
+    log = open(tmp_path / "NC.out").read()
+    assert "Number of symmetric planes .................. = %d" % sum(1 for t in case["spec"]["symmetry"][0] if t) in log
+    assert "Number of equations = %d" % len(case["elems"]) in log
+
+
+# rel-L2 bounds against the reference be.out: dense 1e-6 (7 printed digits); FMM 1e-5 (CGS
+# iteration-path rounding).  Symmetry-plane projects are solved unfolded across the planes:
+# dense ones to 1e-6; FMM ones differ from the reference's reflected-cluster FMM by the two
+# FMM truncation errors (the reference's own runs are 1.4e-4 / 3.5e-4 from the exact surface
+# solution and 3e-5 on the grid)
+DROPIN_TOL = {"fmm_sl_point": 1e-5, "fmm_ml_head": 1e-5, "sym_fmm_sl": 2e-3, "sym_fmm_ml": 2e-3}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch", "fmm_sl_point",
+                                  "fmm_ml_head", "sym_half", "sym_quarter_anti", "sym_octant", "sym_fmm_sl",
+                                  "sym_fmm_ml"])
+def test_dropin_matches_reference_be_out(tmp_path, name):
+    """be.out of every step and, without symmetry planes, the CGS iteration counts and the
+    NC.out analysis tables (the unfolded symmetric system is larger than NumCalc's)."""
+    _need_binary()
+    case = project_from_case(str(tmp_path), name)
+    tol = DROPIN_TOL.get(name, 1e-6)
     print("drop-in case", name, flush=True)
     r = subprocess.run([NUMCALC], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
@@ -147,6 +189,8 @@ def test_dropin_matches_reference_be_out(tmp_path, name):
                 assert np.abs(vals).max() < 1e-12
             else:
                 assert R.rel_l2(vals, ref) < tol, (key, s)
+    if name.startswith("sym_"):
+        return
     if case["spec"]["method_solver"] != 4:
         its = [int(v) for v in re.findall(r"CGS solver: number of iterations = (\d+), relative error = ", log)]
         assert its == list(case["cgs_iterations"])
