@@ -484,7 +484,8 @@ void read_project(std::ofstream &log, Project &P, bool check_normals)
     P.precond = to_int(tok[4]);
     if (P.npw < 0) fatal(log, "Number of the incident plane waves must >= 0!");
     if (P.nps < -1) fatal(log, "Number of the point sources must >= -1!");
-    if (P.nps == -1) fatal(log, "The test point source (number of point sources = -1) is not supported!");
+    // -1 ("test point source", src/NC_Main.cpp:342-343) reads no point source; the count
+    // still enters NumCalc's tests for incident waves (see the driver)
     if (P.interior != 0 && P.interior != 1) fatal(log, "IFINTE must = 0 or 1!", "IFINTE = " + str(P.interior));
     if (P.precond < -3 || P.precond > 2) fatal(log, "IPRECOND must be in [-3 2]!", "IPRECOND = " + str(P.precond));
     if (P.precond <= -3) { P.scan_degree = 3; P.precond = 0; }

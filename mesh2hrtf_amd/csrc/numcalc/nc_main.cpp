@@ -375,6 +375,12 @@ int main(int argc, char **argv)
                 const cplx z = scaled(s) * sym.sign(r) * std::exp(cplx(0.0, -P.h * k * dc));
                 pw.insert(pw.end(), {dir[0], dir[1], dir[2], z.real(), z.imag(), 0.0});
             }
+        // NumCalc adds the incident waves to the right-hand side only when
+        // numIncidentPlaneWaves_ + numPointSources_ > 0 (src/NC_EquationSystem.cpp:247,1286),
+        // but to the evaluation grid whenever either count is nonzero
+        // (src/NC_PostProcessing.cpp:640,664): with the test point source (-1) and one plane
+        // wave, the wave reaches the grid only.
+        const bool rhs_waves = P.npw + P.nps > 0;
         ncg_frequency fr;
         fr.k = k;
         fr.n_rhs = 1;
@@ -382,7 +388,7 @@ int main(int argc, char **argv)
         fr.admittance = adm.data();
         fr.n_point_sources = (int)ps.size() / 6;
         fr.point_sources = ps.empty() ? nullptr : ps.data();
-        fr.n_plane_waves = (int)pw.size() / 6;
+        fr.n_plane_waves = rhs_waves ? (int)pw.size() / 6 : 0;
         fr.plane_waves = pw.empty() ? nullptr : pw.data();
         int rc = ncg_assemble(prob, &fr, precision);
         if (rc) gpu_fatal(log, "ncg_assemble", rc);
@@ -444,6 +450,15 @@ int main(int argc, char **argv)
             std::vector<cplx> phi(m), grad(3 * (size_t)m);
             rc = ncg_field(prob, (double *)phi.data(), (double *)grad.data());
             if (rc) gpu_fatal(log, "ncg_field", rc);
+            if (!rhs_waves)   // the grid-only plane waves: exp(i h k d.x) and its gradient
+                for (size_t w = 0; w < pw.size(); w += 6)
+                    for (int q = 0; q < m; ++q) {
+                        const double *x = &evalp[3 * (size_t)q];
+                        const cplx g = cplx(pw[w + 3], pw[w + 4]) *
+                                       std::exp(cplx(0.0, P.h * k * (pw[w] * x[0] + pw[w + 1] * x[1] + pw[w + 2] * x[2])));
+                        phi[q] += g;
+                        for (int d = 0; d < 3; ++d) grad[3 * (size_t)q + d] += cplx(0.0, P.h * k) * g * pw[w + d];
+                    }
             nc::write_evalgrid(log, P, step + 1, phi, grad, rpfact);
         }
         const time_t tp = time(nullptr);

@@ -209,6 +209,8 @@ class Project:
     # symmetry planes of NC.inp (src/NC_Input.cpp:459-495): per axis x, y, z the plane type
     # (0 none, 1 symmetric, 2 antisymmetric, 3 and 4 as NumCalc reads them) and its coordinate
     symmetry: tuple | None = None
+    # NC.inp point-source count -1 (NumCalc's "test point source"): no point source is read
+    test_point_source: bool = False
 
 
 def _fmt_freq_curve(freqs):
@@ -253,7 +255,8 @@ def write_project(folder: str, prj: Project) -> str:
     L += ["##", "## 1. Main Parameters I",
           "%d %d %d 0 %d 2 1 %d %d" % (groups, n_elems, n_nodes, nsym, prj.method_bem, prj.method_solver),
           "##", "## 2. Main Parameters II",
-          "%d %d 0 0.0000e+00 %d 0 0" % (len(prj.plane_waves), len(prj.point_sources), prj.precond),
+          "%d %d 0 0.0000e+00 %d 0 0" % (len(prj.plane_waves), -1 if prj.test_point_source else len(prj.point_sources),
+                                         prj.precond),
           "##", "## 3. Main Parameters III", "0 0 0 0", "##", "## 4. Main Parameters IV",
           "%s %s 1.0 0.0e+00 0.0 e+00 0.0e+00 0.0e+00" % (repr(prj.c), repr(prj.rho)), "##",
           "NODES", "ObjectMesh/Nodes.txt"]
@@ -274,7 +277,7 @@ def write_project(folder: str, prj: Project) -> str:
         for i, (dx, dy, dz, re, im) in enumerate(prj.plane_waves):
             L.append("%d %.17g %.17g %.17g %.17g -1 %.17g -1" % (i, dx, dy, dz, re, im))
         L.append("##")
-    if prj.point_sources:
+    if prj.point_sources and not prj.test_point_source:
         L.append("POINT SOURCES")
         for i, (x, y, z, re, im) in enumerate(prj.point_sources):
             L.append("%d %.17g %.17g %.17g %.17g -1 %.17g -1" % (i, x, y, z, re, im))

@@ -184,6 +184,15 @@ def cases():
         symmetry=((0, 2, 0), (0.0, 0.0, 0.0)),
         boundary=[P.Boundary(0, nm - 1, "VELO", 0.0, 0.0), P.Boundary(30, 35, "VELO", 0.1, 0.0)],
         plane_waves=[(0.0, 0.6, 0.8, 1.0, 0.0)], title="ML-FMM half sphere, antisymmetric y=0"))
+    # point-source count -1 (the reference's "test point source"): with one plane wave it
+    # reaches the evaluation grid but not the right-hand side; with two it reaches both
+    C["test_source_1pw"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[900.0], eval_grid=P.fibonacci_grid(40, 0.6), test_point_source=True,
+        boundary=[P.Boundary(0, ne - 1, "VELO", 0.0, 0.0), P.Boundary(30, 33, "VELO", 0.1, 0.0)],
+        plane_waves=[(0.0, 0.0, 1.0, 1.0, 0.0)], title="test point source, one plane wave"))
+    C["test_source_2pw"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[900.0], eval_grid=P.fibonacci_grid(40, 0.6), test_point_source=True,
+        plane_waves=[(0.0, 0.0, 1.0, 1.0, 0.0), (0.6, 0.8, 0.0, 0.5, 0.5)], title="test point source, two plane waves"))
     return C
 
 
@@ -194,7 +203,8 @@ def spec_of(prj):
                 plane_waves=[list(map(float, s)) for s in prj.plane_waves],
                 method_bem=prj.method_bem, method_solver=prj.method_solver,
                 precond=prj.precond, c=prj.c, rho=prj.rho,
-                symmetry=None if prj.symmetry is None else [list(prj.symmetry[0]), list(prj.symmetry[1])])
+                symmetry=None if prj.symmetry is None else [list(prj.symmetry[0]), list(prj.symmetry[1])],
+                test_point_source=prj.test_point_source)
 
 
 def run_case(name, prj):

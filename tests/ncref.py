@@ -84,8 +84,19 @@ def problem_arrays(case, freq, h=1.0):
         z = complex(w[3], w[4]) / (1j * rpfact)
         pw.append([*d, z.real, z.imag])
     pw = np.asarray(pw).reshape(-1, 5)
+    # NumCalc adds incident waves to the right-hand side only when the plane-wave count plus
+    # the point-source count (-1 for its "test point source") is positive
+    # (src/NC_EquationSystem.cpp:247), to the evaluation grid whenever either is nonzero
+    nps = -1 if spec.get("test_point_source") else len(spec["point_sources"])
     return dict(n=n, nv=nv, xyz=xyz, kind=kind, bc=val, adm=adm, ps=ps, pw=pw,
-                k=k, h=h, rpfact=rpfact)
+                k=k, h=h, rpfact=rpfact, rhs_waves=len(pw) + nps > 0)
+
+
+def rhs_arrays(pa):
+    """The problem arrays the right-hand side is assembled from."""
+    if pa.get("rhs_waves", True):
+        return pa
+    return dict(pa, pw=pa["pw"][:0], ps=pa["ps"][:0])
 
 
 def unfold_symmetry(pa, spec):

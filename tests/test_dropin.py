@@ -40,7 +40,7 @@ def project_from_case(folder, name, steps=None):
                     point_sources=[tuple(s) for s in spec["point_sources"]],
                     plane_waves=[tuple(s) for s in spec["plane_waves"]], method_solver=spec["method_solver"],
                     method_bem=spec["method_bem"], precond=spec["precond"], c=spec["c"], rho=spec["rho"],
-                    symmetry=spec.get("symmetry"))
+                    symmetry=spec.get("symmetry"), test_point_source=spec.get("test_point_source", False))
     P.write_project(folder, prj)
     return c
 
@@ -167,7 +167,7 @@ DROPIN_TOL = {"fmm_sl_point": 1e-5, "fmm_ml_head": 1e-5, "sym_fmm_sl": 2e-3, "sy
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch", "fmm_sl_point",
                                   "fmm_ml_head", "sym_half", "sym_quarter_anti", "sym_octant", "sym_fmm_sl",
-                                  "sym_fmm_ml"])
+                                  "sym_fmm_ml", "test_source_1pw", "test_source_2pw"])
 def test_dropin_matches_reference_be_out(tmp_path, name):
     """be.out of every step and, without symmetry planes, the CGS iteration counts and the
     NC.out analysis tables (the unfolded symmetric system is larger than NumCalc's)."""

@@ -45,12 +45,13 @@ def test_oracle_matches_reference_numcalc(oracle, name):
     """Dense oracle solve against the reference's be.out.  Symmetry-plane projects (sym_*)
     are solved unfolded across their planes (one, two and three planes; symmetric and
     antisymmetric; plane types 3 and 4; a plane off the origin): the first n unknowns of the
-    unfolded problem reproduce NumCalc's reflected-element solve."""
+    unfolded problem reproduce NumCalc's reflected-element solve.  test_source_*: the point-
+    source count -1, with which one plane wave reaches the grid but not the right-hand side."""
     case = R.load_case(name)
     for s, f in enumerate(case["spec"]["freqs"]):
         pa = R.problem_arrays(case, f)
         pu = R.unfold_symmetry(pa, case["spec"])
-        x = _solve(oracle, case, pu)
+        x = _solve(oracle, case, R.rhs_arrays(pu))
         pB, vB = R.surface_outputs(pa, x[:pa["n"]])
         _check(pB, case["pBoundary"][s], GOLDEN_TOL)
         _check(vB, case["vBoundary"][s], GOLDEN_TOL)
