@@ -327,6 +327,10 @@ struct ncg_problem {
     double *scal = nullptr, *partial = nullptr, *dotout = nullptr, *dvec = nullptr;
     size_t partial_cap = 0;   // complex entries of partial
     double *npart = nullptr;  // [64 load cases][256 blocks] partial norms of the Gram-Schmidt passes
+    // GMRES: pinned landing slots for two Arnoldi columns and the events that mark them
+    cd *hpin = nullptr;
+    size_t hpin_slot = 0;
+    hipEvent_t hev[2] = {nullptr, nullptr};
     // block-Jacobi of a dense operator: blocks of at most 64 consecutive unknowns
     Pool dbj;
     int dbj_nblk = 0, dbj_ready = 0, dbj_f32 = 0;
@@ -449,12 +453,24 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         KernelScope ks(p, 8 + mixed, n * n * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
         // more than 4 load cases: the FP32 operator goes through the matrix cores, up to 16
         // load cases per read of A (the vector kernel reads A once per 4)
+        // (FP32 accumulation for the mixed form's Krylov products, FP64 for its refinement
+        // residual and for the FP64 operator)
         size_t xf = 0, pf = 0;
-        if (mixed && !acc64 && !d && p->nrhs > 4) ncgk::matvec_mfma_scratch(p->n, p->nrhs, &xf, &pf);
-        if (xf && !p->mm_xh.ensure(sizeof(float) * xf) && !p->mm_part.ensure(sizeof(float) * pf))
-            ncgk::matvec_mfma(p->stream, p->A, p->n, p->nrhs, x, y, (float *)p->mm_xh.p, (float *)p->mm_part.p);
-        else
+        const bool fp64acc = !mixed || acc64;
+        if (!d && p->nrhs > 4) {
+            if (fp64acc) ncgk::matvec_mfma64_scratch(p->n, &xf, &pf);
+            else ncgk::matvec_mfma_scratch(p->n, p->nrhs, &xf, &pf);
+        }
+        const size_t el = fp64acc ? sizeof(double) : sizeof(float);
+        if (xf && !p->mm_xh.ensure(el * xf) && !p->mm_part.ensure(el * pf)) {
+            if (fp64acc)
+                ncgk::matvec_mfma64(p->stream, p->A, mixed, p->n, p->nrhs, x, y, (double *)p->mm_xh.p,
+                                    (double *)p->mm_part.p);
+            else
+                ncgk::matvec_mfma(p->stream, p->A, p->n, p->nrhs, x, y, (float *)p->mm_xh.p, (float *)p->mm_part.p);
+        } else {
             ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);
+        }
     } else {
         // y = Near x + sum over levels S (D (T x))  (src/NC_CommonFunctions.cpp:1247-1316)
         const int mixed = p->precision == NCG_MIXED;
@@ -1408,7 +1424,6 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
     iters.assign(nr, 0);
     relres.assign(nr, 1.0);
     std::vector<int> done(nr, 0);
-    std::vector<cd> hcol;
     int total = 0;
     bool first = true;
     while (total < maxit) {
@@ -1440,8 +1455,12 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
         std::vector<std::vector<cd>> g(nr, std::vector<cd>(m + 1)), cs(nr, std::vector<cd>(m)), sn(nr, std::vector<cd>(m));
         std::vector<int> kk(nr, 0);
         for (int r = 0; r < nr; ++r) g[r][0] = beta[r];
-        int k;
-        for (k = 0; k < m && total < maxit; ++k, ++total) {
+        // Arnoldi step k on the device: z = M^-1 v_k, w = A z, two Gram-Schmidt passes and the
+        // norm; dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3] goes to pinned slot
+        // k % 2.  Step k + 1 is launched before step k's column is read back, so the device
+        // is not idle while the host applies the rotations (a converged step leaves one
+        // extra Arnoldi step unused).
+        auto launch_step = [&](int k) -> int {
             double *Vk = V + 2 * (size_t)k * vlen;
             if (bj) {
                 bj_apply_vec(p, Vk, Z);
@@ -1449,16 +1468,33 @@ int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, do
             } else {
                 mv(p, acc64, Vk, nullptr, W);
             }
-            // projections h1, h2 (two Gram-Schmidt passes) and the norm, all on the device:
-            // dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3]
             cd *h1 = (cd *)p->dotout, *h2 = h1 + (size_t)nr * (k + 1), *hn = h2 + (size_t)nr * (k + 1);
             ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h1, p->partial, p->partial_cap, nullptr);
             ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h2, p->partial, p->partial_cap, p->npart);
             ncgk::gs_normalize(p->stream, n, nr, p->npart, W, V + 2 * (size_t)(k + 1) * vlen, hn);
             const size_t ncol = 2 * (size_t)nr * (k + 1) + 3 * (size_t)nr;
-            hcol.resize(ncol);
-            NCG_HIP(hipMemcpyAsync(hcol.data(), h1, sizeof(cd) * ncol, hipMemcpyDeviceToHost, p->stream));
-            NCG_HIP(hipStreamSynchronize(p->stream));
+            NCG_HIP(hipMemcpyAsync(p->hpin + (size_t)(k & 1) * p->hpin_slot, h1, sizeof(cd) * ncol,
+                                   hipMemcpyDeviceToHost, p->stream));
+            NCG_HIP(hipEventRecord(p->hev[k & 1], p->stream));
+            return 0;
+        };
+        const size_t slot = 2 * (size_t)nr * (m + 1) + 3 * (size_t)nr;
+        if (slot > p->hpin_slot) {
+            if (p->hpin) hipHostFree(p->hpin);
+            p->hpin = nullptr;
+            p->hpin_slot = 0;
+            hipError_t e = hipHostMalloc((void **)&p->hpin, sizeof(cd) * 2 * slot, hipHostMallocDefault);
+            if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+            p->hpin_slot = slot;
+        }
+        for (hipEvent_t &e : p->hev)
+            if (!e) NCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        int k;
+        if (total < maxit && (rc = launch_step(0))) return rc;
+        for (k = 0; k < m && total < maxit; ++k, ++total) {
+            if (k + 1 < m && total + 1 < maxit && (rc = launch_step(k + 1))) return rc;
+            NCG_HIP(hipEventSynchronize(p->hev[k & 1]));
+            const cd *hcol = p->hpin + (size_t)(k & 1) * p->hpin_slot;
             bool all = true;
             for (int r = 0; r < nr; ++r) {
                 const double hnr = sqrt(hcol[2 * (size_t)nr * (k + 1) + 3 * r].real());
@@ -1720,6 +1756,9 @@ int ncg_problem_destroy(ncg_problem *p)
     }
     for (EventPair &e : p->ev_mv) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (EventPair &e : p->kpool) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    for (hipEvent_t e : p->hev)
+        if (e) hipEventDestroy(e);
+    if (p->hpin) hipHostFree(p->hpin);
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
     return 0;

@@ -2323,33 +2323,47 @@ typedef float ncg_floatx16 __attribute__((ext_vector_type(16)));
 __global__ void __launch_bounds__(256) k_matvec_mfma(const cx<float> *__restrict__ A, int n, int chunks_per_split,
                                                      const float *__restrict__ xh, float *__restrict__ part)
 {
-    __shared__ float As[kMfmaRows][2 * kMfmaCols + 1];
+    // row stride 66 floats: the 32 rows a wave reads per step fall on distinct banks
+    __shared__ float As[kMfmaRows][2 * kMfmaCols + 2];
     __shared__ float Xs[2 * kMfmaCols][32];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r0 = blockIdx.x * kMfmaRows;
     const int nchunks = (n + kMfmaCols - 1) / kMfmaCols;
     const int c0 = blockIdx.y * chunks_per_split, c1 = min(nchunks, c0 + chunks_per_split);
+    // the next chunk is fetched into registers while the matrix cores work on this one:
+    // A: 16 complex per thread (rows t / 32 + 8 i, column t % 32); X^: 8 floats per thread
+    constexpr int AQ = kMfmaRows * kMfmaCols / 256, XQ = 2 * kMfmaCols * 32 / 256;
+    cx<float> ar[AQ];
+    float xr[XQ];
+    auto fetch = [&](int ch) {
+        const int j0 = ch * kMfmaCols, col = j0 + t % kMfmaCols;
+#pragma unroll
+        for (int i = 0; i < AQ; ++i) {
+            const int row = r0 + t / kMfmaCols + 8 * i;
+            ar[i] = row < n && col < n ? A[(size_t)row * n + col] : cx<float>{0.f, 0.f};
+        }
+#pragma unroll
+        for (int i = 0; i < XQ; ++i) {
+            const int k = 2 * j0 + t / 32 + 8 * i;
+            xr[i] = k < 2 * n ? xh[(size_t)k * 32 + t % 32] : 0.f;
+        }
+    };
     ncg_floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    if (c0 < c1) fetch(c0);
     for (int ch = c0; ch < c1; ++ch) {
-        const int j0 = ch * kMfmaCols;
         __syncthreads();
-        // A tile: 128 rows x 32 complex columns, one complex (float2) per load
-        for (int q = t; q < kMfmaRows * kMfmaCols; q += 256) {
-            const int rr = q / kMfmaCols, jj = q % kMfmaCols, row = r0 + rr, col = j0 + jj;
-            cx<float> a = {0.f, 0.f};
-            if (row < n && col < n) a = A[(size_t)row * n + col];
-            As[rr][2 * jj] = a.re;
-            As[rr][2 * jj + 1] = a.im;
+#pragma unroll
+        for (int i = 0; i < AQ; ++i) {
+            As[t / kMfmaCols + 8 * i][2 * (t % kMfmaCols)] = ar[i].re;
+            As[t / kMfmaCols + 8 * i][2 * (t % kMfmaCols) + 1] = ar[i].im;
         }
-        // X^ tile: 64 real rows x 32 columns
-        for (int q = t; q < 2 * kMfmaCols * 32; q += 256) {
-            const int kk = q / 32, c = q % 32, k = 2 * j0 + kk;
-            Xs[kk][c] = k < 2 * n ? xh[(size_t)k * 32 + c] : 0.f;
-        }
+#pragma unroll
+        for (int i = 0; i < XQ; ++i) Xs[t / 32 + 8 * i][t % 32] = xr[i];
         __syncthreads();
-#pragma unroll 4
+        if (ch + 1 < c1) fetch(ch + 1);
+#pragma unroll 8
         for (int ks = 0; ks < kMfmaCols; ++ks) {
             const float a = As[wv * 32 + (lane & 31)][2 * ks + (lane >> 5)];
             const float b = Xs[2 * ks + (lane >> 5)][lane & 31];
@@ -2378,6 +2392,129 @@ __global__ void __launch_bounds__(256) k_mfma_reduce(int n, int nr, int splits, 
         im += part[((size_t)s * 32 + 16 + r) * n + i];
     }
     y[(size_t)r * n + i] = mk(re, im);
+}
+
+// The same product with FP64 accumulation (v_mfma_f64_16x16x4f64; D lane l, register v:
+// row 4 v + l / 16, column l % 16): the refinement residual of the mixed form (FP32 A)
+// and the FP64 operator.  A wave owns 16 rows and the 32 columns in two accumulators
+// (real parts, imaginary parts); a work-group of 4 waves covers 64 rows.
+constexpr int kMfma64Rows = 64;
+
+__global__ void __launch_bounds__(256) k_xhat64(int n, int nr, const cx<double> *x, double *xh)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 32) return;
+    const int j = t / 32, c = t % 32, r = c % 16;
+    double v0 = 0.0, v1 = 0.0;
+    if (r < nr) {
+        const cx<double> a = x[(size_t)r * n + j];
+        if (c < 16) { v0 = a.re; v1 = -a.im; }
+        else { v0 = a.im; v1 = a.re; }
+    }
+    xh[(size_t)(2 * j) * 32 + c] = v0;
+    xh[(size_t)(2 * j + 1) * 32 + c] = v1;
+}
+
+typedef double ncg_doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename SA>
+__global__ void __launch_bounds__(256) k_matvec_mfma64(const cx<SA> *__restrict__ A, int n, int chunks_per_split,
+                                                       const double *__restrict__ xh, double *__restrict__ part)
+{
+    __shared__ SA As[kMfma64Rows][2 * kMfmaCols + 2];
+    __shared__ double Xs[2 * kMfmaCols][32];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int r0 = blockIdx.x * kMfma64Rows;
+    const int nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    const int c0 = blockIdx.y * chunks_per_split, c1 = min(nchunks, c0 + chunks_per_split);
+    constexpr int AQ = kMfma64Rows * kMfmaCols / 256, XQ = 2 * kMfmaCols * 32 / 256;
+    cx<SA> ar[AQ];
+    double xr[XQ];
+    auto fetch = [&](int ch) {
+        const int j0 = ch * kMfmaCols, col = j0 + t % kMfmaCols;
+#pragma unroll
+        for (int i = 0; i < AQ; ++i) {
+            const int row = r0 + t / kMfmaCols + 8 * i;
+            ar[i] = row < n && col < n ? A[(size_t)row * n + col] : cx<SA>{SA(0), SA(0)};
+        }
+#pragma unroll
+        for (int i = 0; i < XQ; ++i) {
+            const int k = 2 * j0 + t / 32 + 8 * i;
+            xr[i] = k < 2 * n ? xh[(size_t)k * 32 + t % 32] : 0.0;
+        }
+    };
+    ncg_doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    if (c0 < c1) fetch(c0);
+    for (int ch = c0; ch < c1; ++ch) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < AQ; ++i) {
+            As[t / kMfmaCols + 8 * i][2 * (t % kMfmaCols)] = ar[i].re;
+            As[t / kMfmaCols + 8 * i][2 * (t % kMfmaCols) + 1] = ar[i].im;
+        }
+#pragma unroll
+        for (int i = 0; i < XQ; ++i) Xs[t / 32 + 8 * i][t % 32] = xr[i];
+        __syncthreads();
+        if (ch + 1 < c1) fetch(ch + 1);
+#pragma unroll 4
+        for (int ks = 0; ks < 2 * kMfmaCols / 4; ++ks) {
+            const int k = 4 * ks + (lane >> 4);
+            const double a = double(As[wv * 16 + (lane & 15)][k]);
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Xs[k][lane & 15], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Xs[k][16 + (lane & 15)], acc1, 0, 0, 0);
+        }
+    }
+    double *out = part + (size_t)blockIdx.y * 32 * n;
+    const int col = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = r0 + wv * 16 + 4 * v + (lane >> 4);
+        if (row < n) {
+            out[(size_t)col * n + row] = acc0[v];
+            out[(size_t)(16 + col) * n + row] = acc1[v];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mfma64_reduce(int n, int nr, int splits, const double *part, cx<double> *y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nr) return;
+    const int r = t / n, i = t % n;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < splits; ++s) {
+        re += part[((size_t)s * 32 + r) * n + i];
+        im += part[((size_t)s * 32 + 16 + r) * n + i];
+    }
+    y[(size_t)r * n + i] = mk(re, im);
+}
+
+int matvec_mfma64_scratch(int n, size_t *xh_doubles, size_t *part_doubles)
+{
+    const int tiles = (n + kMfma64Rows - 1) / kMfma64Rows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    int splits = std::max(1, std::min(std::min(16, (1024 + tiles - 1) / tiles), (nchunks + 3) / 4));
+    *xh_doubles = (size_t)2 * n * 32;
+    *part_doubles = (size_t)splits * 32 * n;
+    return splits;
+}
+
+void matvec_mfma64(hipStream_t s, const void *A, int a_f32, int n, int nrhs, const void *x, void *y, double *xh,
+                   double *part)
+{
+    size_t xd, pd;
+    const int splits = matvec_mfma64_scratch(n, &xd, &pd);
+    const int tiles = (n + kMfma64Rows - 1) / kMfma64Rows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    const int per = (nchunks + splits - 1) / splits;
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {
+        const int nr = std::min(16, nrhs - r0);
+        k_xhat64<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x + (size_t)r0 * n, xh);
+        if (a_f32)
+            k_matvec_mfma64<float><<<dim3(tiles, splits), 256, 0, s>>>((const cx<float> *)A, n, per, xh, part);
+        else
+            k_matvec_mfma64<double><<<dim3(tiles, splits), 256, 0, s>>>((const cx<double> *)A, n, per, xh, part);
+        k_mfma64_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, splits, part,
+                                                                        (cx<double> *)y + (size_t)r0 * n);
+    }
 }
 
 int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats)

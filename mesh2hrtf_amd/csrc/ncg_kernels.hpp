@@ -49,6 +49,10 @@ void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const
 // up to 16 load cases per pass); scratch sizes from matvec_mfma_scratch
 int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats);
 void matvec_mfma(hipStream_t s, const void *A, int n, int nrhs, const void *x, void *y, float *xh, float *part);
+// the same with FP64 accumulation, A in FP32 (a_f32) or FP64
+int matvec_mfma64_scratch(int n, size_t *xh_doubles, size_t *part_doubles);
+void matvec_mfma64(hipStream_t s, const void *A, int a_f32, int n, int nrhs, const void *x, void *y, double *xh,
+                   double *part);
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d,
             void *y);
 int dot_blocks(int n);
