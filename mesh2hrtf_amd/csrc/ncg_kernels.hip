@@ -1656,17 +1656,18 @@ __global__ void __launch_bounds__(64) k_bj_invert(const int *blk_m, const long l
 // y = M^-1 x: one work-group per block; lane a is row a, the 4 waves split the columns
 // (four per step) of the column-major inverse and meet in LDS
 template <typename TB, int NR>
-__global__ void __launch_bounds__(256) k_bj_apply(int n, const int *blk_b0, const int *blk_m, const int *cl_elem,
-                                                  const long long *boff, const cx<TB> *minv, const cx<double> *x,
-                                                  cx<double> *y)
+__global__ void __launch_bounds__(256) k_bj_apply(int n, int nr, const int *blk_b0, const int *blk_m,
+                                                  const int *cl_elem, const long long *boff, const cx<TB> *minv,
+                                                  const cx<double> *x, cx<double> *y)
 {
+    // nr <= NR load cases are present; the others are zero and not stored
     __shared__ cx<double> xs[NR][kBlk];
     __shared__ double red[3][kBlk][2 * NR];
     const int b = blockIdx.x, b0 = blk_b0[b], m = blk_m[b];
     const int a = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < NR * m; t += 256) {
         const int r = t / m, j = t - r * m;
-        xs[r][j] = x[(size_t)r * n + cl_elem[b0 + j]];
+        xs[r][j] = r < nr ? x[(size_t)r * n + cl_elem[b0 + j]] : mk(0.0, 0.0);
     }
     __syncthreads();
     const cx<TB> *M = minv + boff[b];
@@ -1701,8 +1702,9 @@ __global__ void __launch_bounds__(256) k_bj_apply(int n, const int *blk_b0, cons
         const int e = cl_elem[b0 + a];
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            y[(size_t)r * n + e] = mk(re[r] + red[0][a][2 * r] + red[1][a][2 * r] + red[2][a][2 * r],
-                                      im[r] + red[0][a][2 * r + 1] + red[1][a][2 * r + 1] + red[2][a][2 * r + 1]);
+            if (r < nr)
+                y[(size_t)r * n + e] = mk(re[r] + red[0][a][2 * r] + red[1][a][2 * r] + red[2][a][2 * r],
+                                          im[r] + red[0][a][2 * r + 1] + red[1][a][2 * r + 1] + red[2][a][2 * r + 1]);
     }
 }
 
@@ -2831,16 +2833,19 @@ template <typename TB>
 static void bj_apply_t(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m,
                        const int *cl_elem, const long long *boff, const cx<TB> *M, const void *x, void *y)
 {
-    rhs_chunks(nrhs, [&](int nr, int r0) {
+    // up to 8 load cases per pass over the inverses
+    for (int r0 = 0; r0 < nrhs; r0 += 8) {
+        const int nr = std::min(8, nrhs - r0);
         const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
         switch (nr) {
-        case 1: k_bj_apply<TB, 1><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 2: k_bj_apply<TB, 2><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 3: k_bj_apply<TB, 3><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        default: k_bj_apply<TB, 4><<<nblk, 256, 0, s>>>(n, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 1: k_bj_apply<TB, 1><<<nblk, 256, 0, s>>>(n, 1, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 2: k_bj_apply<TB, 2><<<nblk, 256, 0, s>>>(n, 2, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 3: k_bj_apply<TB, 3><<<nblk, 256, 0, s>>>(n, 3, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 4: k_bj_apply<TB, 4><<<nblk, 256, 0, s>>>(n, 4, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        default: k_bj_apply<TB, 8><<<nblk, 256, 0, s>>>(n, nr, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
         }
-    });
+    }
 }
 void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const int *blk_m, const int *cl_elem,
               const long long *boff, const void *minv, int mixed, const void *x, void *y)
