@@ -30,6 +30,10 @@ roofline     = the kernel with the largest share of the served run's timed kerne
                calibrated on a copy kernel).
 cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) on one
                frequency of one source of the same mesh, on one host core.
+
+--workload batched measures the batched uneven-problems configuration instead: 256 seeded
+small meshes (500-6000 triangles) x 8 frequencies x 16 plane-wave load cases, dense BEM,
+all steps submitted at once; its roofline is the matrix-core multi-load-case apply.
 """
 import argparse
 import json
@@ -62,24 +66,24 @@ def freq_spec(src, f):
     return dict(k=omega / C, nrhs=len(src), point_sources=ps)
 
 
-def cpu_baseline(mesh, grid, src, f, timeout):
+def cpu_baseline(mesh, grid, src, f, timeout, method_bem=4, what=None):
     exe = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
     if not os.path.exists(exe):
         return None
     d = tempfile.mkdtemp(prefix="ncbase_")
     try:
         prj = P.Project(mesh=mesh, freqs=[f], eval_grid=grid, point_sources=[(*src[0], 1.0, 0.0)],
-                        method_bem=4, c=C, rho=RHO, title="cpu baseline")
+                        method_bem=method_bem, c=C, rho=RHO, title="cpu baseline")
         P.write_project(d, prj)
         t0 = time.perf_counter()
         r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
         wall = time.perf_counter() - t0
         if r.returncode != 0:
             return None
+        what = what or ("reference NumCalc (ML-FMM, methodBEM 4, ILU-CGS) on the same %d-element mesh, one source, "
+                        "one frequency (%%.0f Hz) incl. the 1730-point grid" % len(mesh.elems))
         return dict(value=1.0 / wall, unit="frequency-solves/s", cores=1, kind="reference",
-                    sample="reference NumCalc (ML-FMM, methodBEM 4, ILU-CGS) on the same 20480-element mesh, one "
-                           "source, one frequency (%.0f Hz) incl. the 1730-point grid: %.1f s wall on one core"
-                           % (f, wall))
+                    sample=(what % f) + ": %.1f s wall on one core" % wall)
     except subprocess.TimeoutExpired:
         return None
     finally:
@@ -128,6 +132,134 @@ def pick(freqs, count, offset=0):
     return [freqs[int(offset + i * stride) % len(freqs)] for i in range(count)]
 
 
+def batched_problems(count, seed=0):
+    """Batched uneven dense problems: seeded ellipsoids and perturbed spheres, radii
+    0.05-0.12 m, 500-6000 triangles log-uniform, each with 8 frequencies uniform in
+    0.2-8 kHz; 16 plane-wave load cases with directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for i in range(count):
+        fr = max(5, int(round(np.sqrt(np.exp(rng.uniform(np.log(500.0), np.log(6000.0))) / 20.0))))
+        radius = rng.uniform(0.05, 0.12)
+        if i % 2 == 0:
+            axes, rms = tuple(radius * rng.uniform(0.7, 1.0, 3)), 0.0
+        else:
+            axes, rms = (radius, radius, radius), 0.03 * radius
+        meshes.append((P.head_like_mesh(freq=fr, seed=1000 + i, semi_axes=axes, rms=rms),
+                       list(rng.uniform(200.0, 8000.0, 8))))
+    dirs = rng.standard_normal((16, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    return meshes, dirs
+
+
+def plane_wave_spec(dirs, f):
+    omega = 2 * np.pi * f
+    amp = 1.0 / (1j * RHO * omega)
+    return dict(k=omega / C, nrhs=len(dirs),
+                plane_waves=[[d[0], d[1], d[2], amp.real, amp.imag, r] for r, d in enumerate(dirs)])
+
+
+def batched_main(a):
+    """Batched uneven problems: every (mesh, frequency) step of the batch submitted at once to
+    the serving runtime; dense BEM (methodBEM 0), mixed form, the 16 load cases applied on
+    the matrix cores."""
+    meshes, dirs = batched_problems(a.meshes)
+    ns = [len(m.elems) for m, _ in meshes]
+    print("# batched: %d meshes (n %d..%d, mean %d), 8 frequencies x %d plane waves each; lanes=%d" % (
+        len(meshes), min(ns), max(ns), int(np.mean(ns)), len(dirs), a.lanes), file=sys.stderr, flush=True)
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True)
+    ids = []
+    for m, _ in meshes:
+        n = len(m.elems)
+        ids.append(srv.add_mesh(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems,
+                                method_bem=0))
+    for j in [srv.submit(ids[i], [plane_wave_spec(dirs, meshes[i][1][0])], **SERVED) for i in range(min(4, len(ids)))]:
+        j.wait()
+    srv.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    jobs = [srv.submit(mid, [plane_wave_spec(dirs, f) for f in fq], **SERVED) for mid, (_, fq) in zip(ids, meshes)]
+    sts = [j.wait() for j in jobs]
+    elapsed = time.perf_counter() - t0
+    kst = srv.kernel_stats()
+    info = srv.stats()
+    srv.close()
+    steps = sum(len(fq) for _, fq in meshes)
+    print("# batched: %d steps in %.2f s" % (steps, elapsed), file=sys.stderr, flush=True)
+    # every a.check-th mesh: its last frequency against the FP64 plain method (dense CGS)
+    errs, bad = [], 0
+    for i in range(0, len(meshes), max(1, a.check)):
+        m, fq = meshes[i]
+        n = len(m.elems)
+        prob = ncg.Problem(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems,
+                           method_bem=0)
+        sp = plane_wave_spec(dirs, fq[-1])
+        prob.assemble(sp["k"], nrhs=sp["nrhs"], plane_waves=sp["plane_waves"])
+        prob.solve(solver=ncg.CGS, tolerance=1e-9)
+        x0 = prob.solution()
+        prob.close()
+        e = max(np.linalg.norm(jobs[i].solution[-1][r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(len(dirs)))
+        errs.append(e)
+        bad += e > 1e-3
+    failed = sum(s["state"] != "done" for s in sts)
+    valid = 0 if (bad or failed) else steps * len(dirs)
+    # the matrix-core apply on one problem of about 3000 unknowns with nothing else on the GPU
+    iso = None
+    k_iso = min(range(len(meshes)), key=lambda i: abs(ns[i] - 3000))
+    m, fq = meshes[k_iso]
+    n = len(m.elems)
+    prob = ncg.Problem(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems, method_bem=0)
+    try:
+        sp = plane_wave_spec(dirs, fq[0])
+        prob.assemble(sp["k"], nrhs=sp["nrhs"], plane_waves=sp["plane_waves"], precision=ncg.MIXED)
+        prob.kernel_timing(True)
+        prob.kernel_stats(reset=True)
+        prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_tolerance=1e-5)
+        st = prob.kernel_stats().get("k_matvec_mfma<f32>")
+        if st and st["calls"]:
+            t = st["total_ms"] / st["calls"] * 1e-3
+            flops = 2.0 * n * (2 * n) * 32          # real GEMM n x 2n x 32 per pass of 16 load cases
+            ach = st["bytes"] / st["calls"] / t
+            iso = dict(n=n, mean_launch_ms=t * 1e3, launches=st["calls"], bytes_per_launch=st["bytes"] / st["calls"],
+                       achieved_gbs=ach / 1e9, tflops=flops / t / 1e12)
+    finally:
+        prob.close()
+    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
+    roof = None
+    if iso:
+        roof = {"bound": "hbm", "kernel": "k_matvec_mfma<f32>", "achieved": iso["achieved_gbs"],
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": iso["achieved_gbs"] * 1e9 / HBM_PEAK, "traffic": None,
+                "mfma": {"achieved": iso["tflops"], "peak": 157.3, "unit": "TFLOP/s", "frac": iso["tflops"] / 157.3,
+                         "dtype": "fp32 (v_mfma_f32_32x32x2f32)"},
+                "mean_launch_ms": iso["mean_launch_ms"], "launches": iso["launches"],
+                "bytes_per_launch": iso["bytes_per_launch"],
+                "workload": "one problem of %d unknowns, 16 load cases, mixed GMRES; HIP events around each launch "
+                            "(x^ build, matrix-core product, split sum)" % iso["n"],
+                "served_top_kernel": top,
+                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
+    cpu = None
+    if not a.no_cpu:
+        m, fq = meshes[k_iso]
+        cpu = cpu_baseline(m, None, np.zeros((1, 3)), fq[0], a.cpu_timeout, method_bem=0,
+                           what="reference NumCalc (dense TBEM, ILU-CGS) on one batched mesh of %d elements, one "
+                                "point-source load case, one frequency (%%.0f Hz)" % len(m.elems))
+    line = {
+        "metric": METRIC, "value": valid / elapsed, "unit": "frequency-solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": 4, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "batched uneven dense problems: %d meshes (%d-%d triangles), 8 frequencies in 0.2-8 kHz "
+                               "each, 16 plane-wave load cases" % (len(meshes), min(ns), max(ns)),
+                   "form": "served mixed precision, %d lanes, dense BEM, GMRES + block-Jacobi, matrix-core "
+                           "multi-load-case apply" % a.lanes},
+        "p50_ms": float(np.percentile([s["latency_ms"] for s in sts], 50)),
+        "p99_ms": float(np.percentile([s["latency_ms"] for s in sts], 99)),
+        "valid_solves": valid, "solves": steps * len(dirs),
+        "mixed_vs_fp64_rel_l2": {"max": float(max(errs)) if errs else None, "meshes_checked": len(errs)},
+        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done")},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,11 +274,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
+    ap.add_argument("--workload", choices=("head", "batched"), default="head",
+                    help="head: the served head-mesh sweep (default); batched: many small dense problems with 16 "
+                         "plane-wave load cases each")
+    ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
+    ap.add_argument("--check", type=int, default=16, help="batched: every check-th mesh is verified against FP64")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the solve shards by independent (frequency, source) jobs with no "
               "communication, so only --gpus 1 is run here", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batched":
+        return batched_main(a)
 
     freqs = P.config2_freqs()
     mesh = P.config2_mesh(a.nu)

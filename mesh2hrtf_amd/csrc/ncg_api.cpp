@@ -337,7 +337,7 @@ struct ncg_problem {
     int *dbj_elem_blk = nullptr, *dbj_b0 = nullptr, *dbj_m = nullptr, *dbj_ident = nullptr;
     long long *dbj_boff = nullptr, dbj_total = 0;
     Grow dbj_blk;
-    // dense FP32 operator on the matrix cores: X^ operand and column-split partial sums
+    // dense operator on the matrix cores: X^ operand and column-split partial sums
     Grow mm_xh, mm_part;
     int *active = nullptr;
     double *basis = nullptr;
@@ -425,8 +425,8 @@ double elapsed(const EventPair &ev)
 const char *const kKernelNames[] = {"k_near_spmv<f64>",   "k_near_spmv<f32>",   "k_fmm_apply_T<f64>",
                                     "k_fmm_apply_T<f32>", "k_fmm_apply_D<f64>", "k_fmm_apply_D<f32>",
                                     "k_fmm_apply_S<f64>", "k_fmm_apply_S<f32>", "k_matvec<f64>",
-                                    "k_matvec<f32>"};
-constexpr int kNumKernels = 10;
+                                    "k_matvec<f32>",      "k_matvec_mfma<f32>", "k_matvec_mfma64"};
+constexpr int kNumKernels = 12;
 
 struct KernelScope {
     ncg_problem *p;
@@ -450,7 +450,6 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
     const double n = p->n, nr = p->nrhs, chunks = std::ceil(nr / 4.0);
     if (p->plan.method == 0) {
         const int mixed = p->precision == NCG_MIXED;
-        KernelScope ks(p, 8 + mixed, n * n * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
         // more than 4 load cases: the FP32 operator goes through the matrix cores, up to 16
         // load cases per read of A (the vector kernel reads A once per 4)
         // (FP32 accumulation for the mixed form's Krylov products, FP64 for its refinement
@@ -462,13 +461,17 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
             else ncgk::matvec_mfma_scratch(p->n, p->nrhs, &xf, &pf);
         }
         const size_t el = fp64acc ? sizeof(double) : sizeof(float);
+        // algorithmic bytes: A once per pass of (4 vector / 16 matrix-core) load cases, x and y
+        const double vec = 2.0 * nr * n * 16.0, abytes = n * n * (mixed ? 8.0 : 16.0);
         if (xf && !p->mm_xh.ensure(el * xf) && !p->mm_part.ensure(el * pf)) {
+            KernelScope ks(p, fp64acc ? 11 : 10, std::ceil(nr / 16.0) * abytes + vec);
             if (fp64acc)
                 ncgk::matvec_mfma64(p->stream, p->A, mixed, p->n, p->nrhs, x, y, (double *)p->mm_xh.p,
                                     (double *)p->mm_part.p);
             else
                 ncgk::matvec_mfma(p->stream, p->A, p->n, p->nrhs, x, y, (float *)p->mm_xh.p, (float *)p->mm_part.p);
         } else {
+            KernelScope ks(p, 8 + mixed, chunks * abytes + vec);
             ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);
         }
     } else {
