@@ -55,6 +55,7 @@ METRIC = "frequency-solves/sec at rel-L2≤1e-3 vs FP64 oracle; p50/p99 solve la
 C, RHO = 343.0, 1.1839
 HBM_PEAK = 8.0e12
 TRAFFIC_PROFILE = "k_near_spmv_head20480.txt"
+TRAFFIC_PROFILE_BATCHED = "k_matvec_mfma_batched.txt"
 SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50, max_iterations=500,
               refinement_steps=8, refinement_tolerance=1e-5)
 
@@ -107,18 +108,18 @@ def isolated_kernels(geo, src, f, launches=20):
         prob.close()
 
 
-def profiled_traffic(kernel):
-    """measured HBM bytes per launch of `kernel` in the 8000 Hz single-problem timing, from
-    the committed rocprofv3 counter summary of this benchmark (a line 'traffic <kernel>
+def profiled_traffic(kernel, profile=TRAFFIC_PROFILE):
+    """measured HBM bytes per launch of `kernel` in the single-problem timing, from the
+    committed rocprofv3 counter summary of this benchmark (a line 'traffic <kernel>
     <bytes>'), or (None, None)"""
-    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE)
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
         for line in fh:
             w = line.split()
             if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
-                return float(w[2]), "profiles/" + TRAFFIC_PROFILE
+                return float(w[2]), "profiles/" + profile
     return None, None
 
 
@@ -226,14 +227,16 @@ def batched_main(a):
     top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
     roof = None
     if iso:
+        traffic, tsrc = profiled_traffic("k_matvec_mfma<f32>", TRAFFIC_PROFILE_BATCHED)
         roof = {"bound": "hbm", "kernel": "k_matvec_mfma<f32>", "achieved": iso["achieved_gbs"],
-                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": iso["achieved_gbs"] * 1e9 / HBM_PEAK, "traffic": None,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": iso["achieved_gbs"] * 1e9 / HBM_PEAK,
+                "traffic": traffic, "traffic_source": tsrc,
                 "mfma": {"achieved": iso["tflops"], "peak": 157.3, "unit": "TFLOP/s", "frac": iso["tflops"] / 157.3,
                          "dtype": "fp32 (v_mfma_f32_32x32x2f32)"},
                 "mean_launch_ms": iso["mean_launch_ms"], "launches": iso["launches"],
                 "bytes_per_launch": iso["bytes_per_launch"],
-                "workload": "one problem of %d unknowns, 16 load cases, mixed GMRES; HIP events around each launch "
-                            "(x^ build, matrix-core product, split sum)" % iso["n"],
+                "workload": "one problem of %d unknowns, 16 load cases, mixed GMRES; HIP events around each "
+                            "launch of the matrix-core product kernel" % iso["n"],
                 "served_top_kernel": top,
                 "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
     cpu = None

@@ -454,22 +454,22 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         // load cases per read of A (the vector kernel reads A once per 4)
         // (FP32 accumulation for the mixed form's Krylov products, FP64 for its refinement
         // residual and for the FP64 operator)
-        size_t xf = 0, pf = 0;
         const bool fp64acc = !mixed || acc64;
-        if (!d && p->nrhs > 4) {
-            if (fp64acc) ncgk::matvec_mfma64_scratch(p->n, &xf, &pf);
-            else ncgk::matvec_mfma_scratch(p->n, p->nrhs, &xf, &pf);
-        }
         const size_t el = fp64acc ? sizeof(double) : sizeof(float);
         // algorithmic bytes: A once per pass of (4 vector / 16 matrix-core) load cases, x and y
         const double vec = 2.0 * nr * n * 16.0, abytes = n * n * (mixed ? 8.0 : 16.0);
-        if (xf && !p->mm_xh.ensure(el * xf) && !p->mm_part.ensure(el * pf)) {
-            KernelScope ks(p, fp64acc ? 11 : 10, std::ceil(nr / 16.0) * abytes + vec);
-            if (fp64acc)
-                ncgk::matvec_mfma64(p->stream, p->A, mixed, p->n, p->nrhs, x, y, (double *)p->mm_xh.p,
-                                    (double *)p->mm_part.p);
-            else
-                ncgk::matvec_mfma(p->stream, p->A, p->n, p->nrhs, x, y, (float *)p->mm_xh.p, (float *)p->mm_part.p);
+        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, fp64acc);
+        if (!d && p->nrhs > 4 && !p->mm_xh.ensure(el * mp.xh_elems) && !p->mm_part.ensure(el * mp.part_elems)) {
+            for (int r0 = 0; r0 < p->nrhs; r0 += 16) {
+                const int nq = std::min(16, p->nrhs - r0);
+                ncgk::mfma_xhat(p->stream, mp, p->n, nq, x + 2 * (size_t)r0 * p->n, p->mm_xh.p);
+                {
+                    // the product kernel alone: A, this pass's x and y
+                    KernelScope ks(p, fp64acc ? 11 : 10, abytes + 2.0 * nq * n * 16.0);
+                    ncgk::mfma_product(p->stream, mp, p->A, mixed, p->n, p->mm_xh.p, p->mm_part.p);
+                }
+                ncgk::mfma_sum(p->stream, mp, p->n, nq, p->mm_part.p, y + 2 * (size_t)r0 * p->n);
+            }
         } else {
             KernelScope ks(p, 8 + mixed, chunks * abytes + vec);
             ncgk::matvec(p->stream, p->A, mixed, acc64, p->n, p->nrhs, x, d, y);

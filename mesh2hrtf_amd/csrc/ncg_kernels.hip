@@ -2491,58 +2491,47 @@ __global__ void __launch_bounds__(256) k_mfma64_reduce(int n, int nr, int splits
     y[(size_t)r * n + i] = mk(re, im);
 }
 
-int matvec_mfma64_scratch(int n, size_t *xh_doubles, size_t *part_doubles)
+MfmaPlan mfma_plan(int n, int f64)
 {
-    const int tiles = (n + kMfma64Rows - 1) / kMfma64Rows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
-    int splits = std::max(1, std::min(std::min(16, (1024 + tiles - 1) / tiles), (nchunks + 3) / 4));
-    *xh_doubles = (size_t)2 * n * 32;
-    *part_doubles = (size_t)splits * 32 * n;
-    return splits;
+    MfmaPlan m;
+    m.f64 = f64;
+    const int rows = f64 ? kMfma64Rows : kMfmaRows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
+    m.tiles = (n + rows - 1) / rows;
+    // enough work-groups to cover the device (1024 for the FP64 tiles of 64 rows, 512 for
+    // the FP32 ones of 128), each at least 4 column chunks
+    m.splits = std::max(1, std::min(std::min(16, ((f64 ? 1024 : 512) + m.tiles - 1) / m.tiles), (nchunks + 3) / 4));
+    m.per = (nchunks + m.splits - 1) / m.splits;
+    m.xh_elems = (size_t)2 * n * 32;
+    m.part_elems = (size_t)m.splits * 32 * n;
+    return m;
 }
 
-void matvec_mfma64(hipStream_t s, const void *A, int a_f32, int n, int nrhs, const void *x, void *y, double *xh,
-                   double *part)
+void mfma_xhat(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *x, void *xh)
 {
-    size_t xd, pd;
-    const int splits = matvec_mfma64_scratch(n, &xd, &pd);
-    const int tiles = (n + kMfma64Rows - 1) / kMfma64Rows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
-    const int per = (nchunks + splits - 1) / splits;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = std::min(16, nrhs - r0);
-        k_xhat64<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x + (size_t)r0 * n, xh);
-        if (a_f32)
-            k_matvec_mfma64<float><<<dim3(tiles, splits), 256, 0, s>>>((const cx<float> *)A, n, per, xh, part);
-        else
-            k_matvec_mfma64<double><<<dim3(tiles, splits), 256, 0, s>>>((const cx<double> *)A, n, per, xh, part);
-        k_mfma64_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, splits, part,
-                                                                        (cx<double> *)y + (size_t)r0 * n);
-    }
+    if (m.f64) k_xhat64<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x, (double *)xh);
+    else k_xhat<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x, (float *)xh);
 }
 
-int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats)
+void mfma_product(hipStream_t s, const MfmaPlan &m, const void *A, int a_f32, int n, const void *xh, void *part)
 {
-    const int tiles = (n + kMfmaRows - 1) / kMfmaRows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
-    // enough work-groups to cover the device twice, each at least 4 column chunks
-    int splits = std::max(1, std::min(std::min(16, (512 + tiles - 1) / tiles), (nchunks + 3) / 4));
-    *xh_floats = (size_t)2 * n * 32;
-    *part_floats = (size_t)splits * 32 * n;
-    (void)nrhs;
-    return splits;
+    const dim3 grid(m.tiles, m.splits);
+    if (!m.f64)
+        k_matvec_mfma<<<grid, 256, 0, s>>>((const cx<float> *)A, n, m.per, (const float *)xh, (float *)part);
+    else if (a_f32)
+        k_matvec_mfma64<float><<<grid, 256, 0, s>>>((const cx<float> *)A, n, m.per, (const double *)xh, (double *)part);
+    else
+        k_matvec_mfma64<double><<<grid, 256, 0, s>>>((const cx<double> *)A, n, m.per, (const double *)xh,
+                                                     (double *)part);
 }
 
-void matvec_mfma(hipStream_t s, const void *A, int n, int nrhs, const void *x, void *y, float *xh, float *part)
+void mfma_sum(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *part, void *y)
 {
-    size_t xf, pf;
-    const int splits = matvec_mfma_scratch(n, nrhs, &xf, &pf);
-    const int tiles = (n + kMfmaRows - 1) / kMfmaRows, nchunks = (n + kMfmaCols - 1) / kMfmaCols;
-    const int per = (nchunks + splits - 1) / splits;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = std::min(16, nrhs - r0);
-        k_xhat<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x + (size_t)r0 * n, xh);
-        k_matvec_mfma<<<dim3(tiles, splits), 256, 0, s>>>((const cx<float> *)A, n, per, xh, part);
-        k_mfma_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, splits, part,
-                                                                      (cx<double> *)y + (size_t)r0 * n);
-    }
+    if (m.f64)
+        k_mfma64_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, m.splits, (const double *)part,
+                                                                        (cx<double> *)y);
+    else
+        k_mfma_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, m.splits, (const float *)part,
+                                                                      (cx<double> *)y);
 }
 
 int dot_blocks(int n) { return std::min(kDotBlocks, (int)blocks_for(n, 256)); }

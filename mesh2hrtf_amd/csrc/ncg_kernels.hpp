@@ -45,14 +45,17 @@ void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
                    void *A, int mixed, void *rhs, const long long *diag_index = nullptr);
 
-// FP32 dense operator applied to many load cases on the matrix cores (FP32 accumulation,
-// up to 16 load cases per pass); scratch sizes from matvec_mfma_scratch
-int matvec_mfma_scratch(int n, int nrhs, size_t *xh_floats, size_t *part_floats);
-void matvec_mfma(hipStream_t s, const void *A, int n, int nrhs, const void *x, void *y, float *xh, float *part);
-// the same with FP64 accumulation, A in FP32 (a_f32) or FP64
-int matvec_mfma64_scratch(int n, size_t *xh_doubles, size_t *part_doubles);
-void matvec_mfma64(hipStream_t s, const void *A, int a_f32, int n, int nrhs, const void *x, void *y, double *xh,
-                   double *part);
+// Dense operator applied to up to 16 load cases per pass on the matrix cores: x^ (the load
+// cases as a real 2n x 32 operand), the product (column splits of partial sums) and the sum
+// of the splits.  f64: FP64 accumulation (A in FP32 or FP64), else FP32 (A in FP32).
+struct MfmaPlan {
+    int f64 = 0, tiles = 0, splits = 0, per = 0;
+    size_t xh_elems = 0, part_elems = 0;     // scratch, in elements of the accumulation type
+};
+MfmaPlan mfma_plan(int n, int f64);
+void mfma_xhat(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *x, void *xh);
+void mfma_product(hipStream_t s, const MfmaPlan &m, const void *A, int a_f32, int n, const void *xh, void *part);
+void mfma_sum(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *part, void *y);
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d,
             void *y);
 int dot_blocks(int n);
