@@ -233,6 +233,33 @@ def test_dense_many_load_cases_mixed(ncg):
             assert R.rel_l2(x32[r], x64[r]) < 1e-4, r
 
 
+def test_dense_twenty_load_cases(ncg):
+    """20 load cases on a small dense operator (n = 320, less than one 128-row tile of the
+    matrix cores in places): two passes (16 + 4) of the matrix-core applies, FP64 and mixed,
+    against one-load-case solves."""
+    case = R.load_case("soft_pw")
+    pa = R.problem_arrays(case, case["spec"]["freqs"][-1])
+    rng = np.random.default_rng(11)
+    dirs = rng.standard_normal((20, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    pw = [list(dirs[r]) + [1e-3, 2e-4 * r, r] for r in range(20)]
+    bc = np.repeat(pa["bc"][:, 0][None, :], 20, axis=0)
+    prob = gpu_problem(ncg, case, pa)
+    prob.assemble(pa["k"], bc=bc, adm=pa["adm"], plane_waves=pw, precision=ncg.FP64)
+    prob.solve(solver=ncg.CGS, tolerance=1e-11)
+    X = prob.solution()
+    prob.assemble(pa["k"], bc=bc, adm=pa["adm"], plane_waves=pw, precision=ncg.MIXED)
+    st = prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_tolerance=1e-8)
+    assert st["converged"]
+    Xm = prob.solution()
+    for r in (0, 7, 15, 16, 19):
+        prob.assemble(pa["k"], bc=bc[r:r + 1], adm=pa["adm"], plane_waves=[pw[r][:5] + [0]], precision=ncg.FP64)
+        prob.solve(solver=ncg.CGS, tolerance=1e-11)
+        x1 = prob.solution()[0]
+        assert R.rel_l2(X[r], x1) < 1e-8, r
+        assert R.rel_l2(Xm[r], x1) < 1e-4, r
+
+
 def test_apply_matches_system(ncg):
     case = R.load_case("quad")
     pa = R.problem_arrays(case, 2100.0)
