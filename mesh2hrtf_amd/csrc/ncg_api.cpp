@@ -264,6 +264,10 @@ struct DevIlu {
     int nnz = 0, nlevL = 0, nlevU = 0;
     int *rp = nullptr, *col = nullptr, *diag = nullptr;
     int *levL_ptr = nullptr, *levL_rows = nullptr, *levU_ptr = nullptr, *levU_rows = nullptr;
+    // batches of consecutive levels for the triangular solves
+    int nbatL = 0, nbatU = 0;
+    int *batL_ptr = nullptr, *batL_m = nullptr, *batU_ptr = nullptr, *batU_m = nullptr;
+    unsigned char *subL = nullptr, *subU = nullptr;
     double *val = nullptr;
     // level-ordered factors for the triangular solves: offsets, columns, CSR positions
     int *Loff = nullptr, *Lc = nullptr, *Lmap = nullptr, *Uoff = nullptr, *Uc = nullptr, *Umap = nullptr,
@@ -1166,10 +1170,59 @@ int ilu_setup(ncg_problem *p, int degree)
             Umapd[(size_t)u * B + q] = Umap[Uoff[u] + q];
         }
     }
+    // Batches: consecutive levels of at most 64 rows in all, as long as every entry of a
+    // row whose column lies in the batch is among its first kIluB (those come from the
+    // lanes of the batch); a level wider than 64 rows is a batch of its own.  In-batch
+    // columns of the padded copy become -1 - (position - batch start).
+    auto batches = [&](const std::vector<int> &lp, int nlev, const std::vector<int> &rows, const std::vector<int> &lv,
+                       const std::vector<int> &off, const std::vector<int> &cl, std::vector<int> &cdp,
+                       std::vector<int> &bptr, std::vector<int> &bm, std::vector<unsigned char> &sub) {
+        std::vector<int> pos(n);
+        for (int u = 0; u < n; ++u) pos[rows[u]] = u;
+        bptr.assign(1, 0);
+        bm.clear();
+        sub.assign(n, 0);
+        auto fits = [&](int l, int l0) {   // rows of level l reach columns of levels >= l0 only early
+            for (int u = lp[l]; u < lp[l + 1]; ++u)
+                for (int k = off[u] + B; k < off[u + 1]; ++k)
+                    if (lv[cl[k]] >= l0) return false;
+            return true;
+        };
+        for (int l0 = 0; l0 < nlev;) {
+            int l1 = l0 + 1, tot = lp[l0 + 1] - lp[l0];
+            if (tot <= 64)
+                while (l1 < nlev && l1 - l0 < 255 && tot + (lp[l1 + 1] - lp[l1]) <= 64 && fits(l1, l0)) {
+                    tot += lp[l1 + 1] - lp[l1];
+                    ++l1;
+                }
+            const int b0 = lp[l0];
+            for (int u = b0; u < lp[l1]; ++u) {
+                sub[u] = (unsigned char)(lv[rows[u]] - l0);
+                for (int q = 0; q < B && off[u] + q < off[u + 1]; ++q) {
+                    const int j = cl[off[u] + q];
+                    if (tot <= 64 && lv[j] >= l0) cdp[(size_t)u * B + q] = -1 - (pos[j] - b0);
+                }
+            }
+            bptr.push_back(lp[l1]);
+            bm.push_back(l1 - l0);
+            l0 = l1;
+        }
+    };
+    std::vector<int> bpL, bmL, bpU, bmU;
+    std::vector<unsigned char> sbL, sbU;
+    batches(lpL, nL, rowsL, lvL, Loff, Lc, Lcd, bpL, bmL, sbL);
+    batches(lpU, nU, rowsU, lvU, Uoff, Uc, Ucd, bpU, bmU, sbU);
+    I.nbatL = (int)bmL.size();
+    I.nbatU = (int)bmU.size();
     if ((rc = upload_ilu(I, s, &I.Lcd, Lcd)) || (rc = upload_ilu(I, s, &I.Lmapd, Lmapd)) ||
         (rc = upload_ilu(I, s, &I.Ucd, Ucd)) || (rc = upload_ilu(I, s, &I.Umapd, Umapd)) ||
+        (rc = upload_ilu(I, s, &I.batL_ptr, bpL)) || (rc = upload_ilu(I, s, &I.batL_m, bmL)) ||
+        (rc = upload_ilu(I, s, &I.batU_ptr, bpU)) || (rc = upload_ilu(I, s, &I.batU_m, bmU)) ||
+        (rc = I.alloc(&I.subL, (size_t)n)) || (rc = I.alloc(&I.subU, (size_t)n)) ||
         (rc = I.alloc(&I.Lvd, 2 * (size_t)n * B)) || (rc = I.alloc(&I.Uvd, 2 * (size_t)n * B)))
         return rc;
+    NCG_HIP(hipMemcpyAsync(I.subL, sbL.data(), (size_t)n, hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.subU, sbU.data(), (size_t)n, hipMemcpyHostToDevice, s));
     if ((rc = upload_ilu(I, s, &I.Loff, Loff)) || (rc = upload_ilu(I, s, &I.Lc, Lc)) ||
         (rc = upload_ilu(I, s, &I.Lmap, Lmap)) || (rc = upload_ilu(I, s, &I.Uoff, Uoff)) ||
         (rc = upload_ilu(I, s, &I.Uc, Uc)) || (rc = upload_ilu(I, s, &I.Umap, Umap)) ||
@@ -1192,16 +1245,20 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
 {
     const DevIlu &I = p->ilu;
     ncgk::IluSweep L, U;
-    L.nlev = I.nlevL;
-    L.lev_ptr = I.levL_ptr;
+    L.nbat = I.nbatL;
+    L.bat_ptr = I.batL_ptr;
+    L.bat_m = I.batL_m;
+    L.sub = I.subL;
     L.row = I.levL_rows;
     L.off = I.Loff;
     L.c = I.Lc;
     L.v = I.Lv;
     L.cd = I.Lcd;
     L.vd = I.Lvd;
-    U.nlev = I.nlevU;
-    U.lev_ptr = I.levU_ptr;
+    U.nbat = I.nbatU;
+    U.bat_ptr = I.batU_ptr;
+    U.bat_m = I.batU_m;
+    U.sub = I.subU;
     U.row = I.levU_rows;
     U.off = I.Uoff;
     U.c = I.Uc;

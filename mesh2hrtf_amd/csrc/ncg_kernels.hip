@@ -843,27 +843,30 @@ __device__ __forceinline__ cx<double> ilu_row(cx<double> s, int k0, int k1, cons
 
 // One row of a triangular sweep with everything that does not depend on the solution
 // loaded ahead: entry range, the first kIluB columns and values (from the padded copies
-// cd/vd, [position][kIluB]), the right-hand side (forward sweep), the diagonal (backward).
+// cd/vd, [position][kIluB]; a negative column -1 - l names lane l of the same batch), the
+// sub-step of its level inside the batch, the right-hand side (forward sweep), the
+// diagonal (backward).
 constexpr int kIluB = ncgk::kIluB;
 struct IluItem {
-    int i, k0, k1;
+    int i, k0, k1, ss;
     bool on;
     int cc[kIluB];
     cx<double> vv[kIluB];
     cx<double> xv, dv;
 };
 
-// lane's row i of the level [b, b + w) (w <= 64, i already loaded); every load here
+// lane's row i of the batch [b, b + w) (w <= 64, i already loaded); every load here
 // depends only on the position and i.  xr (forward) or d (backward) may be null.
 __device__ __forceinline__ void ilu_item_load(IluItem &it, int lane, int b, int w, int i, const int *off,
-                                              const int *cd, const cx<double> *vd, const cx<double> *xr,
-                                              const cx<double> *d)
+                                              const unsigned char *sub, const int *cd, const cx<double> *vd,
+                                              const cx<double> *xr, const cx<double> *d)
 {
     it.on = lane < w;
     const int u = b + (it.on ? lane : 0);
     it.i = i;
     it.k0 = off[u];
     it.k1 = off[u + 1];
+    it.ss = sub[u];
 #pragma unroll
     for (int q = 0; q < kIluB; ++q) {
         it.cc[q] = cd[(size_t)u * kIluB + q];
@@ -873,54 +876,67 @@ __device__ __forceinline__ void ilu_item_load(IluItem &it, int lane, int b, int 
     it.dv = d ? d[u] : mk(1.0, 0.0);
 }
 
-// s minus the item's row sum, in the entries' order
-__device__ __forceinline__ cx<double> ilu_item_sum(const IluItem &it, cx<double> s, const int *c, const cx<double> *v,
-                                                   const cx<double> *yr)
-{
-    cx<double> yy[kIluB];
-#pragma unroll
-    for (int q = 0; q < kIluB; ++q) yy[q] = yr[it.cc[q]];
-#pragma unroll
-    for (int q = 0; q < kIluB; ++q)
-        if (it.k0 + q < it.k1) s -= it.vv[q] * yy[q];
-    if (it.k1 - it.k0 > kIluB) s = ilu_row(s, it.k0 + kIluB, it.k1, c, v, yr);
-    return s;
-}
-
-// One triangular sweep of one load case by one wave.  A wave's memory operations complete
-// in program order and its lanes share one vector L1, so a row stored at one level is
-// seen by every lane at the next with no barrier (a wavefront-scope fence keeps the
-// compiler from moving loads above the stores).  The rows of level l + 2 and everything
-// else of level l + 1 are loaded while level l is computed, so a level costs one round of
-// solution gathers.  Levels wider than the wave take a plain loop.
+// One triangular sweep of one load case by one wave, a batch of consecutive levels (at
+// most 64 rows) per step.  A batch's solution values from earlier batches are gathered
+// once; its rows then run level by level (sub-steps), taking the values of the batch's
+// earlier levels from the slots their lanes left in LDS, so a batch costs one round of
+// memory gathers however many levels it holds.  Every row sums its entries in
+// the CSR order.  A wave's memory operations complete in program order and its lanes
+// share one vector L1, so rows stored by one batch are seen by the next with no barrier (a
+// wavefront-scope fence keeps the compiler from moving loads above the stores).  The
+// next batch's rows and coefficients are loaded while the current one is computed.
+// Levels wider than the wave form batches of their own and take a plain loop.
 template <bool UPPER>
-__device__ __forceinline__ void ilu_sweep(int nlev, const int *lev_ptr, const int *row, const int *off, const int *c,
-                                          const cx<double> *v, const int *cd, const cx<double> *vd,
+__device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<double> *v, const cx<double> *vd,
                                           const cx<double> *d, const cx<double> *xr, cx<double> *yr)
 {
-    const int lane = threadIdx.x;
-    auto width = [&](int l) { return lev_ptr[l + 1] - lev_ptr[l]; };
-    auto row_of = [&](int l) { return l < nlev && lane < width(l) ? row[lev_ptr[l] + lane] : 0; };
+    __shared__ cx<double> ys[64];   // the batch's rows solved so far, by lane
+    const int lane = threadIdx.x, nb = S.nbat;
+    const int *bp = S.bat_ptr, *row = S.row, *off = S.off, *c = S.c, *cd = S.cd;
+    auto width = [&](int b) { return bp[b + 1] - bp[b]; };
+    auto row_of = [&](int b) { return b < nb && lane < width(b) ? row[bp[b] + lane] : 0; };
     IluItem cur, nxt;
     cur.on = false;
     int rnext = row_of(1);
-    if (nlev > 0 && width(0) <= 64)
-        ilu_item_load(cur, lane, lev_ptr[0], width(0), row_of(0), off, cd, vd, UPPER ? nullptr : xr, UPPER ? d : nullptr);
-    for (int l = 0; l < nlev; ++l) {
-        const int b = lev_ptr[l], w = lev_ptr[l + 1] - b;
+    if (nb > 0 && width(0) <= 64)
+        ilu_item_load(cur, lane, bp[0], width(0), row_of(0), off, S.sub, cd, vd, UPPER ? nullptr : xr,
+                      UPPER ? d : nullptr);
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = bp[b], w = bp[b + 1] - b0;
         nxt.on = false;
-        const int rafter = row_of(l + 2);
-        if (l + 1 < nlev && width(l + 1) <= 64)
-            ilu_item_load(nxt, lane, lev_ptr[l + 1], width(l + 1), rnext, off, cd, vd, UPPER ? nullptr : xr,
+        const int rafter = row_of(b + 2);
+        if (b + 1 < nb && width(b + 1) <= 64)
+            ilu_item_load(nxt, lane, bp[b + 1], width(b + 1), rnext, off, S.sub, cd, vd, UPPER ? nullptr : xr,
                           UPPER ? d : nullptr);
         if (w <= 64) {
-            if (cur.on) {
-                if (UPPER) yr[cur.i] = cdiv(ilu_item_sum(cur, yr[cur.i], c, v, yr), cur.dv);
-                else yr[cur.i] = ilu_item_sum(cur, cur.xv, c, v, yr);
+            // values from earlier batches (in-batch slots read a dummy, replaced below)
+            cx<double> yy[kIluB];
+#pragma unroll
+            for (int q = 0; q < kIluB; ++q) yy[q] = yr[cur.cc[q] >= 0 ? cur.cc[q] : 0];
+            const cx<double> s0 = UPPER ? yr[cur.i] : cur.xv;
+            const int m = S.bat_m[b];
+            for (int st = 0; st < m; ++st) {
+                const bool mine = cur.on && cur.ss == st;
+                // values of the batch's earlier levels from the lanes' slots in LDS, for the
+                // entries some row of this level takes from the batch
+                if (st > 0)
+#pragma unroll
+                    for (int q = 0; q < kIluB; ++q)
+                        if (__any(mine && cur.cc[q] < 0) && mine && cur.cc[q] < 0) yy[q] = ys[-1 - cur.cc[q]];
+                if (mine) {
+                    cx<double> acc = s0;
+#pragma unroll
+                    for (int q = 0; q < kIluB; ++q)
+                        if (cur.k0 + q < cur.k1) acc -= cur.vv[q] * yy[q];
+                    if (cur.k1 - cur.k0 > kIluB) acc = ilu_row(acc, cur.k0 + kIluB, cur.k1, c, v, yr);
+                    if (UPPER) acc = cdiv(acc, cur.dv);
+                    ys[lane] = acc;
+                    yr[cur.i] = acc;
+                }
             }
         } else {
             for (int t = lane; t < w; t += 64) {
-                const int u = b + t, i = row[u];
+                const int u = b0 + t, i = row[u];
                 if (UPPER) yr[i] = cdiv(ilu_row(yr[i], off[u], off[u + 1], c, v, yr), d[u]);
                 else yr[i] = ilu_row(xr[i], off[u], off[u + 1], c, v, yr);
             }
@@ -938,10 +954,8 @@ __device__ __forceinline__ void ilu_sweep(int nlev, const int *lev_ptr, const in
 __global__ void __launch_bounds__(64) k_ilu_solve(int n, ncgk::IluSweep L, ncgk::IluSweep U, const cx<double> *x, cx<double> *y)
 {
     const size_t r = blockIdx.x;
-    ilu_sweep<false>(L.nlev, L.lev_ptr, L.row, L.off, L.c, (const cx<double> *)L.v, L.cd, (const cx<double> *)L.vd,
-                     nullptr, x + r * n, y + r * n);
-    ilu_sweep<true>(U.nlev, U.lev_ptr, U.row, U.off, U.c, (const cx<double> *)U.v, U.cd, (const cx<double> *)U.vd,
-                    (const cx<double> *)U.d, nullptr, y + r * n);
+    ilu_sweep<false>(L, (const cx<double> *)L.v, (const cx<double> *)L.vd, nullptr, x + r * n, y + r * n);
+    ilu_sweep<true>(U, (const cx<double> *)U.v, (const cx<double> *)U.vd, (const cx<double> *)U.d, nullptr, y + r * n);
 }
 
 // ======================================================================================

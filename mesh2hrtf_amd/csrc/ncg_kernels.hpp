@@ -95,9 +95,15 @@ void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows
 // row row[u] with entries c/v[off[u] .. off[u+1]) (CSR order), the first kIluB of them
 // also padded in cd/vd[u * kIluB + q]; d = U's diagonal per position (backward sweep)
 constexpr int kIluB = 8;
+// One triangular sweep's schedule: positions u = 0..n-1 (row row[u]) in batches of
+// consecutive levels, bat_ptr[b]..bat_ptr[b+1] (at most 64 positions, or one wider level),
+// bat_m[b] levels per batch and sub[u] the level of u inside its batch.  The first kIluB
+// entries of every position are also in cd/vd[u][kIluB], a column of the same batch
+// written as -1 - (its position - bat_ptr[b]).
 struct IluSweep {
-    int nlev = 0;
-    const int *lev_ptr = nullptr, *row = nullptr, *off = nullptr, *c = nullptr, *cd = nullptr;
+    int nbat = 0;
+    const int *bat_ptr = nullptr, *bat_m = nullptr, *row = nullptr, *off = nullptr, *c = nullptr, *cd = nullptr;
+    const unsigned char *sub = nullptr;
     const void *v = nullptr, *vd = nullptr, *d = nullptr;
 };
 // out[t] = val[map[t]] (0 where map[t] < 0)
