@@ -781,25 +781,58 @@ __global__ void k_ilu_fill_csr(int n, const long long *ptr, const int *ncol, con
 __global__ void __launch_bounds__(1024) k_ilu_factor(int nlev, const int *lev_ptr, const int *lev_rows, const int *rp,
                                                      const int *col, const int *diag, cx<double> *val, int *err)
 {
+    // a wave per row of the level, a lane per entry of the row, held in registers while
+    // the row's lower entries k (in order) eliminate: l_ik = a_ik / u_kk, then
+    // a_ij -= l_ik u_kj for the later entries j of the row found in row k's upper part
+    constexpr int NW = 1024 / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int l = 0; l < nlev; ++l) {
-        for (int t = lev_ptr[l] + threadIdx.x; t < lev_ptr[l + 1]; t += blockDim.x) {
-            const int i = lev_rows[t];
-            const int end = rp[i + 1];
-            for (int p = rp[i]; p < diag[i]; ++p) {
-                const int k = col[p];
-                const cx<double> piv = val[diag[k]];
-                if (piv.re == 0.0 && piv.im == 0.0) {
-                    atomicOr(err, 8);
-                    continue;
+        for (int t = lev_ptr[l] + wv; t < lev_ptr[l + 1]; t += NW) {
+            const int i = lev_rows[t], r0 = rp[i], r1 = rp[i + 1], dg = diag[i];
+            if (r1 - r0 <= 64) {
+                const int p2 = r0 + lane;
+                const bool on = p2 < r1;
+                cx<double> v = on ? val[p2] : mk(0.0, 0.0);
+                const int j = on ? col[p2] : -1;
+                for (int p = r0; p < dg; ++p) {
+                    const int k = col[p];
+                    const cx<double> piv = val[diag[k]];
+                    if (piv.re == 0.0 && piv.im == 0.0) {
+                        if (lane == 0) atomicOr(err, 8);
+                        continue;
+                    }
+                    const cx<double> lik = cdiv(mk(__shfl(v.re, p - r0), __shfl(v.im, p - r0)), piv);
+                    if (lane == p - r0) v = lik;
+                    if (on && p2 > p) {
+                        const int qe = rp[k + 1];
+                        for (int q = diag[k] + 1; q < qe; ++q) {
+                            const int cq = col[q];
+                            if (cq >= j) {
+                                if (cq == j) v -= lik * val[q];
+                                break;
+                            }
+                        }
+                    }
                 }
-                const cx<double> lik = cdiv(val[p], piv);
-                val[p] = lik;
-                int q = diag[k] + 1;
-                const int qe = rp[k + 1];
-                for (int p2 = p + 1; p2 < end && q < qe; ++p2) {
-                    const int j = col[p2];
-                    while (q < qe && col[q] < j) ++q;
-                    if (q < qe && col[q] == j) val[p2] -= lik * val[q];
+                if (on) val[p2] = v;
+            } else if (lane == 0) {
+                // rows longer than a wave: the entries one after the other
+                for (int p = r0; p < dg; ++p) {
+                    const int k = col[p];
+                    const cx<double> piv = val[diag[k]];
+                    if (piv.re == 0.0 && piv.im == 0.0) {
+                        atomicOr(err, 8);
+                        continue;
+                    }
+                    const cx<double> lik = cdiv(val[p], piv);
+                    val[p] = lik;
+                    int q = diag[k] + 1;
+                    const int qe = rp[k + 1];
+                    for (int p2 = p + 1; p2 < r1 && q < qe; ++p2) {
+                        const int jj = col[p2];
+                        while (q < qe && col[q] < jj) ++q;
+                        if (q < qe && col[q] == jj) val[p2] -= lik * val[q];
+                    }
                 }
             }
         }
