@@ -1665,16 +1665,20 @@ int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st
         const double *u[1] = {Rr}, *v[1] = {Rr};
         if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
         bool ok = true;
+        double worst = 0.0;
         for (int r = 0; r < nr; ++r) {
             rel[r] = bnorm[r] > 0 ? sqrt(dv[3 * r].real()) / bnorm[r] : 0.0;
             if (rel[r] >= target) ok = false;
+            worst = std::max(worst, rel[r]);
         }
         if (ok) break;
-        // inner solve A d = r in FP32 matvecs to ~1e-5 relative (FP32 operator accuracy)
+        // inner solve A d = r in FP32 matvecs, relative to r: what the worst load case still
+        // needs (with a factor 2 to spare), at most 1/2 and not below 1e-5 (the FP32
+        // operator's accuracy)
         NCG_HIP(hipMemsetAsync(Dd, 0, bytes, p->stream));
         std::vector<int> it;
         std::vector<double> rr;
-        const double inner = std::max(1e-5, target * 0.5);
+        const double inner = std::min(0.5, std::max(1e-5, 0.5 * target / worst));
         if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, bj, true, it, rr)))
             return rc;
         for (int r = 0; r < nr; ++r) total[r] += it[r];
