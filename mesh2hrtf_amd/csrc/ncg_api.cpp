@@ -1312,6 +1312,8 @@ int bj_prepare(ncg_problem *p)
     if (F.bj_ready) return 0;
     const DevLevel &D = F.lev.back();
     int rc;
+    // one flag for both: the near values are FP32 exactly in the mixed form, and bj_setup
+    // reads them and writes the inverses in that precision
     F.bj_f32 = p->precision == NCG_MIXED;
     if ((rc = F.bj_blk.ensure((F.bj_f32 ? 8 : 16) * (size_t)F.bj_total))) return rc;
     ncgk::bj_setup(p->stream, p->n, F.nblk, F.elem_blk, D.elem_pos, F.blk_cl, F.blk_b0, F.blk_m, D.cl_ptr, F.near_ptr,
