@@ -31,7 +31,9 @@ roofline     = the kernel with the largest share of the served run's timed kerne
 cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) on one
                frequency of one source of the same mesh, on one host core.
 
---workload batched measures the batched uneven-problems configuration instead: 256 seeded
+--workload graded runs the same measurement on the graded large case (60320 triangles, edges
+1 mm near the ears to 8 mm on the far sides, 40 frequencies 12-23.7 kHz, roofline launch at
+18 kHz).  --workload batched measures the batched uneven-problems configuration instead: 256 seeded
 small meshes (500-6000 triangles) x 8 frequencies x 16 plane-wave load cases, dense BEM,
 all steps submitted at once; its roofline is the matrix-core multi-load-case apply.
 """
@@ -277,9 +279,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
-    ap.add_argument("--workload", choices=("head", "batched"), default="head",
-                    help="head: the served head-mesh sweep (default); batched: many small dense problems with 16 "
-                         "plane-wave load cases each")
+    ap.add_argument("--workload", choices=("head", "graded", "batched"), default="head",
+                    help="head: the served head-mesh sweep (default); graded: the 60k-triangle graded head mesh "
+                         "over 12-24 kHz; batched: many small dense problems with 16 plane-wave load cases each")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--check", type=int, default=16, help="batched: every check-th mesh is verified against FP64")
     a = ap.parse_args()
@@ -290,8 +292,10 @@ def main():
     if a.workload == "batched":
         return batched_main(a)
 
-    freqs = P.config2_freqs()
-    mesh = P.config2_mesh(a.nu)
+    if a.workload == "graded":
+        freqs, mesh, iso_f = P.config3_freqs(), P.graded_head_mesh(), 18000.0
+    else:
+        freqs, mesh, iso_f = P.config2_freqs(), P.config2_mesh(a.nu), 8000.0
     grid = P.fibonacci_grid(1730, 1.2)
     src = P.ear_sources(mesh)
     n, m, nr = len(mesh.elems), len(grid.nodes), len(src)
@@ -379,7 +383,7 @@ def main():
     value = valid / elapsed
 
     # the dominant kernel again on one problem with nothing else on the GPU
-    iso = {} if a.no_isolated else isolated_kernels(geo, src, 8000.0)
+    iso = {} if a.no_isolated else isolated_kernels(geo, src, iso_f)
 
     # roofline of the dominant kernel of the served run, timed on one problem with nothing
     # else on the GPU (launch times under the lanes' sharing are kept as `served`)
@@ -399,10 +403,10 @@ def main():
             ach = d["bytes"] / d["total_ms"] * 1e3
             roof.update({"achieved": ach / 1e9, "frac": ach / HBM_PEAK, "bytes_per_launch": d["bytes"] / d["calls"],
                          "mean_launch_ms": d["total_ms"] / d["calls"], "launches": d["calls"],
-                         "workload": "one problem of the same mesh at 8000 Hz (two-level ML-FMM), 2 load cases, "
-                                     "nothing else on the GPU; HIP events around each launch"})
+                         "workload": "one problem of the same mesh at %.0f Hz, 2 load cases, nothing else on the "
+                                     "GPU; HIP events around each launch" % iso_f})
             roof["traffic"], roof["traffic_source"] = profiled_traffic(top)
-    cpu = None if a.no_cpu else cpu_baseline(mesh, grid, src, 8000.0, a.cpu_timeout)
+    cpu = None if a.no_cpu else cpu_baseline(mesh, grid, src, iso_f, a.cpu_timeout)
     line = {
         "metric": METRIC,
         "value": value,
@@ -416,8 +420,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "head-like mesh %d triangles (seed 0), 2 ear point sources, 1730-point grid at 1.2 m, "
-                               "frequencies 200-16000 Hz step 200; NumCalc methodBEM 4 (SL/ML-FMM)" % n,
+        "config": {"workload": ("head-like mesh %d triangles (seed 0), 2 ear point sources, 1730-point grid at 1.2 m, "
+                                "frequencies 200-16000 Hz step 200; NumCalc methodBEM 4 (SL/ML-FMM)" % n)
+                   if a.workload == "head" else
+                   ("graded head-like mesh %d triangles (seed 0; 1 mm edges near the ears to 8 mm), 2 ear point "
+                    "sources, 1730-point grid at 1.2 m, frequencies 12000-23700 Hz step 300; NumCalc methodBEM 4" % n),
                    "form": "served mixed precision, %d lanes, GMRES + block-Jacobi" % a.lanes,
                    "frequencies": len(timed)},
         "p50_ms": float(np.percentile(pl, 50)),

@@ -880,6 +880,21 @@ int fmm_structure(ncg_problem *p)
     return 0;
 }
 
+// max over n <= L of (2n+1) max(|y_n(x)|, 1/x): a bound on the largest term of a translation
+// series at k d = x (|h_n| ~ |y_n| once n > x, ~ 1/x below), y_n by upward recurrence
+double translation_peak(int L, double x)
+{
+    double y0 = -cos(x) / x, y1 = -cos(x) / (x * x) - sin(x) / x;
+    double m = std::max(std::max(std::fabs(y0), 1.0 / x), 3.0 * std::max(std::fabs(y1), 1.0 / x));
+    for (int q = 1; q < L; ++q) {
+        const double y2 = (2.0 * q + 1.0) / x * y1 - y0;
+        y0 = y1;
+        y1 = y2;
+        m = std::max(m, (2.0 * q + 3.0) * std::max(std::fabs(y1), 1.0 / x));
+    }
+    return m;
+}
+
 // Build and fill the device FMM operator of the current plan (NC_BuildSystemFMBEM,
 // src/NC_EquationSystem.cpp:305-365, with the CSR layouts of NC_AllocateNmtx and
 // NC_AllocateSDTmtxs*, src/NC_Addresses.cpp:1476-1937).  The structure is rebuilt only
@@ -904,10 +919,16 @@ int assemble_fmm(ncg_problem *p)
         DevLevel &D = F.lev[l];
         D.P = L.P;
         D.L = L.L;
-        // When k times the shortest interaction distance is small, the translations sum
-        // large Hankel values against O(1) aggregation and disaggregation entries with
-        // cancellation: such a level keeps T, D and S in FP64 in the mixed form.
-        D.f32 = mixed && p->k * D.dmin >= 1.0;
+        // The translation series sum (2n+1) h_n(k d) terms up to n = L against O(1)
+        // aggregation and disaggregation entries; where they are large they cancel, and FP32
+        // storage of T, D and S (with FP32 signatures) then breaks the refinement (the inner
+        // FP32 operator no longer reduces the FP64 residual).  A level keeps FP64 in the
+        // mixed form unless k d_min >= 1 and peak * k d_min <= 2e8, peak the series' largest
+        // term bound at d_min: the largest value that converged (a 20480-element head at
+        // 16 kHz, 1.2e8) against the smallest that did not (a graded 60320-element head at
+        // 18 kHz, 2.8e9).
+        const double kd = p->k * D.dmin;
+        D.f32 = mixed && kd >= 1.0 && translation_peak(D.L, kd) * kd <= 2e8;
         const size_t sf = D.f32 ? 8 : 16;
         if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(sf * (size_t)n * D.P)) ||
             (rc = D.D.ensure(sf * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(sf * (size_t)n * D.P)) ||

@@ -156,6 +156,39 @@ def head_like_mesh(freq: int = 32, seed: int = 0, semi_axes=(0.075, 0.09, 0.11),
     return Mesh(nodes, _orient_outward(nodes, g.elems))
 
 
+def graded_head_mesh(seed: int = 0, semi_axes=(0.075, 0.09, 0.11), lmax: int = 8, rms: float = 3e-3,
+                     base: int = 14, bands=(0.03, 0.12, 0.135)) -> Mesh:
+    """Head-like surface (the head_like_mesh surface of the same seed) meshed graded towards
+    the ears: the nodes of the geodesic spheres of frequency base * 2^l (l = 3, 2, 1) whose
+    great-circle distance (on the 0.09 m sphere) to the nearer of the +-y ear directions is
+    below bands[0], bands[1], bands[2], plus every node of frequency base, triangulated as
+    their convex hull on the unit sphere and mapped radially onto the surface.  Defaults:
+    60320 triangles, edges about 1 mm within 30 mm of the ears, 2 mm to 120 mm, 4 mm to
+    135 mm and 8 mm on the far sides."""
+    from scipy.spatial import ConvexHull
+    ears = np.array([[0.0, 1.0, 0.0], [0.0, -1.0, 0.0]])
+    pts = [geodesic_sphere(base).nodes]
+    for level, band in zip((3, 2, 1), bands):
+        u = geodesic_sphere(base * 2 ** level).nodes
+        d = 0.09 * np.arccos(np.clip(np.max(u @ ears.T, axis=1), -1.0, 1.0))
+        pts.append(u[d < band])
+    u = np.unique(np.round(np.concatenate(pts), 12), axis=0)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    tris = ConvexHull(u).simplices.astype(np.int64)
+    rng = np.random.default_rng(seed)
+    coef = rng.standard_normal((lmax + 1) ** 2)
+    coef *= rms / np.sqrt(np.sum(coef ** 2) / (4.0 * np.pi))
+    a, b, c = semi_axes
+    r_ell = 1.0 / np.sqrt((u[:, 0] / a) ** 2 + (u[:, 1] / b) ** 2 + (u[:, 2] / c) ** 2)
+    nodes = u * (r_ell + _real_sph_harm_matrix(lmax, u) @ coef)[:, None]
+    return Mesh(nodes, _orient_outward(nodes, tris))
+
+
+def config3_freqs():
+    """12000..23700 Hz step 300 Hz (40 frequencies)."""
+    return [12000.0 + 300.0 * i for i in range(40)]
+
+
 def ear_sources(mesh: Mesh, offset: float = 1e-3):
     """Two point sources ``offset`` outside the surface nodes nearest the +y and -y axes."""
     u = mesh.nodes / np.linalg.norm(mesh.nodes, axis=1, keepdims=True)
