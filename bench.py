@@ -265,6 +265,99 @@ def batched_main(a):
     print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
 
 
+def stream_main(a):
+    """Serving stream: 32 seeded head-like meshes (5k-40k triangles, log-uniform), each with
+    the 1730-point grid; requests = (mesh, contiguous band of 1-50 frequencies in 100 Hz
+    steps within 100 Hz-20 kHz, both ear sources).  The capacity comes from a batch of
+    requests submitted at once; the latency from Poisson arrivals at 70% of it."""
+    rng = np.random.default_rng(0)
+    meshes = []
+    for i in range(a.stream_meshes):
+        fr = int(round(np.sqrt(np.exp(rng.uniform(np.log(5000.0), np.log(40000.0))) / 20.0)))
+        meshes.append(P.head_like_mesh(freq=fr, seed=i))
+    grid = P.fibonacci_grid(1730, 1.2)
+    srcs = [P.ear_sources(m) for m in meshes]
+    ns = [len(m.elems) for m in meshes]
+    reqs = []
+    for _ in range(a.requests):
+        i = int(rng.integers(0, len(meshes)))
+        nf = int(rng.integers(1, 51))
+        f0 = 100.0 * int(rng.integers(1, 201 - nf))
+        reqs.append((i, [f0 + 100.0 * q for q in range(nf)]))
+    print("# stream: %d meshes (n %d..%d), %d requests of 1-50 frequencies, lanes=%d" % (
+        len(meshes), min(ns), max(ns), len(reqs), a.lanes), file=sys.stderr, flush=True)
+    srv = ncg.Server(device=0, lanes=a.lanes)
+    ids = [srv.add_mesh(np.full(len(m.elems), 3), m.nodes[m.elems], np.zeros(len(m.elems), np.int32), grid.nodes,
+                        node_index=m.elems, method_bem=4) for m in meshes]
+    srv.submit(ids[reqs[0][0]], [freq_spec(srcs[reqs[0][0]], reqs[0][1][0])], **SERVED).wait()
+    # capacity: the first quarter of the requests at once
+    ncap = max(1, len(reqs) // 4)
+    t0 = time.perf_counter()
+    jobs = [srv.submit(ids[i], [freq_spec(srcs[i], f) for f in fq], **SERVED) for i, fq in reqs[:ncap]]
+    sts = [j.wait() for j in jobs]
+    cap_el = time.perf_counter() - t0
+    cap_solves = sum(len(fq) for _, fq in reqs[:ncap]) * 2
+    capacity = cap_solves / cap_el
+    failed = sum(s["state"] != "done" for s in sts)
+    # Poisson arrivals of the other requests at 70% of the capacity
+    rest = reqs[ncap:]
+    mean_solves = np.mean([len(fq) * 2 for _, fq in rest]) if rest else 1.0
+    rate = 0.7 * capacity / mean_solves
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(1.0 / rate, len(rest)))
+    pj = []
+    t0 = time.perf_counter()
+    for t_arr, (i, fq) in zip(arrivals, rest):
+        dt = t_arr - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        pj.append((i, fq, srv.submit(ids[i], [freq_spec(srcs[i], f) for f in fq], **SERVED)))
+    pst = [j.wait() for _, _, j in pj]
+    p_el = time.perf_counter() - t0
+    failed += sum(s["state"] != "done" for s in pst)
+    for (i, fq), st in list(zip(reqs[:ncap], sts)) + [((i, fq), st) for (i, fq, _), st in zip(pj, pst)]:
+        if st["state"] != "done":
+            print("# failed request: mesh %d (n %d), %.0f-%.0f Hz, after %d frequencies: %s" % (
+                i, ns[i], fq[0], fq[-1], st["freq_done"], st["error"]), file=sys.stderr, flush=True)
+    lat = [s["latency_ms"] for s in pst] or [float("nan")]
+    info = srv.stats()
+    srv.close()
+    print("# stream: capacity %.1f solves/s (%d solves in %.2f s); Poisson %.2f req/s: p50 %.0f ms p99 %.0f ms" % (
+        capacity, cap_solves, cap_el, rate, np.percentile(lat, 50), np.percentile(lat, 99)), file=sys.stderr, flush=True)
+    # a sample of the Poisson requests against the FP64 plain method at their first frequency
+    errs = []
+    for i, fq, j in pj[:: max(1, len(pj) // a.stream_checks)][:a.stream_checks]:
+        m = meshes[i]
+        prob = ncg.Problem(np.full(len(m.elems), 3), m.nodes[m.elems], np.zeros(len(m.elems), np.int32), grid.nodes,
+                           node_index=m.elems, method_bem=4)
+        sp = freq_spec(srcs[i], fq[0])
+        prob.assemble(sp["k"], nrhs=2, point_sources=sp["point_sources"])
+        prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+        x0 = prob.solution()
+        p0, _ = prob.field(gradient=False)
+        prob.close()
+        errs.append(max(max(np.linalg.norm(j.solution[0][r] - x0[r]) / np.linalg.norm(x0[r]),
+                            np.linalg.norm(j.phi[0][r] - p0[r]) / np.linalg.norm(p0[r])) for r in range(2)))
+    valid = not failed and all(e <= 1e-3 for e in errs)
+    line = {
+        "metric": METRIC, "value": capacity if valid else 0.0, "unit": "frequency-solves/s", "n_gpus": 1,
+        "steps": sum(len(fq) for _, fq in reqs), "warmup": 1, "ms_per_step": cap_el / (cap_solves / 2) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "serving stream: %d head-like meshes (%d-%d triangles) with the 1730-point grid, "
+                               "requests of 1-50 frequencies in 100 Hz-20 kHz x 2 ear sources" % (
+                                   len(meshes), min(ns), max(ns)),
+                   "form": "served mixed precision, %d lanes, GMRES + block-Jacobi" % a.lanes},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "latency": {"poisson_rate_per_s": rate, "requests": len(rest), "offered_solves_per_s": rate * mean_solves,
+                    "served_solves_per_s": sum(len(fq) * 2 for _, fq in rest) / p_el},
+        "capacity": {"requests": ncap, "solves": cap_solves, "seconds": cap_el},
+        "mixed_vs_fp64_rel_l2": {"max": float(max(errs)) if errs else None, "requests_checked": len(errs)},
+        "failed_requests": int(failed),
+        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done")},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,9 +372,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
-    ap.add_argument("--workload", choices=("head", "graded", "batched"), default="head",
+    ap.add_argument("--stream-meshes", type=int, default=32, help="stream: number of meshes")
+    ap.add_argument("--requests", type=int, default=96, help="stream: number of requests")
+    ap.add_argument("--stream-checks", type=int, default=4, help="stream: requests verified against FP64")
+    ap.add_argument("--workload", choices=("head", "graded", "batched", "stream"), default="head",
                     help="head: the served head-mesh sweep (default); graded: the 60k-triangle graded head mesh "
-                         "over 12-24 kHz; batched: many small dense problems with 16 plane-wave load cases each")
+                         "over 12-24 kHz; batched: many small dense problems with 16 plane-wave load cases each; "
+                         "stream: Poisson requests over 32 head-like meshes of 5k-40k triangles")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--check", type=int, default=16, help="batched: every check-th mesh is verified against FP64")
     a = ap.parse_args()
@@ -291,6 +388,8 @@ def main():
         sys.exit(2)
     if a.workload == "batched":
         return batched_main(a)
+    if a.workload == "stream":
+        return stream_main(a)
 
     if a.workload == "graded":
         freqs, mesh, iso_f = P.config3_freqs(), P.graded_head_mesh(), 18000.0

@@ -923,12 +923,12 @@ int assemble_fmm(ncg_problem *p)
         // aggregation and disaggregation entries; where they are large they cancel, and FP32
         // storage of T, D and S (with FP32 signatures) then breaks the refinement (the inner
         // FP32 operator no longer reduces the FP64 residual).  A level keeps FP64 in the
-        // mixed form unless k d_min >= 1 and peak * k d_min <= 2e8, peak the series' largest
-        // term bound at d_min: the largest value that converged (a 20480-element head at
-        // 16 kHz, 1.2e8) against the smallest that did not (a graded 60320-element head at
-        // 18 kHz, 2.8e9).
+        // mixed form unless k d_min >= 1 and peak * k d_min <= 2e7, peak the series' largest
+        // term bound at d_min.  Measured on head-like meshes: up to 7e7 the refinement
+        // converged in one step, 1.2e8 took two (5120 elements at 12.8 kHz), 2.2e8 and more
+        // did not converge (5120 elements at 12.4 kHz; 60320 graded elements at 18 kHz, 2.8e9).
         const double kd = p->k * D.dmin;
-        D.f32 = mixed && kd >= 1.0 && translation_peak(D.L, kd) * kd <= 2e8;
+        D.f32 = mixed && kd >= 1.0 && translation_peak(D.L, kd) * kd <= 2e7;
         const size_t sf = D.f32 ? 8 : 16;
         if ((rc = upload_grow(D.u, s, L.u)) || (rc = upload_grow(D.w, s, L.w)) || (rc = D.T.ensure(sf * (size_t)n * D.P)) ||
             (rc = D.D.ensure(sf * (size_t)std::max(D.nfan, 1) * D.P)) || (rc = D.S.ensure(sf * (size_t)n * D.P)) ||
