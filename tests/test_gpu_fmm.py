@@ -179,3 +179,30 @@ def test_graded_large_mesh_mixed_vs_fp64(ncg, f):
         assert R.rel_l2(x32[r], x64[r]) < 1e-4
         assert R.rel_l2(p32[r], p64[r]) < 1e-4
     prob.close()
+
+
+@pytest.mark.parametrize("f", [12400.0, 16000.0, 20000.0])
+def test_coarse_mesh_high_frequency_mixed_converges(ncg, f):
+    """A coarse head-like mesh (5120 elements, about 4 per wavelength at 12 kHz) high in
+    frequency: the mixed form keeps the levels whose translation series would cancel in
+    FP32 in FP64, and its refinement reaches 1e-5 within 1e-4 of the FP64 solve."""
+    from mesh2hrtf_amd import project as P
+    mesh = P.head_like_mesh(freq=16, seed=2)
+    src = P.ear_sources(mesh)
+    n = len(mesh.elems)
+    omega = 2 * np.pi * f
+    amp = 1.0 / (1j * 1.1839 * omega)
+    ps = [[s[0], s[1], s[2], amp.real, amp.imag, r] for r, s in enumerate(src)]
+    prob = ncg.Problem(np.full(n, 3), mesh.nodes[mesh.elems], np.zeros(n, np.int32), None, node_index=mesh.elems,
+                       method_bem=4)
+    print("coarse mesh f", f, flush=True)
+    prob.assemble(omega / 343.0, nrhs=2, point_sources=ps, precision=ncg.FP64)
+    assert prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, tolerance=1e-10)["converged"]
+    x64 = prob.solution()
+    prob.assemble(omega / 343.0, nrhs=2, point_sources=ps, precision=ncg.MIXED)
+    st = prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_steps=8, refinement_tolerance=1e-5)
+    assert st["converged"] and st["refinement_steps"] <= 2, st
+    x32 = prob.solution()
+    for r in range(2):
+        assert R.rel_l2(x32[r], x64[r]) < 1e-4
+    prob.close()
