@@ -1819,24 +1819,34 @@ __device__ __forceinline__ void wave_sum4(double (&re)[4][NR], double (&im)[4][N
             }
 }
 
+// values of four rows for one batch of U steps of 64 columns from b + lane (indices past sl
+// clamped to sl - 1)
+template <typename S, int U>
+__device__ __forceinline__ void rows4_load(const cx<S> *const (&vr)[4], int b, int sl, int lane, cx<S> (&v)[U][4])
+{
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+        const int u = min(b + 64 * q + lane, sl - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[q][j] = vr[j][u];
+    }
+}
+
 // re/im[j][r] += this lane's share (u = lane + 64 t < sl) of sum_u v_j[u] * xs[r][u] for
-// four rows v_j and NR vectors in LDS (row stride seg).  U steps per batch with every
-// load issued first; indices past sl are clamped and weighted by zero.
+// four rows v_j and NR vectors in LDS (row stride seg).  v holds the first batch on entry
+// (rows4_load at b = 0); each batch's successor is loaded before the batch is used, so
+// two batches of loads are in flight.  Indices past sl are clamped and weighted by zero.
 template <typename S, int NR, int U>
 __device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, const cx<double> *xs, int seg, int lane,
-                                          double (&re)[4][NR], double (&im)[4][NR])
+                                          cx<S> (&v)[U][4], double (&re)[4][NR], double (&im)[4][NR])
 {
-    for (int u0 = lane; u0 < sl; u0 += 64 * U) {
-        cx<S> v[U][4];
+    for (int b = 0; b < sl; b += 64 * U) {   // wave-uniform trip count
+        cx<S> nv[U][4];
+        const bool more = b + 64 * U < sl;
+        if (more) rows4_load<S, U>(vr, b + 64 * U, sl, lane, nv);
 #pragma unroll
         for (int q = 0; q < U; ++q) {
-            const int u = min(u0 + 64 * q, sl - 1);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[q][j] = vr[j][u];
-        }
-#pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int u = u0 + 64 * q;
+            const int u = b + 64 * q + lane;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 cx<double> xv = xs[r * seg + min(u, sl - 1)];
@@ -1845,6 +1855,11 @@ __device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, c
                 for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[q][j].re), double(v[q][j].im), xv);
             }
         }
+        if (more)
+#pragma unroll
+            for (int q = 0; q < U; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[q][j] = nv[q][j];
     }
 }
 
@@ -1878,6 +1893,11 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch
         for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
     for (int s0 = 0; s0 < w; s0 += seg) {
         const int sl = min(seg, w - s0);
+        // unused row slots read the first row of the cluster and are not stored; the first
+        // batch of values is requested before the gather so the two latencies overlap
+        const cx<S> *vs[4] = {vr[0] + s0, vr[1] + s0, vr[2] + s0, vr[3] + s0};
+        cx<S> v[2][4];
+        rows4_load<S, 2>(vs, 0, sl, lane, v);
         __syncthreads();
         for (int u = threadIdx.x; u < sl; u += 256) {
             const int e = col[cb + s0 + u];
@@ -1885,9 +1905,7 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch
             for (int r = 0; r < NR; ++r) xs[r * seg + u] = x[(size_t)r * n + e];
         }
         __syncthreads();
-        // unused row slots read the first row of the cluster and are not stored
-        const cx<S> *vs[4] = {vr[0] + s0, vr[1] + s0, vr[2] + s0, vr[3] + s0};
-        rows4_dot<S, NR, 2>(vs, sl, xs, seg, lane, re, im);
+        rows4_dot<S, NR, 2>(vs, sl, xs, seg, lane, v, re, im);
     }
     wave_sum4<NR>(re, im);
     if (lane == 0)
