@@ -194,6 +194,21 @@ struct DevFmm {
     long long *near_ptr = nullptr, *diag_pos = nullptr;
     int *near_col = nullptr;
     int near_wmax = 0;                 // longest near-field row
+    // near-field chunk table (ncgk::NearChunks): per chunk the column base and width, per
+    // row slot the row and its value offset
+    long long *nd_cb = nullptr, *nd_off = nullptr;
+    int *nd_w = nullptr, *nd_row = nullptr;
+    int nd_n = 0;
+    ncgk::NearChunks near_chunks() const
+    {
+        ncgk::NearChunks r;
+        r.n = nd_n;
+        r.cb = nd_cb;
+        r.off = nd_off;
+        r.w = nd_w;
+        r.row = nd_row;
+        return r;
+    }
     int *elem_cl_fine = nullptr;
     unsigned char *cl_near = nullptr;
     int ncl_fine = 0;
@@ -489,10 +504,9 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         }
         {
             const double nnz = (double)p->fmm.nnz;
-            const DevLevel &DF = p->fmm.lev.back();
             KernelScope ks(p, mixed, chunks * nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * n * 16.0);
-            ncgk::near_spmv(p->stream, p->n, p->nrhs, DF.chunks(), DF.cl_ptr, DF.cl_elem, p->fmm.near_ptr,
-                            p->fmm.near_col, p->fmm.near_wmax, p->fmm.near_val.p, mixed, xin, y);
+            ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_chunks(), p->fmm.near_col, p->fmm.near_wmax,
+                            p->fmm.near_val.p, mixed, xin, y);
         }
         for (DevLevel &L : p->fmm.lev) {
             // algorithmic bytes: the operator once per pass of at most 4 load cases, the
@@ -743,6 +757,7 @@ int fmm_structure(ncg_problem *p)
     DevFmm &F = p->fmm;
     const int n = p->n;
     int rc;
+    std::vector<int> fine_ch_r0, fine_ch_m, fine_ch_cl, fine_cl_elem;   // row chunks of the finest level
     for (const ncgf::Level &L : p->plan.lev) {
         DevLevel D;
         D.ncl = (int)L.cl.size();
@@ -801,6 +816,10 @@ int fmm_structure(ncg_problem *p)
             (rc = upload(F.st, s, &D.fan_idx, fan_idx)) || (rc = upload(F.st, s, &D.fan_row, fan_row)))
             return rc;
         F.lev.push_back(D);
+        fine_ch_cl.swap(ch_cl);
+        fine_ch_r0.swap(ch_r0);
+        fine_ch_m.swap(ch_m);
+        fine_cl_elem.swap(cl_elem);
     }
     // near field of the finest level: row i -> elements of the near clusters of its
     // cluster; row lengths are per-cluster sums, the columns are written on the device
@@ -821,6 +840,26 @@ int fmm_structure(ncg_problem *p)
     }
     std::vector<long long> ptr(n + 1, 0);
     for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + rowlen[cl_of[i]];
+    // chunk table of the near-field apply: the finest level's row chunks with their column
+    // base (the first row of the cluster), width, rows and value offsets
+    const int R = ncgk::kChunkRows, nchf = (int)fine_ch_cl.size();
+    std::vector<long long> nd_cb(nchf), nd_off((size_t)nchf * R);
+    std::vector<int> nd_w(nchf), nd_row((size_t)nchf * R, -1);
+    for (int q = 0; q < nchf; ++q) {
+        const int c = fine_ch_cl[q];
+        nd_cb[q] = ptr[FL.cl[c].elems[0]];
+        nd_w[q] = (int)rowlen[c];
+        for (int a = 0; a < R; ++a) {
+            const size_t t = (size_t)q * R + a;
+            if (a < fine_ch_m[q]) {
+                nd_row[t] = fine_cl_elem[fine_ch_r0[q] + a];
+                nd_off[t] = ptr[nd_row[t]];
+            } else {
+                nd_off[t] = nd_cb[q];   // an unused slot reads the first row and stores nothing
+            }
+        }
+    }
+    F.nd_n = nchf;
     // block-Jacobi layout: offset of the own cluster inside its rows; blocks are runs of
     // at most 64 consecutive elements of a cluster's list, of balanced sizes
     std::vector<int> self_off(ncf, 0), elem_blk(n, -1), blk_cl, blk_b0, blk_m;
@@ -852,6 +891,8 @@ int fmm_structure(ncg_problem *p)
     F.near_wmax = rowlen.empty() ? 0 : (int)*std::max_element(rowlen.begin(), rowlen.end());
     int *d_nn_ptr, *d_nn_idx;
     if ((rc = upload(F.st, s, &F.near_ptr, ptr)) || (rc = F.st.alloc(&F.diag_pos, n)) ||
+        (rc = upload(F.st, s, &F.nd_cb, nd_cb)) || (rc = upload(F.st, s, &F.nd_off, nd_off)) ||
+        (rc = upload(F.st, s, &F.nd_w, nd_w)) || (rc = upload(F.st, s, &F.nd_row, nd_row)) ||
         (rc = F.st.alloc(&F.near_col, F.nnz)) ||
         (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
         (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)) ||

@@ -1865,26 +1865,28 @@ __device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, c
 
 // near field: all rows of a finest cluster hold the same columns (the elements of its
 // near clusters), so a chunk gathers x over that column list into LDS (in segments of
-// seg columns) and the values are the only per-nonzero traffic.
+// seg columns) and the values are the only per-nonzero traffic.  The chunk's rows, value
+// offsets and column list come from one table, so the first loads of a work-group do not
+// depend on each other.
 template <typename S, int NR>
-__global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const int *ch_cl, const int *ch_r0, const int *ch_m,
-                                                   const int *cl_ptr, const int *cl_elem, const long long *ptr,
-                                                   const int *col, const cx<S> *val, const cx<double> *x, cx<double> *y)
+__global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long long *ch_cb, const int *ch_w,
+                                                   const int *ch_row, const long long *ch_off, const int *col,
+                                                   const cx<S> *val, const cx<double> *x, cx<double> *y)
 {
     extern __shared__ double xs_sh[];   // [NR][seg] complex
     cx<double> *xs = (cx<double> *)xs_sh;
-    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
+    const int q = blockIdx.x;
     // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long cb = ptr[cl_elem[cl_ptr[c]]];
-    const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
+    const long long cb = ch_cb[q];
+    const int w = ch_w[q];
     int row[4];
     const cx<S> *vr[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int a = wv + 4 * j;
-        row[j] = a < m ? cl_elem[r0 + a] : -1;
-        vr[j] = val + (row[j] >= 0 ? ptr[row[j]] : cb);
+        const int a = ncgk::kChunkRows * q + wv + 4 * j;
+        row[j] = ch_row[a];
+        vr[j] = val + ch_off[a];
     }
     double re[4][NR], im[4][NR];
 #pragma unroll
@@ -2926,26 +2928,23 @@ void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, 
 constexpr int kNearLdsBytes = 32768;
 
 template <typename S>
-static void near_spmv_t(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-                        const long long *ptr, const int *col, int wmax, const cx<S> *val, const cx<double> *x,
-                        cx<double> *y)
+static void near_spmv_t(hipStream_t s, int n, int nrhs, const NearChunks &nc, const int *col, int wmax,
+                        const cx<S> *val, const cx<double> *x, cx<double> *y)
 {
     rhs_chunks(nrhs, [&](int nr, int r0) {
         // LDS for the longest row (in 64-column steps), at most the budget: more chunks per CU
         const int seg = std::max(64, std::min(kNearLdsBytes / (16 * nr), (wmax + 63) / 64 * 64));
-        NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, rc.n, 256, 16 * seg, n, seg, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
-                           val, x + (size_t)r0 * n, y + (size_t)r0 * n);
+        NCG_NR_DISPATCH_SH(nr, k_near_spmv, S, nc.n, 256, 16 * seg, n, seg, nc.cb, nc.w, nc.row, nc.off, col, val,
+                           x + (size_t)r0 * n, y + (size_t)r0 * n);
     });
 }
-void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-               const long long *ptr, const int *col, int wmax, const void *val, int mixed, const void *x, void *y)
+void near_spmv(hipStream_t s, int n, int nrhs, const NearChunks &nc, const int *col, int wmax, const void *val,
+               int mixed, const void *x, void *y)
 {
     if (mixed)
-        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, wmax, (const cx<float> *)val, (const cx<double> *)x,
-                    (cx<double> *)y);
+        near_spmv_t(s, n, nrhs, nc, col, wmax, (const cx<float> *)val, (const cx<double> *)x, (cx<double> *)y);
     else
-        near_spmv_t(s, n, nrhs, rc, cl_ptr, cl_elem, ptr, col, wmax, (const cx<double> *)val, (const cx<double> *)x,
-                    (cx<double> *)y);
+        near_spmv_t(s, n, nrhs, nc, col, wmax, (const cx<double> *)val, (const cx<double> *)x, (cx<double> *)y);
 }
 template <typename S, typename W>
 static void apply_T_t(hipStream_t s, int n, int ncl, int P, int nrhs, const int *cl_ptr, const int *cl_elem,

@@ -157,9 +157,18 @@ struct RowChunks {
     int n = 0;
     const int *cl = nullptr, *r0 = nullptr, *m = nullptr;
 };
+// Near-field row chunks with their addresses resolved on the host: the rows of chunk q share
+// the column list col[cb[q] .. cb[q] + w[q]); row slot a < kChunkRows holds row
+// row[kChunkRows q + a] (-1: unused) whose values start at off[kChunkRows q + a] (cb[q]
+// for an unused slot).
+struct NearChunks {
+    int n = 0;
+    const long long *cb = nullptr, *off = nullptr;
+    const int *w = nullptr, *row = nullptr;
+};
 // near field of the finest level (CSR whose rows in one cluster share their columns)
-void near_spmv(hipStream_t s, int n, int nrhs, const RowChunks &rc, const int *cl_ptr, const int *cl_elem,
-               const long long *ptr, const int *col, int wmax, const void *val, int mixed, const void *x, void *y);
+void near_spmv(hipStream_t s, int n, int nrhs, const NearChunks &nc, const int *col, int wmax, const void *val,
+               int mixed, const void *x, void *y);
 // far field of one level: f32 selects FP32 T, D, S and FP32 storage of t and s
 // tw_f64: t is FP64 whatever f32 says (signatures for the evaluation grid and the
 // right-hand side's sources)
