@@ -1798,11 +1798,11 @@ __device__ __forceinline__ void waves_sum(double (&re)[NR], double (&im)[NR], do
     }
 }
 
-// Row chunks: the kernels below give each work-group a chunk of at most 16 rows that
-// share one cluster (ch_cl, ch_r0 = first position in the cluster's element list, ch_m
-// = rows).  Each of the 4 waves owns rows a = wave + 4 j and streams their values with
-// four independent loads per lane per step; the vector the rows multiply is staged in
-// LDS once per work-group.
+// Row chunks: the near-field apply below gives each work-group a chunk of at most 16 rows
+// that share one cluster (ncgk::NearChunks: the rows, their value offsets and the shared
+// column list).  Each of the 4 waves owns rows a = wave + 4 j and streams their values
+// with eight independent loads per lane per step; the vector the rows multiply is staged
+// in LDS once per work-group.
 static_assert(ncgk::kChunkRows == 16, "4 waves x 4 rows per chunk");
 
 // sum over the wave of re/im for four rows and NR load cases, then row a gets lane 0's value
