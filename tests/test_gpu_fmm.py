@@ -206,3 +206,39 @@ def test_coarse_mesh_high_frequency_mixed_converges(ncg, f):
     for r in range(2):
         assert R.rel_l2(x32[r], x64[r]) < 1e-4
     prob.close()
+
+
+# rel-L2 of the FMM operator against the dense one on these meshes: 1.9e-4 to 2.8e-4 with
+# NumCalc's expansion lengths, alike in FP64 and in the mixed form; a wrong row, column or
+# load-case mapping gives errors of order one
+FMM_APPLY_TOL = 1e-3
+
+
+@pytest.mark.parametrize("name", ["fmm_sl_point", "fmm_ml_head"])
+def test_fmm_apply_load_case_passes(ncg, name):
+    """The FMM operator applied to five vectors at once (passes of four and one load case;
+    at four, near-field rows wider than one LDS segment) equals single-vector applies in
+    both precisions, and stays within FMM accuracy of the dense operator (method 0)."""
+    case = R.load_case(name)
+    f = case["spec"]["freqs"][-1]
+    pa = R.problem_arrays(case, f)
+    n = pa["n"]
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    print("case", name, "f", f, "n", n, flush=True)
+    dense = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], node_index=case["elems"], method_bem=0)
+    dense.assemble(pa["k"], adm=pa["adm"])
+    A, _ = dense.system()
+    dense.close()
+    ref = x @ A.T
+    prob = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], node_index=case["elems"], method_bem=case["spec"]["method_bem"])
+    for prec in (ncg.FP64, ncg.MIXED):
+        prob.assemble(pa["k"], adm=pa["adm"], nrhs=5, precision=prec)
+        assert prob.fmm_info()["method"] in (1, 3)
+        y = prob.apply(x)
+        for r in range(5):
+            assert R.rel_l2(y[r], prob.apply(x[r:r + 1])[0]) < 1e-12, (prec, r)
+            err = R.rel_l2(y[r], ref[r])
+            print("precision", prec, "load case", r, "rel-L2 vs dense", err, flush=True)
+            assert err < FMM_APPLY_TOL, (prec, r, err)
+    prob.close()
