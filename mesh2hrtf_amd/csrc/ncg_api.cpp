@@ -757,7 +757,6 @@ int fmm_structure(ncg_problem *p)
     DevFmm &F = p->fmm;
     const int n = p->n;
     int rc;
-    std::vector<int> fine_ch_r0, fine_ch_m, fine_ch_cl, fine_cl_elem;   // row chunks of the finest level
     for (const ncgf::Level &L : p->plan.lev) {
         DevLevel D;
         D.ncl = (int)L.cl.size();
@@ -816,10 +815,6 @@ int fmm_structure(ncg_problem *p)
             (rc = upload(F.st, s, &D.fan_idx, fan_idx)) || (rc = upload(F.st, s, &D.fan_row, fan_row)))
             return rc;
         F.lev.push_back(D);
-        fine_ch_cl.swap(ch_cl);
-        fine_ch_r0.swap(ch_r0);
-        fine_ch_m.swap(ch_m);
-        fine_cl_elem.swap(cl_elem);
     }
     // near field of the finest level: row i -> elements of the near clusters of its
     // cluster; row lengths are per-cluster sums, the columns are written on the device
@@ -840,26 +835,29 @@ int fmm_structure(ncg_problem *p)
     }
     std::vector<long long> ptr(n + 1, 0);
     for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + rowlen[cl_of[i]];
-    // chunk table of the near-field apply: the finest level's row chunks with their column
-    // base (the first row of the cluster), width, rows and value offsets
-    const int R = ncgk::kChunkRows, nchf = (int)fine_ch_cl.size();
-    std::vector<long long> nd_cb(nchf), nd_off((size_t)nchf * R);
-    std::vector<int> nd_w(nchf), nd_row((size_t)nchf * R, -1);
-    for (int q = 0; q < nchf; ++q) {
-        const int c = fine_ch_cl[q];
-        nd_cb[q] = ptr[FL.cl[c].elems[0]];
-        nd_w[q] = (int)rowlen[c];
-        for (int a = 0; a < R; ++a) {
-            const size_t t = (size_t)q * R + a;
-            if (a < fine_ch_m[q]) {
-                nd_row[t] = fine_cl_elem[fine_ch_r0[q] + a];
-                nd_off[t] = ptr[nd_row[t]];
-            } else {
-                nd_off[t] = nd_cb[q];   // an unused slot reads the first row and stores nothing
+    // chunk table of the near-field apply: runs of at most kNearRows elements of each finest
+    // cluster (balanced sizes) with their column base (the cluster's first row), width, rows
+    // and value offsets
+    const int R = ncgk::kNearRows;
+    std::vector<long long> nd_cb, nd_off;
+    std::vector<int> nd_w, nd_row;
+    for (int c = 0; c < ncf; ++c) {
+        const std::vector<int> &el = FL.cl[c].elems;
+        const int m = (int)el.size(), nch = (m + R - 1) / R;
+        for (int q = 0, r0 = 0; q < nch; ++q) {
+            const int mq = m / nch + (q < m % nch ? 1 : 0);
+            const long long cb = ptr[el[0]];
+            nd_cb.push_back(cb);
+            nd_w.push_back((int)rowlen[c]);
+            for (int a = 0; a < R; ++a) {
+                // an unused slot reads the first row and stores nothing
+                nd_row.push_back(a < mq ? el[r0 + a] : -1);
+                nd_off.push_back(a < mq ? ptr[el[r0 + a]] : cb);
             }
+            r0 += mq;
         }
     }
-    F.nd_n = nchf;
+    F.nd_n = (int)nd_cb.size();
     // block-Jacobi layout: offset of the own cluster inside its rows; blocks are runs of
     // at most 64 consecutive elements of a cluster's list, of balanced sizes
     std::vector<int> self_off(ncf, 0), elem_blk(n, -1), blk_cl, blk_b0, blk_m;

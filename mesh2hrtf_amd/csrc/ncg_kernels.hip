@@ -1798,19 +1798,19 @@ __device__ __forceinline__ void waves_sum(double (&re)[NR], double (&im)[NR], do
     }
 }
 
-// Row chunks: the near-field apply below gives each work-group a chunk of at most 16 rows
-// that share one cluster (ncgk::NearChunks: the rows, their value offsets and the shared
-// column list).  Each of the 4 waves owns rows a = wave + 4 j and streams their values
-// with eight independent loads per lane per step; the vector the rows multiply is staged
-// in LDS once per work-group.
-static_assert(ncgk::kChunkRows == 16, "4 waves x 4 rows per chunk");
+// Row chunks: the near-field apply below gives each work-group a chunk of at most
+// ncgk::kNearRows rows that share one cluster (ncgk::NearChunks: the rows, their value
+// offsets and the shared column list).  Each of the 4 waves owns rows a = wave + 4 j and
+// streams their values with 2 kNearRowsPerWave independent loads per lane per step; the
+// vector the rows multiply is staged in LDS once per work-group.
+static_assert(ncgk::kNearRows == 4 * ncgk::kNearRowsPerWave, "4 waves per chunk");
 
-// sum over the wave of re/im for four rows and NR load cases, then row a gets lane 0's value
-template <int NR>
-__device__ __forceinline__ void wave_sum4(double (&re)[4][NR], double (&im)[4][NR])
+// sum over the wave of re/im for RW rows and NR load cases, then row a gets lane 0's value
+template <int RW, int NR>
+__device__ __forceinline__ void wave_sum_rows(double (&re)[RW][NR], double (&im)[RW][NR])
 {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < RW; ++j)
 #pragma unroll
         for (int r = 0; r < NR; ++r)
             for (int o = 32; o > 0; o >>= 1) {
@@ -1819,31 +1819,31 @@ __device__ __forceinline__ void wave_sum4(double (&re)[4][NR], double (&im)[4][N
             }
 }
 
-// values of four rows for one batch of U steps of 64 columns from b + lane (indices past sl
+// values of RW rows for one batch of U steps of 64 columns from b + lane (indices past sl
 // clamped to sl - 1)
-template <typename S, int U>
-__device__ __forceinline__ void rows4_load(const cx<S> *const (&vr)[4], int b, int sl, int lane, cx<S> (&v)[U][4])
+template <typename S, int RW, int U>
+__device__ __forceinline__ void rows_load(const cx<S> *const (&vr)[RW], int b, int sl, int lane, cx<S> (&v)[U][RW])
 {
 #pragma unroll
     for (int q = 0; q < U; ++q) {
         const int u = min(b + 64 * q + lane, sl - 1);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[q][j] = vr[j][u];
+        for (int j = 0; j < RW; ++j) v[q][j] = vr[j][u];
     }
 }
 
 // re/im[j][r] += this lane's share (u = lane + 64 t < sl) of sum_u v_j[u] * xs[r][u] for
-// four rows v_j and NR vectors in LDS (row stride seg).  v holds the first batch on entry
-// (rows4_load at b = 0); each batch's successor is loaded before the batch is used, so
+// RW rows v_j and NR vectors in LDS (row stride seg).  v holds the first batch on entry
+// (rows_load at b = 0); each batch's successor is loaded before the batch is used, so
 // two batches of loads are in flight.  Indices past sl are clamped and weighted by zero.
-template <typename S, int NR, int U>
-__device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, const cx<double> *xs, int seg, int lane,
-                                          cx<S> (&v)[U][4], double (&re)[4][NR], double (&im)[4][NR])
+template <typename S, int NR, int RW, int U>
+__device__ __forceinline__ void rows_dot(const cx<S> *const (&vr)[RW], int sl, const cx<double> *xs, int seg, int lane,
+                                         cx<S> (&v)[U][RW], double (&re)[RW][NR], double (&im)[RW][NR])
 {
     for (int b = 0; b < sl; b += 64 * U) {   // wave-uniform trip count
-        cx<S> nv[U][4];
+        cx<S> nv[U][RW];
         const bool more = b + 64 * U < sl;
-        if (more) rows4_load<S, U>(vr, b + 64 * U, sl, lane, nv);
+        if (more) rows_load<S, RW, U>(vr, b + 64 * U, sl, lane, nv);
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int u = b + 64 * q + lane;
@@ -1852,14 +1852,14 @@ __device__ __forceinline__ void rows4_dot(const cx<S> *const (&vr)[4], int sl, c
                 cx<double> xv = xs[r * seg + min(u, sl - 1)];
                 if (u >= sl) xv = mk(0.0, 0.0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cmac(re[j][r], im[j][r], double(v[q][j].re), double(v[q][j].im), xv);
+                for (int j = 0; j < RW; ++j) cmac(re[j][r], im[j][r], double(v[q][j].re), double(v[q][j].im), xv);
             }
         }
         if (more)
 #pragma unroll
             for (int q = 0; q < U; ++q)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) v[q][j] = nv[q][j];
+                for (int j = 0; j < RW; ++j) v[q][j] = nv[q][j];
     }
 }
 
@@ -1880,26 +1880,29 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long cb = ch_cb[q];
     const int w = ch_w[q];
-    int row[4];
-    const cx<S> *vr[4];
+    constexpr int RW = ncgk::kNearRowsPerWave;
+    int row[RW];
+    const cx<S> *vr[RW];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int a = ncgk::kChunkRows * q + wv + 4 * j;
+    for (int j = 0; j < RW; ++j) {
+        const int a = ncgk::kNearRows * q + wv + 4 * j;
         row[j] = ch_row[a];
         vr[j] = val + ch_off[a];
     }
-    double re[4][NR], im[4][NR];
+    double re[RW][NR], im[RW][NR];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < RW; ++j)
 #pragma unroll
         for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
     for (int s0 = 0; s0 < w; s0 += seg) {
         const int sl = min(seg, w - s0);
         // unused row slots read the first row of the cluster and are not stored; the first
         // batch of values is requested before the gather so the two latencies overlap
-        const cx<S> *vs[4] = {vr[0] + s0, vr[1] + s0, vr[2] + s0, vr[3] + s0};
-        cx<S> v[2][4];
-        rows4_load<S, 2>(vs, 0, sl, lane, v);
+        const cx<S> *vs[RW];
+#pragma unroll
+        for (int j = 0; j < RW; ++j) vs[j] = vr[j] + s0;
+        cx<S> v[2][RW];
+        rows_load<S, RW, 2>(vs, 0, sl, lane, v);
         __syncthreads();
         for (int u = threadIdx.x; u < sl; u += 256) {
             const int e = col[cb + s0 + u];
@@ -1907,12 +1910,12 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
             for (int r = 0; r < NR; ++r) xs[r * seg + u] = x[(size_t)r * n + e];
         }
         __syncthreads();
-        rows4_dot<S, NR, 2>(vs, sl, xs, seg, lane, v, re, im);
+        rows_dot<S, NR, RW, 2>(vs, sl, xs, seg, lane, v, re, im);
     }
-    wave_sum4<NR>(re, im);
+    wave_sum_rows<RW, NR>(re, im);
     if (lane == 0)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < RW; ++j)
             if (row[j] >= 0)
 #pragma unroll
                 for (int r = 0; r < NR; ++r) y[(size_t)r * n + row[j]] = mk(re[j][r], im[j][r]);

@@ -158,9 +158,12 @@ struct RowChunks {
     const int *cl = nullptr, *r0 = nullptr, *m = nullptr;
 };
 // Near-field row chunks with their addresses resolved on the host: the rows of chunk q share
-// the column list col[cb[q] .. cb[q] + w[q]); row slot a < kChunkRows holds row
-// row[kChunkRows q + a] (-1: unused) whose values start at off[kChunkRows q + a] (cb[q]
-// for an unused slot).
+// the column list col[cb[q] .. cb[q] + w[q]); row slot a < kNearRows holds row
+// row[kNearRows q + a] (-1: unused) whose values start at off[kNearRows q + a] (cb[q]
+// for an unused slot).  A finest cluster of m elements gives ceil(m / kNearRows) chunks
+// of balanced sizes; 4 waves of kNearRowsPerWave rows each work on one chunk.
+constexpr int kNearRowsPerWave = 3;
+constexpr int kNearRows = 4 * kNearRowsPerWave;
 struct NearChunks {
     int n = 0;
     const long long *cb = nullptr, *off = nullptr;
