@@ -63,10 +63,7 @@ SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=
 
 
 def freq_spec(src, f):
-    omega = 2 * np.pi * f
-    amp = 1.0 / (1j * RHO * omega)
-    ps = [[s[0], s[1], s[2], amp.real, amp.imag, r] for r, s in enumerate(src)]
-    return dict(k=omega / C, nrhs=len(src), point_sources=ps)
+    return P.point_source_spec(src, f)
 
 
 def cpu_baseline(mesh, grid, src, f, timeout, method_bem=4, what=None):
@@ -135,38 +132,11 @@ def pick(freqs, count, offset=0):
     return [freqs[int(offset + i * stride) % len(freqs)] for i in range(count)]
 
 
-def batched_problems(count, seed=0):
-    """Batched uneven dense problems: seeded ellipsoids and perturbed spheres, radii
-    0.05-0.12 m, 500-6000 triangles log-uniform, each with 8 frequencies uniform in
-    0.2-8 kHz; 16 plane-wave load cases with directions uniform on the sphere."""
-    rng = np.random.default_rng(seed)
-    meshes = []
-    for i in range(count):
-        fr = max(5, int(round(np.sqrt(np.exp(rng.uniform(np.log(500.0), np.log(6000.0))) / 20.0))))
-        radius = rng.uniform(0.05, 0.12)
-        if i % 2 == 0:
-            axes, rms = tuple(radius * rng.uniform(0.7, 1.0, 3)), 0.0
-        else:
-            axes, rms = (radius, radius, radius), 0.03 * radius
-        meshes.append((P.head_like_mesh(freq=fr, seed=1000 + i, semi_axes=axes, rms=rms),
-                       list(rng.uniform(200.0, 8000.0, 8))))
-    dirs = rng.standard_normal((16, 3))
-    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-    return meshes, dirs
-
-
-def plane_wave_spec(dirs, f):
-    omega = 2 * np.pi * f
-    amp = 1.0 / (1j * RHO * omega)
-    return dict(k=omega / C, nrhs=len(dirs),
-                plane_waves=[[d[0], d[1], d[2], amp.real, amp.imag, r] for r, d in enumerate(dirs)])
-
-
 def batched_main(a):
     """Batched uneven problems: every (mesh, frequency) step of the batch submitted at once to
     the serving runtime; dense BEM (methodBEM 0), mixed form, the 16 load cases applied on
     the matrix cores."""
-    meshes, dirs = batched_problems(a.meshes)
+    meshes, dirs = P.batched_problems(a.meshes)
     ns = [len(m.elems) for m, _ in meshes]
     print("# batched: %d meshes (n %d..%d, mean %d), 8 frequencies x %d plane waves each; lanes=%d" % (
         len(meshes), min(ns), max(ns), int(np.mean(ns)), len(dirs), a.lanes), file=sys.stderr, flush=True)
@@ -176,11 +146,11 @@ def batched_main(a):
         n = len(m.elems)
         ids.append(srv.add_mesh(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems,
                                 method_bem=0))
-    for j in [srv.submit(ids[i], [plane_wave_spec(dirs, meshes[i][1][0])], **SERVED) for i in range(min(4, len(ids)))]:
+    for j in [srv.submit(ids[i], [P.plane_wave_spec(dirs, meshes[i][1][0])], **SERVED) for i in range(min(4, len(ids)))]:
         j.wait()
     srv.kernel_stats(reset=True)
     t0 = time.perf_counter()
-    jobs = [srv.submit(mid, [plane_wave_spec(dirs, f) for f in fq], **SERVED) for mid, (_, fq) in zip(ids, meshes)]
+    jobs = [srv.submit(mid, [P.plane_wave_spec(dirs, f) for f in fq], **SERVED) for mid, (_, fq) in zip(ids, meshes)]
     sts = [j.wait() for j in jobs]
     elapsed = time.perf_counter() - t0
     kst = srv.kernel_stats()
@@ -195,7 +165,7 @@ def batched_main(a):
         n = len(m.elems)
         prob = ncg.Problem(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems,
                            method_bem=0)
-        sp = plane_wave_spec(dirs, fq[-1])
+        sp = P.plane_wave_spec(dirs, fq[-1])
         prob.assemble(sp["k"], nrhs=sp["nrhs"], plane_waves=sp["plane_waves"])
         prob.solve(solver=ncg.CGS, tolerance=1e-9)
         x0 = prob.solution()
@@ -212,7 +182,7 @@ def batched_main(a):
     n = len(m.elems)
     prob = ncg.Problem(np.full(n, 3), m.nodes[m.elems], np.zeros(n, np.int32), None, node_index=m.elems, method_bem=0)
     try:
-        sp = plane_wave_spec(dirs, fq[0])
+        sp = P.plane_wave_spec(dirs, fq[0])
         prob.assemble(sp["k"], nrhs=sp["nrhs"], plane_waves=sp["plane_waves"], precision=ncg.MIXED)
         prob.kernel_timing(True)
         prob.kernel_stats(reset=True)

@@ -156,7 +156,9 @@ int ncg_problem_destroy(ncg_problem *p);
  * (src/NC_EquationSystem.cpp:79-302, 1408-1812). precision: NCG_FP64 or NCG_MIXED. */
 int ncg_assemble(ncg_problem *p, const ncg_frequency *freq, int32_t precision);
 
-/* Solve the assembled system for all load cases.
+/* Solve the assembled system for all load cases.  An NCG_MIXED operator is solved by
+ * FP64 iterative refinement around FP32 GMRES: it needs solver NCG_SOLVER_GMRES and
+ * precond NCG_PRECOND_BLOCK or NCG_PRECOND_NONE (anything else: NCG_E_ARG).
  * Replaces NC_IterativeSolverCGS / Tfactor_usy+Tfbelim (src/NC_CommonFunctions.cpp:1066-1208,
  * src/NC_TypeDefinition.h:542-594, selected at src/NC_Main.cpp:833-866). */
 int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *stats);

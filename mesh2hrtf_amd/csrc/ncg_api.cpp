@@ -2022,6 +2022,11 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
         rc = 0;
         if (st) st->converged_all = 1;
     } else if (p->precision == NCG_MIXED) {
+        // the mixed form is GMRES refinement: reject a method it would not run
+        if (o->solver != NCG_SOLVER_GMRES)
+            return fail(NCG_E_ARG, "the mixed-precision form solves with GMRES (NCG_SOLVER_GMRES)");
+        if (o->precond != NCG_PRECOND_BLOCK && o->precond != NCG_PRECOND_NONE)
+            return fail(NCG_E_ARG, "the mixed-precision form takes the block-Jacobi or no preconditioner");
         rc = solve_mixed(p, o, st);
     } else if (o->solver == NCG_SOLVER_GMRES) {
         rc = solve_gmres(p, o, st);

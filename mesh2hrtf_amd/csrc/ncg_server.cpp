@@ -66,6 +66,8 @@ struct Job {
     size_t bytes = 0;   // device footprint of one step (highest frequency)
     clk::time_point t_submit, t_first, t_done;
     bool started = false;
+    bool failed = false;   // a step failed: no further step starts, outputs are no longer written
+    int inflight = 0;      // steps of this job running on the lanes
     int freq_done = 0, state = 0, error_code = 0;
     long long iterations = 0;
     std::string error;
@@ -112,24 +114,36 @@ namespace {
 
 int sfail(int code, const std::string &msg) { return ncgi::fail(code, msg); }
 
-void finish_job(ncg_server *S, Job *J, int code, const std::string &msg)
+// Ends a job (lock held).  A failed step marks the job failed and drops its queued steps
+// at once, but the job is reported ended (state 3, waiters woken) only when none of its
+// steps is still running on another lane, so no lane writes into the caller's output
+// buffers after ncg_server_wait has returned.
+void end_job(ncg_server *S, Job *J)
 {
     if (J->state >= 2) return;
     J->t_done = clk::now();
-    if (code) {
+    if (J->failed) {
         J->state = 3;
-        J->error_code = code;
-        J->error = msg;
         S->jobs_failed++;
-        // drop its remaining steps
-        S->queue.erase(std::remove_if(S->queue.begin(), S->queue.end(), [&](const Step &s) { return s.job == J; }),
-                       S->queue.end());
     } else {
         J->state = 2;
         S->jobs_done++;
     }
     S->latencies.push_back(ms_between(J->t_submit, J->t_done));
     S->cv_done.notify_all();
+}
+
+void fail_job(ncg_server *S, Job *J, int code, const std::string &msg)
+{
+    if (J->state >= 2) return;
+    if (!J->failed) {
+        J->failed = true;
+        J->error_code = code;
+        J->error = msg;
+        S->queue.erase(std::remove_if(S->queue.begin(), S->queue.end(), [&](const Step &s) { return s.job == J; }),
+                       S->queue.end());
+    }
+    if (J->inflight == 0) end_job(S, J);
 }
 
 // Choice of the next step for a free lane (lock held).  Returns the queue index, the
@@ -188,7 +202,7 @@ void run_lane(ncg_server *S)
             if (S->running == 0) {
                 // nothing runs that could free memory: the head step cannot fit at all
                 Job *J = S->queue.front().job;
-                finish_job(S, J, NCG_E_MEMORY,
+                fail_job(S, J, NCG_E_MEMORY,
                            "a step of this job needs " + std::to_string(J->bytes >> 20) + " MiB, beyond the " +
                                std::to_string((long long)S->budget >> 20) + " MiB device-memory budget");
                 continue;
@@ -218,6 +232,7 @@ void run_lane(ncg_server *S)
         r->bytes += need;
         S->peak = std::max(S->peak, S->resident);
         S->running++;
+        J->inflight++;
         if (!J->started) {
             J->started = true;
             J->t_first = clk::now();
@@ -249,11 +264,17 @@ void run_lane(ncg_server *S)
             for (int c = 0; c < J->nrhs && c < 64; ++c) its += ss.iterations[c];
         }
         const size_t vec = 2 * (size_t)J->nrhs;
-        if (!rc && J->solution) {
+        bool write = true;
+        {
+            // another step of the job failed meanwhile: its outputs are no longer wanted
+            std::lock_guard<std::mutex> g(S->mu);
+            write = !J->failed;
+        }
+        if (!rc && write && J->solution) {
             rc = ncg_get_solution(r->p, J->solution + vec * J->n * st.fi);
             if (rc) msg = ncg_last_error();
         }
-        if (!rc && J->m > 0 && (J->phi || J->grad)) {
+        if (!rc && write && J->m > 0 && (J->phi || J->grad)) {
             std::vector<double> tmp;
             double *ph = J->phi ? J->phi + vec * J->m * st.fi : nullptr;
             if (!ph) {
@@ -291,10 +312,13 @@ void run_lane(ncg_server *S)
         S->steps_done++;
         S->solves_done += J->nrhs;
         J->iterations += its;
+        J->inflight--;
         if (rc) {
-            finish_job(S, J, rc, msg);
-        } else if (J->state < 2 && ++J->freq_done == (int)J->freq.size()) {
-            finish_job(S, J, 0, "");
+            fail_job(S, J, rc, msg);
+        } else if (J->failed) {
+            if (J->inflight == 0) end_job(S, J);
+        } else if (++J->freq_done == (int)J->freq.size()) {
+            end_job(S, J);
         }
         S->cv_work.notify_all();
     }

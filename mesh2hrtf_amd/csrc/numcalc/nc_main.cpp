@@ -403,6 +403,12 @@ int main(int argc, char **argv)
         memset(&so, 0, sizeof(so));
         so.solver = P.method_solver == 4 ? NCG_SOLVER_LU : (gmres ? NCG_SOLVER_GMRES : NCG_SOLVER_CGS);
         so.precond = P.precond == 0 ? NCG_PRECOND_ILU : P.precond == 1 ? NCG_PRECOND_SCALING : NCG_PRECOND_NONE;
+        if (precision == NCG_MIXED && so.solver != NCG_SOLVER_LU) {
+            // the mixed form is FP64 refinement around FP32 GMRES with the block-Jacobi
+            // preconditioner, whatever IPRECOND says
+            so.solver = NCG_SOLVER_GMRES;
+            so.precond = NCG_PRECOND_BLOCK;
+        }
         so.ilu_scanning_degree = P.scan_degree;
         so.max_iterations = nitermax;
         so.tolerance = 1.0e-9;
@@ -414,7 +420,7 @@ int main(int argc, char **argv)
         if (so.solver != NCG_SOLVER_LU) {
             if (rc == NCG_E_NOCONV) nc::warning(log, "Warning: Maximum number of iterations is reached!");
             else if (rc) gpu_fatal(log, "ncg_solve", rc);
-            log << "\n" << (so.solver == NCG_SOLVER_CGS && precision == NCG_FP64 ? "CGS" : "GMRES")
+            log << "\n" << (so.solver == NCG_SOLVER_CGS ? "CGS" : "GMRES")
                 << " solver: number of iterations = " << st.iterations[0] << ", relative error = " << st.relative_residual[0]
                 << std::endl;
             if (rc == NCG_E_NOCONV && precision == NCG_FP64 && fi.method == 0) {

@@ -315,6 +315,9 @@ class Server:
         cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing))
         _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._meshes = {}
+        # arrays the server reads or writes for each job that has not ended: they stay
+        # referenced here until wait() sees the end (or the server is closed)
+        self._live = {}
 
     def add_mesh(self, nv, xyz, kind, eval_xyz=None, h=1.0, interior=0, node_index=None, group=None, method_bem=4,
                  min_terms=0, cluster_edge=0.0):
@@ -354,6 +357,7 @@ class Server:
         job = _Job(int(mesh), nf, ctypes.cast(arr, ctypes.c_void_p), int(precision), o, _ptr(x), _ptr(phi), _ptr(grad))
         jid = ctypes.c_int64()
         _check(self.lib.ncg_server_submit(self._h, ctypes.byref(job), ctypes.byref(jid)))
+        self._live[jid.value] = (keep, arr, x, phi, grad)
         return ServerJob(self, jid.value, x, phi, grad)
 
     def wait(self, job, timeout=-1.0):
@@ -361,6 +365,8 @@ class Server:
         rc = self.lib.ncg_server_wait(self._h, ctypes.c_int64(job.id), ctypes.c_double(timeout), ctypes.byref(st))
         if rc not in (0, -5) and st.state != 3:   # a failed job reports through its status
             _check(rc)
+        if st.state >= 2:
+            self._live.pop(job.id, None)
         return _status_dict(st)
 
     def query(self, job):
@@ -381,8 +387,9 @@ class Server:
 
     def close(self):
         if self._h:
-            self.lib.ncg_server_destroy(self._h)
+            self.lib.ncg_server_destroy(self._h)   # finishes the queued work first
             self._h = ctypes.c_void_p()
+        self._live.clear()
 
     def __del__(self):
         try:

@@ -344,6 +344,46 @@ def config2_freqs():
     return [200.0 * (i + 1) for i in range(80)]
 
 
+def batched_problems(count: int, seed: int = 0):
+    """Batched uneven dense problems: seeded ellipsoids (even index) and perturbed spheres
+    (odd index), radii 0.05-0.12 m, 500-6000 triangles log-uniform, each with 8 frequencies
+    uniform in 0.2-8 kHz; 16 plane-wave load cases with directions uniform on the sphere
+    (normalised Gaussian 3-vectors, drawn from their own generator so they do not depend on
+    ``count``).  Returns ([(mesh, freqs)], dirs)."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for i in range(count):
+        fr = max(5, int(round(np.sqrt(np.exp(rng.uniform(np.log(500.0), np.log(6000.0))) / 20.0))))
+        radius = rng.uniform(0.05, 0.12)
+        if i % 2 == 0:
+            axes, rms = tuple(radius * rng.uniform(0.7, 1.0, 3)), 0.0
+        else:
+            axes, rms = (radius, radius, radius), 0.03 * radius
+        meshes.append((head_like_mesh(freq=fr, seed=1000 + i, semi_axes=axes, rms=rms),
+                       list(rng.uniform(200.0, 8000.0, 8))))
+    dirs = np.random.default_rng(seed + 1).standard_normal((16, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    return meshes, dirs
+
+
+def plane_wave_spec(dirs, f, c: float = 343.0, rho: float = 1.1839):
+    """One frequency step with one plane-wave load case per direction, amplitude 1 in NC.inp
+    terms (the library takes source strengths divided by i rho omega)."""
+    omega = 2 * np.pi * f
+    amp = 1.0 / (1j * rho * omega)
+    return dict(k=omega / c, nrhs=len(dirs),
+                plane_waves=[[d[0], d[1], d[2], amp.real, amp.imag, r] for r, d in enumerate(dirs)])
+
+
+def point_source_spec(src, f, c: float = 343.0, rho: float = 1.1839):
+    """One frequency step with one point-source load case per source position, amplitude 1
+    in NC.inp terms."""
+    omega = 2 * np.pi * f
+    amp = 1.0 / (1j * rho * omega)
+    return dict(k=omega / c, nrhs=len(src),
+                point_sources=[[s[0], s[1], s[2], amp.real, amp.imag, r] for r, s in enumerate(src)])
+
+
 # ----------------------------------------------------------------------------- be.out
 
 def read_be_file(path):

@@ -21,7 +21,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from mesh2hrtf_amd import project as P  # noqa: E402
-from ncref import nc_tables  # noqa: E402
+from ncref import c2_project, mesh_digest, nc_tables  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -193,6 +193,14 @@ def cases():
     C["test_source_2pw"] = dict(prj=P.Project(
         mesh=ico2, freqs=[900.0], eval_grid=P.fibonacci_grid(40, 0.6), test_point_source=True,
         plane_waves=[(0.0, 0.0, 1.0, 1.0, 0.0), (0.6, 0.8, 0.0, 0.5, 0.5)], title="test point source, two plane waves"))
+    # one mesh of the batched uneven dense problems (project.batched_problems: index 3, a
+    # perturbed sphere of 1620 triangles) at its first two frequencies, first plane wave
+    bm, dirs = P.batched_problems(8)
+    mesh3, f3 = bm[3]
+    C["c4_mesh3"] = dict(prj=P.Project(
+        mesh=mesh3, freqs=[float(f3[0]), float(f3[1])], eval_grid=P.fibonacci_grid(50, 0.5),
+        plane_waves=[(float(dirs[0][0]), float(dirs[0][1]), float(dirs[0][2]), 1.0, 0.0)],
+        title="batched dense problem 3 (perturbed sphere), plane wave 0"))
     return C
 
 
@@ -246,12 +254,60 @@ def run_case(name, prj):
         name, len(m.elems), 0 if g is None else len(g.nodes), len(prj.freqs), wall, iters))
 
 
+# ---- the head-like configuration at its full size ---------------------------------------
+# ncref.c2_project(): 20480-element seeded head-like mesh, one ear source, the 1730-point
+# grid, four frequencies.  The mesh is not stored: the generator is deterministic and the
+# fixture keeps a digest of it.  Values are stored as complex64 (the files print 7 digits).
+
+
+def run_c2_case(name="c2_head"):
+    prj = c2_project()
+    mesh, grid = prj.mesh, prj.eval_grid
+    C2_FREQS = prj.freqs
+    with tempfile.TemporaryDirectory() as d:
+        P.write_project(d, prj)
+        # one reference process per frequency step, side by side (as manage_numcalc runs them)
+        t0 = time.time()
+        procs = [subprocess.Popen([REF, "-istart", str(s + 1), "-iend", str(s + 1)], cwd=d,
+                                  stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                 for s in range(len(C2_FREQS))]
+        codes = [p.wait() for p in procs]
+        wall = time.time() - t0
+        if any(codes):
+            raise RuntimeError("%s: NumCalc failed %s" % (name, codes))
+        iters, relerr, tables = [], [], []
+        for s in range(len(C2_FREQS)):
+            nout = open(os.path.join(d, "NC%d-%d.out" % (s + 1, s + 1))).read()
+            its = [l for l in nout.splitlines() if l.startswith("CGS solver: number of iterations")]
+            iters.append(int(its[0].split("=")[1].split(",")[0]))
+            relerr.append(float(its[0].split("relative error =")[1]))
+            tables.append(nc_tables(nout))
+        out = {}
+        for key in ("pBoundary", "pEvalGrid", "vEvalGrid"):
+            vals = []
+            for s in range(len(C2_FREQS)):
+                ids, v = P.read_be_file(os.path.join(d, "be.out", "be.%d" % (s + 1), key))
+                vals.append(v)
+            out[key] = np.asarray(vals).astype(np.complex64)
+            out[key + "_ids"] = ids
+    np.savez_compressed(
+        os.path.join(OUT, "case_%s.npz" % name), mesh_sha256=np.array(mesh_digest(mesh)),
+        eval_nodes=grid.nodes, spec=np.array(json.dumps(spec_of(prj))),
+        cgs_iterations=np.asarray(iters, dtype=np.int64), cgs_relerr=np.asarray(relerr),
+        nc_tables=np.array(json.dumps(tables)), ref_wall_s=np.float64(wall), **out)
+    print("%-12s N=%d M=%d steps=%d wall=%.1fs iters=%s" % (name, len(mesh.elems), len(grid.nodes),
+                                                             len(C2_FREQS), wall, iters))
+
+
 def main(argv):
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
     C = cases()
-    for name in (argv or list(C)):
-        run_case(name, C[name]["prj"])
+    for name in (argv or list(C) + ["c2_head"]):
+        if name == "c2_head":
+            run_c2_case(name)
+        else:
+            run_case(name, C[name]["prj"])
 
 
 if __name__ == "__main__":

@@ -314,3 +314,28 @@ def same_table_line(a, b, rtol=2e-5):
         if abs(fx - fy) > rtol * max(abs(fx), abs(fy)):
             return False
     return True
+
+
+# ---- the head-like configuration at its full size (fixture case_c2_head.npz) ------------
+C2_FREQS = [1000.0, 4600.0, 9800.0, 15800.0]
+
+
+def mesh_digest(mesh):
+    """sha256 of a mesh's node coordinates and element rows (pins the seeded generator)."""
+    import hashlib
+    h = hashlib.sha256(np.ascontiguousarray(mesh.nodes).tobytes())
+    h.update(np.ascontiguousarray(mesh.elems, np.int64).tobytes())
+    return h.hexdigest()
+
+
+def c2_project():
+    """The 20480-element seeded head-like mesh, its +y ear point source (amplitude 1), the
+    1730-point grid at 1.2 m and four frequencies of the 200-16000 Hz sweep, NumCalc
+    methodBEM 4 (SL-FMM at 1 kHz, ML-FMM above), ILU-CGS."""
+    from mesh2hrtf_amd import project as P
+    mesh = P.config2_mesh(32)
+    grid = P.fibonacci_grid(1730, 1.2)
+    src = P.ear_sources(mesh)[0]
+    return P.Project(mesh=mesh, freqs=list(C2_FREQS), eval_grid=grid, method_bem=4,
+                     point_sources=[(src[0], src[1], src[2], 1.0, 0.0)],
+                     title="head-like mesh nu=32 (20480 triangles), +y ear source")

@@ -1,0 +1,98 @@
+"""The head-like configuration at its full size against reference NumCalc.
+
+Fixture tests/golden/case_c2_head.npz: the 20480-element seeded head-like mesh with its +y
+ear point source and the 1730-point grid at 1.2 m, at 1, 4.6, 9.8 and 15.8 kHz (SL-FMM at
+1 kHz, two-level ML-FMM above), be.out and NC.out of the reference NumCalc compiled from its
+sources (oracle/_ref), one process per step.
+
+* the drop-in executable (FP64 plain method: NumCalc's method choice, ILU-CGS to 1e-9):
+  be.out within rel-L2 1e-5 of the reference, the reference's CGS iteration count and NC.out
+  analysis tables at every step;
+* the served mixed form (FP32 bulk, FP64 refinement, GMRES + block-Jacobi) through the
+  serving runtime: surface and grid pressure within rel-L2 1e-3 (the metric's bar) of the
+  reference; measured about 1e-4.
+"""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ncref as R
+from mesh2hrtf_amd import project as P
+
+NUMCALC = os.path.join(R.ROOT, "mesh2hrtf_amd", "bin", "NumCalc")
+DROPIN_TOL = 1e-5
+SERVED_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def c2():
+    z = R.load_case("c2_head")
+    prj = R.c2_project()
+    assert R.mesh_digest(prj.mesh) == str(z["mesh_sha256"])
+    return z, prj
+
+
+@pytest.mark.gpu
+def test_c2_dropin_matches_reference(tmp_path, c2):
+    z, prj = c2
+    if not os.path.exists(NUMCALC):
+        pytest.fail("drop-in NumCalc not built: run __graft_entry__.build()")
+    P.write_project(str(tmp_path), prj)
+    print("c2 drop-in: 4 steps, n=20480", flush=True)
+    r = subprocess.run([NUMCALC], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    log = open(tmp_path / "NC.out").read()
+    its = [int(v) for v in re.findall(r"CGS solver: number of iterations = (\d+), relative error = ", log)]
+    assert its == list(z["cgs_iterations"])
+    ref_tables = json.loads(str(z["nc_tables"]))
+    ours = R.nc_tables(log)
+    assert len(ours) == len(ref_tables)
+    for s in range(len(prj.freqs)):
+        assert len(ours[s]) == len(ref_tables[s][0]), (s, ours[s], ref_tables[s][0])
+        for a, b in zip(ours[s], ref_tables[s][0]):
+            assert R.same_table_line(a, b), (s, a, b)
+        for key in ("pBoundary", "pEvalGrid", "vEvalGrid"):
+            ids, vals = P.read_be_file(str(tmp_path / "be.out" / ("be.%d" % (s + 1)) / key))
+            assert np.array_equal(ids, z[key + "_ids"])
+            e = R.rel_l2(vals, z[key][s].astype(np.complex128))
+            print("step", s + 1, key, "rel-L2 %.2e" % e, flush=True)
+            assert e < DROPIN_TOL, (key, s, e)
+
+
+@pytest.mark.gpu
+def test_c2_served_mixed_matches_reference(c2):
+    from mesh2hrtf_amd import ncg
+    z, prj = c2
+    mesh, grid = prj.mesh, prj.eval_grid
+    n = len(mesh.elems)
+    src = prj.point_sources[0]
+    srv = ncg.Server(device=0, lanes=4)
+    mid = srv.add_mesh(np.full(n, 3), mesh.nodes[mesh.elems], np.zeros(n, np.int32), grid.nodes,
+                       node_index=mesh.elems, method_bem=4)
+    freqs = []
+    for f in prj.freqs:
+        omega = 2 * np.pi * f
+        amp = 1.0 / (1j * prj.rho * omega)
+        freqs.append(dict(k=omega / prj.c, nrhs=1, point_sources=[[src[0], src[1], src[2], amp.real, amp.imag, 0]]))
+    job = srv.submit(mid, freqs, precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50,
+                     max_iterations=500, refinement_steps=8, refinement_tolerance=1e-5)
+    st = job.wait()
+    srv.close()
+    assert st["state"] == "done", st
+    for s, f in enumerate(prj.freqs):
+        rp = prj.rho * 2 * np.pi * f
+        es = R.rel_l2(1j * rp * job.solution[s][0], z["pBoundary"][s].astype(np.complex128))
+        ee = R.rel_l2(1j * rp * job.phi[s][0], z["pEvalGrid"][s].astype(np.complex128))
+        print("served f=%.0f surface %.2e grid %.2e" % (f, es, ee), flush=True)
+        assert es < SERVED_TOL and ee < SERVED_TOL, (f, es, ee)
+
+
+def test_c2_fixture_mesh_is_the_generator():
+    """The fixture stores no mesh: the seeded generator must still give the same one."""
+    z = R.load_case("c2_head")
+    assert R.mesh_digest(R.c2_project().mesh) == str(z["mesh_sha256"])
+    assert list(z["cgs_iterations"]) == [38, 26, 20, 14]

@@ -179,7 +179,9 @@ def test_mixed_precision_within_tolerance(ncg):
         x64 = prob.solution()[0]
         p64, _ = prob.field()
         assemble(prob, pa, precision=ncg.MIXED)
-        st = prob.solve(refinement_tolerance=1e-7)
+        with pytest.raises(ncg.NcgError):
+            prob.solve(solver=ncg.CGS)          # the mixed form is GMRES refinement only
+        st = prob.solve(solver=ncg.GMRES, refinement_tolerance=1e-7)
         assert st["converged"]
         x32 = prob.solution()[0]
         p32, _ = prob.field()

@@ -125,3 +125,32 @@ def test_memory_admission(ncg):
         assert j.wait(timeout=60)["state"] == "done"
     finally:
         srv.close()
+
+
+def test_failed_step_ends_job_after_running_steps(ncg):
+    """One frequency of a multi-frequency job fails (a point source with a load-case index
+    out of range, rejected by ncg_assemble on the host) on a 2-lane server: the job reports
+    failed with that error, and only after every running step of it has finished, so no
+    output is written after wait returns; the server then serves the next job."""
+    case = R.load_case("head8")
+    srv = ncg.Server(lanes=2)
+    try:
+        md = srv.add_mesh(**mesh_args(case, 0))
+        specs = freq_specs(case, [1000.0, 1500.0, 2000.0, 2500.0, 3000.0, 3500.0])
+        bad = dict(specs[2])
+        bad["point_sources"] = [list(v[:5]) + [7] for v in bad["point_sources"]]
+        specs[2] = bad
+        print("server: job with a failing step", flush=True)
+        j = srv.submit(md, specs, solver=0, precond=2, precision=ncg.FP64)
+        st = j.wait(timeout=120)
+        assert st["state"] == "failed" and st["error_code"] == -1 and "rhs index" in st["error"], st
+        snap_x, snap_phi = j.solution.copy(), j.phi.copy()
+        import time
+        time.sleep(1.0)
+        assert np.array_equal(j.solution, snap_x) and np.array_equal(j.phi, snap_phi)
+        info = srv.stats()
+        assert info["jobs_failed"] == 1 and info["queued_steps"] == 0
+        ok = srv.submit(md, freq_specs(case, [2000.0]), solver=0, precond=2, precision=ncg.FP64)
+        assert ok.wait(timeout=60)["state"] == "done"
+    finally:
+        srv.close()
