@@ -335,6 +335,7 @@ def main():
                     help="frequency steps in the timed region (the 80-frequency sweep, cyclically)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--group", type=int, default=4, help="served steps a lane solves together (ncg_solve_group)")
     ap.add_argument("--nu", type=int, default=32, help="geodesic frequency of the mesh (32 -> 20480 triangles)")
     ap.add_argument("--plain-steps", type=int, default=-1,
                     help="frequencies the FP64 plain method runs (-1: every frequency of the timed steps)")
@@ -375,7 +376,7 @@ def main():
           file=sys.stderr, flush=True)
 
     # ---- served mixed form: all K steps submitted at once -------------------------------
-    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True)
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group)
     mid = srv.add_mesh(**geo)
     warm = [srv.submit(mid, [freq_spec(src, f)], **SERVED) for f in pick(freqs, a.warmup, offset=1)]
     for j in warm:
@@ -511,7 +512,7 @@ def main():
         "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
                                  "eval_grid_max": float(max(err_e)) if err_e else None,
                                  "steps_checked": len(err_s)},
-        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done")},
+        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done", "lane_ms")},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

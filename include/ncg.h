@@ -163,6 +163,18 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *freq, int32_t precision);
  * src/NC_TypeDefinition.h:542-594, selected at src/NC_Main.cpp:833-866). */
 int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *stats);
 
+/* Solve several assembled problems together (frequency steps of one mesh or of different
+ * meshes, equal n_rhs and precision, one device) with GMRES or the mixed form: every Krylov
+ * step is one launch per operation for the whole group, so the operator applies of the
+ * group share each launch, and the host waits once per step for all of them.  Each
+ * problem's result is the one ncg_solve gives it alone.  stats: [count] or NULL.  The
+ * problems' streams are joined for the call; calls on them must not overlap it.
+ * Replaces the per-frequency solves of NC_IterativeSolverCGS (src/NC_CommonFunctions.cpp:
+ * 1066-1208) as run by manage_numcalc's one process per frequency
+ * (mesh2hrtf/NumCalc/manage_numcalc.py:376-430): here the frequencies share launches. */
+int ncg_solve_group(ncg_problem *const *problems, int32_t count, const ncg_solver_options *opt,
+                    ncg_solve_stats *stats);
+
 /* The method and FMM plan of the last ncg_assemble (what NumCalc logs per step). */
 int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info);
 
@@ -228,6 +240,8 @@ typedef struct {
                                            0 -> 90% of the free memory at creation          */
     int32_t problems_per_lane;          /* meshes a lane keeps resident (LRU); 0 -> 2         */
     int32_t kernel_timing;              /* 1: per-kernel timing of every step (ncg_kernel_stats) */
+    int32_t group;                      /* GMRES / mixed steps a lane solves together
+                                           (ncg_solve_group); 0 -> 4, 1: one at a time       */
 } ncg_server_config;
 
 typedef struct {
@@ -257,6 +271,8 @@ typedef struct {
     double wall_s;                      /* since creation                                    */
     int64_t resident_bytes, peak_resident_bytes;
     int32_t lanes, queued_steps;
+    double lane_ms[4];                  /* lane wall time summed over the lanes: assembly,
+                                           solve, outputs (solution and grid), waiting       */
 } ncg_server_info;
 
 int ncg_server_create(const ncg_server_config *cfg, ncg_server **out);

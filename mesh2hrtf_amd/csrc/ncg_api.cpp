@@ -377,6 +377,17 @@ struct ncg_problem {
     size_t nrhs_fmm = 0;
     DevIlu ilu;
     Grow out_phi, out_grad;   // evaluation-grid results before the copy to the host
+    // grouped Krylov solves (this problem as a member or as the group's leader): GMRES state
+    // of the member, and for the leader the launch descriptors (device copy and pinned
+    // host staging), pinned landing slots for flags and norms, and the events that order
+    // the members' streams around the group's
+    Grow gm;
+    Grow gdesc;
+    char *gstage = nullptr;
+    size_t gstage_cap = 0;
+    void *gpin = nullptr;
+    hipEvent_t gev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t join = nullptr;
     // per-kernel timing of the operator apply (off unless requested)
     int ktime = 0;
     std::vector<EventPair> kpool;
@@ -1510,258 +1521,845 @@ int solve_cgs(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     return all ? 0 : fail(NCG_E_NOCONV, "Warning: Maximum number of iterations is reached!");
 }
 
-// ---------------------------------------------------------------------------------- GMRES
-// Restarted GMRES(m) per load case with Hermitian inner products and classical
-// Gram-Schmidt applied twice.  The Arnoldi vectors, projections and norms stay on the
-// device; one copy of the new Hessenberg column per iteration goes to the host for the
-// Givens rotations and the convergence test.  Optional right preconditioning by the
-// block-Jacobi inverse (A M^-1 u = b, x = M^-1 u).  acc64 selects the matvec
-// accumulation (FP32 operator: 0 = FP32 inner solve).  Solves for the correction in 'x'
-// starting from its current content; x_zero says that content is zero, so the first
-// residual is b without a matvec.
-int gmres(ncg_problem *p, const double *b, double *x, int restart, int maxit, double tol, int acc64, int bj,
-          bool x_zero, std::vector<int> &iters, std::vector<double> &relres)
-{
-    const int n = p->n, nr = p->nrhs;
-    const size_t vlen = (size_t)nr * n;
-    const int m = std::max(2, std::min(restart, 200));
-    const size_t need = (size_t)(m + 1) * vlen * 2;
-    if (need > p->basis_cap) {
-        hipFree(p->basis);
-        int rc = dalloc(&p->basis, need);
-        if (rc) return rc;
-        p->basis_cap = need;
-    }
-    int rc = ensure_work(p, 8);
-    if (rc) return rc;
-    double *W = p->work[0], *T = p->work[1], *Z = p->work[2];
-    double *V = p->basis;
-    std::vector<cd> coef(2 * nr), dv;
-    std::vector<double> bnorm(nr);
+// ---------------------------------------------------------------------------------- grouped solves
+// A group is one or more assembled problems (frequencies of one mesh or of several) solved
+// together: each step of the Krylov loop is one launch per operation for all of them
+// (ncgk::g_* kernels, one descriptor per problem or problem level), so the operator applies
+// of several frequencies share every launch and the host waits once per step for the
+// whole group, not once per problem.  GMRES runs on the device: Gram-Schmidt passes,
+// Hessenberg columns, Givens rotations and the convergence test (kernels g_gm_*); the host
+// only reads each step's per-problem done flags, kFlagDepth steps behind the launches.
+// The group runs on its leader's stream; the members' streams are joined before and after.
+constexpr int kGroupMax = 64;
+constexpr int kFlagDepth = 2;
+
+// descriptor arrays of one launch sequence, built on the host and uploaded at once into
+// the leader's device buffer through its pinned staging buffer
+struct Stage {
+    ncg_problem *lead;
+    std::vector<char> host;
+    explicit Stage(ncg_problem *l) : lead(l) {}
+    template <typename T>
+    size_t put(const std::vector<T> &v)
     {
-        const double *u[1] = {b}, *v[1] = {b};
-        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
-        for (int r = 0; r < nr; ++r) bnorm[r] = sqrt(dv[3 * r].real());
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + sizeof(T) * std::max<size_t>(v.size(), 1), 0);
+        if (!v.empty()) memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
+        return off;
     }
-    iters.assign(nr, 0);
-    relres.assign(nr, 1.0);
-    std::vector<int> done(nr, 0);
-    int total = 0;
-    bool first = true;
-    while (total < maxit) {
-        // r = b - A x  -> W
-        NCG_HIP(hipMemcpyAsync(W, b, sizeof(cd) * vlen, hipMemcpyDeviceToDevice, p->stream));
-        if (!(first && x_zero)) {
-            mv(p, acc64, x, nullptr, T);
-            for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
-            NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-            ncgk::axpby(p->stream, n, nr, p->scal, T, n, W, n);
-        }
-        first = false;
-        {
-            const double *u[1] = {W}, *v[1] = {W};
-            if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
-        }
-        std::vector<double> beta(nr);
-        bool all_done = true;
-        for (int r = 0; r < nr; ++r) {
-            beta[r] = sqrt(dv[3 * r].real());
-            relres[r] = bnorm[r] > 0 ? beta[r] / bnorm[r] : 0.0;
-            done[r] = relres[r] < tol;
-            if (!done[r]) all_done = false;
-        }
-        if (all_done) break;
-        // V0 = W / beta (the norm is still in p->dotout)
-        ncgk::scale_norm(p->stream, n, nr, p->dotout, W, V);
-        std::vector<std::vector<cd>> H(nr, std::vector<cd>((size_t)(m + 1) * m));
-        std::vector<std::vector<cd>> g(nr, std::vector<cd>(m + 1)), cs(nr, std::vector<cd>(m)), sn(nr, std::vector<cd>(m));
-        std::vector<int> kk(nr, 0);
-        for (int r = 0; r < nr; ++r) g[r][0] = beta[r];
-        // Arnoldi step k on the device: z = M^-1 v_k, w = A z, two Gram-Schmidt passes and the
-        // norm; dotout = [h1: nr*(k+1)] [h2: nr*(k+1)] [norm dots: nr*3] goes to pinned slot
-        // k % 2.  Step k + 1 is launched before step k's column is read back, so the device
-        // is not idle while the host applies the rotations (a converged step leaves one
-        // extra Arnoldi step unused).
-        auto launch_step = [&](int k) -> int {
-            double *Vk = V + 2 * (size_t)k * vlen;
-            if (bj) {
-                bj_apply_vec(p, Vk, Z);
-                mv(p, acc64, Z, nullptr, W);
-            } else {
-                mv(p, acc64, Vk, nullptr, W);
-            }
-            cd *h1 = (cd *)p->dotout, *h2 = h1 + (size_t)nr * (k + 1), *hn = h2 + (size_t)nr * (k + 1);
-            ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h1, p->partial, p->partial_cap, nullptr);
-            ncgk::gs_pass(p->stream, n, nr, k + 1, V, W, h2, p->partial, p->partial_cap, p->npart);
-            ncgk::gs_normalize(p->stream, n, nr, p->npart, W, V + 2 * (size_t)(k + 1) * vlen, hn);
-            const size_t ncol = 2 * (size_t)nr * (k + 1) + 3 * (size_t)nr;
-            NCG_HIP(hipMemcpyAsync(p->hpin + (size_t)(k & 1) * p->hpin_slot, h1, sizeof(cd) * ncol,
-                                   hipMemcpyDeviceToHost, p->stream));
-            NCG_HIP(hipEventRecord(p->hev[k & 1], p->stream));
-            return 0;
-        };
-        const size_t slot = 2 * (size_t)nr * (m + 1) + 3 * (size_t)nr;
-        if (slot > p->hpin_slot) {
-            if (p->hpin) hipHostFree(p->hpin);
-            p->hpin = nullptr;
-            p->hpin_slot = 0;
-            hipError_t e = hipHostMalloc((void **)&p->hpin, sizeof(cd) * 2 * slot, hipHostMallocDefault);
-            if (e != hipSuccess) return fail(NCG_E_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-            p->hpin_slot = slot;
-        }
-        for (hipEvent_t &e : p->hev)
-            if (!e) NCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        int k;
-        if (total < maxit && (rc = launch_step(0))) return rc;
-        for (k = 0; k < m && total < maxit; ++k, ++total) {
-            if (k + 1 < m && total + 1 < maxit && (rc = launch_step(k + 1))) return rc;
-            NCG_HIP(hipEventSynchronize(p->hev[k & 1]));
-            const cd *hcol = p->hpin + (size_t)(k & 1) * p->hpin_slot;
-            bool all = true;
-            for (int r = 0; r < nr; ++r) {
-                const double hnr = sqrt(hcol[2 * (size_t)nr * (k + 1) + 3 * r].real());
-                if (done[r]) continue;
-                std::vector<cd> &Hr = H[r];
-                for (int i = 0; i <= k; ++i)
-                    Hr[(size_t)i * m + k] = hcol[(size_t)r * (k + 1) + i] + hcol[(size_t)nr * (k + 1) + (size_t)r * (k + 1) + i];
-                Hr[(size_t)(k + 1) * m + k] = hnr;
-                for (int i = 0; i < k; ++i) {   // apply previous rotations
-                    const cd a = Hr[(size_t)i * m + k], bb = Hr[(size_t)(i + 1) * m + k];
-                    Hr[(size_t)i * m + k] = std::conj(cs[r][i]) * a + std::conj(sn[r][i]) * bb;
-                    Hr[(size_t)(i + 1) * m + k] = -sn[r][i] * a + cs[r][i] * bb;
-                }
-                const cd a = Hr[(size_t)k * m + k], bb = Hr[(size_t)(k + 1) * m + k];
-                const double den = sqrt(std::norm(a) + std::norm(bb));
-                cs[r][k] = den > 0 ? a / den : cd(1, 0);
-                sn[r][k] = den > 0 ? bb / den : cd(0, 0);
-                Hr[(size_t)k * m + k] = den;
-                Hr[(size_t)(k + 1) * m + k] = 0.0;
-                g[r][k + 1] = -sn[r][k] * g[r][k];
-                g[r][k] = std::conj(cs[r][k]) * g[r][k];
-                kk[r] = k + 1;
-                iters[r]++;
-                relres[r] = bnorm[r] > 0 ? std::abs(g[r][k + 1]) / bnorm[r] : 0.0;
-                if (relres[r] < tol) done[r] = 1;
-                if (!done[r]) all = false;
-            }
-            if (all) { ++total; ++k; break; }
-        }
-        // y = H^-1 g per load case; x += M^-1 V y
-        const int kmax = *std::max_element(kk.begin(), kk.end());
-        std::vector<cd> y((size_t)nr * std::max(kmax, 1), cd(0, 0));
-        for (int r = 0; r < nr; ++r) {
-            for (int i = kk[r] - 1; i >= 0; --i) {
-                cd s = g[r][i];
-                for (int c = i + 1; c < kk[r]; ++c) s -= H[r][(size_t)i * m + c] * y[(size_t)r * kmax + c];
-                y[(size_t)r * kmax + i] = s / H[r][(size_t)i * m + i];
-            }
-        }
-        if (kmax > 0) {
-            NCG_HIP(hipMemcpyAsync(p->dotout, y.data(), sizeof(cd) * nr * kmax, hipMemcpyHostToDevice, p->stream));
-            if (bj) {
-                NCG_HIP(hipMemsetAsync(Z, 0, sizeof(cd) * vlen, p->stream));
-                ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, Z, 1);
-                bj_apply_vec(p, Z, T);
-                for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(1, 0); }
-                NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-                ncgk::axpby(p->stream, n, nr, p->scal, T, n, x, n);
-            } else {
-                ncgk::multiaxpy(p->stream, n, nr, kmax, V, p->dotout, x, 1);
-            }
-        }
-        bool fin = true;
-        for (int r = 0; r < nr; ++r) if (!done[r]) fin = false;
-        if (fin) break;
+    template <typename T>
+    const T *dev(size_t off) const
+    {
+        return (const T *)((const char *)lead->gdesc.p + off);
     }
-    NCG_HIP(hipStreamSynchronize(p->stream));
+    // the previous upload from the staging buffer has completed when the stream is idle
+    int upload(hipStream_t s)
+    {
+        NCG_HIP(hipStreamSynchronize(s));
+        const size_t bytes = std::max<size_t>(host.size(), 256);
+        if (bytes > lead->gstage_cap) {
+            const size_t cap = std::max(bytes, 2 * lead->gstage_cap);
+            if (lead->gstage) hipHostFree(lead->gstage);
+            lead->gstage = nullptr;
+            lead->gstage_cap = 0;
+            NCG_HIP(hipHostMalloc((void **)&lead->gstage, cap, hipHostMallocDefault));
+            lead->gstage_cap = cap;
+        }
+        int rc = lead->gdesc.ensure(bytes);
+        if (rc) return rc;
+        memcpy(lead->gstage, host.data(), host.size());
+        NCG_HIP(hipMemcpyAsync(lead->gdesc.p, lead->gstage, host.size(), hipMemcpyHostToDevice, s));
+        return 0;
+    }
+};
+
+// prefix sums of per-item work-group counts
+std::vector<int> prefix(const std::vector<int> &count)
+{
+    std::vector<int> p(count.size() + 1, 0);
+    for (size_t i = 0; i < count.size(); ++i) p[i + 1] = p[i] + count[i];
+    return p;
+}
+
+// byte offsets of a member's GMRES state in its gm buffer
+struct GmLayout {
+    size_t H, cs, sn, g, yv, h, sc, ic, skip, total;
+};
+GmLayout gm_layout(int nrhs, int m)
+{
+    GmLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t r = o;
+        o += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    L.H = take(16 * (size_t)nrhs * (m + 1) * m);
+    L.cs = take(16 * (size_t)nrhs * m);
+    L.sn = take(16 * (size_t)nrhs * m);
+    L.g = take(16 * (size_t)nrhs * (m + 1));
+    L.yv = take(16 * (size_t)nrhs * m);
+    L.h = take(16 * ((size_t)2 * nrhs * (m + 1) + 3 * (size_t)nrhs));
+    L.sc = take(8 * 4 * (size_t)nrhs);
+    L.ic = take(4 * 4 * (size_t)nrhs);
+    L.skip = take(4);
+    L.total = o;
+    return L;
+}
+
+// leader's pinned landing area: [kFlagDepth][kGroupMax] flags, [kGroupMax][64] norms,
+// [kGroupMax][64] iterations and residual estimates; device twin in the leader's gm buffer
+// past the member state (gm_out)
+constexpr size_t kPinFlags = 0, kPinNorms = 4096, kPinIts = kPinNorms + 8 * 64 * kGroupMax,
+                 kPinRel = kPinIts + 4 * 64 * kGroupMax, kPinBytes = kPinRel + 8 * 64 * kGroupMax;
+
+int ensure_pin(ncg_problem *p)
+{
+    if (p->gpin) return 0;
+    NCG_HIP(hipHostMalloc(&p->gpin, kPinBytes, hipHostMallocDefault));
+    for (hipEvent_t &e : p->gev)
+        if (!e) NCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return 0;
 }
 
-int solve_gmres(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+// One apply y = A x of every member, as grouped launches (FMM members with at most 4 load
+// cases per pass: near field, then T and D of every level, then S level by level, so each
+// y sums as the single-problem apply does) and single-problem applies (dense operators).
+// sig64: FMM signatures t and s in FP64 (the refinement residual).
+struct ApplyPlan {
+    struct Launch {
+        int kind, cls, nr, seg, nitems, blocks, kid;
+        size_t items, start;
+        double bytes;
+    };
+    std::vector<Launch> seq;
+    std::vector<int> single;   // members applied one by one
+    int acc64 = 1, sig64 = 0;
+};
+
+void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<const double *> &in,
+                const std::vector<double *> &out, const std::vector<int *> &skip, int acc64, int sig64, ApplyPlan &A)
 {
-    const size_t bytes = sizeof(cd) * (size_t)p->nrhs * p->n;
-    int rc;
-    const int bj = o->precond == NCG_PRECOND_BLOCK;
-    if (bj && (rc = bj_prepare(p))) return rc;
-    NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
-    std::vector<int> it;
-    std::vector<double> rr;
-    rc = gmres(p, p->rhs, p->x, o->restart > 0 ? o->restart : 50, o->max_iterations, o->tolerance, 1, bj, true, it,
-               rr);
-    if (rc) return rc;
-    harvest_mv(p);
-    int all = 1;
-    for (int r = 0; r < p->nrhs; ++r) {
-        if (rr[r] >= o->tolerance) all = 0;
-        if (st && r < 64) { st->iterations[r] = it[r]; st->relative_residual[r] = rr[r]; }
+    A.seq.clear();
+    A.single.clear();
+    A.acc64 = acc64;
+    A.sig64 = sig64;
+    int nrhs = 0;
+    std::vector<int> fmm;
+    for (size_t i = 0; i < P.size(); ++i) {
+        if (P[i]->plan.method == 0 || P[i]->nrhs > 4) A.single.push_back((int)i);
+        else fmm.push_back((int)i);
+        nrhs = std::max(nrhs, P[i]->nrhs);
     }
-    if (st) st->converged_all = all;
-    return all ? 0 : fail(NCG_E_NOCONV, "GMRES: maximum number of iterations reached");
+    if (fmm.empty()) return;
+    const int nr = P[fmm[0]]->nrhs;
+    // near field: one launch per value precision
+    for (int cls = 0; cls < 2; ++cls) {
+        std::vector<ncgk::GNear> it;
+        std::vector<int> cnt;
+        int seg = 64;
+        double bytes = 0;
+        for (int i : fmm) {
+            ncg_problem *p = P[i];
+            const DevFmm &F = p->fmm;
+            const int mixed = p->precision == NCG_MIXED;
+            if (mixed != cls) continue;
+            ncgk::GNear g;
+            g.n = p->n;
+            g.nch = F.nd_n;
+            g.cb = F.nd_cb;
+            g.off = F.nd_off;
+            g.w = F.nd_w;
+            g.row = F.nd_row;
+            g.col = F.near_col;
+            g.val = F.near_val.p;
+            g.x = in[i];
+            g.y = out[i];
+            g.skip = skip[i];
+            it.push_back(g);
+            cnt.push_back(F.nd_n);
+            seg = std::max(seg, std::max(64, std::min(32768 / (16 * nr), (F.near_wmax + 63) / 64 * 64)));
+            bytes += (double)F.nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * p->n * 16.0;
+        }
+        if (it.empty()) continue;
+        const std::vector<int> pre = prefix(cnt);
+        A.seq.push_back({0, cls, nr, seg, (int)it.size(), pre.back(), cls ? 1 : 0, st.put(it), st.put(pre), bytes});
+    }
+    // far field: classes 0 FP32 values + FP32 signatures, 1 FP32 values + FP64 signatures,
+    // 2 FP64; T and D of every level in one launch per class, S per level and class
+    size_t maxlev = 0;
+    for (int i : fmm) maxlev = std::max(maxlev, P[i]->fmm.lev.size());
+    auto level_item = [&](int i, size_t l, int &cls) {
+        ncg_problem *p = P[i];
+        DevLevel &L = p->fmm.lev[l];
+        cls = L.f32 ? (sig64 ? 1 : 0) : 2;
+        ncgk::GLevel g;
+        g.n = p->n;
+        g.ncl = L.ncl;
+        g.P = L.P;
+        g.ntl = L.ntl;
+        g.cl_ptr = L.cl_ptr;
+        g.cl_elem = L.cl_elem;
+        g.fan_ptr = L.fan_ptr;
+        g.fan_idx = L.fan_idx;
+        g.tl_cl = L.tl_cl;
+        g.tl_a0 = L.tl_a0;
+        g.tl_m = L.tl_m;
+        g.Tv = L.T.p;
+        g.Dv = L.D.p;
+        g.Sv = L.S.p;
+        g.x = in[i];
+        g.tw = L.tw.p;
+        g.sw = L.sw.p;
+        g.y = out[i];
+        g.skip = skip[i];
+        return g;
+    };
+    for (int kind = 1; kind <= 2; ++kind)
+        for (int cls = 0; cls < 3; ++cls) {
+            std::vector<ncgk::GLevel> it;
+            std::vector<int> cnt;
+            double bytes = 0;
+            for (int i : fmm)
+                for (size_t l = 0; l < P[i]->fmm.lev.size(); ++l) {
+                    int c;
+                    ncgk::GLevel g = level_item(i, l, c);
+                    if (c != cls) continue;
+                    it.push_back(g);
+                    cnt.push_back(g.ncl * ((g.P + 63) / 64));
+                    const double sf = c == 2 ? 16.0 : 8.0, sw = c == 0 ? 8.0 : 16.0, n = g.n, Pp = g.P, ncl = g.ncl;
+                    const double nfan = P[i]->fmm.lev[l].nfan;
+                    bytes += kind == 1 ? n * Pp * sf + nr * n * 16.0 + nr * ncl * Pp * sw
+                                       : nfan * Pp * sf + 2.0 * nr * ncl * Pp * sw;
+                }
+            if (it.empty()) continue;
+            const std::vector<int> pre = prefix(cnt);
+            const int kid = kind == 1 ? (cls == 2 ? 2 : 3) : (cls == 2 ? 4 : 5);
+            A.seq.push_back({kind, cls, nr, 0, (int)it.size(), pre.back(), kid, st.put(it), st.put(pre), bytes});
+        }
+    for (size_t l = 0; l < maxlev; ++l)
+        for (int cls = 0; cls < 3; ++cls) {
+            std::vector<ncgk::GLevel> it;
+            std::vector<int> cnt;
+            double bytes = 0;
+            for (int i : fmm) {
+                if (l >= P[i]->fmm.lev.size()) continue;
+                int c;
+                ncgk::GLevel g = level_item(i, l, c);
+                if (c != cls) continue;
+                it.push_back(g);
+                cnt.push_back(g.ntl);
+                const double sf = c == 2 ? 16.0 : 8.0, sw = c == 0 ? 8.0 : 16.0, n = g.n, Pp = g.P, ncl = g.ncl;
+                bytes += n * Pp * sf + nr * ncl * Pp * sw + 2.0 * nr * n * 16.0;
+            }
+            if (it.empty()) continue;
+            const std::vector<int> pre = prefix(cnt);
+            A.seq.push_back({3, cls, nr, 0, (int)it.size(), pre.back(), cls == 2 ? 6 : 7, st.put(it), st.put(pre), bytes});
+        }
 }
 
-// Mixed precision: FP32 operator; inner GMRES with FP32 matvec accumulation and FP64
-// Arnoldi; outer refinement x += d with the residual b - A x accumulated in FP64.
-int solve_mixed(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
+void run_apply(const Stage &st, ncg_problem *lead, const std::vector<ncg_problem *> &P,
+               const std::vector<const double *> &in, const std::vector<double *> &out, const ApplyPlan &A)
 {
-    const int n = p->n, nr = p->nrhs;
-    const size_t vlen = (size_t)nr * n, bytes = sizeof(cd) * vlen;
-    int rc = ensure_work(p, 10);
-    if (rc) return rc;
-    const int bj = o->precond == NCG_PRECOND_BLOCK;
-    if (bj && (rc = bj_prepare(p))) return rc;
-    double *Rr = p->work[8], *Dd = p->work[9];
-    NCG_HIP(hipMemsetAsync(p->x, 0, bytes, p->stream));
-    NCG_HIP(hipMemcpyAsync(Rr, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
-    std::vector<double> bnorm(nr), rel(nr, 1.0);
-    std::vector<int> total(nr, 0);
-    std::vector<cd> dv, coef(2 * nr);
-    {
-        const double *u[1] = {p->rhs}, *v[1] = {p->rhs};
-        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
-        for (int r = 0; r < nr; ++r) bnorm[r] = sqrt(dv[3 * r].real());
+    hipStream_t s = lead->stream;
+    for (const ApplyPlan::Launch &L : A.seq) {
+        KernelScope ks(lead, L.kid, L.bytes);
+        switch (L.kind) {
+        case 0:
+            ncgk::g_near_spmv(s, L.cls, L.nr, L.seg, st.dev<ncgk::GNear>(L.items), st.dev<int>(L.start), L.nitems,
+                              L.blocks);
+            break;
+        case 1:
+            ncgk::g_apply_T(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
+                            L.nitems, L.blocks);
+            break;
+        case 2:
+            ncgk::g_apply_D(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
+                            L.nitems, L.blocks);
+            break;
+        default:
+            ncgk::g_apply_S(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
+                            L.nitems, L.blocks);
+            break;
+        }
+    }
+    // single-problem applies on the group's stream (the members' streams are the leader's
+    // while the group runs)
+    for (int i : A.single) mv(P[i], A.acc64, in[i], nullptr, out[i]);
+}
+
+// block-Jacobi z = M^-1 x of every member (passes of up to 8 load cases)
+struct BjPlan {
+    struct Launch {
+        int f32, nr, nitems, blocks;
+        size_t items, start;
+    };
+    std::vector<Launch> seq;
+};
+void plan_bj(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<const double *> &x,
+             const std::vector<double *> &y, const std::vector<int *> &skip, BjPlan &B)
+{
+    B.seq.clear();
+    const int nrhs = P[0]->nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += 8) {
+        const int nr = std::min(8, nrhs - r0);
+        for (int f32 = 0; f32 < 2; ++f32) {
+            std::vector<ncgk::GBj> it;
+            std::vector<int> cnt;
+            for (size_t i = 0; i < P.size(); ++i) {
+                ncg_problem *p = P[i];
+                ncgk::GBj g;
+                g.n = p->n;
+                g.nr = nr;
+                if (p->plan.method == 0) {
+                    if (p->dbj_f32 != f32) continue;
+                    g.nblk = p->dbj_nblk;
+                    g.b0 = p->dbj_b0;
+                    g.m = p->dbj_m;
+                    g.elem = p->dbj_ident;
+                    g.boff = p->dbj_boff;
+                    g.minv = p->dbj_blk.p;
+                } else {
+                    const DevFmm &F = p->fmm;
+                    if (F.bj_f32 != f32) continue;
+                    g.nblk = F.nblk;
+                    g.b0 = F.blk_b0;
+                    g.m = F.blk_m;
+                    g.elem = F.lev.back().cl_elem;
+                    g.boff = F.boff;
+                    g.minv = F.bj_blk.p;
+                }
+                g.x = x[i] + 2 * (size_t)r0 * p->n;
+                g.y = y[i] + 2 * (size_t)r0 * p->n;
+                g.skip = skip[i];
+                it.push_back(g);
+                cnt.push_back(g.nblk);
+            }
+            if (it.empty()) continue;
+            const std::vector<int> pre = prefix(cnt);
+            B.seq.push_back({f32, nr, (int)it.size(), pre.back(), st.put(it), st.put(pre)});
+        }
+    }
+}
+void run_bj(const Stage &st, hipStream_t s, const BjPlan &B)
+{
+    for (const BjPlan::Launch &L : B.seq)
+        ncgk::g_bj_apply(s, L.f32, L.nr, st.dev<ncgk::GBj>(L.items), st.dev<int>(L.start), L.nitems, L.blocks);
+}
+
+// vector operation of every member: y = a | a - b | y + a | 0 (complex entries nrhs n)
+struct VecOp {
+    int op = 0, nitems = 0, blocks = 0;
+    size_t items = 0, start = 0;
+};
+VecOp plan_vec(Stage &st, int op, const std::vector<ncg_problem *> &P, const std::vector<const double *> &a,
+               const std::vector<const double *> &b, const std::vector<double *> &y, const std::vector<int *> &skip)
+{
+    std::vector<ncgk::GVec> it;
+    std::vector<int> cnt;
+    for (size_t i = 0; i < P.size(); ++i) {
+        ncgk::GVec g;
+        g.len = (long long)P[i]->nrhs * P[i]->n;
+        g.a = a.empty() ? nullptr : a[i];
+        g.b = b.empty() ? nullptr : b[i];
+        g.y = y[i];
+        g.skip = skip.empty() ? nullptr : skip[i];
+        it.push_back(g);
+        cnt.push_back((int)((g.len + 255) / 256));
+    }
+    const std::vector<int> pre = prefix(cnt);
+    VecOp v;
+    v.op = op;
+    v.nitems = (int)it.size();
+    v.blocks = pre.back();
+    v.items = st.put(it);
+    v.start = st.put(pre);
+    return v;
+}
+void run_vec(const Stage &st, hipStream_t s, const VecOp &v)
+{
+    ncgk::g_vec(s, v.op, st.dev<ncgk::GVec>(v.items), st.dev<int>(v.start), v.nitems, v.blocks);
+}
+
+// squared norms of every member's vectors v into the leader's pinned norms [member][64]
+struct NormOp {
+    int nitems = 0, blocks = 0;
+    size_t items = 0, start = 0;
+};
+NormOp plan_norms(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<const double *> &v)
+{
+    std::vector<ncgk::GNorm> it;
+    std::vector<int> cnt;
+    for (size_t i = 0; i < P.size(); ++i) {
+        ncgk::GNorm g;
+        g.n = P[i]->n;
+        g.nrhs = P[i]->nrhs;
+        g.gsb = std::min(256, (P[i]->n + 255) / 256);
+        g.v = v[i];
+        g.npart = P[i]->npart;
+        g.skip = nullptr;
+        it.push_back(g);
+        cnt.push_back(g.gsb * g.nrhs);
+    }
+    const std::vector<int> pre = prefix(cnt);
+    NormOp o;
+    o.nitems = (int)it.size();
+    o.blocks = pre.back();
+    o.items = st.put(it);
+    o.start = st.put(pre);
+    return o;
+}
+// The leader's gm buffer holds, past its own GMRES state, the device side of the gathers:
+// [kGroupMax][64] norms and residual estimates (doubles), iterations (ints), and
+// [kFlagDepth][kGroupMax] flags.
+constexpr size_t kTailNorms = 0, kTailRel = 8 * 64 * kGroupMax, kTailIts = 16 * 64 * kGroupMax,
+                 kTailFlags = 20 * 64 * kGroupMax, kLeadTail = kTailFlags + 4 * kFlagDepth * kGroupMax;
+char *lead_tail(ncg_problem *lead) { return (char *)lead->gm.p + lead->gm.cap - kLeadTail; }
+
+// runs the norms and waits for them: out[i][r] = |v_i,r|
+int run_norms(const Stage &st, ncg_problem *lead, const NormOp &o, std::vector<std::vector<double>> &out,
+              const std::vector<ncg_problem *> &P)
+{
+    hipStream_t s = lead->stream;
+    double *dev = (double *)(lead_tail(lead) + kTailNorms);
+    ncgk::g_norms(s, st.dev<ncgk::GNorm>(o.items), st.dev<int>(o.start), o.nitems, o.blocks, dev);
+    double *pin = (double *)((char *)lead->gpin + kPinNorms);
+    NCG_HIP(hipMemcpyAsync(pin, dev, 8 * 64 * (size_t)o.nitems, hipMemcpyDeviceToHost, s));
+    NCG_HIP(hipStreamSynchronize(s));
+    out.assign(P.size(), std::vector<double>());
+    for (size_t i = 0; i < P.size(); ++i) {
+        out[i].resize(P[i]->nrhs);
+        for (int r = 0; r < P[i]->nrhs; ++r) out[i][r] = sqrt(pin[64 * i + r]);
+    }
+    return 0;
+}
+
+
+// Restarted GMRES(m) of every member on its system A x = b (b, x: device [nrhs][n]), x
+// starting from zero, per member relative tolerance tol[i]; bj: right block-Jacobi
+// preconditioning (x = M^-1 u).  acc64 selects the dense matvec accumulation (FP32 operator:
+// 0 = FP32 inner solve).  iters / relres per member and load case.
+int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const double *> &b,
+                const std::vector<double *> &x, const std::vector<double> &tol, int restart, int maxit, int acc64,
+                int bj, std::vector<std::vector<int>> &iters, std::vector<std::vector<double>> &relres)
+{
+    const int G = (int)P.size();
+    ncg_problem *lead = P[0];
+    hipStream_t s = lead->stream;
+    const int m = std::max(2, std::min(restart, 200));
+    int rc;
+    if ((rc = ensure_pin(lead))) return rc;
+    std::vector<GmLayout> lay(G);
+    for (int i = 0; i < G; ++i) {
+        ncg_problem *p = P[i];
+        const size_t vlen = (size_t)p->nrhs * p->n, need = (size_t)(m + 1) * vlen * 2;
+        if (need > p->basis_cap) {
+            NCG_HIP(hipStreamSynchronize(s));
+            hipFree(p->basis);
+            p->basis = nullptr;
+            p->basis_cap = 0;
+            if ((rc = dalloc(&p->basis, need))) return rc;
+            p->basis_cap = need;
+        }
+        if ((rc = ensure_work(p, 10))) return rc;
+        // multidot partials: at least one slice per basis vector and load case
+        if ((size_t)(m + 1) * p->nrhs > p->partial_cap) {
+            NCG_HIP(hipStreamSynchronize(s));
+            hipFree(p->partial);
+            p->partial = nullptr;
+            p->partial_cap = 0;
+            if ((rc = dalloc(&p->partial, 2 * (size_t)(m + 1) * p->nrhs))) return rc;
+            p->partial_cap = (size_t)(m + 1) * p->nrhs;
+        }
+        lay[i] = gm_layout(p->nrhs, m);
+        if ((rc = p->gm.ensure(lay[i].total + (i == 0 ? kLeadTail : 0)))) return rc;
+    }
+    char *tail = lead_tail(lead);
+    int *dflags = (int *)(tail + kTailFlags), *dits = (int *)(tail + kTailIts);
+    double *drel = (double *)(tail + kTailRel);
+    // descriptors
+    Stage st(lead);
+    std::vector<ncgk::GGm> gi(G);
+    std::vector<int> c_md(G), c_gs(G), c_vec(G);
+    std::vector<int *> skip(G);
+    std::vector<const double *> Zc(G), Wc(G), xc(G), bc(G);
+    std::vector<double *> Z(G), W(G), T(G), xv(G);
+    for (int i = 0; i < G; ++i) {
+        ncg_problem *p = P[i];
+        const GmLayout &L = lay[i];
+        char *base = (char *)p->gm.p;
+        ncgk::GGm &g = gi[i];
+        g.n = p->n;
+        g.nrhs = p->nrhs;
+        g.gsb = std::min(256, (p->n + 255) / 256);
+        g.nsl = (int)std::max<size_t>(
+            1, std::min<size_t>((size_t)std::max(1, (p->n + 1023) / 1024), p->partial_cap / ((size_t)(m + 1) * p->nrhs)));
+        g.slice = (p->n + g.nsl - 1) / g.nsl;
+        g.b = b[i];
+        g.x = x[i];
+        g.V = p->basis;
+        g.W = W[i] = p->work[0];
+        g.T = T[i] = p->work[1];
+        g.Z = Z[i] = p->work[2];
+        g.part = p->partial;
+        g.h = base + L.h;
+        g.npart = p->npart;
+        g.H = base + L.H;
+        g.cs = base + L.cs;
+        g.sn = base + L.sn;
+        g.g = base + L.g;
+        g.yv = base + L.yv;
+        g.sc = (double *)(base + L.sc);
+        g.ic = (int *)(base + L.ic);
+        g.skip = skip[i] = (int *)(base + L.skip);
+        g.tol = tol[i];
+        c_md[i] = p->nrhs * g.nsl;
+        c_gs[i] = g.gsb * p->nrhs;
+        c_vec[i] = (int)(((size_t)p->nrhs * p->n + 255) / 256);
+        Zc[i] = Z[i];
+        Wc[i] = W[i];
+        xc[i] = x[i];
+        bc[i] = b[i];
+        xv[i] = x[i];
+    }
+    const size_t o_gm = st.put(gi);
+    const std::vector<int> p_md = prefix(c_md), p_gs = prefix(c_gs), p_vec = prefix(c_vec);
+    const size_t o_md = st.put(p_md), o_gs = st.put(p_gs), o_vec = st.put(p_vec);
+    const std::vector<int *> noskip(G, nullptr);
+    const VecOp v_zero = plan_vec(st, ncgk::kVecZero, P, {}, {}, xv, {});
+    const VecOp v_w_b = plan_vec(st, ncgk::kVecCopy, P, bc, {}, W, {});
+    const VecOp v_w_res = plan_vec(st, ncgk::kVecSub, P, bc, std::vector<const double *>(T.begin(), T.end()), W, skip);
+    const VecOp v_x_add = plan_vec(st, ncgk::kVecAdd, P, std::vector<const double *>(T.begin(), T.end()), {}, xv, {});
+    // per Arnoldi step k: Z = M^-1 V_k (or Z = V_k), then W = A Z
+    std::vector<BjPlan> bjk(bj ? m : 0);
+    std::vector<VecOp> cpk(bj ? 0 : m);
+    for (int k = 0; k < m; ++k) {
+        std::vector<const double *> Vk(G);
+        for (int i = 0; i < G; ++i) Vk[i] = P[i]->basis + 2 * (size_t)k * P[i]->nrhs * P[i]->n;
+        if (bj) plan_bj(st, P, Vk, Z, skip, bjk[k]);
+        else cpk[k] = plan_vec(st, ncgk::kVecCopy, P, Vk, {}, Z, skip);
+    }
+    BjPlan bj_end;
+    if (bj) plan_bj(st, P, Zc, T, noskip, bj_end);
+    ApplyPlan ap_step, ap_res;
+    plan_apply(st, P, Zc, W, skip, acc64, 0, ap_step);
+    plan_apply(st, P, xc, T, skip, acc64, 0, ap_res);
+    if ((rc = st.upload(s))) return rc;
+    const ncgk::GGm *dgm = st.dev<ncgk::GGm>(o_gm);
+    const int *d_md = st.dev<int>(o_md), *d_gs = st.dev<int>(o_gs), *d_vec = st.dev<int>(o_vec);
+    int *pin_flags = (int *)((char *)lead->gpin + kPinFlags);
+
+    run_vec(st, s, v_zero);
+    ncgk::g_gm_clear(s, dgm, G);
+    int total = 0;
+    bool first = true, all_done = false;
+    std::vector<int> flags(G, 0);
+    while (true) {
+        if (first) {
+            run_vec(st, s, v_w_b);
+        } else {
+            run_apply(st, lead, P, xc, T, ap_res);
+            run_vec(st, s, v_w_res);
+        }
+        // |W|, V_0 = W / |W|, beta and the done flags of this cycle
+        ncgk::g_gs_update(s, 0, 1, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gs_normalize(s, -1, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gm_start(s, m, first ? 1 : 0, dgm, G);
+        first = false;
+        // Arnoldi steps; the flags of step k are read after step k + kFlagDepth - 1 is queued
+        int launched = 0, checked = 0;
+        auto check = [&](int step) -> int {
+            NCG_HIP(hipEventSynchronize(lead->gev[step % kFlagDepth]));
+            const int *f = pin_flags + (step % kFlagDepth) * kGroupMax;
+            all_done = true;
+            for (int i = 0; i < G; ++i) {
+                flags[i] = f[i];
+                if (!f[i]) all_done = false;
+            }
+            return 0;
+        };
+        // the flags after the cycle start
+        ncgk::g_gm_flags(s, dgm, G, dflags);
+        NCG_HIP(hipMemcpyAsync(pin_flags + (kFlagDepth - 1) * kGroupMax, dflags, 4 * G, hipMemcpyDeviceToHost, s));
+        NCG_HIP(hipEventRecord(lead->gev[kFlagDepth - 1], s));
+        if ((rc = check(kFlagDepth - 1))) return rc;
+        for (int k = 0; k < m && total < maxit && !all_done; ++k) {
+            if (bj) run_bj(st, s, bjk[k]);
+            else run_vec(st, s, cpk[k]);
+            run_apply(st, lead, P, Zc, W, ap_step);
+            ncgk::g_multidot(s, k + 1, dgm, d_md, G, (k + 1) * p_md.back());
+            ncgk::g_gs_update(s, k + 1, 0, m, dgm, d_gs, G, p_gs.back());
+            ncgk::g_multidot(s, k + 1, dgm, d_md, G, (k + 1) * p_md.back());
+            ncgk::g_gs_update(s, k + 1, 1, m, dgm, d_gs, G, p_gs.back());
+            ncgk::g_gs_normalize(s, k, m, dgm, d_gs, G, p_gs.back());
+            ncgk::g_gm_rotate(s, k, m, dgm, G);
+            ncgk::g_gm_flags(s, dgm, G, dflags + (k % kFlagDepth) * kGroupMax);
+            NCG_HIP(hipMemcpyAsync(pin_flags + (k % kFlagDepth) * kGroupMax, dflags + (k % kFlagDepth) * kGroupMax,
+                                   4 * G, hipMemcpyDeviceToHost, s));
+            NCG_HIP(hipEventRecord(lead->gev[k % kFlagDepth], s));
+            ++total;
+            launched = k + 1;
+            if (launched - checked >= kFlagDepth) {
+                if ((rc = check(checked))) return rc;
+                ++checked;
+            }
+        }
+        while (!all_done && checked < launched) {
+            if ((rc = check(checked))) return rc;
+            ++checked;
+        }
+        // x += M^-1 V y (or V y)
+        ncgk::g_gm_solve(s, m, dgm, G);
+        if (bj) {
+            ncgk::g_gm_update(s, m, 1, dgm, d_vec, G, p_vec.back());
+            run_bj(st, s, bj_end);
+            run_vec(st, s, v_x_add);
+        } else {
+            ncgk::g_gm_update(s, m, 0, dgm, d_vec, G, p_vec.back());
+        }
+        if (all_done || total >= maxit) break;
+    }
+    // iterations and residual estimates
+    int *pin_its = (int *)((char *)lead->gpin + kPinIts);
+    double *pin_rel = (double *)((char *)lead->gpin + kPinRel);
+    ncgk::g_gm_stats(s, dgm, G, dits, drel);
+    NCG_HIP(hipMemcpyAsync(pin_its, dits, 4 * 64 * (size_t)G, hipMemcpyDeviceToHost, s));
+    NCG_HIP(hipMemcpyAsync(pin_rel, drel, 8 * 64 * (size_t)G, hipMemcpyDeviceToHost, s));
+    NCG_HIP(hipStreamSynchronize(s));
+    NCG_HIP(hipGetLastError());
+    iters.assign(G, std::vector<int>());
+    relres.assign(G, std::vector<double>());
+    for (int i = 0; i < G; ++i)
+        for (int r = 0; r < P[i]->nrhs; ++r) {
+            iters[i].push_back(pin_its[64 * i + r]);
+            relres[i].push_back(pin_rel[64 * i + r]);
+        }
+    return 0;
+}
+
+// Mixed precision for every member: FP32 operator, inner GMRES with FP32 matvec
+// accumulation and FP64 Arnoldi, outer refinement x += d with the residual b - A x applied
+// with FP64 accumulation and, on FMM levels, FP64 signatures.  tol of the inner solve per
+// member: what its worst load case still needs (a factor 2 to spare), at most 1/2 and not
+// below 1e-5.  iters / relres per member and load case (relres: the refinement residual).
+int mixed_group(const std::vector<ncg_problem *> &P, const ncg_solver_options *o, int bj,
+                std::vector<std::vector<int>> &iters, std::vector<std::vector<double>> &relres, std::vector<int> &steps)
+{
+    const int G = (int)P.size();
+    ncg_problem *lead = P[0];
+    hipStream_t s = lead->stream;
+    int rc;
+    for (ncg_problem *p : P)
+        if ((rc = ensure_work(p, 10))) return rc;
+    if ((rc = ensure_pin(lead)) || (rc = lead->gm.ensure(gm_layout(lead->nrhs, std::max(2, std::min(o->restart > 0 ? o->restart : 50, 200))).total + kLeadTail)))
+        return rc;
+    std::vector<const double *> rhs(G), Rc(G), xc(G), Dc(G);
+    std::vector<double *> R(G), D(G), X(G), T(G);
+    for (int i = 0; i < G; ++i) {
+        rhs[i] = P[i]->rhs;
+        Rc[i] = R[i] = P[i]->work[8];
+        Dc[i] = D[i] = P[i]->work[9];
+        xc[i] = X[i] = P[i]->x;
+        T[i] = P[i]->work[1];
     }
     const double target = o->refinement_tolerance > 0 ? o->refinement_tolerance : o->tolerance;
-    const int steps = o->refinement_steps > 0 ? o->refinement_steps : 5;
-    int s;
-    for (s = 0; s < steps; ++s) {
-        const double *u[1] = {Rr}, *v[1] = {Rr};
-        if ((rc = dotv(p, 1, u, v, 1, dv))) return rc;
-        bool ok = true;
-        double worst = 0.0;
-        for (int r = 0; r < nr; ++r) {
-            rel[r] = bnorm[r] > 0 ? sqrt(dv[3 * r].real()) / bnorm[r] : 0.0;
-            if (rel[r] >= target) ok = false;
-            worst = std::max(worst, rel[r]);
+    const int nsteps = o->refinement_steps > 0 ? o->refinement_steps : 5;
+    std::vector<std::vector<double>> bnorm, rn;
+    iters.assign(G, std::vector<int>());
+    relres.assign(G, std::vector<double>());
+    steps.assign(G, 0);
+    for (int i = 0; i < G; ++i) {
+        iters[i].assign(P[i]->nrhs, 0);
+        relres[i].assign(P[i]->nrhs, 1.0);
+    }
+    {
+        Stage st(lead);
+        const VecOp x0 = plan_vec(st, ncgk::kVecZero, P, {}, {}, X, {});
+        const VecOp r0 = plan_vec(st, ncgk::kVecCopy, P, rhs, {}, R, {});
+        const NormOp nb = plan_norms(st, P, rhs);
+        if ((rc = st.upload(s))) return rc;
+        run_vec(st, s, x0);
+        run_vec(st, s, r0);
+        if ((rc = run_norms(st, lead, nb, bnorm, P))) return rc;
+    }
+    std::vector<int> active;
+    for (int step = 0; step < nsteps + 1; ++step) {
+        // relative refinement residual of every member still refining
+        std::vector<ncg_problem *> A;
+        std::vector<int> idx;
+        {
+            Stage st(lead);
+            std::vector<ncg_problem *> all(P);
+            const NormOp nr = plan_norms(st, all, Rc);
+            if ((rc = st.upload(s))) return rc;
+            if ((rc = run_norms(st, lead, nr, rn, all))) return rc;
         }
-        if (ok) break;
-        // inner solve A d = r in FP32 matvecs, relative to r: what the worst load case still
-        // needs (with a factor 2 to spare), at most 1/2 and not below 1e-5 (the FP32
-        // operator's accuracy)
-        NCG_HIP(hipMemsetAsync(Dd, 0, bytes, p->stream));
-        std::vector<int> it;
-        std::vector<double> rr;
-        const double inner = std::min(0.5, std::max(1e-5, 0.5 * target / worst));
-        if ((rc = gmres(p, Rr, Dd, o->restart > 0 ? o->restart : 50, o->max_iterations, inner, 0, bj, true, it, rr)))
+        std::vector<double> tol;
+        for (int i = 0; i < G; ++i) {
+            if (step > 0 && std::find(active.begin(), active.end(), i) == active.end()) continue;
+            double worst = 0.0;
+            bool ok = true;
+            for (int r = 0; r < P[i]->nrhs; ++r) {
+                relres[i][r] = bnorm[i][r] > 0 ? rn[i][r] / bnorm[i][r] : 0.0;
+                if (!(relres[i][r] < target)) ok = false;
+                worst = std::max(worst, relres[i][r]);
+            }
+            if (ok || step == nsteps) continue;
+            A.push_back(P[i]);
+            idx.push_back(i);
+            tol.push_back(std::min(0.5, std::max(1e-5, 0.5 * target / worst)));
+        }
+        active = idx;
+        if (A.empty()) break;
+        std::vector<const double *> bA;
+        std::vector<double *> DA, TA, XA;
+        std::vector<const double *> DAc, XAc;
+        for (int i : idx) {
+            bA.push_back(Rc[i]);
+            DA.push_back(D[i]);
+            DAc.push_back(Dc[i]);
+            TA.push_back(T[i]);
+            XA.push_back(X[i]);
+            XAc.push_back(xc[i]);
+            steps[i]++;
+        }
+        std::vector<std::vector<int>> it;
+        std::vector<std::vector<double>> rr;
+        if ((rc = gmres_group(A, bA, DA, tol, o->restart > 0 ? o->restart : 50, o->max_iterations, 0, bj, it, rr)))
             return rc;
-        for (int r = 0; r < nr; ++r) total[r] += it[r];
-        // x += d ; r = b - A x (FP64 accumulation)
-        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(1, 0); coef[2 * r + 1] = cd(1, 0); }
-        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-        ncgk::axpby(p->stream, n, nr, p->scal, Dd, n, p->x, n);
-        mv(p, 1, p->x, nullptr, Dd);
-        NCG_HIP(hipMemcpyAsync(Rr, p->rhs, bytes, hipMemcpyDeviceToDevice, p->stream));
-        for (int r = 0; r < nr; ++r) { coef[2 * r] = cd(-1, 0); coef[2 * r + 1] = cd(1, 0); }
-        NCG_HIP(hipMemcpyAsync(p->scal, coef.data(), sizeof(cd) * 2 * nr, hipMemcpyHostToDevice, p->stream));
-        ncgk::axpby(p->stream, n, nr, p->scal, Dd, n, Rr, n);
+        for (size_t a = 0; a < idx.size(); ++a)
+            for (int r = 0; r < P[idx[a]]->nrhs; ++r) iters[idx[a]][r] += it[a][r];
+        // x += d ; r = b - A x with FP64 accumulation and signatures
+        Stage st(lead);
+        std::vector<const double *> rhsA, TAc(TA.begin(), TA.end());
+        std::vector<double *> RA;
+        for (int i : idx) {
+            rhsA.push_back(rhs[i]);
+            RA.push_back(R[i]);
+        }
+        const VecOp addd = plan_vec(st, ncgk::kVecAdd, A, DAc, {}, XA, {});
+        const VecOp res = plan_vec(st, ncgk::kVecSub, A, rhsA, TAc, RA, {});
+        ApplyPlan ap;
+        plan_apply(st, A, XAc, TA, std::vector<int *>(A.size(), nullptr), 1, 1, ap);
+        if ((rc = st.upload(s))) return rc;
+        run_vec(st, s, addd);
+        run_apply(st, lead, A, XAc, TA, ap);
+        run_vec(st, s, res);
     }
-    harvest_mv(p);
+    return 0;
+}
+
+// Joins the members' streams into the leader's for the duration of a group solve: the
+// leader waits for what is queued on each member, the members' launches go to the leader's
+// stream, and afterwards each member's stream waits for the group.
+struct GroupStreams {
+    std::vector<ncg_problem *> P;
+    std::vector<hipStream_t> saved;
+    explicit GroupStreams(const std::vector<ncg_problem *> &p) : P(p), saved(p.size(), nullptr) {}
+    int join()
+    {
+        for (size_t i = 1; i < P.size(); ++i) {
+            if (!P[i]->join) NCG_HIP(hipEventCreateWithFlags(&P[i]->join, hipEventDisableTiming));
+            NCG_HIP(hipEventRecord(P[i]->join, P[i]->stream));
+            NCG_HIP(hipStreamWaitEvent(P[0]->stream, P[i]->join, 0));
+            saved[i] = P[i]->stream;
+            P[i]->stream = P[0]->stream;
+        }
+        return 0;
+    }
+    void leave()
+    {
+        if (P.size() < 2) return;
+        if (!P[0]->join && hipEventCreateWithFlags(&P[0]->join, hipEventDisableTiming) != hipSuccess) {
+            hipStreamSynchronize(P[0]->stream);
+            for (size_t i = 1; i < P.size(); ++i)
+                if (saved[i]) P[i]->stream = saved[i];
+            return;
+        }
+        hipEventRecord(P[0]->join, P[0]->stream);
+        for (size_t i = 1; i < P.size(); ++i)
+            if (saved[i]) {
+                P[i]->stream = saved[i];
+                hipStreamWaitEvent(saved[i], P[0]->join, 0);
+            }
+    }
+};
+
+// The grouped solves are GMRES: the FP64 operator's, or the mixed form's FP64 refinement
+// around FP32 GMRES; both take the block-Jacobi or no preconditioner.
+int group_options(int precision, const ncg_solver_options *o)
+{
+    if (precision == NCG_MIXED && o->solver != NCG_SOLVER_GMRES)
+        return fail(NCG_E_ARG, "the mixed-precision form solves with GMRES (NCG_SOLVER_GMRES)");
+    if (o->solver != NCG_SOLVER_GMRES) return fail(NCG_E_ARG, "a group solve runs GMRES (NCG_SOLVER_GMRES)");
+    if (o->precond != NCG_PRECOND_BLOCK && o->precond != NCG_PRECOND_NONE)
+        return fail(NCG_E_ARG, "GMRES and the mixed-precision form take the block-Jacobi or no preconditioner");
+    return 0;
+}
+
+// GMRES (FP64 operator) or mixed-precision refinement of every member of a group.
+int solve_group(const std::vector<ncg_problem *> &P, const ncg_solver_options *o, ncg_solve_stats *st)
+{
+    const int G = (int)P.size();
+    ncg_problem *lead = P[0];
+    const int bj = o->precond == NCG_PRECOND_BLOCK;
+    GroupStreams gs(P);
+    int rc = gs.join();
+    if (rc) return rc;
+    record(lead, lead->ev_solve.a);
+    std::vector<std::vector<int>> it;
+    std::vector<std::vector<double>> rr;
+    std::vector<int> rsteps(G, 0);
+    for (ncg_problem *p : P)
+        if (bj && !rc) rc = bj_prepare(p);
+    const bool mixed = lead->precision == NCG_MIXED;
+    double tol = o->tolerance;
+    if (!rc) {
+        if (mixed) {
+            rc = mixed_group(P, o, bj, it, rr, rsteps);
+            tol = o->refinement_tolerance > 0 ? o->refinement_tolerance : o->tolerance;
+        } else {
+            std::vector<const double *> b(G);
+            std::vector<double *> x(G);
+            for (int i = 0; i < G; ++i) {
+                b[i] = P[i]->rhs;
+                x[i] = P[i]->x;
+            }
+            rc = gmres_group(P, b, x, std::vector<double>(G, o->tolerance), o->restart > 0 ? o->restart : 50,
+                             o->max_iterations, 1, bj, it, rr);
+        }
+    }
+    record(lead, lead->ev_solve.b);
+    for (ncg_problem *p : P) harvest_mv(p);
+    gs.leave();
+    if (rc) return rc;
+    const double ms = elapsed(lead->ev_solve);
     int all = 1;
-    for (int r = 0; r < nr; ++r) {
-        if (rel[r] >= target) all = 0;
-        if (st && r < 64) { st->iterations[r] = total[r]; st->relative_residual[r] = rel[r]; }
+    for (int i = 0; i < G; ++i) {
+        ncg_problem *p = P[i];
+        int conv = 1;
+        for (int r = 0; r < p->nrhs; ++r)
+            if (!(rr[i][r] < tol)) conv = 0;
+        if (!conv) all = 0;
+        p->t_solve = ms;
+        p->solved = 1;
+        if (st) {
+            ncg_solve_stats &s = st[i];
+            memset(&s, 0, sizeof(s));
+            for (int r = 0; r < p->nrhs && r < 64; ++r) {
+                s.iterations[r] = it[i][r];
+                s.relative_residual[r] = rr[i][r];
+            }
+            s.converged_all = conv;
+            s.refinement_steps = rsteps[i];
+            s.ms_assemble = p->t_asm;
+            s.ms_solve = ms;
+            s.ms_field = p->t_field;
+        }
     }
-    if (st) { st->converged_all = all; st->refinement_steps = s; }
-    return all ? 0 : fail(NCG_E_NOCONV, "mixed-precision refinement did not reach its tolerance");
+    if (all) return 0;
+    return fail(NCG_E_NOCONV, mixed ? "mixed-precision refinement did not reach its tolerance"
+                                    : "GMRES: maximum number of iterations reached");
 }
 
 }  // namespace
@@ -1885,6 +2483,13 @@ int ncg_problem_destroy(ncg_problem *p)
     for (hipEvent_t e : p->hev)
         if (e) hipEventDestroy(e);
     if (p->hpin) hipHostFree(p->hpin);
+    p->gm.release();
+    p->gdesc.release();
+    if (p->gstage) hipHostFree(p->gstage);
+    if (p->gpin) hipHostFree(p->gpin);
+    for (hipEvent_t e : p->gev)
+        if (e) hipEventDestroy(e);
+    if (p->join) hipEventDestroy(p->join);
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
     return 0;
@@ -2021,15 +2626,10 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
         p->assembled = 0;   // the operator now holds its factors
         rc = 0;
         if (st) st->converged_all = 1;
-    } else if (p->precision == NCG_MIXED) {
-        // the mixed form is GMRES refinement: reject a method it would not run
-        if (o->solver != NCG_SOLVER_GMRES)
-            return fail(NCG_E_ARG, "the mixed-precision form solves with GMRES (NCG_SOLVER_GMRES)");
-        if (o->precond != NCG_PRECOND_BLOCK && o->precond != NCG_PRECOND_NONE)
-            return fail(NCG_E_ARG, "the mixed-precision form takes the block-Jacobi or no preconditioner");
-        rc = solve_mixed(p, o, st);
-    } else if (o->solver == NCG_SOLVER_GMRES) {
-        rc = solve_gmres(p, o, st);
+    } else if (p->precision == NCG_MIXED || o->solver == NCG_SOLVER_GMRES) {
+        if ((rc = group_options(p->precision, o))) return rc;
+        std::vector<ncg_problem *> one(1, p);
+        return solve_group(one, o, st);
     } else {
         rc = solve_cgs(p, o, st);
     }
@@ -2043,6 +2643,29 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     }
     if (rc == 0 || rc == NCG_E_NOCONV) p->solved = 1;
     return rc;
+}
+
+int ncg_solve_group(ncg_problem *const *probs, int32_t count, const ncg_solver_options *o, ncg_solve_stats *stats)
+{
+    if (!probs || !o || count < 1) return fail(NCG_E_ARG, "ncg_solve_group: null argument or empty group");
+    if (count > kGroupMax) return fail(NCG_E_ARG, "ncg_solve_group: at most 64 problems per group");
+    if (o->max_iterations < 0) return fail(NCG_E_ARG, "ncg_solve_group: negative iteration limit");
+    std::vector<ncg_problem *> P(probs, probs + count);
+    for (int i = 0; i < count; ++i) {
+        ncg_problem *p = P[i];
+        if (!p) return fail(NCG_E_ARG, "ncg_solve_group: null problem");
+        if (!p->assembled) return fail(NCG_E_STATE, "ncg_solve_group: assemble every problem first");
+        if (p->device != P[0]->device) return fail(NCG_E_ARG, "ncg_solve_group: the problems must share a device");
+        if (p->precision != P[0]->precision) return fail(NCG_E_ARG, "ncg_solve_group: the problems must share a precision");
+        if (p->nrhs != P[0]->nrhs) return fail(NCG_E_ARG, "ncg_solve_group: the problems must have equal n_rhs");
+        for (int j = 0; j < i; ++j)
+            if (P[j] == p) return fail(NCG_E_ARG, "ncg_solve_group: a problem appears twice");
+    }
+    int rc = group_options(P[0]->precision, o);
+    if (rc) return rc;
+    NCG_HIP(hipSetDevice(P[0]->device));
+    if (stats) memset(stats, 0, sizeof(*stats) * (size_t)count);
+    return solve_group(P, o, stats);
 }
 
 int ncg_get_solution(ncg_problem *p, double *x)

@@ -496,11 +496,10 @@ __global__ void k_row_scaling_csr(int n, const long long *ptr, const cx<S> *val,
 // GMRES: h[r][k] = <V_k, w_r> for k < kcount (Hermitian).  One block per (k, r, slice of
 // n) writes a partial sum; k_multidot_sum adds the slices in a fixed order.
 // V layout: [basis k][rhs r][n]; w: [r][n]; part: [(k * nrhs + r) * nslice + slice].
-__global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, int slice, const cx<double> *V,
-                                                  const cx<double> *w, cx<double> *part)
+__device__ __forceinline__ void multidot_item(int k, int r, int sl, int nsl, int n, int nrhs, int slice,
+                                              const cx<double> *V, const cx<double> *w, cx<double> *part)
 {
     __shared__ double sre[256], sim[256];
-    const int k = blockIdx.x, r = blockIdx.y, sl = blockIdx.z, nsl = gridDim.z;
     const cx<double> *v = V + ((size_t)k * nrhs + r) * n, *ww = w + (size_t)r * n;
     const int i0 = sl * slice, i1 = min(n, i0 + slice);
     double re = 0.0, im = 0.0;
@@ -533,6 +532,12 @@ __global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, i
     if (threadIdx.x == 0) part[((size_t)k * nrhs + r) * nsl + sl] = mk(sre[0], sim[0]);
 }
 
+__global__ void __launch_bounds__(256) k_multidot(int n, int nrhs, int kcount, int slice, const cx<double> *V,
+                                                  const cx<double> *w, cx<double> *part)
+{
+    multidot_item(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, n, nrhs, slice, V, w, part);
+}
+
 __global__ void k_multidot_sum(int nrhs, int kcount, int nsl, const cx<double> *part, cx<double> *h)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -560,21 +565,22 @@ __global__ void k_multiaxpy(int n, int nrhs, int kcount, const cx<double> *V, co
 // h[r][:] for the host.  With npart, block x also writes its partial sum of |w_new|^2
 // to npart[r][x] (gridDim.x <= 256).
 constexpr int kGsMaxBasis = 256;
-__global__ void __launch_bounds__(256) k_gs_update(int n, int nrhs, int kcount, int nsl, const cx<double> *part,
-                                                   const cx<double> *V, cx<double> *w, cx<double> *hout, double *npart)
+__device__ __forceinline__ void gs_update_item(int bx, int r, int gx, int n, int nrhs, int kcount, int nsl,
+                                               const cx<double> *part, const cx<double> *V, cx<double> *w,
+                                               cx<double> *hout, double *npart)
 {
     __shared__ cx<double> hs[kGsMaxBasis];
     __shared__ double red[256];
-    const int r = blockIdx.y, t = threadIdx.x;
+    const int t = threadIdx.x;
     for (int k = t; k < kcount; k += 256) {
         cx<double> s = mk(0.0, 0.0);
         for (int q = 0; q < nsl; ++q) s += part[((size_t)k * nrhs + r) * nsl + q];
         hs[k] = s;
-        if (blockIdx.x == 0) hout[(size_t)r * kcount + k] = s;
+        if (bx == 0) hout[(size_t)r * kcount + k] = s;
     }
     __syncthreads();
     double nn = 0.0;
-    for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) {
+    for (int i = bx * 256 + t; i < n; i += gx * 256) {
         cx<double> s = w[(size_t)r * n + i];
         for (int k = 0; k < kcount; ++k) s -= V[((size_t)k * nrhs + r) * n + i] * hs[k];
         w[(size_t)r * n + i] = s;
@@ -587,16 +593,22 @@ __global__ void __launch_bounds__(256) k_gs_update(int n, int nrhs, int kcount, 
         if (t < s) red[t] += red[t + s];
         __syncthreads();
     }
-    if (t == 0) npart[(size_t)r * gridDim.x + blockIdx.x] = red[0];
+    if (t == 0) npart[(size_t)r * gx + bx] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_gs_update(int n, int nrhs, int kcount, int nsl, const cx<double> *part,
+                                                   const cx<double> *V, cx<double> *w, cx<double> *hout, double *npart)
+{
+    gs_update_item(blockIdx.x, blockIdx.y, gridDim.x, n, nrhs, kcount, nsl, part, V, w, hout, npart);
 }
 
 // v[r] = w[r] / sqrt(sum_x npart[r][x]) after k_gs_update; block x = 0 stores the
 // squared norm as nout[r][0] (the layout of a dots() result).
-__global__ void __launch_bounds__(256) k_gs_normalize(int n, int nb, const double *npart, const cx<double> *w,
-                                                      cx<double> *v, cx<double> *nout)
+__device__ __forceinline__ void gs_normalize_item(int bx, int r, int gx, int n, int nb, const double *npart,
+                                                  const cx<double> *w, cx<double> *v, cx<double> *nout)
 {
     __shared__ double red[256];
-    const int r = blockIdx.y, t = threadIdx.x;
+    const int t = threadIdx.x;
     red[t] = t < nb ? npart[(size_t)r * nb + t] : 0.0;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -604,10 +616,16 @@ __global__ void __launch_bounds__(256) k_gs_normalize(int n, int nb, const doubl
         __syncthreads();
     }
     const double nrm2 = red[0];
-    if (blockIdx.x == 0 && t == 0) nout[3 * r] = mk(nrm2, 0.0);
+    if (bx == 0 && t == 0) nout[3 * r] = mk(nrm2, 0.0);
     const double sc = sqrt(nrm2);
     const double inv = sc > 1e-300 ? 1.0 / sc : 0.0;
-    for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) v[(size_t)r * n + i] = w[(size_t)r * n + i] * inv;
+    for (int i = bx * 256 + t; i < n; i += gx * 256) v[(size_t)r * n + i] = w[(size_t)r * n + i] * inv;
+}
+
+__global__ void __launch_bounds__(256) k_gs_normalize(int n, int nb, const double *npart, const cx<double> *w,
+                                                      cx<double> *v, cx<double> *nout)
+{
+    gs_normalize_item(blockIdx.x, blockIdx.y, gridDim.x, n, nb, npart, w, v, nout);
 }
 
 // x[r] += sum_k V_k[r] y[r][k]
@@ -1703,14 +1721,14 @@ __global__ void __launch_bounds__(64) k_bj_invert(const int *blk_m, const long l
 // y = M^-1 x: one work-group per block; lane a is row a, the 4 waves split the columns
 // (four per step) of the column-major inverse and meet in LDS
 template <typename TB, int NR>
-__global__ void __launch_bounds__(256) k_bj_apply(int n, int nr, const int *blk_b0, const int *blk_m,
-                                                  const int *cl_elem, const long long *boff, const cx<TB> *minv,
-                                                  const cx<double> *x, cx<double> *y)
+__device__ __forceinline__ void bj_apply_item(int b, int n, int nr, const int *blk_b0, const int *blk_m,
+                                              const int *cl_elem, const long long *boff, const cx<TB> *minv,
+                                              const cx<double> *x, cx<double> *y)
 {
     // nr <= NR load cases are present; the others are zero and not stored
     __shared__ cx<double> xs[NR][kBlk];
     __shared__ double red[3][kBlk][2 * NR];
-    const int b = blockIdx.x, b0 = blk_b0[b], m = blk_m[b];
+    const int b0 = blk_b0[b], m = blk_m[b];
     const int a = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < NR * m; t += 256) {
         const int r = t / m, j = t - r * m;
@@ -1753,6 +1771,14 @@ __global__ void __launch_bounds__(256) k_bj_apply(int n, int nr, const int *blk_
                 y[(size_t)r * n + e] = mk(re[r] + red[0][a][2 * r] + red[1][a][2 * r] + red[2][a][2 * r],
                                           im[r] + red[0][a][2 * r + 1] + red[1][a][2 * r + 1] + red[2][a][2 * r + 1]);
     }
+}
+
+template <typename TB, int NR>
+__global__ void __launch_bounds__(256) k_bj_apply(int n, int nr, const int *blk_b0, const int *blk_m,
+                                                  const int *cl_elem, const long long *boff, const cx<TB> *minv,
+                                                  const cx<double> *x, cx<double> *y)
+{
+    bj_apply_item<TB, NR>(blockIdx.x, n, nr, blk_b0, blk_m, cl_elem, boff, minv, x, y);
 }
 
 // v_r = w_r / sqrt(re <w_r, w_r>) with the norm read from a device dot result [r][3]
@@ -1869,13 +1895,12 @@ __device__ __forceinline__ void rows_dot(const cx<S> *const (&vr)[RW], int sl, c
 // offsets and column list come from one table, so the first loads of a work-group do not
 // depend on each other.
 template <typename S, int NR>
-__global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long long *ch_cb, const int *ch_w,
-                                                   const int *ch_row, const long long *ch_off, const int *col,
-                                                   const cx<S> *val, const cx<double> *x, cx<double> *y)
+__device__ __forceinline__ void near_spmv_item(int q, int n, int seg, const long long *ch_cb, const int *ch_w,
+                                               const int *ch_row, const long long *ch_off, const int *col,
+                                               const cx<S> *val, const cx<double> *x, cx<double> *y)
 {
     extern __shared__ double xs_sh[];   // [NR][seg] complex
     cx<double> *xs = (cx<double> *)xs_sh;
-    const int q = blockIdx.x;
     // the wave index is uniform: readfirstlane lets per-wave indices and loads go scalar
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long cb = ch_cb[q];
@@ -1921,18 +1946,25 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
                 for (int r = 0; r < NR; ++r) y[(size_t)r * n + row[j]] = mk(re[j][r], im[j][r]);
 }
 
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long long *ch_cb, const int *ch_w,
+                                                   const int *ch_row, const long long *ch_off, const int *col,
+                                                   const cx<S> *val, const cx<double> *x, cx<double> *y)
+{
+    near_spmv_item<S, NR>(blockIdx.x, n, seg, ch_cb, ch_w, ch_row, ch_off, col, val, x, y);
+}
+
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
 // of 64 sphere points); its waves take every kSumWaves-th element of the cluster, four
 // per step, and their partial sums meet in LDS.  W is the storage type of t.
 template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr,
-                                                                const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
-                                                                cx<W> *tw)
+__device__ __forceinline__ void apply_T_item(int blk, int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
+                                             const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
 {
     __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
     constexpr int NW = kSumWaves;
     const int tiles = (P + 63) / 64;
-    const int c = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int c = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
@@ -1969,17 +2001,25 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, 
         for (int r = 0; r < NR; ++r) tw[(size_t)r * ncl * P + (size_t)c * P + p] = cx<W>{W(re[r]), W(im[r])};
 }
 
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, int P, const int *cl_ptr,
+                                                                const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
+                                                                cx<W> *tw)
+{
+    apply_T_item<S, W, NR>(blockIdx.x, n, ncl, P, cl_ptr, cl_elem, Tv, x, tw);
+}
+
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
 // per (cluster, tile of 64 points), the waves splitting the interaction list four
 // entries per step.  WI / WO are the storage types of t and s.
 template <typename S, typename WI, typename WO, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
-                                                                const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
+__device__ __forceinline__ void apply_D_item(int blk, int ncl, int P, const int *fan_ptr, const int *fan_idx,
+                                             const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
 {
     __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
     constexpr int NW = kSumWaves;
     const int tiles = (P + 63) / 64;
-    const int i = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const int i = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = tile * 64 + lane;
     const bool on = p < P;
@@ -2022,20 +2062,26 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, 
         for (int r = 0; r < NR; ++r) sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(re[r]), WO(im[r])};
 }
 
+template <typename S, typename WI, typename WO, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
+                                                                const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
+{
+    apply_D_item<S, WI, WO, NR>(blockIdx.x, ncl, P, fan_ptr, fan_idx, Dv, tw, sw);
+}
+
 // y[e] += sum_p S[e, p] s[cluster(e), p].  One work-group of 8 waves per tile of at most
 // 64 elements of one cluster (tl_*: cluster, first local element, elements): lane a is
 // element a0 + a of the cluster's column-major S block, so every load is a contiguous run
 // of the block; the waves split the points, four per step, s[c, p] is wave-uniform, and
 // the partial sums meet in an LDS tree.
 template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, int P, const int *tl_cl,
-                                                                const int *tl_a0, const int *tl_m, const int *cl_ptr,
-                                                                const int *cl_elem, const cx<S> *Sv, const cx<W> *sw,
-                                                                cx<double> *y)
+__device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const int *tl_cl, const int *tl_a0,
+                                             const int *tl_m, const int *cl_ptr, const int *cl_elem, const cx<S> *Sv,
+                                             const cx<W> *sw, cx<double> *y)
 {
     __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
     constexpr int NW = kSumWaves;
-    const int q = blockIdx.x, c = tl_cl[q], a0 = tl_a0[q], m = tl_m[q];
+    const int c = tl_cl[q], a0 = tl_a0[q], m = tl_m[q];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
     const bool on = lane < m;
@@ -2074,6 +2120,15 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, 
 #pragma unroll
         for (int r = 0; r < NR; ++r) y[(size_t)r * n + e] += mk(re[r], im[r]);
     }
+}
+
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, int P, const int *tl_cl,
+                                                                const int *tl_a0, const int *tl_m, const int *cl_ptr,
+                                                                const int *cl_elem, const cx<S> *Sv, const cx<W> *sw,
+                                                                cx<double> *y)
+{
+    apply_S_item<S, W, NR>(blockIdx.x, n, ncl, P, tl_cl, tl_a0, tl_m, cl_ptr, cl_elem, Sv, sw, y);
 }
 
 // ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
@@ -2252,6 +2307,303 @@ __global__ void __launch_bounds__(256) k_fmm_eval_far(int m, int P, int nev, dou
             fphi[(size_t)r * m + x] = mk(v[0], v[1]);
             for (int kk = 0; kk < 3; ++kk) fgrad[((size_t)r * m + x) * 3 + kk] = mk(v[2 + 2 * kk], v[3 + 2 * kk]);
         }
+    }
+}
+
+// ======================================================================================
+// Grouped launches (ncgk::GNear, GLevel, GBj, GVec, GGm): one launch covers the same
+// operation on every problem of a group solved in lockstep.  Work-group b belongs to the
+// item d with start[d] <= b < start[d + 1] (scaled by `scale` where an item's count is a
+// multiple of a launch-wide factor); it runs the single-problem kernel's work-group
+// b - start[d] on that item's arrays, or nothing while the item's skip flag is set.
+// ======================================================================================
+__device__ __forceinline__ int g_item(const int *start, int nitems, int b, int scale = 1)
+{
+    int d = 0;
+    while (d + 1 < nitems && b >= scale * start[d + 1]) ++d;   // wave-uniform
+    return d;
+}
+
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, const int *start, int nitems, int seg)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GNear it = items[d];
+    if (it.skip && *it.skip) return;
+    near_spmv_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cb, it.w, it.row, it.off, it.col, (const cx<S> *)it.val,
+                          (const cx<double> *)it.x, (cx<double> *)it.y);
+}
+
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GLevel it = items[d];
+    if (it.skip && *it.skip) return;
+    apply_T_item<S, W, NR>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem, (const cx<S> *)it.Tv,
+                           (const cx<double> *)it.x, (cx<W> *)it.tw);
+}
+
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_D(const ncgk::GLevel *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GLevel it = items[d];
+    if (it.skip && *it.skip) return;
+    apply_D_item<S, W, W, NR>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx, (const cx<S> *)it.Dv,
+                              (const cx<W> *)it.tw, (cx<W> *)it.sw);
+}
+
+template <typename S, typename W, int NR>
+__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_S(const ncgk::GLevel *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GLevel it = items[d];
+    if (it.skip && *it.skip) return;
+    apply_S_item<S, W, NR>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.tl_cl, it.tl_a0, it.tl_m, it.cl_ptr,
+                           it.cl_elem, (const cx<S> *)it.Sv, (const cx<W> *)it.sw, (cx<double> *)it.y);
+}
+
+template <typename TB, int NR>
+__global__ void __launch_bounds__(256) kg_bj_apply(const ncgk::GBj *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GBj it = items[d];
+    if (it.skip && *it.skip) return;
+    bj_apply_item<TB, NR>(blockIdx.x - start[d], it.n, it.nr, it.b0, it.m, it.elem, it.boff, (const cx<TB> *)it.minv,
+                          (const cx<double> *)it.x, (cx<double> *)it.y);
+}
+
+__global__ void __launch_bounds__(256) kg_vec(int op, const ncgk::GVec *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GVec it = items[d];
+    if (it.skip && *it.skip) return;
+    const long long t = (long long)(blockIdx.x - start[d]) * 256 + threadIdx.x;
+    if (t >= it.len) return;
+    const cx<double> *a = (const cx<double> *)it.a, *b = (const cx<double> *)it.b;
+    cx<double> *y = (cx<double> *)it.y;
+    switch (op) {
+    case ncgk::kVecCopy: y[t] = a[t]; break;
+    case ncgk::kVecSub: y[t] = a[t] - b[t]; break;
+    case ncgk::kVecAdd: y[t] += a[t]; break;
+    default: y[t] = mk(0.0, 0.0); break;
+    }
+}
+
+// per item kcount * nrhs * nsl work-groups (start holds the prefix of nrhs * nsl)
+__global__ void __launch_bounds__(256) kg_multidot(int kcount, const ncgk::GGm *items, const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x, kcount);
+    const ncgk::GGm it = items[d];
+    if (*it.skip) return;
+    const int b = blockIdx.x - kcount * start[d];
+    const int sl = b % it.nsl, t = b / it.nsl, r = t % it.nrhs, k = t / it.nrhs;
+    multidot_item(k, r, sl, it.nsl, it.n, it.nrhs, it.slice, (const cx<double> *)it.V, (const cx<double> *)it.W,
+                  (cx<double> *)it.part);
+}
+
+// per item gsb * nrhs work-groups
+__global__ void __launch_bounds__(256) kg_gs_update(int kcount, int pass, int m, const ncgk::GGm *items,
+                                                    const int *start, int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GGm it = items[d];
+    if (*it.skip) return;
+    const int b = blockIdx.x - start[d];
+    cx<double> *hout = (cx<double> *)it.h + (size_t)pass * it.nrhs * (m + 1);
+    gs_update_item(b % it.gsb, b / it.gsb, it.gsb, it.n, it.nrhs, kcount, it.nsl, (const cx<double> *)it.part,
+                   (const cx<double> *)it.V, (cx<double> *)it.W, hout, pass ? it.npart : nullptr);
+}
+
+__global__ void __launch_bounds__(256) kg_gs_normalize(int k, int m, const ncgk::GGm *items, const int *start,
+                                                       int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GGm it = items[d];
+    if (*it.skip) return;
+    const int b = blockIdx.x - start[d];
+    cx<double> *v = (cx<double> *)it.V + (size_t)(k + 1) * it.nrhs * it.n;
+    cx<double> *nout = (cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
+    gs_normalize_item(b % it.gsb, b / it.gsb, it.gsb, it.n, it.gsb, it.npart, (const cx<double> *)it.W, v, nout);
+}
+
+// one wave per item, lane r = load case; the problem is done once every load case is
+__device__ __forceinline__ void g_set_skip(const ncgk::GGm &it, bool done_r)
+{
+    const unsigned long long open = __ballot(threadIdx.x < (unsigned)it.nrhs && !done_r);
+    if (threadIdx.x == 0) *it.skip = open == 0ull ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(64) kg_gm_start(int m, int first, const ncgk::GGm *items)
+{
+    const ncgk::GGm it = items[blockIdx.x];
+    if (!first && *it.skip) return;   // converged in an earlier cycle: no restart
+    const int r = threadIdx.x;
+    bool dn = true;
+    if (r < it.nrhs) {
+        const cx<double> *hn = (const cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
+        double *sc = it.sc + 4 * r;
+        int *ic = it.ic + 4 * r;
+        const double beta = sqrt(hn[3 * r].re);
+        if (first) {
+            sc[1] = beta;      // the first residual is b (x = 0)
+            ic[2] = 0;
+        }
+        sc[0] = beta;
+        sc[2] = sc[1] > 0.0 ? beta / sc[1] : 0.0;
+        dn = sc[2] < it.tol;   // a NaN residual is not converged
+        ic[0] = 0;
+        ic[1] = dn ? 1 : 0;
+        cx<double> *g = (cx<double> *)it.g + (size_t)r * (m + 1);
+        g[0] = mk(beta, 0.0);
+    }
+    g_set_skip(it, dn);
+}
+
+__global__ void __launch_bounds__(64) kg_gm_rotate(int k, int m, const ncgk::GGm *items)
+{
+    const ncgk::GGm it = items[blockIdx.x];
+    if (*it.skip) return;
+    const int r = threadIdx.x, kc = k + 1;
+    bool dn = true;
+    if (r < it.nrhs) {
+        int *ic = it.ic + 4 * r;
+        double *sc = it.sc + 4 * r;
+        dn = ic[1] != 0;
+        if (!dn) {
+            const cx<double> *h1 = (const cx<double> *)it.h + (size_t)r * kc;
+            const cx<double> *h2 = (const cx<double> *)it.h + (size_t)it.nrhs * (m + 1) + (size_t)r * kc;
+            const cx<double> *hn = (const cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
+            cx<double> *H = (cx<double> *)it.H + (size_t)r * (m + 1) * m;
+            cx<double> *cs = (cx<double> *)it.cs + (size_t)r * m, *sn = (cx<double> *)it.sn + (size_t)r * m;
+            cx<double> *g = (cx<double> *)it.g + (size_t)r * (m + 1);
+            for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = h1[i] + h2[i];
+            H[(size_t)(k + 1) * m + k] = mk(sqrt(hn[3 * r].re), 0.0);
+            for (int i = 0; i < k; ++i) {   // previous rotations
+                const cx<double> a = H[(size_t)i * m + k], bb = H[(size_t)(i + 1) * m + k];
+                const cx<double> ci = mk(cs[i].re, -cs[i].im), si = mk(sn[i].re, -sn[i].im);
+                H[(size_t)i * m + k] = ci * a + si * bb;
+                H[(size_t)(i + 1) * m + k] = mk(0.0, 0.0) - sn[i] * a + cs[i] * bb;
+            }
+            const cx<double> a = H[(size_t)k * m + k], bb = H[(size_t)(k + 1) * m + k];
+            const double den = sqrt(a.re * a.re + a.im * a.im + bb.re * bb.re + bb.im * bb.im);
+            const cx<double> c = den > 0.0 ? a * (1.0 / den) : mk(1.0, 0.0);
+            const cx<double> sv = den > 0.0 ? bb * (1.0 / den) : mk(0.0, 0.0);
+            cs[k] = c;
+            sn[k] = sv;
+            H[(size_t)k * m + k] = mk(den, 0.0);
+            H[(size_t)(k + 1) * m + k] = mk(0.0, 0.0);
+            const cx<double> gk = g[k];
+            g[k + 1] = mk(0.0, 0.0) - sv * gk;
+            g[k] = mk(c.re, -c.im) * gk;
+            ic[0] = k + 1;
+            ic[2] += 1;
+            const double gn = sqrt(g[k + 1].re * g[k + 1].re + g[k + 1].im * g[k + 1].im);
+            sc[2] = sc[1] > 0.0 ? gn / sc[1] : 0.0;
+            dn = sc[2] < it.tol;
+            ic[1] = dn ? 1 : 0;
+        }
+    }
+    g_set_skip(it, dn);
+}
+
+// y = H^-1 g by back substitution over the k rotated columns, zero past k
+__global__ void __launch_bounds__(64) kg_gm_solve(int m, const ncgk::GGm *items)
+{
+    const ncgk::GGm it = items[blockIdx.x];
+    const int r = threadIdx.x;
+    if (r >= it.nrhs) return;
+    const int kk = it.ic[4 * r];
+    const cx<double> *H = (const cx<double> *)it.H + (size_t)r * (m + 1) * m;
+    const cx<double> *g = (const cx<double> *)it.g + (size_t)r * (m + 1);
+    cx<double> *y = (cx<double> *)it.yv + (size_t)r * m;
+    for (int i = m - 1; i >= kk; --i) y[i] = mk(0.0, 0.0);
+    for (int i = kk - 1; i >= 0; --i) {
+        cx<double> s = g[i];
+        for (int c = i + 1; c < kk; ++c) s -= H[(size_t)i * m + c] * y[c];
+        y[i] = cdiv(s, H[(size_t)i * m + i]);
+    }
+}
+
+// Z = V y (to_z) or x += V y over the load case's k basis vectors; per item
+// ceil(nrhs n / 256) work-groups (no skip: a problem that converged in this cycle still
+// takes its update)
+__global__ void __launch_bounds__(256) kg_gm_update(int m, int to_z, const ncgk::GGm *items, const int *start,
+                                                    int nitems)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GGm it = items[d];
+    const long long t = (long long)(blockIdx.x - start[d]) * 256 + threadIdx.x;
+    const long long len = (long long)it.nrhs * it.n;
+    if (t >= len) return;
+    const int r = (int)(t / it.n);
+    const long long i = t % it.n;
+    const int kk = it.ic[4 * r];
+    const cx<double> *V = (const cx<double> *)it.V, *y = (const cx<double> *)it.yv + (size_t)r * m;
+    cx<double> *out = to_z ? (cx<double> *)it.Z : (cx<double> *)it.x;
+    cx<double> s = to_z ? mk(0.0, 0.0) : out[t];
+    for (int k = 0; k < kk; ++k) s += V[((size_t)k * it.nrhs + r) * it.n + i] * y[k];
+    out[t] = s;
+}
+
+// skip flags of a group: cleared at the start of a solve, gathered for the host per step
+__global__ void kg_gm_clear(const ncgk::GGm *items, int nitems)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nitems) *items[i].skip = 0;
+}
+__global__ void kg_gm_flags(const ncgk::GGm *items, int nitems, int *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nitems) out[i] = *items[i].skip;
+}
+// per item and load case: iterations (counter 2) and relative residual estimate (scalar 2)
+__global__ void kg_gm_stats(const ncgk::GGm *items, int *its, double *rel)
+{
+    const ncgk::GGm it = items[blockIdx.x];
+    const int r = threadIdx.x;
+    if (r < it.nrhs) {
+        its[blockIdx.x * 64 + r] = it.ic[4 * r + 2];
+        rel[blockIdx.x * 64 + r] = it.sc[4 * r + 2];
+    }
+}
+
+// squared norms per load case: partial sums per work-group (gsb * nrhs per item, the
+// Gram-Schmidt pass's partition), then a fixed tree per load case
+__global__ void __launch_bounds__(256) kg_norm_partial(const ncgk::GNorm *items, const int *start, int nitems)
+{
+    __shared__ double red[256];
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GNorm it = items[d];
+    if (it.skip && *it.skip) return;
+    const int b = blockIdx.x - start[d], bx = b % it.gsb, r = b / it.gsb, t = threadIdx.x;
+    const cx<double> *v = (const cx<double> *)it.v + (size_t)r * it.n;
+    double nn = 0.0;
+    for (int i = bx * 256 + t; i < it.n; i += it.gsb * 256) nn += v[i].re * v[i].re + v[i].im * v[i].im;
+    red[t] = nn;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (t < s2) red[t] += red[t + s2];
+        __syncthreads();
+    }
+    if (t == 0) it.npart[(size_t)r * it.gsb + bx] = red[0];
+}
+__global__ void __launch_bounds__(256) kg_norm_reduce(const ncgk::GNorm *items, double *out)
+{
+    __shared__ double red[256];
+    const ncgk::GNorm it = items[blockIdx.x];
+    if (it.skip && *it.skip) return;
+    const int t = threadIdx.x;
+    for (int r = 0; r < it.nrhs; ++r) {
+        red[t] = t < it.gsb ? it.npart[(size_t)r * it.gsb + t] : 0.0;
+        __syncthreads();
+        for (int s2 = 128; s2 > 0; s2 >>= 1) {
+            if (t < s2) red[t] += red[t + s2];
+            __syncthreads();
+        }
+        if (t == 0) out[blockIdx.x * 64 + r] = red[0];
+        __syncthreads();
     }
 }
 
@@ -3057,6 +3409,137 @@ void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double 
 {
     k_fmm_eval_far<<<blocks_for(m, 4), 256, 0, s>>>(m, P, nev, hk, pts, pt_ev, ecen, u, nrhs, (const cx<double> *)z,
                                                     (cx<double> *)fphi, (cx<double> *)fgrad);
+}
+
+// ---- grouped launches --------------------------------------------------------------------
+#define NCG_G_NR(NR, CALL) \
+    switch (NR) {          \
+    case 1: CALL(1); break; \
+    case 2: CALL(2); break; \
+    case 3: CALL(3); break; \
+    default: CALL(4); break; \
+    }
+
+void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, const int *start, int nitems, int blocks)
+{
+    if (blocks <= 0) return;
+#define NCG_CALL(NR)                                                                                               \
+    if (mixed) kg_near_spmv<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);              \
+    else kg_near_spmv<double, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);
+    NCG_G_NR(nr, NCG_CALL)
+#undef NCG_CALL
+}
+
+template <typename S, typename W>
+static void g_T_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+#define NCG_CALL(NR) kg_apply_T<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
+    NCG_G_NR(nr, NCG_CALL)
+#undef NCG_CALL
+}
+template <typename S, typename W>
+static void g_D_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+#define NCG_CALL(NR) kg_apply_D<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
+    NCG_G_NR(nr, NCG_CALL)
+#undef NCG_CALL
+}
+template <typename S, typename W>
+static void g_S_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+#define NCG_CALL(NR) kg_apply_S<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
+    NCG_G_NR(nr, NCG_CALL)
+#undef NCG_CALL
+}
+void g_apply_T(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+    if (blocks <= 0) return;
+    if (f32 && !sig64) g_T_t<float, float>(s, nr, items, start, nitems, blocks);
+    else if (f32) g_T_t<float, double>(s, nr, items, start, nitems, blocks);
+    else g_T_t<double, double>(s, nr, items, start, nitems, blocks);
+}
+void g_apply_D(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+    if (blocks <= 0) return;
+    if (f32 && !sig64) g_D_t<float, float>(s, nr, items, start, nitems, blocks);
+    else if (f32) g_D_t<float, double>(s, nr, items, start, nitems, blocks);
+    else g_D_t<double, double>(s, nr, items, start, nitems, blocks);
+}
+void g_apply_S(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+{
+    if (blocks <= 0) return;
+    if (f32 && !sig64) g_S_t<float, float>(s, nr, items, start, nitems, blocks);
+    else if (f32) g_S_t<float, double>(s, nr, items, start, nitems, blocks);
+    else g_S_t<double, double>(s, nr, items, start, nitems, blocks);
+}
+template <typename TB>
+static void g_bj_t(hipStream_t s, int nr, const GBj *items, const int *start, int nitems, int blocks)
+{
+    switch (nr) {
+    case 1: kg_bj_apply<TB, 1><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    case 2: kg_bj_apply<TB, 2><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    case 3: kg_bj_apply<TB, 3><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    case 4: kg_bj_apply<TB, 4><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    default: kg_bj_apply<TB, 8><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    }
+}
+void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks)
+{
+    if (blocks <= 0) return;
+    if (f32) g_bj_t<float>(s, nr, items, start, nitems, blocks);
+    else g_bj_t<double>(s, nr, items, start, nitems, blocks);
+}
+#undef NCG_G_NR
+void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks)
+{
+    if (blocks > 0) kg_vec<<<blocks, 256, 0, s>>>(op, items, start, nitems);
+}
+void g_multidot(hipStream_t s, int kcount, const GGm *items, const int *start, int nitems, int blocks)
+{
+    if (blocks > 0) kg_multidot<<<blocks, 256, 0, s>>>(kcount, items, start, nitems);
+}
+void g_gs_update(hipStream_t s, int kcount, int pass, int m, const GGm *items, const int *start, int nitems, int blocks)
+{
+    if (blocks > 0) kg_gs_update<<<blocks, 256, 0, s>>>(kcount, pass, m, items, start, nitems);
+}
+void g_gs_normalize(hipStream_t s, int k, int m, const GGm *items, const int *start, int nitems, int blocks)
+{
+    if (blocks > 0) kg_gs_normalize<<<blocks, 256, 0, s>>>(k, m, items, start, nitems);
+}
+void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems)
+{
+    if (nitems > 0) kg_gm_start<<<nitems, 64, 0, s>>>(m, first, items);
+}
+void g_gm_rotate(hipStream_t s, int k, int m, const GGm *items, int nitems)
+{
+    if (nitems > 0) kg_gm_rotate<<<nitems, 64, 0, s>>>(k, m, items);
+}
+void g_gm_solve(hipStream_t s, int m, const GGm *items, int nitems)
+{
+    if (nitems > 0) kg_gm_solve<<<nitems, 64, 0, s>>>(m, items);
+}
+void g_gm_update(hipStream_t s, int m, int to_z, const GGm *items, const int *start, int nitems, int blocks)
+{
+    if (blocks > 0) kg_gm_update<<<blocks, 256, 0, s>>>(m, to_z, items, start, nitems);
+}
+
+void g_gm_clear(hipStream_t s, const GGm *items, int nitems)
+{
+    if (nitems > 0) kg_gm_clear<<<blocks_for(nitems, 64), 64, 0, s>>>(items, nitems);
+}
+void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out)
+{
+    if (nitems > 0) kg_gm_flags<<<blocks_for(nitems, 64), 64, 0, s>>>(items, nitems, out);
+}
+void g_gm_stats(hipStream_t s, const GGm *items, int nitems, int *its, double *rel)
+{
+    if (nitems > 0) kg_gm_stats<<<nitems, 64, 0, s>>>(items, its, rel);
+}
+void g_norms(hipStream_t s, const GNorm *items, const int *start, int nitems, int blocks, double *out)
+{
+    if (blocks <= 0) return;
+    kg_norm_partial<<<blocks, 256, 0, s>>>(items, start, nitems);
+    kg_norm_reduce<<<nitems, 256, 0, s>>>(items, out);
 }
 
 }  // namespace ncgk

@@ -200,4 +200,113 @@ void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_pt
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad);
 
+// ---------------------------------------------------------------------------------------
+// Grouped launches: one launch applies the same operation to several problems (or problem
+// levels) of a group solved in lockstep.  `items` is a device array with one entry per
+// problem (or level) and `start` (device, nitems + 1 ints) the prefix sums of their
+// work-group counts; a work-group finds its item by a scan of `start` (items are few) and
+// does nothing while the item's flag *skip (device int, may be null) is nonzero, i.e. once
+// that problem's Krylov solve has converged.  Each item's arithmetic is the single-problem
+// kernel's, work-group for work-group, so results do not depend on the grouping.
+struct GNear {      // near field of the finest level (NearChunks, col, values)
+    int n, nch;
+    const long long *cb, *off;
+    const int *w, *row, *col;
+    const void *val;
+    const void *x;
+    void *y;
+    const int *skip;
+};
+struct GLevel {     // one FMM level: T (x -> t), D (t -> s) and S (s -> y += ) of a problem
+    int n, ncl, P, ntl;
+    const int *cl_ptr, *cl_elem, *fan_ptr, *fan_idx, *tl_cl, *tl_a0, *tl_m;
+    const void *Tv, *Dv, *Sv;
+    const void *x;
+    void *tw, *sw, *y;
+    const int *skip;
+};
+struct GBj {        // block-Jacobi blocks (FMM finest clusters or dense diagonal blocks)
+    int n, nr, nblk;
+    const int *b0, *m, *elem;
+    const long long *boff;
+    const void *minv;
+    const void *x;
+    void *y;
+    const int *skip;
+};
+// GMRES(m) state of one problem of a group, on the device: the Hessenberg matrices, Givens
+// rotations, right-hand sides g and solutions y of every load case, the Arnoldi work
+// arrays, and per load case the scalars [beta, |b|, relative residual, -] and the
+// counters [k, done, iterations, -].  skip (= the problem's flag of the grouped launches)
+// is set once every load case has converged.
+struct GGm {
+    int n, nrhs, nsl, slice, gsb;
+    const void *b;                      // right-hand sides [nrhs][n] (the Krylov system's)
+    void *x;                            // solution [nrhs][n]
+    void *V;                            // basis [m + 1][nrhs][n]
+    void *W;                            // new Arnoldi vector / residual [nrhs][n]
+    void *Z, *T;                        // preconditioned vector and apply output [nrhs][n]
+    void *part;                         // multidot slice partials
+    void *h;                            // [2][nrhs][m + 1] projections of the two passes, then [nrhs][3] norms
+    double *npart;                      // [nrhs][gsb] partial squared norms
+    void *H, *cs, *sn, *g, *yv;         // [nrhs][(m+1) m], [nrhs][m], [nrhs][m], [nrhs][m+1], [nrhs][m]
+    double *sc;                         // [nrhs][4]
+    int *ic;                            // [nrhs][4]
+    int *skip;
+    double tol;                         // relative residual target of every load case
+};
+// element-wise vector operations of a group: per item y = a (op kVecCopy), y = a - b
+// (kVecSub), y += a (kVecAdd), y = 0 (kVecZero); len = complex entries
+enum { kVecCopy = 0, kVecSub = 1, kVecAdd = 2, kVecZero = 3 };
+struct GVec {
+    long long len;
+    const void *a, *b;
+    void *y;
+    const int *skip;
+};
+// launchers (blocks = start[nitems], the prefix the host uploaded)
+void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, const int *start, int nitems,
+                 int blocks);
+// T and D of every item (each writes its own t or s); S adds into y, so a launch may hold at
+// most one level of a problem (the host launches one level index at a time).  sig64: t and
+// s in FP64 whatever the values' precision (the refinement residual).
+void g_apply_T(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks);
+void g_apply_D(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks);
+void g_apply_S(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks);
+void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks);
+void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks);
+// GMRES(m) of a group in lockstep; at Arnoldi step k the basis holds kcount = k + 1 vectors.
+//   g_gs_update pass 0/1: projections (multidot partials, g_multidot) summed, w -= V h;
+//   pass 1 also leaves the partial norms.  kcount = 0 with pass 1: partial norms of W only.
+//   g_gs_normalize: V[k + 1] = W / |W| (k = -1: V[0]), squared norms to h's norm slots.
+//   g_gm_start: beta, residual, done flags of a cycle (first: |b| = beta, counters reset).
+//   g_gm_rotate: Hessenberg column k, Givens rotations, residual estimate, done flags.
+//   g_gm_solve: y = H^-1 g per load case (zero past its k); g_gm_update: Z = V y (to_z) or
+//   x += V y.
+void g_multidot(hipStream_t s, int kcount, const GGm *items, const int *start, int nitems, int blocks);
+void g_gs_update(hipStream_t s, int kcount, int pass, int m, const GGm *items, const int *start, int nitems,
+                 int blocks);
+void g_gs_normalize(hipStream_t s, int k, int m, const GGm *items, const int *start, int nitems, int blocks);
+void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems);
+void g_gm_rotate(hipStream_t s, int k, int m, const GGm *items, int nitems);
+void g_gm_solve(hipStream_t s, int m, const GGm *items, int nitems);
+void g_gm_update(hipStream_t s, int m, int to_z, const GGm *items, const int *start, int nitems, int blocks);
+// skip flags cleared (start of a solve) or gathered into out[nitems]; per item and load case
+// the iterations and residual estimates into its/rel[item][64]
+void g_gm_clear(hipStream_t s, const GGm *items, int nitems);
+void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out);
+void g_gm_stats(hipStream_t s, const GGm *items, int nitems, int *its, double *rel);
+// squared norms |v_r|^2 of every item's load cases into out[item][64]; per item gsb * nrhs
+// work-groups (start: their prefix), npart [nrhs][gsb] scratch
+struct GNorm {
+    int n, nrhs, gsb;
+    const void *v;
+    double *npart;
+    const int *skip;
+};
+void g_norms(hipStream_t s, const GNorm *items, const int *start, int nitems, int blocks, double *out);
+
 }  // namespace ncgk

@@ -108,6 +108,7 @@ struct ncg_server {
     int64_t jobs_done = 0, jobs_failed = 0, steps_done = 0, solves_done = 0;
     clk::time_point t0;
     std::map<std::string, ncg_kernel_stat> kstats;   // per-kernel timing of finished steps
+    double host_ms[4] = {0, 0, 0, 0};                 // lane wall time: assemble, solve, outputs, waiting
 };
 
 namespace {
@@ -146,14 +147,32 @@ void fail_job(ncg_server *S, Job *J, int code, const std::string &msg)
     if (J->inflight == 0) end_job(S, J);
 }
 
-// Choice of the next step for a free lane (lock held).  Returns the queue index, the
-// pool entry to reuse (or null: create) and the idle entries to evict, or -1.
-int choose(ncg_server *S, Res **reuse, std::vector<Res *> &evict, size_t &need)
+// Steps a lane may solve together (ncg_solve_group): GMRES or mixed-precision steps of equal
+// precision, load-case count and solver options.
+bool groupable(const Job *J)
 {
-    const size_t cap = (size_t)S->cfg.lanes * (size_t)S->cfg.problems_per_lane;
+    return J->solver.solver == NCG_SOLVER_GMRES &&
+           (J->solver.precond == NCG_PRECOND_BLOCK || J->solver.precond == NCG_PRECOND_NONE);
+}
+bool same_group(const Job *a, const Job *b)
+{
+    return groupable(a) && groupable(b) && a->precision == b->precision && a->nrhs == b->nrhs &&
+           a->solver.precond == b->solver.precond && a->solver.max_iterations == b->solver.max_iterations &&
+           a->solver.tolerance == b->solver.tolerance && a->solver.restart == b->solver.restart &&
+           a->solver.refinement_steps == b->solver.refinement_steps &&
+           a->solver.refinement_tolerance == b->solver.refinement_tolerance;
+}
+
+// Choice of the next step for a free lane (lock held).  Returns the queue index, the
+// pool entry to reuse (or null: create) and the idle entries to evict, or -1.  With `with`
+// set, only steps that may join that job's group are considered.
+int choose(ncg_server *S, Res **reuse, std::vector<Res *> &evict, size_t &need, const Job *with = nullptr)
+{
+    const size_t cap = (size_t)S->cfg.lanes * (size_t)std::max(S->cfg.problems_per_lane, S->cfg.group);
     const size_t scan = std::min<size_t>(S->queue.size(), 64);
     for (size_t q = 0; q < scan; ++q) {
         const Step &st = S->queue[q];
+        if (with && !same_group(with, st.job)) continue;
         const int mesh = st.job->mesh;
         Res *r = nullptr;
         for (auto &e : S->pool)
@@ -183,9 +202,55 @@ int choose(ncg_server *S, Res **reuse, std::vector<Res *> &evict, size_t &need)
     return -1;
 }
 
+// Takes step q of the queue for a lane (lock held): reserves its pool entry and memory,
+// unlinks the entries to evict (the caller destroys them after unlocking).
+struct Taken {
+    Step st;
+    Res *r = nullptr;
+    int rc = 0;
+    std::string msg;
+    long long its = 0;
+};
+Taken take(ncg_server *S, int q, Res *r, const std::vector<Res *> &evict, size_t need, std::vector<Res *> &dead)
+{
+    Taken t;
+    t.st = S->queue[q];
+    S->queue.erase(S->queue.begin() + q);
+    Job *J = t.st.job;
+    for (Res *e : evict) {
+        S->resident -= e->bytes;
+        for (size_t i = 0; i < S->pool.size(); ++i)
+            if (S->pool[i].get() == e) {
+                S->pool[i].release();
+                S->pool.erase(S->pool.begin() + i);
+                break;
+            }
+        dead.push_back(e);
+    }
+    if (!r) {
+        S->pool.push_back(std::unique_ptr<Res>(new Res()));
+        r = S->pool.back().get();
+        r->mesh = J->mesh;
+    }
+    r->busy = true;
+    S->resident += need;
+    r->bytes += need;
+    S->peak = std::max(S->peak, S->resident);
+    S->running++;
+    J->inflight++;
+    if (!J->started) {
+        J->started = true;
+        J->t_first = clk::now();
+        J->state = 1;
+    }
+    t.r = r;
+    return t;
+}
+
 void run_lane(ncg_server *S)
 {
     std::unique_lock<std::mutex> lk(S->mu);
+    clk::time_point t_idle = clk::now();
     for (;;) {
         Res *r = nullptr;
         std::vector<Res *> evict;
@@ -203,122 +268,145 @@ void run_lane(ncg_server *S)
                 // nothing runs that could free memory: the head step cannot fit at all
                 Job *J = S->queue.front().job;
                 fail_job(S, J, NCG_E_MEMORY,
-                           "a step of this job needs " + std::to_string(J->bytes >> 20) + " MiB, beyond the " +
-                               std::to_string((long long)S->budget >> 20) + " MiB device-memory budget");
+                         "a step of this job needs " + std::to_string(J->bytes >> 20) + " MiB, beyond the " +
+                             std::to_string((long long)S->budget >> 20) + " MiB device-memory budget");
                 continue;
             }
             S->cv_work.wait(lk);
         }
-        const Step st = S->queue[q];
-        S->queue.erase(S->queue.begin() + q);
-        Job *J = st.job;
-        // reserve memory and pool slots, unlink the evicted entries
-        for (Res *e : evict) {
-            S->resident -= e->bytes;
-            for (size_t i = 0; i < S->pool.size(); ++i)
-                if (S->pool[i].get() == e) {
-                    S->pool[i].release();
-                    S->pool.erase(S->pool.begin() + i);
-                    break;
-                }
+        // the first step, then as many compatible steps as the group holds and memory allows
+        std::vector<Res *> dead;
+        std::vector<Taken> grp;
+        grp.push_back(take(S, q, r, evict, need, dead));
+        const Job *J0 = grp[0].st.job;
+        while ((int)grp.size() < S->cfg.group && groupable(J0)) {
+            Res *r2 = nullptr;
+            std::vector<Res *> ev2;
+            size_t need2 = 0;
+            const int q2 = choose(S, &r2, ev2, need2, J0);
+            if (q2 < 0) break;
+            grp.push_back(take(S, q2, r2, ev2, need2, dead));
         }
-        if (!r) {
-            S->pool.push_back(std::unique_ptr<Res>(new Res()));
-            r = S->pool.back().get();
-            r->mesh = J->mesh;
-        }
-        r->busy = true;
-        S->resident += need;
-        r->bytes += need;
-        S->peak = std::max(S->peak, S->resident);
-        S->running++;
-        J->inflight++;
-        if (!J->started) {
-            J->started = true;
-            J->t_first = clk::now();
-            J->state = 1;
-        }
-        const MeshRec *M = S->meshes[J->mesh].get();
+        const clk::time_point t_start = clk::now();
+        S->host_ms[3] += ms_between(t_idle, t_start);
         lk.unlock();
 
-        for (Res *e : evict) {
+        for (Res *e : dead) {
             ncg_problem_destroy(e->p);
             delete e;
         }
-        int rc = 0;
-        std::string msg;
-        long long its = 0;
-        if (!r->p) {
-            rc = ncg_problem_create(&M->g, S->cfg.device, &r->p);
-            if (rc) msg = ncg_last_error();
-            else if (S->cfg.kernel_timing) ncg_set_kernel_timing(r->p, 1);
-        }
-        if (!rc) {
-            rc = ncg_assemble(r->p, &J->freq[st.fi].f, J->precision);
-            if (rc) msg = ncg_last_error();
-        }
-        if (!rc) {
-            ncg_solve_stats ss;
-            rc = ncg_solve(r->p, &J->solver, &ss);
-            if (rc) msg = ncg_last_error();
-            for (int c = 0; c < J->nrhs && c < 64; ++c) its += ss.iterations[c];
-        }
-        const size_t vec = 2 * (size_t)J->nrhs;
-        bool write = true;
-        {
-            // another step of the job failed meanwhile: its outputs are no longer wanted
-            std::lock_guard<std::mutex> g(S->mu);
-            write = !J->failed;
-        }
-        if (!rc && write && J->solution) {
-            rc = ncg_get_solution(r->p, J->solution + vec * J->n * st.fi);
-            if (rc) msg = ncg_last_error();
-        }
-        if (!rc && write && J->m > 0 && (J->phi || J->grad)) {
-            std::vector<double> tmp;
-            double *ph = J->phi ? J->phi + vec * J->m * st.fi : nullptr;
-            if (!ph) {
-                tmp.resize(vec * J->m);
-                ph = tmp.data();
+        // problems and operators of every step of the group
+        for (Taken &t : grp) {
+            const Job *J = t.st.job;
+            const MeshRec *M = S->meshes[J->mesh].get();
+            if (!t.r->p) {
+                t.rc = ncg_problem_create(&M->g, S->cfg.device, &t.r->p);
+                if (t.rc) t.msg = ncg_last_error();
+                else if (S->cfg.kernel_timing) ncg_set_kernel_timing(t.r->p, 1);
             }
-            rc = ncg_field(r->p, ph, J->grad ? J->grad + 3 * vec * J->m * st.fi : nullptr);
-            if (rc) msg = ncg_last_error();
+            if (!t.rc) {
+                t.rc = ncg_assemble(t.r->p, &J->freq[t.st.fi].f, J->precision);
+                if (t.rc) t.msg = ncg_last_error();
+            }
         }
-
-        ncg_kernel_stat ks[16];
-        int32_t nks = 0;
-        if (S->cfg.kernel_timing && r->p && ncg_kernel_stats(r->p, ks, 16, &nks, 1) != 0) nks = 0;
-
-        lk.lock();
-        for (int i = 0; i < nks && i < 16; ++i) {
-            ncg_kernel_stat &a = S->kstats[ks[i].name];
-            if (a.launches == 0) snprintf(a.name, sizeof(a.name), "%s", ks[i].name);
-            a.launches += ks[i].launches;
-            a.total_ms += ks[i].total_ms;
-            a.bytes += ks[i].bytes;
-        }
-        S->running--;
-        r->busy = false;
-        r->last = ++S->tick;
-        if (rc && !r->p) {
-            // the problem could not be created: give its reservation back
-            S->resident -= r->bytes;
-            for (size_t i = 0; i < S->pool.size(); ++i)
-                if (S->pool[i].get() == r) {
-                    S->pool.erase(S->pool.begin() + i);
-                    break;
+        const clk::time_point t_asm = clk::now();
+        // solve: the group's assembled steps together, or the one step alone
+        std::vector<Taken *> ok;
+        for (Taken &t : grp)
+            if (!t.rc) ok.push_back(&t);
+        if (ok.size() > 1) {
+            std::vector<ncg_problem *> P;
+            for (Taken *t : ok) P.push_back(t->r->p);
+            std::vector<ncg_solve_stats> ss(P.size());
+            const int rc = ncg_solve_group(P.data(), (int32_t)P.size(), &J0->solver, ss.data());
+            const std::string msg = rc ? ncg_last_error() : "";
+            for (size_t i = 0; i < ok.size(); ++i) {
+                for (int c = 0; c < ok[i]->st.job->nrhs && c < 64; ++c) ok[i]->its += ss[i].iterations[c];
+                if (rc == NCG_E_NOCONV ? !ss[i].converged_all : rc != 0) {
+                    ok[i]->rc = rc;
+                    ok[i]->msg = msg;
                 }
+            }
+        } else if (ok.size() == 1) {
+            ncg_solve_stats ss;
+            Taken &t = *ok[0];
+            t.rc = ncg_solve(t.r->p, &t.st.job->solver, &ss);
+            if (t.rc) t.msg = ncg_last_error();
+            for (int c = 0; c < t.st.job->nrhs && c < 64; ++c) t.its += ss.iterations[c];
         }
-        S->steps_done++;
-        S->solves_done += J->nrhs;
-        J->iterations += its;
-        J->inflight--;
-        if (rc) {
-            fail_job(S, J, rc, msg);
-        } else if (J->failed) {
-            if (J->inflight == 0) end_job(S, J);
-        } else if (++J->freq_done == (int)J->freq.size()) {
-            end_job(S, J);
+        const clk::time_point t_solve = clk::now();
+        // outputs
+        for (Taken &t : grp) {
+            Job *J = t.st.job;
+            const size_t vec = 2 * (size_t)J->nrhs;
+            bool write = true;
+            {
+                // another step of the job failed meanwhile: its outputs are no longer wanted
+                std::lock_guard<std::mutex> g(S->mu);
+                write = !J->failed;
+            }
+            if (!t.rc && write && J->solution) {
+                t.rc = ncg_get_solution(t.r->p, J->solution + vec * J->n * t.st.fi);
+                if (t.rc) t.msg = ncg_last_error();
+            }
+            if (!t.rc && write && J->m > 0 && (J->phi || J->grad)) {
+                std::vector<double> tmp;
+                double *ph = J->phi ? J->phi + vec * J->m * t.st.fi : nullptr;
+                if (!ph) {
+                    tmp.resize(vec * J->m);
+                    ph = tmp.data();
+                }
+                t.rc = ncg_field(t.r->p, ph, J->grad ? J->grad + 3 * vec * J->m * t.st.fi : nullptr);
+                if (t.rc) t.msg = ncg_last_error();
+            }
+        }
+        std::vector<std::vector<ncg_kernel_stat>> kst(grp.size());
+        for (size_t i = 0; i < grp.size(); ++i) {
+            ncg_kernel_stat ks[16];
+            int32_t nks = 0;
+            if (S->cfg.kernel_timing && grp[i].r->p && ncg_kernel_stats(grp[i].r->p, ks, 16, &nks, 1) == 0)
+                kst[i].assign(ks, ks + std::min(nks, 16));
+        }
+
+        const clk::time_point t_out = clk::now();
+        lk.lock();
+        S->host_ms[0] += ms_between(t_start, t_asm);
+        S->host_ms[1] += ms_between(t_asm, t_solve);
+        S->host_ms[2] += ms_between(t_solve, t_out);
+        t_idle = clk::now();
+        for (size_t g = 0; g < grp.size(); ++g) {
+            Taken &t = grp[g];
+            Job *J = t.st.job;
+            for (const ncg_kernel_stat &k : kst[g]) {
+                ncg_kernel_stat &a = S->kstats[k.name];
+                if (a.launches == 0) snprintf(a.name, sizeof(a.name), "%s", k.name);
+                a.launches += k.launches;
+                a.total_ms += k.total_ms;
+                a.bytes += k.bytes;
+            }
+            S->running--;
+            t.r->busy = false;
+            t.r->last = ++S->tick;
+            if (t.rc && !t.r->p) {
+                // the problem could not be created: give its reservation back
+                S->resident -= t.r->bytes;
+                for (size_t i = 0; i < S->pool.size(); ++i)
+                    if (S->pool[i].get() == t.r) {
+                        S->pool.erase(S->pool.begin() + i);
+                        break;
+                    }
+            }
+            S->steps_done++;
+            S->solves_done += J->nrhs;
+            J->iterations += t.its;
+            J->inflight--;
+            if (t.rc) {
+                fail_job(S, J, t.rc, t.msg);
+            } else if (J->failed) {
+                if (J->inflight == 0) end_job(S, J);
+            } else if (++J->freq_done == (int)J->freq.size()) {
+                end_job(S, J);
+            }
         }
         S->cv_work.notify_all();
     }
@@ -349,6 +437,8 @@ int ncg_server_create(const ncg_server_config *cfg, ncg_server **out)
     if (c.lanes <= 0) c.lanes = 4;
     if (c.lanes > 16) return sfail(NCG_E_ARG, "ncg_server_create: at most 16 lanes");
     if (c.problems_per_lane <= 0) c.problems_per_lane = 2;
+    if (c.group <= 0) c.group = 4;
+    if (c.group > 64) return sfail(NCG_E_ARG, "ncg_server_create: at most 64 steps per group");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sfail(NCG_E_DEVICE, "no HIP device available");
     if (c.device < 0 || c.device >= ndev) return sfail(NCG_E_ARG, "device index out of range");
@@ -510,6 +600,7 @@ int ncg_server_stats(ncg_server *S, ncg_server_info *st)
     st->peak_resident_bytes = (int64_t)S->peak;
     st->lanes = S->cfg.lanes;
     st->queued_steps = (int32_t)S->queue.size();
+    for (int i = 0; i < 4; ++i) st->lane_ms[i] = S->host_ms[i];
     return 0;
 }
 

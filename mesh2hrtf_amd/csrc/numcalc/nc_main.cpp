@@ -403,9 +403,9 @@ int main(int argc, char **argv)
         memset(&so, 0, sizeof(so));
         so.solver = P.method_solver == 4 ? NCG_SOLVER_LU : (gmres ? NCG_SOLVER_GMRES : NCG_SOLVER_CGS);
         so.precond = P.precond == 0 ? NCG_PRECOND_ILU : P.precond == 1 ? NCG_PRECOND_SCALING : NCG_PRECOND_NONE;
-        if (precision == NCG_MIXED && so.solver != NCG_SOLVER_LU) {
-            // the mixed form is FP64 refinement around FP32 GMRES with the block-Jacobi
-            // preconditioner, whatever IPRECOND says
+        if ((precision == NCG_MIXED || so.solver == NCG_SOLVER_GMRES) && so.solver != NCG_SOLVER_LU) {
+            // GMRES and the mixed form (FP64 refinement around FP32 GMRES) run with the
+            // block-Jacobi preconditioner, whatever IPRECOND says
             so.solver = NCG_SOLVER_GMRES;
             so.precond = NCG_PRECOND_BLOCK;
         }

@@ -89,7 +89,7 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
            "ncg_assemble", "ncg_solve", "ncg_problem_fmm_info", "ncg_get_solution", "ncg_field", "ncg_get_system",
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
            "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
-           "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats")
+           "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group")
 
 
 class _KernelStat(ctypes.Structure):
@@ -104,7 +104,7 @@ def _kernel_dict(arr, count):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("memory_budget_bytes", ctypes.c_double),
-                ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32)]
+                ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32), ("group", ctypes.c_int32)]
 
 
 class _Job(ctypes.Structure):
@@ -123,7 +123,8 @@ class _ServerInfo(ctypes.Structure):
     _fields_ = [("jobs_done", ctypes.c_int64), ("jobs_failed", ctypes.c_int64), ("steps_done", ctypes.c_int64),
                 ("solves_done", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("resident_bytes", ctypes.c_int64),
-                ("peak_resident_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("queued_steps", ctypes.c_int32)]
+                ("peak_resident_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("queued_steps", ctypes.c_int32),
+                ("lane_ms", ctypes.c_double * 4)]
 
 _lib = None
 
@@ -288,6 +289,22 @@ class Problem:
         return dict(zip(keys, list(t)))
 
 
+def solve_group(problems, solver=GMRES, precond=BLOCK, max_iterations=500, tolerance=1e-9, restart=50,
+                refinement_steps=8, refinement_tolerance=1e-6, allow_noconv=False):
+    """ncg_solve_group: solve several assembled problems (equal n_rhs and precision) in
+    lockstep, sharing every Krylov launch; returns one stats dict per problem."""
+    lib = load()
+    arr = (ctypes.c_void_p * len(problems))(*[p._h.value for p in problems])
+    o = _Options(solver, precond, max_iterations, tolerance, restart, refinement_steps, refinement_tolerance, 0)
+    st = (_Stats * len(problems))()
+    rc = lib.ncg_solve_group(arr, len(problems), ctypes.byref(o), st)
+    if rc != 0 and not (allow_noconv and rc == -6):
+        _check(rc)
+    return [dict(iterations=list(st[i].iterations[:p.nrhs]), relative_residual=list(st[i].relative_residual[:p.nrhs]),
+                 converged=bool(st[i].converged_all), refinement_steps=st[i].refinement_steps,
+                 ms_solve=st[i].ms_solve) for i, p in enumerate(problems)]
+
+
 def _status_dict(st):
     return dict(state=("queued", "running", "done", "failed")[st.state], error_code=st.error_code,
                 freq_done=st.freq_done, iterations=st.iterations, latency_ms=st.latency_ms, queue_ms=st.queue_ms,
@@ -309,10 +326,11 @@ class Server:
     """Serving runtime (ncg_server_*): jobs of (mesh, frequencies, load cases) on one GPU
     with continuous batching over `lanes` concurrent steps and resident problems."""
 
-    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False):
+    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False, group=0):
         self.lib = load()
         self._h = ctypes.c_void_p()
-        cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing))
+        cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing),
+                            int(group))
         _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._meshes = {}
         # arrays the server reads or writes for each job that has not ended: they stay
@@ -383,7 +401,9 @@ class Server:
     def stats(self):
         s = _ServerInfo()
         _check(self.lib.ncg_server_stats(self._h, ctypes.byref(s)))
-        return {k: getattr(s, k) for k, _ in _ServerInfo._fields_}
+        d = {k: getattr(s, k) for k, _ in _ServerInfo._fields_}
+        d["lane_ms"] = dict(zip(("assemble", "solve", "outputs", "waiting"), list(s.lane_ms)))
+        return d
 
     def close(self):
         if self._h:

@@ -249,7 +249,7 @@ class Project:
 def _fmt_freq_curve(freqs):
     lines = ["0 %d" % (len(freqs) + 1), "0.000000 0.000000e+00 0.0"]
     for i, f in enumerate(freqs):
-        lines.append("%.6f %.6e 0.0" % ((i + 1) * 1e-6, f))
+        lines.append("%.6f %.16e 0.0" % ((i + 1) * 1e-6, f))
     return lines
 
 
