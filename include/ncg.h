@@ -175,6 +175,18 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *st
 int ncg_solve_group(ncg_problem *const *problems, int32_t count, const ncg_solver_options *opt,
                     ncg_solve_stats *stats);
 
+/* Launch shapes of the grouped FMM applies, chosen by measurement (ncg_autotune) rather
+ * than fixed: waves per work-group (1, 2, 4 or 8) that split each T, D and S sum.  A group
+ * solve uses its first problem's tuning.  No counterpart in the reference. */
+typedef struct {
+    int32_t t_waves, d_waves, s_waves;  /* default 8                                         */
+    int32_t tuned;                      /* 1 once set by ncg_autotune                         */
+    int32_t reserved0;
+    int32_t reserved[3];
+} ncg_tuning;
+int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t);
+int ncg_get_tuning(ncg_problem *p, ncg_tuning *t);
+
 /* The method and FMM plan of the last ncg_assemble (what NumCalc logs per step). */
 int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info);
 
@@ -202,8 +214,9 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
  * duration of the dominant kernel of each phase over the calls since the last reset
  * (HIP events on the problem's stream).  Layout of t[8]:
  * [0] assemble ms  [1] solve ms  [2] field ms  [3] far-field quadrature kernel avg ms
- * [4] matvec kernel avg ms  [5] matvec launches  [6] quadrature evaluations of the
- * last assembly (far + near + singular)  [7] bytes of the stored operator. */
+ * [4] matvec kernel avg ms  [5] matvec launches  [6] far-field quadrature evaluations
+ * of the last dense assembly (counted only while kernel timing is on, else 0)
+ * [7] bytes of the stored operator. */
 int ncg_timings(ncg_problem *p, double *t, int32_t reset);
 
 /* Per-kernel timing of the operator apply (near-field, T, D, S or dense matvec

@@ -329,6 +329,7 @@ struct ncg_problem {
     double k = 0.0;
     int nrhs = 0, precision = NCG_FP64;
     int assembled = 0, solved = 0;
+    int asm_pending = 0;      // assembly queued on the stream, its error flags not yet read
     void *A = nullptr;
     size_t A_cap = 0;
     double *adm = nullptr;
@@ -383,6 +384,7 @@ struct ncg_problem {
     // the members' streams around the group's
     Grow gm;
     Grow gdesc;
+    ncg_tuning tune{8, 8, 8, 0, 0, {0, 0, 0}};   // launch shapes of the grouped applies (leader's apply)
     char *gstage = nullptr;
     size_t gstage_cap = 0;
     void *gpin = nullptr;
@@ -1340,6 +1342,69 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
     ncgk::ilu_solve(p->stream, p->n, p->nrhs, L, U, x, y);
 }
 
+// Waits for a queued assembly and reads its error flags (the subdivision limits of the
+// quadrature); the operator may be used only after this.
+int assemble_finish(ncg_problem *p)
+{
+    if (!p->asm_pending) return 0;
+    const int dense = p->asm_pending == 2;
+    p->asm_pending = 0;
+    int rc = check_err(p);
+    if (rc) {
+        p->assembled = 0;
+        return rc;
+    }
+    if (dense && p->ktime) {
+        unsigned long long ev = 0;
+        NCG_HIP(hipMemcpyAsync(&ev, p->evals, sizeof(ev), hipMemcpyDeviceToHost, p->stream));
+        NCG_HIP(hipStreamSynchronize(p->stream));
+        p->last_evals = (double)ev;
+    }
+    p->t_asm = elapsed(p->ev_asm);
+    if (dense) {
+        p->far_ms_sum += elapsed(p->ev_far);
+        p->far_calls++;
+    }
+    return 0;
+}
+
+// Queues the evaluation-grid field of the solved load cases on the problem's stream into
+// the device buffers out_phi / out_grad; field_collect copies them out and reads the error
+// flags (ncg_field is the two in a row).
+int field_enqueue(ncg_problem *p, bool want_grad)
+{
+    const int n = p->n, m = p->m, nr = p->nrhs;
+    int rc = ensure_work(p, 8);
+    if (rc) return rc;
+    double *pot = p->work[0], *vel = p->work[1];
+    if ((rc = p->out_phi.ensure(16 * (size_t)nr * m)) || (rc = p->out_grad.ensure(48 * (size_t)nr * m))) return rc;
+    double *dphi = p->out_phi.d(), *dgrad = p->out_grad.d();
+    record(p, p->ev_field.a);
+    ncgk::field_prep(p->stream, n, nr, p->kind, p->adm, p->interior ? -1.0 : 1.0, p->bcval, p->x, pot, vel);
+    if (p->plan.method != 0) {
+        rc = field_fmm(p, pot, vel, dphi, want_grad ? dgrad : nullptr);
+        if (rc) return rc;
+    } else {
+        ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
+                    want_grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
+    }
+    record(p, p->ev_field.b);
+    NCG_HIP(hipGetLastError());
+    return 0;
+}
+
+int field_collect(ncg_problem *p, double *phi, double *grad)
+{
+    const int m = p->m, nr = p->nrhs;
+    hipError_t e = hipMemcpyAsync(phi, p->out_phi.p, sizeof(cd) * nr * m, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess && grad)
+        e = hipMemcpyAsync(grad, p->out_grad.p, sizeof(cd) * 3 * nr * m, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("ncg_field: ") + hipGetErrorString(e));
+    p->t_field = elapsed(p->ev_field);
+    return check_err(p);
+}
+
 // ---------------------------------------------------------------------------------- block-Jacobi
 // Inverse of every finest cluster's self block of the near field (FMM plans only),
 // computed once per assembly on first use.
@@ -1770,16 +1835,16 @@ void run_apply(const Stage &st, ncg_problem *lead, const std::vector<ncg_problem
                               L.blocks);
             break;
         case 1:
-            ncgk::g_apply_T(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
-                            L.nitems, L.blocks);
+            ncgk::g_apply_T(s, L.cls < 2, L.cls == 1, lead->tune.t_waves, L.nr, st.dev<ncgk::GLevel>(L.items),
+                            st.dev<int>(L.start), L.nitems, L.blocks);
             break;
         case 2:
-            ncgk::g_apply_D(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
-                            L.nitems, L.blocks);
+            ncgk::g_apply_D(s, L.cls < 2, L.cls == 1, lead->tune.d_waves, L.nr, st.dev<ncgk::GLevel>(L.items),
+                            st.dev<int>(L.start), L.nitems, L.blocks);
             break;
         default:
-            ncgk::g_apply_S(s, L.cls < 2, L.cls == 1, L.nr, st.dev<ncgk::GLevel>(L.items), st.dev<int>(L.start),
-                            L.nitems, L.blocks);
+            ncgk::g_apply_S(s, L.cls < 2, L.cls == 1, lead->tune.s_waves, L.nr, st.dev<ncgk::GLevel>(L.items),
+                            st.dev<int>(L.start), L.nitems, L.blocks);
             break;
         }
     }
@@ -2495,7 +2560,14 @@ int ncg_problem_destroy(ncg_problem *p)
     return 0;
 }
 
-int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
+}  // extern "C"
+
+namespace {
+
+// Queues the assembly of one frequency step on the problem's stream; with wait, also
+// waits for it and reads its error flags (ncg_assemble), else the flags are read by
+// assemble_finish before the operator is used.
+int assemble_impl(ncg_problem *p, const ncg_frequency *f, int32_t precision, bool wait)
 {
     if (!p || !f) return fail(NCG_E_ARG, "ncg_assemble: null argument");
     if (!(f->k > 0.0)) return fail(NCG_E_ARG, "ncg_assemble: wave number must be positive");
@@ -2575,17 +2647,17 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
         record(p, p->ev_far.b);
         record(p, p->ev_asm.b);
         NCG_HIP(hipGetLastError());
-        if ((rc = check_err(p))) return rc;
         p->last_evals = 0.0;
-        p->t_asm = elapsed(p->ev_asm);
         p->assembled = 1;
-        return 0;
+        p->asm_pending = 1;
+        return wait ? assemble_finish(p) : 0;
     }
     if ((rc = ensure_near_table(p))) return rc;
     record(p, p->ev_asm.a);
     record(p, p->ev_far.a);
     p->dbj_ready = 0;
-    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->evals);
+    // quadrature evaluations are counted only while kernel timing is on (measurement)
+    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->ktime ? p->evals : nullptr);
     record(p, p->ev_far.b);
     ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err);
     if (p->nsrc) ncgk::rhs_sources(s, p->elements(), b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err);
@@ -2593,22 +2665,30 @@ int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
                         p->nsrc > 0, p->A, mixed, p->rhs);
     record(p, p->ev_asm.b);
     NCG_HIP(hipGetLastError());
-    if ((rc = check_err(p))) return rc;
-    unsigned long long ev = 0;
-    NCG_HIP(hipMemcpy(&ev, p->evals, sizeof(ev), hipMemcpyDeviceToHost));
-    p->last_evals = (double)ev;
-    p->t_asm = elapsed(p->ev_asm);
-    p->far_ms_sum += elapsed(p->ev_far);
-    p->far_calls++;
+    p->last_evals = 0.0;
     p->assembled = 1;
-    return 0;
+    p->asm_pending = 2;   // dense: also the far-field kernel timing
+    return wait ? assemble_finish(p) : 0;
 }
+
+}  // namespace
+
+extern "C" {
+
+int ncg_assemble(ncg_problem *p, const ncg_frequency *f, int32_t precision)
+{
+    return assemble_impl(p, f, precision, true);
+}
+
 
 int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
 {
     if (!p || !o) return fail(NCG_E_ARG, "ncg_solve: null argument");
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_solve: assemble first");
     if (o->max_iterations < 0) return fail(NCG_E_ARG, "ncg_solve: negative iteration limit");
+    NCG_HIP(hipSetDevice(p->device));
+    int rcf = assemble_finish(p);
+    if (rcf) return rcf;
     if (o->precond < NCG_PRECOND_ILU || o->precond > NCG_PRECOND_BLOCK)
         return fail(NCG_E_ARG, "ncg_solve: precond must be NCG_PRECOND_ILU, _SCALING, _NONE or _BLOCK");
     NCG_HIP(hipSetDevice(p->device));
@@ -2645,6 +2725,24 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *o, ncg_solve_stats *st)
     return rc;
 }
 
+static int valid_waves(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
+
+int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t)
+{
+    if (!p || !t) return fail(NCG_E_ARG, "ncg_set_tuning: null argument");
+    if (!valid_waves(t->t_waves) || !valid_waves(t->d_waves) || !valid_waves(t->s_waves))
+        return fail(NCG_E_ARG, "ncg_set_tuning: waves per work-group must be 1, 2, 4 or 8");
+    p->tune = *t;
+    return 0;
+}
+
+int ncg_get_tuning(ncg_problem *p, ncg_tuning *t)
+{
+    if (!p || !t) return fail(NCG_E_ARG, "ncg_get_tuning: null argument");
+    *t = p->tune;
+    return 0;
+}
+
 int ncg_solve_group(ncg_problem *const *probs, int32_t count, const ncg_solver_options *o, ncg_solve_stats *stats)
 {
     if (!probs || !o || count < 1) return fail(NCG_E_ARG, "ncg_solve_group: null argument or empty group");
@@ -2664,6 +2762,8 @@ int ncg_solve_group(ncg_problem *const *probs, int32_t count, const ncg_solver_o
     int rc = group_options(P[0]->precision, o);
     if (rc) return rc;
     NCG_HIP(hipSetDevice(P[0]->device));
+    for (ncg_problem *p : P)
+        if ((rc = assemble_finish(p))) return rc;
     if (stats) memset(stats, 0, sizeof(*stats) * (size_t)count);
     return solve_group(P, o, stats);
 }
@@ -2683,29 +2783,8 @@ int ncg_field(ncg_problem *p, double *phi, double *grad)
     if (!p->solved) return fail(NCG_E_STATE, "ncg_field: solve first");
     if (p->m == 0) return 0;
     NCG_HIP(hipSetDevice(p->device));
-    const int n = p->n, m = p->m, nr = p->nrhs;
-    int rc = ensure_work(p, 8);
-    if (rc) return rc;
-    double *pot = p->work[0], *vel = p->work[1];
-    if ((rc = p->out_phi.ensure(16 * (size_t)nr * m)) || (rc = p->out_grad.ensure(48 * (size_t)nr * m))) return rc;
-    double *dphi = p->out_phi.d(), *dgrad = p->out_grad.d();
-    record(p, p->ev_field.a);
-    ncgk::field_prep(p->stream, n, nr, p->kind, p->adm, p->interior ? -1.0 : 1.0, p->bcval, p->x, pot, vel);
-    if (p->plan.method != 0) {
-        rc = field_fmm(p, pot, vel, dphi, grad ? dgrad : nullptr);
-        if (rc) return rc;
-    } else {
-        ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
-                    grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
-    }
-    record(p, p->ev_field.b);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(phi, dphi, sizeof(cd) * nr * m, hipMemcpyDeviceToHost, p->stream);
-    if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, dgrad, sizeof(cd) * 3 * nr * m, hipMemcpyDeviceToHost, p->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    if (e != hipSuccess) return fail(NCG_E_DEVICE, std::string("ncg_field: ") + hipGetErrorString(e));
-    p->t_field = elapsed(p->ev_field);
-    return check_err(p);
+    int rc = field_enqueue(p, grad != nullptr);
+    return rc ? rc : field_collect(p, phi, grad);
 }
 
 int ncg_get_system(ncg_problem *p, double *A, double *b)
@@ -2714,6 +2793,8 @@ int ncg_get_system(ncg_problem *p, double *A, double *b)
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_get_system: assemble first");
     if (p->plan.method != 0) return fail(NCG_E_STATE, "ncg_get_system: the FMM operator is not stored densely");
     NCG_HIP(hipSetDevice(p->device));
+    int rcf = assemble_finish(p);
+    if (rcf) return rcf;
     const size_t nn = (size_t)p->n * p->n;
     if (A) {
         if (p->precision == NCG_MIXED) {
@@ -2734,7 +2815,9 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
     if (!p->assembled) return fail(NCG_E_STATE, "ncg_apply: assemble first");
     if (n_rhs < 1 || n_rhs > p->nrhs) return fail(NCG_E_ARG, "ncg_apply: n_rhs must be in [1, assembled n_rhs]");
     NCG_HIP(hipSetDevice(p->device));
-    int rc = ensure_work(p, 8);
+    int rc = assemble_finish(p);
+    if (rc) return rc;
+    rc = ensure_work(p, 8);
     if (rc) return rc;
     const size_t bytes = sizeof(cd) * (size_t)n_rhs * p->n;
     const int keep = p->nrhs;
@@ -2800,6 +2883,28 @@ int ncg_timings(ncg_problem *p, double *t, int32_t reset)
 namespace ncgi {
 
 int fail(int code, const std::string &msg) { return ::fail(code, msg); }
+
+int assemble_async(ncg_problem *p, const ncg_frequency *f, int precision)
+{
+    return assemble_impl(p, f, precision, false);
+}
+int assemble_wait(ncg_problem *p)
+{
+    if (hipSetDevice(p->device) != hipSuccess) return ::fail(NCG_E_DEVICE, "hipSetDevice failed");
+    return assemble_finish(p);
+}
+int field_begin(ncg_problem *p, bool want_grad)
+{
+    if (!p->solved) return ::fail(NCG_E_STATE, "ncg_field: solve first");
+    if (p->m == 0) return 0;
+    if (hipSetDevice(p->device) != hipSuccess) return ::fail(NCG_E_DEVICE, "hipSetDevice failed");
+    return field_enqueue(p, want_grad);
+}
+int field_end(ncg_problem *p, double *phi, double *grad)
+{
+    if (p->m == 0) return 0;
+    return field_collect(p, phi, grad);
+}
 
 void planner_mesh(const ncg_geometry *g, ncgf::Mesh &M, ncgf::Options &O)
 {

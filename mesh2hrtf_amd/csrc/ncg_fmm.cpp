@@ -184,7 +184,7 @@ void finish_level(const Mesh &M, Level &L)
 // cells of side h around its centre; every other pair is far.  The interaction list of
 // a cluster is its far clusters (level 0 and single level) or its far clusters whose
 // fathers are near its father (the children of the father's near clusters, ascending).
-void lists(Level &L, double min_dist, const Level *F)
+void lists(Level &L, double min_dist, const Level *F, double *dfar = nullptr)
 {
     const int nc = (int)L.cl.size();
     const double fac = 2.0 / sqrt(3.0);
@@ -221,6 +221,8 @@ void lists(Level &L, double min_dist, const Level *F)
     }
     long long geom_near = 0, canceled = 0;   // pairs j < i
     unsigned long long cancel_hash = 0;
+    // pairs outside the searched cells are farther apart than h
+    double dfar_seen = h;
     for (int i = 0; i < nc; ++i) {
         std::vector<int> &nr = L.cl[i].near;
         nr.clear();
@@ -245,6 +247,7 @@ void lists(Level &L, double min_dist, const Level *F)
                         const double w = L.cl[i].radius + L.cl[j].radius;
                         const bool far_geom = dd > fac * w;
                         const bool far = far_geom && dd > min_dist;
+                        if (far_geom) dfar_seen = std::min(dfar_seen, dd);
                         if (!far) nr.push_back(j);
                         if (j < i) {
                             if (!far_geom) geom_near++;
@@ -260,6 +263,7 @@ void lists(Level &L, double min_dist, const Level *F)
                 }
         std::sort(nr.begin(), nr.end());
     }
+    if (dfar) *dfar = dfar_seen;
     L.n_far_geom = (long long)nc * (nc - 1) / 2 - geom_near;
     L.n_canceled = canceled;
     L.cancel_hash = cancel_hash;
@@ -572,9 +576,25 @@ bool make_plan(const Mesh &M, const Options &O, double k, Plan &P)
             M.trees.push_back(std::move(t));
             tree = &M.trees.back();
         }
-        P.lev = tree->lev;
-        for (size_t lv = 0; lv < P.lev.size(); ++lv)
-            lists(P.lev[lv], P.min_dist, method == 3 && lv > 0 ? &P.lev[lv - 1] : nullptr);
+        // lists without cancelled pairs, once per tree
+        Tree *tw = const_cast<Tree *>(tree);
+        if (!tw->geo_ready) {
+            tw->geo = tw->lev;
+            tw->dfar.assign(tw->geo.size(), 0.0);
+            for (size_t lv = 0; lv < tw->geo.size(); ++lv)
+                lists(tw->geo[lv], 0.0, method == 3 && lv > 0 ? &tw->geo[lv - 1] : nullptr, &tw->dfar[lv]);
+            tw->geo_ready = true;
+        }
+        bool geo = true;
+        for (size_t lv = 0; lv < tw->geo.size(); ++lv)
+            if (!(P.min_dist < tw->dfar[lv])) geo = false;
+        if (geo) {
+            P.lev = tw->geo;
+        } else {
+            P.lev = tree->lev;
+            for (size_t lv = 0; lv < P.lev.size(); ++lv)
+                lists(P.lev[lv], P.min_dist, method == 3 && lv > 0 ? &P.lev[lv - 1] : nullptr);
+        }
         // canceled-pair ratios (src/NC_Addresses.cpp:224-283)
         P.cancel_ratio.clear();
         bool fallback = false;

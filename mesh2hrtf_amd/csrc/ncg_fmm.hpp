@@ -45,6 +45,12 @@ struct EvalCluster {
 struct Tree {
     int method = -1, nlev = 0;
     std::vector<Level> lev;
+    // the levels with their near and interaction lists when no pair is cancelled (min_dist
+    // below every level's shortest geometrically far centre distance, dfar): the lists of
+    // every such frequency, built once
+    bool geo_ready = false;
+    std::vector<Level> geo;
+    std::vector<double> dfar;
     bool ev_ready = false;
     double ev_edge = 0.0, ev_rmax = 0.0, ev_rmin = 0.0;
     std::vector<EvalCluster> ev;

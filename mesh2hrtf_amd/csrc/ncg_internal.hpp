@@ -14,6 +14,14 @@ namespace ncgi {
 // records the message ncg_last_error() returns on this thread and passes the code on
 int fail(int code, const std::string &msg);
 
+// ncg_assemble without the wait: the kernels are queued on the problem's stream and the
+// assembly's error flags are read by assemble_wait (or the next solve / apply)
+int assemble_async(ncg_problem *p, const ncg_frequency *f, int precision);
+int assemble_wait(ncg_problem *p);
+// ncg_field in two halves: queue the field kernels, then copy the results out (waits)
+int field_begin(ncg_problem *p, bool want_grad);
+int field_end(ncg_problem *p, double *phi, double *grad);
+
 // planner mesh and options of an ABI geometry
 void planner_mesh(const ncg_geometry *g, ncgf::Mesh &M, ncgf::Options &O);
 

@@ -86,7 +86,8 @@ __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, co
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int i0 = blockIdx.y * rows_per_block;
     const int i1 = min(i0 + rows_per_block, n);
-    if (j >= n) return;
+    unsigned long long nev = 0;
+    if (j < n) {
     const int nv = E.nv[j];
     const int kind = E.kind[j];
     const cx<T> admj = {T(adm[2 * j]), T(adm[2 * j + 1])};
@@ -101,7 +102,6 @@ __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, co
     const T e1x = T(P[0][0] - P[2][0]), e1y = T(P[0][1] - P[2][1]), e1z = T(P[0][2] - P[2][2]);
     const T e2x = T(P[1][0] - P[2][0]), e2y = T(P[1][1] - P[2][1]), e2z = T(P[1][2] - P[2][2]);
     const T hk = T(bm.hk), k2 = T(bm.k2), tau = T(bm.tau), bim = T(bm.beta_im);
-    unsigned long long nev = 0;
     for (int i = i0; i < i1; ++i) {
         if (i == j) continue;
         const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
@@ -138,7 +138,12 @@ __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, co
         const cx<T> a = bm_entry<T>(acc, kind, admj, tau, bim);
         A[(size_t)i * n + j] = cx<S>{S(a.re), S(a.im)};
     }
-    if (evals) atomicAdd(evals, nev);
+    }
+    // evaluation count (measurement only): one atomic per wave
+    if (evals) {
+        for (int o = 32; o > 0; o >>= 1) nev += __shfl_xor(nev, o);
+        if ((threadIdx.x & 63) == 0 && nev) atomicAdd(evals, nev);
+    }
 }
 
 // ======================================================================================
@@ -1957,12 +1962,11 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
 // of 64 sphere points); its waves take every kSumWaves-th element of the cluster, four
 // per step, and their partial sums meet in LDS.  W is the storage type of t.
-template <typename S, typename W, int NR>
+template <typename S, typename W, int NR, int NW = kSumWaves>
 __device__ __forceinline__ void apply_T_item(int blk, int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
                                              const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
 {
-    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
-    constexpr int NW = kSumWaves;
+    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
     const int tiles = (P + 63) / 64;
     const int c = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -2012,12 +2016,11 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, 
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
 // per (cluster, tile of 64 points), the waves splitting the interaction list four
 // entries per step.  WI / WO are the storage types of t and s.
-template <typename S, typename WI, typename WO, int NR>
+template <typename S, typename WI, typename WO, int NR, int NW = kSumWaves>
 __device__ __forceinline__ void apply_D_item(int blk, int ncl, int P, const int *fan_ptr, const int *fan_idx,
                                              const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
 {
-    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
-    constexpr int NW = kSumWaves;
+    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
     const int tiles = (P + 63) / 64;
     const int i = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -2074,13 +2077,12 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, 
 // element a0 + a of the cluster's column-major S block, so every load is a contiguous run
 // of the block; the waves split the points, four per step, s[c, p] is wave-uniform, and
 // the partial sums meet in an LDS tree.
-template <typename S, typename W, int NR>
+template <typename S, typename W, int NR, int NW = kSumWaves>
 __device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const int *tl_cl, const int *tl_a0,
                                              const int *tl_m, const int *cl_ptr, const int *cl_elem, const cx<S> *Sv,
                                              const cx<W> *sw, cx<double> *y)
 {
-    __shared__ double red[kSumWaves / 2 * 64 * 2 * NR];
-    constexpr int NW = kSumWaves;
+    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
     const int c = tl_cl[q], a0 = tl_a0[q], m = tl_m[q];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
@@ -2334,33 +2336,33 @@ __global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, co
                           (const cx<double> *)it.x, (cx<double> *)it.y);
 }
 
-template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
+template <typename S, typename W, int NR, int NW>
+__global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_T_item<S, W, NR>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem, (const cx<S> *)it.Tv,
+    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem, (const cx<S> *)it.Tv,
                            (const cx<double> *)it.x, (cx<W> *)it.tw);
 }
 
-template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_D(const ncgk::GLevel *items, const int *start, int nitems)
+template <typename S, typename W, int NR, int NW>
+__global__ void __launch_bounds__(64 * NW) kg_apply_D(const ncgk::GLevel *items, const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_D_item<S, W, W, NR>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx, (const cx<S> *)it.Dv,
+    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx, (const cx<S> *)it.Dv,
                               (const cx<W> *)it.tw, (cx<W> *)it.sw);
 }
 
-template <typename S, typename W, int NR>
-__global__ void __launch_bounds__(64 * kSumWaves) kg_apply_S(const ncgk::GLevel *items, const int *start, int nitems)
+template <typename S, typename W, int NR, int NW>
+__global__ void __launch_bounds__(64 * NW) kg_apply_S(const ncgk::GLevel *items, const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_S_item<S, W, NR>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.tl_cl, it.tl_a0, it.tl_m, it.cl_ptr,
+    apply_S_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.tl_cl, it.tl_a0, it.tl_m, it.cl_ptr,
                            it.cl_elem, (const cx<S> *)it.Sv, (const cx<W> *)it.sw, (cx<double> *)it.y);
 }
 
@@ -3430,47 +3432,77 @@ void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, 
 #undef NCG_CALL
 }
 
+// waves per work-group of the grouped far-field applies: 1, 2, 4 or 8
+#define NCG_G_NW(NW_, NR_, CALL) \
+    switch (NW_) {                \
+    case 1: NCG_G_NR(NR_, CALL##1) break; \
+    case 2: NCG_G_NR(NR_, CALL##2) break; \
+    case 4: NCG_G_NR(NR_, CALL##4) break; \
+    default: NCG_G_NR(NR_, CALL##8) break; \
+    }
 template <typename S, typename W>
-static void g_T_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+static void g_T_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL(NR) kg_apply_T<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
-    NCG_G_NR(nr, NCG_CALL)
-#undef NCG_CALL
+#define NCG_CALL1(NR) kg_apply_T<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
+#define NCG_CALL2(NR) kg_apply_T<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
+#define NCG_CALL4(NR) kg_apply_T<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
+#define NCG_CALL8(NR) kg_apply_T<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+    NCG_G_NW(nw, nr, NCG_CALL)
+#undef NCG_CALL1
+#undef NCG_CALL2
+#undef NCG_CALL4
+#undef NCG_CALL8
 }
 template <typename S, typename W>
-static void g_D_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+static void g_D_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL(NR) kg_apply_D<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
-    NCG_G_NR(nr, NCG_CALL)
-#undef NCG_CALL
+#define NCG_CALL1(NR) kg_apply_D<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
+#define NCG_CALL2(NR) kg_apply_D<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
+#define NCG_CALL4(NR) kg_apply_D<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
+#define NCG_CALL8(NR) kg_apply_D<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+    NCG_G_NW(nw, nr, NCG_CALL)
+#undef NCG_CALL1
+#undef NCG_CALL2
+#undef NCG_CALL4
+#undef NCG_CALL8
 }
 template <typename S, typename W>
-static void g_S_t(hipStream_t s, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+static void g_S_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL(NR) kg_apply_S<S, W, NR><<<blocks, 64 * kSumWaves, 0, s>>>(items, start, nitems);
-    NCG_G_NR(nr, NCG_CALL)
-#undef NCG_CALL
+#define NCG_CALL1(NR) kg_apply_S<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
+#define NCG_CALL2(NR) kg_apply_S<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
+#define NCG_CALL4(NR) kg_apply_S<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
+#define NCG_CALL8(NR) kg_apply_S<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+    NCG_G_NW(nw, nr, NCG_CALL)
+#undef NCG_CALL1
+#undef NCG_CALL2
+#undef NCG_CALL4
+#undef NCG_CALL8
 }
-void g_apply_T(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+#undef NCG_G_NW
+void g_apply_T(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks)
 {
     if (blocks <= 0) return;
-    if (f32 && !sig64) g_T_t<float, float>(s, nr, items, start, nitems, blocks);
-    else if (f32) g_T_t<float, double>(s, nr, items, start, nitems, blocks);
-    else g_T_t<double, double>(s, nr, items, start, nitems, blocks);
+    if (f32 && !sig64) g_T_t<float, float>(s, nw, nr, items, start, nitems, blocks);
+    else if (f32) g_T_t<float, double>(s, nw, nr, items, start, nitems, blocks);
+    else g_T_t<double, double>(s, nw, nr, items, start, nitems, blocks);
 }
-void g_apply_D(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+void g_apply_D(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks)
 {
     if (blocks <= 0) return;
-    if (f32 && !sig64) g_D_t<float, float>(s, nr, items, start, nitems, blocks);
-    else if (f32) g_D_t<float, double>(s, nr, items, start, nitems, blocks);
-    else g_D_t<double, double>(s, nr, items, start, nitems, blocks);
+    if (f32 && !sig64) g_D_t<float, float>(s, nw, nr, items, start, nitems, blocks);
+    else if (f32) g_D_t<float, double>(s, nw, nr, items, start, nitems, blocks);
+    else g_D_t<double, double>(s, nw, nr, items, start, nitems, blocks);
 }
-void g_apply_S(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems, int blocks)
+void g_apply_S(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
+               int blocks)
 {
     if (blocks <= 0) return;
-    if (f32 && !sig64) g_S_t<float, float>(s, nr, items, start, nitems, blocks);
-    else if (f32) g_S_t<float, double>(s, nr, items, start, nitems, blocks);
-    else g_S_t<double, double>(s, nr, items, start, nitems, blocks);
+    if (f32 && !sig64) g_S_t<float, float>(s, nw, nr, items, start, nitems, blocks);
+    else if (f32) g_S_t<float, double>(s, nw, nr, items, start, nitems, blocks);
+    else g_S_t<double, double>(s, nw, nr, items, start, nitems, blocks);
 }
 template <typename TB>
 static void g_bj_t(hipStream_t s, int nr, const GBj *items, const int *start, int nitems, int blocks)

@@ -269,12 +269,13 @@ void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, 
                  int blocks);
 // T and D of every item (each writes its own t or s); S adds into y, so a launch may hold at
 // most one level of a problem (the host launches one level index at a time).  sig64: t and
-// s in FP64 whatever the values' precision (the refinement residual).
-void g_apply_T(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+// s in FP64 whatever the values' precision (the refinement residual).  nw: waves per
+// work-group (1, 2, 4, 8) splitting each item's sum, met in an LDS tree.
+void g_apply_T(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
                int blocks);
-void g_apply_D(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+void g_apply_D(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
                int blocks);
-void g_apply_S(hipStream_t s, int f32, int sig64, int nr, const GLevel *items, const int *start, int nitems,
+void g_apply_S(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
                int blocks);
 void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks);
 void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks);

@@ -295,7 +295,8 @@ void run_lane(ncg_server *S)
             ncg_problem_destroy(e->p);
             delete e;
         }
-        // problems and operators of every step of the group
+        // problems and operators of every step of the group: every assembly is queued on its
+        // problem's stream first, so they run side by side, then their error flags are read
         for (Taken &t : grp) {
             const Job *J = t.st.job;
             const MeshRec *M = S->meshes[J->mesh].get();
@@ -305,10 +306,12 @@ void run_lane(ncg_server *S)
                 else if (S->cfg.kernel_timing) ncg_set_kernel_timing(t.r->p, 1);
             }
             if (!t.rc) {
-                t.rc = ncg_assemble(t.r->p, &J->freq[t.st.fi].f, J->precision);
+                t.rc = ncgi::assemble_async(t.r->p, &J->freq[t.st.fi].f, J->precision);
                 if (t.rc) t.msg = ncg_last_error();
             }
         }
+        for (Taken &t : grp)
+            if (!t.rc && (t.rc = ncgi::assemble_wait(t.r->p))) t.msg = ncg_last_error();
         const clk::time_point t_asm = clk::now();
         // solve: the group's assembled steps together, or the one step alone
         std::vector<Taken *> ok;
@@ -335,30 +338,38 @@ void run_lane(ncg_server *S)
             for (int c = 0; c < t.st.job->nrhs && c < 64; ++c) t.its += ss.iterations[c];
         }
         const clk::time_point t_solve = clk::now();
-        // outputs
-        for (Taken &t : grp) {
+        // outputs: the solutions, then every grid field queued before the first is read
+        std::vector<char> write(grp.size(), 1);
+        {
+            // a job with a step that failed meanwhile: its outputs are no longer wanted
+            std::lock_guard<std::mutex> g(S->mu);
+            for (size_t i = 0; i < grp.size(); ++i) write[i] = !grp[i].st.job->failed;
+        }
+        for (size_t i = 0; i < grp.size(); ++i) {
+            Taken &t = grp[i];
             Job *J = t.st.job;
+            if (!t.rc && write[i] && J->solution) {
+                t.rc = ncg_get_solution(t.r->p, J->solution + 2 * (size_t)J->nrhs * J->n * t.st.fi);
+                if (t.rc) t.msg = ncg_last_error();
+            }
+            if (!t.rc && write[i] && J->m > 0 && (J->phi || J->grad)) {
+                t.rc = ncgi::field_begin(t.r->p, J->grad != nullptr);
+                if (t.rc) t.msg = ncg_last_error();
+            }
+        }
+        for (size_t i = 0; i < grp.size(); ++i) {
+            Taken &t = grp[i];
+            Job *J = t.st.job;
+            if (t.rc || !write[i] || J->m == 0 || !(J->phi || J->grad)) continue;
             const size_t vec = 2 * (size_t)J->nrhs;
-            bool write = true;
-            {
-                // another step of the job failed meanwhile: its outputs are no longer wanted
-                std::lock_guard<std::mutex> g(S->mu);
-                write = !J->failed;
+            std::vector<double> tmp;
+            double *ph = J->phi ? J->phi + vec * J->m * t.st.fi : nullptr;
+            if (!ph) {
+                tmp.resize(vec * J->m);
+                ph = tmp.data();
             }
-            if (!t.rc && write && J->solution) {
-                t.rc = ncg_get_solution(t.r->p, J->solution + vec * J->n * t.st.fi);
-                if (t.rc) t.msg = ncg_last_error();
-            }
-            if (!t.rc && write && J->m > 0 && (J->phi || J->grad)) {
-                std::vector<double> tmp;
-                double *ph = J->phi ? J->phi + vec * J->m * t.st.fi : nullptr;
-                if (!ph) {
-                    tmp.resize(vec * J->m);
-                    ph = tmp.data();
-                }
-                t.rc = ncg_field(t.r->p, ph, J->grad ? J->grad + 3 * vec * J->m * t.st.fi : nullptr);
-                if (t.rc) t.msg = ncg_last_error();
-            }
+            t.rc = ncgi::field_end(t.r->p, ph, J->grad ? J->grad + 3 * vec * J->m * t.st.fi : nullptr);
+            if (t.rc) t.msg = ncg_last_error();
         }
         std::vector<std::vector<ncg_kernel_stat>> kst(grp.size());
         for (size_t i = 0; i < grp.size(); ++i) {

@@ -89,7 +89,13 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
            "ncg_assemble", "ncg_solve", "ncg_problem_fmm_info", "ncg_get_solution", "ncg_field", "ncg_get_system",
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
            "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
-           "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group")
+           "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group",
+           "ncg_set_tuning", "ncg_get_tuning")
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("t_waves", ctypes.c_int32), ("d_waves", ctypes.c_int32), ("s_waves", ctypes.c_int32),
+                ("tuned", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class _KernelStat(ctypes.Structure):
@@ -280,6 +286,15 @@ class Problem:
         cnt = ctypes.c_int32()
         _check(self.lib.ncg_kernel_stats(self._h, arr, 16, ctypes.byref(cnt), int(reset)))
         return _kernel_dict(arr, cnt.value)
+
+    def tuning(self):
+        t = _Tuning()
+        _check(self.lib.ncg_get_tuning(self._h, ctypes.byref(t)))
+        return dict(t_waves=t.t_waves, d_waves=t.d_waves, s_waves=t.s_waves, tuned=t.tuned)
+
+    def set_tuning(self, t_waves=8, d_waves=8, s_waves=8):
+        t = _Tuning(int(t_waves), int(d_waves), int(s_waves), 0, 0)
+        _check(self.lib.ncg_set_tuning(self._h, ctypes.byref(t)))
 
     def timings(self, reset=False):
         t = (ctypes.c_double * 8)()
