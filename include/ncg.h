@@ -210,6 +210,16 @@ int ncg_get_system(ncg_problem *p, double *A, double *b);
 /* y = A x for n_rhs vectors [n_rhs][n] (host buffers), with the stored operator. */
 int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
 
+/* y = A x like ncg_apply, but on device buffers (x, y: [n_rhs][n] complex in device memory
+ * of the problem's device) and without allocating anything: the apply's scratch (FMM
+ * signatures, matrix-core operand and partial sums) comes from one 64 MiB array per device
+ * that the library holds until the process ends; an apply that needs more fails with
+ * NCG_E_ARG.  Calls on one device share that array: calls that overlap on one device race.
+ * Returns after the apply has completed on the problem's stream.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid allocation). */
+int ncg_apply_noalloc(ncg_problem *p, const double *x, double *y, int32_t n_rhs);
+
 /* Per-phase device timings of the last assemble/solve/field calls, and the average
  * duration of the dominant kernel of each phase over the calls since the last reset
  * (HIP events on the problem's stream).  Layout of t[8]:

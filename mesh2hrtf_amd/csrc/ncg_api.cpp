@@ -21,6 +21,7 @@
 
 #include <array>
 #include <map>
+#include <mutex>
 
 using cd = std::complex<double>;
 
@@ -469,16 +470,29 @@ struct KernelScope {
     ~KernelScope();
 };
 
-void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
+// Scratch of an allocation-free apply (ncg_apply_noalloc): the FMM signatures t and s of
+// every level, or the matrix-core operand and partial sums of a dense operator, carved from
+// one caller-independent device array; no events are recorded.
+struct MvScratch {
+    char *base = nullptr;
+    size_t bytes = 0;
+};
+
+void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y, const MvScratch *na = nullptr)
 {
-    if (p->mv_used == p->ev_mv.size()) {
-        EventPair e;
-        hipEventCreate(&e.a);
-        hipEventCreate(&e.b);
-        p->ev_mv.push_back(e);
+    EventPair *e = nullptr;
+    if (!na) {
+        if (p->mv_used == p->ev_mv.size()) {
+            EventPair ep;
+            hipEventCreate(&ep.a);
+            hipEventCreate(&ep.b);
+            p->ev_mv.push_back(ep);
+        }
+        e = &p->ev_mv[p->mv_used++];
+        record(p, e->a);
     }
-    EventPair &e = p->ev_mv[p->mv_used++];
-    record(p, e.a);
+    const int ktime_keep = p->ktime;
+    if (na) p->ktime = 0;
     const double n = p->n, nr = p->nrhs, chunks = std::ceil(nr / 4.0);
     if (p->plan.method == 0) {
         const int mixed = p->precision == NCG_MIXED;
@@ -491,16 +505,26 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
         // algorithmic bytes: A once per pass of (4 vector / 16 matrix-core) load cases, x and y
         const double vec = 2.0 * nr * n * 16.0, abytes = n * n * (mixed ? 8.0 : 16.0);
         const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, fp64acc);
-        if (!d && p->nrhs > 4 && !p->mm_xh.ensure(el * mp.xh_elems) && !p->mm_part.ensure(el * mp.part_elems)) {
+        void *xh = nullptr, *part = nullptr;
+        bool use_mfma = !d && p->nrhs > 4;
+        if (use_mfma && na) {
+            xh = na->base;
+            part = na->base + ((el * mp.xh_elems + 255) & ~(size_t)255);
+        } else if (use_mfma) {
+            use_mfma = !p->mm_xh.ensure(el * mp.xh_elems) && !p->mm_part.ensure(el * mp.part_elems);
+            xh = p->mm_xh.p;
+            part = p->mm_part.p;
+        }
+        if (use_mfma) {
             for (int r0 = 0; r0 < p->nrhs; r0 += 16) {
                 const int nq = std::min(16, p->nrhs - r0);
-                ncgk::mfma_xhat(p->stream, mp, p->n, nq, x + 2 * (size_t)r0 * p->n, p->mm_xh.p);
+                ncgk::mfma_xhat(p->stream, mp, p->n, nq, x + 2 * (size_t)r0 * p->n, xh);
                 {
                     // the product kernel alone: A, this pass's x and y
                     KernelScope ks(p, fp64acc ? 11 : 10, abytes + 2.0 * nq * n * 16.0);
-                    ncgk::mfma_product(p->stream, mp, p->A, mixed, p->n, p->mm_xh.p, p->mm_part.p);
+                    ncgk::mfma_product(p->stream, mp, p->A, mixed, p->n, xh, part);
                 }
-                ncgk::mfma_sum(p->stream, mp, p->n, nq, p->mm_part.p, y + 2 * (size_t)r0 * p->n);
+                ncgk::mfma_sum(p->stream, mp, p->n, nq, part, y + 2 * (size_t)r0 * p->n);
             }
         } else {
             KernelScope ks(p, 8 + mixed, chunks * abytes + vec);
@@ -521,28 +545,48 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y)
             ncgk::near_spmv(p->stream, p->n, p->nrhs, p->fmm.near_chunks(), p->fmm.near_col, p->fmm.near_wmax,
                             p->fmm.near_val.p, mixed, xin, y);
         }
+        char *sc = na ? na->base : nullptr;
         for (DevLevel &L : p->fmm.lev) {
             // algorithmic bytes: the operator once per pass of at most 4 load cases, the
             // vectors once (t and s in the level's storage precision)
             const double sf = L.f32 ? 8.0 : 16.0, P = L.P, ncl = L.ncl, nfan = L.nfan;
+            void *tw = L.tw.p, *sw = L.sw.p;
+            if (sc) {
+                const size_t sig = ((size_t)p->nrhs * L.ncl * L.P * (L.f32 ? 8 : 16) + 255) & ~(size_t)255;
+                tw = sc;
+                sw = sc + sig;
+                sc += 2 * sig;
+            }
             {
                 KernelScope ks(p, 2 + L.f32, chunks * n * P * sf + nr * n * 16.0 + nr * ncl * P * sf);
-                ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, 0, xin,
-                                  L.tw.p);
+                ncgk::fmm_apply_T(p->stream, p->n, L.ncl, L.P, p->nrhs, L.cl_ptr, L.cl_elem, L.T.p, L.f32, 0, xin, tw);
             }
             {
                 KernelScope ks(p, 4 + L.f32, chunks * nfan * P * sf + 2.0 * nr * ncl * P * sf);
-                ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, 0, L.tw.p,
-                                  L.sw.p);
+                ncgk::fmm_apply_D(p->stream, L.ncl, L.P, p->nrhs, L.fan_ptr, L.fan_idx, L.D.p, L.f32, 0, tw, sw);
             }
             {
                 KernelScope ks(p, 6 + L.f32, chunks * n * P * sf + nr * ncl * P * sf + 2.0 * nr * n * 16.0);
                 ncgk::fmm_apply_S(p->stream, p->n, L.ncl, L.P, p->nrhs, L.tiles(), L.cl_ptr, L.cl_elem, L.S.p, L.f32,
-                                  L.sw.p, y);
+                                  sw, y);
             }
         }
     }
-    record(p, e.b);
+    p->ktime = ktime_keep;
+    if (e) record(p, e->b);
+}
+
+// scratch bytes of one allocation-free apply of n_rhs load cases (see MvScratch)
+size_t mv_scratch_bytes(const ncg_problem *p, int n_rhs)
+{
+    if (p->plan.method == 0) {
+        if (n_rhs <= 4) return 0;
+        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, 1);
+        return ((sizeof(double) * mp.xh_elems + 255) & ~(size_t)255) + sizeof(double) * mp.part_elems;
+    }
+    size_t b = 0;
+    for (const DevLevel &L : p->fmm.lev) b += 2 * (((size_t)n_rhs * L.ncl * L.P * (L.f32 ? 8 : 16) + 255) & ~(size_t)255);
+    return b;
 }
 
 void harvest_mv(ncg_problem *p)
@@ -2827,6 +2871,66 @@ int ncg_apply(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
     p->nrhs = keep;
     NCG_HIP(hipMemcpyAsync(y, p->work[1], bytes, hipMemcpyDeviceToHost, p->stream));
     harvest_mv(p);
+    NCG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// ---- allocation-free apply ------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Scratch of ncg_apply_noalloc: one array per device, held until the process ends.  In host
+// code its name is only a stand-in for the device address, which hipGetSymbolAddress
+// gives once per device (kept in scratch_ptr); calls on one device share it.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {nullptr};
+
+int scratch_address(int device, void **out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(NCG_E_ARG, "ncg_apply_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(NCG_E_DEVICE, std::string("hipGetSymbolAddress (scratch): ") + hipGetErrorString(e));
+        scratch_ptr[device] = ptr;
+    }
+    *out = scratch_ptr[device];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ncg_apply_noalloc(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
+{
+    if (!p || !x || !y) return fail(NCG_E_ARG, "ncg_apply_noalloc: null argument");
+    if (!p->assembled) return fail(NCG_E_STATE, "ncg_apply_noalloc: assemble first");
+    if (n_rhs < 1 || n_rhs > p->nrhs)
+        return fail(NCG_E_ARG, "ncg_apply_noalloc: n_rhs must be in [1, assembled n_rhs]");
+    NCG_HIP(hipSetDevice(p->device));
+    int rc = assemble_finish(p);
+    if (rc) return rc;
+    const size_t need = mv_scratch_bytes(p, n_rhs);
+    if (need > kScratchBytes)
+        return fail(NCG_E_ARG, "ncg_apply_noalloc: this apply needs " + std::to_string(need >> 20) +
+                                   " MiB of scratch, more than the " + std::to_string(kScratchBytes >> 20) + " MiB it has");
+    void *base = nullptr;
+    if ((rc = scratch_address(p->device, &base))) return rc;
+    MvScratch na;
+    na.base = (char *)base;
+    na.bytes = kScratchBytes;
+    const int keep = p->nrhs;
+    p->nrhs = n_rhs;
+    mv(p, 1, x, nullptr, y, &na);
+    p->nrhs = keep;
+    NCG_HIP(hipGetLastError());
     NCG_HIP(hipStreamSynchronize(p->stream));
     return 0;
 }

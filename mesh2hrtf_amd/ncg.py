@@ -90,7 +90,7 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
            "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
            "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group",
-           "ncg_set_tuning", "ncg_get_tuning")
+           "ncg_set_tuning", "ncg_get_tuning", "ncg_apply_noalloc")
 
 
 class _Tuning(ctypes.Structure):
