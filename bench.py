@@ -4,38 +4,42 @@ Workload (the configuration the metric is quoted on): seeded head-like mesh, ell
 0.075/0.09/0.11 m radially perturbed by a real spherical-harmonic sum (degree <= 8, RMS
 3 mm, seed 0), geodesic frequency 32 -> 20480 triangles; two reciprocal point sources
 1 mm off the surface at the +-y ear positions; 1730-point Fibonacci evaluation grid at
-1.2 m; frequencies 200..16000 Hz step 200 Hz, method as NumCalc chooses it for
-methodBEM 4 (SL-FMM below lambda/D = 1, two-level ML-FMM above).  A step is one frequency:
-operator assembly, the solve of both sources and the field at the evaluation grid.
+1.2 m; frequencies 200..16000 Hz step 200 Hz, method as NumCalc chooses it for methodBEM 4
+(SL-FMM below lambda/D = 1, two-level ML-FMM above).  A frequency step is operator assembly,
+the solve of both sources and the field at the evaluation grid.  A bench *step* is one pass
+over the workload: the 80-frequency sweep with both sources (160 solves), one job of the
+serving runtime.
 
 value        = frequency-solves/s (one solve = one frequency x one source) of the served
-               mixed-precision form: K steps (by default three passes over the sweep) are
-               submitted at once to the serving
-               runtime (ncg.Server: `lanes` steps in flight on their own streams, problems
-               resident across frequencies), FP32-complex bulk operator with FP64
+               mixed-precision form over the K timed sweeps, submitted at once to the serving
+               runtime (ncg.Server: lanes of grouped steps on their own streams, problems
+               resident across frequencies): FP32-complex bulk operator with FP64
                near/singular integrals, FP64 Krylov dots and refinement residual, GMRES with
-               the block-Jacobi preconditioner.  Only solves whose surface and grid pressures
-               are within rel-L2 1e-3 of the FP64 plain method at the same frequency count.
+               the block-Jacobi preconditioner.  Only solves checked against the FP64 plain
+               method at the same frequency and within rel-L2 1e-3 of it (surface and grid
+               pressure) count; checked_solves says how many were compared.
 plain_fp64   = the FP64 plain method over the sweep's frequencies, one step after the other
                as the drop-in NumCalc runs them: NumCalc's method choice, CGS with its ILU
                preconditioner (IPRECOND 0, what mesh2input writes), tolerance 1e-9.
 p50/p99      = per-request latency of single-frequency requests (both sources) arriving as a
                Poisson stream at 70% of the measured throughput.
-roofline     = the kernel with the largest share of the served run's timed kernels, timed
-               with HIP events on one problem of the same mesh at 8000 Hz with nothing else
-               on the GPU: algorithmic bytes per launch / mean launch time.  `served` holds
-               the same kernel's figures in the served run, where the lanes share the GPU.
-               `traffic` is the measured HBM bytes per launch of that single-problem timing
-               from the committed counter profile (rocprofv3 FETCH_SIZE and WRITE_SIZE,
-               calibrated on a copy kernel).
-cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) on one
-               frequency of one source of the same mesh, on one host core.
+roofline     = the apply kernel with the largest summed launch time in the timed region, from
+               HIP events around each of its launches on the stream it runs on: algorithmic
+               bytes per launch / mean launch time.  `traffic` is the measured HBM bytes per
+               launch from the committed counter profile (rocprofv3 FETCH_SIZE and
+               WRITE_SIZE, calibrated on a copy kernel).
+cpu_baseline = reference NumCalc (oracle/_ref, built from the reference sources) run the way
+               manage_numcalc runs it, through the project runner: one process per (source,
+               frequency step) of a sample of the sweep, at most one per physical core
+               (bounded by --cpu-cores), admitted by -estimate_ram; pool throughput, per-job
+               p50/p99, and the GPU FP64 results' rel-L2 against its be.out.
 
 --workload graded runs the same measurement on the graded large case (60320 triangles, edges
-1 mm near the ears to 8 mm on the far sides, 40 frequencies 12-23.7 kHz, roofline launch at
-18 kHz).  --workload batched measures the batched uneven-problems configuration instead: 256 seeded
-small meshes (500-6000 triangles) x 8 frequencies x 16 plane-wave load cases, dense BEM,
-all steps submitted at once; its roofline is the matrix-core multi-load-case apply.
+1 mm near the ears to 8 mm on the far sides, 40 frequencies 12-23.7 kHz).  --workload batched
+measures the batched uneven-problems configuration instead: 256 seeded small meshes
+(500-6000 triangles) x 8 frequencies x 16 plane-wave load cases, dense BEM, all steps
+submitted at once; its roofline is the matrix-core multi-load-case apply.  --workload stream
+serves Poisson requests over 32 head-like meshes of 5k-40k triangles.
 """
 import argparse
 import json
@@ -328,19 +332,256 @@ def stream_main(a):
     print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
 
 
+def cpu_pool(mesh, grid, src, freqs, cores, timeout):
+    """Reference NumCalc (oracle/_ref, built from the reference sources) run as
+    manage_numcalc runs it: a project with one source folder per ear source, one process per
+    (source, frequency step) through the project runner, at most `cores` at once, admitted
+    by the -estimate_ram figures.  Returns the pool's figures and the reference be.out of
+    every step, or None without the reference build."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
+    if not os.path.exists(exe):
+        return None
+    from mesh2hrtf_amd import runner
+    d = tempfile.mkdtemp(prefix="ncpool_")
+    try:
+        prj = P.Project(mesh=mesh, freqs=list(freqs), eval_grid=grid, method_bem=4, c=C, rho=RHO,
+                        title="CPU baseline pool")
+        folders = P.write_sources(d, prj, src)
+        t_est = time.perf_counter()
+        every, todo = runner.jobs_of(d, exe)
+        est_s = time.perf_counter() - t_est
+        res = runner.run(d, exe, max_instances=cores, jobs=todo)
+        out = {}
+        for k, folder in enumerate(folders):
+            for step, f in enumerate(freqs, start=1):
+                path = os.path.join(folder, "be.out", "be.%d" % step)
+                if os.path.isfile(os.path.join(path, "pEvalGrid")):
+                    out[(k, f)] = (P.read_be_file(os.path.join(path, "pBoundary"))[1],
+                                   P.read_be_file(os.path.join(path, "pEvalGrid"))[1])
+        return res, out, est_s
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def host_cpu():
+    """lscpu figures of this host: sockets, cores per socket, threads per core, model"""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "Model name", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def head_main(a):
+    if a.workload == "graded":
+        freqs, mesh, iso_f = P.config3_freqs(), P.graded_head_mesh(), 18000.0
+    else:
+        freqs, mesh, iso_f = P.config2_freqs(), P.config2_mesh(a.nu), 8000.0
+    grid = P.fibonacci_grid(1730, 1.2)
+    src = P.ear_sources(mesh)
+    n, m, nr = len(mesh.elems), len(grid.nodes), len(src)
+    geo = dict(nv=np.full(n, 3), xyz=mesh.nodes[mesh.elems], kind=np.zeros(n, np.int32), eval_xyz=grid.nodes,
+               node_index=mesh.elems, method_bem=4)
+    nf = len(freqs)
+    print("# n=%d eval=%d sources=%d lanes=%d group=%d; a step is one pass over %d frequencies; %d timed steps" % (
+        n, m, nr, a.lanes, a.group, nf, a.steps), file=sys.stderr, flush=True)
+
+    # ---- served mixed form: every step (sweep) submitted at once ------------------------
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group)
+    mid = srv.add_mesh(**geo)
+    sweep = [freq_spec(src, f) for f in freqs]
+    for j in [srv.submit(mid, sweep, **SERVED) for _ in range(a.warmup)]:
+        j.wait()
+    srv.kernel_stats(reset=True)
+    lane0 = dict(srv.stats()["lane_ms"])
+    t0 = time.perf_counter()
+    jobs = [srv.submit(mid, sweep, **SERVED) for _ in range(a.steps)]
+    sts = [j.wait() for j in jobs]
+    elapsed = time.perf_counter() - t0
+    kst = srv.kernel_stats()
+    lane1 = srv.stats()["lane_ms"]
+    failed = [s for s in sts if s["state"] != "done"]
+    if failed:
+        print("# served steps failed: %s" % failed[:3], file=sys.stderr, flush=True)
+    print("# served: %d sweeps (%d frequency steps) in %.2f s, %.2f ms per frequency step" % (
+        a.steps, a.steps * nf, elapsed, elapsed / (a.steps * nf) * 1e3), file=sys.stderr, flush=True)
+
+    # ---- latency: single-frequency Poisson requests at 70% of the measured rate ---------
+    rate = 0.7 * a.steps * nf / elapsed
+    rng = np.random.default_rng(1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, a.poisson))
+    pfreqs = [freqs[i] for i in rng.integers(0, nf, a.poisson)]
+    pjobs = []
+    t0p = time.perf_counter()
+    for t_arr, f in zip(arrivals, pfreqs):
+        dt = t_arr - (time.perf_counter() - t0p)
+        if dt > 0:
+            time.sleep(dt)
+        pjobs.append(srv.submit(mid, [freq_spec(src, f)], **SERVED))
+    pl = [j.wait()["latency_ms"] for j in pjobs] or [float("nan")]
+    print("# poisson %.1f req/s: latency p50 %.1f ms p99 %.1f ms" % (rate, np.percentile(pl, 50), np.percentile(pl, 99)),
+          file=sys.stderr, flush=True)
+    info = srv.stats()
+    srv.close()
+
+    # ---- FP64 plain method (the drop-in), one frequency after the other -----------------
+    nplain = nf if a.plain_steps < 0 else min(a.plain_steps, nf)
+    plain_f = pick(freqs, nplain)
+    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
+                       method_bem=4)
+    sp = freq_spec(src, plain_f[0])
+    prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
+    prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+    ref, plain_lat = {}, []
+    t0 = time.perf_counter()
+    for f in plain_f:
+        ts = time.perf_counter()
+        sp = freq_spec(src, f)
+        prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
+        prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+        x0 = prob.solution()
+        p0, _ = prob.field(gradient=False)
+        plain_lat.append((time.perf_counter() - ts) * 1e3)
+        ref[f] = (x0, p0)
+    plain_elapsed = time.perf_counter() - t0
+    prob.close()
+    print("# plain FP64: %d frequencies in %.1f s" % (nplain, plain_elapsed), file=sys.stderr, flush=True)
+
+    # every timed solve against the plain solution at its frequency; only checked solves count
+    err_s, err_e, checked, valid = [], [], 0, 0
+    for j, s in zip(jobs, sts):
+        for q, f in enumerate(freqs):
+            if f not in ref or s["state"] != "done":
+                continue
+            x0, p0 = ref[f]
+            es = max(np.linalg.norm(j.solution[q][r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(nr))
+            ee = max(np.linalg.norm(j.phi[q][r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(nr))
+            err_s.append(es)
+            err_e.append(ee)
+            checked += nr
+            valid += nr if (es <= 1e-3 and ee <= 1e-3) else 0
+    value = valid / elapsed
+
+    # ---- roofline: the apply kernel with the largest share of the timed region ----------
+    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
+    roof = None
+    if top:
+        s = kst[top]
+        ach = s["bytes"] / (s["total_ms"] * 1e-3)
+        traffic, tsrc = profiled_traffic(top)
+        roof = {"bound": "hbm", "kernel": top, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": traffic, "traffic_source": tsrc,
+                "bytes_per_launch": s["bytes"] / s["calls"], "mean_launch_ms": s["total_ms"] / s["calls"],
+                "launches": s["calls"],
+                "timing": "HIP events around each launch on the group's stream over the timed region, "
+                          "%d lanes sharing the GPU" % a.lanes,
+                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in sorted(kst)}}
+        if not a.no_isolated:
+            iso = isolated_kernels(geo, src, iso_f)
+            roof["isolated_single_problem"] = {
+                k: {"mean_launch_ms": v["total_ms"] / v["calls"], "achieved_gbs": v["bytes"] / v["total_ms"] / 1e6}
+                for k, v in iso.items() if v["calls"]}
+
+    # ---- CPU baseline: the reference NumCalc as a manage_numcalc-style pool -------------
+    cpu = None
+    if not a.no_cpu:
+        from mesh2hrtf_amd import runner
+        cores = min(runner.physical_cores(), a.cpu_cores)
+        cf = pick(freqs, max(1, cores // nr))
+        pool = cpu_pool(mesh, grid, src, cf, cores, a.cpu_timeout)
+        if pool:
+            res, out, est_s = pool
+            sm = res.summary()
+            e_ref = []
+            for (k, f), (pB, pE) in out.items():
+                if f in ref:
+                    x0, p0 = ref[f]
+                    rp = RHO * 2 * np.pi * f
+                    e_ref.append(max(np.linalg.norm(1j * rp * x0[k] - pB) / np.linalg.norm(pB),
+                                     np.linalg.norm(1j * rp * p0[k] - pE) / np.linalg.norm(pE)))
+            cpu = {"value": sm["jobs"] / sm["wall_s"], "unit": "frequency-solves/s", "cores": cores,
+                   "kind": "reference",
+                   "sample": "reference NumCalc (oracle/_ref, methodBEM 4: SL/ML-FMM, ILU-CGS) on the same %d-element "
+                             "mesh with the 1730-point grid, run like manage_numcalc: one process per (source, "
+                             "frequency step), %d at once, %d jobs (%d frequencies %s Hz x %d sources), admitted by "
+                             "-estimate_ram; wall from the first start to the last exit" % (
+                                 n, cores, sm["jobs"], len(cf), "/".join("%.0f" % f for f in cf), nr),
+                   "wall_s": sm["wall_s"], "job_p50_s": sm["p50_s"], "job_p99_s": sm["p99_s"],
+                   "max_running": sm["max_running"], "failed_jobs": sm["failed"], "estimate_ram_s": est_s,
+                   "host": host_cpu(),
+                   "gpu_fp64_vs_reference_rel_l2_max": float(max(e_ref)) if e_ref else None}
+            print("# CPU pool: %d jobs in %.1f s on %d cores; GPU FP64 vs reference max rel-L2 %s" % (
+                sm["jobs"], sm["wall_s"], cores, cpu["gpu_fp64_vs_reference_rel_l2_max"]), file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "frequency-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": ("head-like mesh %d triangles (seed 0), 2 ear point sources, 1730-point grid at 1.2 m, "
+                                "frequencies 200-16000 Hz step 200; NumCalc methodBEM 4 (SL/ML-FMM); one step = one "
+                                "pass over the %d frequencies x 2 sources" % (n, nf))
+                   if a.workload == "head" else
+                   ("graded head-like mesh %d triangles (seed 0; 1 mm edges near the ears to 8 mm), 2 ear point "
+                    "sources, 1730-point grid at 1.2 m, frequencies 12000-23700 Hz step 300; NumCalc methodBEM 4; "
+                    "one step = one pass over the %d frequencies x 2 sources" % (n, nf)),
+                   "form": "served mixed precision: FP32 operator and inner GMRES(50) + block-Jacobi, FP64 refinement "
+                           "to relative residual %g (residual with FP64 accumulation and FMM signatures); %d lanes, "
+                           "%d steps per grouped solve" % (SERVED["refinement_tolerance"], a.lanes, a.group),
+                   "frequencies": nf},
+        "p50_ms": float(np.percentile(pl, 50)),
+        "p99_ms": float(np.percentile(pl, 99)),
+        "latency": {"requests": a.poisson, "poisson_rate_per_s": rate,
+                    "what": "single-frequency requests (both sources) arriving as a Poisson stream at 70% of the "
+                            "measured rate; submit to result"},
+        "ms_per_frequency_step": elapsed / (a.steps * nf) * 1e3,
+        "valid_solves": valid,
+        "checked_solves": checked,
+        "solves": a.steps * nf * nr,
+        "plain_fp64": {"value": nplain * nr / plain_elapsed, "ms_per_frequency": plain_elapsed / nplain * 1e3,
+                       "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99)),
+                       "frequencies": nplain,
+                       "form": "the drop-in's FP64 plain method: NumCalc's method choice, ILU-CGS to 1e-9, one "
+                               "frequency at a time"},
+        "speedup_vs_plain": value / (nplain * nr / plain_elapsed),
+        "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
+                                 "eval_grid_max": float(max(err_e)) if err_e else None},
+        "server": {"peak_resident_bytes": info["peak_resident_bytes"],
+                   "lane_ms_timed": {k: round(lane1[k] - lane0[k], 1) for k in lane1}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=240,
-                    help="frequency steps in the timed region (the 80-frequency sweep, cyclically)")
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed steps; a step is one pass over the workload's frequency sweep (80 frequencies x 2 "
+                         "sources for the head mesh)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed sweeps")
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--group", type=int, default=4, help="served steps a lane solves together (ncg_solve_group)")
     ap.add_argument("--nu", type=int, default=32, help="geodesic frequency of the mesh (32 -> 20480 triangles)")
     ap.add_argument("--plain-steps", type=int, default=-1,
-                    help="frequencies the FP64 plain method runs (-1: every frequency of the timed steps)")
-    ap.add_argument("--poisson", type=int, default=-1, help="requests of the latency run (-1: steps)")
+                    help="frequencies the FP64 plain method runs (-1: every frequency of the sweep)")
+    ap.add_argument("--poisson", type=int, default=160, help="requests of the latency run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cores", type=int, default=16, help="CPU baseline: at most this many NumCalc processes")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--stream-meshes", type=int, default=32, help="stream: number of meshes")
@@ -361,162 +602,7 @@ def main():
         return batched_main(a)
     if a.workload == "stream":
         return stream_main(a)
-
-    if a.workload == "graded":
-        freqs, mesh, iso_f = P.config3_freqs(), P.graded_head_mesh(), 18000.0
-    else:
-        freqs, mesh, iso_f = P.config2_freqs(), P.config2_mesh(a.nu), 8000.0
-    grid = P.fibonacci_grid(1730, 1.2)
-    src = P.ear_sources(mesh)
-    n, m, nr = len(mesh.elems), len(grid.nodes), len(src)
-    geo = dict(nv=np.full(n, 3), xyz=mesh.nodes[mesh.elems], kind=np.zeros(n, np.int32), eval_xyz=grid.nodes,
-               node_index=mesh.elems, method_bem=4)
-    timed = pick(freqs, a.steps)
-    print("# n=%d eval=%d sources=%d lanes=%d; %d timed frequencies" % (n, m, nr, a.lanes, len(timed)),
-          file=sys.stderr, flush=True)
-
-    # ---- served mixed form: all K steps submitted at once -------------------------------
-    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group)
-    mid = srv.add_mesh(**geo)
-    warm = [srv.submit(mid, [freq_spec(src, f)], **SERVED) for f in pick(freqs, a.warmup, offset=1)]
-    for j in warm:
-        j.wait()
-    srv.kernel_stats(reset=True)
-    t0 = time.perf_counter()
-    jobs = [srv.submit(mid, [freq_spec(src, f)], **SERVED) for f in timed]
-    sts = [j.wait() for j in jobs]
-    elapsed = time.perf_counter() - t0
-    kst = srv.kernel_stats()
-    failed = [s for s in sts if s["state"] != "done"]
-    if failed:
-        print("# served steps failed: %s" % failed[:3], file=sys.stderr, flush=True)
-    batch_lat = [s["latency_ms"] for s in sts]
-    print("# served: %d steps in %.2f s (%.1f ms/step), batch latency p50 %.0f ms p99 %.0f ms" % (
-        len(timed), elapsed, elapsed / len(timed) * 1e3, np.percentile(batch_lat, 50), np.percentile(batch_lat, 99)),
-        file=sys.stderr, flush=True)
-
-    # ---- latency: Poisson arrivals at 70% of the measured step rate ---------------------
-    npois = a.steps if a.poisson < 0 else a.poisson
-    rate = 0.7 * len(timed) / elapsed
-    rng = np.random.default_rng(1)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, npois))
-    pfreqs = [freqs[i] for i in rng.integers(0, len(freqs), npois)]
-    pjobs = []
-    t0p = time.perf_counter()
-    for t_arr, f in zip(arrivals, pfreqs):
-        dt = t_arr - (time.perf_counter() - t0p)
-        if dt > 0:
-            time.sleep(dt)
-        pjobs.append(srv.submit(mid, [freq_spec(src, f)], **SERVED))
-    pl = [j.wait()["latency_ms"] for j in pjobs] or [float("nan")]
-    print("# poisson %.1f req/s: latency p50 %.1f ms p99 %.1f ms" % (rate, np.percentile(pl, 50), np.percentile(pl, 99)),
-          file=sys.stderr, flush=True)
-    info = srv.stats()
-    srv.close()
-
-    # ---- FP64 plain method (the drop-in), one step after the other ----------------------
-    uniq = sorted(set(timed))
-    nplain = len(uniq) if a.plain_steps < 0 else min(a.plain_steps, len(uniq))
-    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
-                       method_bem=4)
-    sp = freq_spec(src, uniq[0])
-    prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
-    prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
-    ref, plain_lat = {}, []
-    t0 = time.perf_counter()
-    for f in uniq[:nplain]:
-        ts = time.perf_counter()
-        sp = freq_spec(src, f)
-        prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
-        prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
-        x0 = prob.solution()
-        p0, _ = prob.field(gradient=False)
-        plain_lat.append((time.perf_counter() - ts) * 1e3)
-        ref[f] = (x0, p0)
-        print("# plain f=%5.0f %.1f ms" % (f, plain_lat[-1]), file=sys.stderr, flush=True)
-    plain_elapsed = time.perf_counter() - t0
-    prob.close()
-
-    # every timed step against the plain solution at its frequency
-    err_s, err_e, ok = [], [], []
-    for f, j, s in zip(timed, jobs, sts):
-        good = s["state"] == "done"
-        if good and f in ref:
-            x0, p0 = ref[f]
-            es = max(np.linalg.norm(j.solution[0][r] - x0[r]) / np.linalg.norm(x0[r]) for r in range(nr))
-            ee = max(np.linalg.norm(j.phi[0][r] - p0[r]) / np.linalg.norm(p0[r]) for r in range(nr))
-            err_s.append(es)
-            err_e.append(ee)
-            good = es <= 1e-3 and ee <= 1e-3
-        ok.append(good)
-    valid = int(sum(bool(v) for v in ok)) * nr
-    value = valid / elapsed
-
-    # the dominant kernel again on one problem with nothing else on the GPU
-    iso = {} if a.no_isolated else isolated_kernels(geo, src, iso_f)
-
-    # roofline of the dominant kernel of the served run, timed on one problem with nothing
-    # else on the GPU (launch times under the lanes' sharing are kept as `served`)
-    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
-    roof = None
-    if top:
-        s = kst[top]
-        served = {"achieved": s["bytes"] / s["total_ms"] * 1e3 / 1e9, "frac": s["bytes"] / s["total_ms"] * 1e3 / HBM_PEAK,
-                  "bytes_per_launch": s["bytes"] / s["calls"], "mean_launch_ms": s["total_ms"] / s["calls"],
-                  "launches": s["calls"], "timing": "HIP events around each launch on the problem streams, %d lanes "
-                                                    "sharing the GPU" % a.lanes,
-                  "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in kst}}
-        roof = {"bound": "hbm", "kernel": top, "achieved": None, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": None,
-                "traffic": None, "served": served}
-        if top in iso:
-            d = iso[top]
-            ach = d["bytes"] / d["total_ms"] * 1e3
-            roof.update({"achieved": ach / 1e9, "frac": ach / HBM_PEAK, "bytes_per_launch": d["bytes"] / d["calls"],
-                         "mean_launch_ms": d["total_ms"] / d["calls"], "launches": d["calls"],
-                         "workload": "one problem of the same mesh at %.0f Hz, 2 load cases, nothing else on the "
-                                     "GPU; HIP events around each launch" % iso_f})
-            roof["traffic"], roof["traffic_source"] = profiled_traffic(top)
-    cpu = None if a.no_cpu else cpu_baseline(mesh, grid, src, iso_f, a.cpu_timeout)
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "frequency-solves/s",
-        "n_gpus": 1,
-        "steps": len(timed),
-        "warmup": a.warmup,
-        "ms_per_step": elapsed / len(timed) * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": ("head-like mesh %d triangles (seed 0), 2 ear point sources, 1730-point grid at 1.2 m, "
-                                "frequencies 200-16000 Hz step 200; NumCalc methodBEM 4 (SL/ML-FMM)" % n)
-                   if a.workload == "head" else
-                   ("graded head-like mesh %d triangles (seed 0; 1 mm edges near the ears to 8 mm), 2 ear point "
-                    "sources, 1730-point grid at 1.2 m, frequencies 12000-23700 Hz step 300; NumCalc methodBEM 4" % n),
-                   "form": "served mixed precision, %d lanes, GMRES + block-Jacobi" % a.lanes,
-                   "frequencies": len(timed)},
-        "p50_ms": float(np.percentile(pl, 50)),
-        "p99_ms": float(np.percentile(pl, 99)),
-        "latency": {"poisson_rate_per_s": rate, "requests": npois,
-                    "batch_p50_ms": float(np.percentile(batch_lat, 50)),
-                    "batch_p99_ms": float(np.percentile(batch_lat, 99))},
-        "valid_solves": valid,
-        "solves": len(timed) * nr,
-        "plain_fp64": {"value": nplain * nr / plain_elapsed, "ms_per_step": plain_elapsed / max(nplain, 1) * 1e3,
-                       "p50_ms": float(np.percentile(plain_lat, 50)) if plain_lat else None,
-                       "p99_ms": float(np.percentile(plain_lat, 99)) if plain_lat else None,
-                       "form": "NumCalc's method choice, CGS + ILU, tol 1e-9, one step at a time"},
-        "speedup_vs_plain": value / (nplain * nr / plain_elapsed) if nplain else None,
-        "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
-                                 "eval_grid_max": float(max(err_e)) if err_e else None,
-                                 "steps_checked": len(err_s)},
-        "server": {k: info[k] for k in ("peak_resident_bytes", "steps_done", "solves_done", "lane_ms")},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
+    return head_main(a)
 
 
 if __name__ == "__main__":

@@ -395,8 +395,8 @@ struct ncg_problem {
     int ktime = 0;
     std::vector<EventPair> kpool;
     std::vector<KernelRec> kpend;
-    double k_ms[16] = {0}, k_bytes[16] = {0};
-    long long k_calls[16] = {0};
+    double k_ms[32] = {0}, k_bytes[32] = {0};
+    long long k_calls[32] = {0};
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -455,11 +455,15 @@ double elapsed(const EventPair &ev)
 // matvec with per-call timing (the Krylov loops' dominant kernel)
 // Per-kernel timing of the operator apply (enabled by ncg_set_kernel_timing): HIP events
 // around each launch on the problem's stream, with the algorithmic bytes of the launch.
+// ids 12.. are the grouped launches of the same operations (kernel id + kGroupedKid)
 const char *const kKernelNames[] = {"k_near_spmv<f64>",   "k_near_spmv<f32>",   "k_fmm_apply_T<f64>",
                                     "k_fmm_apply_T<f32>", "k_fmm_apply_D<f64>", "k_fmm_apply_D<f32>",
                                     "k_fmm_apply_S<f64>", "k_fmm_apply_S<f32>", "k_matvec<f64>",
-                                    "k_matvec<f32>",      "k_matvec_mfma<f32>", "k_matvec_mfma64"};
-constexpr int kNumKernels = 12;
+                                    "k_matvec<f32>",      "k_matvec_mfma<f32>", "k_matvec_mfma64",
+                                    "kg_near_spmv<f64>",  "kg_near_spmv<f32>",  "kg_apply_T<f64>",
+                                    "kg_apply_T<f32>",    "kg_apply_D<f64>",    "kg_apply_D<f32>",
+                                    "kg_apply_S<f64>",    "kg_apply_S<f32>"};
+constexpr int kNumKernels = 20, kGroupedKid = 12;
 
 struct KernelScope {
     ncg_problem *p;
@@ -1872,7 +1876,7 @@ void run_apply(const Stage &st, ncg_problem *lead, const std::vector<ncg_problem
 {
     hipStream_t s = lead->stream;
     for (const ApplyPlan::Launch &L : A.seq) {
-        KernelScope ks(lead, L.kid, L.bytes);
+        KernelScope ks(lead, kGroupedKid + L.kid, L.bytes);
         switch (L.kind) {
         case 0:
             ncgk::g_near_spmv(s, L.cls, L.nr, L.seg, st.dev<ncgk::GNear>(L.items), st.dev<int>(L.start), L.nitems,

@@ -282,9 +282,9 @@ class Problem:
         _check(self.lib.ncg_set_kernel_timing(self._h, int(on)))
 
     def kernel_stats(self, reset=False):
-        arr = (_KernelStat * 16)()
+        arr = (_KernelStat * 32)()
         cnt = ctypes.c_int32()
-        _check(self.lib.ncg_kernel_stats(self._h, arr, 16, ctypes.byref(cnt), int(reset)))
+        _check(self.lib.ncg_kernel_stats(self._h, arr, 32, ctypes.byref(cnt), int(reset)))
         return _kernel_dict(arr, cnt.value)
 
     def tuning(self):
@@ -408,9 +408,9 @@ class Server:
         return _status_dict(st)
 
     def kernel_stats(self, reset=False):
-        arr = (_KernelStat * 16)()
+        arr = (_KernelStat * 32)()
         cnt = ctypes.c_int32()
-        _check(self.lib.ncg_server_kernel_stats(self._h, arr, 16, ctypes.byref(cnt), int(reset)))
+        _check(self.lib.ncg_server_kernel_stats(self._h, arr, 32, ctypes.byref(cnt), int(reset)))
         return _kernel_dict(arr, cnt.value)
 
     def stats(self):

@@ -322,6 +322,19 @@ def write_project(folder: str, prj: Project) -> str:
     return path
 
 
+def write_sources(folder: str, prj: Project, sources) -> list:
+    """A Mesh2HRTF-style project folder: one ``NumCalc/source_<k>`` NumCalc project per point
+    source (k = 1, 2, ...), each the project ``prj`` with that source alone (amplitude 1),
+    as mesh2input lays out reciprocal simulations.  Returns the source folders."""
+    out = []
+    for k, s in enumerate(sources, start=1):
+        d = os.path.join(folder, "NumCalc", "source_%d" % k)
+        p = Project(**{**prj.__dict__, "point_sources": [(float(s[0]), float(s[1]), float(s[2]), 1.0, 0.0)]})
+        write_project(d, p)
+        out.append(d)
+    return out
+
+
 # ----------------------------------------------------------------------------- configs
 
 def config1_project(freqs=None) -> Project:

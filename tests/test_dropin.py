@@ -213,3 +213,34 @@ def test_dropin_step_range_and_log_names(tmp_path):
     log = open(tmp_path / "NC1-1.out").read()
     assert "CGS solver: number of iterations = 0" in log
     assert "Warning: Maximum number of iterations is reached!" in log
+
+
+@pytest.mark.gpu
+def test_runner_drives_the_dropin_like_the_reference(tmp_path):
+    """The project runner (the manage_numcalc counterpart) drives the drop-in: one process per
+    (source, step), three at once; every be.out equals the reference NumCalc's, run the same
+    way, to its printed digits."""
+    _need_binary()
+    from mesh2hrtf_amd import runner
+    ref_exe = os.path.join(ROOT, "oracle", "_ref", "NumCalc")
+    if not os.path.exists(ref_exe):
+        pytest.fail("reference NumCalc not built (make -C oracle ref)")
+    prj = P.Project(mesh=P.icosphere(2, 0.1), freqs=[600.0, 1800.0, 3000.0], eval_grid=P.fibonacci_grid(40, 0.6),
+                    title="runner drop-in")
+    srcs = [(0.0, 0.0, 0.25), (0.15, -0.1, 0.05)]
+    ours = P.write_sources(str(tmp_path / "ours"), prj, srcs)
+    refs = P.write_sources(str(tmp_path / "ref"), prj, srcs)
+    print("runner: drop-in", flush=True)
+    r1 = runner.run(str(tmp_path / "ours"), NUMCALC, max_instances=3)
+    r2 = runner.run(str(tmp_path / "ref"), ref_exe, max_instances=3)
+    assert r1.unfinished == [] and r2.unfinished == [] and r1.summary()["failed"] == 0
+    for a, b in zip(ours, refs):
+        for step in (1, 2, 3):
+            for key in ("pBoundary", "vBoundary", "pEvalGrid", "vEvalGrid"):
+                ia, va = P.read_be_file(os.path.join(a, "be.out", "be.%d" % step, key))
+                ib, vb = P.read_be_file(os.path.join(b, "be.out", "be.%d" % step, key))
+                assert np.array_equal(ia, ib)
+                if np.linalg.norm(vb) == 0:
+                    assert np.abs(va).max() < 1e-12
+                else:
+                    assert R.rel_l2(va, vb) < 1e-6, (a, step, key)
