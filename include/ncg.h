@@ -175,17 +175,32 @@ int ncg_solve(ncg_problem *p, const ncg_solver_options *opt, ncg_solve_stats *st
 int ncg_solve_group(ncg_problem *const *problems, int32_t count, const ncg_solver_options *opt,
                     ncg_solve_stats *stats);
 
-/* Launch shapes of the grouped FMM applies, chosen by measurement (ncg_autotune) rather
- * than fixed: waves per work-group (1, 2, 4 or 8) that split each T, D and S sum.  A group
- * solve uses its first problem's tuning.  No counterpart in the reference. */
+/* Launch shapes the operator applies and the dense assembly run with, chosen by measurement
+ * on the problems at hand (ncg_autotune) rather than fixed.  The FMM fields change how the
+ * grouped applies split their sums (hence the last bits of a result); the near-field mode
+ * and the assembly rows do not change any value.  A group solve uses its first problem's
+ * tuning.  No counterpart in the reference. */
 typedef struct {
-    int32_t t_waves, d_waves, s_waves;  /* default 8                                         */
-    int32_t tuned;                      /* 1 once set by ncg_autotune                         */
-    int32_t reserved0;
-    int32_t reserved[3];
+    int32_t t_waves, d_waves, s_waves;  /* waves per work-group of the grouped T, D, S applies:
+                                           1, 2, 4 or 8 (default 8)                          */
+    int32_t tuned;                      /* 1 once chosen by ncg_autotune                      */
+    int32_t near_mode;                  /* near-field work-group: 0 one row chunk (default),
+                                           1 one finest cluster (x gathered once)           */
+    int32_t mfma_splits;                /* dense, more than 4 load cases: column splits of the
+                                           matrix-core product, 1..16 (0: from the size)     */
+    int32_t assemble_rows;              /* dense: collocation rows per work-group of the
+                                           far-field quadrature, 8..256 (0: 32)             */
+    int32_t reserved;
 } ncg_tuning;
 int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t);
 int ncg_get_tuning(ncg_problem *p, ncg_tuning *t);
+/* Times the candidate launch shapes on assembled problems (a group as ncg_solve_group takes
+ * it: equal n_rhs and precision, one device): near-field mode and T, D, S waves of the
+ * grouped FMM applies, matrix-core splits of dense applies with more than 4 load cases, rows
+ * per work-group of the dense far-field quadrature (re-run on the assembled frequency; the
+ * operator's values do not change).  reps applies per candidate.  Sets the fastest choice on
+ * every problem and returns it in *out (or NULL). */
+int ncg_autotune(ncg_problem *const *problems, int32_t count, int32_t reps, ncg_tuning *out);
 
 /* The method and FMM plan of the last ncg_assemble (what NumCalc logs per step). */
 int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info);
@@ -265,6 +280,12 @@ typedef struct {
     int32_t kernel_timing;              /* 1: per-kernel timing of every step (ncg_kernel_stats) */
     int32_t group;                      /* GMRES / mixed steps a lane solves together
                                            (ncg_solve_group); 0 -> 4, 1: one at a time       */
+    int32_t autotune;                   /* 1: the first group of each operator shape (mesh
+                                           size, method, clusters per level, n_rhs, precision)
+                                           is timed with ncg_autotune and its choice used for
+                                           every later group of that shape                  */
+    const char *tuning_file;            /* text file of tuned shapes read at creation and
+                                           written at ncg_server_destroy, or NULL            */
 } ncg_server_config;
 
 typedef struct {
@@ -296,6 +317,7 @@ typedef struct {
     int32_t lanes, queued_steps;
     double lane_ms[4];                  /* lane wall time summed over the lanes: assembly,
                                            solve, outputs (solution and grid), waiting       */
+    int32_t tuned_shapes;               /* operator shapes with a tuning (ncg_server_tuning)  */
 } ncg_server_info;
 
 int ncg_server_create(const ncg_server_config *cfg, ncg_server **out);
@@ -310,6 +332,8 @@ int ncg_server_stats(ncg_server *s, ncg_server_info *stats);
 /* Per-kernel timing summed over the steps finished since the last reset (requires
  * kernel_timing in the configuration). */
 int ncg_server_kernel_stats(ncg_server *s, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset);
+/* The tuning of operator shape `index` (0 .. tuned_shapes - 1) and its key text. */
+int ncg_server_tuning(ncg_server *s, int32_t index, char *key, int32_t key_cap, ncg_tuning *tuning);
 /* Finishes the queued work, stops the lanes, frees every problem. */
 int ncg_server_destroy(ncg_server *s);
 

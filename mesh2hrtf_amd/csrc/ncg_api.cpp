@@ -200,6 +200,7 @@ struct DevFmm {
     long long *nd_cb = nullptr, *nd_off = nullptr;
     int *nd_w = nullptr, *nd_row = nullptr;
     int nd_n = 0;
+    int *nd_cptr = nullptr;            // finest cluster c owns chunks nd_cptr[c] .. nd_cptr[c + 1]
     ncgk::NearChunks near_chunks() const
     {
         ncgk::NearChunks r;
@@ -385,7 +386,7 @@ struct ncg_problem {
     // the members' streams around the group's
     Grow gm;
     Grow gdesc;
-    ncg_tuning tune{8, 8, 8, 0, 0, {0, 0, 0}};   // launch shapes of the grouped applies (leader's apply)
+    ncg_tuning tune{8, 8, 8, 0, 0, 0, 0, 0};   // launch shapes (ncg_autotune); a group uses its leader's
     char *gstage = nullptr;
     size_t gstage_cap = 0;
     void *gpin = nullptr;
@@ -508,7 +509,7 @@ void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y, 
         const size_t el = fp64acc ? sizeof(double) : sizeof(float);
         // algorithmic bytes: A once per pass of (4 vector / 16 matrix-core) load cases, x and y
         const double vec = 2.0 * nr * n * 16.0, abytes = n * n * (mixed ? 8.0 : 16.0);
-        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, fp64acc);
+        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, fp64acc, p->tune.mfma_splits);
         void *xh = nullptr, *part = nullptr;
         bool use_mfma = !d && p->nrhs > 4;
         if (use_mfma && na) {
@@ -585,7 +586,7 @@ size_t mv_scratch_bytes(const ncg_problem *p, int n_rhs)
 {
     if (p->plan.method == 0) {
         if (n_rhs <= 4) return 0;
-        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, 1);
+        const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, 1, p->tune.mfma_splits);
         return ((sizeof(double) * mp.xh_elems + 255) & ~(size_t)255) + sizeof(double) * mp.part_elems;
     }
     size_t b = 0;
@@ -901,10 +902,11 @@ int fmm_structure(ncg_problem *p)
     // and value offsets
     const int R = ncgk::kNearRows;
     std::vector<long long> nd_cb, nd_off;
-    std::vector<int> nd_w, nd_row;
+    std::vector<int> nd_w, nd_row, nd_cptr(1, 0);
     for (int c = 0; c < ncf; ++c) {
         const std::vector<int> &el = FL.cl[c].elems;
         const int m = (int)el.size(), nch = (m + R - 1) / R;
+        nd_cptr.push_back(nd_cptr.back() + nch);
         for (int q = 0, r0 = 0; q < nch; ++q) {
             const int mq = m / nch + (q < m % nch ? 1 : 0);
             const long long cb = ptr[el[0]];
@@ -952,6 +954,7 @@ int fmm_structure(ncg_problem *p)
     if ((rc = upload(F.st, s, &F.near_ptr, ptr)) || (rc = F.st.alloc(&F.diag_pos, n)) ||
         (rc = upload(F.st, s, &F.nd_cb, nd_cb)) || (rc = upload(F.st, s, &F.nd_off, nd_off)) ||
         (rc = upload(F.st, s, &F.nd_w, nd_w)) || (rc = upload(F.st, s, &F.nd_row, nd_row)) ||
+        (rc = upload(F.st, s, &F.nd_cptr, nd_cptr)) ||
         (rc = F.st.alloc(&F.near_col, F.nnz)) ||
         (rc = upload(F.st, s, &F.elem_cl_fine, cl_of)) || (rc = upload(F.st, s, &F.cl_near, cnear)) ||
         (rc = upload(F.st, s, &d_nn_ptr, nn_ptr)) || (rc = upload(F.st, s, &d_nn_idx, nn_idx)) ||
@@ -1750,6 +1753,9 @@ struct ApplyPlan {
     int acc64 = 1, sig64 = 0;
 };
 
+// near-field launch shape of a group: its leader's tuning (0 chunk per work-group, 1 cluster)
+int lead_tune_near(const std::vector<ncg_problem *> &P) { return P[0]->tune.near_mode; }
+
 void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<const double *> &in,
                 const std::vector<double *> &out, const std::vector<int *> &skip, int acc64, int sig64, ApplyPlan &A)
 {
@@ -1780,6 +1786,8 @@ void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vecto
             ncgk::GNear g;
             g.n = p->n;
             g.nch = F.nd_n;
+            g.ncl = F.ncl_fine;
+            g.cptr = F.nd_cptr;
             g.cb = F.nd_cb;
             g.off = F.nd_off;
             g.w = F.nd_w;
@@ -1790,7 +1798,7 @@ void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vecto
             g.y = out[i];
             g.skip = skip[i];
             it.push_back(g);
-            cnt.push_back(F.nd_n);
+            cnt.push_back(lead_tune_near(P) ? F.ncl_fine : F.nd_n);
             seg = std::max(seg, std::max(64, std::min(32768 / (16 * nr), (F.near_wmax + 63) / 64 * 64)));
             bytes += (double)F.nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * p->n * 16.0;
         }
@@ -1879,8 +1887,8 @@ void run_apply(const Stage &st, ncg_problem *lead, const std::vector<ncg_problem
         KernelScope ks(lead, kGroupedKid + L.kid, L.bytes);
         switch (L.kind) {
         case 0:
-            ncgk::g_near_spmv(s, L.cls, L.nr, L.seg, st.dev<ncgk::GNear>(L.items), st.dev<int>(L.start), L.nitems,
-                              L.blocks);
+            ncgk::g_near_spmv(s, L.cls, lead->tune.near_mode, L.nr, L.seg, st.dev<ncgk::GNear>(L.items),
+                              st.dev<int>(L.start), L.nitems, L.blocks);
             break;
         case 1:
             ncgk::g_apply_T(s, L.cls < 2, L.cls == 1, lead->tune.t_waves, L.nr, st.dev<ncgk::GLevel>(L.items),
@@ -2705,7 +2713,8 @@ int assemble_impl(ncg_problem *p, const ncg_frequency *f, int32_t precision, boo
     record(p, p->ev_far.a);
     p->dbj_ready = 0;
     // quadrature evaluations are counted only while kernel timing is on (measurement)
-    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, 32, p->ktime ? p->evals : nullptr);
+    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, p->tune.assemble_rows > 0 ? p->tune.assemble_rows : 32,
+                       p->ktime ? p->evals : nullptr);
     record(p, p->ev_far.b);
     ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err);
     if (p->nsrc) ncgk::rhs_sources(s, p->elements(), b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err);
@@ -2780,6 +2789,10 @@ int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t)
     if (!p || !t) return fail(NCG_E_ARG, "ncg_set_tuning: null argument");
     if (!valid_waves(t->t_waves) || !valid_waves(t->d_waves) || !valid_waves(t->s_waves))
         return fail(NCG_E_ARG, "ncg_set_tuning: waves per work-group must be 1, 2, 4 or 8");
+    if (t->near_mode != 0 && t->near_mode != 1) return fail(NCG_E_ARG, "ncg_set_tuning: near_mode must be 0 or 1");
+    if (t->mfma_splits < 0 || t->mfma_splits > 16) return fail(NCG_E_ARG, "ncg_set_tuning: mfma_splits must be 0..16");
+    if (t->assemble_rows != 0 && (t->assemble_rows < 8 || t->assemble_rows > 256))
+        return fail(NCG_E_ARG, "ncg_set_tuning: assemble_rows must be 0 or 8..256");
     p->tune = *t;
     return 0;
 }
@@ -2788,6 +2801,164 @@ int ncg_get_tuning(ncg_problem *p, ncg_tuning *t)
 {
     if (!p || !t) return fail(NCG_E_ARG, "ncg_get_tuning: null argument");
     *t = p->tune;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// summed kernel time (ms) of the listed timing ids on p since the last reset, then reset
+double take_ms(ncg_problem *p, std::initializer_list<int> ids)
+{
+    harvest_mv(p);
+    double t = 0.0;
+    for (int id : ids) t += p->k_ms[id];
+    for (int id = 0; id < kNumKernels; ++id) {
+        p->k_ms[id] = p->k_bytes[id] = 0.0;
+        p->k_calls[id] = 0;
+    }
+    return t;
+}
+
+int autotune_group(const std::vector<ncg_problem *> &P, int reps, ncg_tuning &best)
+{
+    ncg_problem *lead = P[0];
+    hipStream_t s = lead->stream;
+    const int G = (int)P.size();
+    int rc;
+    best = lead->tune;
+    std::vector<int> keep(G);
+    for (int i = 0; i < G; ++i) {
+        keep[i] = P[i]->ktime;
+        P[i]->ktime = 1;
+        if ((rc = ensure_work(P[i], 8))) return rc;
+        take_ms(P[i], {});
+    }
+    std::vector<const double *> in(G);
+    std::vector<double *> out(G);
+    for (int i = 0; i < G; ++i) {
+        in[i] = P[i]->x;
+        out[i] = P[i]->work[1];
+    }
+    const std::vector<int *> noskip(G, nullptr);
+    const bool fmm = std::any_of(P.begin(), P.end(), [](ncg_problem *p) { return p->plan.method != 0 && p->nrhs <= 4; });
+    if (fmm) {
+        // near-field mode (changes no value)
+        double tbest = 1e300;
+        for (int mode = 0; mode < 2; ++mode) {
+            lead->tune.near_mode = mode;
+            Stage st(lead);
+            ApplyPlan ap;
+            plan_apply(st, P, in, out, noskip, 1, 0, ap);
+            if ((rc = st.upload(s))) return rc;
+            run_apply(st, lead, P, in, out, ap);   // warm
+            take_ms(lead, {});
+            for (int r = 0; r < reps; ++r) run_apply(st, lead, P, in, out, ap);
+            const double t = take_ms(lead, {kGroupedKid + 0, kGroupedKid + 1});
+            if (t < tbest) {
+                tbest = t;
+                best.near_mode = mode;
+            }
+        }
+        lead->tune.near_mode = best.near_mode;
+        // waves per work-group of T, D and S, each kind on its own
+        double tT = 1e300, tD = 1e300, tS = 1e300;
+        Stage st(lead);
+        ApplyPlan ap;
+        plan_apply(st, P, in, out, noskip, 1, 0, ap);
+        if ((rc = st.upload(s))) return rc;
+        for (int w : {8, 4, 2}) {
+            lead->tune.t_waves = lead->tune.d_waves = lead->tune.s_waves = w;
+            run_apply(st, lead, P, in, out, ap);
+            take_ms(lead, {});
+            for (int r = 0; r < reps; ++r) run_apply(st, lead, P, in, out, ap);
+            harvest_mv(lead);
+            const double a = lead->k_ms[kGroupedKid + 2] + lead->k_ms[kGroupedKid + 3];
+            const double b = lead->k_ms[kGroupedKid + 4] + lead->k_ms[kGroupedKid + 5];
+            const double c = lead->k_ms[kGroupedKid + 6] + lead->k_ms[kGroupedKid + 7];
+            take_ms(lead, {});
+            if (a < tT) { tT = a; best.t_waves = w; }
+            if (b < tD) { tD = b; best.d_waves = w; }
+            if (c < tS) { tS = c; best.s_waves = w; }
+        }
+    }
+    // dense operators: matrix-core splits (more than 4 load cases) and far-field rows
+    std::vector<ncg_problem *> dense;
+    for (ncg_problem *p : P)
+        if (p->plan.method == 0) dense.push_back(p);
+    if (!dense.empty()) {
+        if (dense[0]->nrhs > 4) {
+            const int auto_splits = ncgk::mfma_plan(dense[0]->n, 0, 0).splits;
+            double tbest = 1e300;
+            for (int sp : {0, 1, 2, 4, 8, 16}) {
+                if (sp == auto_splits) continue;
+                for (ncg_problem *p : dense) p->tune.mfma_splits = sp;
+                double t = 0.0;
+                for (ncg_problem *p : dense) {
+                    mv(p, 0, p->x, nullptr, p->work[1]);
+                    take_ms(p, {});
+                    for (int r = 0; r < reps; ++r) mv(p, 0, p->x, nullptr, p->work[1]);
+                    t += take_ms(p, {10, 11});
+                }
+                if (t < tbest) {
+                    tbest = t;
+                    best.mfma_splits = sp;
+                }
+            }
+        }
+        double tbest = 1e300;
+        ncg_problem *p = dense[0];
+        EventPair ev = p->ev_far;
+        for (int rows : {16, 32, 64, 128}) {
+            record(p, ev.a);
+            for (int r = 0; r < std::max(1, reps / 2); ++r)
+                ncgk::assemble_far(s, p->elements(), p->bm(), p->adm, p->A, p->precision == NCG_MIXED, rows, nullptr);
+            record(p, ev.b);
+            NCG_HIP(hipStreamSynchronize(s));
+            const double t = elapsed(ev);
+            if (t < tbest) {
+                tbest = t;
+                best.assemble_rows = rows;
+            }
+        }
+    }
+    NCG_HIP(hipGetLastError());
+    NCG_HIP(hipStreamSynchronize(s));
+    best.tuned = 1;
+    for (int i = 0; i < G; ++i) {
+        P[i]->tune = best;
+        P[i]->ktime = keep[i];
+        take_ms(P[i], {});
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ncg_autotune(ncg_problem *const *probs, int32_t count, int32_t reps, ncg_tuning *out)
+{
+    if (!probs || count < 1 || count > kGroupMax) return fail(NCG_E_ARG, "ncg_autotune: 1..64 problems");
+    std::vector<ncg_problem *> P(probs, probs + count);
+    for (ncg_problem *p : P) {
+        if (!p) return fail(NCG_E_ARG, "ncg_autotune: null problem");
+        if (!p->assembled) return fail(NCG_E_STATE, "ncg_autotune: assemble every problem first");
+        if (p->device != P[0]->device || p->nrhs != P[0]->nrhs || p->precision != P[0]->precision)
+            return fail(NCG_E_ARG, "ncg_autotune: the problems must share device, n_rhs and precision");
+    }
+    NCG_HIP(hipSetDevice(P[0]->device));
+    int rc;
+    for (ncg_problem *p : P)
+        if ((rc = assemble_finish(p))) return rc;
+    GroupStreams gs(P);
+    if ((rc = gs.join())) return rc;
+    ncg_tuning best;
+    rc = autotune_group(P, std::max(1, reps), best);
+    gs.leave();
+    if (rc) return rc;
+    if (out) *out = best;
     return 0;
 }
 

@@ -2336,6 +2336,84 @@ __global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, co
                           (const cx<double> *)it.x, (cx<double> *)it.y);
 }
 
+// Near field with one work-group per finest cluster: when the cluster's column list fits one
+// LDS segment, x over it is gathered once and the cluster's row chunks are streamed one after
+// the other, the next chunk's first values requested before the current chunk's rows are
+// reduced and stored; each row sums as in near_spmv_item (same lanes, same order).  Wider
+// clusters run near_spmv_item chunk by chunk.
+template <typename S, int NR>
+__device__ __forceinline__ void near_cluster_item(int c, int n, int seg, const int *cptr, const long long *ch_cb,
+                                                  const int *ch_w, const int *ch_row, const long long *ch_off,
+                                                  const int *col, const cx<S> *val, const cx<double> *x,
+                                                  cx<double> *y)
+{
+    extern __shared__ double xs_sh[];   // [NR][seg] complex
+    cx<double> *xs = (cx<double> *)xs_sh;
+    const int q0 = cptr[c], q1 = cptr[c + 1];
+    const int w = ch_w[q0];
+    if (w > seg) {
+        for (int q = q0; q < q1; ++q) near_spmv_item<S, NR>(q, n, seg, ch_cb, ch_w, ch_row, ch_off, col, val, x, y);
+        return;
+    }
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long cb = ch_cb[q0];
+    constexpr int RW = ncgk::kNearRowsPerWave;
+    const cx<S> *vr[RW];
+    int row[RW];
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int a = ncgk::kNearRows * q0 + wv + 4 * j;
+        row[j] = ch_row[a];
+        vr[j] = val + ch_off[a];
+    }
+    cx<S> v[2][RW];
+    rows_load<S, RW, 2>(vr, 0, w, lane, v);
+    for (int u = threadIdx.x; u < w; u += 256) {
+        const int e = col[cb + u];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) xs[r * seg + u] = x[(size_t)r * n + e];
+    }
+    __syncthreads();
+    for (int q = q0; q < q1; ++q) {
+        double re[RW][NR], im[RW][NR];
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
+        rows_dot<S, NR, RW, 2>(vr, w, xs, seg, lane, v, re, im);
+        int rq[RW];
+#pragma unroll
+        for (int j = 0; j < RW; ++j) rq[j] = row[j];
+        if (q + 1 < q1) {
+#pragma unroll
+            for (int j = 0; j < RW; ++j) {
+                const int a = ncgk::kNearRows * (q + 1) + wv + 4 * j;
+                row[j] = ch_row[a];
+                vr[j] = val + ch_off[a];
+            }
+            rows_load<S, RW, 2>(vr, 0, w, lane, v);
+        }
+        wave_sum_rows<RW, NR>(re, im);
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+                if (rq[j] >= 0)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) y[(size_t)r * n + rq[j]] = mk(re[j][r], im[j][r]);
+    }
+}
+
+template <typename S, int NR>
+__global__ void __launch_bounds__(256) kg_near_cluster(const ncgk::GNear *items, const int *start, int nitems,
+                                                       int seg)
+{
+    const int d = g_item(start, nitems, blockIdx.x);
+    const ncgk::GNear it = items[d];
+    if (it.skip && *it.skip) return;
+    near_cluster_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cptr, it.cb, it.w, it.row, it.off, it.col,
+                             (const cx<S> *)it.val, (const cx<double> *)it.x, (cx<double> *)it.y);
+}
+
 template <typename S, typename W, int NR, int NW>
 __global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
 {
@@ -2915,7 +2993,7 @@ __global__ void __launch_bounds__(256) k_mfma64_reduce(int n, int nr, int splits
     y[(size_t)r * n + i] = mk(re, im);
 }
 
-MfmaPlan mfma_plan(int n, int f64)
+MfmaPlan mfma_plan(int n, int f64, int splits)
 {
     MfmaPlan m;
     m.f64 = f64;
@@ -2924,6 +3002,7 @@ MfmaPlan mfma_plan(int n, int f64)
     // enough work-groups to cover the device (1024 for the FP64 tiles of 64 rows, 512 for
     // the FP32 ones of 128), each at least 4 column chunks
     m.splits = std::max(1, std::min(std::min(16, ((f64 ? 1024 : 512) + m.tiles - 1) / m.tiles), (nchunks + 3) / 4));
+    if (splits > 0) m.splits = std::min(16, splits);
     m.per = (nchunks + m.splits - 1) / m.splits;
     m.xh_elems = (size_t)2 * n * 32;
     m.part_elems = (size_t)m.splits * 32 * n;
@@ -3422,11 +3501,14 @@ void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double 
     default: CALL(4); break; \
     }
 
-void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, const int *start, int nitems, int blocks)
+void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNear *items, const int *start,
+                 int nitems, int blocks)
 {
     if (blocks <= 0) return;
 #define NCG_CALL(NR)                                                                                               \
-    if (mixed) kg_near_spmv<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);              \
+    if (mode && mixed) kg_near_cluster<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);  \
+    else if (mode) kg_near_cluster<double, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);      \
+    else if (mixed) kg_near_spmv<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);         \
     else kg_near_spmv<double, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);
     NCG_G_NR(nr, NCG_CALL)
 #undef NCG_CALL

@@ -52,7 +52,8 @@ struct MfmaPlan {
     int f64 = 0, tiles = 0, splits = 0, per = 0;
     size_t xh_elems = 0, part_elems = 0;     // scratch, in elements of the accumulation type
 };
-MfmaPlan mfma_plan(int n, int f64);
+// splits: column splits of the product (0: from the size)
+MfmaPlan mfma_plan(int n, int f64, int splits = 0);
 void mfma_xhat(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *x, void *xh);
 void mfma_product(hipStream_t s, const MfmaPlan &m, const void *A, int a_f32, int n, const void *xh, void *part);
 void mfma_sum(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *part, void *y);
@@ -209,7 +210,8 @@ void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double 
 // that problem's Krylov solve has converged.  Each item's arithmetic is the single-problem
 // kernel's, work-group for work-group, so results do not depend on the grouping.
 struct GNear {      // near field of the finest level (NearChunks, col, values)
-    int n, nch;
+    int n, nch, ncl;                    // unknowns, row chunks, finest clusters
+    const int *cptr;                    // cluster c owns chunks cptr[c] .. cptr[c + 1]
     const long long *cb, *off;
     const int *w, *row, *col;
     const void *val;
@@ -265,8 +267,9 @@ struct GVec {
     const int *skip;
 };
 // launchers (blocks = start[nitems], the prefix the host uploaded)
-void g_near_spmv(hipStream_t s, int mixed, int nr, int seg, const GNear *items, const int *start, int nitems,
-                 int blocks);
+// mode 0: a work-group per row chunk; 1: per finest cluster (x gathered once for its chunks)
+void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNear *items, const int *start,
+                 int nitems, int blocks);
 // T and D of every item (each writes its own t or s); S adds into y, so a launch may hold at
 // most one level of a problem (the host launches one level index at a time).  sig64: t and
 // s in FP64 whatever the values' precision (the refinement residual).  nw: waves per

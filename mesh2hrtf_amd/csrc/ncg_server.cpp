@@ -15,9 +15,11 @@
 // manage_numcalc (mesh2hrtf/NumCalc/manage_numcalc.py:376-430) and admitted by their
 // Memory.txt RAM estimates (:236-290).
 #include <hip/hip_runtime.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -109,6 +111,11 @@ struct ncg_server {
     clk::time_point t0;
     std::map<std::string, ncg_kernel_stat> kstats;   // per-kernel timing of finished steps
     double host_ms[4] = {0, 0, 0, 0};                 // lane wall time: assemble, solve, outputs, waiting
+    // launch-shape tunings per operator shape (ncg_autotune), and the shapes being timed
+    std::map<std::string, ncg_tuning> tunings;
+    std::map<std::string, int> tuning_busy;
+    std::condition_variable cv_tune;
+    std::string tuning_file;
 };
 
 namespace {
@@ -145,6 +152,78 @@ void fail_job(ncg_server *S, Job *J, int code, const std::string &msg)
                        S->queue.end());
     }
     if (J->inflight == 0) end_job(S, J);
+}
+
+// Operator shape of an assembled problem, the key of its launch-shape tuning: unknowns,
+// load cases, precision, method and clusters per level (they fix the launches' sizes).
+std::string shape_key(ncg_problem *p, int n, int nrhs, int precision)
+{
+    ncg_fmm_info fi;
+    if (ncg_problem_fmm_info(p, &fi) != 0) return std::string();
+    std::string k = "n " + std::to_string(n) + " rhs " + std::to_string(nrhs) + " precision " +
+                    std::to_string(precision) + " method " + std::to_string(fi.method) + " clusters";
+    for (int l = 0; l < fi.levels && l < 10; ++l) k += " " + std::to_string(fi.clusters[l]);
+    return k;
+}
+
+void load_tunings(ncg_server *S)
+{
+    FILE *f = fopen(S->tuning_file.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char *bar = strchr(line, '|');
+        if (!bar) continue;
+        *bar = 0;
+        ncg_tuning t{};
+        if (sscanf(bar + 1, "%d %d %d %d %d %d", &t.t_waves, &t.d_waves, &t.s_waves, &t.near_mode, &t.mfma_splits,
+                   &t.assemble_rows) == 6) {
+            t.tuned = 1;
+            S->tunings[line] = t;
+        }
+    }
+    fclose(f);
+}
+
+void save_tunings(ncg_server *S)
+{
+    FILE *f = fopen(S->tuning_file.c_str(), "w");
+    if (!f) return;
+    for (auto &kv : S->tunings)
+        fprintf(f, "%s|%d %d %d %d %d %d\n", kv.first.c_str(), kv.second.t_waves, kv.second.d_waves,
+                kv.second.s_waves, kv.second.near_mode, kv.second.mfma_splits, kv.second.assemble_rows);
+    fclose(f);
+}
+
+// The group's tuning (lock not held): the stored one of its shape, or (autotune) timed on
+// this group while other lanes with the same shape wait for it.
+void tune_group(ncg_server *S, const std::vector<ncg_problem *> &P, int n, int nrhs, int precision)
+{
+    const std::string key = shape_key(P[0], n, nrhs, precision);
+    if (key.empty()) return;
+    ncg_tuning t{};
+    bool have = false, mine = false;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_tune.wait(lk, [&] { return !S->tuning_busy.count(key); });
+        auto it = S->tunings.find(key);
+        if (it != S->tunings.end()) {
+            t = it->second;
+            have = true;
+        } else if (S->cfg.autotune) {
+            S->tuning_busy[key] = 1;
+            mine = true;
+        }
+    }
+    if (mine) {
+        have = ncg_autotune(P.data(), (int32_t)P.size(), 3, &t) == 0;
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (have) S->tunings[key] = t;
+        S->tuning_busy.erase(key);
+        S->cv_tune.notify_all();
+    }
+    if (have)
+        for (ncg_problem *p : P) ncg_set_tuning(p, &t);
 }
 
 // Steps a lane may solve together (ncg_solve_group): GMRES or mixed-precision steps of equal
@@ -312,11 +391,16 @@ void run_lane(ncg_server *S)
         }
         for (Taken &t : grp)
             if (!t.rc && (t.rc = ncgi::assemble_wait(t.r->p))) t.msg = ncg_last_error();
-        const clk::time_point t_asm = clk::now();
         // solve: the group's assembled steps together, or the one step alone
         std::vector<Taken *> ok;
         for (Taken &t : grp)
             if (!t.rc) ok.push_back(&t);
+        if (!ok.empty() && groupable(J0)) {
+            std::vector<ncg_problem *> P;
+            for (Taken *t : ok) P.push_back(t->r->p);
+            tune_group(S, P, ok[0]->st.job->n, J0->nrhs, J0->precision);
+        }
+        const clk::time_point t_asm = clk::now();
         if (ok.size() > 1) {
             std::vector<ncg_problem *> P;
             for (Taken *t : ok) P.push_back(t->r->p);
@@ -462,6 +546,9 @@ int ncg_server_create(const ncg_server_config *cfg, ncg_server **out)
     }
     ncg_server *S = new ncg_server();
     S->cfg = c;
+    S->cfg.tuning_file = nullptr;
+    if (c.tuning_file) S->tuning_file = c.tuning_file;
+    if (!S->tuning_file.empty()) load_tunings(S);
     S->budget = budget;
     S->t0 = clk::now();
     for (int i = 0; i < c.lanes; ++i) S->lanes.emplace_back(run_lane, S);
@@ -612,6 +699,7 @@ int ncg_server_stats(ncg_server *S, ncg_server_info *st)
     st->lanes = S->cfg.lanes;
     st->queued_steps = (int32_t)S->queue.size();
     for (int i = 0; i < 4; ++i) st->lane_ms[i] = S->host_ms[i];
+    st->tuned_shapes = (int32_t)S->tunings.size();
     return 0;
 }
 
@@ -640,7 +728,20 @@ int ncg_server_destroy(ncg_server *S)
     for (std::thread &t : S->lanes) t.join();
     for (auto &e : S->pool)
         if (e->p) ncg_problem_destroy(e->p);
+    if (!S->tuning_file.empty()) save_tunings(S);
     delete S;
+    return 0;
+}
+
+int ncg_server_tuning(ncg_server *S, int32_t index, char *key, int32_t key_cap, ncg_tuning *tuning)
+{
+    if (!S || !tuning) return sfail(NCG_E_ARG, "ncg_server_tuning: null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (index < 0 || index >= (int32_t)S->tunings.size()) return sfail(NCG_E_ARG, "ncg_server_tuning: no such shape");
+    auto it = S->tunings.begin();
+    std::advance(it, index);
+    if (key && key_cap > 0) snprintf(key, (size_t)key_cap, "%s", it->first.c_str());
+    *tuning = it->second;
     return 0;
 }
 

@@ -90,12 +90,17 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
            "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
            "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group",
-           "ncg_set_tuning", "ncg_get_tuning", "ncg_apply_noalloc")
+           "ncg_set_tuning", "ncg_get_tuning", "ncg_apply_noalloc", "ncg_autotune", "ncg_server_tuning")
 
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("t_waves", ctypes.c_int32), ("d_waves", ctypes.c_int32), ("s_waves", ctypes.c_int32),
-                ("tuned", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("tuned", ctypes.c_int32), ("near_mode", ctypes.c_int32), ("mfma_splits", ctypes.c_int32),
+                ("assemble_rows", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def _tuning_dict(t):
+    return {k: getattr(t, k) for k, _ in _Tuning._fields_ if k != "reserved"}
 
 
 class _KernelStat(ctypes.Structure):
@@ -110,7 +115,8 @@ def _kernel_dict(arr, count):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("memory_budget_bytes", ctypes.c_double),
-                ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32), ("group", ctypes.c_int32)]
+                ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32), ("group", ctypes.c_int32),
+                ("autotune", ctypes.c_int32), ("tuning_file", ctypes.c_char_p)]
 
 
 class _Job(ctypes.Structure):
@@ -130,7 +136,7 @@ class _ServerInfo(ctypes.Structure):
                 ("solves_done", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("resident_bytes", ctypes.c_int64),
                 ("peak_resident_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("queued_steps", ctypes.c_int32),
-                ("lane_ms", ctypes.c_double * 4)]
+                ("lane_ms", ctypes.c_double * 4), ("tuned_shapes", ctypes.c_int32)]
 
 _lib = None
 
@@ -290,10 +296,10 @@ class Problem:
     def tuning(self):
         t = _Tuning()
         _check(self.lib.ncg_get_tuning(self._h, ctypes.byref(t)))
-        return dict(t_waves=t.t_waves, d_waves=t.d_waves, s_waves=t.s_waves, tuned=t.tuned)
+        return _tuning_dict(t)
 
-    def set_tuning(self, t_waves=8, d_waves=8, s_waves=8):
-        t = _Tuning(int(t_waves), int(d_waves), int(s_waves), 0, 0)
+    def set_tuning(self, t_waves=8, d_waves=8, s_waves=8, near_mode=0, mfma_splits=0, assemble_rows=0):
+        t = _Tuning(int(t_waves), int(d_waves), int(s_waves), 0, int(near_mode), int(mfma_splits), int(assemble_rows))
         _check(self.lib.ncg_set_tuning(self._h, ctypes.byref(t)))
 
     def timings(self, reset=False):
@@ -320,6 +326,16 @@ def solve_group(problems, solver=GMRES, precond=BLOCK, max_iterations=500, toler
                  ms_solve=st[i].ms_solve) for i, p in enumerate(problems)]
 
 
+def autotune(problems, reps=3):
+    """ncg_autotune: time the candidate launch shapes on these assembled problems, set the
+    fastest on each, and return it."""
+    lib = load()
+    arr = (ctypes.c_void_p * len(problems))(*[p._h.value for p in problems])
+    t = _Tuning()
+    _check(lib.ncg_autotune(arr, len(problems), int(reps), ctypes.byref(t)))
+    return _tuning_dict(t)
+
+
 def _status_dict(st):
     return dict(state=("queued", "running", "done", "failed")[st.state], error_code=st.error_code,
                 freq_done=st.freq_done, iterations=st.iterations, latency_ms=st.latency_ms, queue_ms=st.queue_ms,
@@ -341,11 +357,12 @@ class Server:
     """Serving runtime (ncg_server_*): jobs of (mesh, frequencies, load cases) on one GPU
     with continuous batching over `lanes` concurrent steps and resident problems."""
 
-    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False, group=0):
+    def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False, group=0,
+                 autotune=False, tuning_file=None):
         self.lib = load()
         self._h = ctypes.c_void_p()
         cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing),
-                            int(group))
+                            int(group), int(bool(autotune)), tuning_file.encode() if tuning_file else None)
         _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._meshes = {}
         # arrays the server reads or writes for each job that has not ended: they stay
@@ -419,6 +436,16 @@ class Server:
         d = {k: getattr(s, k) for k, _ in _ServerInfo._fields_}
         d["lane_ms"] = dict(zip(("assemble", "solve", "outputs", "waiting"), list(s.lane_ms)))
         return d
+
+    def tunings(self):
+        """[(operator shape, tuning dict)] the server has chosen (ncg_server_tuning)"""
+        out = []
+        for i in range(self.stats()["tuned_shapes"]):
+            key = ctypes.create_string_buffer(256)
+            t = _Tuning()
+            _check(self.lib.ncg_server_tuning(self._h, i, key, 256, ctypes.byref(t)))
+            out.append((key.value.decode(), _tuning_dict(t)))
+        return out
 
     def close(self):
         if self._h:
