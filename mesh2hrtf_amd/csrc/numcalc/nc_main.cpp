@@ -89,10 +89,12 @@ void plan_info(std::ostream &os, const ncg_fmm_info &fi, bool first, int n)
            << fi.eval_radius_max << "         " << std::setw(6) << fi.eval_radius_min << std::endl;
 }
 
-// FMM expansion table of every FMM step (src/NC_Main.cpp:491-522)
+// FMM expansion table of every FMM step (src/NC_Main.cpp:491-522); the multilevel method
+// starts it with one more empty line (src/NC_Main.cpp:448-449)
 void expansion_info(std::ostream &os, const ncg_fmm_info &fi)
 {
     if (fi.method == 0) return;
+    if (fi.method == 3) os << std::endl;
     os << "\nInformations about the FMM expansion:" << std::endl;
     os << "   Level    N. expa. terms    N. p. theta    N. p. phi    N. p. sphere" << std::endl;
     for (int l = 0; l < fi.levels; ++l)
@@ -480,8 +482,10 @@ int main(int argc, char **argv)
             *os << "Solving the equation system            : " << e2 << std::endl;
             *os << "Post processing                        : " << e3 << std::endl;
             *os << "Total                                  : " << e1 + e2 + e3 << std::endl;
-            *os << "GPU time (ms): assembly " << t[0] << ", solve " << t[1] << ", evaluation grid " << t[2] << std::endl;
         }
+        // device times go to the console only: NC.out keeps NumCalc's text line for line
+        std::cout << "GPU time (ms): assembly " << t[0] << ", solve " << t[1] << ", evaluation grid " << t[2]
+                  << std::endl;
         tlast = tp;
     }
     if (!estimate_ram) {

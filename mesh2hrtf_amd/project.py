@@ -223,6 +223,9 @@ class Boundary:
     kind: str          # VELO | PRES | ADMI | IMPE
     re: float
     im: float
+    # frequency curves scaling the real and imaginary part (NC.inp CURVES ids; -1 none)
+    re_curve: int = -1
+    im_curve: int = -1
 
 
 @dataclass
@@ -244,6 +247,13 @@ class Project:
     symmetry: tuple | None = None
     # NC.inp point-source count -1 (NumCalc's "test point source"): no point source is read
     test_point_source: bool = False
+    # IFINTE (src/NC_Input.cpp:296): 1 solves the interior problem
+    interior: bool = False
+    # CURVES block (src/NC_Input.cpp:1090-1145): (id, [(frequency, factor), ...]) per curve;
+    # the curves of each source's real and imaginary amplitude, (-1, -1) for none
+    curves: list = field(default_factory=list)
+    point_source_curves: list = field(default_factory=list)
+    plane_wave_curves: list = field(default_factory=list)
 
 
 def _fmt_freq_curve(freqs):
@@ -288,8 +298,8 @@ def write_project(folder: str, prj: Project) -> str:
     L += ["##", "## 1. Main Parameters I",
           "%d %d %d 0 %d 2 1 %d %d" % (groups, n_elems, n_nodes, nsym, prj.method_bem, prj.method_solver),
           "##", "## 2. Main Parameters II",
-          "%d %d 0 0.0000e+00 %d 0 0" % (len(prj.plane_waves), -1 if prj.test_point_source else len(prj.point_sources),
-                                         prj.precond),
+          "%d %d %d 0.0000e+00 %d 0 0" % (len(prj.plane_waves), -1 if prj.test_point_source else len(prj.point_sources),
+                                          1 if prj.interior else 0, prj.precond),
           "##", "## 3. Main Parameters III", "0 0 0 0", "##", "## 4. Main Parameters IV",
           "%s %s 1.0 0.0e+00 0.0 e+00 0.0e+00 0.0e+00" % (repr(prj.c), repr(prj.rho)), "##",
           "NODES", "ObjectMesh/Nodes.txt"]
@@ -303,17 +313,25 @@ def write_project(folder: str, prj: Project) -> str:
         L += ["SYMMETRY", "%d %d %d" % tuple(prj.symmetry[0]), "%.17g %.17g %.17g" % tuple(prj.symmetry[1]), "##"]
     L.append("BOUNDARY")
     for b in bnd:
-        L.append("ELEM %d TO %d %s %.17g -1 %.17g -1" % (b.lo, b.hi, b.kind, b.re, b.im))
+        L.append("ELEM %d TO %d %s %.17g %d %.17g %d" % (b.lo, b.hi, b.kind, b.re, b.re_curve, b.im, b.im_curve))
     L += ["RETU", "##"]
     if prj.plane_waves:
         L.append("PLANE WAVES")
         for i, (dx, dy, dz, re, im) in enumerate(prj.plane_waves):
-            L.append("%d %.17g %.17g %.17g %.17g -1 %.17g -1" % (i, dx, dy, dz, re, im))
+            cr, ci = prj.plane_wave_curves[i] if prj.plane_wave_curves else (-1, -1)
+            L.append("%d %.17g %.17g %.17g %.17g %d %.17g %d" % (i, dx, dy, dz, re, cr, im, ci))
         L.append("##")
     if prj.point_sources and not prj.test_point_source:
         L.append("POINT SOURCES")
         for i, (x, y, z, re, im) in enumerate(prj.point_sources):
-            L.append("%d %.17g %.17g %.17g %.17g -1 %.17g -1" % (i, x, y, z, re, im))
+            cr, ci = prj.point_source_curves[i] if prj.point_source_curves else (-1, -1)
+            L.append("%d %.17g %.17g %.17g %.17g %d %.17g %d" % (i, x, y, z, re, cr, im, ci))
+        L.append("##")
+    if prj.curves:
+        L += ["CURVES", "%d %d" % (len(prj.curves), max(len(pts) for _, pts in prj.curves))]
+        for cid, pts in prj.curves:
+            L.append("%d %d" % (cid, len(pts)))
+            L += ["%.17g %.17g" % (f, v) for f, v in pts]
         L.append("##")
     L += ["POST PROCESS", "##", "END"]
     path = os.path.join(folder, "NC.inp")

@@ -193,6 +193,46 @@ def cases():
     C["test_source_2pw"] = dict(prj=P.Project(
         mesh=ico2, freqs=[900.0], eval_grid=P.fibonacci_grid(40, 0.6), test_point_source=True,
         plane_waves=[(0.0, 0.0, 1.0, 1.0, 0.0), (0.6, 0.8, 0.0, 0.5, 0.5)], title="test point source, two plane waves"))
+    # frequency CURVES (src/NC_Input.cpp:1090-1145, src/NC_EquationSystem.cpp:1815-2009):
+    # steps below, inside and above the curves' points; admittance and source amplitudes
+    # scaled per part, a velocity line carrying curve ids (which NumCalc ignores)
+    curves = [(1, [(600.0, 0.5), (1500.0, 1.2), (2500.0, 0.8)]),
+              (2, [(300.0, 2.0), (2200.0, 0.25)]),
+              (3, [(800.0, 1.0), (1200.0, 3.0), (1800.0, -1.0), (2600.0, 0.5)]),
+              (4, [(1000.0, 0.7)])]
+    C["curves"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[400.0, 1000.0, 2000.0, 3000.0], eval_grid=P.fibonacci_grid(60, 0.6),
+        boundary=[P.Boundary(0, ne - 1, "ADMI", 0.5, 0.2, 1, 2), P.Boundary(30, 33, "VELO", 0.1, 0.0, 3, 4)],
+        point_sources=[(0.05, 0.0, 0.25, 1.0, 0.5)], point_source_curves=[(3, 2)],
+        plane_waves=[(0.0, 0.6, 0.8, 0.5, 0.2)], plane_wave_curves=[(4, 1)], curves=curves,
+        title="frequency curves on admittance and sources"))
+    C["curves_fmm"] = dict(prj=P.Project(
+        mesh=ico3, freqs=[700.0, 1400.0, 2400.0], eval_grid=P.fibonacci_grid(60, 0.8), method_bem=1,
+        boundary=[P.Boundary(0, n3 - 1, "ADMI", 0.3, 0.1, 2, 1)],
+        point_sources=[(0.1, 0.1, 0.15, 0.5, 0.5)], point_source_curves=[(1, 3)], curves=curves,
+        title="SL-FMM frequency curves"))
+    # interior problems (IFINTE 1: tau = -1, no Burton-Miller term; src/NC_Input.cpp:346,
+    # src/NC_EquationSystem.cpp:86): a source and the grid inside an absorbing sphere
+    C["interior"] = dict(prj=P.Project(
+        mesh=ico2, freqs=[500.0, 1700.0], eval_grid=P.fibonacci_grid(40, 0.05), interior=True,
+        boundary=[P.Boundary(0, ne - 1, "ADMI", 0.5, 0.2)], point_sources=[(0.02, 0.01, -0.03, 1.0, 0.0)],
+        title="interior problem, absorbing sphere"))
+    C["interior_fmm"] = dict(prj=P.Project(
+        mesh=ico3, freqs=[1200.0], eval_grid=P.fibonacci_grid(40, 0.06), interior=True, method_bem=1,
+        boundary=[P.Boundary(0, n3 - 1, "ADMI", 0.4, 0.1), P.Boundary(10, 14, "VELO", 0.1, 0.0)],
+        point_sources=[(0.0, 0.03, 0.02, 1.0, 0.0)], title="SL-FMM interior problem"))
+    # ILU scanning degrees (IPRECOND -1..-3: src/NC_Input.cpp:306-320, thresholds
+    # src/NC_CommonFunctions.cpp:1366-1404) on the dense, SL- and ML-FMM paths
+    for d in (1, 2, 3):
+        C["scan%d_tbem" % d] = dict(prj=P.Project(
+            mesh=ico3, freqs=[1000.0], eval_grid=P.fibonacci_grid(60, 1.2), precond=-d,
+            point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], title="dense ILU scanning degree %d" % d))
+        C["scan%d_sl" % d] = dict(prj=P.Project(
+            mesh=ico3, freqs=[2000.0], eval_grid=P.fibonacci_grid(60, 0.8), method_bem=1, precond=-d,
+            point_sources=[(0.0, 0.0, 0.2, 1.0, 0.0)], title="SL-FMM ILU scanning degree %d" % d))
+        C["scan%d_ml" % d] = dict(prj=P.Project(
+            mesh=head12, freqs=[6000.0], eval_grid=P.fibonacci_grid(60, 1.2), method_bem=4, precond=-d,
+            point_sources=[(src12[0], src12[1], src12[2], 1.0, 0.0)], title="ML-FMM ILU scanning degree %d" % d))
     # one mesh of the batched uneven dense problems (project.batched_problems: index 3, a
     # perturbed sphere of 1620 triangles) at its first two frequencies, first plane wave
     bm, dirs = P.batched_problems(8)
@@ -206,13 +246,16 @@ def cases():
 
 def spec_of(prj):
     return dict(freqs=list(prj.freqs),
-                boundary=[(b.lo, b.hi, b.kind, b.re, b.im) for b in prj.boundary],
+                boundary=[(b.lo, b.hi, b.kind, b.re, b.im, b.re_curve, b.im_curve) for b in prj.boundary],
                 point_sources=[list(map(float, s)) for s in prj.point_sources],
                 plane_waves=[list(map(float, s)) for s in prj.plane_waves],
                 method_bem=prj.method_bem, method_solver=prj.method_solver,
                 precond=prj.precond, c=prj.c, rho=prj.rho,
                 symmetry=None if prj.symmetry is None else [list(prj.symmetry[0]), list(prj.symmetry[1])],
-                test_point_source=prj.test_point_source)
+                test_point_source=prj.test_point_source, interior=prj.interior,
+                curves=[[cid, [list(p) for p in pts]] for cid, pts in prj.curves],
+                point_source_curves=[list(c) for c in prj.point_source_curves],
+                plane_wave_curves=[list(c) for c in prj.plane_wave_curves])
 
 
 def run_case(name, prj):

@@ -5,7 +5,9 @@ element 2410 (source_1) or 2411 (source_2) and 4 evaluation points at 1 m, 100..
 NumCalc solves steps 1-11 with the single-level FMM and 12-60 with the 2-level direct
 ML-FMM.  This script copies the data the reference's tests hold (mesh, be.out values of
 selected steps, the cluster/expansion/nonzero figures of NC*.out) into
-tests/golden/shtf.npz.  Run from the repository root, in a container that has the
+tests/golden/shtf.npz, and the text of source_1's input and outputs (NC.inp, the evaluation
+grid files, NC1-20.out, the first and last lines of be.out/be.1 and be.12) into
+tests/golden/shtf_text.npz for the layout tests.  Run from the repository root, in a container that has the
 reference checkout.
 """
 import json
@@ -101,5 +103,28 @@ def main():
     print("wrote tests/golden/shtf.npz", {k: v.shape for k, v in data.items()})
 
 
+TEXT_STEPS = [1, 12]
+HEAD_LINES = 13
+
+
+def text_fixture():
+    d = os.path.join(SRC, "NumCalc", "source_1")
+    grid = os.path.join(SRC, "EvaluationGrids", "FourPointHorPlane_r100cm")
+    out = dict(nc_inp=open(os.path.join(d, "NC.inp")).read(),
+               grid_nodes=open(os.path.join(grid, "Nodes.txt")).read(),
+               grid_elements=open(os.path.join(grid, "Elements.txt")).read(),
+               nc_out=open(os.path.join(d, "NC1-20.out")).read())
+    for s in TEXT_STEPS:
+        for key in ("pBoundary", "vBoundary", "pEvalGrid", "vEvalGrid"):
+            lines = open(os.path.join(d, "be.out", "be.%d" % s, key)).read().split("\n")
+            out["be%d_%s_head" % (s, key)] = "\n".join(lines[:HEAD_LINES])
+            out["be%d_%s_tail" % (s, key)] = "\n".join(lines[-3:])
+            out["be%d_%s_lines" % (s, key)] = len(lines)
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "shtf_text.npz"),
+                        **{k: np.array(v) for k, v in out.items()})
+    print("wrote tests/golden/shtf_text.npz")
+
+
 if __name__ == "__main__":
     main()
+    text_fixture()

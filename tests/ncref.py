@@ -28,7 +28,8 @@ def case_names():
 
 
 def element_bc(case):
-    """Per-element (kind, nodal value (n,4), admittance) as given in NC.inp, before scaling."""
+    """Per-element (kind, nodal value (n,4), admittance, admittance curve ids (n,2)) as given
+    in NC.inp, before scaling."""
     elems = case["elems"]
     n = len(elems)
     ext = case["elem_ids"]
@@ -36,7 +37,10 @@ def element_bc(case):
     kind = np.zeros(n, dtype=np.int32)
     val = np.zeros((n, 4), dtype=np.complex128)
     adm = np.zeros(n, dtype=np.complex128)
-    for lo, hi, key, re, im in case["spec"]["boundary"]:
+    acurve = np.full((n, 2), -1, dtype=np.int64)
+    for line in case["spec"]["boundary"]:
+        lo, hi, key, re, im = line[:5]
+        curves = tuple(line[5:7]) if len(line) >= 7 else (-1, -1)
         z = complex(re, im)
         nb = {"VELO": 0, "PRES": 1, "ADMI": 2, "IMPE": 2}[key]
         if key == "IMPE":
@@ -55,9 +59,32 @@ def element_bc(case):
                 kind[i] = 1
             else:
                 adm[i] = z
+                if curves[0] >= 0:   # NumCalc keeps curves of admittance lines only
+                    acurve[i] = curves
                 if kind[i] == 0:
                     kind[i] = 2
-    return kind, val, adm
+    return kind, val, adm, acurve
+
+
+def curve_factor(spec, cid, f):
+    """Factor of frequency curve ``cid`` at ``f``: linear between its points, its end values
+    outside them (src/NC_EquationSystem.cpp:1963-2009)."""
+    for c, pts in spec.get("curves", []):
+        if c == cid:
+            fr = np.array([p[0] for p in pts])
+            fa = np.array([p[1] for p in pts])
+            return float(np.interp(f, fr, fa))
+    raise KeyError("curve %d" % cid)
+
+
+def _scaled(spec, z, curves, f):
+    """A source amplitude or admittance with its real and imaginary part scaled by curves."""
+    re, im = z.real, z.imag
+    if curves[0] >= 0:
+        re *= curve_factor(spec, curves[0], f)
+    if curves[1] >= 0:
+        im *= curve_factor(spec, curves[1], f)
+    return complex(re, im)
 
 
 def problem_arrays(case, freq, h=1.0):
@@ -68,20 +95,24 @@ def problem_arrays(case, freq, h=1.0):
     nv = np.full(n, elems.shape[1], dtype=np.int32)
     xyz = np.zeros((n, 4, 3))
     xyz[:, :elems.shape[1], :] = nodes[elems]
-    kind, val, adm = element_bc(case)
+    kind, val, adm, acurve = element_bc(case)
+    adm = np.array([_scaled(spec, a, c, freq) for a, c in zip(adm, acurve)], np.complex128)
     omega = 2.0 * np.pi * freq
     k = omega / spec["c"]
     rpfact = spec["rho"] * omega * h
     val = val.copy()
     val[kind == 1] = val[kind == 1] / (1j * rpfact)
     adm = np.where(kind == 2, adm * (1j * h * k), 0.0)
-    ps = np.array([[s[0], s[1], s[2], *(lambda z: (z.real, z.imag))(complex(s[3], s[4]) / (1j * rpfact))]
-                   for s in spec["point_sources"]]).reshape(-1, 5)
+    psc = spec.get("point_source_curves") or [(-1, -1)] * len(spec["point_sources"])
+    pwc = spec.get("plane_wave_curves") or [(-1, -1)] * len(spec["plane_waves"])
+    ps = np.array([[s[0], s[1], s[2], *(lambda z: (z.real, z.imag))(
+        _scaled(spec, complex(s[3], s[4]), c, freq) / (1j * rpfact))]
+                   for s, c in zip(spec["point_sources"], psc)]).reshape(-1, 5)
     pw = []
-    for w in spec["plane_waves"]:
+    for w, c in zip(spec["plane_waves"], pwc):
         d = np.asarray(w[:3], dtype=np.float64)
         d = d / np.sqrt(d @ d)
-        z = complex(w[3], w[4]) / (1j * rpfact)
+        z = _scaled(spec, complex(w[3], w[4]), c, freq) / (1j * rpfact)
         pw.append([*d, z.real, z.imag])
     pw = np.asarray(pw).reshape(-1, 5)
     # NumCalc adds incident waves to the right-hand side only when the plane-wave count plus
@@ -89,7 +120,7 @@ def problem_arrays(case, freq, h=1.0):
     # (src/NC_EquationSystem.cpp:247), to the evaluation grid whenever either is nonzero
     nps = -1 if spec.get("test_point_source") else len(spec["point_sources"])
     return dict(n=n, nv=nv, xyz=xyz, kind=kind, bc=val, adm=adm, ps=ps, pw=pw,
-                k=k, h=h, rpfact=rpfact, rhs_waves=len(pw) + nps > 0)
+                k=k, h=h, rpfact=rpfact, rhs_waves=len(pw) + nps > 0, interior=int(bool(spec.get("interior"))))
 
 
 def rhs_arrays(pa):
@@ -157,8 +188,9 @@ def unfold_symmetry(pa, spec):
 def surface_outputs(pa, sol):
     """pBoundary and vBoundary values from the solved unknowns."""
     kind = pa["kind"]
+    tau = -1.0 if pa.get("interior") else 1.0   # src/NC_3dFunctions.cpp:334
     phi = np.where(kind == 1, pa["bc"][:, 0], sol)
-    v = np.where(kind == 1, sol, pa["bc"][:, 0] - np.where(kind == 2, sol * pa["adm"], 0.0))
+    v = np.where(kind == 1, sol, pa["bc"][:, 0] - np.where(kind == 2, tau * sol * pa["adm"], 0.0))
     return 1j * pa["rpfact"] * phi, v
 
 
@@ -192,7 +224,7 @@ class Oracle:
         p = _Problem(pa["n"], keep["nv"].ctypes.data, keep["xyz"].ctypes.data,
                      keep["kind"].ctypes.data, keep["bc"].ctypes.data, keep["adm"].ctypes.data,
                      len(keep["ps"]), keep["ps"].ctypes.data, len(keep["pw"]), keep["pw"].ctypes.data,
-                     pa["k"], pa["h"], 0)
+                     pa["k"], pa["h"], int(pa.get("interior", 0)))
         return p, keep
 
     def assemble(self, pa):

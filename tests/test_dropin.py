@@ -40,7 +40,11 @@ def project_from_case(folder, name, steps=None):
                     point_sources=[tuple(s) for s in spec["point_sources"]],
                     plane_waves=[tuple(s) for s in spec["plane_waves"]], method_solver=spec["method_solver"],
                     method_bem=spec["method_bem"], precond=spec["precond"], c=spec["c"], rho=spec["rho"],
-                    symmetry=spec.get("symmetry"), test_point_source=spec.get("test_point_source", False))
+                    symmetry=spec.get("symmetry"), test_point_source=spec.get("test_point_source", False),
+                    interior=bool(spec.get("interior")), curves=[(c, [tuple(p) for p in pts]) for c, pts in
+                                                                 spec.get("curves", [])],
+                    point_source_curves=[tuple(c) for c in spec.get("point_source_curves", [])],
+                    plane_wave_curves=[tuple(c) for c in spec.get("plane_wave_curves", [])])
     P.write_project(folder, prj)
     return c
 
@@ -161,13 +165,16 @@ def test_symmetry_estimate_ram_runs(tmp_path, name):
 # dense ones to 1e-6; FMM ones differ from the reference's reflected-cluster FMM by the two
 # FMM truncation errors (the reference's own runs are 1.4e-4 / 3.5e-4 from the exact surface
 # solution and 3e-5 on the grid)
-DROPIN_TOL = {"fmm_sl_point": 1e-5, "fmm_ml_head": 1e-5, "sym_fmm_sl": 2e-3, "sym_fmm_ml": 2e-3}
+DROPIN_TOL = {"fmm_sl_point": 1e-5, "fmm_ml_head": 1e-5, "sym_fmm_sl": 2e-3, "sym_fmm_ml": 2e-3, "curves_fmm": 1e-5,
+              "interior_fmm": 1e-5, **{"scan%d_%s" % (d, m): 1e-5 for d in (1, 2, 3) for m in ("sl", "ml")}}
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["soft_pw", "mixed_pres", "quad", "gauss", "velo_patch", "fmm_sl_point",
                                   "fmm_ml_head", "sym_half", "sym_quarter_anti", "sym_octant", "sym_fmm_sl",
-                                  "sym_fmm_ml", "test_source_1pw", "test_source_2pw"])
+                                  "sym_fmm_ml", "test_source_1pw", "test_source_2pw", "curves", "curves_fmm",
+                                  "interior", "interior_fmm"] +
+                         ["scan%d_%s" % (d, m) for d in (1, 2, 3) for m in ("tbem", "sl", "ml")])
 def test_dropin_matches_reference_be_out(tmp_path, name):
     """be.out of every step and, without symmetry planes, the CGS iteration counts and the
     NC.out analysis tables (the unfolded symmetric system is larger than NumCalc's)."""
@@ -244,3 +251,144 @@ def test_runner_drives_the_dropin_like_the_reference(tmp_path):
                     assert np.abs(va).max() < 1e-12
                 else:
                     assert R.rel_l2(va, vb) < 1e-6, (a, step, key)
+
+
+# ---- text layout of be.out and NC.out against the reference's SHTF project ------------------
+# tests/golden/shtf_text.npz holds the reference's own source_1 input (NC.inp, evaluation grid
+# files) and output text (NC1-20.out; first and last lines of be.out/be.1 and be.12); the
+# object mesh is rebuilt from tests/golden/shtf.npz.  be.out data lines are printed as
+# NC_WriteResultsObjectMesh / NC_WriteResultsEvaluationGrid print them
+# (src/NC_PostProcessing.cpp:447-546, 688-749):
+BE_FORMATS = {"pBoundary": "%5d % E % E", "vBoundary": "%5d % E % E", "pEvalGrid": "%5d % E % E",
+              "vEvalGrid": "%5d %E %E %E %E %E %E"}
+_NUM = re.compile(r"[-+]?(\d+\.?\d*|\.\d+)([eE][-+]?\d+)?")
+
+
+def _be_line_ok(line, fmt):
+    w = line.split()
+    try:
+        vals = (int(w[0]),) + tuple(float(v) for v in w[1:])
+        return fmt % vals == line
+    except (ValueError, TypeError, IndexError):
+        return False
+
+
+def log_skeleton(text):
+    """NC.out with every number replaced by '#', runs of blanks by one, and the wall-clock
+    stamps cut: the text a reader of the log relies on, independent of the values."""
+    out = []
+    for line in text.splitlines():
+        if line.startswith(("Start time:", "End time:")):
+            line = line.split(":")[0]
+        out.append(re.sub(r" +", " ", _NUM.sub("#", line)).rstrip())
+    return out
+
+
+def report_fields(text):
+    """Per step what mesh2hrtf's project report reads from NC*.out
+    (mesh2hrtf/Output2HRTF/write_output_report.py:161-240): frequency, the iteration-limit
+    flag, iteration count, relative error and the presence of the four time lines."""
+    out = {}
+    for blk in text.split(">> S T E P   N U M B E R   A N D   F R E Q U E N C Y <<")[1:]:
+        step = int(re.search(r"Step (\d+),", blk).group(1))
+        f = re.search(r"Frequency = ([\d.eE+-]+) Hz", blk)
+        it = re.search(r"number of iterations = (\d+),", blk)
+        err = re.search(r"relative error = (.+)", blk)
+        times = [bool(re.search(p + r"\d+", blk)) for p in (
+            r"Assembling the equation system         : ", r"Solving the equation system            : ",
+            r"Post processing                        : ", r"Total                                  : ")]
+        out[step] = dict(freq=float(f.group(1)) if f else None, maxit="Maximum number of iterations is reached!" in blk,
+                         its=int(it.group(1)) if it else None, err=float(err.group(1)) if err else None, times=times)
+    return out
+
+
+def shtf_project(root):
+    """The reference's SHTF project folder, source_1, under ``root``; returns its path."""
+    z = np.load(os.path.join(R.GOLDEN, "shtf.npz"))
+    t = np.load(os.path.join(R.GOLDEN, "shtf_text.npz"))
+    om = os.path.join(root, "ObjectMeshes", "Reference")
+    os.makedirs(om)
+    with open(os.path.join(om, "Nodes.txt"), "w") as fh:
+        fh.write("%d\n" % len(z["nodes"]))
+        for i, p in zip(z["node_ids"], z["nodes"]):
+            fh.write("%d %.6f %.6f %.6f\n" % (i, p[0], p[1], p[2]))
+    with open(os.path.join(om, "Elements.txt"), "w") as fh:
+        fh.write("%d\n" % len(z["elems"]))
+        for i, e in zip(z["elem_ids"], z["elems"]):
+            a, b, c = z["node_ids"][e]
+            fh.write("%d %d %d %d 0 0 0\n" % (i, a, b, c))
+    grid = os.path.join(root, "EvaluationGrids", "FourPointHorPlane_r100cm")
+    os.makedirs(grid)
+    with open(os.path.join(grid, "Nodes.txt"), "w") as fh:
+        fh.write(str(t["grid_nodes"]))
+    with open(os.path.join(grid, "Elements.txt"), "w") as fh:
+        fh.write(str(t["grid_elements"]))
+    src = os.path.join(root, "NumCalc", "source_1")
+    os.makedirs(src)
+    with open(os.path.join(src, "NC.inp"), "w") as fh:
+        fh.write(str(t["nc_inp"]))
+    return src
+
+
+def test_be_formats_reproduce_the_reference_text():
+    """The printf layouts above print every stored reference be.out data line back to the
+    same text, so the layout check of the drop-in's files pins them to the reference's."""
+    t = np.load(os.path.join(R.GOLDEN, "shtf_text.npz"))
+    checked = 0
+    for s in (1, 12):
+        for key, fmt in BE_FORMATS.items():
+            head = str(t["be%d_%s_head" % (s, key)]).split("\n")
+            tail = [l for l in str(t["be%d_%s_tail" % (s, key)]).split("\n") if l]
+            assert head[0] == "Mesh2HRTF 1.0.0"
+            for line in [l for l in head[3:] if l] + tail:
+                assert _be_line_ok(line, fmt), (s, key, line)
+                checked += 1
+    assert checked == 2 * (2 * (10 + 2) + 2 * 6)   # boundary files: 10 + 2 lines; grid files: all 4 + 2
+
+
+def test_log_skeleton_masks_values_only():
+    ref = str(np.load(os.path.join(R.GOLDEN, "shtf_text.npz"))["nc_out"])
+    sk = log_skeleton(ref)
+    assert "Step #, Frequency = # Hz" in sk and "CGS solver: number of iterations = #, relative error = #" in sk
+    assert "End time" in sk and "Single level fast multipole BEM" in sk
+    f = report_fields(ref)
+    assert sorted(f) == list(range(1, 21)) and f[1]["its"] == 29 and f[12]["freq"] == 1200.0
+    assert all(all(v["times"]) for v in f.values())
+
+
+@pytest.mark.gpu
+def test_dropin_shtf_text_layout(tmp_path):
+    """The drop-in on the reference's SHTF project, steps 1-20 in one process as the
+    reference ran them: NC1-20.out has the reference's text line for line with the values
+    masked (input echo, frequency table, per-step method, cluster and expansion tables,
+    nonzeros, solver line, time statistics, end stamp), the same report fields (frequency,
+    iterations, no iteration limit, relative error below 1e-9), and be.out/be.1 and be.12
+    have the reference's header lines, line counts, element / node ids and printf layout."""
+    import difflib
+    _need_binary()
+    t = np.load(os.path.join(R.GOLDEN, "shtf_text.npz"))
+    src = shtf_project(str(tmp_path))
+    print("drop-in SHTF steps 1-20", flush=True)
+    r = subprocess.run([NUMCALC, "-istart", "1", "-iend", "20"], cwd=src, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    ours_text = open(os.path.join(src, "NC1-20.out")).read()
+    ref_text = str(t["nc_out"])
+    ours, ref = log_skeleton(ours_text), log_skeleton(ref_text)
+    diff = list(difflib.unified_diff(ref, ours, "reference", "drop-in", lineterm="", n=1))
+    assert not diff, "\n".join(diff[:80])
+    fo, fr = report_fields(ours_text), report_fields(ref_text)
+    assert sorted(fo) == sorted(fr)
+    for s in fr:
+        assert fo[s]["freq"] == fr[s]["freq"] and fo[s]["its"] == fr[s]["its"], (s, fo[s], fr[s])
+        assert not fo[s]["maxit"] and all(fo[s]["times"]) and fo[s]["err"] < 1e-9
+    for s in (1, 12):
+        for key, fmt in BE_FORMATS.items():
+            lines = open(os.path.join(src, "be.out", "be.%d" % s, key)).read().split("\n")
+            head = str(t["be%d_%s_head" % (s, key)]).split("\n")
+            tail = str(t["be%d_%s_tail" % (s, key)]).split("\n")
+            assert len(lines) == int(t["be%d_%s_lines" % (s, key)]), (s, key)
+            assert lines[:3] == head[:3], (s, key, lines[:3])
+            assert [l.split()[0] for l in lines[3:len(head)] if l] == [l.split()[0] for l in head[3:] if l]
+            assert [l.split()[0] for l in lines[-3:] if l] == [l.split()[0] for l in tail if l]
+            bad = [l for l in lines[3:] if l and not _be_line_ok(l, fmt)]
+            assert not bad, (s, key, bad[:3])

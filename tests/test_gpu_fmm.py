@@ -145,6 +145,40 @@ def test_block_jacobi_gmres_same_solution(ncg, name):
     assert res["block"][1] < res["none"][1]
 
 
+@pytest.mark.parametrize("name", ["fmm_ml_head", "fmm_sl_point", "fmm_sl_adm"])
+def test_mixed_refinement_residual_is_the_fp64_operators(ncg, name):
+    """The mixed form's refinement residual (FP32-stored operator, FP64 signatures and FP64
+    sums) measured against the FP64 operator's own residual: with r64 = A64 (x64 - xm) and
+    A64 x64 = b to CGS's 1e-11, the relative residual of the mixed solution xm under the
+    FP64 operator stays within 3e-6 when the mixed solve reports < 1e-6, at every step in
+    at most 3 refinement steps; the mixed solution is within 1e-5 of the FP64 one."""
+    case = R.load_case(name)
+    spec = case["spec"]
+    for f in spec["freqs"]:
+        pa = R.problem_arrays(case, f)
+        ps = [list(v) + [0] for v in pa["ps"]]
+        pw = [list(v) + [0] for v in pa["pw"]]
+        kw = dict(bc=pa["bc"][:, 0][None, :], adm=pa["adm"], point_sources=ps, plane_waves=pw)
+        p64 = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], None, node_index=case["elems"], method_bem=spec["method_bem"])
+        pm = ncg.Problem(pa["nv"], pa["xyz"], pa["kind"], None, node_index=case["elems"], method_bem=spec["method_bem"])
+        p64.assemble(pa["k"], precision=ncg.FP64, **kw)
+        pm.assemble(pa["k"], precision=ncg.MIXED, **kw)
+        assert p64.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-11)["converged"]
+        st = pm.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_tolerance=1e-6, refinement_steps=8)
+        x64, xm = p64.solution(), pm.solution()
+        b = p64.apply(x64)
+        r64 = p64.apply(x64 - xm)
+        rel64 = np.linalg.norm(r64[0]) / np.linalg.norm(b[0])
+        print("%s f=%.0f reported %.2e FP64-operator residual %.2e refinement steps %d vs FP64 %.2e" % (
+            name, f, st["relative_residual"][0], rel64, st["refinement_steps"], R.rel_l2(xm[0], x64[0])), flush=True)
+        assert st["converged"] and st["relative_residual"][0] < 1e-6
+        assert st["refinement_steps"] <= 3
+        assert rel64 < 3e-6
+        assert R.rel_l2(xm[0], x64[0]) < 1e-5
+        p64.close()
+        pm.close()
+
+
 @pytest.mark.parametrize("f", [12000.0, 23700.0])
 def test_graded_large_mesh_mixed_vs_fp64(ncg, f):
     """The graded large case: the head-like surface meshed with 60320 triangles, edges 1 mm
