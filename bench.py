@@ -392,7 +392,7 @@ def head_main(a):
         n, m, nr, a.lanes, a.group, nf, a.steps), file=sys.stderr, flush=True)
 
     # ---- served mixed form: every step (sweep) submitted at once ------------------------
-    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group)
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group, autotune=not a.no_autotune)
     mid = srv.add_mesh(**geo)
     sweep = [freq_spec(src, f) for f in freqs]
     for j in [srv.submit(mid, sweep, **SERVED) for _ in range(a.warmup)]:
@@ -427,29 +427,31 @@ def head_main(a):
     print("# poisson %.1f req/s: latency p50 %.1f ms p99 %.1f ms" % (rate, np.percentile(pl, 50), np.percentile(pl, 99)),
           file=sys.stderr, flush=True)
     info = srv.stats()
+    tunings = srv.tunings()
     srv.close()
 
     # ---- FP64 plain method (the drop-in), one frequency after the other -----------------
     nplain = nf if a.plain_steps < 0 else min(a.plain_steps, nf)
     plain_f = pick(freqs, nplain)
-    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
-                       method_bem=4)
-    sp = freq_spec(src, plain_f[0])
-    prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
-    prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
-    ref, plain_lat = {}, []
-    t0 = time.perf_counter()
-    for f in plain_f:
-        ts = time.perf_counter()
-        sp = freq_spec(src, f)
+    ref, plain_lat, plain_elapsed = {}, [], 0.0
+    if plain_f:
+        prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
+                           method_bem=4)
+        sp = freq_spec(src, plain_f[0])
         prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
         prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
-        x0 = prob.solution()
-        p0, _ = prob.field(gradient=False)
-        plain_lat.append((time.perf_counter() - ts) * 1e3)
-        ref[f] = (x0, p0)
-    plain_elapsed = time.perf_counter() - t0
-    prob.close()
+        t0 = time.perf_counter()
+        for f in plain_f:
+            ts = time.perf_counter()
+            sp = freq_spec(src, f)
+            prob.assemble(sp["k"], nrhs=nr, point_sources=sp["point_sources"])
+            prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)
+            x0 = prob.solution()
+            p0, _ = prob.field(gradient=False)
+            plain_lat.append((time.perf_counter() - ts) * 1e3)
+            ref[f] = (x0, p0)
+        plain_elapsed = time.perf_counter() - t0
+        prob.close()
     print("# plain FP64: %d frequencies in %.1f s" % (nplain, plain_elapsed), file=sys.stderr, flush=True)
 
     # every timed solve against the plain solution at its frequency; only checked solves count
@@ -551,16 +553,17 @@ def head_main(a):
         "valid_solves": valid,
         "checked_solves": checked,
         "solves": a.steps * nf * nr,
-        "plain_fp64": {"value": nplain * nr / plain_elapsed, "ms_per_frequency": plain_elapsed / nplain * 1e3,
+        "plain_fp64": None if not nplain else {"value": nplain * nr / plain_elapsed, "ms_per_frequency": plain_elapsed / nplain * 1e3,
                        "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99)),
                        "frequencies": nplain,
                        "form": "the drop-in's FP64 plain method: NumCalc's method choice, ILU-CGS to 1e-9, one "
                                "frequency at a time"},
-        "speedup_vs_plain": value / (nplain * nr / plain_elapsed),
+        "speedup_vs_plain": value / (nplain * nr / plain_elapsed) if nplain else None,
         "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
                                  "eval_grid_max": float(max(err_e)) if err_e else None},
         "server": {"peak_resident_bytes": info["peak_resident_bytes"],
-                   "lane_ms_timed": {k: round(lane1[k] - lane0[k], 1) for k in lane1}},
+                   "lane_ms_timed": {k: round(lane1[k] - lane0[k], 1) for k in lane1},
+                   "launch_shapes": {k: t for k, t in tunings} if tunings else "defaults (autotune off)"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -582,6 +585,8 @@ def main():
     ap.add_argument("--poisson", type=int, default=160, help="requests of the latency run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=16, help="CPU baseline: at most this many NumCalc processes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="served run: keep the default launch shapes instead of measuring them per operator shape")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--stream-meshes", type=int, default=32, help="stream: number of meshes")
