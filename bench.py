@@ -60,7 +60,7 @@ from mesh2hrtf_amd import ncg, project as P  # noqa: E402
 METRIC = "frequency-solves/sec at rel-L2≤1e-3 vs FP64 oracle; p50/p99 solve latency"
 C, RHO = 343.0, 1.1839
 HBM_PEAK = 8.0e12
-TRAFFIC_PROFILE = "k_near_spmv_head20480.txt"
+TRAFFIC_PROFILE = "kg_near_spmv_head20480.txt"
 TRAFFIC_PROFILE_BATCHED = "k_matvec_mfma_batched.txt"
 SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50, max_iterations=500,
               refinement_steps=8, refinement_tolerance=1e-5)
@@ -111,19 +111,29 @@ def isolated_kernels(geo, src, f, launches=20):
         prob.close()
 
 
-def profiled_traffic(kernel, profile=TRAFFIC_PROFILE):
-    """measured HBM bytes per launch of `kernel` in the single-problem timing, from the
-    committed rocprofv3 counter summary of this benchmark (a line 'traffic <kernel>
-    <bytes>'), or (None, None)"""
+def profiled_traffic(kernel, profile=TRAFFIC_PROFILE, bytes_per_launch=None, near_modes=(0,)):
+    """measured HBM bytes per launch of `kernel`, from the committed rocprofv3 counter
+    summary of this benchmark, or (None, None).  The summary holds either a line
+    'traffic <kernel> <bytes>' (bytes per launch of a fixed launch) or lines
+    'traffic_ratio <kernel> near_mode=<m> <ratio>' (counter bytes over algorithmic bytes of
+    the same dispatches); a ratio is applied to this run's algorithmic bytes per launch,
+    averaged over the near-field modes the run's launch shapes use."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None, None
+    ratios = {}
     with open(path) as fh:
         for line in fh:
             w = line.split()
             if len(w) == 3 and w[0] == "traffic" and w[1] == kernel:
                 return float(w[2]), "profiles/" + profile
-    return None, None
+            if len(w) == 4 and w[0] == "traffic_ratio" and w[1] == kernel and w[2].startswith("near_mode="):
+                ratios[int(w[2].split("=")[1])] = float(w[3])
+    use = [ratios[m] for m in sorted(set(near_modes)) if m in ratios]
+    if not use or bytes_per_launch is None or len(use) != len(set(near_modes)):
+        return None, None
+    r = sum(use) / len(use)
+    return bytes_per_launch * r, "profiles/%s (counter/algorithmic ratio %.3f)" % (profile, r)
 
 
 def pick(freqs, count, offset=0):
@@ -392,7 +402,8 @@ def head_main(a):
         n, m, nr, a.lanes, a.group, nf, a.steps), file=sys.stderr, flush=True)
 
     # ---- served mixed form: every step (sweep) submitted at once ------------------------
-    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group, autotune=not a.no_autotune)
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group, autotune=not a.no_autotune,
+                     tuning_file=a.tuning_file)
     mid = srv.add_mesh(**geo)
     sweep = [freq_spec(src, f) for f in freqs]
     for j in [srv.submit(mid, sweep, **SERVED) for _ in range(a.warmup)]:
@@ -475,14 +486,17 @@ def head_main(a):
     if top:
         s = kst[top]
         ach = s["bytes"] / (s["total_ms"] * 1e-3)
-        traffic, tsrc = profiled_traffic(top)
+        modes = [t["near_mode"] for _, t in tunings] if tunings else [0]
+        traffic, tsrc = profiled_traffic(top, bytes_per_launch=s["bytes"] / s["calls"], near_modes=modes)
         roof = {"bound": "hbm", "kernel": top, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": traffic, "traffic_source": tsrc,
                 "bytes_per_launch": s["bytes"] / s["calls"], "mean_launch_ms": s["total_ms"] / s["calls"],
                 "launches": s["calls"],
                 "timing": "HIP events around each launch on the group's stream over the timed region, "
                           "%d lanes sharing the GPU" % a.lanes,
-                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in sorted(kst)}}
+                "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in sorted(kst)},
+                "timed_kernels": {k: {"calls": kst[k]["calls"], "total_ms": round(kst[k]["total_ms"], 3),
+                                      "algorithmic_bytes": kst[k]["bytes"]} for k in sorted(kst)}}
         if not a.no_isolated:
             iso = isolated_kernels(geo, src, iso_f)
             roof["isolated_single_problem"] = {
@@ -587,6 +601,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=16, help="CPU baseline: at most this many NumCalc processes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="served run: keep the default launch shapes instead of measuring them per operator shape")
+    ap.add_argument("--tuning-file", default=None,
+                    help="served run: launch shapes per operator shape to load (and to save what is tuned)")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--stream-meshes", type=int, default=32, help="stream: number of meshes")
