@@ -111,6 +111,48 @@ def isolated_kernels(geo, src, f, launches=20):
         prob.close()
 
 
+# VALU peaks of MI355X: FP32 157.3 TFLOP/s counts packed (two-wide) FMAs, which this build
+# does not emit, so its FP32 kernels are also quoted against the one-wide 78.6; FP64 78.6
+VALU_PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
+FP32_QUADRATURE = ("k_fmm_near<f32>", "k_assemble_far<f32>", "k_field<f32>")
+
+
+def quadrature_roofline(kst):
+    """FLOP/s of the assembly and field kernels (algorithmic FLOPs: 60 per Green's-function
+    evaluation counted on the device, 16 per T plane-wave term, 20 per S entry, 7 per D
+    series term) against the VALU peak of their arithmetic."""
+    out = {}
+    for k, v in sorted(kst.items()):
+        if not v.get("flops") or not v["total_ms"]:
+            continue
+        prec = "fp32" if k in FP32_QUADRATURE else "fp64"
+        fs = v["flops"] / (v["total_ms"] * 1e-3)
+        out[k] = {"calls": v["calls"], "mean_launch_ms": round(v["total_ms"] / v["calls"], 4),
+                  "gflop_per_launch": round(v["flops"] / v["calls"] / 1e9, 3), "tflops": round(fs / 1e12, 3),
+                  "precision": prec, "frac_of_peak": round(fs / VALU_PEAK[prec], 4)}
+        if prec == "fp32":
+            out[k]["frac_of_unpacked_fp32"] = round(fs / 78.6e12, 4)
+    return out
+
+
+def isolated_assembly(geo, src, f):
+    """kernel statistics of one mixed-form assembly and field pass at f with nothing else on
+    the GPU (the evaluation counts come from a first, counting pass)"""
+    prob = ncg.Problem(geo["nv"], geo["xyz"], geo["kind"], geo["eval_xyz"], node_index=geo["node_index"],
+                       method_bem=geo.get("method_bem", 4))
+    try:
+        sp = freq_spec(src, f)
+        prob.kernel_timing(True)
+        for _ in range(2):
+            prob.kernel_stats(reset=True)
+            prob.assemble(sp["k"], nrhs=sp["nrhs"], point_sources=sp["point_sources"], precision=ncg.MIXED)
+            prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_tolerance=1e-5)
+            prob.field(gradient=True)
+        return {k: v for k, v in prob.kernel_stats().items() if v.get("flops")}
+    finally:
+        prob.close()
+
+
 def profiled_traffic(kernel, profile=TRAFFIC_PROFILE, bytes_per_launch=None, near_modes=(0,)):
     """measured HBM bytes per launch of `kernel`, from the committed rocprofv3 counter
     summary of this benchmark, or (None, None).  The summary holds either a line
@@ -210,7 +252,7 @@ def batched_main(a):
                        achieved_gbs=ach / 1e9, tflops=flops / t / 1e12)
     finally:
         prob.close()
-    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
+    top = max((k for k in kst if kst[k]["bytes"] > 0), key=lambda k: kst[k]["total_ms"], default=None)
     roof = None
     if iso:
         traffic, tsrc = profiled_traffic("k_matvec_mfma<f32>", TRAFFIC_PROFILE_BATCHED)
@@ -481,7 +523,7 @@ def head_main(a):
     value = valid / elapsed
 
     # ---- roofline: the apply kernel with the largest share of the timed region ----------
-    top = max(kst, key=lambda k: kst[k]["total_ms"]) if kst else None
+    top = max((k for k in kst if kst[k]["bytes"] > 0), key=lambda k: kst[k]["total_ms"], default=None)
     roof = None
     if top:
         s = kst[top]
@@ -496,12 +538,15 @@ def head_main(a):
                           "%d lanes sharing the GPU" % a.lanes,
                 "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in sorted(kst)},
                 "timed_kernels": {k: {"calls": kst[k]["calls"], "total_ms": round(kst[k]["total_ms"], 3),
-                                      "algorithmic_bytes": kst[k]["bytes"]} for k in sorted(kst)}}
+                                      "algorithmic_bytes": kst[k]["bytes"]} for k in sorted(kst) if kst[k]["bytes"]},
+                "quadrature_served": quadrature_roofline(kst)}
         if not a.no_isolated:
             iso = isolated_kernels(geo, src, iso_f)
             roof["isolated_single_problem"] = {
                 k: {"mean_launch_ms": v["total_ms"] / v["calls"], "achieved_gbs": v["bytes"] / v["total_ms"] / 1e6}
-                for k, v in iso.items() if v["calls"]}
+                for k, v in iso.items() if v["calls"] and v["bytes"]}
+            roof["quadrature_isolated"] = {"%.0f Hz" % f: quadrature_roofline(isolated_assembly(geo, src, f))
+                                           for f in (iso_f, freqs[-1])}
 
     # ---- CPU baseline: the reference NumCalc as a manage_numcalc-style pool -------------
     cpu = None

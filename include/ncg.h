@@ -239,21 +239,29 @@ int ncg_apply_noalloc(ncg_problem *p, const double *x, double *y, int32_t n_rhs)
  * duration of the dominant kernel of each phase over the calls since the last reset
  * (HIP events on the problem's stream).  Layout of t[8]:
  * [0] assemble ms  [1] solve ms  [2] field ms  [3] far-field quadrature kernel avg ms
- * [4] matvec kernel avg ms  [5] matvec launches  [6] far-field quadrature evaluations
- * of the last dense assembly (counted only while kernel timing is on, else 0)
+ * [4] matvec kernel avg ms  [5] matvec launches  [6] Green's-function evaluations of
+ * the last assembly's quadrature (regular, subdivided and singular; counted once per plan
+ * while kernel timing is on, else 0)
  * [7] bytes of the stored operator. */
 int ncg_timings(ncg_problem *p, double *t, int32_t reset);
 
 /* Per-kernel timing of the operator apply (near-field, T, D, S or dense matvec
- * launches): HIP events around every launch on the problem's stream while enabled, and
- * the algorithmic bytes each launch must move (stored operator once per 4 load cases,
- * vectors in and out).  No counterpart in the reference; for measurement.  Stats are
+ * launches, single-problem and grouped) and of the assembly and field kernels (regular,
+ * subdivided and singular quadrature, FMM T / S / D tables, evaluation-grid field): HIP
+ * events around every launch on the problem's stream while enabled, the algorithmic bytes
+ * each apply launch must move (stored operator once per 4 load cases, vectors in and out)
+ * and the algorithmic FLOPs of each assembly launch: 60 per Green's-function evaluation
+ * (G and its three normal derivatives at one quadrature point; the evaluations are counted
+ * on the device once per plan, in the first assembly and field pass with timing on), 16 per
+ * plane-wave evaluation of T (element Gauss point x sphere point), 20 per S entry, 7 per
+ * term of the D series.  No counterpart in the reference; for measurement.  Stats are
  * accumulated since the last reset; count receives the number of kernels with launches. */
 typedef struct {
     char name[48];
     int64_t launches;
     double total_ms;
     double bytes;
+    double flops;
 } ncg_kernel_stat;
 int ncg_set_kernel_timing(ncg_problem *p, int32_t on);
 int ncg_kernel_stats(ncg_problem *p, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset);

@@ -269,12 +269,27 @@ struct Fnv {
     }
 };
 
-// one timed launch: kernel id, event pair slot, algorithmic bytes
+// one timed launch: kernel id, event pair slot, algorithmic bytes, and its algorithmic
+// FLOPs: given, or (quad >= 0) the counted evaluations of quadrature kind quad times that
+// kind's FLOPs per evaluation
 struct KernelRec {
     int id;
     size_t slot;
     double bytes;
+    int quad;
+    double flops;
 };
+
+// Quadrature kinds whose Green's-function evaluations are counted on the device in the
+// measurement pass of a plan (the counts depend on the geometry and the plan only): dense
+// regular and subdivided pairs, singular diagonal, FMM regular and subdivided near pairs,
+// evaluation-grid field.
+enum { kQFar = 0, kQDenseNear, kQDiag, kQFmmNear, kQFmmAdj, kQField, kQuadKinds };
+// FLOPs per Green's-function evaluation (G, dG/dn_y, dG/dn_x, d2G/dn_x dn_y; FMA = 2,
+// sqrt, reciprocal, sin and cos = 1 each), and per plane-wave evaluation of the FMM
+// aggregation T (per element Gauss point and sphere point), per disaggregation entry S,
+// and per term of the translation series D
+constexpr double kGreenFlops = 60.0, kTFlops = 16.0, kSFlops = 20.0, kDTermFlops = 7.0;
 
 // device state of the incomplete LU preconditioner (pattern CSR and level schedules)
 struct DevIlu {
@@ -342,7 +357,12 @@ struct ncg_problem {
     int *src = nullptr;
     int nsrc = 0;
     int *err = nullptr;
+    // quadrature evaluation counts: device counters of the measurement pass, the counts of
+    // each kind and the structure signature they hold for, the kinds being counted
     unsigned long long *evals = nullptr;
+    double q_evals[kQuadKinds] = {0};
+    uint64_t q_sig[kQuadKinds] = {0};
+    int q_pending = 0;
     // Krylov workspace
     std::vector<double *> work;
     size_t work_cap = 0;
@@ -396,8 +416,8 @@ struct ncg_problem {
     int ktime = 0;
     std::vector<EventPair> kpool;
     std::vector<KernelRec> kpend;
-    double k_ms[32] = {0}, k_bytes[32] = {0};
-    long long k_calls[32] = {0};
+    double k_ms[64] = {0}, k_bytes[64] = {0}, k_flops[64] = {0};
+    long long k_calls[64] = {0};
 
     ncgk::DevElements elements() const { return {n, X, nv, cen, nrm, area, jac, kind}; }
     ncgk::BMParams bm() const
@@ -463,17 +483,33 @@ const char *const kKernelNames[] = {"k_near_spmv<f64>",   "k_near_spmv<f32>",   
                                     "k_matvec<f32>",      "k_matvec_mfma<f32>", "k_matvec_mfma64",
                                     "kg_near_spmv<f64>",  "kg_near_spmv<f32>",  "kg_apply_T<f64>",
                                     "kg_apply_T<f32>",    "kg_apply_D<f64>",    "kg_apply_D<f32>",
-                                    "kg_apply_S<f64>",    "kg_apply_S<f32>"};
-constexpr int kNumKernels = 20, kGroupedKid = 12;
+                                    "kg_apply_S<f64>",    "kg_apply_S<f32>",
+                                    // quadrature and FMM table kernels of the assembly and the field
+                                    "k_fmm_near<f64>",    "k_fmm_near<f32>",    "k_fmm_near_adj",
+                                    "k_assemble_diag",    "k_fmm_T<f64>",       "k_fmm_T<f32>",
+                                    "k_fmm_S<f64>",       "k_fmm_S<f32>",       "k_fmm_D",
+                                    "k_assemble_far<f64>", "k_assemble_far<f32>", "k_assemble_near",
+                                    "k_field<f64>",       "k_field<f32>"};
+constexpr int kNumKernels = 34, kGroupedKid = 12;
+enum {
+    kKidFmmNear = 20, kKidFmmAdj = 22, kKidDiag, kKidFmmT, kKidFmmS = 26, kKidFmmD = 28, kKidFar = 29,
+    kKidDenseNear = 31, kKidField = 32
+};
 
 struct KernelScope {
     ncg_problem *p;
     int id;
     double bytes;
     EventPair *ev = nullptr;
-    KernelScope(ncg_problem *p_, int id_, double bytes_);
+    KernelScope(ncg_problem *p_, int id_, double bytes_, int quad = -1, double flops = 0.0);
     ~KernelScope();
 };
+
+// Counter of quadrature kind q for the assembly or field pass about to be queued: the
+// device counter when kernel timing is on and kind q has no count for structure sig yet
+// (the measurement pass; q_collect reads the counts back), else nullptr.
+unsigned long long *q_counter(ncg_problem *p, int q, uint64_t sig);
+void q_collect(ncg_problem *p);
 
 // Scratch of an allocation-free apply (ncg_apply_noalloc): the FMM signatures t and s of
 // every level, or the matrix-core operand and partial sums of a dense operator, carved from
@@ -602,16 +638,45 @@ void harvest_mv(ncg_problem *p)
         p->mv_calls++;
     }
     p->mv_used = 0;
+    q_collect(p);
+    static const double qflops[kQuadKinds] = {kGreenFlops, kGreenFlops, kGreenFlops,
+                                              kGreenFlops, kGreenFlops, kGreenFlops};
     for (size_t i = 0; i < p->kpend.size(); ++i) {
         const KernelRec &r = p->kpend[i];
         p->k_ms[r.id] += elapsed(p->kpool[r.slot]);
         p->k_calls[r.id]++;
         p->k_bytes[r.id] += r.bytes;
+        p->k_flops[r.id] += r.quad >= 0 ? p->q_evals[r.quad] * qflops[r.quad] : r.flops;
     }
     p->kpend.clear();
 }
 
-KernelScope::KernelScope(ncg_problem *p_, int id_, double bytes_) : p(p_), id(id_), bytes(bytes_)
+unsigned long long *q_counter(ncg_problem *p, int q, uint64_t sig)
+{
+    if (!p->ktime || !p->evals) return nullptr;
+    if (p->q_sig[q] == sig && !(p->q_pending & (1 << q))) return nullptr;
+    if (!(p->q_pending & (1 << q))) {
+        hipMemsetAsync(p->evals + q, 0, sizeof(unsigned long long), p->stream);
+        p->q_pending |= 1 << q;
+    }
+    p->q_sig[q] = sig;
+    return p->evals + q;
+}
+
+void q_collect(ncg_problem *p)
+{
+    if (!p->q_pending) return;
+    unsigned long long c[kQuadKinds];
+    if (hipMemcpyAsync(c, p->evals, sizeof(c), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+        return;
+    for (int q = 0; q < kQuadKinds; ++q)
+        if (p->q_pending & (1 << q)) p->q_evals[q] = (double)c[q];
+    p->q_pending = 0;
+}
+
+KernelScope::KernelScope(ncg_problem *p_, int id_, double bytes_, int quad, double flops)
+    : p(p_), id(id_), bytes(bytes_)
 {
     if (!p->ktime) return;
     const size_t slot = p->kpend.size();
@@ -621,7 +686,7 @@ KernelScope::KernelScope(ncg_problem *p_, int id_, double bytes_) : p(p_), id(id
         hipEventCreate(&e.b);
         p->kpool.push_back(e);
     }
-    p->kpend.push_back(KernelRec{id, slot, bytes});
+    p->kpend.push_back(KernelRec{id, slot, bytes, quad, flops});
     ev = &p->kpool[slot];
     record(p, ev->a);
 }
@@ -1044,20 +1109,40 @@ int assemble_fmm(ncg_problem *p)
     // values
     const ncgk::DevElements E = p->elements();
     const DevLevel &DF = F.lev.back();
-    ncgk::fmm_near(s, E, b, p->adm, DF.chunks(), DF.cl_ptr, DF.cl_elem, F.near_ptr, F.near_col, F.near_wmax,
-                   F.near_val.p, mixed);
-    ncgk::fmm_near_adj(s, E, b, p->adm, p->near_rows, p->near_cols, F.adj_pos, p->nnear, F.near_val.p, mixed, p->err);
+    {
+        KernelScope ks(p, kKidFmmNear + mixed, 0.0, kQFmmNear);
+        ncgk::fmm_near(s, E, b, p->adm, DF.chunks(), DF.cl_ptr, DF.cl_elem, F.near_ptr, F.near_col, F.near_wmax,
+                       F.near_val.p, mixed, q_counter(p, kQFmmNear, F.sig));
+    }
+    {
+        KernelScope ks(p, kKidFmmAdj, 0.0, kQFmmAdj);
+        ncgk::fmm_near_adj(s, E, b, p->adm, p->near_rows, p->near_cols, F.adj_pos, p->nnear, F.near_val.p, mixed,
+                           p->err, q_counter(p, kQFmmAdj, F.sig));
+    }
     if (p->nsrc)
         ncgk::rhs_sources(s, E, b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err, F.elem_cl_fine, F.cl_near,
                           F.ncl_fine);
-    ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0,
-                        F.near_val.p, mixed, p->rhs, F.diag_pos);
+    {
+        KernelScope ks(p, kKidDiag, 0.0, kQDiag);
+        ncgk::assemble_diag(s, E, b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src, p->nsrc > 0,
+                            F.near_val.p, mixed, p->rhs, F.diag_pos, q_counter(p, kQDiag, 1));
+    }
     for (DevLevel &D : F.lev) {
         if ((rc = F.dcoef.ensure(sizeof(double) * (size_t)std::max(D.nfan, 1) * (3 + 2 * D.L)))) return rc;
-        ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32,
-                    F.dcoef.d());
-        ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, D.f32);
-        ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.cl_ptr, D.cl_elem, D.u.d(), D.w.d(), D.S.p, D.f32);
+        const double nP = (double)n * D.P;
+        {
+            KernelScope ks(p, kKidFmmD, 0.0, -1, (double)D.nfan * D.P * D.L * kDTermFlops);
+            ncgk::fmm_D(s, D.L, D.P, p->k, h, D.ccen, D.ncl, D.fan_row, D.fan_idx, D.nfan, D.u.d(), D.D.p, D.f32,
+                        F.dcoef.d());
+        }
+        {
+            KernelScope ks(p, kKidFmmT + D.f32, 0.0, -1, nP * 4.0 * kTFlops);
+            ncgk::fmm_T(s, E, b, p->adm, D.P, D.ccen, D.elem_cl, D.elem_pos, D.u.d(), D.T.p, D.f32);
+        }
+        {
+            KernelScope ks(p, kKidFmmS + D.f32, 0.0, -1, nP * kSFlops);
+            ncgk::fmm_S(s, E, b, D.P, D.ccen, D.elem_cl, D.cl_ptr, D.cl_elem, D.u.d(), D.w.d(), D.S.p, D.f32);
+        }
         if (p->nsrc) {
             // right-hand side far part: b += S D t_v (NC_RHSvecTof*, src/NC_EquationSystem.cpp:1068-1118)
             ncgk::fmm_Tvec_bc(s, E, b, D.P, D.ncl, D.ccen, D.cl_ptr, D.cl_elem, D.u.d(), p->bcval, nr, D.tv.d());
@@ -1147,8 +1232,13 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
     ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
                      nr, F.z.p, D0.f32);
     ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
-    ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad,
-                p->precision == NCG_MIXED, p->err, F.pt_ev, F.el_ptr, F.el_idx, F.fphi.p, F.fgrad.p);
+    {
+        const int f32 = p->precision == NCG_MIXED;
+        KernelScope ks(p, kKidField + f32, 0.0, kQField);
+        ncgk::field(s, p->elements(), b, p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi, dgrad, f32,
+                    p->err, F.pt_ev, F.el_ptr, F.el_idx, F.fphi.p, F.fgrad.p,
+                    q_counter(p, kQField, F.sig * 31 + F.ev_sig + 2));
+    }
     return 0;
 }
 
@@ -1405,12 +1495,18 @@ int assemble_finish(ncg_problem *p)
         p->assembled = 0;
         return rc;
     }
-    if (dense && p->ktime) {
-        unsigned long long ev = 0;
-        NCG_HIP(hipMemcpyAsync(&ev, p->evals, sizeof(ev), hipMemcpyDeviceToHost, p->stream));
-        NCG_HIP(hipStreamSynchronize(p->stream));
-        p->last_evals = (double)ev;
-    }
+    q_collect(p);
+    // quadrature evaluations of this assembly, when its kinds have been counted
+    const int kinds = dense ? (1 << kQFar) | (1 << kQDenseNear) | (1 << kQDiag)
+                            : (1 << kQFmmNear) | (1 << kQFmmAdj) | (1 << kQDiag);
+    p->last_evals = 0.0;
+    bool known = true;
+    for (int q = 0; q < kQuadKinds; ++q)
+        if (kinds & (1 << q)) {
+            if (p->q_sig[q] == 0) known = false;
+            p->last_evals += p->q_evals[q];
+        }
+    if (!known) p->last_evals = 0.0;
     p->t_asm = elapsed(p->ev_asm);
     if (dense) {
         p->far_ms_sum += elapsed(p->ev_far);
@@ -1436,8 +1532,11 @@ int field_enqueue(ncg_problem *p, bool want_grad)
         rc = field_fmm(p, pot, vel, dphi, want_grad ? dgrad : nullptr);
         if (rc) return rc;
     } else {
+        const int f32 = p->precision == NCG_MIXED;
+        KernelScope ks(p, kKidField + f32, 0.0, kQField);
         ncgk::field(p->stream, p->elements(), p->bm(), p->eval, m, nr, pot, vel, p->ps, p->nps, p->pw, p->npw, dphi,
-                    want_grad ? dgrad : nullptr, p->precision == NCG_MIXED, p->err);
+                    want_grad ? dgrad : nullptr, f32, p->err, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    q_counter(p, kQField, 1));
     }
     record(p, p->ev_field.b);
     NCG_HIP(hipGetLastError());
@@ -2552,7 +2651,7 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     if ((rc = dalloc(&p->X, 12 * (size_t)n)) || (rc = dalloc(&p->cen, 3 * (size_t)n)) ||
         (rc = dalloc(&p->nrm, 3 * (size_t)n)) || (rc = dalloc(&p->area, n)) || (rc = dalloc(&p->jac, n)) ||
         (rc = dalloc(&p->nv, n)) || (rc = dalloc(&p->kind, n)) || (rc = dalloc(&p->eval, 3 * (size_t)p->m)) ||
-        (rc = dalloc(&p->adm, 2 * (size_t)n)) || (rc = dalloc(&p->err, 1)) || (rc = dalloc(&p->evals, 1)) ||
+        (rc = dalloc(&p->adm, 2 * (size_t)n)) || (rc = dalloc(&p->err, 1)) || (rc = dalloc(&p->evals, (size_t)kQuadKinds)) ||
         (rc = dalloc(&p->dvec, n)) || (rc = dalloc(&p->active, 64)))
         return bail(rc);
     const int nb = ncgk::dot_blocks(n);
@@ -2693,7 +2792,6 @@ int assemble_impl(ncg_problem *p, const ncg_frequency *f, int32_t precision, boo
     NCG_HIP(hipMemcpyAsync(p->adm, admh.data(), sizeof(double) * admh.size(), hipMemcpyHostToDevice, s));
     NCG_HIP(hipMemcpyAsync(p->bcval, bch.data(), sizeof(double) * bch.size(), hipMemcpyHostToDevice, s));
     if (p->nsrc) NCG_HIP(hipMemcpyAsync(p->src, srch.data(), sizeof(int) * p->nsrc, hipMemcpyHostToDevice, s));
-    NCG_HIP(hipMemsetAsync(p->evals, 0, sizeof(unsigned long long), s));
     const ncgk::BMParams b = p->bm();
     const int mixed = precision == NCG_MIXED;
     if (p->plan.method != 0) {
@@ -2712,14 +2810,25 @@ int assemble_impl(ncg_problem *p, const ncg_frequency *f, int32_t precision, boo
     record(p, p->ev_asm.a);
     record(p, p->ev_far.a);
     p->dbj_ready = 0;
-    // quadrature evaluations are counted only while kernel timing is on (measurement)
-    ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed, p->tune.assemble_rows > 0 ? p->tune.assemble_rows : 32,
-                       p->ktime ? p->evals : nullptr);
+    // quadrature evaluations are counted once per problem, in the first assembly with
+    // kernel timing on (measurement)
+    {
+        KernelScope ks(p, kKidFar + mixed, 0.0, kQFar);
+        ncgk::assemble_far(s, p->elements(), b, p->adm, p->A, mixed,
+                           p->tune.assemble_rows > 0 ? p->tune.assemble_rows : 32, q_counter(p, kQFar, 1));
+    }
     record(p, p->ev_far.b);
-    ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err);
+    {
+        KernelScope ks(p, kKidDenseNear, 0.0, kQDenseNear);
+        ncgk::assemble_near(s, p->elements(), b, p->adm, p->near_rows, p->near_cols, p->nnear, p->A, mixed, p->err,
+                            q_counter(p, kQDenseNear, 1));
+    }
     if (p->nsrc) ncgk::rhs_sources(s, p->elements(), b, p->src, p->nsrc, p->bcval, nr, p->rhs_src, p->err);
-    ncgk::assemble_diag(s, p->elements(), b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src,
-                        p->nsrc > 0, p->A, mixed, p->rhs);
+    {
+        KernelScope ks(p, kKidDiag, 0.0, kQDiag);
+        ncgk::assemble_diag(s, p->elements(), b, p->adm, p->bcval, nr, p->ps, p->nps, p->pw, p->npw, p->rhs_src,
+                            p->nsrc > 0, p->A, mixed, p->rhs, nullptr, q_counter(p, kQDiag, 1));
+    }
     record(p, p->ev_asm.b);
     NCG_HIP(hipGetLastError());
     p->last_evals = 0.0;
@@ -2815,7 +2924,7 @@ double take_ms(ncg_problem *p, std::initializer_list<int> ids)
     double t = 0.0;
     for (int id : ids) t += p->k_ms[id];
     for (int id = 0; id < kNumKernels; ++id) {
-        p->k_ms[id] = p->k_bytes[id] = 0.0;
+        p->k_ms[id] = p->k_bytes[id] = p->k_flops[id] = 0.0;
         p->k_calls[id] = 0;
     }
     return t;
@@ -3253,6 +3362,7 @@ int ncg_kernel_stats(ncg_problem *p, ncg_kernel_stat *out, int32_t cap, int32_t 
             out[c].launches = p->k_calls[id];
             out[c].total_ms = p->k_ms[id];
             out[c].bytes = p->k_bytes[id];
+            out[c].flops = p->k_flops[id];
         }
         ++c;
     }
@@ -3260,7 +3370,7 @@ int ncg_kernel_stats(ncg_problem *p, ncg_kernel_stat *out, int32_t cap, int32_t 
     if (reset)
         for (int id = 0; id < kNumKernels; ++id) {
             p->k_calls[id] = 0;
-            p->k_ms[id] = p->k_bytes[id] = 0.0;
+            p->k_ms[id] = p->k_bytes[id] = p->k_flops[id] = 0.0;
         }
     return 0;
 }

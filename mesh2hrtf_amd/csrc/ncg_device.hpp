@@ -318,7 +318,8 @@ __device__ __forceinline__ void bm_point(BMAcc<T> &acc, T qx, T qy, T qz, T nyx,
 
 // Singular (self) integrals of element j at its own centre, FP64
 // (NC_SingularIntegration, src/NC_EquationSystem.cpp:1408-1597).
-__device__ inline void bm_singular(const LocalElem &L, const double *nx, double hk, double k2, BMAcc<double> &acc)
+__device__ inline void bm_singular(const LocalElem &L, const double *nx, double hk, double k2, BMAcc<double> &acc,
+                                   unsigned long long *nev = nullptr)
 {
     acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
     const int nv = L.nv;
@@ -352,6 +353,7 @@ __device__ inline void bm_singular(const LocalElem &L, const double *nx, double 
                 accE += G * ((w0 * nx[0] + w1 * nx[1] + w2 * nx[2]) * wg);
             }
         }
+        if (nev) *nev += 12;   // 4 sections x 3 points
         acc.E += accE;
         // weakly singular parts: two collapsed (Duffy) sub-triangles per edge, 4x4 Gauss
         for (int sec = 0; sec < 2; ++sec) {
@@ -389,20 +391,22 @@ __device__ inline void bm_singular(const LocalElem &L, const double *nx, double 
                     acc.HT += gd * rnx;
                     acc.E += g * (k2 * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]));
                 }
+            if (nev) *nev += 16;   // 4 x 4 points
         }
     }
 }
 
 // FP64 integrals of any pair (regular, subdivided or singular) — used for the near
 // pairs, the right-hand-side source columns and the diagonal.  Returns a kErr* flag.
-__device__ inline int bm_pair(const Elements &E, int i, int j, double hk, double k2, BMAcc<double> &acc)
+__device__ inline int bm_pair(const Elements &E, int i, int j, double hk, double k2, BMAcc<double> &acc,
+                              unsigned long long *nev = nullptr)
 {
     const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
     const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
     LocalElem L;
     load_local(E, j, x, L);
     if (i == j) {
-        bm_singular(L, nx, hk, k2, acc);
+        bm_singular(L, nx, hk, k2, acc, nev);
         return 0;
     }
     acc.G = acc.H = acc.HT = acc.E = mk(0.0, 0.0);
@@ -410,6 +414,7 @@ __device__ inline int bm_pair(const Elements &E, int i, int j, double hk, double
         double q[3], ny[3], jac;
         point_on(L, s, t, q, ny, jac);
         bm_point<double>(acc, q[0], q[1], q[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * jac, hk, k2);
+        if (nev) ++*nev;
     });
 }
 

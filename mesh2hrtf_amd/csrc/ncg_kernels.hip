@@ -78,6 +78,14 @@ __global__ void k_near_fill(Elements E, const long long *offset, int *rows, int 
 // the diagonal are left to k_assemble_near / k_assemble_diag.  Writes are coalesced:
 // consecutive lanes write consecutive columns of one row.
 // ======================================================================================
+// Quadrature evaluation counts (the measurement pass of a plan only, evals != nullptr): the
+// wave's count added to *evals with one atomic.  Every lane of the wave calls it.
+__device__ __forceinline__ void wave_count(unsigned long long *evals, unsigned long long nev)
+{
+    for (int o = 32; o > 0; o >>= 1) nev += __shfl_xor(nev, o);
+    if ((threadIdx.x & 63) == 0 && nev) atomicAdd(evals, nev);
+}
+
 template <typename T, typename S>
 __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, const double *adm,
                                                        cx<S> *A, int rows_per_block, unsigned long long *evals)
@@ -139,11 +147,7 @@ __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, co
         A[(size_t)i * n + j] = cx<S>{S(a.re), S(a.im)};
     }
     }
-    // evaluation count (measurement only): one atomic per wave
-    if (evals) {
-        for (int o = 32; o > 0; o >>= 1) nev += __shfl_xor(nev, o);
-        if ((threadIdx.x & 63) == 0 && nev) atomicAdd(evals, nev);
-    }
+    if (evals) wave_count(evals, nev);
 }
 
 // ======================================================================================
@@ -151,16 +155,20 @@ __global__ void __launch_bounds__(256) k_assemble_far(Elements E, BMConst bm, co
 // ======================================================================================
 template <typename S>
 __global__ void __launch_bounds__(128) k_assemble_near(Elements E, BMConst bm, const double *adm, const int *rows,
-                                                        const int *cols, long long npairs, cx<S> *A, int *err)
+                                                        const int *cols, long long npairs, cx<S> *A, int *err,
+                                                        unsigned long long *evals)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    const int i = rows[p], j = cols[p];
-    BMAcc<double> acc;
-    const int e = bm_pair(E, i, j, bm.hk, bm.k2, acc);
-    if (e) atomicOr(err, e);
-    const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
-    A[(size_t)i * E.n + j] = cx<S>{S(a.re), S(a.im)};
+    unsigned long long nev = 0;
+    if (p < npairs) {
+        const int i = rows[p], j = cols[p];
+        BMAcc<double> acc;
+        const int e = bm_pair(E, i, j, bm.hk, bm.k2, acc, &nev);
+        if (e) atomicOr(err, e);
+        const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
+        A[(size_t)i * E.n + j] = cx<S>{S(a.re), S(a.im)};
+    }
+    if (evals) wave_count(evals, nev);
 }
 
 // ======================================================================================
@@ -209,9 +217,15 @@ __global__ void __launch_bounds__(256) k_rhs_sources(Elements E, BMConst bm, con
 __global__ void __launch_bounds__(64) k_assemble_diag(Elements E, BMConst bm, const double *adm, const cx<double> *bcval,
                                                       int nrhs, const double *ps, int nps, const double *pw, int npw,
                                                       const cx<double> *rhs_src, int has_src, void *Aout,
-                                                      int mixed, cx<double> *b, const long long *diag_index)
+                                                      int mixed, cx<double> *b, const long long *diag_index,
+                                                      unsigned long long *evals)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (evals) {   // singular rule: the same count for every element of a kind
+        unsigned long long nev = 0;
+        if (i < E.n) nev = E.nv[i] * 44ull;   // per edge 4 x 3 line points, 2 x 16 surface points
+        wave_count(evals, nev);
+    }
     if (i >= E.n) return;
     const int n = E.n;
     BMAcc<double> acc;
@@ -1075,7 +1089,8 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
                                                const cx<double> *pot, const cx<double> *vel, const double *ps, int nps,
                                                const double *pw, int npw, cx<double> *phi, cx<double> *grad, int *err,
                                                const int *pt_ev, const int *el_ptr, const int *el_idx,
-                                               const cx<double> *fphi, const cx<double> *fgrad)
+                                               const cx<double> *fphi, const cx<double> *fgrad,
+                                               unsigned long long *evals)
 {
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1089,6 +1104,7 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
         agr[r][0] = agr[r][1] = agr[r][2] = mk(0.0, 0.0);
     }
     const T hk = T(bm.hk), k2 = T(bm.k2);
+    unsigned long long nev = 0;
     // all elements (dense method) or the elements of the near clusters of the point's
     // evaluation cluster (FMM)
     const int k0 = el_ptr ? el_ptr[pt_ev[p]] : 0, k1 = el_ptr ? el_ptr[pt_ev[p] + 1] : n;
@@ -1105,6 +1121,7 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
                 double q[3], ny[3], J;
                 point_on(L, s, t, q, ny, J);
                 field_point<double>(f, q[0], q[1], q[2], ny[0], ny[1], ny[2], w * J, bm.hk, bm.k2);
+                ++nev;
             });
             if (e) atomicOr(err, e);
         } else {
@@ -1117,6 +1134,7 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
                     field_point<T>(f, T(q[0]), T(q[1]), T(q[2]), T(ny[0]), T(ny[1]), T(ny[2]),
                                    T(c_tri[off + g][2] * J), hk, k2);
                 }
+                nev += cnt;
             } else {
                 const int base = mo * (mo - 1) / 2;
                 for (int u = 0; u < mo; ++u)
@@ -1126,6 +1144,7 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
                         field_point<T>(f, T(q[0]), T(q[1]), T(q[2]), T(ny[0]), T(ny[1]), T(ny[2]),
                                        T(c_gl_w[base + u] * c_gl_w[base + v] * J), hk, k2);
                     }
+                nev += mo * mo;
             }
         }
         for (int r = 0; r < nr; ++r) {
@@ -1134,6 +1153,7 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
             for (int d = 0; d < 3; ++d) agr[r][d] += f.K3[d] * pj - f.K4[d] * vj;
         }
     }
+    if (evals) wave_count(evals, nev);
     for (int r = 0; r < nr; ++r) {
         double v[8] = {aphi[r].re, aphi[r].im, agr[r][0].re, agr[r][0].im, agr[r][1].re, agr[r][1].im,
                        agr[r][2].re, agr[r][2].im};
@@ -1478,10 +1498,11 @@ __global__ void k_adj_positions(long long npairs, const int *rows, const int *co
 template <typename S>
 __global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, const double *adm, const int *rows,
                                                       const int *cols, const long long *pos, long long npairs,
-                                                      cx<S> *vals, int *err)
+                                                      cx<S> *vals, int *err, unsigned long long *evals)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npairs || pos[q] < 0) return;
+    unsigned long long nev = 0;
+    if (q < npairs && pos[q] >= 0) {
     const int i = rows[q], j = cols[q];
     const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
     const double nx[3] = {E.nrm[i], E.nrm[E.n + i], E.nrm[2 * E.n + i]};
@@ -1493,10 +1514,13 @@ __global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, co
         double q2[3], ny[3], J;
         point_on(L, s, t, q2, ny, J);
         bm_point<double>(acc, q2[0], q2[1], q2[2], ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], w * J, bm.hk, bm.k2);
+        ++nev;
     });
     if (e) atomicOr(err, e);
     const cx<double> a = bm_entry<double>(acc, E.kind[j], mk(adm[2 * j], adm[2 * j + 1]), bm.tau, bm.beta_im);
     vals[pos[q]] = cx<S>{S(a.re), S(a.im)};
+    }
+    if (evals) wave_count(evals, nev);
 }
 
 // Regular near-field entries (NC_ComputeEntriesNforFMM, src/NC_EquationSystem.cpp:1243-1405,
@@ -1509,14 +1533,16 @@ __global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, co
 template <typename S, typename T>
 __global__ void __launch_bounds__(256) k_fmm_near(Elements E, BMConst bm, const double *adm, const int *ch_cl,
                                                   const int *ch_r0, const int *ch_m, const int *cl_ptr,
-                                                  const int *cl_elem, const long long *ptr, const int *col, cx<S> *vals)
+                                                  const int *cl_elem, const long long *ptr, const int *col, cx<S> *vals,
+                                                  unsigned long long *evals)
 {
     const int n = E.n;
     const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
     const long long cb = ptr[cl_elem[cl_ptr[c]]];
     const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
     const int u = blockIdx.y * 256 + threadIdx.x;
-    if (u >= w) return;
+    unsigned long long nev = 0;
+    if (u < w) {
     const int j = col[cb + u];
     const int nv = E.nv[j];
     const int kind = E.kind[j];
@@ -1550,6 +1576,7 @@ __global__ void __launch_bounds__(256) k_fmm_near(Elements E, BMConst bm, const 
                 bm_point<T>(acc, px + s * e1x + t * e2x, py + s * e1y + t * e2y, pz + s * e1z + t * e2z, nyx, nyy, nyz,
                             nxx, nxy, nxz, wg * jacj, hk, k2);
             }
+            nev += cnt;
         } else {
             LocalElem L;
             L.nv = 4;
@@ -1563,10 +1590,13 @@ __global__ void __launch_bounds__(256) k_fmm_near(Elements E, BMConst bm, const 
                     bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
                                 T(c_gl_w[base + s2] * c_gl_w[base + t2] * J), hk, k2);
                 }
+            nev += mg * mg;
         }
         const cx<T> e = bm_entry<T>(acc, kind, admj, tau, bim);
         vals[ptr[i] + u] = cx<S>{S(e.re), S(e.im)};
     }
+    }
+    if (evals) wave_count(evals, nev);
 }
 
 // near-field CSR of the finest level: row i holds the elements of the near clusters of
@@ -2745,13 +2775,15 @@ void assemble_far(hipStream_t s, const DevElements &D, const BMParams &b, const 
 }
 
 void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
-                   const int *cols, long long npairs, void *A, int mixed, int *err)
+                   const int *cols, long long npairs, void *A, int mixed, int *err, unsigned long long *evals)
 {
     if (npairs == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for(npairs, 128);
-    if (mixed) k_assemble_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<float> *)A, err);
-    else k_assemble_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<double> *)A, err);
+    if (mixed)
+        k_assemble_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<float> *)A, err, evals);
+    else
+        k_assemble_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<double> *)A, err, evals);
 }
 
 void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
@@ -2767,12 +2799,12 @@ void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const i
 
 void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const void *bcval,
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
-                   void *A, int mixed, void *rhs, const long long *diag_index)
+                   void *A, int mixed, void *rhs, const long long *diag_index, unsigned long long *evals)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     k_assemble_diag<<<blocks_for(D.n, 64), 64, 0, s>>>(E, make_bm(b), adm, (const cx<double> *)bcval, nrhs, ps, nps, pw,
                                                        npw, (const cx<double> *)rhs_src, has_src, A, mixed,
-                                                       (cx<double> *)rhs, diag_index);
+                                                       (cx<double> *)rhs, diag_index, evals);
 }
 
 void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs, const void *x, const double *d, void *y)
@@ -3193,7 +3225,8 @@ void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double
 }
 void field(hipStream_t s, const DevElements &D, const BMParams &b, const double *pts, int m, int nrhs, const void *pot,
            const void *vel, const double *ps, int nps, const double *pw, int npw, void *phi, void *grad, int fp32,
-           int *err, const int *pt_ev, const int *el_ptr, const int *el_idx, const void *fphi, const void *fgrad)
+           int *err, const int *pt_ev, const int *el_ptr, const int *el_idx, const void *fphi, const void *fgrad,
+           unsigned long long *evals)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     for (int r0 = 0; r0 < nrhs; r0 += kFieldChunk) {
@@ -3201,12 +3234,12 @@ void field(hipStream_t s, const DevElements &D, const BMParams &b, const double 
             k_field<float><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                             (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
                                                             (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
-                                                            (const cx<double> *)fphi, (const cx<double> *)fgrad);
+                                                            (const cx<double> *)fphi, (const cx<double> *)fgrad, evals);
         else
             k_field<double><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                              (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
                                                              (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
-                                                             (const cx<double> *)fphi, (const cx<double> *)fgrad);
+                                                             (const cx<double> *)fphi, (const cx<double> *)fgrad, evals);
     }
 }
 
@@ -3246,17 +3279,17 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
 }
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
-              int mixed)
+              int mixed, unsigned long long *evals)
 {
     if (rc.n == 0 || wmax == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const dim3 g(rc.n, (wmax + 255) / 256);
     if (mixed)
         k_fmm_near<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
-                                                   (cx<float> *)vals);
+                                                   (cx<float> *)vals, evals);
     else
         k_fmm_near<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
-                                                     (cx<double> *)vals);
+                                                     (cx<double> *)vals, evals);
 }
 void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
                    const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
@@ -3267,13 +3300,18 @@ void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *
                                                             nn_idx, nn_off, near_ptr, pos);
 }
 void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
-                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err)
+                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err,
+                  unsigned long long *evals)
 {
     if (npairs == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for(npairs, 128);
-    if (mixed) k_fmm_near_adj<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<float> *)vals, err);
-    else k_fmm_near_adj<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<double> *)vals, err);
+    if (mixed)
+        k_fmm_near_adj<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<float> *)vals, err,
+                                                evals);
+    else
+        k_fmm_near_adj<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<double> *)vals, err,
+                                                 evals);
 }
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag)

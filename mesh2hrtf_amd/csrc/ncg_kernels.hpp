@@ -34,16 +34,19 @@ void element_geometry(hipStream_t s, int n, const double *X, const int8_t *nv, d
 void near_count(hipStream_t s, const DevElements &D, int *count);
 void near_fill(hipStream_t s, const DevElements &D, const long long *offset, int *rows, int *cols);
 
+// evals (quadrature launchers): nullptr, or a counter that the launch adds its number of
+// Green's-function evaluations to (the measurement pass of a plan)
 void assemble_far(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, void *A, int mixed,
                   int rows_per_block, unsigned long long *evals);
 void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
-                   const int *cols, long long npairs, void *A, int mixed, int *err);
+                   const int *cols, long long npairs, void *A, int mixed, int *err, unsigned long long *evals = nullptr);
 void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
                  int nrhs, void *rhs_src, int *err, const int *elem_cluster = nullptr,
                  const unsigned char *cluster_near = nullptr, int nclusters = 0);
 void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const void *bcval,
                    int nrhs, const double *ps, int nps, const double *pw, int npw, const void *rhs_src, int has_src,
-                   void *A, int mixed, void *rhs, const long long *diag_index = nullptr);
+                   void *A, int mixed, void *rhs, const long long *diag_index = nullptr,
+                   unsigned long long *evals = nullptr);
 
 // Dense operator applied to up to 16 load cases per pass on the matrix cores: x^ (the load
 // cases as a real 2n x 32 operand), the product (column splits of partial sums) and the sum
@@ -117,7 +120,7 @@ void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double
 void field(hipStream_t s, const DevElements &D, const BMParams &b, const double *pts, int m, int nrhs, const void *pot,
            const void *vel, const double *ps, int nps, const double *pw, int npw, void *phi, void *grad, int fp32,
            int *err, const int *pt_ev = nullptr, const int *el_ptr = nullptr, const int *el_idx = nullptr,
-           const void *fphi = nullptr, const void *fgrad = nullptr);
+           const void *fphi = nullptr, const void *fgrad = nullptr, unsigned long long *evals = nullptr);
 
 // fast multipole operators (layouts in ncg_kernels.hip)
 // coef: scratch of nfan * (3 + 2 L) doubles (series coefficients per interaction)
@@ -135,7 +138,8 @@ void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *
                    const int *elem_pos, const int *cl_ptr, const int *nn_ptr, const int *nn_idx, const int *nn_off,
                    const long long *near_ptr, long long *pos);
 void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
-                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err);
+                  const int *cols, const long long *pos, long long npairs, void *vals, int mixed, int *err,
+                  unsigned long long *evals = nullptr);
 // block-Jacobi preconditioner: blocks of at most 64 consecutive elements of a finest
 // cluster, gathered from the near-field CSR and inverted in place (complex FP64)
 void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *elem_pos, const int *blk_cl,
@@ -187,7 +191,7 @@ void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks
 // fmm_near_adj and the diagonal to assemble_diag.  wmax = longest row.
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
-              int mixed);
+              int mixed, unsigned long long *evals = nullptr);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
                   void *tv, int fp32);

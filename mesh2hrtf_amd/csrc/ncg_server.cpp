@@ -457,10 +457,10 @@ void run_lane(ncg_server *S)
         }
         std::vector<std::vector<ncg_kernel_stat>> kst(grp.size());
         for (size_t i = 0; i < grp.size(); ++i) {
-            ncg_kernel_stat ks[32];
+            ncg_kernel_stat ks[64];
             int32_t nks = 0;
-            if (S->cfg.kernel_timing && grp[i].r->p && ncg_kernel_stats(grp[i].r->p, ks, 32, &nks, 1) == 0)
-                kst[i].assign(ks, ks + std::min(nks, 32));
+            if (S->cfg.kernel_timing && grp[i].r->p && ncg_kernel_stats(grp[i].r->p, ks, 64, &nks, 1) == 0)
+                kst[i].assign(ks, ks + std::min(nks, 64));
         }
 
         const clk::time_point t_out = clk::now();
@@ -478,6 +478,7 @@ void run_lane(ncg_server *S)
                 a.launches += k.launches;
                 a.total_ms += k.total_ms;
                 a.bytes += k.bytes;
+                a.flops += k.flops;
             }
             S->running--;
             t.r->busy = false;

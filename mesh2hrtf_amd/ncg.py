@@ -105,11 +105,12 @@ def _tuning_dict(t):
 
 class _KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
-                ("bytes", ctypes.c_double)]
+                ("bytes", ctypes.c_double), ("flops", ctypes.c_double)]
 
 
 def _kernel_dict(arr, count):
-    return {arr[i].name.decode(): dict(calls=arr[i].launches, total_ms=arr[i].total_ms, bytes=arr[i].bytes)
+    return {arr[i].name.decode(): dict(calls=arr[i].launches, total_ms=arr[i].total_ms, bytes=arr[i].bytes,
+                                       flops=arr[i].flops)
             for i in range(min(count, len(arr)))}
 
 
@@ -288,9 +289,9 @@ class Problem:
         _check(self.lib.ncg_set_kernel_timing(self._h, int(on)))
 
     def kernel_stats(self, reset=False):
-        arr = (_KernelStat * 32)()
+        arr = (_KernelStat * 64)()
         cnt = ctypes.c_int32()
-        _check(self.lib.ncg_kernel_stats(self._h, arr, 32, ctypes.byref(cnt), int(reset)))
+        _check(self.lib.ncg_kernel_stats(self._h, arr, 64, ctypes.byref(cnt), int(reset)))
         return _kernel_dict(arr, cnt.value)
 
     def tuning(self):
@@ -306,7 +307,7 @@ class Problem:
         t = (ctypes.c_double * 8)()
         _check(self.lib.ncg_timings(self._h, t, int(reset)))
         keys = ("ms_assemble", "ms_solve", "ms_field", "far_kernel_ms", "matvec_kernel_ms", "matvec_calls",
-                "far_evaluations", "operator_bytes")
+                "quadrature_evaluations", "operator_bytes")
         return dict(zip(keys, list(t)))
 
 
@@ -425,9 +426,9 @@ class Server:
         return _status_dict(st)
 
     def kernel_stats(self, reset=False):
-        arr = (_KernelStat * 32)()
+        arr = (_KernelStat * 64)()
         cnt = ctypes.c_int32()
-        _check(self.lib.ncg_server_kernel_stats(self._h, arr, 32, ctypes.byref(cnt), int(reset)))
+        _check(self.lib.ncg_server_kernel_stats(self._h, arr, 64, ctypes.byref(cnt), int(reset)))
         return _kernel_dict(arr, cnt.value)
 
     def stats(self):
