@@ -1384,10 +1384,10 @@ int ilu_setup(ncg_problem *p, int degree)
             Umapd[(size_t)u * B + q] = Umap[Uoff[u] + q];
         }
     }
-    // Batches: consecutive levels of at most 64 rows in all, as long as every entry of a
-    // row whose column lies in the batch is among its first kIluB (those come from the
-    // lanes of the batch); a level wider than 64 rows is a batch of its own.  In-batch
-    // columns of the padded copy become -1 - (position - batch start).
+    // Batches: consecutive levels of at most kIluRows rows in all, as long as every entry of
+    // a row whose column lies in the batch is among its first kIluB (those come from the
+    // threads of the batch); a level wider than kIluRows rows is a batch of its own.
+    // In-batch columns of the padded copy become -1 - (position - batch start).
     auto batches = [&](const std::vector<int> &lp, int nlev, const std::vector<int> &rows, const std::vector<int> &lv,
                        const std::vector<int> &off, const std::vector<int> &cl, std::vector<int> &cdp,
                        std::vector<int> &bptr, std::vector<int> &bm, std::vector<unsigned char> &sub) {
@@ -1402,10 +1402,11 @@ int ilu_setup(ncg_problem *p, int degree)
                     if (lv[cl[k]] >= l0) return false;
             return true;
         };
+        constexpr int W = ncgk::kIluRows;
         for (int l0 = 0; l0 < nlev;) {
             int l1 = l0 + 1, tot = lp[l0 + 1] - lp[l0];
-            if (tot <= 64)
-                while (l1 < nlev && l1 - l0 < 255 && tot + (lp[l1 + 1] - lp[l1]) <= 64 && fits(l1, l0)) {
+            if (tot <= W)
+                while (l1 < nlev && l1 - l0 < 255 && tot + (lp[l1 + 1] - lp[l1]) <= W && fits(l1, l0)) {
                     tot += lp[l1 + 1] - lp[l1];
                     ++l1;
                 }
@@ -1414,7 +1415,7 @@ int ilu_setup(ncg_problem *p, int degree)
                 sub[u] = (unsigned char)(lv[rows[u]] - l0);
                 for (int q = 0; q < B && off[u] + q < off[u + 1]; ++q) {
                     const int j = cl[off[u] + q];
-                    if (tot <= 64 && lv[j] >= l0) cdp[(size_t)u * B + q] = -1 - (pos[j] - b0);
+                    if (tot <= W && lv[j] >= l0) cdp[(size_t)u * B + q] = -1 - (pos[j] - b0);
                 }
             }
             bptr.push_back(lp[l1]);
@@ -1443,7 +1444,37 @@ int ilu_setup(ncg_problem *p, int degree)
         (rc = upload_ilu(I, s, &I.Udmap, Udmap)) || (rc = I.alloc(&I.Lv, 2 * (size_t)I.nL)) ||
         (rc = I.alloc(&I.Uv, 2 * (size_t)I.nU)) || (rc = I.alloc(&I.Ud, 2 * (size_t)n)))
         return rc;
-    ncgk::ilu_factor(s, nL, I.levL_ptr, I.levL_rows, I.rp, I.col, I.diag, I.val, p->err);
+    // the factorization's update map: per row the pivot positions of its lower entries,
+    // then per lower entry (column k) and entry (column j) the position of u_kj in row k's
+    // upper part (the first upper column >= j, when it is j), or -1
+    std::vector<long long> uoff(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const long long R = rp[i + 1] - rp[i], Lw = diag[i] - rp[i];
+        uoff[i + 1] = uoff[i] + (R <= 64 ? Lw + Lw * R : 0);
+    }
+    std::vector<int> upd((size_t)std::max<long long>(uoff[n], 1), -1);
+    for (int i = 0; i < n; ++i) {
+        const int r0 = rp[i], r1 = rp[i + 1], R = r1 - r0, Lw = diag[i] - r0;
+        if (R > 64) continue;
+        int *um = upd.data() + uoff[i];
+        for (int a = 0; a < Lw; ++a) {
+            const int k = col[r0 + a];
+            um[a] = diag[k];
+            int q = diag[k] + 1;
+            const int qe = rp[k + 1];
+            for (int e = a + 1; e < R; ++e) {
+                const int j = col[r0 + e];
+                while (q < qe && col[q] < j) ++q;
+                um[Lw + (size_t)a * R + e] = q < qe && col[q] == j ? q : -1;
+            }
+        }
+    }
+    long long *d_uoff = nullptr;
+    int *d_upd = nullptr;
+    if ((rc = I.alloc(&d_uoff, (size_t)n + 1)) || (rc = I.alloc(&d_upd, upd.size()))) return rc;
+    NCG_HIP(hipMemcpyAsync(d_uoff, uoff.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(d_upd, upd.data(), sizeof(int) * upd.size(), hipMemcpyHostToDevice, s));
+    ncgk::ilu_factor(s, nL, I.levL_ptr, I.levL_rows, I.rp, I.col, I.diag, d_uoff, d_upd, I.val, p->err);
     ncgk::ilu_pack(s, I.nL, I.Lmap, I.val, I.Lv);
     ncgk::ilu_pack(s, I.nU, I.Umap, I.val, I.Uv);
     ncgk::ilu_pack(s, n, I.Udmap, I.val, I.Ud);

@@ -815,45 +815,57 @@ __global__ void k_ilu_fill_csr(int n, const long long *ptr, const int *ncol, con
 }
 
 // IKJ factorization in place, one work-group walking the levels of the L dependency graph
+// ILU factorization level by level, a wave per row of the level, a lane per entry of
+// the row held in registers while the row's lower entries k (in order) eliminate:
+// l_ik = a_ik / u_kk, then a_ij -= l_ik u_kj for the later entries j of the row that row
+// k's upper part holds.  From uoff[i] (computed on the host from the pattern) upd holds
+// the positions of the pivots u_kk of row i's L lower entries, then for lower entry a and
+// entry e (R entries) the position of u_kj or -1 at L + a R + e; a row's pivots and u_kj
+// of a chunk of kFacChunk lower entries thus come in one round of loads, all from rows of
+// earlier levels.  Rows longer than a wave walk row k.
+constexpr int kFacChunk = 8;
 __global__ void __launch_bounds__(1024) k_ilu_factor(int nlev, const int *lev_ptr, const int *lev_rows, const int *rp,
-                                                     const int *col, const int *diag, cx<double> *val, int *err)
+                                                     const int *col, const int *diag, const long long *uoff,
+                                                     const int *upd, cx<double> *val, int *err)
 {
-    // a wave per row of the level, a lane per entry of the row, held in registers while
-    // the row's lower entries k (in order) eliminate: l_ik = a_ik / u_kk, then
-    // a_ij -= l_ik u_kj for the later entries j of the row found in row k's upper part
     constexpr int NW = 1024 / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int l = 0; l < nlev; ++l) {
         for (int t = lev_ptr[l] + wv; t < lev_ptr[l + 1]; t += NW) {
             const int i = lev_rows[t], r0 = rp[i], r1 = rp[i + 1], dg = diag[i];
-            if (r1 - r0 <= 64) {
-                const int p2 = r0 + lane;
-                const bool on = p2 < r1;
-                cx<double> v = on ? val[p2] : mk(0.0, 0.0);
-                const int j = on ? col[p2] : -1;
-                for (int p = r0; p < dg; ++p) {
-                    const int k = col[p];
-                    const cx<double> piv = val[diag[k]];
-                    if (piv.re == 0.0 && piv.im == 0.0) {
-                        if (lane == 0) atomicOr(err, 8);
-                        continue;
+            const int R = r1 - r0;
+            if (R <= 64) {
+                const bool on = lane < R;
+                cx<double> v = on ? val[r0 + lane] : mk(0.0, 0.0);
+                const int *um = upd + uoff[i];
+                const int Lw = dg - r0;
+                for (int a0 = 0; a0 < Lw; a0 += kFacChunk) {
+                    cx<double> piv[kFacChunk], u[kFacChunk];
+                    bool has[kFacChunk];
+#pragma unroll
+                    for (int c = 0; c < kFacChunk; ++c) {
+                        const int a = a0 + c;
+                        const bool in = a < Lw;
+                        piv[c] = in ? val[um[a]] : mk(1.0, 0.0);
+                        const int q = in && on ? um[Lw + (size_t)a * R + lane] : -1;
+                        has[c] = q >= 0;
+                        u[c] = has[c] ? val[q] : mk(0.0, 0.0);
                     }
-                    const cx<double> lik = cdiv(mk(__shfl(v.re, p - r0), __shfl(v.im, p - r0)), piv);
-                    if (lane == p - r0) v = lik;
-                    if (on && p2 > p) {
-                        const int qe = rp[k + 1];
-                        for (int q = diag[k] + 1; q < qe; ++q) {
-                            const int cq = col[q];
-                            if (cq >= j) {
-                                if (cq == j) v -= lik * val[q];
-                                break;
-                            }
+#pragma unroll
+                    for (int c = 0; c < kFacChunk; ++c) {
+                        const int a = a0 + c;
+                        if (r0 + a >= dg) break;
+                        if (piv[c].re == 0.0 && piv[c].im == 0.0) {
+                            if (lane == 0) atomicOr(err, 8);
+                            continue;
                         }
+                        const cx<double> lik = cdiv(mk(__shfl(v.re, a), __shfl(v.im, a)), piv[c]);
+                        if (lane == a) v = lik;
+                        if (lane > a && has[c]) v -= lik * u[c];
                     }
                 }
-                if (on) val[p2] = v;
+                if (on) val[r0 + lane] = v;
             } else if (lane == 0) {
-                // rows longer than a wave: the entries one after the other
                 for (int p = r0; p < dg; ++p) {
                     const int k = col[p];
                     const cx<double> piv = val[diag[k]];
@@ -925,14 +937,14 @@ struct IluItem {
     cx<double> xv, dv;
 };
 
-// lane's row i of the batch [b, b + w) (w <= 64, i already loaded); every load here
-// depends only on the position and i.  xr (forward) or d (backward) may be null.
-__device__ __forceinline__ void ilu_item_load(IluItem &it, int lane, int b, int w, int i, const int *off,
+// thread t's row i of the batch [b, b + w) (w <= kIluRows, i already loaded); every load
+// here depends only on the position and i.  xr (forward) or d (backward) may be null.
+__device__ __forceinline__ void ilu_item_load(IluItem &it, int t, int b, int w, int i, const int *off,
                                               const unsigned char *sub, const int *cd, const cx<double> *vd,
                                               const cx<double> *xr, const cx<double> *d)
 {
-    it.on = lane < w;
-    const int u = b + (it.on ? lane : 0);
+    it.on = t < w;
+    const int u = b + (it.on ? t : 0);
     it.i = i;
     it.k0 = off[u];
     it.k1 = off[u + 1];
@@ -946,39 +958,40 @@ __device__ __forceinline__ void ilu_item_load(IluItem &it, int lane, int b, int 
     it.dv = d ? d[u] : mk(1.0, 0.0);
 }
 
-// One triangular sweep of one load case by one wave, a batch of consecutive levels (at
-// most 64 rows) per step.  A batch's solution values from earlier batches are gathered
-// once; its rows then run level by level (sub-steps), taking the values of the batch's
-// earlier levels from the slots their lanes left in LDS, so a batch costs one round of
-// memory gathers however many levels it holds.  Every row sums its entries in
-// the CSR order.  A wave's memory operations complete in program order and its lanes
-// share one vector L1, so rows stored by one batch are seen by the next with no barrier (a
-// wavefront-scope fence keeps the compiler from moving loads above the stores).  The
-// next batch's rows and coefficients are loaded while the current one is computed.
-// Levels wider than the wave form batches of their own and take a plain loop.
+// One triangular sweep of one load case by one work-group of kIluRows threads, a batch of
+// consecutive levels (at most kIluRows rows, thread t taking position t of the batch) per
+// step.  A batch's solution values from earlier batches are gathered once; its rows then
+// run level by level (sub-steps, a barrier after each), taking the values of the batch's
+// earlier levels from the slots their threads left in LDS, so a batch costs one round of
+// memory gathers however many levels it holds.  Every row sums its entries in the CSR
+// order.  The work-group runs on one CU: after a batch its stores are made visible to the
+// work-group (release fence, barrier) before the next batch gathers.  The next batch's
+// rows and coefficients are loaded while the current one is computed.  Levels wider than
+// the work-group form batches of their own and take a plain loop.
 template <bool UPPER>
 __device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<double> *v, const cx<double> *vd,
                                           const cx<double> *d, const cx<double> *xr, cx<double> *yr)
 {
-    __shared__ cx<double> ys[64];   // the batch's rows solved so far, by lane
-    const int lane = threadIdx.x, nb = S.nbat;
+    constexpr int W = ncgk::kIluRows;
+    __shared__ cx<double> ys[W];   // the batch's rows solved so far, by thread
+    const int t = threadIdx.x, nb = S.nbat;
     const int *bp = S.bat_ptr, *row = S.row, *off = S.off, *c = S.c, *cd = S.cd;
     auto width = [&](int b) { return bp[b + 1] - bp[b]; };
-    auto row_of = [&](int b) { return b < nb && lane < width(b) ? row[bp[b] + lane] : 0; };
+    auto row_of = [&](int b) { return b < nb && t < width(b) ? row[bp[b] + t] : 0; };
     IluItem cur, nxt;
     cur.on = false;
     int rnext = row_of(1);
-    if (nb > 0 && width(0) <= 64)
-        ilu_item_load(cur, lane, bp[0], width(0), row_of(0), off, S.sub, cd, vd, UPPER ? nullptr : xr,
+    if (nb > 0 && width(0) <= W)
+        ilu_item_load(cur, t, bp[0], width(0), row_of(0), off, S.sub, cd, vd, UPPER ? nullptr : xr,
                       UPPER ? d : nullptr);
     for (int b = 0; b < nb; ++b) {
         const int b0 = bp[b], w = bp[b + 1] - b0;
         nxt.on = false;
         const int rafter = row_of(b + 2);
-        if (b + 1 < nb && width(b + 1) <= 64)
-            ilu_item_load(nxt, lane, bp[b + 1], width(b + 1), rnext, off, S.sub, cd, vd, UPPER ? nullptr : xr,
+        if (b + 1 < nb && width(b + 1) <= W)
+            ilu_item_load(nxt, t, bp[b + 1], width(b + 1), rnext, off, S.sub, cd, vd, UPPER ? nullptr : xr,
                           UPPER ? d : nullptr);
-        if (w <= 64) {
+        if (w <= W) {
             // values from earlier batches (in-batch slots read a dummy, replaced below)
             cx<double> yy[kIluB];
 #pragma unroll
@@ -987,41 +1000,51 @@ __device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<doub
             const int m = S.bat_m[b];
             for (int st = 0; st < m; ++st) {
                 const bool mine = cur.on && cur.ss == st;
-                // values of the batch's earlier levels from the lanes' slots in LDS, for the
-                // entries some row of this level takes from the batch
-                if (st > 0)
-#pragma unroll
-                    for (int q = 0; q < kIluB; ++q)
-                        if (__any(mine && cur.cc[q] < 0) && mine && cur.cc[q] < 0) yy[q] = ys[-1 - cur.cc[q]];
                 if (mine) {
+                    // values of the batch's earlier levels from the threads' slots in LDS
+                    if (st > 0)
+#pragma unroll
+                        for (int q = 0; q < kIluB; ++q)
+                            if (cur.cc[q] < 0) yy[q] = ys[-1 - cur.cc[q]];
                     cx<double> acc = s0;
 #pragma unroll
                     for (int q = 0; q < kIluB; ++q)
                         if (cur.k0 + q < cur.k1) acc -= cur.vv[q] * yy[q];
                     if (cur.k1 - cur.k0 > kIluB) acc = ilu_row(acc, cur.k0 + kIluB, cur.k1, c, v, yr);
                     if (UPPER) acc = cdiv(acc, cur.dv);
-                    ys[lane] = acc;
+                    ys[t] = acc;
                     yr[cur.i] = acc;
+                }
+                if (st + 1 < m) {
+                    // this level's LDS slots before the next level reads them (LDS only: the
+                    // global stores complete in the background until the batch ends)
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                    __builtin_amdgcn_s_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 }
             }
         } else {
-            for (int t = lane; t < w; t += 64) {
-                const int u = b0 + t, i = row[u];
+            for (int q = t; q < w; q += W) {
+                const int u = b0 + q, i = row[u];
                 if (UPPER) yr[i] = cdiv(ilu_row(yr[i], off[u], off[u + 1], c, v, yr), d[u]);
                 else yr[i] = ilu_row(xr[i], off[u], off[u + 1], c, v, yr);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        // the batch's stores before the next batch's gathers; its LDS slots before reuse
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         cur = nxt;
         rnext = rafter;
     }
 }
 
-// y = (L U)^-1 x: one wave per load case (block r of 64 threads), forward over the L
-// levels, backward over the U levels.  The sums run in the CSR order of each row, as the
-// factorization stored them.  x and y may be the same vector (a row's right-hand side is
-// read before the row is written).
-__global__ void __launch_bounds__(64) k_ilu_solve(int n, ncgk::IluSweep L, ncgk::IluSweep U, const cx<double> *x, cx<double> *y)
+// y = (L U)^-1 x: one work-group of kIluRows threads per load case (block r), forward
+// over the L levels, backward over the U levels.  The sums run in the CSR order of each
+// row, as the factorization stored them.  x and y may be the same vector (a row's
+// right-hand side is read before the row is written).
+__global__ void __launch_bounds__(ncgk::kIluRows) k_ilu_solve(int n, ncgk::IluSweep L, ncgk::IluSweep U,
+                                                             const cx<double> *x, cx<double> *y)
 {
     const size_t r = blockIdx.x;
     ilu_sweep<false>(L, (const cx<double> *)L.v, (const cx<double> *)L.vd, nullptr, x + r * n, y + r * n);
@@ -3191,9 +3214,9 @@ void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const i
                                                           (cx<double> *)val);
 }
 void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
-                const int *diag, void *val, int *err)
+                const int *diag, const long long *uoff, const int *upd, void *val, int *err)
 {
-    k_ilu_factor<<<1, 1024, 0, s>>>(nlev, lev_ptr, lev_rows, rp, col, diag, (cx<double> *)val, err);
+    k_ilu_factor<<<1, 1024, 0, s>>>(nlev, lev_ptr, lev_rows, rp, col, diag, uoff, upd, (cx<double> *)val, err);
 }
 void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out)
 {
@@ -3202,7 +3225,7 @@ void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out
 }
 void ilu_solve(hipStream_t s, int n, int nrhs, const IluSweep &L, const IluSweep &U, const void *x, void *y)
 {
-    k_ilu_solve<<<nrhs, 64, 0, s>>>(n, L, U, (const cx<double> *)x, (cx<double> *)y);
+    k_ilu_solve<<<nrhs, ncgk::kIluRows, 0, s>>>(n, L, U, (const cx<double> *)x, (cx<double> *)y);
 }
 void lu_factor(hipStream_t s, void *A, int n)
 {

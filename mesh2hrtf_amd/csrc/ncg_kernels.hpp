@@ -93,14 +93,19 @@ void ilu_count(hipStream_t s, const void *A, int n, const long long *ptr, const 
                int *cnt);
 void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
               const int *rp, int *col, void *val);
+// upd from uoff[i]: the pivot positions diag[k] of row i's L lower entries (k their
+// columns), then for lower entry a and entry e (R entries) the position of u_kj (j = col of
+// entry e) in row k's upper part or -1, at L + a R + e
 void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
-                const int *diag, void *val, int *err);
+                const int *diag, const long long *uoff, const int *upd, void *val, int *err);
 // one level-scheduled triangular sweep of the incomplete LU: position u of the schedule is
 // row row[u] with entries c/v[off[u] .. off[u+1]) (CSR order), the first kIluB of them
 // also padded in cd/vd[u * kIluB + q]; d = U's diagonal per position (backward sweep)
 constexpr int kIluB = 8;
+// rows of a batch of the triangular sweeps = threads of the sweep's work-group
+constexpr int kIluRows = 256;
 // One triangular sweep's schedule: positions u = 0..n-1 (row row[u]) in batches of
-// consecutive levels, bat_ptr[b]..bat_ptr[b+1] (at most 64 positions, or one wider level),
+// consecutive levels, bat_ptr[b]..bat_ptr[b+1] (at most kIluRows positions, or one wider level),
 // bat_m[b] levels per batch and sub[u] the level of u inside its batch.  The first kIluB
 // entries of every position are also in cd/vd[u][kIluB], a column of the same batch
 // written as -1 - (its position - bat_ptr[b]).
