@@ -299,7 +299,7 @@ struct DevIlu {
     // batches of consecutive levels for the triangular solves
     int nbatL = 0, nbatU = 0;
     int *batL_ptr = nullptr, *batL_m = nullptr, *batU_ptr = nullptr, *batU_m = nullptr;
-    unsigned char *subL = nullptr, *subU = nullptr;
+    unsigned char *subL = nullptr, *subU = nullptr, *flagL = nullptr, *flagU = nullptr;
     double *val = nullptr;
     // level-ordered factors for the triangular solves: offsets, columns, CSR positions
     int *Loff = nullptr, *Lc = nullptr, *Lmap = nullptr, *Uoff = nullptr, *Uc = nullptr, *Umap = nullptr,
@@ -1386,11 +1386,15 @@ int ilu_setup(ncg_problem *p, int degree)
     }
     // Batches: consecutive levels of at most kIluRows rows in all, as long as every entry of
     // a row whose column lies in the batch is among its first kIluB (those come from the
-    // threads of the batch); a level wider than kIluRows rows is a batch of its own.
-    // In-batch columns of the padded copy become -1 - (position - batch start).
+    // ring); a level wider than kIluRows rows is a batch of its own.  Then where each
+    // padded entry is read: from the ring while it still holds the value (position
+    // p >= batch end - kIluRing), else from memory (flag bit 1).  A memory read, and every
+    // entry past the kIluB-th or of a wide batch, needs the stores of p's batch made visible
+    // (flag bit 0) at the end of a batch between p's batch and the batch before the reader.
     auto batches = [&](const std::vector<int> &lp, int nlev, const std::vector<int> &rows, const std::vector<int> &lv,
                        const std::vector<int> &off, const std::vector<int> &cl, std::vector<int> &cdp,
-                       std::vector<int> &bptr, std::vector<int> &bm, std::vector<unsigned char> &sub) {
+                       std::vector<int> &bptr, std::vector<int> &bm, std::vector<unsigned char> &sub,
+                       std::vector<unsigned char> &flag) {
         std::vector<int> pos(n);
         for (int u = 0; u < n; ++u) pos[rows[u]] = u;
         bptr.assign(1, 0);
@@ -1402,7 +1406,7 @@ int ilu_setup(ncg_problem *p, int degree)
                     if (lv[cl[k]] >= l0) return false;
             return true;
         };
-        constexpr int W = ncgk::kIluRows;
+        constexpr int W = ncgk::kIluRows, RING = ncgk::kIluRing;
         for (int l0 = 0; l0 < nlev;) {
             int l1 = l0 + 1, tot = lp[l0 + 1] - lp[l0];
             if (tot <= W)
@@ -1410,23 +1414,41 @@ int ilu_setup(ncg_problem *p, int degree)
                     tot += lp[l1 + 1] - lp[l1];
                     ++l1;
                 }
-            const int b0 = lp[l0];
-            for (int u = b0; u < lp[l1]; ++u) {
-                sub[u] = (unsigned char)(lv[rows[u]] - l0);
-                for (int q = 0; q < B && off[u] + q < off[u + 1]; ++q) {
-                    const int j = cl[off[u] + q];
-                    if (tot <= W && lv[j] >= l0) cdp[(size_t)u * B + q] = -1 - (pos[j] - b0);
-                }
-            }
+            for (int u = lp[l0]; u < lp[l1]; ++u) sub[u] = (unsigned char)(lv[rows[u]] - l0);
             bptr.push_back(lp[l1]);
             bm.push_back(l1 - l0);
             l0 = l1;
         }
+        const int nb = (int)bm.size();
+        std::vector<int> bat_of(n);
+        for (int b = 0; b < nb; ++b)
+            for (int u = bptr[b]; u < bptr[b + 1]; ++u) bat_of[u] = b;
+        flag.assign(nb, 0);
+        auto visible = [&](int lo, int hi) {
+            for (int f = hi; f >= lo && f >= 0; --f)
+                if (flag[f] & 1) return true;
+            return false;
+        };
+        for (int b = 0; b < nb; ++b) {
+            const int b0 = bptr[b], w = bptr[b + 1] - b0;
+            const bool narrow = w <= W;
+            int needB = -1;
+            for (int u = b0; u < b0 + w; ++u)
+                for (int k = off[u]; k < off[u + 1]; ++k) {
+                    const int q = k - off[u], j = cl[k], pj = pos[j];
+                    if (narrow && q < B && pj >= b0 + w - RING) cdp[(size_t)u * B + q] = -1 - (pj % RING);
+                    else {
+                        needB = std::max(needB, bat_of[pj]);
+                        if (narrow && q < B) flag[b] |= 2;
+                    }
+                }
+            if (needB >= 0 && !visible(needB, b - 1)) flag[b - 1] |= 1;
+        }
     };
     std::vector<int> bpL, bmL, bpU, bmU;
-    std::vector<unsigned char> sbL, sbU;
-    batches(lpL, nL, rowsL, lvL, Loff, Lc, Lcd, bpL, bmL, sbL);
-    batches(lpU, nU, rowsU, lvU, Uoff, Uc, Ucd, bpU, bmU, sbU);
+    std::vector<unsigned char> sbL, sbU, fgL, fgU;
+    batches(lpL, nL, rowsL, lvL, Loff, Lc, Lcd, bpL, bmL, sbL, fgL);
+    batches(lpU, nU, rowsU, lvU, Uoff, Uc, Ucd, bpU, bmU, sbU, fgU);
     I.nbatL = (int)bmL.size();
     I.nbatU = (int)bmU.size();
     if ((rc = upload_ilu(I, s, &I.Lcd, Lcd)) || (rc = upload_ilu(I, s, &I.Lmapd, Lmapd)) ||
@@ -1434,10 +1456,13 @@ int ilu_setup(ncg_problem *p, int degree)
         (rc = upload_ilu(I, s, &I.batL_ptr, bpL)) || (rc = upload_ilu(I, s, &I.batL_m, bmL)) ||
         (rc = upload_ilu(I, s, &I.batU_ptr, bpU)) || (rc = upload_ilu(I, s, &I.batU_m, bmU)) ||
         (rc = I.alloc(&I.subL, (size_t)n)) || (rc = I.alloc(&I.subU, (size_t)n)) ||
+        (rc = I.alloc(&I.flagL, fgL.size())) || (rc = I.alloc(&I.flagU, fgU.size())) ||
         (rc = I.alloc(&I.Lvd, 2 * (size_t)n * B)) || (rc = I.alloc(&I.Uvd, 2 * (size_t)n * B)))
         return rc;
     NCG_HIP(hipMemcpyAsync(I.subL, sbL.data(), (size_t)n, hipMemcpyHostToDevice, s));
     NCG_HIP(hipMemcpyAsync(I.subU, sbU.data(), (size_t)n, hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.flagL, fgL.data(), fgL.size(), hipMemcpyHostToDevice, s));
+    NCG_HIP(hipMemcpyAsync(I.flagU, fgU.data(), fgU.size(), hipMemcpyHostToDevice, s));
     if ((rc = upload_ilu(I, s, &I.Loff, Loff)) || (rc = upload_ilu(I, s, &I.Lc, Lc)) ||
         (rc = upload_ilu(I, s, &I.Lmap, Lmap)) || (rc = upload_ilu(I, s, &I.Uoff, Uoff)) ||
         (rc = upload_ilu(I, s, &I.Uc, Uc)) || (rc = upload_ilu(I, s, &I.Umap, Umap)) ||
@@ -1494,6 +1519,7 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
     L.bat_ptr = I.batL_ptr;
     L.bat_m = I.batL_m;
     L.sub = I.subL;
+    L.bat_flag = I.flagL;
     L.row = I.levL_rows;
     L.off = I.Loff;
     L.c = I.Lc;
@@ -1504,6 +1530,7 @@ void ilu_apply(ncg_problem *p, const double *x, double *y)
     U.bat_ptr = I.batU_ptr;
     U.bat_m = I.batU_m;
     U.sub = I.subU;
+    U.bat_flag = I.flagU;
     U.row = I.levU_rows;
     U.off = I.Uoff;
     U.c = I.Uc;

@@ -938,7 +938,7 @@ struct IluItem {
 };
 
 // thread t's row i of the batch [b, b + w) (w <= kIluRows, i already loaded); every load
-// here depends only on the position and i.  xr (forward) or d (backward) may be null.
+// here depends only on the position and i; d (backward sweep only) may be null.
 __device__ __forceinline__ void ilu_item_load(IluItem &it, int t, int b, int w, int i, const int *off,
                                               const unsigned char *sub, const int *cd, const cx<double> *vd,
                                               const cx<double> *xr, const cx<double> *d)
@@ -954,65 +954,96 @@ __device__ __forceinline__ void ilu_item_load(IluItem &it, int t, int b, int w, 
         it.cc[q] = cd[(size_t)u * kIluB + q];
         it.vv[q] = vd[(size_t)u * kIluB + q];
     }
-    it.xv = xr ? xr[i] : mk(0.0, 0.0);
+    it.xv = xr[i];
     it.dv = d ? d[u] : mk(1.0, 0.0);
 }
 
+constexpr int kIluSched = 1024;   // batches whose schedule is kept in LDS
 // One triangular sweep of one load case by one work-group of kIluRows threads, a batch of
 // consecutive levels (at most kIluRows rows, thread t taking position t of the batch) per
-// step.  A batch's solution values from earlier batches are gathered once; its rows then
-// run level by level (sub-steps, a barrier after each), taking the values of the batch's
-// earlier levels from the slots their threads left in LDS, so a batch costs one round of
-// memory gathers however many levels it holds.  Every row sums its entries in the CSR
-// order.  The work-group runs on one CU: after a batch its stores are made visible to the
-// work-group (release fence, barrier) before the next batch gathers.  The next batch's
-// rows and coefficients are loaded while the current one is computed.  Levels wider than
-// the work-group form batches of their own and take a plain loop.
-template <bool UPPER>
+// step; its rows run level by level (sub-steps, an LDS barrier after each).  Every solved
+// value goes to memory and to an LDS ring (slot = position mod kIluRing); an entry among a
+// row's first kIluB reads its value from the ring while the ring still holds it (the
+// host-built schedule encodes which).  The loads of the next batch's rows are requested
+// at the start of a batch and stay in flight while its sub-steps run on LDS and registers
+// only: the schedule sits in LDS (LDS_SCHED), the few values no longer in the ring are read
+// before that request (the batch is flagged), and every select is done per component.
+// Stores become visible to the work-group at the ends of the batches the schedule flags,
+// before any later batch reads them from memory.  Rows past their kIluB-th entry and levels
+// wider than the work-group (a batch of their own, a plain loop) read memory.  Every row
+// sums its entries in the CSR order.  xr: the right-hand side (forward sweep) or the
+// forward sweep's result (backward, the same vector as yr).
+template <bool UPPER, bool LDS_SCHED>
 __device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<double> *v, const cx<double> *vd,
-                                          const cx<double> *d, const cx<double> *xr, cx<double> *yr)
+                                          const cx<double> *d, const cx<double> *xr, cx<double> *yr,
+                                          cx<double> *ring)
 {
     constexpr int W = ncgk::kIluRows;
-    __shared__ cx<double> ys[W];   // the batch's rows solved so far, by thread
     const int t = threadIdx.x, nb = S.nbat;
-    const int *bp = S.bat_ptr, *row = S.row, *off = S.off, *c = S.c, *cd = S.cd;
-    auto width = [&](int b) { return bp[b + 1] - bp[b]; };
-    auto row_of = [&](int b) { return b < nb && t < width(b) ? row[bp[b] + t] : 0; };
+    const int *row = S.row, *off = S.off, *c = S.c, *cd = S.cd;
+    // the schedule (batch starts, levels and flags) in LDS for the whole sweep (LDS_SCHED),
+    // so that no memory load besides the prefetch of the next batch is in flight
+    __shared__ int sched[2 * kIluSched + 1];
+    if (LDS_SCHED) {
+        for (int q = t; q <= nb; q += W) sched[q] = S.bat_ptr[q];
+        for (int q = t; q < nb; q += W) sched[kIluSched + 1 + q] = S.bat_m[q] | (S.bat_flag[q] << 16);
+        __syncthreads();
+    }
+    auto bpv = [&](int b) { return LDS_SCHED ? sched[b] : S.bat_ptr[b]; };
+    auto mfl = [&](int b) { return LDS_SCHED ? sched[kIluSched + 1 + b] : S.bat_m[b] | (S.bat_flag[b] << 16); };
+    auto width = [&](int b) { return bpv(b + 1) - bpv(b); };
+    auto row_of = [&](int b) { return b < nb && t < width(b) ? row[bpv(b) + t] : 0; };
     IluItem cur, nxt;
     cur.on = false;
     int rnext = row_of(1);
     if (nb > 0 && width(0) <= W)
-        ilu_item_load(cur, t, bp[0], width(0), row_of(0), off, S.sub, cd, vd, UPPER ? nullptr : xr,
+        ilu_item_load(cur, t, bpv(0), width(0), row_of(0), off, S.sub, cd, vd, xr,
                       UPPER ? d : nullptr);
     for (int b = 0; b < nb; ++b) {
-        const int b0 = bp[b], w = bp[b + 1] - b0;
+        const int b0 = bpv(b), w = bpv(b + 1) - b0;
+        // this batch's rows (requested while the batch before ran) have arrived: every vector
+        // memory operation but the last (this thread's store of the batch before) is done,
+        // so the loads issued below are the only ones in flight during the sub-steps
+        __builtin_amdgcn_s_waitcnt(0x0F71);
+        // the batch's entries whose values are no longer in the ring, read from memory before
+        // the next batch is requested (rare; the batch is flagged)
+        cx<double> yy[kIluB];
+#pragma unroll
+        for (int q = 0; q < kIluB; ++q) yy[q] = mk(0.0, 0.0);
+        if ((mfl(b) >> 16) & 2) {
+#pragma unroll
+            for (int q = 0; q < kIluB; ++q) yy[q] = yr[cur.cc[q] >= 0 ? cur.cc[q] : 0];
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // arrived before anything else is requested
+        }
         nxt.on = false;
         const int rafter = row_of(b + 2);
         if (b + 1 < nb && width(b + 1) <= W)
-            ilu_item_load(nxt, t, bp[b + 1], width(b + 1), rnext, off, S.sub, cd, vd, UPPER ? nullptr : xr,
+            ilu_item_load(nxt, t, bpv(b + 1), width(b + 1), rnext, off, S.sub, cd, vd, xr,
                           UPPER ? d : nullptr);
         if (w <= W) {
-            // values from earlier batches (in-batch slots read a dummy, replaced below)
-            cx<double> yy[kIluB];
-#pragma unroll
-            for (int q = 0; q < kIluB; ++q) yy[q] = yr[cur.cc[q] >= 0 ? cur.cc[q] : 0];
-            const cx<double> s0 = UPPER ? yr[cur.i] : cur.xv;
-            const int m = S.bat_m[b];
-            for (int st = 0; st < m; ++st) {
+            const cx<double> s0 = cur.xv;
+            const int m = mfl(b) & 0xffff;
+                for (int st = 0; st < m; ++st) {
                 const bool mine = cur.on && cur.ss == st;
                 if (mine) {
-                    // values of the batch's earlier levels from the threads' slots in LDS
-                    if (st > 0)
-#pragma unroll
-                        for (int q = 0; q < kIluB; ++q)
-                            if (cur.cc[q] < 0) yy[q] = ys[-1 - cur.cc[q]];
                     cx<double> acc = s0;
+                    // every ring read first (one wait), then the entries in order; an
+                    // entry past the row's end leaves acc as it is
+                    cx<double> rv[kIluB];
 #pragma unroll
-                    for (int q = 0; q < kIluB; ++q)
-                        if (cur.k0 + q < cur.k1) acc -= cur.vv[q] * yy[q];
+                    for (int q = 0; q < kIluB; ++q) rv[q] = ring[cur.cc[q] < 0 ? -1 - cur.cc[q] : 0];
+#pragma unroll
+                    for (int q = 0; q < kIluB; ++q) {
+                        // component selects (a select of the struct goes through scratch)
+                        const bool ring_q = cur.cc[q] < 0, in_q = cur.k0 + q < cur.k1;
+                        const cx<double> y = mk(ring_q ? rv[q].re : yy[q].re, ring_q ? rv[q].im : yy[q].im);
+                        const cx<double> a = acc - cur.vv[q] * y;
+                        acc.re = in_q ? a.re : acc.re;
+                        acc.im = in_q ? a.im : acc.im;
+                    }
                     if (cur.k1 - cur.k0 > kIluB) acc = ilu_row(acc, cur.k0 + kIluB, cur.k1, c, v, yr);
                     if (UPPER) acc = cdiv(acc, cur.dv);
-                    ys[t] = acc;
+                    ring[(b0 + t) % ncgk::kIluRing] = acc;
                     yr[cur.i] = acc;
                 }
                 if (st + 1 < m) {
@@ -1024,16 +1055,21 @@ __device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<doub
                 }
             }
         } else {
-            for (int q = t; q < w; q += W) {
+                for (int q = t; q < w; q += W) {
                 const int u = b0 + q, i = row[u];
-                if (UPPER) yr[i] = cdiv(ilu_row(yr[i], off[u], off[u + 1], c, v, yr), d[u]);
-                else yr[i] = ilu_row(xr[i], off[u], off[u + 1], c, v, yr);
+                cx<double> acc = ilu_row(xr[i], off[u], off[u + 1], c, v, yr);
+                if (UPPER) acc = cdiv(acc, d[u]);
+                ring[u % ncgk::kIluRing] = acc;
+                yr[i] = acc;
             }
         }
-        // the batch's stores before the next batch's gathers; its LDS slots before reuse
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if ((mfl(b) >> 16) & 1) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        }
         cur = nxt;
         rnext = rafter;
     }
@@ -1046,9 +1082,19 @@ __device__ __forceinline__ void ilu_sweep(const ncgk::IluSweep &S, const cx<doub
 __global__ void __launch_bounds__(ncgk::kIluRows) k_ilu_solve(int n, ncgk::IluSweep L, ncgk::IluSweep U,
                                                              const cx<double> *x, cx<double> *y)
 {
+    __shared__ cx<double> ring[ncgk::kIluRing];
     const size_t r = blockIdx.x;
-    ilu_sweep<false>(L, (const cx<double> *)L.v, (const cx<double> *)L.vd, nullptr, x + r * n, y + r * n);
-    ilu_sweep<true>(U, (const cx<double> *)U.v, (const cx<double> *)U.vd, (const cx<double> *)U.d, nullptr, y + r * n);
+    if (L.nbat <= kIluSched)
+        ilu_sweep<false, true>(L, (const cx<double> *)L.v, (const cx<double> *)L.vd, nullptr, x + r * n, y + r * n, ring);
+    else
+        ilu_sweep<false, false>(L, (const cx<double> *)L.v, (const cx<double> *)L.vd, nullptr, x + r * n, y + r * n, ring);
+    __syncthreads();
+    if (U.nbat <= kIluSched)
+        ilu_sweep<true, true>(U, (const cx<double> *)U.v, (const cx<double> *)U.vd, (const cx<double> *)U.d, y + r * n,
+                              y + r * n, ring);
+    else
+        ilu_sweep<true, false>(U, (const cx<double> *)U.v, (const cx<double> *)U.vd, (const cx<double> *)U.d, y + r * n,
+                               y + r * n, ring);
 }
 
 // ======================================================================================

@@ -102,17 +102,20 @@ void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows
 // row row[u] with entries c/v[off[u] .. off[u+1]) (CSR order), the first kIluB of them
 // also padded in cd/vd[u * kIluB + q]; d = U's diagonal per position (backward sweep)
 constexpr int kIluB = 8;
-// rows of a batch of the triangular sweeps = threads of the sweep's work-group
-constexpr int kIluRows = 256;
+// rows of a batch of the triangular sweeps = threads of the sweep's work-group, and the
+// positions of the LDS ring that keeps the latest solved values of a sweep
+constexpr int kIluRows = 256, kIluRing = 8192;
 // One triangular sweep's schedule: positions u = 0..n-1 (row row[u]) in batches of
-// consecutive levels, bat_ptr[b]..bat_ptr[b+1] (at most kIluRows positions, or one wider level),
-// bat_m[b] levels per batch and sub[u] the level of u inside its batch.  The first kIluB
-// entries of every position are also in cd/vd[u][kIluB], a column of the same batch
-// written as -1 - (its position - bat_ptr[b]).
+// consecutive levels, bat_ptr[b]..bat_ptr[b+1] (at most kIluRows positions, or one wider
+// level), bat_m[b] levels per batch and sub[u] the level of u inside its batch.  The first
+// kIluB entries of every position are also in cd/vd[u][kIluB]: cd < 0 is the ring slot
+// -1 - cd (position mod kIluRing) of a value the ring still holds, cd >= 0 a column read
+// from memory.  bat_flag[b]: bit 0, the batch ends with its stores made visible to the
+// work-group; bit 1, some row of the batch reads a padded entry from memory.
 struct IluSweep {
     int nbat = 0;
     const int *bat_ptr = nullptr, *bat_m = nullptr, *row = nullptr, *off = nullptr, *c = nullptr, *cd = nullptr;
-    const unsigned char *sub = nullptr;
+    const unsigned char *sub = nullptr, *bat_flag = nullptr;
     const void *v = nullptr, *vd = nullptr, *d = nullptr;
 };
 // out[t] = val[map[t]] (0 where map[t] < 0)
