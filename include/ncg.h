@@ -177,8 +177,9 @@ int ncg_solve_group(ncg_problem *const *problems, int32_t count, const ncg_solve
 
 /* Launch shapes the operator applies and the dense assembly run with, chosen by measurement
  * on the problems at hand (ncg_autotune) rather than fixed.  The FMM fields change how the
- * grouped applies split their sums (hence the last bits of a result); the near-field mode
- * and the assembly rows do not change any value.  A group solve uses its first problem's
+ * grouped applies split their sums (hence the last bits of a result); the near-field mode,
+ * the near-field assembly mapping and the assembly rows do not change any value.  A group
+ * solve uses its first problem's
  * tuning.  No counterpart in the reference. */
 typedef struct {
     int32_t t_waves, d_waves, s_waves;  /* waves per work-group of the grouped T, D, S applies:
@@ -190,7 +191,9 @@ typedef struct {
                                            matrix-core product, 1..16 (0: from the size)     */
     int32_t assemble_rows;              /* dense: collocation rows per work-group of the
                                            far-field quadrature, 8..256 (0: 32)             */
-    int32_t reserved;
+    int32_t near_assembly;              /* FMM near-field quadrature: 0 a thread per column
+                                           (geometry in registers, default), 1 a wave per
+                                           collocation row (column geometry in LDS)         */
 } ncg_tuning;
 int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t);
 int ncg_get_tuning(ncg_problem *p, ncg_tuning *t);

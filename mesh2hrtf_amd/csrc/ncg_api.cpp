@@ -1112,7 +1112,7 @@ int assemble_fmm(ncg_problem *p)
     {
         KernelScope ks(p, kKidFmmNear + mixed, 0.0, kQFmmNear);
         ncgk::fmm_near(s, E, b, p->adm, DF.chunks(), DF.cl_ptr, DF.cl_elem, F.near_ptr, F.near_col, F.near_wmax,
-                       F.near_val.p, mixed, q_counter(p, kQFmmNear, F.sig));
+                       F.near_val.p, mixed, q_counter(p, kQFmmNear, F.sig), p->tune.near_assembly);
     }
     {
         KernelScope ks(p, kKidFmmAdj, 0.0, kQFmmAdj);
@@ -2960,6 +2960,8 @@ int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t)
     if (t->mfma_splits < 0 || t->mfma_splits > 16) return fail(NCG_E_ARG, "ncg_set_tuning: mfma_splits must be 0..16");
     if (t->assemble_rows != 0 && (t->assemble_rows < 8 || t->assemble_rows > 256))
         return fail(NCG_E_ARG, "ncg_set_tuning: assemble_rows must be 0 or 8..256");
+    if (t->near_assembly != 0 && t->near_assembly != 1)
+        return fail(NCG_E_ARG, "ncg_set_tuning: near_assembly must be 0 or 1");
     p->tune = *t;
     return 0;
 }
@@ -3029,6 +3031,28 @@ int autotune_group(const std::vector<ncg_problem *> &P, int reps, ncg_tuning &be
             }
         }
         lead->tune.near_mode = best.near_mode;
+        // near-field quadrature mapping (changes no value): the leader's regular near field
+        // assembled again in both mappings
+        {
+            ncg_problem *p = lead;
+            DevFmm &F = p->fmm;
+            const DevLevel &DF = F.lev.back();
+            EventPair ev = p->ev_far;
+            double tq = 1e300;
+            for (int mode = 0; mode < 2; ++mode) {
+                record(p, ev.a);
+                for (int r = 0; r < std::max(1, reps / 2); ++r)
+                    ncgk::fmm_near(s, p->elements(), p->bm(), p->adm, DF.chunks(), DF.cl_ptr, DF.cl_elem, F.near_ptr,
+                                   F.near_col, F.near_wmax, F.near_val.p, p->precision == NCG_MIXED, nullptr, mode);
+                record(p, ev.b);
+                NCG_HIP(hipStreamSynchronize(s));
+                const double t = elapsed(ev);
+                if (t < tq) {
+                    tq = t;
+                    best.near_assembly = mode;
+                }
+            }
+        }
         // waves per work-group of T, D and S, each kind on its own
         double tT = 1e300, tD = 1e300, tS = 1e300;
         Stage st(lead);

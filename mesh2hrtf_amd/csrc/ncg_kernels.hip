@@ -1592,6 +1592,84 @@ __global__ void __launch_bounds__(128) k_fmm_near_adj(Elements E, BMConst bm, co
     if (evals) wave_count(evals, nev);
 }
 
+// One column element of the regular near-field quadrature: its corners (FP64), the
+// triangle edges from corner 2, normal and Jacobian in the quadrature's arithmetic T.
+template <typename T>
+struct NearCol {
+    int nv, kind;
+    cx<T> adm;
+    double cj[3], area;
+    double P[4][3];
+    T ny[3], jac, e1[3], e2[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void near_col_load(const Elements &E, const double *adm, int j, NearCol<T> &C)
+{
+    const int n = E.n;
+    C.nv = E.nv[j];
+    C.kind = E.kind[j];
+    C.adm = cx<T>{T(adm[2 * j]), T(adm[2 * j + 1])};
+    C.cj[0] = E.cen[j];
+    C.cj[1] = E.cen[n + j];
+    C.cj[2] = E.cen[2 * n + j];
+    C.area = E.area[j];
+    for (int d = 0; d < 3; ++d) C.ny[d] = T(E.nrm[d * n + j]);
+    C.jac = T(E.jac[j]);
+    for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d) C.P[a][d] = a < C.nv ? E.X[(3 * a + d) * n + j] : 0.0;
+    // triangle: x(s,t) = x2 + s (x0 - x2) + t (x1 - x2)
+    for (int d = 0; d < 3; ++d) {
+        C.e1[d] = T(C.P[0][d] - C.P[2][d]);
+        C.e2[d] = T(C.P[1][d] - C.P[2][d]);
+    }
+}
+
+// Entry (row i, column C) of the regular near field: false for the diagonal and for a
+// subdivided pair (those come from k_assemble_diag and k_fmm_near_adj).  nev counts the
+// Green's-function evaluations.
+template <typename T>
+__device__ __forceinline__ bool near_entry(const Elements &E, const BMConst &bm, const NearCol<T> &C, int i, int j,
+                                           cx<T> &out, unsigned long long &nev)
+{
+    const int n = E.n;
+    if (i == j) return false;
+    const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
+    const double ratdis = ratio_to_centre(C.cj[0], C.cj[1], C.cj[2], x, C.area);
+    if (ratdis < kTolf) return false;
+    const T nxx = T(E.nrm[i]), nxy = T(E.nrm[n + i]), nxz = T(E.nrm[2 * n + i]);
+    const T hk = T(bm.hk), k2 = T(bm.k2), tau = T(bm.tau), bim = T(bm.beta_im);
+    const int mg = gauss_order(ratdis);
+    BMAcc<T> acc;
+    acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
+    if (C.nv == 3) {
+        const T px = T(C.P[2][0] - x[0]), py = T(C.P[2][1] - x[1]), pz = T(C.P[2][2] - x[2]);
+        const int off = tri_rule_offset(mg), cnt = tri_rule_count(mg);
+        for (int g = 0; g < cnt; ++g) {
+            const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), wg = T(c_tri[off + g][2]);
+            bm_point<T>(acc, px + s * C.e1[0] + t * C.e2[0], py + s * C.e1[1] + t * C.e2[1],
+                        pz + s * C.e1[2] + t * C.e2[2], C.ny[0], C.ny[1], C.ny[2], nxx, nxy, nxz, wg * C.jac, hk, k2);
+        }
+        nev += cnt;
+    } else {
+        LocalElem L;
+        L.nv = 4;
+        for (int b = 0; b < 4; ++b)
+            for (int d = 0; d < 3; ++d) L.p[b][d] = C.P[b][d] - x[d];
+        const int base = mg * (mg - 1) / 2;
+        for (int s2 = 0; s2 < mg; ++s2)
+            for (int t2 = 0; t2 < mg; ++t2) {
+                double q2[3], ny[3], J;
+                point_on(L, c_gl_x[base + s2], c_gl_x[base + t2], q2, ny, J);
+                bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
+                            T(c_gl_w[base + s2] * c_gl_w[base + t2] * J), hk, k2);
+            }
+        nev += mg * mg;
+    }
+    out = bm_entry<T>(acc, C.kind, C.adm, tau, bim);
+    return true;
+}
+
 // Regular near-field entries (NC_ComputeEntriesNforFMM, src/NC_EquationSystem.cpp:1243-1405,
 // without the subdivided pairs, which k_fmm_near_adj writes).  The rows of a finest
 // cluster share their column list, so a work-group takes a row chunk of one cluster
@@ -1605,65 +1683,53 @@ __global__ void __launch_bounds__(256) k_fmm_near(Elements E, BMConst bm, const 
                                                   const int *cl_elem, const long long *ptr, const int *col, cx<S> *vals,
                                                   unsigned long long *evals)
 {
-    const int n = E.n;
     const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
     const long long cb = ptr[cl_elem[cl_ptr[c]]];
     const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
     const int u = blockIdx.y * 256 + threadIdx.x;
     unsigned long long nev = 0;
     if (u < w) {
-    const int j = col[cb + u];
-    const int nv = E.nv[j];
-    const int kind = E.kind[j];
-    const cx<T> admj = {T(adm[2 * j]), T(adm[2 * j + 1])};
-    const double cj[3] = {E.cen[j], E.cen[n + j], E.cen[2 * n + j]};
-    const double areaj = E.area[j];
-    const T nyx = T(E.nrm[j]), nyy = T(E.nrm[n + j]), nyz = T(E.nrm[2 * n + j]);
-    const T jacj = T(E.jac[j]);
-    double P[4][3];
-    for (int a = 0; a < nv; ++a)
-        for (int d = 0; d < 3; ++d) P[a][d] = E.X[(3 * a + d) * n + j];
-    // triangle: x(s,t) = x2 + s (x0 - x2) + t (x1 - x2)
-    const T e1x = T(P[0][0] - P[2][0]), e1y = T(P[0][1] - P[2][1]), e1z = T(P[0][2] - P[2][2]);
-    const T e2x = T(P[1][0] - P[2][0]), e2y = T(P[1][1] - P[2][1]), e2z = T(P[1][2] - P[2][2]);
-    const T hk = T(bm.hk), k2 = T(bm.k2), tau = T(bm.tau), bim = T(bm.beta_im);
-    for (int a = 0; a < m; ++a) {
-        const int i = cl_elem[r0 + a];
-        if (i == j) continue;   // diagonal: k_assemble_diag
-        const double x[3] = {E.cen[i], E.cen[n + i], E.cen[2 * n + i]};
-        const double ratdis = ratio_to_centre(cj[0], cj[1], cj[2], x, areaj);
-        if (ratdis < kTolf) continue;   // subdivided pair: k_fmm_near_adj
-        const T nxx = T(E.nrm[i]), nxy = T(E.nrm[n + i]), nxz = T(E.nrm[2 * n + i]);
-        const int mg = gauss_order(ratdis);
-        BMAcc<T> acc;
-        acc.G = acc.H = acc.HT = acc.E = mk(T(0), T(0));
-        if (nv == 3) {
-            const T px = T(P[2][0] - x[0]), py = T(P[2][1] - x[1]), pz = T(P[2][2] - x[2]);
-            const int off = tri_rule_offset(mg), cnt = tri_rule_count(mg);
-            for (int g = 0; g < cnt; ++g) {
-                const T s = T(c_tri[off + g][0]), t = T(c_tri[off + g][1]), wg = T(c_tri[off + g][2]);
-                bm_point<T>(acc, px + s * e1x + t * e2x, py + s * e1y + t * e2y, pz + s * e1z + t * e2z, nyx, nyy, nyz,
-                            nxx, nxy, nxz, wg * jacj, hk, k2);
-            }
-            nev += cnt;
-        } else {
-            LocalElem L;
-            L.nv = 4;
-            for (int b = 0; b < 4; ++b)
-                for (int d = 0; d < 3; ++d) L.p[b][d] = P[b][d] - x[d];
-            const int base = mg * (mg - 1) / 2;
-            for (int s2 = 0; s2 < mg; ++s2)
-                for (int t2 = 0; t2 < mg; ++t2) {
-                    double q2[3], ny[3], J;
-                    point_on(L, c_gl_x[base + s2], c_gl_x[base + t2], q2, ny, J);
-                    bm_point<T>(acc, T(q2[0]), T(q2[1]), T(q2[2]), T(ny[0]), T(ny[1]), T(ny[2]), nxx, nxy, nxz,
-                                T(c_gl_w[base + s2] * c_gl_w[base + t2] * J), hk, k2);
-                }
-            nev += mg * mg;
+        const int j = col[cb + u];
+        NearCol<T> C;
+        near_col_load(E, adm, j, C);
+        for (int a = 0; a < m; ++a) {
+            const int i = cl_elem[r0 + a];
+            cx<T> e;
+            if (near_entry(E, bm, C, i, j, e, nev)) vals[ptr[i] + u] = cx<S>{S(e.re), S(e.im)};
         }
-        const cx<T> e = bm_entry<T>(acc, kind, admj, tau, bim);
-        vals[ptr[i] + u] = cx<S>{S(e.re), S(e.im)};
     }
+    if (evals) wave_count(evals, nev);
+}
+
+// The same entries with one wave per collocation row: the work-group stages the
+// geometry of its 256 columns in LDS, then each wave takes every 4th row of the chunk
+// and its lanes walk the columns 64 at a time, reading the column from LDS.  Same
+// arithmetic per entry as k_fmm_near (near_entry).
+template <typename S, typename T>
+__global__ void __launch_bounds__(256) k_fmm_near_rows(Elements E, BMConst bm, const double *adm, const int *ch_cl,
+                                                       const int *ch_r0, const int *ch_m, const int *cl_ptr,
+                                                       const int *cl_elem, const long long *ptr, const int *col,
+                                                       cx<S> *vals, unsigned long long *evals)
+{
+    __shared__ NearCol<T> cols[256];
+    const int q = blockIdx.x, c = ch_cl[q], r0 = ch_r0[q], m = ch_m[q];
+    const long long cb = ptr[cl_elem[cl_ptr[c]]];
+    const int w = (int)(ptr[cl_elem[cl_ptr[c]] + 1] - cb);
+    const int u0 = blockIdx.y * 256, nu = min(256, w - u0);
+    if (threadIdx.x < nu) near_col_load(E, adm, col[cb + u0 + threadIdx.x], cols[threadIdx.x]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long nev = 0;
+    for (int a = wv; a < m; a += 4) {
+        const int i = cl_elem[r0 + a];
+        for (int v0 = 0; v0 < nu; v0 += 64) {
+            const int v = v0 + lane;
+            if (v < nu) {
+                const NearCol<T> C = cols[v];
+                cx<T> e;
+                if (near_entry(E, bm, C, i, col[cb + u0 + v], e, nev)) vals[ptr[i] + u0 + v] = cx<S>{S(e.re), S(e.im)};
+            }
+        }
     }
     if (evals) wave_count(evals, nev);
 }
@@ -3348,14 +3414,20 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
 }
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
-              int mixed, unsigned long long *evals)
+              int mixed, unsigned long long *evals, int rows_mode)
 {
     if (rc.n == 0 || wmax == 0) return;
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const dim3 g(rc.n, (wmax + 255) / 256);
-    if (mixed)
+    if (mixed && rows_mode)
+        k_fmm_near_rows<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
+                                                        col, (cx<float> *)vals, evals);
+    else if (mixed)
         k_fmm_near<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                                                    (cx<float> *)vals, evals);
+    else if (rows_mode)
+        k_fmm_near_rows<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
+                                                          col, (cx<double> *)vals, evals);
     else
         k_fmm_near<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                                                      (cx<double> *)vals, evals);

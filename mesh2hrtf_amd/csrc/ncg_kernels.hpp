@@ -196,10 +196,12 @@ void fmm_apply_D(hipStream_t s, int ncl, int P, int nrhs, const int *fan_ptr, co
 void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks &tl, const int *cl_ptr,
                  const int *cl_elem, const void *Sv, int f32, const void *sw, void *y);
 // regular near-field entries by (row chunk, 256 columns); the subdivided pairs are left to
-// fmm_near_adj and the diagonal to assemble_diag.  wmax = longest row.
+// fmm_near_adj and the diagonal to assemble_diag.  wmax = longest row.  rows_mode 0: a
+// thread per column with its geometry in registers; 1: a wave per row, the columns'
+// geometry in LDS (the same values)
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
-              int mixed, unsigned long long *evals = nullptr);
+              int mixed, unsigned long long *evals = nullptr, int rows_mode = 0);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
                   void *tv, int fp32);

@@ -176,8 +176,8 @@ void load_tunings(ncg_server *S)
         if (!bar) continue;
         *bar = 0;
         ncg_tuning t{};
-        if (sscanf(bar + 1, "%d %d %d %d %d %d", &t.t_waves, &t.d_waves, &t.s_waves, &t.near_mode, &t.mfma_splits,
-                   &t.assemble_rows) == 6) {
+        if (sscanf(bar + 1, "%d %d %d %d %d %d %d", &t.t_waves, &t.d_waves, &t.s_waves, &t.near_mode, &t.mfma_splits,
+                   &t.assemble_rows, &t.near_assembly) >= 6) {
             t.tuned = 1;
             S->tunings[line] = t;
         }
@@ -190,8 +190,9 @@ void save_tunings(ncg_server *S)
     FILE *f = fopen(S->tuning_file.c_str(), "w");
     if (!f) return;
     for (auto &kv : S->tunings)
-        fprintf(f, "%s|%d %d %d %d %d %d\n", kv.first.c_str(), kv.second.t_waves, kv.second.d_waves,
-                kv.second.s_waves, kv.second.near_mode, kv.second.mfma_splits, kv.second.assemble_rows);
+        fprintf(f, "%s|%d %d %d %d %d %d %d\n", kv.first.c_str(), kv.second.t_waves, kv.second.d_waves,
+                kv.second.s_waves, kv.second.near_mode, kv.second.mfma_splits, kv.second.assemble_rows,
+                kv.second.near_assembly);
     fclose(f);
 }
 

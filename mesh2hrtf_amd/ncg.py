@@ -96,11 +96,11 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
 class _Tuning(ctypes.Structure):
     _fields_ = [("t_waves", ctypes.c_int32), ("d_waves", ctypes.c_int32), ("s_waves", ctypes.c_int32),
                 ("tuned", ctypes.c_int32), ("near_mode", ctypes.c_int32), ("mfma_splits", ctypes.c_int32),
-                ("assemble_rows", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("assemble_rows", ctypes.c_int32), ("near_assembly", ctypes.c_int32)]
 
 
 def _tuning_dict(t):
-    return {k: getattr(t, k) for k, _ in _Tuning._fields_ if k != "reserved"}
+    return {k: getattr(t, k) for k, _ in _Tuning._fields_}
 
 
 class _KernelStat(ctypes.Structure):
@@ -299,8 +299,9 @@ class Problem:
         _check(self.lib.ncg_get_tuning(self._h, ctypes.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, t_waves=8, d_waves=8, s_waves=8, near_mode=0, mfma_splits=0, assemble_rows=0):
-        t = _Tuning(int(t_waves), int(d_waves), int(s_waves), 0, int(near_mode), int(mfma_splits), int(assemble_rows))
+    def set_tuning(self, t_waves=8, d_waves=8, s_waves=8, near_mode=0, mfma_splits=0, assemble_rows=0, near_assembly=0):
+        t = _Tuning(int(t_waves), int(d_waves), int(s_waves), 0, int(near_mode), int(mfma_splits), int(assemble_rows),
+                    int(near_assembly))
         _check(self.lib.ncg_set_tuning(self._h, ctypes.byref(t)))
 
     def timings(self, reset=False):

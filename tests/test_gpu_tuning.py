@@ -1,6 +1,7 @@
 """Launch-shape tuning (ncg_autotune, ncg_set_tuning, the server's per-shape tunings).
 
-* the near-field work-group mode and the dense far-field rows per work-group change no value:
+* the near-field work-group mode, the near-field assembly mapping (thread per column or
+  wave per collocation row) and the dense far-field rows per work-group change no value:
   results bit for bit equal;
 * the far-field waves per work-group and the matrix-core splits change only the summation
   order: solutions within rel-L2 1e-10 (FP64) of the default shapes';
@@ -35,13 +36,22 @@ def test_near_mode_and_waves():
     opts = dict(solver=ncg.GMRES, precond=ncg.BLOCK, tolerance=1e-10)
     res = {}
     for key, tune in (("default", {}), ("cluster", dict(near_mode=1)), ("waves4", dict(t_waves=4, d_waves=2,
-                                                                                      s_waves=4))):
-        probs[0].set_tuning(**tune)
+                                                                                      s_waves=4)),
+                      ("rows", dict(near_assembly=1))):
+        for p in probs:
+            p.set_tuning(**tune)
+        if key == "rows":   # the near field assembled again with a wave per collocation row
+            for p, f in zip(probs, (3000.0, 5000.0, 8000.0)):
+                pa = R.problem_arrays(R.load_case("fmm_ml_head"), f)
+                p.assemble(pa["k"], nrhs=2, point_sources=[list(v) + [r] for r in range(2) for v in pa["ps"]],
+                           precision=ncg.FP64)
         print("tuning", key, flush=True)
         st = ncg.solve_group(probs, **opts)
         assert all(s["converged"] for s in st)
         res[key] = [p.solution() for p in probs]
     for a, b in zip(res["default"], res["cluster"]):
+        assert np.array_equal(a, b)
+    for a, b in zip(res["default"], res["rows"]):
         assert np.array_equal(a, b)
     for a, b in zip(res["default"], res["waves4"]):
         for r in range(2):
@@ -49,7 +59,7 @@ def test_near_mode_and_waves():
     t = ncg.autotune(probs, reps=2)
     print("chosen", t, flush=True)
     assert t["tuned"] == 1 and t["t_waves"] in (2, 4, 8) and t["d_waves"] in (2, 4, 8) and t["s_waves"] in (2, 4, 8)
-    assert t["near_mode"] in (0, 1)
+    assert t["near_mode"] in (0, 1) and t["near_assembly"] in (0, 1)
     for p in probs:
         assert p.tuning() == t
     st = ncg.solve_group(probs, **opts)
