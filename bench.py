@@ -63,7 +63,7 @@ HBM_PEAK = 8.0e12
 TRAFFIC_PROFILE = "kg_near_spmv_head20480.txt"
 TRAFFIC_PROFILE_BATCHED = "k_matvec_mfma_batched.txt"
 SERVED = dict(precision=ncg.MIXED, solver=ncg.GMRES, precond=ncg.BLOCK, restart=50, max_iterations=500,
-              refinement_steps=8, refinement_tolerance=1e-5)
+              refinement_steps=8, refinement_tolerance=1e-6)
 
 
 def freq_spec(src, f):
@@ -618,6 +618,12 @@ def head_main(a):
                        "form": "the drop-in's FP64 plain method: NumCalc's method choice, ILU-CGS to 1e-9, one "
                                "frequency at a time"},
         "speedup_vs_plain": value / (nplain * nr / plain_elapsed) if nplain else None,
+        "refinement": {"tolerance": SERVED["refinement_tolerance"],
+                       "steps_max": max(s["refinement_steps_max"] for s in sts),
+                       "relative_residual_max": max(s["relative_residual_max"] for s in sts),
+                       "sweeps_with_every_step_within_tolerance": "%d of %d" % (
+                           sum(1 for s in sts if s["state"] == "done" and s["relative_residual_max"] <= SERVED[
+                               "refinement_tolerance"]), a.steps)},
         "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
                                  "eval_grid_max": float(max(err_e)) if err_e else None},
         "server": {"peak_resident_bytes": info["peak_resident_bytes"],
@@ -646,6 +652,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=16, help="CPU baseline: at most this many NumCalc processes")
     ap.add_argument("--no-autotune", action="store_true",
                     help="served run: keep the default launch shapes instead of measuring them per operator shape")
+    ap.add_argument("--refinement-tolerance", type=float, default=None,
+                    help="served mixed form: FP64 refinement residual target (default %g)" % SERVED["refinement_tolerance"])
     ap.add_argument("--tuning-file", default=None,
                     help="served run: launch shapes per operator shape to load (and to save what is tuned)")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
@@ -660,6 +668,8 @@ def main():
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--check", type=int, default=16, help="batched: every check-th mesh is verified against FP64")
     a = ap.parse_args()
+    if a.refinement_tolerance is not None:
+        SERVED["refinement_tolerance"] = a.refinement_tolerance
     if a.gpus != 1:
         print("bench.py measures one GPU: the solve shards by independent (frequency, source) jobs with no "
               "communication, so only --gpus 1 is run here", file=sys.stderr)

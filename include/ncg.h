@@ -317,6 +317,9 @@ typedef struct {
     int32_t iterations;                 /* Krylov iterations summed over steps and load cases */
     double latency_ms;                  /* submit -> last step done                          */
     double queue_ms;                    /* submit -> first step started                      */
+    int32_t refinement_steps_max;       /* mixed form: most refinement steps of a step       */
+    double relative_residual_max;       /* worst final relative residual of a step and load
+                                           case (the FP64 refinement residual, mixed form)   */
     char error[256];
 } ncg_job_status;
 

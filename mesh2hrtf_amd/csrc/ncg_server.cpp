@@ -72,6 +72,8 @@ struct Job {
     int inflight = 0;      // steps of this job running on the lanes
     int freq_done = 0, state = 0, error_code = 0;
     long long iterations = 0;
+    int refine_max = 0;
+    double relres_max = 0.0;
     std::string error;
 };
 
@@ -290,6 +292,8 @@ struct Taken {
     int rc = 0;
     std::string msg;
     long long its = 0;
+    int refine = 0;        // refinement steps of the mixed form
+    double relres = 0.0;   // worst final relative residual over the load cases
 };
 Taken take(ncg_server *S, int q, Res *r, const std::vector<Res *> &evict, size_t need, std::vector<Res *> &dead)
 {
@@ -409,7 +413,11 @@ void run_lane(ncg_server *S)
             const int rc = ncg_solve_group(P.data(), (int32_t)P.size(), &J0->solver, ss.data());
             const std::string msg = rc ? ncg_last_error() : "";
             for (size_t i = 0; i < ok.size(); ++i) {
-                for (int c = 0; c < ok[i]->st.job->nrhs && c < 64; ++c) ok[i]->its += ss[i].iterations[c];
+                for (int c = 0; c < ok[i]->st.job->nrhs && c < 64; ++c) {
+                    ok[i]->its += ss[i].iterations[c];
+                    ok[i]->relres = std::max(ok[i]->relres, ss[i].relative_residual[c]);
+                }
+                ok[i]->refine = ss[i].refinement_steps;
                 if (rc == NCG_E_NOCONV ? !ss[i].converged_all : rc != 0) {
                     ok[i]->rc = rc;
                     ok[i]->msg = msg;
@@ -420,7 +428,11 @@ void run_lane(ncg_server *S)
             Taken &t = *ok[0];
             t.rc = ncg_solve(t.r->p, &t.st.job->solver, &ss);
             if (t.rc) t.msg = ncg_last_error();
-            for (int c = 0; c < t.st.job->nrhs && c < 64; ++c) t.its += ss.iterations[c];
+            for (int c = 0; c < t.st.job->nrhs && c < 64; ++c) {
+                t.its += ss.iterations[c];
+                t.relres = std::max(t.relres, ss.relative_residual[c]);
+            }
+            t.refine = ss.refinement_steps;
         }
         const clk::time_point t_solve = clk::now();
         // outputs: the solutions, then every grid field queued before the first is read
@@ -496,6 +508,8 @@ void run_lane(ncg_server *S)
             S->steps_done++;
             S->solves_done += J->nrhs;
             J->iterations += t.its;
+            J->refine_max = std::max(J->refine_max, t.refine);
+            J->relres_max = std::max(J->relres_max, t.relres);
             J->inflight--;
             if (t.rc) {
                 fail_job(S, J, t.rc, t.msg);
@@ -516,6 +530,8 @@ void fill_status(const Job *J, ncg_job_status *st)
     st->error_code = J->error_code;
     st->freq_done = J->freq_done;
     st->iterations = (int32_t)std::min<long long>(J->iterations, 0x7fffffff);
+    st->refinement_steps_max = J->refine_max;
+    st->relative_residual_max = J->relres_max;
     if (J->state >= 2) st->latency_ms = ms_between(J->t_submit, J->t_done);
     if (J->started) st->queue_ms = ms_between(J->t_submit, J->t_first);
     snprintf(st->error, sizeof(st->error), "%s", J->error.c_str());

@@ -129,6 +129,7 @@ class _Job(ctypes.Structure):
 class _JobStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("error_code", ctypes.c_int32), ("freq_done", ctypes.c_int32),
                 ("iterations", ctypes.c_int32), ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("refinement_steps_max", ctypes.c_int32), ("relative_residual_max", ctypes.c_double),
                 ("error", ctypes.c_char * 256)]
 
 
@@ -341,6 +342,7 @@ def autotune(problems, reps=3):
 def _status_dict(st):
     return dict(state=("queued", "running", "done", "failed")[st.state], error_code=st.error_code,
                 freq_done=st.freq_done, iterations=st.iterations, latency_ms=st.latency_ms, queue_ms=st.queue_ms,
+                refinement_steps_max=st.refinement_steps_max, relative_residual_max=st.relative_residual_max,
                 error=st.error.decode())
 
 
