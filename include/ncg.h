@@ -186,7 +186,10 @@ typedef struct {
                                            1, 2, 4 or 8 (default 8)                          */
     int32_t tuned;                      /* 1 once chosen by ncg_autotune                      */
     int32_t near_mode;                  /* near-field work-group: 0 one row chunk (default),
-                                           1 one finest cluster (x gathered once)           */
+                                           1 one finest cluster (x gathered once), 2 and 3
+                                           persistent work-groups over the row chunks of
+                                           every member: 2 from work queues (one per XCD),
+                                           3 by a static stride                              */
     int32_t mfma_splits;                /* dense, more than 4 load cases: column splits of the
                                            matrix-core product, 1..16 (0: from the size)     */
     int32_t assemble_rows;              /* dense: collocation rows per work-group of the

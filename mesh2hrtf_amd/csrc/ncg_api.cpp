@@ -410,6 +410,7 @@ struct ncg_problem {
     char *gstage = nullptr;
     size_t gstage_cap = 0;
     void *gpin = nullptr;
+    int *queue = nullptr;   // work-queue counters of the persistent launches this problem leads (zero between launches)
     hipEvent_t gev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t join = nullptr;
     // per-kernel timing of the operator apply (off unless requested)
@@ -1955,7 +1956,7 @@ void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vecto
             g.y = out[i];
             g.skip = skip[i];
             it.push_back(g);
-            cnt.push_back(lead_tune_near(P) ? F.ncl_fine : F.nd_n);
+            cnt.push_back(lead_tune_near(P) == 1 ? F.ncl_fine : F.nd_n);
             seg = std::max(seg, std::max(64, std::min(32768 / (16 * nr), (F.near_wmax + 63) / 64 * 64)));
             bytes += (double)F.nnz * (mixed ? 8.0 : 16.0) + 2.0 * nr * p->n * 16.0;
         }
@@ -2045,7 +2046,7 @@ void run_apply(const Stage &st, ncg_problem *lead, const std::vector<ncg_problem
         switch (L.kind) {
         case 0:
             ncgk::g_near_spmv(s, L.cls, lead->tune.near_mode, L.nr, L.seg, st.dev<ncgk::GNear>(L.items),
-                              st.dev<int>(L.start), L.nitems, L.blocks);
+                              st.dev<int>(L.start), L.nitems, L.blocks, lead->queue);
             break;
         case 1:
             ncgk::g_apply_T(s, L.cls < 2, L.cls == 1, lead->tune.t_waves, L.nr, st.dev<ncgk::GLevel>(L.items),
@@ -2710,7 +2711,7 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
         (rc = dalloc(&p->nrm, 3 * (size_t)n)) || (rc = dalloc(&p->area, n)) || (rc = dalloc(&p->jac, n)) ||
         (rc = dalloc(&p->nv, n)) || (rc = dalloc(&p->kind, n)) || (rc = dalloc(&p->eval, 3 * (size_t)p->m)) ||
         (rc = dalloc(&p->adm, 2 * (size_t)n)) || (rc = dalloc(&p->err, 1)) || (rc = dalloc(&p->evals, (size_t)kQuadKinds)) ||
-        (rc = dalloc(&p->dvec, n)) || (rc = dalloc(&p->active, 64)))
+        (rc = dalloc(&p->dvec, n)) || (rc = dalloc(&p->active, 64)) || (rc = dalloc(&p->queue, 320)))
         return bail(rc);
     const int nb = ncgk::dot_blocks(n);
     // partial sums of the dot kernels: 3 dots x 64 load cases per block, and at least
@@ -2725,6 +2726,7 @@ int ncg_problem_create(const ncg_geometry *g, int32_t device, ncg_problem **out)
     hipMemcpyAsync(p->kind, kh.data(), n, hipMemcpyHostToDevice, s);
     if (p->m) hipMemcpyAsync(p->eval, g->eval_xyz, sizeof(double) * 3 * p->m, hipMemcpyHostToDevice, s);
     hipMemsetAsync(p->err, 0, sizeof(int), s);
+    hipMemsetAsync(p->queue, 0, 320 * sizeof(int), s);
     ncgk::element_geometry(s, n, p->X, p->nv, p->cen, p->nrm, p->area, p->jac);
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(fail(NCG_E_DEVICE, std::string("geometry kernels: ") + hipGetErrorString(e)));
@@ -2741,7 +2743,8 @@ int ncg_problem_destroy(ncg_problem *p)
                     (void *)p->nv, (void *)p->kind, (void *)p->near_rows, (void *)p->near_cols, p->A, (void *)p->adm,
                     (void *)p->bcval, (void *)p->rhs, (void *)p->rhs_src, (void *)p->x, (void *)p->ps, (void *)p->pw,
                     (void *)p->src, (void *)p->err, (void *)p->evals, (void *)p->scal, (void *)p->partial,
-                    (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis, (void *)p->npart})
+                    (void *)p->dotout, (void *)p->dvec, (void *)p->active, (void *)p->basis, (void *)p->npart,
+                    (void *)p->queue})
         if (q) hipFree(q);
     for (double *w : p->work) hipFree(w);
     p->fmm.release();
@@ -2956,7 +2959,7 @@ int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t)
     if (!p || !t) return fail(NCG_E_ARG, "ncg_set_tuning: null argument");
     if (!valid_waves(t->t_waves) || !valid_waves(t->d_waves) || !valid_waves(t->s_waves))
         return fail(NCG_E_ARG, "ncg_set_tuning: waves per work-group must be 1, 2, 4 or 8");
-    if (t->near_mode != 0 && t->near_mode != 1) return fail(NCG_E_ARG, "ncg_set_tuning: near_mode must be 0 or 1");
+    if (t->near_mode < 0 || t->near_mode > 3) return fail(NCG_E_ARG, "ncg_set_tuning: near_mode must be 0, 1, 2 or 3");
     if (t->mfma_splits < 0 || t->mfma_splits > 16) return fail(NCG_E_ARG, "ncg_set_tuning: mfma_splits must be 0..16");
     if (t->assemble_rows != 0 && (t->assemble_rows < 8 || t->assemble_rows > 256))
         return fail(NCG_E_ARG, "ncg_set_tuning: assemble_rows must be 0 or 8..256");
@@ -3015,7 +3018,7 @@ int autotune_group(const std::vector<ncg_problem *> &P, int reps, ncg_tuning &be
     if (fmm) {
         // near-field mode (changes no value)
         double tbest = 1e300;
-        for (int mode = 0; mode < 2; ++mode) {
+        for (int mode = 0; mode < 4; ++mode) {
             lead->tune.near_mode = mode;
             Stage st(lead);
             ApplyPlan ap;

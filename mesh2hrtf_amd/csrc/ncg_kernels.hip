@@ -7,6 +7,10 @@
 //   element data: structure of arrays (see ncg::Elements)
 #include "ncg_kernels.hpp"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "ncg_device.hpp"
 
 namespace ncg {
@@ -2579,6 +2583,146 @@ __global__ void __launch_bounds__(256) kg_near_cluster(const ncgk::GNear *items,
                              (const cx<S> *)it.val, (const cx<double> *)it.x, (cx<double> *)it.y);
 }
 
+// Near field as a persistent launch: a grid sized to what the device holds at once walks
+// the row chunks of every problem of the group, so no work-group waits for a launch slot
+// and the next chunk's first values are in flight while the current chunk's rows are
+// reduced and stored.  Two schedules: DYN, work queues, one per XCD (work-groups are
+// dispatched to the 8 XCDs in turn, blockIdx.x % 8, and XCD x takes chunks from the x-th
+// eighth of the list by an atomic counter queue[32 x]: device-scope atomics are served
+// beside memory, so one counter per XCD spreads them over 8 lines); or a static stride
+// (work-group b takes chunks b, b + grid, ...).  Every row sums as in near_spmv_item: the
+// same lanes take the same columns in the same order, and the wave sum is the same.  With
+// DYN the last work-group to leave returns the counters to zero, so the next launch on the
+// stream starts from empty queues.
+constexpr int kQueueXcds = 8, kQueueStride = 32, kQueueInts = kQueueStride * (kQueueXcds + 1);
+
+template <typename S, int RW>
+struct NearUnit {
+    int q, w, n;
+    long long cb;
+    int row[RW];
+    const cx<S> *vr[RW];
+    const int *col;
+    const cx<double> *x;
+    cx<double> *y;
+};
+
+template <typename S, int RW>
+__device__ __forceinline__ void near_unit(const ncgk::GNear *items, const int *start, int nitems, int b, int wv,
+                                          NearUnit<S, RW> &u)
+{
+    const int d = g_item(start, nitems, b);
+    const ncgk::GNear it = items[d];
+    u.q = b - start[d];
+    u.n = it.n;
+    u.cb = it.cb[u.q];
+    // a converged problem's chunks are empty units (no loads, no stores)
+    u.w = (it.skip && *it.skip) ? 0 : it.w[u.q];
+    u.col = it.col;
+    u.x = (const cx<double> *)it.x;
+    u.y = (cx<double> *)it.y;
+#pragma unroll
+    for (int j = 0; j < RW; ++j) {
+        const int a = ncgk::kNearRows * u.q + wv + 4 * j;
+        u.row[j] = it.row[a];
+        u.vr[j] = (const cx<S> *)it.val + it.off[a];
+    }
+}
+
+template <typename S, int NR, bool DYN>
+__global__ void __launch_bounds__(256) kq_near_spmv(const ncgk::GNear *items, const int *start, int nitems, int seg,
+                                                    int *queue)
+{
+    extern __shared__ double xs_sh[];   // [NR][seg] complex
+    cx<double> *xs = (cx<double> *)xs_sh;
+    __shared__ int s_next[2];
+    constexpr int RW = ncgk::kNearRowsPerWave, U = 2;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int total = start[nitems];
+    const int xcd = blockIdx.x % kQueueXcds;
+    const int lo = (int)((long long)total * xcd / kQueueXcds), hi = (int)((long long)total * (xcd + 1) / kQueueXcds);
+    int *ctr = queue + kQueueStride * xcd;
+    // the first unit; indices read back from LDS are uniform (readfirstlane keeps the unit's
+    // descriptor, rows and value addresses in scalar registers)
+    int b;
+    if (DYN) {
+        if (threadIdx.x == 0) s_next[0] = lo + atomicAdd(ctr, 1);
+        __syncthreads();
+        b = __builtin_amdgcn_readfirstlane(s_next[0]);
+        if (b >= hi) b = total;
+    } else {
+        b = blockIdx.x;
+    }
+    int slot = 1;
+    NearUnit<S, RW> cu;
+    cx<S> v[U][RW];
+    if (b < total) {
+        near_unit<S, RW>(items, start, nitems, b, wv, cu);
+        if (cu.w > 0) rows_load<S, RW, U>(cu.vr, 0, min(seg, cu.w), lane, v);
+    }
+    while (b < total) {   // uniform over the work-group
+        // DYN: the next unit's index, published by the barrier after the gather
+        if (DYN && threadIdx.x == 0) s_next[slot] = lo + atomicAdd(ctr, 1);
+        double re[RW][NR], im[RW][NR];
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
+        for (int s0 = 0; s0 < cu.w; s0 += seg) {
+            const int sl = min(seg, cu.w - s0);
+            const cx<S> *vs[RW];
+#pragma unroll
+            for (int j = 0; j < RW; ++j) vs[j] = cu.vr[j] + s0;
+            if (s0 > 0) {   // later segments (rows longer than seg): as near_spmv_item
+                __syncthreads();
+                rows_load<S, RW, U>(vs, 0, sl, lane, v);
+            }
+            for (int t = threadIdx.x; t < sl; t += 256) {
+                const int e = cu.col[cu.cb + s0 + t];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) xs[r * seg + t] = cu.x[(size_t)r * cu.n + e];
+            }
+            __syncthreads();
+            rows_dot<S, NR, RW, U>(vs, sl, xs, seg, lane, v, re, im);
+        }
+        if (cu.w == 0) __syncthreads();   // publish s_next as the gather's barrier would
+        int bn;
+        if (DYN) {
+            bn = __builtin_amdgcn_readfirstlane(s_next[slot]);
+            if (bn >= hi) bn = total;
+        } else {
+            bn = b + gridDim.x;
+        }
+        // the next unit's first values are requested before this unit's rows are summed
+        NearUnit<S, RW> nu;
+        if (bn < total) {
+            near_unit<S, RW>(items, start, nitems, bn, wv, nu);
+            if (nu.w > 0) rows_load<S, RW, U>(nu.vr, 0, min(seg, nu.w), lane, v);
+        }
+        if (cu.w > 0) {
+            wave_sum_rows<RW, NR>(re, im);
+            if (lane == 0)
+#pragma unroll
+                for (int j = 0; j < RW; ++j)
+                    if (cu.row[j] >= 0)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) cu.y[(size_t)r * cu.n + cu.row[j]] = mk(re[j][r], im[j][r]);
+        }
+        __syncthreads();   // xs and the other s_next slot are free again
+        cu = nu;
+        b = bn;
+        slot ^= 1;
+    }
+    if (DYN && threadIdx.x == 0) {
+        __threadfence();
+        int *done = queue + kQueueStride * kQueueXcds;
+        if (atomicAdd(done, 1) == (int)gridDim.x - 1) {
+            for (int x = 0; x < kQueueXcds; ++x) atomicExch(queue + kQueueStride * x, 0);
+            atomicExch(done, 0);
+        }
+    }
+}
+
 template <typename S, typename W, int NR, int NW>
 __global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
 {
@@ -3680,17 +3824,56 @@ void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double 
     default: CALL(4); break; \
     }
 
+// work-groups of a persistent launch: what the device holds at once (work-groups per CU of
+// the kernel at this LDS size, times the CUs), at most one per unit of work
+// (the occupancy query is made once per kernel, LDS size and device)
+template <typename K>
+static int resident_grid(K kernel, int lds, int units)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, int>, int> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const auto key = std::make_tuple((const void *)kernel, lds, dev);
+    int grid = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto f = known.find(key);
+        if (f != known.end()) grid = f->second;
+    }
+    if (!grid) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, lds) != hipSuccess || per < 1) per = 1;
+        grid = per * cus;
+        std::lock_guard<std::mutex> g(mu);
+        known[key] = grid;
+    }
+    return std::max(1, std::min(units, grid));
+}
+
 void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNear *items, const int *start,
-                 int nitems, int blocks)
+                 int nitems, int blocks, int *queue)
 {
     if (blocks <= 0) return;
+    const int lds = 16 * seg * std::max(1, std::min(nr, 4));
+#define NCG_QCALL(S, NR, DYN)                                                                                      \
+    {                                                                                                              \
+        const int g = resident_grid(kq_near_spmv<S, NR, DYN>, lds, blocks);                                        \
+        kq_near_spmv<S, NR, DYN><<<g, 256, lds, s>>>(items, start, nitems, seg, queue);                           \
+    }
 #define NCG_CALL(NR)                                                                                               \
-    if (mode && mixed) kg_near_cluster<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);  \
-    else if (mode) kg_near_cluster<double, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);      \
-    else if (mixed) kg_near_spmv<float, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);         \
-    else kg_near_spmv<double, NR><<<blocks, 256, 16 * seg * NR, s>>>(items, start, nitems, seg);
+    if (mode == 2 && mixed) NCG_QCALL(float, NR, true)                                                            \
+    else if (mode == 2) NCG_QCALL(double, NR, true)                                                               \
+    else if (mode == 3 && mixed) NCG_QCALL(float, NR, false)                                                      \
+    else if (mode == 3) NCG_QCALL(double, NR, false)                                                              \
+    else if (mode && mixed) kg_near_cluster<float, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);        \
+    else if (mode) kg_near_cluster<double, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);                \
+    else if (mixed) kg_near_spmv<float, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);                   \
+    else kg_near_spmv<double, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);
     NCG_G_NR(nr, NCG_CALL)
 #undef NCG_CALL
+#undef NCG_QCALL
 }
 
 // waves per work-group of the grouped far-field applies: 1, 2, 4 or 8

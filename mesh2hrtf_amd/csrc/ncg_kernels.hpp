@@ -281,9 +281,11 @@ struct GVec {
     const int *skip;
 };
 // launchers (blocks = start[nitems], the prefix the host uploaded)
-// mode 0: a work-group per row chunk; 1: per finest cluster (x gathered once for its chunks)
+// mode 0: a work-group per row chunk; 1: per finest cluster (x gathered once for its chunks);
+// 2, 3: persistent work-groups over the row chunks, 2 from per-XCD work queues (`queue`: 288
+// device ints, zero between launches; the launch leaves them zero), 3 by a static stride
 void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNear *items, const int *start,
-                 int nitems, int blocks);
+                 int nitems, int blocks, int *queue);
 // T and D of every item (each writes its own t or s); S adds into y, so a launch may hold at
 // most one level of a problem (the host launches one level index at a time).  sig64: t and
 // s in FP64 whatever the values' precision (the refinement residual).  nw: waves per

@@ -35,8 +35,8 @@ def test_near_mode_and_waves():
     probs = _fmm_group(ncg)
     opts = dict(solver=ncg.GMRES, precond=ncg.BLOCK, tolerance=1e-10)
     res = {}
-    for key, tune in (("default", {}), ("cluster", dict(near_mode=1)), ("waves4", dict(t_waves=4, d_waves=2,
-                                                                                      s_waves=4)),
+    for key, tune in (("default", {}), ("cluster", dict(near_mode=1)), ("queue", dict(near_mode=2)),
+                      ("queue4", dict(near_mode=3)), ("waves4", dict(t_waves=4, d_waves=2, s_waves=4)),
                       ("rows", dict(near_assembly=1))):
         for p in probs:
             p.set_tuning(**tune)
@@ -49,8 +49,9 @@ def test_near_mode_and_waves():
         st = ncg.solve_group(probs, **opts)
         assert all(s["converged"] for s in st)
         res[key] = [p.solution() for p in probs]
-    for a, b in zip(res["default"], res["cluster"]):
-        assert np.array_equal(a, b)
+    for other in ("cluster", "queue", "queue4"):
+        for a, b in zip(res["default"], res[other]):
+            assert np.array_equal(a, b), other
     for a, b in zip(res["default"], res["rows"]):
         assert np.array_equal(a, b)
     for a, b in zip(res["default"], res["waves4"]):
@@ -59,7 +60,7 @@ def test_near_mode_and_waves():
     t = ncg.autotune(probs, reps=2)
     print("chosen", t, flush=True)
     assert t["tuned"] == 1 and t["t_waves"] in (2, 4, 8) and t["d_waves"] in (2, 4, 8) and t["s_waves"] in (2, 4, 8)
-    assert t["near_mode"] in (0, 1) and t["near_assembly"] in (0, 1)
+    assert t["near_mode"] in (0, 1, 2, 3) and t["near_assembly"] in (0, 1)
     for p in probs:
         assert p.tuning() == t
     st = ncg.solve_group(probs, **opts)
@@ -68,6 +69,28 @@ def test_near_mode_and_waves():
         x = p.solution()
         for r in range(2):
             assert R.rel_l2(x[r], a[r]) < 1e-10
+    for p in probs:
+        p.close()
+
+
+def test_near_modes_mixed_precision():
+    """the FP32-valued near field in every work-group mode: the same mixed-form solutions bit
+    for bit (the persistent launches included; the work-queue mode runs twice in a row so
+    the second solve starts from the queues the first left)"""
+    from mesh2hrtf_amd import ncg
+    probs = _fmm_group(ncg, precision=ncg.MIXED)
+    opts = dict(solver=ncg.GMRES, precond=ncg.BLOCK, tolerance=1e-6, refinement_tolerance=1e-6)
+    res = {}
+    for key, mode in (("chunk", 0), ("cluster", 1), ("queue", 2), ("queue4", 3), ("queue_again", 2)):
+        for p in probs:
+            p.set_tuning(near_mode=mode)
+        print("near_mode", key, flush=True)
+        st = ncg.solve_group(probs, **opts)
+        assert all(s["converged"] for s in st)
+        res[key] = [p.solution() for p in probs]
+    for other in ("cluster", "queue", "queue4", "queue_again"):
+        for a, b in zip(res["chunk"], res[other]):
+            assert np.array_equal(a, b), other
     for p in probs:
         p.close()
 
