@@ -444,25 +444,37 @@ def head_main(a):
         n, m, nr, a.lanes, a.group, nf, a.steps), file=sys.stderr, flush=True)
 
     # ---- served mixed form: every step (sweep) submitted at once ------------------------
-    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=True, group=a.group, autotune=not a.no_autotune,
-                     tuning_file=a.tuning_file)
+    srv = ncg.Server(device=0, lanes=a.lanes, kernel_timing=False, group=a.group, autotune=not a.no_autotune,
+                     tuning_file=a.tuning_file, graphs=not a.eager)
     mid = srv.add_mesh(**geo)
     sweep = [freq_spec(src, f) for f in freqs]
     for j in [srv.submit(mid, sweep, **SERVED) for _ in range(a.warmup)]:
         j.wait()
-    srv.kernel_stats(reset=True)
     lane0 = dict(srv.stats()["lane_ms"])
     t0 = time.perf_counter()
     jobs = [srv.submit(mid, sweep, **SERVED) for _ in range(a.steps)]
     sts = [j.wait() for j in jobs]
     elapsed = time.perf_counter() - t0
-    kst = srv.kernel_stats()
     lane1 = srv.stats()["lane_ms"]
     failed = [s for s in sts if s["state"] != "done"]
     if failed:
         print("# served steps failed: %s" % failed[:3], file=sys.stderr, flush=True)
     print("# served: %d sweeps (%d frequency steps) in %.2f s, %.2f ms per frequency step" % (
         a.steps, a.steps * nf, elapsed, elapsed / (a.steps * nf) * 1e3), file=sys.stderr, flush=True)
+
+    # ---- the same sweeps again with HIP events around every kernel launch (the roofline's
+    # timing; events time single launches, so the Arnoldi steps run kernel by kernel here)
+    nroof = max(1, min(a.steps, a.roofline_steps))
+    srv.set_kernel_timing(True)
+    srv.kernel_stats(reset=True)
+    t0r = time.perf_counter()
+    rjobs = [srv.submit(mid, sweep, **SERVED) for _ in range(nroof)]
+    rsts = [j.wait() for j in rjobs]
+    elapsed_r = time.perf_counter() - t0r
+    kst = srv.kernel_stats()
+    srv.set_kernel_timing(False)
+    print("# timed-kernel pass: %d sweeps in %.2f s, %.2f ms per frequency step" % (
+        nroof, elapsed_r, elapsed_r / (nroof * nf) * 1e3), file=sys.stderr, flush=True)
 
     # ---- latency: single-frequency Poisson requests at 70% of the measured rate ---------
     rate = 0.7 * a.steps * nf / elapsed
@@ -534,8 +546,10 @@ def head_main(a):
                 "frac": ach / HBM_PEAK, "traffic": traffic, "traffic_source": tsrc,
                 "bytes_per_launch": s["bytes"] / s["calls"], "mean_launch_ms": s["total_ms"] / s["calls"],
                 "launches": s["calls"],
-                "timing": "HIP events around each launch on the group's stream over the timed region, "
-                          "%d lanes sharing the GPU" % a.lanes,
+                "timing": "HIP events around each launch on the group's stream, over %d further sweeps of the "
+                          "same workload run right after the timed region with per-launch events (%.2f ms per "
+                          "frequency step there); %d lanes sharing the GPU" % (
+                              nroof, elapsed_r / (nroof * nf) * 1e3, a.lanes),
                 "share_of_timed_kernel_ms": {k: round(kst[k]["total_ms"], 2) for k in sorted(kst)},
                 "timed_kernels": {k: {"calls": kst[k]["calls"], "total_ms": round(kst[k]["total_ms"], 3),
                                       "algorithmic_bytes": kst[k]["bytes"]} for k in sorted(kst) if kst[k]["bytes"]},
@@ -601,7 +615,9 @@ def head_main(a):
                     "one step = one pass over the %d frequencies x 2 sources" % (n, nf)),
                    "form": "served mixed precision: FP32 operator and inner GMRES(50) + block-Jacobi, FP64 refinement "
                            "to relative residual %g (residual with FP64 accumulation and FMM signatures); %d lanes, "
-                           "%d steps per grouped solve" % (SERVED["refinement_tolerance"], a.lanes, a.group),
+                           "%d steps per grouped solve; Arnoldi steps %s" % (
+                               SERVED["refinement_tolerance"], a.lanes, a.group,
+                               "launched kernel by kernel" if a.eager else "replayed as captured HIP graphs"),
                    "frequencies": nf},
         "p50_ms": float(np.percentile(pl, 50)),
         "p99_ms": float(np.percentile(pl, 99)),
@@ -627,6 +643,9 @@ def head_main(a):
         "mixed_vs_fp64_rel_l2": {"surface_max": float(max(err_s)) if err_s else None,
                                  "eval_grid_max": float(max(err_e)) if err_e else None},
         "server": {"peak_resident_bytes": info["peak_resident_bytes"],
+                   "arnoldi_steps": {"replayed_from_graphs": info["graph_steps"], "kernel_by_kernel": info["eager_steps"],
+                                     "note": "whole run: the timed sweeps replay captured graphs, the per-launch "
+                                             "timed pass and the tuning runs launch kernel by kernel"},
                    "lane_ms_timed": {k: round(lane1[k] - lane0[k], 1) for k in lane1},
                    "launch_shapes": {k: t for k, t in tunings} if tunings else "defaults (autotune off)"},
         "roofline": roof,
@@ -657,6 +676,10 @@ def main():
     ap.add_argument("--tuning-file", default=None,
                     help="served run: launch shapes per operator shape to load (and to save what is tuned)")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated kernel timing")
+    ap.add_argument("--eager", action="store_true",
+                    help="served run: launch the GMRES Arnoldi steps kernel by kernel instead of replaying graphs")
+    ap.add_argument("--roofline-steps", type=int, default=5,
+                    help="sweeps of the per-launch timed pass that gives the roofline (at most --steps)")
     ap.add_argument("--cpu-timeout", type=float, default=120.0)
     ap.add_argument("--stream-meshes", type=int, default=32, help="stream: number of meshes")
     ap.add_argument("--requests", type=int, default=96, help="stream: number of requests")

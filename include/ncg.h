@@ -208,6 +208,23 @@ int ncg_get_tuning(ncg_problem *p, ncg_tuning *t);
  * every problem and returns it in *out (or NULL). */
 int ncg_autotune(ncg_problem *const *problems, int32_t count, int32_t reps, ncg_tuning *out);
 
+/* Arnoldi steps of the GMRES solves (ncg_solve, ncg_solve_group) that p leads: replayed as
+ * HIP graphs (on, the default) or launched kernel by kernel (off).  A step's graph is built
+ * once from its launches (kernel nodes in launch order; no stream capture, so other
+ * threads' allocations and waits are unaffected).  Every launch of a step reads the step
+ * index from device memory, so one graph serves every step of a solve; graphs are cached
+ * per leader by a digest of all the launch arguments they hold and reused by later solves
+ * whose launches are the same.  Not used while kernel timing
+ * is on (its events time single launches) or when a member of the group is an FMM
+ * operator with more than 4 load cases.  Results are the same either way, bit for bit.
+ * Replaces the per-iteration host loop of NC_IterativeSolverCGS
+ * (src/NC_CommonFunctions.cpp:1126-1176); no counterpart for the graphs themselves. */
+int ncg_set_graphs(ncg_problem *p, int32_t on);
+/* Arnoldi steps of the solves p led since it was created: out[0] graphs captured,
+ * out[1] steps replayed from a graph, out[2] steps launched kernel by kernel, out[3]
+ * graphs held in p's cache.  No counterpart in the reference. */
+int ncg_graph_stats(ncg_problem *p, int64_t *out);
+
 /* The method and FMM plan of the last ncg_assemble (what NumCalc logs per step). */
 int ncg_problem_fmm_info(ncg_problem *p, ncg_fmm_info *info);
 
@@ -300,6 +317,8 @@ typedef struct {
                                            every later group of that shape                  */
     const char *tuning_file;            /* text file of tuned shapes read at creation and
                                            written at ncg_server_destroy, or NULL            */
+    int32_t eager;                      /* 1: Arnoldi steps launched kernel by kernel instead
+                                           of replayed as captured graphs (ncg_set_graphs)  */
 } ncg_server_config;
 
 typedef struct {
@@ -335,6 +354,8 @@ typedef struct {
     double lane_ms[4];                  /* lane wall time summed over the lanes: assembly,
                                            solve, outputs (solution and grid), waiting       */
     int32_t tuned_shapes;               /* operator shapes with a tuning (ncg_server_tuning)  */
+    int64_t graph_steps, eager_steps;   /* GMRES Arnoldi steps replayed from captured graphs,
+                                           and launched kernel by kernel (ncg_set_graphs)    */
 } ncg_server_info;
 
 int ncg_server_create(const ncg_server_config *cfg, ncg_server **out);
@@ -349,6 +370,9 @@ int ncg_server_stats(ncg_server *s, ncg_server_info *stats);
 /* Per-kernel timing summed over the steps finished since the last reset (requires
  * kernel_timing in the configuration). */
 int ncg_server_kernel_stats(ncg_server *s, ncg_kernel_stat *out, int32_t cap, int32_t *count, int32_t reset);
+/* Turns per-kernel timing of the steps started from now on on or off (the configuration's
+ * kernel_timing is the initial state). */
+int ncg_server_set_kernel_timing(ncg_server *s, int32_t on);
 /* The tuning of operator shape `index` (0 .. tuned_shapes - 1) and its key text. */
 int ncg_server_tuning(ncg_server *s, int32_t index, char *key, int32_t key_cap, ncg_tuning *tuning);
 /* Finishes the queued work, stops the lanes, frees every problem. */

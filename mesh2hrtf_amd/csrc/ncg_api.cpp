@@ -413,6 +413,13 @@ struct ncg_problem {
     int *queue = nullptr;   // work-queue counters of the persistent launches this problem leads (zero between launches)
     hipEvent_t gev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t join = nullptr;
+    // Arnoldi steps of the grouped solves this problem leads replayed as captured HIP graphs
+    // (off: launched one by one); executable graphs by the digest of every launch argument
+    // they hold, most recently used first; no_events: the apply records no timing events
+    // (while a step is being captured)
+    int graphs = 1, graph_fail = 0, no_events = 0;
+    std::vector<std::pair<unsigned long long, hipGraphExec_t>> gcache;
+    long long g_captured = 0, g_replayed = 0, g_eager = 0;   // graphs captured, steps replayed / launched one by one
     // per-kernel timing of the operator apply (off unless requested)
     int ktime = 0;
     std::vector<EventPair> kpool;
@@ -523,7 +530,7 @@ struct MvScratch {
 void mv(ncg_problem *p, int acc64, const double *x, const double *d, double *y, const MvScratch *na = nullptr)
 {
     EventPair *e = nullptr;
-    if (!na) {
+    if (!na && !p->no_events) {
         if (p->mv_used == p->ev_mv.size()) {
             EventPair ep;
             hipEventCreate(&ep.a);
@@ -2075,8 +2082,9 @@ struct BjPlan {
     };
     std::vector<Launch> seq;
 };
+// kstride: x is the member's Krylov basis, offset per Arnoldi step by the device step counter
 void plan_bj(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<const double *> &x,
-             const std::vector<double *> &y, const std::vector<int *> &skip, BjPlan &B)
+             const std::vector<double *> &y, const std::vector<int *> &skip, BjPlan &B, bool kstride = false)
 {
     B.seq.clear();
     const int nrhs = P[0]->nrhs;
@@ -2109,6 +2117,7 @@ void plan_bj(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<c
                     g.minv = F.bj_blk.p;
                 }
                 g.x = x[i] + 2 * (size_t)r0 * p->n;
+                g.xk = kstride ? 2 * (long long)p->nrhs * p->n : 0;
                 g.y = y[i] + 2 * (size_t)r0 * p->n;
                 g.skip = skip[i];
                 it.push_back(g);
@@ -2120,10 +2129,10 @@ void plan_bj(Stage &st, const std::vector<ncg_problem *> &P, const std::vector<c
         }
     }
 }
-void run_bj(const Stage &st, hipStream_t s, const BjPlan &B)
+void run_bj(const Stage &st, hipStream_t s, const BjPlan &B, const int *kp = nullptr)
 {
     for (const BjPlan::Launch &L : B.seq)
-        ncgk::g_bj_apply(s, L.f32, L.nr, st.dev<ncgk::GBj>(L.items), st.dev<int>(L.start), L.nitems, L.blocks);
+        ncgk::g_bj_apply(s, L.f32, L.nr, st.dev<ncgk::GBj>(L.items), st.dev<int>(L.start), L.nitems, L.blocks, kp);
 }
 
 // vector operation of every member: y = a | a - b | y + a | 0 (complex entries nrhs n)
@@ -2132,7 +2141,8 @@ struct VecOp {
     size_t items = 0, start = 0;
 };
 VecOp plan_vec(Stage &st, int op, const std::vector<ncg_problem *> &P, const std::vector<const double *> &a,
-               const std::vector<const double *> &b, const std::vector<double *> &y, const std::vector<int *> &skip)
+               const std::vector<const double *> &b, const std::vector<double *> &y, const std::vector<int *> &skip,
+               bool kstride = false)
 {
     std::vector<ncgk::GVec> it;
     std::vector<int> cnt;
@@ -2140,6 +2150,7 @@ VecOp plan_vec(Stage &st, int op, const std::vector<ncg_problem *> &P, const std
         ncgk::GVec g;
         g.len = (long long)P[i]->nrhs * P[i]->n;
         g.a = a.empty() ? nullptr : a[i];
+        g.ak = kstride ? 2 * g.len : 0;
         g.b = b.empty() ? nullptr : b[i];
         g.y = y[i];
         g.skip = skip.empty() ? nullptr : skip[i];
@@ -2155,9 +2166,9 @@ VecOp plan_vec(Stage &st, int op, const std::vector<ncg_problem *> &P, const std
     v.start = st.put(pre);
     return v;
 }
-void run_vec(const Stage &st, hipStream_t s, const VecOp &v)
+void run_vec(const Stage &st, hipStream_t s, const VecOp &v, const int *kp = nullptr)
 {
-    ncgk::g_vec(s, v.op, st.dev<ncgk::GVec>(v.items), st.dev<int>(v.start), v.nitems, v.blocks);
+    ncgk::g_vec(s, v.op, st.dev<ncgk::GVec>(v.items), st.dev<int>(v.start), v.nitems, v.blocks, kp);
 }
 
 // squared norms of every member's vectors v into the leader's pinned norms [member][64]
@@ -2192,7 +2203,8 @@ NormOp plan_norms(Stage &st, const std::vector<ncg_problem *> &P, const std::vec
 // [kGroupMax][64] norms and residual estimates (doubles), iterations (ints), and
 // [kFlagDepth][kGroupMax] flags.
 constexpr size_t kTailNorms = 0, kTailRel = 8 * 64 * kGroupMax, kTailIts = 16 * 64 * kGroupMax,
-                 kTailFlags = 20 * 64 * kGroupMax, kLeadTail = kTailFlags + 4 * kFlagDepth * kGroupMax;
+                 kTailFlags = 20 * 64 * kGroupMax, kTailK = kTailFlags + 4 * kFlagDepth * kGroupMax,
+                 kLeadTail = kTailK + 256;
 char *lead_tail(ncg_problem *lead) { return (char *)lead->gm.p + lead->gm.cap - kLeadTail; }
 
 // runs the norms and waits for them: out[i][r] = |v_i,r|
@@ -2213,6 +2225,125 @@ int run_norms(const Stage &st, ncg_problem *lead, const NormOp &o, std::vector<s
     return 0;
 }
 
+
+// ---- captured Arnoldi steps
+// A step may be captured when none of its launches is timed on its own (kernel timing
+// records events around single launches) and every member applied on its own is a dense
+// operator (its apply's launch arguments are few and go into the digest).
+bool graphs_allowed(const std::vector<ncg_problem *> &P)
+{
+    if (!P[0]->graphs || P[0]->graph_fail) return false;
+    for (ncg_problem *p : P)
+        if (p->ktime || (p->plan.method != 0 && p->nrhs > 4)) return false;
+    return true;
+}
+
+void key_bj(Fnv &key, const BjPlan &B)
+{
+    key.add((long long)B.seq.size());
+    for (const BjPlan::Launch &L : B.seq) {
+        key.add(L.f32);
+        key.add(L.nr);
+        key.add(L.nitems);
+        key.add(L.blocks);
+        key.add((long long)L.items);
+        key.add((long long)L.start);
+    }
+}
+
+// the digest of an apply's launch arguments; a dense member applied on its own gets its
+// matrix-core buffers sized here, before any capture, so its apply allocates nothing
+void key_apply(Fnv &key, ncg_problem *lead, const std::vector<ncg_problem *> &P, const std::vector<const double *> &in,
+               const std::vector<double *> &out, const ApplyPlan &A)
+{
+    key.add((long long)(uintptr_t)lead->queue);
+    key.add(lead->tune.near_mode);
+    key.add(lead->tune.t_waves);
+    key.add(lead->tune.d_waves);
+    key.add(lead->tune.s_waves);
+    key.add(A.acc64);
+    key.add(A.sig64);
+    key.add((long long)A.seq.size());
+    for (const ApplyPlan::Launch &L : A.seq) {
+        key.add(L.kind);
+        key.add(L.cls);
+        key.add(L.nr);
+        key.add(L.seg);
+        key.add(L.nitems);
+        key.add(L.blocks);
+        key.add(L.kid);
+        key.add((long long)L.items);
+        key.add((long long)L.start);
+    }
+    for (int i : A.single) {
+        ncg_problem *p = P[i];
+        const int mixed = p->precision == NCG_MIXED;
+        const bool fp64acc = !mixed || A.acc64;
+        const size_t el = fp64acc ? sizeof(double) : sizeof(float);
+        if (p->nrhs > 4) {
+            const ncgk::MfmaPlan mp = ncgk::mfma_plan(p->n, fp64acc, p->tune.mfma_splits);
+            if (p->mm_xh.ensure(el * mp.xh_elems) || p->mm_part.ensure(el * mp.part_elems)) key.add(-1);
+        }
+        key.add((long long)(uintptr_t)p);
+        key.add((long long)(uintptr_t)p->A);
+        key.add(p->n);
+        key.add(p->nrhs);
+        key.add(mixed);
+        key.add(p->tune.mfma_splits);
+        key.add((long long)(uintptr_t)p->mm_xh.p);
+        key.add((long long)p->mm_xh.cap);
+        key.add((long long)(uintptr_t)p->mm_part.p);
+        key.add((long long)p->mm_part.cap);
+        key.add((long long)(uintptr_t)in[i]);
+        key.add((long long)(uintptr_t)out[i]);
+    }
+}
+
+constexpr size_t kGraphCache = 32;
+
+// The executable graph of `launch` (the launches it makes on the leader's stream) for the
+// digest `key`: from the leader's cache, or recorded now — the launches become a chain of
+// kernel nodes (ncgk::set_graph_recorder) without stream capture, so the other lanes'
+// allocations and waits meanwhile are unaffected.  nullptr where recording fails: the
+// caller then launches one by one, and the leader stops trying.
+template <typename F>
+hipGraphExec_t cached_graph(ncg_problem *lead, const std::vector<ncg_problem *> &P, unsigned long long key,
+                            F &&launch)
+{
+    std::vector<std::pair<unsigned long long, hipGraphExec_t>> &c = lead->gcache;
+    for (size_t i = 0; i < c.size(); ++i)
+        if (c[i].first == key) {
+            std::rotate(c.begin(), c.begin() + i, c.begin() + i + 1);
+            return c[0].second;
+        }
+    ncgk::GraphRecorder rec;
+    hipError_t e = hipGraphCreate(&rec.graph, 0);
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) {
+        for (ncg_problem *p : P) p->no_events = 1;
+        ncgk::set_graph_recorder(&rec);
+        launch();
+        ncgk::set_graph_recorder(nullptr);
+        for (ncg_problem *p : P) p->no_events = 0;
+        e = rec.err;
+        if (e == hipSuccess && rec.nodes > 0) e = hipGraphInstantiate(&x, rec.graph, nullptr, nullptr, 0);
+        hipGraphDestroy(rec.graph);
+    }
+    if (e != hipSuccess || !x) {
+        (void)hipGetLastError();
+        lead->graph_fail = 1;
+        return nullptr;
+    }
+    if (c.size() >= kGraphCache) {
+        // the evicted graph may still run on the stream
+        hipStreamSynchronize(lead->stream);
+        hipGraphExecDestroy(c.back().second);
+        c.pop_back();
+    }
+    c.insert(c.begin(), std::make_pair(key, x));
+    lead->g_captured++;
+    return x;
+}
 
 // Restarted GMRES(m) of every member on its system A x = b (b, x: device [nrhs][n]), x
 // starting from zero, per member relative tolerance tol[i]; bj: right block-Jacobi
@@ -2254,7 +2385,7 @@ int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const dou
         if ((rc = p->gm.ensure(lay[i].total + (i == 0 ? kLeadTail : 0)))) return rc;
     }
     char *tail = lead_tail(lead);
-    int *dflags = (int *)(tail + kTailFlags), *dits = (int *)(tail + kTailIts);
+    int *dflags = (int *)(tail + kTailFlags), *dits = (int *)(tail + kTailIts), *kstep = (int *)(tail + kTailK);
     double *drel = (double *)(tail + kTailRel);
     // descriptors
     Stage st(lead);
@@ -2309,14 +2440,16 @@ int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const dou
     const VecOp v_w_b = plan_vec(st, ncgk::kVecCopy, P, bc, {}, W, {});
     const VecOp v_w_res = plan_vec(st, ncgk::kVecSub, P, bc, std::vector<const double *>(T.begin(), T.end()), W, skip);
     const VecOp v_x_add = plan_vec(st, ncgk::kVecAdd, P, std::vector<const double *>(T.begin(), T.end()), {}, xv, {});
-    // per Arnoldi step k: Z = M^-1 V_k (or Z = V_k), then W = A Z
-    std::vector<BjPlan> bjk(bj ? m : 0);
-    std::vector<VecOp> cpk(bj ? 0 : m);
-    for (int k = 0; k < m; ++k) {
-        std::vector<const double *> Vk(G);
-        for (int i = 0; i < G; ++i) Vk[i] = P[i]->basis + 2 * (size_t)k * P[i]->nrhs * P[i]->n;
-        if (bj) plan_bj(st, P, Vk, Z, skip, bjk[k]);
-        else cpk[k] = plan_vec(st, ncgk::kVecCopy, P, Vk, {}, Z, skip);
+    // one Arnoldi step, the same launches for every k (the kernels read k from the device
+    // counter kstep): Z = M^-1 V_k (or Z = V_k), W = A Z, two Gram-Schmidt passes, V_k+1,
+    // the Hessenberg column and its rotations, the done flags into slot k % kFlagDepth
+    BjPlan bj_step;
+    VecOp cp_step;
+    {
+        std::vector<const double *> Vb(G);
+        for (int i = 0; i < G; ++i) Vb[i] = P[i]->basis;
+        if (bj) plan_bj(st, P, Vb, Z, skip, bj_step, true);
+        else cp_step = plan_vec(st, ncgk::kVecCopy, P, Vb, {}, Z, skip, true);
     }
     BjPlan bj_end;
     if (bj) plan_bj(st, P, Zc, T, noskip, bj_end);
@@ -2327,6 +2460,42 @@ int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const dou
     const ncgk::GGm *dgm = st.dev<ncgk::GGm>(o_gm);
     const int *d_md = st.dev<int>(o_md), *d_gs = st.dev<int>(o_gs), *d_vec = st.dev<int>(o_vec);
     int *pin_flags = (int *)((char *)lead->gpin + kPinFlags);
+    auto arnoldi_step = [&]() {
+        if (bj) run_bj(st, s, bj_step, kstep);
+        else run_vec(st, s, cp_step, kstep);
+        run_apply(st, lead, P, Zc, W, ap_step);
+        ncgk::g_multidot(s, kstep, dgm, d_md, G, ncgk::kMdLanes * p_md.back());
+        ncgk::g_gs_update(s, 0, kstep, 0, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_multidot(s, kstep, dgm, d_md, G, ncgk::kMdLanes * p_md.back());
+        ncgk::g_gs_update(s, 0, kstep, 1, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gs_normalize(s, 0, kstep, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gm_rotate(s, 0, kstep, m, dgm, G);
+        ncgk::g_gm_flags(s, dgm, G, dflags, kstep, kFlagDepth);
+    };
+    // the step as a captured graph, where the group allows it (none of its launches is
+    // timed on its own)
+    hipGraphExec_t step_graph = nullptr;
+    if (graphs_allowed(P)) {
+        Fnv key;
+        key.add((long long)m);
+        key.add((long long)bj);
+        key.add((long long)acc64);
+        key.add((long long)G);
+        key.add((long long)(uintptr_t)lead->gdesc.p);
+        key.add((long long)(uintptr_t)tail);
+        key.add((long long)o_gm);
+        key.add((long long)o_md);
+        key.add((long long)o_gs);
+        key.add((long long)p_md.back());
+        key.add((long long)p_gs.back());
+        key_bj(key, bj_step);
+        key.add((long long)cp_step.nitems);
+        key.add((long long)cp_step.blocks);
+        key.add((long long)cp_step.items);
+        key.add((long long)cp_step.start);
+        key_apply(key, lead, P, Zc, W, ap_step);
+        step_graph = cached_graph(lead, P, key.h, arnoldi_step);
+    }
 
     run_vec(st, s, v_zero);
     ncgk::g_gm_clear(s, dgm, G);
@@ -2340,10 +2509,10 @@ int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const dou
             run_apply(st, lead, P, xc, T, ap_res);
             run_vec(st, s, v_w_res);
         }
-        // |W|, V_0 = W / |W|, beta and the done flags of this cycle
-        ncgk::g_gs_update(s, 0, 1, m, dgm, d_gs, G, p_gs.back());
-        ncgk::g_gs_normalize(s, -1, m, dgm, d_gs, G, p_gs.back());
-        ncgk::g_gm_start(s, m, first ? 1 : 0, dgm, G);
+        // |W|, V_0 = W / |W|, beta and the done flags of this cycle; the step counter to 0
+        ncgk::g_gs_update(s, 0, nullptr, 1, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gs_normalize(s, -1, nullptr, m, dgm, d_gs, G, p_gs.back());
+        ncgk::g_gm_start(s, m, first ? 1 : 0, dgm, G, kstep);
         first = false;
         // Arnoldi steps; the flags of step k are read after step k + kFlagDepth - 1 is queued
         int launched = 0, checked = 0;
@@ -2358,21 +2527,19 @@ int gmres_group(const std::vector<ncg_problem *> &P, const std::vector<const dou
             return 0;
         };
         // the flags after the cycle start
-        ncgk::g_gm_flags(s, dgm, G, dflags);
-        NCG_HIP(hipMemcpyAsync(pin_flags + (kFlagDepth - 1) * kGroupMax, dflags, 4 * G, hipMemcpyDeviceToHost, s));
+        ncgk::g_gm_flags(s, dgm, G, dflags + (kFlagDepth - 1) * kGroupMax);
+        NCG_HIP(hipMemcpyAsync(pin_flags + (kFlagDepth - 1) * kGroupMax, dflags + (kFlagDepth - 1) * kGroupMax, 4 * G,
+                               hipMemcpyDeviceToHost, s));
         NCG_HIP(hipEventRecord(lead->gev[kFlagDepth - 1], s));
         if ((rc = check(kFlagDepth - 1))) return rc;
         for (int k = 0; k < m && total < maxit && !all_done; ++k) {
-            if (bj) run_bj(st, s, bjk[k]);
-            else run_vec(st, s, cpk[k]);
-            run_apply(st, lead, P, Zc, W, ap_step);
-            ncgk::g_multidot(s, k + 1, dgm, d_md, G, (k + 1) * p_md.back());
-            ncgk::g_gs_update(s, k + 1, 0, m, dgm, d_gs, G, p_gs.back());
-            ncgk::g_multidot(s, k + 1, dgm, d_md, G, (k + 1) * p_md.back());
-            ncgk::g_gs_update(s, k + 1, 1, m, dgm, d_gs, G, p_gs.back());
-            ncgk::g_gs_normalize(s, k, m, dgm, d_gs, G, p_gs.back());
-            ncgk::g_gm_rotate(s, k, m, dgm, G);
-            ncgk::g_gm_flags(s, dgm, G, dflags + (k % kFlagDepth) * kGroupMax);
+            if (step_graph) {
+                NCG_HIP(hipGraphLaunch(step_graph, s));
+                lead->g_replayed++;
+            } else {
+                arnoldi_step();
+                lead->g_eager++;
+            }
             NCG_HIP(hipMemcpyAsync(pin_flags + (k % kFlagDepth) * kGroupMax, dflags + (k % kFlagDepth) * kGroupMax,
                                    4 * G, hipMemcpyDeviceToHost, s));
             NCG_HIP(hipEventRecord(lead->gev[k % kFlagDepth], s));
@@ -2771,6 +2938,9 @@ int ncg_problem_destroy(ncg_problem *p)
     for (hipEvent_t e : p->gev)
         if (e) hipEventDestroy(e);
     if (p->join) hipEventDestroy(p->join);
+    if (p->stream) hipStreamSynchronize(p->stream);
+    for (auto &g : p->gcache) hipGraphExecDestroy(g.second);
+    p->gcache.clear();
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
     return 0;
@@ -2966,6 +3136,24 @@ int ncg_set_tuning(ncg_problem *p, const ncg_tuning *t)
     if (t->near_assembly != 0 && t->near_assembly != 1)
         return fail(NCG_E_ARG, "ncg_set_tuning: near_assembly must be 0 or 1");
     p->tune = *t;
+    return 0;
+}
+
+int ncg_set_graphs(ncg_problem *p, int32_t on)
+{
+    if (!p) return fail(NCG_E_ARG, "ncg_set_graphs: null problem");
+    p->graphs = on ? 1 : 0;
+    p->graph_fail = 0;
+    return 0;
+}
+
+int ncg_graph_stats(ncg_problem *p, int64_t *out)
+{
+    if (!p || !out) return fail(NCG_E_ARG, "ncg_graph_stats: null argument");
+    out[0] = p->g_captured;
+    out[1] = p->g_replayed;
+    out[2] = p->g_eager;
+    out[3] = (int64_t)p->gcache.size();
     return 0;
 }
 

@@ -10,8 +10,54 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <utility>
 
 #include "ncg_device.hpp"
+
+// Every launch goes through ncg_launch: queued on its stream, or, while the calling thread
+// has a graph recorder set (ncgk::set_graph_recorder), added to the recorder's graph as a
+// kernel node after the previous one.  Recording needs no stream capture, so other threads
+// may allocate, free and synchronise meanwhile.
+namespace {
+thread_local ncgk::GraphRecorder *t_recorder = nullptr;
+
+template <typename Tup, size_t... I>
+void arg_pointers(Tup &t, void **a, std::index_sequence<I...>)
+{
+    ((a[I] = (void *)&std::get<I>(t)), ...);
+}
+
+template <typename... P, typename... A>
+void ncg_launch(void (*k)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t s, A &&...a)
+{
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<std::decay_t<P>...> v(static_cast<std::decay_t<P>>(a)...);
+    void *args[sizeof...(P) + 1];
+    arg_pointers(v, args, std::index_sequence_for<P...>{});
+    if (ncgk::GraphRecorder *r = t_recorder) {
+        if (r->err != hipSuccess) return;
+        hipKernelNodeParams prm{};
+        prm.func = (void *)k;
+        prm.gridDim = grid;
+        prm.blockDim = block;
+        prm.sharedMemBytes = (unsigned)lds;
+        prm.kernelParams = args;
+        prm.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        r->err = hipGraphAddKernelNode(&node, r->graph, r->last ? &r->last : nullptr, r->last ? 1 : 0, &prm);
+        if (r->err == hipSuccess) {
+            r->last = node;
+            r->nodes++;
+        }
+        return;
+    }
+    (void)hipLaunchKernel((const void *)k, grid, block, args, lds, s);
+}
+}  // namespace
+
+namespace ncgk {
+void set_graph_recorder(GraphRecorder *r) { t_recorder = r; }
+}  // namespace ncgk
 
 namespace ncg {
 
@@ -2754,23 +2800,27 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_S(const ncgk::GLevel *items,
 }
 
 template <typename TB, int NR>
-__global__ void __launch_bounds__(256) kg_bj_apply(const ncgk::GBj *items, const int *start, int nitems)
+__global__ void __launch_bounds__(256) kg_bj_apply(const ncgk::GBj *items, const int *start, int nitems,
+                                                   const int *kp)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GBj it = items[d];
     if (it.skip && *it.skip) return;
+    const double *x = (const double *)it.x + (kp ? (size_t)*kp * it.xk : 0);
     bj_apply_item<TB, NR>(blockIdx.x - start[d], it.n, it.nr, it.b0, it.m, it.elem, it.boff, (const cx<TB> *)it.minv,
-                          (const cx<double> *)it.x, (cx<double> *)it.y);
+                          (const cx<double> *)x, (cx<double> *)it.y);
 }
 
-__global__ void __launch_bounds__(256) kg_vec(int op, const ncgk::GVec *items, const int *start, int nitems)
+__global__ void __launch_bounds__(256) kg_vec(int op, const ncgk::GVec *items, const int *start, int nitems,
+                                              const int *kp)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GVec it = items[d];
     if (it.skip && *it.skip) return;
     const long long t = (long long)(blockIdx.x - start[d]) * 256 + threadIdx.x;
     if (t >= it.len) return;
-    const cx<double> *a = (const cx<double> *)it.a, *b = (const cx<double> *)it.b;
+    const cx<double> *a = (const cx<double> *)((const double *)it.a + (kp ? (size_t)*kp * it.ak : 0));
+    const cx<double> *b = (const cx<double> *)it.b;
     cx<double> *y = (cx<double> *)it.y;
     switch (op) {
     case ncgk::kVecCopy: y[t] = a[t]; break;
@@ -2780,37 +2830,45 @@ __global__ void __launch_bounds__(256) kg_vec(int op, const ncgk::GVec *items, c
     }
 }
 
-// per item kcount * nrhs * nsl work-groups (start holds the prefix of nrhs * nsl)
-__global__ void __launch_bounds__(256) kg_multidot(int kcount, const ncgk::GGm *items, const int *start, int nitems)
+// per item kMdLanes * nrhs * nsl work-groups (start holds the prefix of nrhs * nsl): work-group
+// lane jj forms the partials of the basis vectors j = jj, jj + kMdLanes, ... below kcount =
+// *kp + 1, each as one work-group of the per-vector form would
+__global__ void __launch_bounds__(256) kg_multidot(const int *kp, const ncgk::GGm *items, const int *start, int nitems)
 {
-    const int d = g_item(start, nitems, blockIdx.x, kcount);
+    const int d = g_item(start, nitems, blockIdx.x, ncgk::kMdLanes);
     const ncgk::GGm it = items[d];
     if (*it.skip) return;
-    const int b = blockIdx.x - kcount * start[d];
-    const int sl = b % it.nsl, t = b / it.nsl, r = t % it.nrhs, k = t / it.nrhs;
-    multidot_item(k, r, sl, it.nsl, it.n, it.nrhs, it.slice, (const cx<double> *)it.V, (const cx<double> *)it.W,
-                  (cx<double> *)it.part);
+    const int kcount = *kp + 1;
+    const int b = blockIdx.x - ncgk::kMdLanes * start[d];
+    const int sl = b % it.nsl, t = b / it.nsl, r = t % it.nrhs, jj = t / it.nrhs;
+    for (int k = jj; k < kcount; k += ncgk::kMdLanes) {
+        multidot_item(k, r, sl, it.nsl, it.n, it.nrhs, it.slice, (const cx<double> *)it.V, (const cx<double> *)it.W,
+                      (cx<double> *)it.part);
+        __syncthreads();   // the reduction's LDS before the next vector's partial sums
+    }
 }
 
 // per item gsb * nrhs work-groups
-__global__ void __launch_bounds__(256) kg_gs_update(int kcount, int pass, int m, const ncgk::GGm *items,
-                                                    const int *start, int nitems)
+__global__ void __launch_bounds__(256) kg_gs_update(int kcount, const int *kp, int pass, int m,
+                                                    const ncgk::GGm *items, const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GGm it = items[d];
     if (*it.skip) return;
+    const int kc = kp ? *kp + 1 : kcount;
     const int b = blockIdx.x - start[d];
     cx<double> *hout = (cx<double> *)it.h + (size_t)pass * it.nrhs * (m + 1);
-    gs_update_item(b % it.gsb, b / it.gsb, it.gsb, it.n, it.nrhs, kcount, it.nsl, (const cx<double> *)it.part,
+    gs_update_item(b % it.gsb, b / it.gsb, it.gsb, it.n, it.nrhs, kc, it.nsl, (const cx<double> *)it.part,
                    (const cx<double> *)it.V, (cx<double> *)it.W, hout, pass ? it.npart : nullptr);
 }
 
-__global__ void __launch_bounds__(256) kg_gs_normalize(int k, int m, const ncgk::GGm *items, const int *start,
-                                                       int nitems)
+__global__ void __launch_bounds__(256) kg_gs_normalize(int kh, const int *kp, int m, const ncgk::GGm *items,
+                                                       const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GGm it = items[d];
     if (*it.skip) return;
+    const int k = kp ? *kp : kh;
     const int b = blockIdx.x - start[d];
     cx<double> *v = (cx<double> *)it.V + (size_t)(k + 1) * it.nrhs * it.n;
     cx<double> *nout = (cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
@@ -2824,8 +2882,9 @@ __device__ __forceinline__ void g_set_skip(const ncgk::GGm &it, bool done_r)
     if (threadIdx.x == 0) *it.skip = open == 0ull ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(64) kg_gm_start(int m, int first, const ncgk::GGm *items)
+__global__ void __launch_bounds__(64) kg_gm_start(int m, int first, const ncgk::GGm *items, int *kp)
 {
+    if (kp && blockIdx.x == 0 && threadIdx.x == 0) *kp = 0;
     const ncgk::GGm it = items[blockIdx.x];
     if (!first && *it.skip) return;   // converged in an earlier cycle: no restart
     const int r = threadIdx.x;
@@ -2850,10 +2909,11 @@ __global__ void __launch_bounds__(64) kg_gm_start(int m, int first, const ncgk::
     g_set_skip(it, dn);
 }
 
-__global__ void __launch_bounds__(64) kg_gm_rotate(int k, int m, const ncgk::GGm *items)
+__global__ void __launch_bounds__(64) kg_gm_rotate(int kh, const int *kp, int m, const ncgk::GGm *items)
 {
     const ncgk::GGm it = items[blockIdx.x];
     if (*it.skip) return;
+    const int k = kp ? *kp : kh;
     const int r = threadIdx.x, kc = k + 1;
     bool dn = true;
     if (r < it.nrhs) {
@@ -2942,10 +3002,16 @@ __global__ void kg_gm_clear(const ncgk::GGm *items, int nitems)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nitems) *items[i].skip = 0;
 }
-__global__ void kg_gm_flags(const ncgk::GGm *items, int nitems, int *out)
+// one work-group of 64 threads (nitems <= 64)
+__global__ void __launch_bounds__(64) kg_gm_flags(const ncgk::GGm *items, int nitems, int *out, int *kp, int depth)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nitems) out[i] = *items[i].skip;
+    const int i = threadIdx.x;
+    const int slot = kp ? *kp % depth : 0;
+    if (i < nitems) out[slot * 64 + i] = *items[i].skip;
+    if (kp) {
+        __syncthreads();   // every lane has read the counter
+        if (i == 0) *kp = *kp + 1;
+    }
 }
 // per item and load case: iterations (counter 2) and relative residual estimate (scalar 2)
 __global__ void kg_gm_stats(const ncgk::GGm *items, int *its, double *rel)
@@ -3016,19 +3082,19 @@ hipError_t set_gauss_tables(const double *x, const double *w)
 void element_geometry(hipStream_t s, int n, const double *X, const int8_t *nv, double *cen, double *nrm, double *area,
                       double *jac)
 {
-    k_element_geometry<<<blocks_for(n, 128), 128, 0, s>>>(n, X, nv, cen, nrm, area, jac);
+    ncg_launch(k_element_geometry, dim3(blocks_for(n, 128)), dim3(128), 0, s, n, X, nv, cen, nrm, area, jac);
 }
 
 void near_count(hipStream_t s, const DevElements &D, int *count)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    k_near_count<<<blocks_for(D.n, 64), 64, 0, s>>>(E, count);
+    ncg_launch(k_near_count, dim3(blocks_for(D.n, 64)), dim3(64), 0, s, E, count);
 }
 
 void near_fill(hipStream_t s, const DevElements &D, const long long *offset, int *rows, int *cols)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    k_near_fill<<<blocks_for(D.n, 64), 64, 0, s>>>(E, offset, rows, cols);
+    ncg_launch(k_near_fill, dim3(blocks_for(D.n, 64)), dim3(64), 0, s, E, offset, rows, cols);
 }
 
 static BMConst make_bm(const BMParams &b)
@@ -3048,9 +3114,9 @@ void assemble_far(hipStream_t s, const DevElements &D, const BMParams &b, const 
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     dim3 grid(blocks_for(D.n, 256), blocks_for(D.n, rows_per_block));
     if (mixed)
-        k_assemble_far<float, float><<<grid, 256, 0, s>>>(E, make_bm(b), adm, (cx<float> *)A, rows_per_block, evals);
+        ncg_launch(k_assemble_far<float, float>, dim3(grid), dim3(256), 0, s, E, make_bm(b), adm, (cx<float> *)A, rows_per_block, evals);
     else
-        k_assemble_far<double, double><<<grid, 256, 0, s>>>(E, make_bm(b), adm, (cx<double> *)A, rows_per_block, evals);
+        ncg_launch(k_assemble_far<double, double>, dim3(grid), dim3(256), 0, s, E, make_bm(b), adm, (cx<double> *)A, rows_per_block, evals);
 }
 
 void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
@@ -3060,9 +3126,9 @@ void assemble_near(hipStream_t s, const DevElements &D, const BMParams &b, const
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for(npairs, 128);
     if (mixed)
-        k_assemble_near<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<float> *)A, err, evals);
+        ncg_launch(k_assemble_near<float>, dim3(g), dim3(128), 0, s, E, make_bm(b), adm, rows, cols, npairs, (cx<float> *)A, err, evals);
     else
-        k_assemble_near<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, npairs, (cx<double> *)A, err, evals);
+        ncg_launch(k_assemble_near<double>, dim3(g), dim3(128), 0, s, E, make_bm(b), adm, rows, cols, npairs, (cx<double> *)A, err, evals);
 }
 
 void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const int *src, int nsrc, const void *bcval,
@@ -3071,7 +3137,7 @@ void rhs_sources(hipStream_t s, const DevElements &D, const BMParams &b, const i
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk)
-        k_rhs_sources<<<blocks_for(D.n, 4), 256, 0, s>>>(E, make_bm(b), src, nsrc, (const cx<double> *)bcval, nrhs, r0,
+        ncg_launch(k_rhs_sources, dim3(blocks_for(D.n, 4)), dim3(256), 0, s, E, make_bm(b), src, nsrc, (const cx<double> *)bcval, nrhs, r0,
                                                          (cx<double> *)rhs_src, err, elem_cluster, cluster_near,
                                                          nclusters);
 }
@@ -3081,7 +3147,7 @@ void assemble_diag(hipStream_t s, const DevElements &D, const BMParams &b, const
                    void *A, int mixed, void *rhs, const long long *diag_index, unsigned long long *evals)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    k_assemble_diag<<<blocks_for(D.n, 64), 64, 0, s>>>(E, make_bm(b), adm, (const cx<double> *)bcval, nrhs, ps, nps, pw,
+    ncg_launch(k_assemble_diag, dim3(blocks_for(D.n, 64)), dim3(64), 0, s, E, make_bm(b), adm, (const cx<double> *)bcval, nrhs, ps, nps, pw,
                                                        npw, (const cx<double> *)rhs_src, has_src, A, mixed,
                                                        (cx<double> *)rhs, diag_index, evals);
 }
@@ -3095,10 +3161,10 @@ void matvec(hipStream_t s, const void *A, int mixed, int acc64, int n, int nrhs,
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
 #define NCG_MV(S, ACC)                                                                                 \
     switch (nr) {                                                                                      \
-    case 1: k_matvec<S, ACC, 1><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
-    case 2: k_matvec<S, ACC, 2><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
-    case 3: k_matvec<S, ACC, 3><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;             \
-    default: k_matvec<S, ACC, 4><<<g, 256, 0, s>>>((const cx<S> *)A, n, xx, d, yy); break;            \
+    case 1: ncg_launch(k_matvec<S, ACC, 1>, dim3(g), dim3(256), 0, s, (const cx<S> *)A, n, xx, d, yy); break;             \
+    case 2: ncg_launch(k_matvec<S, ACC, 2>, dim3(g), dim3(256), 0, s, (const cx<S> *)A, n, xx, d, yy); break;             \
+    case 3: ncg_launch(k_matvec<S, ACC, 3>, dim3(g), dim3(256), 0, s, (const cx<S> *)A, n, xx, d, yy); break;             \
+    default: ncg_launch(k_matvec<S, ACC, 4>, dim3(g), dim3(256), 0, s, (const cx<S> *)A, n, xx, d, yy); break;            \
     }
         if (!mixed) { NCG_MV(double, double) }
         else if (acc64) { NCG_MV(float, double) }
@@ -3322,29 +3388,29 @@ MfmaPlan mfma_plan(int n, int f64, int splits)
 
 void mfma_xhat(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *x, void *xh)
 {
-    if (m.f64) k_xhat64<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x, (double *)xh);
-    else k_xhat<<<blocks_for((size_t)n * 32, 256), 256, 0, s>>>(n, nr, (const cx<double> *)x, (float *)xh);
+    if (m.f64) ncg_launch(k_xhat64, dim3(blocks_for((size_t)n * 32, 256)), dim3(256), 0, s, n, nr, (const cx<double> *)x, (double *)xh);
+    else ncg_launch(k_xhat, dim3(blocks_for((size_t)n * 32, 256)), dim3(256), 0, s, n, nr, (const cx<double> *)x, (float *)xh);
 }
 
 void mfma_product(hipStream_t s, const MfmaPlan &m, const void *A, int a_f32, int n, const void *xh, void *part)
 {
     const dim3 grid(m.tiles, m.splits);
     if (!m.f64)
-        k_matvec_mfma<<<grid, 256, 0, s>>>((const cx<float> *)A, n, m.per, (const float *)xh, (float *)part);
+        ncg_launch(k_matvec_mfma, dim3(grid), dim3(256), 0, s, (const cx<float> *)A, n, m.per, (const float *)xh, (float *)part);
     else if (a_f32)
-        k_matvec_mfma64<float><<<grid, 256, 0, s>>>((const cx<float> *)A, n, m.per, (const double *)xh, (double *)part);
+        ncg_launch(k_matvec_mfma64<float>, dim3(grid), dim3(256), 0, s, (const cx<float> *)A, n, m.per, (const double *)xh, (double *)part);
     else
-        k_matvec_mfma64<double><<<grid, 256, 0, s>>>((const cx<double> *)A, n, m.per, (const double *)xh,
+        ncg_launch(k_matvec_mfma64<double>, dim3(grid), dim3(256), 0, s, (const cx<double> *)A, n, m.per, (const double *)xh,
                                                      (double *)part);
 }
 
 void mfma_sum(hipStream_t s, const MfmaPlan &m, int n, int nr, const void *part, void *y)
 {
     if (m.f64)
-        k_mfma64_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, m.splits, (const double *)part,
+        ncg_launch(k_mfma64_reduce, dim3(blocks_for((size_t)n * nr, 256)), dim3(256), 0, s, n, nr, m.splits, (const double *)part,
                                                                         (cx<double> *)y);
     else
-        k_mfma_reduce<<<blocks_for((size_t)n * nr, 256), 256, 0, s>>>(n, nr, m.splits, (const float *)part,
+        ncg_launch(k_mfma_reduce, dim3(blocks_for((size_t)n * nr, 256)), dim3(256), 0, s, n, nr, m.splits, (const float *)part,
                                                                       (cx<double> *)y);
 }
 
@@ -3354,60 +3420,60 @@ void dots(hipStream_t s, int n, int nrhs, int ndots, const void *const *u, const
           void *partial, void *out)
 {
     const int nb = dot_blocks(n);
-    k_dots<<<nb, 256, 0, s>>>(n, nrhs, ndots, (const cx<double> *)u[0], (const cx<double> *)v[0],
+    ncg_launch(k_dots, dim3(nb), dim3(256), 0, s, n, nrhs, ndots, (const cx<double> *)u[0], (const cx<double> *)v[0],
                               (const cx<double> *)(ndots > 1 ? u[1] : u[0]), (const cx<double> *)(ndots > 1 ? v[1] : v[0]),
                               (const cx<double> *)(ndots > 2 ? u[2] : u[0]), (const cx<double> *)(ndots > 2 ? v[2] : v[0]),
                               conj_mask, (cx<double> *)partial);
-    k_reduce_partials<<<1, 256, 0, s>>>(nb, nrhs, ndots, (const cx<double> *)partial, (cx<double> *)out);
+    ncg_launch(k_reduce_partials, dim3(1), dim3(256), 0, s, nb, nrhs, ndots, (const cx<double> *)partial, (cx<double> *)out);
 }
 
 void cgs_alpha(hipStream_t s, int nrhs, int first, const void *dts, void *sc, const int *active)
 {
-    k_cgs_alpha<<<1, 64, 0, s>>>(nrhs, first, (const cx<double> *)dts, (cx<double> *)sc, active);
+    ncg_launch(k_cgs_alpha, dim3(1), dim3(64), 0, s, nrhs, first, (const cx<double> *)dts, (cx<double> *)sc, active);
 }
 void cgs_beta(hipStream_t s, int nrhs, const void *dts, void *sc, const int *active)
 {
-    k_cgs_beta<<<1, 64, 0, s>>>(nrhs, (const cx<double> *)dts, (cx<double> *)sc, active);
+    ncg_launch(k_cgs_beta, dim3(1), dim3(64), 0, s, nrhs, (const cx<double> *)dts, (cx<double> *)sc, active);
 }
 void cgs_update1(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *u, const void *v, void *q,
                  void *uq)
 {
-    k_cgs_update1<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+    ncg_launch(k_cgs_update1, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, (const cx<double> *)sc, active,
                                                                      (const cx<double> *)u, (const cx<double> *)v,
                                                                      (cx<double> *)q, (cx<double> *)uq);
 }
 void cgs_update2(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *uq, const void *auq,
                  void *x, void *r)
 {
-    k_cgs_update2<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+    ncg_launch(k_cgs_update2, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, (const cx<double> *)sc, active,
                                                                      (const cx<double> *)uq, (const cx<double> *)auq,
                                                                      (cx<double> *)x, (cx<double> *)r);
 }
 void cgs_update3(hipStream_t s, int n, int nrhs, const void *sc, const int *active, const void *r, const void *q, void *u,
                  void *p)
 {
-    k_cgs_update3<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)sc, active,
+    ncg_launch(k_cgs_update3, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, (const cx<double> *)sc, active,
                                                                      (const cx<double> *)r, (const cx<double> *)q,
                                                                      (cx<double> *)u, (cx<double> *)p);
 }
 void axpby(hipStream_t s, int n, int nrhs, const void *coef, const void *x, size_t ldx, void *y, size_t ldy)
 {
-    k_axpby<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)coef, (const cx<double> *)x,
+    ncg_launch(k_axpby, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, (const cx<double> *)coef, (const cx<double> *)x,
                                                                ldx, (cx<double> *)y, ldy);
 }
 void scale_vec(hipStream_t s, int n, int nrhs, const double *d, void *x)
 {
-    k_scale_vec<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, d, (cx<double> *)x);
+    ncg_launch(k_scale_vec, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, d, (cx<double> *)x);
 }
 void row_scaling(hipStream_t s, const void *A, int mixed, int n, double *d)
 {
-    if (mixed) k_row_scaling<float><<<blocks_for(n, 4), 256, 0, s>>>((const cx<float> *)A, n, d);
-    else k_row_scaling<double><<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d);
+    if (mixed) ncg_launch(k_row_scaling<float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, (const cx<float> *)A, n, d);
+    else ncg_launch(k_row_scaling<double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, (const cx<double> *)A, n, d);
 }
 void row_scaling_csr(hipStream_t s, int n, const long long *ptr, const void *val, int mixed, double *d)
 {
-    if (mixed) k_row_scaling_csr<float><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<float> *)val, d);
-    else k_row_scaling_csr<double><<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, (const cx<double> *)val, d);
+    if (mixed) ncg_launch(k_row_scaling_csr<float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, ptr, (const cx<float> *)val, d);
+    else ncg_launch(k_row_scaling_csr<double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, ptr, (const cx<double> *)val, d);
 }
 // slices of k_multidot: about 1024 entries each (one load step of four per thread), as
 // many as the partial buffer holds
@@ -3421,9 +3487,9 @@ void multidot(hipStream_t s, int n, int nrhs, int kcount, const void *V, const v
 {
     const int nsl = multidot_slices(n, nrhs, kcount, part_cap);
     const int slice = (n + nsl - 1) / nsl;
-    k_multidot<<<dim3(kcount, nrhs, nsl), 256, 0, s>>>(n, nrhs, kcount, slice, (const cx<double> *)V,
+    ncg_launch(k_multidot, dim3(dim3(kcount, nrhs, nsl)), dim3(256), 0, s, n, nrhs, kcount, slice, (const cx<double> *)V,
                                                        (const cx<double> *)w, (cx<double> *)part);
-    k_multidot_sum<<<blocks_for((size_t)nrhs * kcount, 256), 256, 0, s>>>(nrhs, kcount, nsl, (const cx<double> *)part,
+    ncg_launch(k_multidot_sum, dim3(blocks_for((size_t)nrhs * kcount, 256)), dim3(256), 0, s, nrhs, kcount, nsl, (const cx<double> *)part,
                                                                            (cx<double> *)h);
 }
 static unsigned gs_blocks(int n) { return std::min<unsigned>(256u, blocks_for(n, 256)); }
@@ -3432,73 +3498,73 @@ void gs_pass(hipStream_t s, int n, int nrhs, int kcount, const void *V, void *w,
 {
     const int nsl = multidot_slices(n, nrhs, kcount, part_cap);
     const int slice = (n + nsl - 1) / nsl;
-    k_multidot<<<dim3(kcount, nrhs, nsl), 256, 0, s>>>(n, nrhs, kcount, slice, (const cx<double> *)V,
+    ncg_launch(k_multidot, dim3(dim3(kcount, nrhs, nsl)), dim3(256), 0, s, n, nrhs, kcount, slice, (const cx<double> *)V,
                                                        (const cx<double> *)w, (cx<double> *)part);
-    k_gs_update<<<dim3(gs_blocks(n), nrhs), 256, 0, s>>>(n, nrhs, kcount, nsl, (const cx<double> *)part,
+    ncg_launch(k_gs_update, dim3(dim3(gs_blocks(n), nrhs)), dim3(256), 0, s, n, nrhs, kcount, nsl, (const cx<double> *)part,
                                                          (const cx<double> *)V, (cx<double> *)w, (cx<double> *)h, npart);
 }
 void gs_normalize(hipStream_t s, int n, int nrhs, const double *npart, const void *w, void *v, void *nout)
 {
-    k_gs_normalize<<<dim3(gs_blocks(n), nrhs), 256, 0, s>>>(n, (int)gs_blocks(n), npart, (const cx<double> *)w,
+    ncg_launch(k_gs_normalize, dim3(dim3(gs_blocks(n), nrhs)), dim3(256), 0, s, n, (int)gs_blocks(n), npart, (const cx<double> *)w,
                                                             (cx<double> *)v, (cx<double> *)nout);
 }
 void multiaxpy(hipStream_t s, int n, int nrhs, int kcount, const void *V, const void *h, void *w, int add)
 {
     if (add)
-        k_multiaxpy_add<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V,
+        ncg_launch(k_multiaxpy_add, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, kcount, (const cx<double> *)V,
                                                                            (const cx<double> *)h, (cx<double> *)w);
     else
-        k_multiaxpy<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kcount, (const cx<double> *)V,
+        ncg_launch(k_multiaxpy, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, kcount, (const cx<double> *)V,
                                                                        (const cx<double> *)h, (cx<double> *)w);
 }
 void ilu_count(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
                int *cnt)
 {
     if (ptr)
-        k_ilu_count_csr<<<blocks_for(n, 4), 256, 0, s>>>(n, ptr, ncol, (const cx<double> *)A, d, thr, cnt);
+        ncg_launch(k_ilu_count_csr, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, ptr, ncol, (const cx<double> *)A, d, thr, cnt);
     else
-        k_ilu_count_dense<<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d, thr, cnt);
+        ncg_launch(k_ilu_count_dense, dim3(blocks_for(n, 4)), dim3(256), 0, s, (const cx<double> *)A, n, d, thr, cnt);
 }
 void ilu_fill(hipStream_t s, const void *A, int n, const long long *ptr, const int *ncol, const double *d, double thr,
               const int *rp, int *col, void *val)
 {
     if (ptr)
-        k_ilu_fill_csr<<<blocks_for(n, 256), 256, 0, s>>>(n, ptr, ncol, (const cx<double> *)A, d, thr, rp, col,
+        ncg_launch(k_ilu_fill_csr, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ptr, ncol, (const cx<double> *)A, d, thr, rp, col,
                                                            (cx<double> *)val);
     else
-        k_ilu_fill_dense<<<blocks_for(n, 4), 256, 0, s>>>((const cx<double> *)A, n, d, thr, rp, col,
+        ncg_launch(k_ilu_fill_dense, dim3(blocks_for(n, 4)), dim3(256), 0, s, (const cx<double> *)A, n, d, thr, rp, col,
                                                           (cx<double> *)val);
 }
 void ilu_factor(hipStream_t s, int nlev, const int *lev_ptr, const int *lev_rows, const int *rp, const int *col,
                 const int *diag, const long long *uoff, const int *upd, void *val, int *err)
 {
-    k_ilu_factor<<<1, 1024, 0, s>>>(nlev, lev_ptr, lev_rows, rp, col, diag, uoff, upd, (cx<double> *)val, err);
+    ncg_launch(k_ilu_factor, dim3(1), dim3(1024), 0, s, nlev, lev_ptr, lev_rows, rp, col, diag, uoff, upd, (cx<double> *)val, err);
 }
 void ilu_pack(hipStream_t s, int cnt, const int *map, const void *val, void *out)
 {
     if (cnt > 0)
-        k_ilu_pack<<<blocks_for(cnt, 256), 256, 0, s>>>(cnt, map, (const cx<double> *)val, (cx<double> *)out);
+        ncg_launch(k_ilu_pack, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, cnt, map, (const cx<double> *)val, (cx<double> *)out);
 }
 void ilu_solve(hipStream_t s, int n, int nrhs, const IluSweep &L, const IluSweep &U, const void *x, void *y)
 {
-    k_ilu_solve<<<nrhs, ncgk::kIluRows, 0, s>>>(n, L, U, (const cx<double> *)x, (cx<double> *)y);
+    ncg_launch(k_ilu_solve, dim3(nrhs), dim3(ncgk::kIluRows), 0, s, n, L, U, (const cx<double> *)x, (cx<double> *)y);
 }
 void lu_factor(hipStream_t s, void *A, int n)
 {
     for (int i = 0; i < n - 1; ++i) {
         const int rem = n - 1 - i;
-        k_lu_update<<<dim3(blocks_for(rem, 256), rem), 256, 0, s>>>((cx<double> *)A, n, i);
-        k_lu_scale<<<blocks_for(rem, 256), 256, 0, s>>>((cx<double> *)A, n, i);
+        ncg_launch(k_lu_update, dim3(dim3(blocks_for(rem, 256), rem)), dim3(256), 0, s, (cx<double> *)A, n, i);
+        ncg_launch(k_lu_scale, dim3(blocks_for(rem, 256)), dim3(256), 0, s, (cx<double> *)A, n, i);
     }
 }
 void lu_solve(hipStream_t s, const void *A, int n, int nrhs, void *b)
 {
-    k_lu_solve<<<nrhs, 1024, 0, s>>>((const cx<double> *)A, n, (cx<double> *)b);
+    ncg_launch(k_lu_solve, dim3(nrhs), dim3(1024), 0, s, (const cx<double> *)A, n, (cx<double> *)b);
 }
 void field_prep(hipStream_t s, int n, int nrhs, const int8_t *kind, const double *adm, double tau, const void *bcval,
                 const void *x, void *pot, void *vel)
 {
-    k_field_prep<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, kind, adm, tau, (const cx<double> *)bcval,
+    ncg_launch(k_field_prep, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, kind, adm, tau, (const cx<double> *)bcval,
                                                                     (const cx<double> *)x, (cx<double> *)pot,
                                                                     (cx<double> *)vel);
 }
@@ -3510,12 +3576,12 @@ void field(hipStream_t s, const DevElements &D, const BMParams &b, const double 
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     for (int r0 = 0; r0 < nrhs; r0 += kFieldChunk) {
         if (fp32)
-            k_field<float><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
+            ncg_launch(k_field<float>, dim3(blocks_for(m, 4)), dim3(256), 0, s, E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                             (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
                                                             (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
                                                             (const cx<double> *)fphi, (const cx<double> *)fgrad, evals);
         else
-            k_field<double><<<blocks_for(m, 4), 256, 0, s>>>(E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
+            ncg_launch(k_field<double>, dim3(blocks_for(m, 4)), dim3(256), 0, s, E, make_bm(b), pts, m, nrhs, r0, (const cx<double> *)pot,
                                                              (const cx<double> *)vel, ps, nps, pw, npw, (cx<double> *)phi,
                                                              (cx<double> *)grad, err, pt_ev, el_ptr, el_idx,
                                                              (const cx<double> *)fphi, (const cx<double> *)fgrad, evals);
@@ -3527,23 +3593,23 @@ void fmm_D(hipStream_t s, int L, int P, double k, double h, const double *ccen, 
            const int *fan_idx, int nfan, const double *u, void *Dv, int mixed, double *coef)
 {
     if (nfan == 0) return;
-    k_fmm_D_coef<<<blocks_for(nfan, 128), 128, 0, s>>>(nfan, L, k, h, ccen, fan_row, fan_idx, coef);
-    if (mixed) k_fmm_D<float><<<nfan, 128, 0, s>>>(L, P, coef, u, (cx<float> *)Dv);
-    else k_fmm_D<double><<<nfan, 128, 0, s>>>(L, P, coef, u, (cx<double> *)Dv);
+    ncg_launch(k_fmm_D_coef, dim3(blocks_for(nfan, 128)), dim3(128), 0, s, nfan, L, k, h, ccen, fan_row, fan_idx, coef);
+    if (mixed) ncg_launch(k_fmm_D<float>, dim3(nfan), dim3(128), 0, s, L, P, coef, u, (cx<float> *)Dv);
+    else ncg_launch(k_fmm_D<double>, dim3(nfan), dim3(128), 0, s, L, P, coef, u, (cx<double> *)Dv);
 }
 void fmm_T(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, int P, const double *ccen,
            const int *elem_cl, const int *elem_pos, const double *u, void *Tv, int mixed)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const dim3 g(blocks_for(D.n, 4), (P + 63) / 64);
-    if (mixed) k_fmm_T<float><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<float> *)Tv);
-    else k_fmm_T<double><<<g, 256, 0, s>>>(E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<double> *)Tv);
+    if (mixed) ncg_launch(k_fmm_T<float>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<float> *)Tv);
+    else ncg_launch(k_fmm_T<double>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, P, ccen, elem_cl, elem_pos, u, (cx<double> *)Tv);
 }
 void fmm_Tvec_bc(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                  const int *cl_ptr, const int *cl_elem, const double *u, const void *bcval, int nrhs, void *tv)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    k_fmm_Tvec_bc<<<blocks_for((size_t)ncl * P, 256), 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u,
+    ncg_launch(k_fmm_Tvec_bc, dim3(blocks_for((size_t)ncl * P, 256)), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u,
                                                                    (const cx<double> *)bcval, nrhs, (cx<double> *)tv);
 }
 void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const double *ccen, const int *elem_cl,
@@ -3552,9 +3618,9 @@ void fmm_S(hipStream_t s, const DevElements &D, const BMParams &b, int P, const 
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for((size_t)D.n * P, 256);
     if (mixed)
-        k_fmm_S<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<float> *)Sv);
+        ncg_launch(k_fmm_S<float>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<float> *)Sv);
     else
-        k_fmm_S<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<double> *)Sv);
+        ncg_launch(k_fmm_S<double>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ccen, elem_cl, cl_ptr, cl_elem, u, w, (cx<double> *)Sv);
 }
 void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const RowChunks &rc,
               const int *cl_ptr, const int *cl_elem, const long long *ptr, const int *col, int wmax, void *vals,
@@ -3564,16 +3630,16 @@ void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const doub
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const dim3 g(rc.n, (wmax + 255) / 256);
     if (mixed && rows_mode)
-        k_fmm_near_rows<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
+        ncg_launch(k_fmm_near_rows<float, float>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
                                                         col, (cx<float> *)vals, evals);
     else if (mixed)
-        k_fmm_near<float, float><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+        ncg_launch(k_fmm_near<float, float>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                                                    (cx<float> *)vals, evals);
     else if (rows_mode)
-        k_fmm_near_rows<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
+        ncg_launch(k_fmm_near_rows<double, double>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr,
                                                           col, (cx<double> *)vals, evals);
     else
-        k_fmm_near<double, double><<<g, 256, 0, s>>>(E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
+        ncg_launch(k_fmm_near<double, double>, dim3(g), dim3(256), 0, s, E, make_bm(b), adm, rc.cl, rc.r0, rc.m, cl_ptr, cl_elem, ptr, col,
                                                      (cx<double> *)vals, evals);
 }
 void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *cols, const int *elem_cl,
@@ -3581,7 +3647,7 @@ void adj_positions(hipStream_t s, long long npairs, const int *rows, const int *
                    const long long *near_ptr, long long *pos)
 {
     if (npairs == 0) return;
-    k_adj_positions<<<blocks_for(npairs, 256), 256, 0, s>>>(npairs, rows, cols, elem_cl, elem_pos, cl_ptr, nn_ptr,
+    ncg_launch(k_adj_positions, dim3(blocks_for(npairs, 256)), dim3(256), 0, s, npairs, rows, cols, elem_cl, elem_pos, cl_ptr, nn_ptr,
                                                             nn_idx, nn_off, near_ptr, pos);
 }
 void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const double *adm, const int *rows,
@@ -3592,16 +3658,16 @@ void fmm_near_adj(hipStream_t s, const DevElements &D, const BMParams &b, const 
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for(npairs, 128);
     if (mixed)
-        k_fmm_near_adj<float><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<float> *)vals, err,
+        ncg_launch(k_fmm_near_adj<float>, dim3(g), dim3(128), 0, s, E, make_bm(b), adm, rows, cols, pos, npairs, (cx<float> *)vals, err,
                                                 evals);
     else
-        k_fmm_near_adj<double><<<g, 128, 0, s>>>(E, make_bm(b), adm, rows, cols, pos, npairs, (cx<double> *)vals, err,
+        ncg_launch(k_fmm_near_adj<double>, dim3(g), dim3(128), 0, s, E, make_bm(b), adm, rows, cols, pos, npairs, (cx<double> *)vals, err,
                                                  evals);
 }
 void near_csr(hipStream_t s, int n, const int *elem_cl, const int *cl_ptr, const int *cl_elem, const int *nn_ptr,
               const int *nn_idx, const long long *ptr, int *rows, int *cols, long long *diag)
 {
-    k_near_csr<<<blocks_for(n, 4), 256, 0, s>>>(n, elem_cl, cl_ptr, cl_elem, nn_ptr, nn_idx, ptr, rows, cols, diag);
+    ncg_launch(k_near_csr, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_cl, cl_ptr, cl_elem, nn_ptr, nn_idx, ptr, rows, cols, diag);
 }
 
 // load cases in chunks of up to 4 per launch: fn(NR, r0) launches one chunk
@@ -3619,29 +3685,29 @@ void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *el
               const void *nval, int mixed, const long long *boff, void *blk, int *err)
 {
     if (mixed) {
-        k_bj_gather<float, float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
+        ncg_launch(k_bj_gather<float, float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
                                                                    near_ptr, self_off, (const cx<float> *)nval, boff,
                                                                    (cx<float> *)blk);
-        k_bj_invert<float><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<float> *)blk, err);
+        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<float> *)blk, err);
     } else {
-        k_bj_gather<double, double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m,
+        ncg_launch(k_bj_gather<double, double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m,
                                                                      cl_ptr, near_ptr, self_off,
                                                                      (const cx<double> *)nval, boff, (cx<double> *)blk);
-        k_bj_invert<double><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
+        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<double> *)blk, err);
     }
 }
 void bj_setup_dense(hipStream_t s, int n, int nblk, const int *elem_blk, const int *blk_b0, const int *blk_m,
                     const void *A, int mixed, const long long *boff, void *blk, int *err)
 {
     if (mixed) {
-        k_bj_gather_dense<float, float><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, blk_b0, blk_m, (const cx<float> *)A,
+        ncg_launch(k_bj_gather_dense<float, float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, blk_b0, blk_m, (const cx<float> *)A,
                                                                          boff, (cx<float> *)blk);
-        k_bj_invert<float><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<float> *)blk, err);
+        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<float> *)blk, err);
     } else {
-        k_bj_gather_dense<double, double><<<blocks_for(n, 4), 256, 0, s>>>(n, elem_blk, blk_b0, blk_m,
+        ncg_launch(k_bj_gather_dense<double, double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, blk_b0, blk_m,
                                                                            (const cx<double> *)A, boff,
                                                                            (cx<double> *)blk);
-        k_bj_invert<double><<<nblk, 64, 0, s>>>(blk_m, boff, (cx<double> *)blk, err);
+        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<double> *)blk, err);
     }
 }
 template <typename TB>
@@ -3654,11 +3720,11 @@ static void bj_apply_t(hipStream_t s, int n, int nrhs, int nblk, const int *blk_
         const cx<double> *xx = (const cx<double> *)x + (size_t)r0 * n;
         cx<double> *yy = (cx<double> *)y + (size_t)r0 * n;
         switch (nr) {
-        case 1: k_bj_apply<TB, 1><<<nblk, 256, 0, s>>>(n, 1, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 2: k_bj_apply<TB, 2><<<nblk, 256, 0, s>>>(n, 2, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 3: k_bj_apply<TB, 3><<<nblk, 256, 0, s>>>(n, 3, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        case 4: k_bj_apply<TB, 4><<<nblk, 256, 0, s>>>(n, 4, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
-        default: k_bj_apply<TB, 8><<<nblk, 256, 0, s>>>(n, nr, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 1: ncg_launch(k_bj_apply<TB, 1>, dim3(nblk), dim3(256), 0, s, n, 1, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 2: ncg_launch(k_bj_apply<TB, 2>, dim3(nblk), dim3(256), 0, s, n, 2, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 3: ncg_launch(k_bj_apply<TB, 3>, dim3(nblk), dim3(256), 0, s, n, 3, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        case 4: ncg_launch(k_bj_apply<TB, 4>, dim3(nblk), dim3(256), 0, s, n, 4, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
+        default: ncg_launch(k_bj_apply<TB, 8>, dim3(nblk), dim3(256), 0, s, n, nr, blk_b0, blk_m, cl_elem, boff, M, xx, yy); break;
         }
     }
 }
@@ -3670,17 +3736,17 @@ void bj_apply(hipStream_t s, int n, int nrhs, int nblk, const int *blk_b0, const
 }
 void scale_norm(hipStream_t s, int n, int nrhs, const void *dts, const void *w, void *v)
 {
-    k_scale_norm<<<blocks_for((size_t)n * nrhs, 256), 256, 0, s>>>(n, nrhs, (const cx<double> *)dts,
+    ncg_launch(k_scale_norm, dim3(blocks_for((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, (const cx<double> *)dts,
                                                                     (const cx<double> *)w, (cx<double> *)v);
 }
 
 #define NCG_TARGS(...) __VA_ARGS__
 #define NCG_NR_DISPATCH_SH(NR, KERNEL, TARGS, GRID, BLOCK, SH, ...)                     \
     switch (NR) {                                                                       \
-    case 1: KERNEL<TARGS, 1><<<GRID, BLOCK, (SH) * 1, s>>>(__VA_ARGS__); break;         \
-    case 2: KERNEL<TARGS, 2><<<GRID, BLOCK, (SH) * 2, s>>>(__VA_ARGS__); break;         \
-    case 3: KERNEL<TARGS, 3><<<GRID, BLOCK, (SH) * 3, s>>>(__VA_ARGS__); break;         \
-    default: KERNEL<TARGS, 4><<<GRID, BLOCK, (SH) * 4, s>>>(__VA_ARGS__); break;        \
+    case 1: ncg_launch(KERNEL<TARGS, 1>, dim3(GRID), dim3(BLOCK), (SH) * 1, s, __VA_ARGS__); break;         \
+    case 2: ncg_launch(KERNEL<TARGS, 2>, dim3(GRID), dim3(BLOCK), (SH) * 2, s, __VA_ARGS__); break;         \
+    case 3: ncg_launch(KERNEL<TARGS, 3>, dim3(GRID), dim3(BLOCK), (SH) * 3, s, __VA_ARGS__); break;         \
+    default: ncg_launch(KERNEL<TARGS, 4>, dim3(GRID), dim3(BLOCK), (SH) * 4, s, __VA_ARGS__); break;        \
     }
 
 // LDS budget of one near-field chunk: seg columns x NR load cases x 16 bytes
@@ -3771,17 +3837,17 @@ void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P,
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for((size_t)ncl * P, 256);
     if (fp32)
-        k_fmm_Tvec_sol<float><<<g, 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
+        ncg_launch(k_fmm_Tvec_sol<float>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
                                                 (const cx<double> *)vel, nrhs, (cx<double> *)tv);
     else
-        k_fmm_Tvec_sol<double><<<g, 256, 0, s>>>(E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
+        ncg_launch(k_fmm_Tvec_sol<double>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
                                                  (const cx<double> *)vel, nrhs, (cx<double> *)tv);
 }
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
                      const int *pair_ev, const int *pair_cl, int npair, double *tab)
 {
     if (npair == 0) return;
-    k_fmm_pair_hankel<<<blocks_for(npair, 128), 128, 0, s>>>(npair, L, k, h, ecen, ccen, pair_ev, pair_cl, tab);
+    ncg_launch(k_fmm_pair_hankel, dim3(blocks_for(npair, 128)), dim3(128), 0, s, npair, L, k, h, ecen, ccen, pair_ev, pair_cl, tab);
 }
 void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
                 const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
@@ -3793,17 +3859,17 @@ void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_pt
         cx<double> *zz = (cx<double> *)z + (size_t)r0 * nev * P;
         if (fp32) {
             switch (nr) {
-            case 1: k_fmm_eval_z<1, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 2: k_fmm_eval_z<2, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 3: k_fmm_eval_z<3, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            default: k_fmm_eval_z<4, float><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 1: ncg_launch(k_fmm_eval_z<1, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 2: ncg_launch(k_fmm_eval_z<2, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 3: ncg_launch(k_fmm_eval_z<3, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            default: ncg_launch(k_fmm_eval_z<4, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
             }
         } else {
             switch (nr) {
-            case 1: k_fmm_eval_z<1, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 2: k_fmm_eval_z<2, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 3: k_fmm_eval_z<3, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            default: k_fmm_eval_z<4, double><<<nev, 256, 0, s>>>(nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 1: ncg_launch(k_fmm_eval_z<1, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 2: ncg_launch(k_fmm_eval_z<2, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            case 3: ncg_launch(k_fmm_eval_z<3, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+            default: ncg_launch(k_fmm_eval_z<4, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
             }
         }
     });
@@ -3811,7 +3877,7 @@ void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_pt
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad)
 {
-    k_fmm_eval_far<<<blocks_for(m, 4), 256, 0, s>>>(m, P, nev, hk, pts, pt_ev, ecen, u, nrhs, (const cx<double> *)z,
+    ncg_launch(k_fmm_eval_far, dim3(blocks_for(m, 4)), dim3(256), 0, s, m, P, nev, hk, pts, pt_ev, ecen, u, nrhs, (const cx<double> *)z,
                                                     (cx<double> *)fphi, (cx<double> *)fgrad);
 }
 
@@ -3860,17 +3926,17 @@ void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNea
 #define NCG_QCALL(S, NR, DYN)                                                                                      \
     {                                                                                                              \
         const int g = resident_grid(kq_near_spmv<S, NR, DYN>, lds, blocks);                                        \
-        kq_near_spmv<S, NR, DYN><<<g, 256, lds, s>>>(items, start, nitems, seg, queue);                           \
+        ncg_launch(kq_near_spmv<S, NR, DYN>, dim3(g), dim3(256), lds, s, items, start, nitems, seg, queue);                           \
     }
 #define NCG_CALL(NR)                                                                                               \
     if (mode == 2 && mixed) NCG_QCALL(float, NR, true)                                                            \
     else if (mode == 2) NCG_QCALL(double, NR, true)                                                               \
     else if (mode == 3 && mixed) NCG_QCALL(float, NR, false)                                                      \
     else if (mode == 3) NCG_QCALL(double, NR, false)                                                              \
-    else if (mode && mixed) kg_near_cluster<float, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);        \
-    else if (mode) kg_near_cluster<double, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);                \
-    else if (mixed) kg_near_spmv<float, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);                   \
-    else kg_near_spmv<double, NR><<<blocks, 256, lds, s>>>(items, start, nitems, seg);
+    else if (mode && mixed) ncg_launch(kg_near_cluster<float, NR>, dim3(blocks), dim3(256), lds, s, items, start, nitems, seg);        \
+    else if (mode) ncg_launch(kg_near_cluster<double, NR>, dim3(blocks), dim3(256), lds, s, items, start, nitems, seg);                \
+    else if (mixed) ncg_launch(kg_near_spmv<float, NR>, dim3(blocks), dim3(256), lds, s, items, start, nitems, seg);                   \
+    else ncg_launch(kg_near_spmv<double, NR>, dim3(blocks), dim3(256), lds, s, items, start, nitems, seg);
     NCG_G_NR(nr, NCG_CALL)
 #undef NCG_CALL
 #undef NCG_QCALL
@@ -3887,10 +3953,10 @@ void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNea
 template <typename S, typename W>
 static void g_T_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL1(NR) kg_apply_T<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
-#define NCG_CALL2(NR) kg_apply_T<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
-#define NCG_CALL4(NR) kg_apply_T<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
-#define NCG_CALL8(NR) kg_apply_T<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+#define NCG_CALL1(NR) ncg_launch(kg_apply_T<S, W, NR, 1>, dim3(blocks), dim3(64), 0, s, items, start, nitems);
+#define NCG_CALL2(NR) ncg_launch(kg_apply_T<S, W, NR, 2>, dim3(blocks), dim3(128), 0, s, items, start, nitems);
+#define NCG_CALL4(NR) ncg_launch(kg_apply_T<S, W, NR, 4>, dim3(blocks), dim3(256), 0, s, items, start, nitems);
+#define NCG_CALL8(NR) ncg_launch(kg_apply_T<S, W, NR, 8>, dim3(blocks), dim3(512), 0, s, items, start, nitems);
     NCG_G_NW(nw, nr, NCG_CALL)
 #undef NCG_CALL1
 #undef NCG_CALL2
@@ -3900,10 +3966,10 @@ static void g_T_t(hipStream_t s, int nw, int nr, const GLevel *items, const int 
 template <typename S, typename W>
 static void g_D_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL1(NR) kg_apply_D<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
-#define NCG_CALL2(NR) kg_apply_D<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
-#define NCG_CALL4(NR) kg_apply_D<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
-#define NCG_CALL8(NR) kg_apply_D<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+#define NCG_CALL1(NR) ncg_launch(kg_apply_D<S, W, NR, 1>, dim3(blocks), dim3(64), 0, s, items, start, nitems);
+#define NCG_CALL2(NR) ncg_launch(kg_apply_D<S, W, NR, 2>, dim3(blocks), dim3(128), 0, s, items, start, nitems);
+#define NCG_CALL4(NR) ncg_launch(kg_apply_D<S, W, NR, 4>, dim3(blocks), dim3(256), 0, s, items, start, nitems);
+#define NCG_CALL8(NR) ncg_launch(kg_apply_D<S, W, NR, 8>, dim3(blocks), dim3(512), 0, s, items, start, nitems);
     NCG_G_NW(nw, nr, NCG_CALL)
 #undef NCG_CALL1
 #undef NCG_CALL2
@@ -3913,10 +3979,10 @@ static void g_D_t(hipStream_t s, int nw, int nr, const GLevel *items, const int 
 template <typename S, typename W>
 static void g_S_t(hipStream_t s, int nw, int nr, const GLevel *items, const int *start, int nitems, int blocks)
 {
-#define NCG_CALL1(NR) kg_apply_S<S, W, NR, 1><<<blocks, 64, 0, s>>>(items, start, nitems);
-#define NCG_CALL2(NR) kg_apply_S<S, W, NR, 2><<<blocks, 128, 0, s>>>(items, start, nitems);
-#define NCG_CALL4(NR) kg_apply_S<S, W, NR, 4><<<blocks, 256, 0, s>>>(items, start, nitems);
-#define NCG_CALL8(NR) kg_apply_S<S, W, NR, 8><<<blocks, 512, 0, s>>>(items, start, nitems);
+#define NCG_CALL1(NR) ncg_launch(kg_apply_S<S, W, NR, 1>, dim3(blocks), dim3(64), 0, s, items, start, nitems);
+#define NCG_CALL2(NR) ncg_launch(kg_apply_S<S, W, NR, 2>, dim3(blocks), dim3(128), 0, s, items, start, nitems);
+#define NCG_CALL4(NR) ncg_launch(kg_apply_S<S, W, NR, 4>, dim3(blocks), dim3(256), 0, s, items, start, nitems);
+#define NCG_CALL8(NR) ncg_launch(kg_apply_S<S, W, NR, 8>, dim3(blocks), dim3(512), 0, s, items, start, nitems);
     NCG_G_NW(nw, nr, NCG_CALL)
 #undef NCG_CALL1
 #undef NCG_CALL2
@@ -3949,73 +4015,76 @@ void g_apply_S(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *
     else g_S_t<double, double>(s, nw, nr, items, start, nitems, blocks);
 }
 template <typename TB>
-static void g_bj_t(hipStream_t s, int nr, const GBj *items, const int *start, int nitems, int blocks)
+static void g_bj_t(hipStream_t s, int nr, const GBj *items, const int *start, int nitems, int blocks, const int *kp)
 {
     switch (nr) {
-    case 1: kg_bj_apply<TB, 1><<<blocks, 256, 0, s>>>(items, start, nitems); break;
-    case 2: kg_bj_apply<TB, 2><<<blocks, 256, 0, s>>>(items, start, nitems); break;
-    case 3: kg_bj_apply<TB, 3><<<blocks, 256, 0, s>>>(items, start, nitems); break;
-    case 4: kg_bj_apply<TB, 4><<<blocks, 256, 0, s>>>(items, start, nitems); break;
-    default: kg_bj_apply<TB, 8><<<blocks, 256, 0, s>>>(items, start, nitems); break;
+    case 1: ncg_launch(kg_bj_apply<TB, 1>, dim3(blocks), dim3(256), 0, s, items, start, nitems, kp); break;
+    case 2: ncg_launch(kg_bj_apply<TB, 2>, dim3(blocks), dim3(256), 0, s, items, start, nitems, kp); break;
+    case 3: ncg_launch(kg_bj_apply<TB, 3>, dim3(blocks), dim3(256), 0, s, items, start, nitems, kp); break;
+    case 4: ncg_launch(kg_bj_apply<TB, 4>, dim3(blocks), dim3(256), 0, s, items, start, nitems, kp); break;
+    default: ncg_launch(kg_bj_apply<TB, 8>, dim3(blocks), dim3(256), 0, s, items, start, nitems, kp); break;
     }
 }
-void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks)
+void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks,
+                const int *kp)
 {
     if (blocks <= 0) return;
-    if (f32) g_bj_t<float>(s, nr, items, start, nitems, blocks);
-    else g_bj_t<double>(s, nr, items, start, nitems, blocks);
+    if (f32) g_bj_t<float>(s, nr, items, start, nitems, blocks, kp);
+    else g_bj_t<double>(s, nr, items, start, nitems, blocks, kp);
 }
 #undef NCG_G_NR
-void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks)
+void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks, const int *kp)
 {
-    if (blocks > 0) kg_vec<<<blocks, 256, 0, s>>>(op, items, start, nitems);
+    if (blocks > 0) ncg_launch(kg_vec, dim3(blocks), dim3(256), 0, s, op, items, start, nitems, kp);
 }
-void g_multidot(hipStream_t s, int kcount, const GGm *items, const int *start, int nitems, int blocks)
+void g_multidot(hipStream_t s, const int *kp, const GGm *items, const int *start, int nitems, int blocks)
 {
-    if (blocks > 0) kg_multidot<<<blocks, 256, 0, s>>>(kcount, items, start, nitems);
+    if (blocks > 0) ncg_launch(kg_multidot, dim3(blocks), dim3(256), 0, s, kp, items, start, nitems);
 }
-void g_gs_update(hipStream_t s, int kcount, int pass, int m, const GGm *items, const int *start, int nitems, int blocks)
+void g_gs_update(hipStream_t s, int kcount, const int *kp, int pass, int m, const GGm *items, const int *start,
+                 int nitems, int blocks)
 {
-    if (blocks > 0) kg_gs_update<<<blocks, 256, 0, s>>>(kcount, pass, m, items, start, nitems);
+    if (blocks > 0) ncg_launch(kg_gs_update, dim3(blocks), dim3(256), 0, s, kcount, kp, pass, m, items, start, nitems);
 }
-void g_gs_normalize(hipStream_t s, int k, int m, const GGm *items, const int *start, int nitems, int blocks)
+void g_gs_normalize(hipStream_t s, int k, const int *kp, int m, const GGm *items, const int *start, int nitems,
+                    int blocks)
 {
-    if (blocks > 0) kg_gs_normalize<<<blocks, 256, 0, s>>>(k, m, items, start, nitems);
+    if (blocks > 0) ncg_launch(kg_gs_normalize, dim3(blocks), dim3(256), 0, s, k, kp, m, items, start, nitems);
 }
-void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems)
+void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems, int *kp)
 {
-    if (nitems > 0) kg_gm_start<<<nitems, 64, 0, s>>>(m, first, items);
+    if (nitems > 0) ncg_launch(kg_gm_start, dim3(nitems), dim3(64), 0, s, m, first, items, kp);
 }
-void g_gm_rotate(hipStream_t s, int k, int m, const GGm *items, int nitems)
+void g_gm_rotate(hipStream_t s, int k, const int *kp, int m, const GGm *items, int nitems)
 {
-    if (nitems > 0) kg_gm_rotate<<<nitems, 64, 0, s>>>(k, m, items);
+    if (nitems > 0) ncg_launch(kg_gm_rotate, dim3(nitems), dim3(64), 0, s, k, kp, m, items);
 }
 void g_gm_solve(hipStream_t s, int m, const GGm *items, int nitems)
 {
-    if (nitems > 0) kg_gm_solve<<<nitems, 64, 0, s>>>(m, items);
+    if (nitems > 0) ncg_launch(kg_gm_solve, dim3(nitems), dim3(64), 0, s, m, items);
 }
 void g_gm_update(hipStream_t s, int m, int to_z, const GGm *items, const int *start, int nitems, int blocks)
 {
-    if (blocks > 0) kg_gm_update<<<blocks, 256, 0, s>>>(m, to_z, items, start, nitems);
+    if (blocks > 0) ncg_launch(kg_gm_update, dim3(blocks), dim3(256), 0, s, m, to_z, items, start, nitems);
 }
 
 void g_gm_clear(hipStream_t s, const GGm *items, int nitems)
 {
-    if (nitems > 0) kg_gm_clear<<<blocks_for(nitems, 64), 64, 0, s>>>(items, nitems);
+    if (nitems > 0) ncg_launch(kg_gm_clear, dim3(blocks_for(nitems, 64)), dim3(64), 0, s, items, nitems);
 }
-void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out)
+void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out, int *kp, int depth)
 {
-    if (nitems > 0) kg_gm_flags<<<blocks_for(nitems, 64), 64, 0, s>>>(items, nitems, out);
+    if (nitems > 0 && nitems <= 64) ncg_launch(kg_gm_flags, dim3(1), dim3(64), 0, s, items, nitems, out, kp, depth);
 }
 void g_gm_stats(hipStream_t s, const GGm *items, int nitems, int *its, double *rel)
 {
-    if (nitems > 0) kg_gm_stats<<<nitems, 64, 0, s>>>(items, its, rel);
+    if (nitems > 0) ncg_launch(kg_gm_stats, dim3(nitems), dim3(64), 0, s, items, its, rel);
 }
 void g_norms(hipStream_t s, const GNorm *items, const int *start, int nitems, int blocks, double *out)
 {
     if (blocks <= 0) return;
-    kg_norm_partial<<<blocks, 256, 0, s>>>(items, start, nitems);
-    kg_norm_reduce<<<nitems, 256, 0, s>>>(items, out);
+    ncg_launch(kg_norm_partial, dim3(blocks), dim3(256), 0, s, items, start, nitems);
+    ncg_launch(kg_norm_reduce, dim3(nitems), dim3(256), 0, s, items, out);
 }
 
 }  // namespace ncgk

@@ -9,6 +9,17 @@
 
 namespace ncgk {
 
+// While a recorder is set on the calling thread, the launchers below add their kernels to
+// recorder->graph as a chain of kernel nodes (each after the previous one) instead of
+// queuing them on a stream; err holds the first failure.  nullptr: launch again.
+struct GraphRecorder {
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t err = hipSuccess;
+    int nodes = 0;
+};
+void set_graph_recorder(GraphRecorder *r);
+
 // device element arrays (structure of arrays of length n)
 struct DevElements {
     int n;
@@ -246,7 +257,8 @@ struct GBj {        // block-Jacobi blocks (FMM finest clusters or dense diagona
     const int *b0, *m, *elem;
     const long long *boff;
     const void *minv;
-    const void *x;
+    const void *x;                      // input; with a step counter kp: x + *kp * xk doubles
+    long long xk;
     void *y;
     const int *skip;
 };
@@ -276,7 +288,8 @@ struct GGm {
 enum { kVecCopy = 0, kVecSub = 1, kVecAdd = 2, kVecZero = 3 };
 struct GVec {
     long long len;
-    const void *a, *b;
+    const void *a, *b;                  // with a step counter kp: a + *kp * ak doubles
+    long long ak;
     void *y;
     const int *skip;
 };
@@ -296,28 +309,42 @@ void g_apply_D(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *
                int blocks);
 void g_apply_S(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
                int blocks);
-void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks);
-void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks);
+// kp (device int, or null): the group's Arnoldi step counter, which offsets each item's
+// input by its stride (GBj::xk, GVec::ak) so one launch serves every step
+void g_bj_apply(hipStream_t s, int f32, int nr, const GBj *items, const int *start, int nitems, int blocks,
+                const int *kp = nullptr);
+void g_vec(hipStream_t s, int op, const GVec *items, const int *start, int nitems, int blocks,
+           const int *kp = nullptr);
 // GMRES(m) of a group in lockstep; at Arnoldi step k the basis holds kcount = k + 1 vectors.
-//   g_gs_update pass 0/1: projections (multidot partials, g_multidot) summed, w -= V h;
+// The step kernels read k from the group's device counter kp (set to 0 by g_gm_start,
+// advanced by g_gm_flags at the end of each step), so the launches of an Arnoldi step do
+// not depend on k and can be captured once and replayed as a HIP graph; where kp is null
+// the k (kcount) argument is used.
+//   g_multidot: slice partials of <V_j, W> for j < kcount; per item kMdLanes * nrhs * nsl
+//   work-groups (start holds the prefix of nrhs * nsl), lane group jj takes j = jj mod kMdLanes.
+//   g_gs_update pass 0/1: projections (multidot partials) summed, w -= V h;
 //   pass 1 also leaves the partial norms.  kcount = 0 with pass 1: partial norms of W only.
 //   g_gs_normalize: V[k + 1] = W / |W| (k = -1: V[0]), squared norms to h's norm slots.
-//   g_gm_start: beta, residual, done flags of a cycle (first: |b| = beta, counters reset).
+//   g_gm_start: beta, residual, done flags of a cycle (first: |b| = beta, counters reset);
+//   sets *kp = 0.
 //   g_gm_rotate: Hessenberg column k, Givens rotations, residual estimate, done flags.
 //   g_gm_solve: y = H^-1 g per load case (zero past its k); g_gm_update: Z = V y (to_z) or
 //   x += V y.
-void g_multidot(hipStream_t s, int kcount, const GGm *items, const int *start, int nitems, int blocks);
-void g_gs_update(hipStream_t s, int kcount, int pass, int m, const GGm *items, const int *start, int nitems,
-                 int blocks);
-void g_gs_normalize(hipStream_t s, int k, int m, const GGm *items, const int *start, int nitems, int blocks);
-void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems);
-void g_gm_rotate(hipStream_t s, int k, int m, const GGm *items, int nitems);
+constexpr int kMdLanes = 16;
+void g_multidot(hipStream_t s, const int *kp, const GGm *items, const int *start, int nitems, int blocks);
+void g_gs_update(hipStream_t s, int kcount, const int *kp, int pass, int m, const GGm *items, const int *start,
+                 int nitems, int blocks);
+void g_gs_normalize(hipStream_t s, int k, const int *kp, int m, const GGm *items, const int *start, int nitems,
+                    int blocks);
+void g_gm_start(hipStream_t s, int m, int first, const GGm *items, int nitems, int *kp);
+void g_gm_rotate(hipStream_t s, int k, const int *kp, int m, const GGm *items, int nitems);
 void g_gm_solve(hipStream_t s, int m, const GGm *items, int nitems);
 void g_gm_update(hipStream_t s, int m, int to_z, const GGm *items, const int *start, int nitems, int blocks);
-// skip flags cleared (start of a solve) or gathered into out[nitems]; per item and load case
-// the iterations and residual estimates into its/rel[item][64]
+// skip flags cleared (start of a solve) or gathered into out[nitems] (nitems <= 64): with a
+// step counter kp into out + (*kp % depth) * 64, and *kp advanced by one after; per item and
+// load case the iterations and residual estimates into its/rel[item][64]
 void g_gm_clear(hipStream_t s, const GGm *items, int nitems);
-void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out);
+void g_gm_flags(hipStream_t s, const GGm *items, int nitems, int *out, int *kp = nullptr, int depth = 1);
 void g_gm_stats(hipStream_t s, const GGm *items, int nitems, int *its, double *rel);
 // squared norms |v_r|^2 of every item's load cases into out[item][64]; per item gsb * nrhs
 // work-groups (start: their prefix), npart [nrhs][gsb] scratch

@@ -25,6 +25,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -118,6 +119,8 @@ struct ncg_server {
     std::map<std::string, int> tuning_busy;
     std::condition_variable cv_tune;
     std::string tuning_file;
+    std::atomic<int> ktime{0};   // per-kernel timing of the steps started from now on
+    int64_t graph_steps = 0, eager_steps = 0;   // Arnoldi steps of the lanes' solves (replayed / one by one)
 };
 
 namespace {
@@ -381,13 +384,17 @@ void run_lane(ncg_server *S)
         }
         // problems and operators of every step of the group: every assembly is queued on its
         // problem's stream first, so they run side by side, then their error flags are read
+        const int ktime = S->ktime.load();
         for (Taken &t : grp) {
             const Job *J = t.st.job;
             const MeshRec *M = S->meshes[J->mesh].get();
             if (!t.r->p) {
                 t.rc = ncg_problem_create(&M->g, S->cfg.device, &t.r->p);
                 if (t.rc) t.msg = ncg_last_error();
-                else if (S->cfg.kernel_timing) ncg_set_kernel_timing(t.r->p, 1);
+            }
+            if (t.r->p) {
+                ncg_set_kernel_timing(t.r->p, ktime);
+                ncg_set_graphs(t.r->p, !S->cfg.eager);
             }
             if (!t.rc) {
                 t.rc = ncgi::assemble_async(t.r->p, &J->freq[t.st.fi].f, J->precision);
@@ -406,6 +413,8 @@ void run_lane(ncg_server *S)
             tune_group(S, P, ok[0]->st.job->n, J0->nrhs, J0->precision);
         }
         const clk::time_point t_asm = clk::now();
+        int64_t g0[4] = {0, 0, 0, 0}, g1[4] = {0, 0, 0, 0};
+        if (!ok.empty()) ncg_graph_stats(ok[0]->r->p, g0);
         if (ok.size() > 1) {
             std::vector<ncg_problem *> P;
             for (Taken *t : ok) P.push_back(t->r->p);
@@ -434,6 +443,7 @@ void run_lane(ncg_server *S)
             }
             t.refine = ss.refinement_steps;
         }
+        if (!ok.empty()) ncg_graph_stats(ok[0]->r->p, g1);
         const clk::time_point t_solve = clk::now();
         // outputs: the solutions, then every grid field queued before the first is read
         std::vector<char> write(grp.size(), 1);
@@ -472,7 +482,7 @@ void run_lane(ncg_server *S)
         for (size_t i = 0; i < grp.size(); ++i) {
             ncg_kernel_stat ks[64];
             int32_t nks = 0;
-            if (S->cfg.kernel_timing && grp[i].r->p && ncg_kernel_stats(grp[i].r->p, ks, 64, &nks, 1) == 0)
+            if (ktime && grp[i].r->p && ncg_kernel_stats(grp[i].r->p, ks, 64, &nks, 1) == 0)
                 kst[i].assign(ks, ks + std::min(nks, 64));
         }
 
@@ -481,6 +491,8 @@ void run_lane(ncg_server *S)
         S->host_ms[0] += ms_between(t_start, t_asm);
         S->host_ms[1] += ms_between(t_asm, t_solve);
         S->host_ms[2] += ms_between(t_solve, t_out);
+        S->graph_steps += g1[1] - g0[1];
+        S->eager_steps += g1[2] - g0[2];
         t_idle = clk::now();
         for (size_t g = 0; g < grp.size(); ++g) {
             Taken &t = grp[g];
@@ -565,6 +577,7 @@ int ncg_server_create(const ncg_server_config *cfg, ncg_server **out)
     ncg_server *S = new ncg_server();
     S->cfg = c;
     S->cfg.tuning_file = nullptr;
+    S->ktime = c.kernel_timing ? 1 : 0;
     if (c.tuning_file) S->tuning_file = c.tuning_file;
     if (!S->tuning_file.empty()) load_tunings(S);
     S->budget = budget;
@@ -718,6 +731,8 @@ int ncg_server_stats(ncg_server *S, ncg_server_info *st)
     st->queued_steps = (int32_t)S->queue.size();
     for (int i = 0; i < 4; ++i) st->lane_ms[i] = S->host_ms[i];
     st->tuned_shapes = (int32_t)S->tunings.size();
+    st->graph_steps = S->graph_steps;
+    st->eager_steps = S->eager_steps;
     return 0;
 }
 
@@ -732,6 +747,13 @@ int ncg_server_kernel_stats(ncg_server *S, ncg_kernel_stat *out, int32_t cap, in
     }
     *count = c;
     if (reset) S->kstats.clear();
+    return 0;
+}
+
+int ncg_server_set_kernel_timing(ncg_server *S, int32_t on)
+{
+    if (!S) return sfail(NCG_E_ARG, "ncg_server_set_kernel_timing: null server");
+    S->ktime = on ? 1 : 0;
     return 0;
 }
 

@@ -90,7 +90,8 @@ EXPORTS = ("ncg_last_error", "ncg_device_count", "ncg_fmm_plan", "ncg_problem_cr
            "ncg_apply", "ncg_timings", "ncg_server_create", "ncg_server_add_mesh", "ncg_server_submit",
            "ncg_server_wait", "ncg_server_query", "ncg_server_stats", "ncg_server_destroy",
            "ncg_set_kernel_timing", "ncg_kernel_stats", "ncg_server_kernel_stats", "ncg_solve_group",
-           "ncg_set_tuning", "ncg_get_tuning", "ncg_apply_noalloc", "ncg_autotune", "ncg_server_tuning")
+           "ncg_set_tuning", "ncg_get_tuning", "ncg_apply_noalloc", "ncg_autotune", "ncg_server_tuning",
+           "ncg_set_graphs", "ncg_graph_stats", "ncg_server_set_kernel_timing")
 
 
 class _Tuning(ctypes.Structure):
@@ -117,7 +118,7 @@ def _kernel_dict(arr, count):
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("memory_budget_bytes", ctypes.c_double),
                 ("problems_per_lane", ctypes.c_int32), ("kernel_timing", ctypes.c_int32), ("group", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("tuning_file", ctypes.c_char_p)]
+                ("autotune", ctypes.c_int32), ("tuning_file", ctypes.c_char_p), ("eager", ctypes.c_int32)]
 
 
 class _Job(ctypes.Structure):
@@ -138,7 +139,8 @@ class _ServerInfo(ctypes.Structure):
                 ("solves_done", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("wall_s", ctypes.c_double), ("resident_bytes", ctypes.c_int64),
                 ("peak_resident_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("queued_steps", ctypes.c_int32),
-                ("lane_ms", ctypes.c_double * 4), ("tuned_shapes", ctypes.c_int32)]
+                ("lane_ms", ctypes.c_double * 4), ("tuned_shapes", ctypes.c_int32),
+                ("graph_steps", ctypes.c_int64), ("eager_steps", ctypes.c_int64)]
 
 _lib = None
 
@@ -289,6 +291,17 @@ class Problem:
     def kernel_timing(self, on=True):
         _check(self.lib.ncg_set_kernel_timing(self._h, int(on)))
 
+    def graphs(self, on=True):
+        """ncg_set_graphs: GMRES Arnoldi steps replayed as captured HIP graphs (default) or
+        launched kernel by kernel"""
+        _check(self.lib.ncg_set_graphs(self._h, int(on)))
+
+    def graph_stats(self):
+        """ncg_graph_stats: Arnoldi-step graphs captured, steps replayed, steps launched one by one"""
+        out = (ctypes.c_int64 * 4)()
+        _check(self.lib.ncg_graph_stats(self._h, out))
+        return dict(zip(("captured", "replayed", "eager", "cached"), list(out)))
+
     def kernel_stats(self, reset=False):
         arr = (_KernelStat * 64)()
         cnt = ctypes.c_int32()
@@ -362,11 +375,12 @@ class Server:
     with continuous batching over `lanes` concurrent steps and resident problems."""
 
     def __init__(self, device=0, lanes=4, memory_budget=0.0, problems_per_lane=2, kernel_timing=False, group=0,
-                 autotune=False, tuning_file=None):
+                 autotune=False, tuning_file=None, graphs=True):
         self.lib = load()
         self._h = ctypes.c_void_p()
         cfg = _ServerConfig(int(device), int(lanes), float(memory_budget), int(problems_per_lane), int(kernel_timing),
-                            int(group), int(bool(autotune)), tuning_file.encode() if tuning_file else None)
+                            int(group), int(bool(autotune)), tuning_file.encode() if tuning_file else None,
+                            int(not graphs))
         _check(self.lib.ncg_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         self._meshes = {}
         # arrays the server reads or writes for each job that has not ended: they stay
@@ -383,7 +397,7 @@ class Server:
         return mid.value
 
     def submit(self, mesh, freqs, precision=MIXED, solver=GMRES, precond=BLOCK, max_iterations=500, tolerance=1e-9,
-               restart=50, refinement_steps=8, refinement_tolerance=1e-5, ilu_scanning_degree=0, solution=True,
+               restart=50, refinement_steps=8, refinement_tolerance=1e-6, ilu_scanning_degree=0, solution=True,
                field=True, gradient=False):
         """freqs: list of dicts with k and optionally bc ((nrhs, n) complex), adm (n complex),
         point_sources / plane_waves (rows x y z re im rhs), nrhs."""
@@ -427,6 +441,10 @@ class Server:
         st = _JobStatus()
         _check(self.lib.ncg_server_query(self._h, ctypes.c_int64(job.id), ctypes.byref(st)))
         return _status_dict(st)
+
+    def set_kernel_timing(self, on=True):
+        """per-kernel timing of the steps started from now on (turns graph replay off while on)"""
+        _check(self.lib.ncg_server_set_kernel_timing(self._h, int(on)))
 
     def kernel_stats(self, reset=False):
         arr = (_KernelStat * 64)()
