@@ -101,24 +101,39 @@ __global__ void k_element_geometry(int n, const double *X, const int8_t *nv, dou
 // near-pair table: pairs (i, j != i) whose element j must be subdivided as seen from
 // the collocation point of i (ratio < 1.3 at level 0).  Frequency independent.
 // ======================================================================================
-__global__ void k_near_count(Elements E, int *count)
+// One wave per collocation row i, lanes over the columns j; fill keeps the columns of a
+// row in ascending order (ballot prefix per step of 64 columns).
+__global__ void __launch_bounds__(256) k_near_count(Elements E, int *count)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= E.n) return;
     const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
     int c = 0;
-    for (int j = 0; j < E.n; ++j) c += (j != i && pair_is_near(E, j, x)) ? 1 : 0;
-    count[i] = c;
+    for (int j0 = 0; j0 < E.n; j0 += 64) {
+        const int j = j0 + lane;
+        c += __popcll(__ballot(j < E.n && j != i && pair_is_near(E, j, x)));
+    }
+    if (lane == 0) count[i] = c;
 }
 
-__global__ void k_near_fill(Elements E, const long long *offset, int *rows, int *cols)
+__global__ void __launch_bounds__(256) k_near_fill(Elements E, const long long *offset, int *rows, int *cols)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= E.n) return;
     const double x[3] = {E.cen[i], E.cen[E.n + i], E.cen[2 * E.n + i]};
     long long o = offset[i];
-    for (int j = 0; j < E.n; ++j)
-        if (j != i && pair_is_near(E, j, x)) { rows[o] = i; cols[o] = j; ++o; }
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int j0 = 0; j0 < E.n; j0 += 64) {
+        const int j = j0 + lane;
+        const bool near = j < E.n && j != i && pair_is_near(E, j, x);
+        const unsigned long long m = __ballot(near);
+        if (near) {
+            const long long a = o + __popcll(m & below);
+            rows[a] = i;
+            cols[a] = j;
+        }
+        o += __popcll(m);
+    }
 }
 
 // ======================================================================================
@@ -1854,88 +1869,94 @@ __global__ void __launch_bounds__(256) k_bj_gather_dense(int n, const int *elem_
 }
 
 // In-place Gauss-Jordan inversion with partial (row) pivoting of one block (m <= 64) by
-// one wave, the block in LDS: during the elimination lane j owns column j, so a row swap,
-// the scaling of the pivot row and the update of every row are lane-parallel and need no
-// barrier (a wave's LDS operations complete in order).  The row interchanges are undone
-// as column interchanges at the end, lane = row, and the inverse is written back
-// column-major.  TB is the arithmetic and storage type (FP32 in the mixed form).
+// one work-group of 4 waves, the block in LDS, lane j owning column j: per pivot step the
+// first wave finds the pivot, swaps and scales the pivot row, and after a barrier the four
+// waves eliminate their rows (i = wave mod 4) with the same arithmetic per entry as one
+// wave would.  The row interchanges are undone as column interchanges at the end, lane =
+// row, and the inverse is written back column-major.  TB is the arithmetic and storage
+// type (FP32 in the mixed form).
 template <typename TB>
-__global__ void __launch_bounds__(64) k_bj_invert(const int *blk_m, const long long *boff, cx<TB> *blk, int *err)
+__global__ void __launch_bounds__(256) k_bj_invert(const int *blk_m, const long long *boff, cx<TB> *blk, int *err)
 {
     __shared__ cx<TB> A[kBlk * kBlk];
     __shared__ int perm[kBlk];
-    const int b = blockIdx.x, m = blk_m[b], lane = threadIdx.x;
+    const int b = blockIdx.x, m = blk_m[b], lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     cx<TB> *G = blk + boff[b];
     // row-major m x m from the gather
     if (lane < m)
-        for (int i = 0; i < m; ++i) A[i * kBlk + lane] = G[(size_t)i * m + lane];
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int i = wv; i < m; i += 4) A[i * kBlk + lane] = G[(size_t)i * m + lane];
+    __syncthreads();
     bool singular = false;
     for (int k = 0; k < m; ++k) {
-        // pivot: the largest |A[i][k]|, i >= k (first on ties)
-        TB v = TB(-1);
-        int idx = lane;
-        if (lane >= k && lane < m) {
-            const cx<TB> a = A[lane * kBlk + k];
-            v = a.re * a.re + a.im * a.im;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const TB v2 = __shfl_xor(v, o);
-            const int i2 = __shfl_xor(idx, o);
-            if (v2 > v || (v2 == v && i2 < idx)) {
-                v = v2;
-                idx = i2;
+        if (wv == 0) {
+            // pivot: the largest |A[i][k]|, i >= k (first on ties)
+            TB v = TB(-1);
+            int idx = lane;
+            if (lane >= k && lane < m) {
+                const cx<TB> a = A[lane * kBlk + k];
+                v = a.re * a.re + a.im * a.im;
             }
-        }
-        const int pr = __builtin_amdgcn_readfirstlane(idx);
-        if (!(v > TB(0))) singular = true;
-        if (lane == 0) perm[k] = pr;
-        cx<TB> rk = mk(TB(0), TB(0));
-        if (lane < m) {
-            rk = A[pr * kBlk + lane];
-            if (pr != k) {
-                A[pr * kBlk + lane] = A[k * kBlk + lane];
+            for (int o = 32; o > 0; o >>= 1) {
+                const TB v2 = __shfl_xor(v, o);
+                const int i2 = __shfl_xor(idx, o);
+                if (v2 > v || (v2 == v && i2 < idx)) {
+                    v = v2;
+                    idx = i2;
+                }
             }
+            const int pr = __builtin_amdgcn_readfirstlane(idx);
+            if (!(v > TB(0))) singular = true;
+            if (lane == 0) perm[k] = pr;
+            cx<TB> rk = mk(TB(0), TB(0));
+            if (lane < m) {
+                rk = A[pr * kBlk + lane];
+                if (pr != k) A[pr * kBlk + lane] = A[k * kBlk + lane];
+            }
+            const cx<TB> pv = {__shfl(rk.re, k), __shfl(rk.im, k)};   // the pivot, from lane k's column
+            const cx<TB> inv = cdiv(mk(TB(1), TB(0)), pv);
+            rk = lane == k ? inv : rk * inv;
+            if (lane < m) A[k * kBlk + lane] = rk;
         }
-        const cx<TB> pv = {__shfl(rk.re, k), __shfl(rk.im, k)};   // the pivot, from lane k's column
-        const cx<TB> inv = cdiv(mk(TB(1), TB(0)), pv);
-        rk = lane == k ? inv : rk * inv;
-        if (lane < m) A[k * kBlk + lane] = rk;
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        // every other row i: A[i][j] -= A[i][k] * rk_j (column k: 0 - A[i][k] * inv).  Sixteen
-        // rows per batch, their column-k entries read by every lane before any is written.
+        __syncthreads();
+        // every other row i: A[i][j] -= A[i][k] * rk_j (column k: 0 - A[i][k] * inv).  Wave w
+        // takes the rows i = w mod 4, sixteen per batch, their column-k entries read by
+        // every lane before any is written.
+        const cx<TB> rk = A[k * kBlk + min(lane, m - 1)];
         constexpr int RB = 16;
-        for (int i0 = 0; i0 < m; i0 += RB) {
+        for (int i0 = wv; i0 < m; i0 += 4 * RB) {
             cx<TB> f[RB], base[RB];
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
-                const int i = min(i0 + q, m - 1);
+                const int i = min(i0 + 4 * q, m - 1);
                 f[q] = A[i * kBlk + k];
                 base[q] = A[i * kBlk + min(lane, m - 1)];
             }
 #pragma unroll
             for (int q = 0; q < RB; ++q) {
-                const int i = i0 + q;
+                const int i = i0 + 4 * q;
                 if (i < m && i != k && lane < m)
                     A[i * kBlk + lane] = (lane == k ? mk(TB(0), TB(0)) : base[q]) - f[q] * rk;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __syncthreads();
     }
-    if (singular && lane == 0) atomicOr(err, 16);
-    // undo the row interchanges as column interchanges, lane = row
-    for (int k = m - 1; k >= 0; --k) {
-        const int pk = perm[k];
-        if (pk != k && lane < m) {
-            const cx<TB> t0 = A[lane * kBlk + k];
-            A[lane * kBlk + k] = A[lane * kBlk + pk];
-            A[lane * kBlk + pk] = t0;
+    if (wv == 0) {
+        if (singular && lane == 0) atomicOr(err, 16);
+        // undo the row interchanges as column interchanges, lane = row
+        for (int k = m - 1; k >= 0; --k) {
+            const int pk = perm[k];
+            if (pk != k && lane < m) {
+                const cx<TB> t0 = A[lane * kBlk + k];
+                A[lane * kBlk + k] = A[lane * kBlk + pk];
+                A[lane * kBlk + pk] = t0;
+            }
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __syncthreads();
     // column-major: G[j * m + i] = inverse[i][j], lane = i
     if (lane < m)
-        for (int j = 0; j < m; ++j) G[(size_t)j * m + lane] = A[lane * kBlk + j];
+        for (int j = wv; j < m; j += 4) G[(size_t)j * m + lane] = A[lane * kBlk + j];
 }
 
 // y = M^-1 x: one work-group per block; lane a is row a, the 4 waves split the columns
@@ -3088,13 +3109,13 @@ void element_geometry(hipStream_t s, int n, const double *X, const int8_t *nv, d
 void near_count(hipStream_t s, const DevElements &D, int *count)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    ncg_launch(k_near_count, dim3(blocks_for(D.n, 64)), dim3(64), 0, s, E, count);
+    ncg_launch(k_near_count, dim3(blocks_for(D.n, 4)), dim3(256), 0, s, E, count);
 }
 
 void near_fill(hipStream_t s, const DevElements &D, const long long *offset, int *rows, int *cols)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
-    ncg_launch(k_near_fill, dim3(blocks_for(D.n, 64)), dim3(64), 0, s, E, offset, rows, cols);
+    ncg_launch(k_near_fill, dim3(blocks_for(D.n, 4)), dim3(256), 0, s, E, offset, rows, cols);
 }
 
 static BMConst make_bm(const BMParams &b)
@@ -3688,12 +3709,12 @@ void bj_setup(hipStream_t s, int n, int nblk, const int *elem_blk, const int *el
         ncg_launch(k_bj_gather<float, float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m, cl_ptr,
                                                                    near_ptr, self_off, (const cx<float> *)nval, boff,
                                                                    (cx<float> *)blk);
-        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<float> *)blk, err);
+        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(256), 0, s, blk_m, boff, (cx<float> *)blk, err);
     } else {
         ncg_launch(k_bj_gather<double, double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, elem_pos, blk_cl, blk_b0, blk_m,
                                                                      cl_ptr, near_ptr, self_off,
                                                                      (const cx<double> *)nval, boff, (cx<double> *)blk);
-        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<double> *)blk, err);
+        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(256), 0, s, blk_m, boff, (cx<double> *)blk, err);
     }
 }
 void bj_setup_dense(hipStream_t s, int n, int nblk, const int *elem_blk, const int *blk_b0, const int *blk_m,
@@ -3702,12 +3723,12 @@ void bj_setup_dense(hipStream_t s, int n, int nblk, const int *elem_blk, const i
     if (mixed) {
         ncg_launch(k_bj_gather_dense<float, float>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, blk_b0, blk_m, (const cx<float> *)A,
                                                                          boff, (cx<float> *)blk);
-        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<float> *)blk, err);
+        ncg_launch(k_bj_invert<float>, dim3(nblk), dim3(256), 0, s, blk_m, boff, (cx<float> *)blk, err);
     } else {
         ncg_launch(k_bj_gather_dense<double, double>, dim3(blocks_for(n, 4)), dim3(256), 0, s, n, elem_blk, blk_b0, blk_m,
                                                                            (const cx<double> *)A, boff,
                                                                            (cx<double> *)blk);
-        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(64), 0, s, blk_m, boff, (cx<double> *)blk, err);
+        ncg_launch(k_bj_invert<double>, dim3(nblk), dim3(256), 0, s, blk_m, boff, (cx<double> *)blk, err);
     }
 }
 template <typename TB>
