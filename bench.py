@@ -458,10 +458,13 @@ def head_main(a):
     for j in [srv.submit(mid, sweep, **SERVED) for _ in range(a.warmup)]:
         j.wait()
     lane0 = dict(srv.stats()["lane_ms"])
+    mono = {}
+    mono["timed"] = [time.monotonic_ns()]
     t0 = time.perf_counter()
     jobs = [srv.submit(mid, sweep, **SERVED) for _ in range(a.steps)]
     sts = [j.wait() for j in jobs]
     elapsed = time.perf_counter() - t0
+    mono["timed"].append(time.monotonic_ns())
     lane1 = srv.stats()["lane_ms"]
     failed = [s for s in sts if s["state"] != "done"]
     if failed:
@@ -474,10 +477,12 @@ def head_main(a):
     nroof = max(1, min(a.steps, a.roofline_steps))
     srv.set_kernel_timing(True)
     srv.kernel_stats(reset=True)
+    mono["per_launch_pass"] = [time.monotonic_ns()]
     t0r = time.perf_counter()
     rjobs = [srv.submit(mid, sweep, **SERVED) for _ in range(nroof)]
     rsts = [j.wait() for j in rjobs]
     elapsed_r = time.perf_counter() - t0r
+    mono["per_launch_pass"].append(time.monotonic_ns())
     kst = srv.kernel_stats()
     srv.set_kernel_timing(False)
     print("# timed-kernel pass: %d sweeps in %.2f s, %.2f ms per frequency step" % (
@@ -657,6 +662,7 @@ def head_main(a):
                    "launch_shapes": {k: t for k, t in tunings} if tunings else "defaults (autotune off)"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "regions_monotonic_ns": mono,
     }
     print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
 

@@ -2942,21 +2942,25 @@ __global__ void __launch_bounds__(64) kg_gm_rotate(int kh, const int *kp, int m,
         double *sc = it.sc + 4 * r;
         dn = ic[1] != 0;
         if (!dn) {
-            const cx<double> *h1 = (const cx<double> *)it.h + (size_t)r * kc;
-            const cx<double> *h2 = (const cx<double> *)it.h + (size_t)it.nrhs * (m + 1) + (size_t)r * kc;
-            const cx<double> *hn = (const cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
-            cx<double> *H = (cx<double> *)it.H + (size_t)r * (m + 1) * m;
-            cx<double> *cs = (cx<double> *)it.cs + (size_t)r * m, *sn = (cx<double> *)it.sn + (size_t)r * m;
-            cx<double> *g = (cx<double> *)it.g + (size_t)r * (m + 1);
-            for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = h1[i] + h2[i];
-            H[(size_t)(k + 1) * m + k] = mk(sqrt(hn[3 * r].re), 0.0);
+            // separate arrays of the member's GMRES state (restrict: the column's stores do
+            // not order the loads of the next rotations)
+            const cx<double> *__restrict__ h1 = (const cx<double> *)it.h + (size_t)r * kc;
+            const cx<double> *__restrict__ h2 = (const cx<double> *)it.h + (size_t)it.nrhs * (m + 1) + (size_t)r * kc;
+            const cx<double> *__restrict__ hn = (const cx<double> *)it.h + (size_t)2 * it.nrhs * (m + 1);
+            cx<double> *__restrict__ H = (cx<double> *)it.H + (size_t)r * (m + 1) * m;
+            cx<double> *__restrict__ cs = (cx<double> *)it.cs + (size_t)r * m;
+            cx<double> *__restrict__ sn = (cx<double> *)it.sn + (size_t)r * m;
+            cx<double> *__restrict__ g = (cx<double> *)it.g + (size_t)r * (m + 1);
+            // the new column h_i = h1_i + h2_i (i <= k), then the previous rotations; the
+            // entry carried down the column stays in registers (no store-then-load chain)
+            cx<double> a = h1[0] + h2[0];
             for (int i = 0; i < k; ++i) {   // previous rotations
-                const cx<double> a = H[(size_t)i * m + k], bb = H[(size_t)(i + 1) * m + k];
+                const cx<double> bb = h1[i + 1] + h2[i + 1];
                 const cx<double> ci = mk(cs[i].re, -cs[i].im), si = mk(sn[i].re, -sn[i].im);
                 H[(size_t)i * m + k] = ci * a + si * bb;
-                H[(size_t)(i + 1) * m + k] = mk(0.0, 0.0) - sn[i] * a + cs[i] * bb;
+                a = mk(0.0, 0.0) - sn[i] * a + cs[i] * bb;
             }
-            const cx<double> a = H[(size_t)k * m + k], bb = H[(size_t)(k + 1) * m + k];
+            const cx<double> bb = mk(sqrt(hn[3 * r].re), 0.0);
             const double den = sqrt(a.re * a.re + a.im * a.im + bb.re * bb.re + bb.im * bb.im);
             const cx<double> c = den > 0.0 ? a * (1.0 / den) : mk(1.0, 0.0);
             const cx<double> sv = den > 0.0 ? bb * (1.0 / den) : mk(0.0, 0.0);
