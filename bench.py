@@ -567,12 +567,14 @@ def head_main(a):
                                       "algorithmic_bytes": kst[k]["bytes"]} for k in sorted(kst) if kst[k]["bytes"]},
                 "quadrature_served": quadrature_roofline(kst)}
         if not a.no_isolated:
+            mono["isolated"] = [time.monotonic_ns()]
             iso = isolated_kernels(geo, src, iso_f)
             roof["isolated_single_problem"] = {
                 k: {"mean_launch_ms": v["total_ms"] / v["calls"], "achieved_gbs": v["bytes"] / v["total_ms"] / 1e6}
                 for k, v in iso.items() if v["calls"] and v["bytes"]}
             roof["quadrature_isolated"] = {"%.0f Hz" % f: quadrature_roofline(isolated_assembly(geo, src, f))
                                            for f in (iso_f, freqs[-1])}
+            mono["isolated"].append(time.monotonic_ns())
 
     # ---- CPU baseline: the reference NumCalc as a manage_numcalc-style pool -------------
     cpu = None
