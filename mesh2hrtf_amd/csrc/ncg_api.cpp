@@ -3256,7 +3256,7 @@ int autotune_group(const std::vector<ncg_problem *> &P, int reps, ncg_tuning &be
         ApplyPlan ap;
         plan_apply(st, P, in, out, noskip, 1, 0, ap);
         if ((rc = st.upload(s))) return rc;
-        for (int w : {8, 4, 2}) {
+        for (int w : {8, 4, 2, 1}) {
             lead->tune.t_waves = lead->tune.d_waves = lead->tune.s_waves = w;
             run_apply(st, lead, P, in, out, ap);
             take_ms(lead, {});

@@ -59,7 +59,7 @@ def test_near_mode_and_waves():
             assert R.rel_l2(b[r], a[r]) < 1e-10
     t = ncg.autotune(probs, reps=2)
     print("chosen", t, flush=True)
-    assert t["tuned"] == 1 and t["t_waves"] in (2, 4, 8) and t["d_waves"] in (2, 4, 8) and t["s_waves"] in (2, 4, 8)
+    assert t["tuned"] == 1 and t["t_waves"] in (1, 2, 4, 8) and t["d_waves"] in (1, 2, 4, 8) and t["s_waves"] in (1, 2, 4, 8)
     assert t["near_mode"] in (0, 1, 2, 3) and t["near_assembly"] in (0, 1)
     for p in probs:
         assert p.tuning() == t
