@@ -2311,14 +2311,19 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, 
 // y[e] += sum_p S[e, p] s[cluster(e), p].  One work-group of 8 waves per tile of at most
 // 64 elements of one cluster (tl_*: cluster, first local element, elements): lane a is
 // element a0 + a of the cluster's column-major S block, so every load is a contiguous run
-// of the block; the waves split the points, four per step, s[c, p] is wave-uniform, and
-// the partial sums meet in an LDS tree.
+// of the block; the waves split the points, four per step, and the partial sums meet in an
+// LDS tree.  s[c, p] is wave-uniform and read from LDS, where the work-group stages it
+// kSStage points at a time (read through the scalar cache instead, a launch took 1.4x as
+// long).
+constexpr int kSStage = 256;
 template <typename S, typename W, int NR, int NW = kSumWaves>
 __device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const int *tl_cl, const int *tl_a0,
                                              const int *tl_m, const int *cl_ptr, const int *cl_elem, const cx<S> *Sv,
                                              const cx<W> *sw, cx<double> *y)
 {
+    static_assert(kSStage % NW == 0, "a stage keeps each point on its wave");
     __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
+    __shared__ cx<double> ss[NR][kSStage];
     const int c = tl_cl[q], a0 = tl_a0[q], m = tl_m[q];
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
@@ -2329,26 +2334,32 @@ __device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
-    if (on) {
-        int p = wv;
-        for (; p + 3 * NW < P; p += 4 * NW) {
-            cx<S> a[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) a[j] = blk[(size_t)(p + NW * j) * mc];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const cx<W> t = sc[r * ld + p + NW * j];
-                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), mk(double(t.re), double(t.im)));
-                }
-        }
-        for (; p < P; p += NW) {
-            const cx<S> a = blk[(size_t)p * mc];
+    for (int p0 = 0; p0 < P; p0 += kSStage) {
+        const int pn = min(P, p0 + kSStage);
+        if (p0 > 0) __syncthreads();   // the previous stage has been read
+        for (int u = threadIdx.x; u < pn - p0; u += 64 * NW)
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const cx<W> t = sc[r * ld + p];
-                cmac(re[r], im[r], double(a.re), double(a.im), mk(double(t.re), double(t.im)));
+                const cx<W> t = sc[r * ld + p0 + u];
+                ss[r][u] = mk(double(t.re), double(t.im));
+            }
+        __syncthreads();
+        if (on) {
+            int p = p0 + wv;
+            for (; p + 3 * NW < pn; p += 4 * NW) {
+                cx<S> a[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a[j] = blk[(size_t)(p + NW * j) * mc];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r)
+                        cmac(re[r], im[r], double(a[j].re), double(a[j].im), ss[r][p + NW * j - p0]);
+            }
+            for (; p < pn; p += NW) {
+                const cx<S> a = blk[(size_t)p * mc];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), ss[r][p - p0]);
             }
         }
     }
