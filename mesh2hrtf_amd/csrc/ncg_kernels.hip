@@ -61,6 +61,22 @@ void set_graph_recorder(GraphRecorder *r) { t_recorder = r; }
 
 namespace ncg {
 
+// Pointers into device memory that device code receives through descriptors in memory are
+// generic to the compiler, and generic loads are flat loads (slower, and waited for
+// together with LDS operations).  Parameters declared NCG_GLOBAL tell it they address
+// global memory, so the loads through them are global loads.  The host pass, which only
+// checks device code, sees plain pointers.
+#ifdef __HIP_DEVICE_COMPILE__
+#define NCG_GLOBAL __attribute__((address_space(1)))
+#else
+#define NCG_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ NCG_GLOBAL T *to_global(T *p)
+{
+    return (NCG_GLOBAL T *)p;
+}
+
 // ======================================================================================
 // geometry: centre, unit normal, area, Jacobian per element (NC_ReadMesh,
 // src/NC_Input.cpp:522-605; triangle centre s = t = 1/3 and area 0.5*J, quad centre
@@ -2089,7 +2105,8 @@ __device__ __forceinline__ void wave_sum_rows(double (&re)[RW][NR], double (&im)
 // values of RW rows for one batch of U steps of 64 columns from b + lane (indices past sl
 // clamped to sl - 1)
 template <typename S, int RW, int U>
-__device__ __forceinline__ void rows_load(const cx<S> *const (&vr)[RW], int b, int sl, int lane, cx<S> (&v)[U][RW])
+__device__ __forceinline__ void rows_load(const NCG_GLOBAL cx<S> *const (&vr)[RW], int b, int sl, int lane,
+                                          cx<S> (&v)[U][RW])
 {
 #pragma unroll
     for (int q = 0; q < U; ++q) {
@@ -2104,7 +2121,8 @@ __device__ __forceinline__ void rows_load(const cx<S> *const (&vr)[RW], int b, i
 // (rows_load at b = 0); each batch's successor is loaded before the batch is used, so
 // two batches of loads are in flight.  Indices past sl are clamped and weighted by zero.
 template <typename S, int NR, int RW, int U>
-__device__ __forceinline__ void rows_dot(const cx<S> *const (&vr)[RW], int sl, const cx<double> *xs, int seg, int lane,
+__device__ __forceinline__ void rows_dot(const NCG_GLOBAL cx<S> *const (&vr)[RW], int sl, const cx<double> *xs, int seg,
+                                         int lane,
                                          cx<S> (&v)[U][RW], double (&re)[RW][NR], double (&im)[RW][NR])
 {
     for (int b = 0; b < sl; b += 64 * U) {   // wave-uniform trip count
@@ -2136,9 +2154,11 @@ __device__ __forceinline__ void rows_dot(const cx<S> *const (&vr)[RW], int sl, c
 // offsets and column list come from one table, so the first loads of a work-group do not
 // depend on each other.
 template <typename S, int NR>
-__device__ __forceinline__ void near_spmv_item(int q, int n, int seg, const long long *ch_cb, const int *ch_w,
-                                               const int *ch_row, const long long *ch_off, const int *col,
-                                               const cx<S> *val, const cx<double> *x, cx<double> *y)
+__device__ __forceinline__ void near_spmv_item(int q, int n, int seg, const NCG_GLOBAL long long *ch_cb,
+                                               const NCG_GLOBAL int *ch_w, const NCG_GLOBAL int *ch_row,
+                                               const NCG_GLOBAL long long *ch_off, const NCG_GLOBAL int *col,
+                                               const NCG_GLOBAL cx<S> *val, const NCG_GLOBAL cx<double> *x,
+                                               NCG_GLOBAL cx<double> *y)
 {
     extern __shared__ double xs_sh[];   // [NR][seg] complex
     cx<double> *xs = (cx<double> *)xs_sh;
@@ -2148,7 +2168,7 @@ __device__ __forceinline__ void near_spmv_item(int q, int n, int seg, const long
     const int w = ch_w[q];
     constexpr int RW = ncgk::kNearRowsPerWave;
     int row[RW];
-    const cx<S> *vr[RW];
+    const NCG_GLOBAL cx<S> *vr[RW];
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int a = ncgk::kNearRows * q + wv + 4 * j;
@@ -2164,7 +2184,7 @@ __device__ __forceinline__ void near_spmv_item(int q, int n, int seg, const long
         const int sl = min(seg, w - s0);
         // unused row slots read the first row of the cluster and are not stored; the first
         // batch of values is requested before the gather so the two latencies overlap
-        const cx<S> *vs[RW];
+        const NCG_GLOBAL cx<S> *vs[RW];
 #pragma unroll
         for (int j = 0; j < RW; ++j) vs[j] = vr[j] + s0;
         cx<S> v[2][RW];
@@ -2192,15 +2212,17 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
                                                    const int *ch_row, const long long *ch_off, const int *col,
                                                    const cx<S> *val, const cx<double> *x, cx<double> *y)
 {
-    near_spmv_item<S, NR>(blockIdx.x, n, seg, ch_cb, ch_w, ch_row, ch_off, col, val, x, y);
+    near_spmv_item<S, NR>(blockIdx.x, n, seg, to_global(ch_cb), to_global(ch_w), to_global(ch_row), to_global(ch_off),
+                          to_global(col), to_global(val), to_global(x), to_global(y));
 }
 
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
 // of 64 sphere points); its waves take every kSumWaves-th element of the cluster, four
 // per step, and their partial sums meet in LDS.  W is the storage type of t.
 template <typename S, typename W, int NR, int NW = kSumWaves>
-__device__ __forceinline__ void apply_T_item(int blk, int n, int ncl, int P, const int *cl_ptr, const int *cl_elem,
-                                             const cx<S> *Tv, const cx<double> *x, cx<W> *tw)
+__device__ __forceinline__ void apply_T_item(int blk, int n, int ncl, int P, const NCG_GLOBAL int *cl_ptr,
+                                             const NCG_GLOBAL int *cl_elem, const NCG_GLOBAL cx<S> *Tv,
+                                             const NCG_GLOBAL cx<double> *x, NCG_GLOBAL cx<W> *tw)
 {
     __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
     const int tiles = (P + 63) / 64;
@@ -2246,15 +2268,17 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, 
                                                                 const int *cl_elem, const cx<S> *Tv, const cx<double> *x,
                                                                 cx<W> *tw)
 {
-    apply_T_item<S, W, NR>(blockIdx.x, n, ncl, P, cl_ptr, cl_elem, Tv, x, tw);
+    apply_T_item<S, W, NR>(blockIdx.x, n, ncl, P, to_global(cl_ptr), to_global(cl_elem), to_global(Tv), to_global(x),
+                           to_global(tw));
 }
 
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
 // per (cluster, tile of 64 points), the waves splitting the interaction list four
 // entries per step.  WI / WO are the storage types of t and s.
 template <typename S, typename WI, typename WO, int NR, int NW = kSumWaves>
-__device__ __forceinline__ void apply_D_item(int blk, int ncl, int P, const int *fan_ptr, const int *fan_idx,
-                                             const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
+__device__ __forceinline__ void apply_D_item(int blk, int ncl, int P, const NCG_GLOBAL int *fan_ptr,
+                                             const NCG_GLOBAL int *fan_idx, const NCG_GLOBAL cx<S> *Dv,
+                                             const NCG_GLOBAL cx<WI> *tw, NCG_GLOBAL cx<WO> *sw)
 {
     __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
     const int tiles = (P + 63) / 64;
@@ -2305,7 +2329,8 @@ template <typename S, typename WI, typename WO, int NR>
 __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, const int *fan_ptr, const int *fan_idx,
                                                                 const cx<S> *Dv, const cx<WI> *tw, cx<WO> *sw)
 {
-    apply_D_item<S, WI, WO, NR>(blockIdx.x, ncl, P, fan_ptr, fan_idx, Dv, tw, sw);
+    apply_D_item<S, WI, WO, NR>(blockIdx.x, ncl, P, to_global(fan_ptr), to_global(fan_idx), to_global(Dv),
+                                to_global(tw), to_global(sw));
 }
 
 // y[e] += sum_p S[e, p] s[cluster(e), p].  One work-group of 8 waves per tile of at most
@@ -2317,9 +2342,11 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_D(int ncl, int P, 
 // long).
 constexpr int kSStage = 256;
 template <typename S, typename W, int NR, int NW = kSumWaves>
-__device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const int *tl_cl, const int *tl_a0,
-                                             const int *tl_m, const int *cl_ptr, const int *cl_elem, const cx<S> *Sv,
-                                             const cx<W> *sw, cx<double> *y)
+__device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const NCG_GLOBAL int *tl_cl,
+                                             const NCG_GLOBAL int *tl_a0, const NCG_GLOBAL int *tl_m,
+                                             const NCG_GLOBAL int *cl_ptr, const NCG_GLOBAL int *cl_elem,
+                                             const NCG_GLOBAL cx<S> *Sv, const NCG_GLOBAL cx<W> *sw,
+                                             NCG_GLOBAL cx<double> *y)
 {
     static_assert(kSStage % NW == 0, "a stage keeps each point on its wave");
     __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
@@ -2328,9 +2355,9 @@ __device__ __forceinline__ void apply_S_item(int q, int n, int ncl, int P, const
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = cl_ptr[c], mc = cl_ptr[c + 1] - b;
     const bool on = lane < m;
-    const cx<S> *blk = Sv + (size_t)b * P + a0 + (on ? lane : 0);
+    const NCG_GLOBAL cx<S> *blk = Sv + (size_t)b * P + a0 + (on ? lane : 0);
     const size_t ld = (size_t)ncl * P;
-    const cx<W> *sc = sw + (size_t)c * P;
+    const NCG_GLOBAL cx<W> *sc = sw + (size_t)c * P;
     double re[NR], im[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
@@ -2377,7 +2404,8 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, 
                                                                 const int *cl_elem, const cx<S> *Sv, const cx<W> *sw,
                                                                 cx<double> *y)
 {
-    apply_S_item<S, W, NR>(blockIdx.x, n, ncl, P, tl_cl, tl_a0, tl_m, cl_ptr, cl_elem, Sv, sw, y);
+    apply_S_item<S, W, NR>(blockIdx.x, n, ncl, P, to_global(tl_cl), to_global(tl_a0), to_global(tl_m), to_global(cl_ptr),
+                           to_global(cl_elem), to_global(Sv), to_global(sw), to_global(y));
 }
 
 // ---- FMM evaluation grid (NC_ContributionFMM, src/NC_PostProcessing.cpp:978-1214) ------
@@ -2579,8 +2607,9 @@ __global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, co
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GNear it = items[d];
     if (it.skip && *it.skip) return;
-    near_spmv_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cb, it.w, it.row, it.off, it.col, (const cx<S> *)it.val,
-                          (const cx<double> *)it.x, (cx<double> *)it.y);
+    near_spmv_item<S, NR>(blockIdx.x - start[d], it.n, seg, to_global(it.cb), to_global(it.w), to_global(it.row),
+                          to_global(it.off), to_global(it.col), to_global((const cx<S> *)it.val),
+                          to_global((const cx<double> *)it.x), to_global((cx<double> *)it.y));
 }
 
 // Near field with one work-group per finest cluster: when the cluster's column list fits one
@@ -2589,10 +2618,11 @@ __global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, co
 // reduced and stored; each row sums as in near_spmv_item (same lanes, same order).  Wider
 // clusters run near_spmv_item chunk by chunk.
 template <typename S, int NR>
-__device__ __forceinline__ void near_cluster_item(int c, int n, int seg, const int *cptr, const long long *ch_cb,
-                                                  const int *ch_w, const int *ch_row, const long long *ch_off,
-                                                  const int *col, const cx<S> *val, const cx<double> *x,
-                                                  cx<double> *y)
+__device__ __forceinline__ void near_cluster_item(int c, int n, int seg, const NCG_GLOBAL int *cptr,
+                                                  const NCG_GLOBAL long long *ch_cb, const NCG_GLOBAL int *ch_w,
+                                                  const NCG_GLOBAL int *ch_row, const NCG_GLOBAL long long *ch_off,
+                                                  const NCG_GLOBAL int *col, const NCG_GLOBAL cx<S> *val,
+                                                  const NCG_GLOBAL cx<double> *x, NCG_GLOBAL cx<double> *y)
 {
     extern __shared__ double xs_sh[];   // [NR][seg] complex
     cx<double> *xs = (cx<double> *)xs_sh;
@@ -2605,7 +2635,7 @@ __device__ __forceinline__ void near_cluster_item(int c, int n, int seg, const i
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long cb = ch_cb[q0];
     constexpr int RW = ncgk::kNearRowsPerWave;
-    const cx<S> *vr[RW];
+    const NCG_GLOBAL cx<S> *vr[RW];
     int row[RW];
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
@@ -2657,8 +2687,9 @@ __global__ void __launch_bounds__(256) kg_near_cluster(const ncgk::GNear *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GNear it = items[d];
     if (it.skip && *it.skip) return;
-    near_cluster_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cptr, it.cb, it.w, it.row, it.off, it.col,
-                             (const cx<S> *)it.val, (const cx<double> *)it.x, (cx<double> *)it.y);
+    near_cluster_item<S, NR>(blockIdx.x - start[d], it.n, seg, to_global(it.cptr), to_global(it.cb), to_global(it.w),
+                             to_global(it.row), to_global(it.off), to_global(it.col), to_global((const cx<S> *)it.val),
+                             to_global((const cx<double> *)it.x), to_global((cx<double> *)it.y));
 }
 
 // Near field as a persistent launch: a grid sized to what the device holds at once walks
@@ -2679,10 +2710,10 @@ struct NearUnit {
     int q, w, n;
     long long cb;
     int row[RW];
-    const cx<S> *vr[RW];
-    const int *col;
-    const cx<double> *x;
-    cx<double> *y;
+    const NCG_GLOBAL cx<S> *vr[RW];
+    const NCG_GLOBAL int *col;
+    const NCG_GLOBAL cx<double> *x;
+    NCG_GLOBAL cx<double> *y;
 };
 
 template <typename S, int RW>
@@ -2693,17 +2724,17 @@ __device__ __forceinline__ void near_unit(const ncgk::GNear *items, const int *s
     const ncgk::GNear it = items[d];
     u.q = b - start[d];
     u.n = it.n;
-    u.cb = it.cb[u.q];
+    u.cb = to_global(it.cb)[u.q];
     // a converged problem's chunks are empty units (no loads, no stores)
-    u.w = (it.skip && *it.skip) ? 0 : it.w[u.q];
-    u.col = it.col;
-    u.x = (const cx<double> *)it.x;
-    u.y = (cx<double> *)it.y;
+    u.w = (it.skip && *it.skip) ? 0 : to_global(it.w)[u.q];
+    u.col = to_global(it.col);
+    u.x = to_global((const cx<double> *)it.x);
+    u.y = to_global((cx<double> *)it.y);
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int a = ncgk::kNearRows * u.q + wv + 4 * j;
-        u.row[j] = it.row[a];
-        u.vr[j] = (const cx<S> *)it.val + it.off[a];
+        u.row[j] = to_global(it.row)[a];
+        u.vr[j] = to_global((const cx<S> *)it.val) + to_global(it.off)[a];
     }
 }
 
@@ -2748,7 +2779,7 @@ __global__ void __launch_bounds__(256) kq_near_spmv(const ncgk::GNear *items, co
             for (int r = 0; r < NR; ++r) re[j][r] = im[j][r] = 0.0;
         for (int s0 = 0; s0 < cu.w; s0 += seg) {
             const int sl = min(seg, cu.w - s0);
-            const cx<S> *vs[RW];
+            const NCG_GLOBAL cx<S> *vs[RW];
 #pragma unroll
             for (int j = 0; j < RW; ++j) vs[j] = cu.vr[j] + s0;
             if (s0 > 0) {   // later segments (rows longer than seg): as near_spmv_item
@@ -2807,8 +2838,9 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem, (const cx<S> *)it.Tv,
-                           (const cx<double> *)it.x, (cx<W> *)it.tw);
+    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, to_global(it.cl_ptr), to_global(it.cl_elem),
+                               to_global((const cx<S> *)it.Tv), to_global((const cx<double> *)it.x),
+                               to_global((cx<W> *)it.tw));
 }
 
 template <typename S, typename W, int NR, int NW>
@@ -2817,8 +2849,9 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_D(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx, (const cx<S> *)it.Dv,
-                              (const cx<W> *)it.tw, (cx<W> *)it.sw);
+    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, to_global(it.fan_ptr), to_global(it.fan_idx),
+                                  to_global((const cx<S> *)it.Dv), to_global((const cx<W> *)it.tw),
+                                  to_global((cx<W> *)it.sw));
 }
 
 template <typename S, typename W, int NR, int NW>
@@ -2827,8 +2860,10 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_S(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_S_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.tl_cl, it.tl_a0, it.tl_m, it.cl_ptr,
-                           it.cl_elem, (const cx<S> *)it.Sv, (const cx<W> *)it.sw, (cx<double> *)it.y);
+    apply_S_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, to_global(it.tl_cl), to_global(it.tl_a0),
+                               to_global(it.tl_m), to_global(it.cl_ptr), to_global(it.cl_elem),
+                               to_global((const cx<S> *)it.Sv), to_global((const cx<W> *)it.sw),
+                               to_global((cx<double> *)it.y));
 }
 
 template <typename TB, int NR>
