@@ -5,6 +5,8 @@
 //   A      [n][n]  cx<S>, S = double (NCG_FP64) or float (NCG_MIXED)
 //   vectors [nrhs][n] cx<double>
 //   element data: structure of arrays (see ncg::Elements)
+// the descriptors of the grouped launches hold global-memory pointers in this unit's device pass
+#define NCG_KERNELS_TU 1
 #include "ncg_kernels.hpp"
 
 #include <map>
@@ -61,16 +63,7 @@ void set_graph_recorder(GraphRecorder *r) { t_recorder = r; }
 
 namespace ncg {
 
-// Pointers into device memory that device code receives through descriptors in memory are
-// generic to the compiler, and generic loads are flat loads (slower, and waited for
-// together with LDS operations).  Parameters declared NCG_GLOBAL tell it they address
-// global memory, so the loads through them are global loads.  The host pass, which only
-// checks device code, sees plain pointers.
-#ifdef __HIP_DEVICE_COMPILE__
-#define NCG_GLOBAL __attribute__((address_space(1)))
-#else
-#define NCG_GLOBAL
-#endif
+// a pointer into global memory (NCG_GLOBAL, ncg_kernels.hpp) from a generic one
 template <typename T>
 __device__ __forceinline__ NCG_GLOBAL T *to_global(T *p)
 {
@@ -421,11 +414,12 @@ __global__ void __launch_bounds__(256) k_dots(int n, int nrhs, int ndots, const 
                                               const cx<double> *v2, int conj_mask, cx<double> *partial)
 {
     __shared__ double sre[256], sim[256];
-    const cx<double> *U[3] = {u0, u1, u2}, *V[3] = {v0, v1, v2};
+    const NCG_GLOBAL cx<double> *U[3] = {to_global(u0), to_global(u1), to_global(u2)};
+    const NCG_GLOBAL cx<double> *V[3] = {to_global(v0), to_global(v1), to_global(v2)};
     for (int r = 0; r < nrhs; ++r)
         for (int q = 0; q < ndots; ++q) {
             double re = 0.0, im = 0.0;
-            const cx<double> *u = U[q] + (size_t)r * n, *v = V[q] + (size_t)r * n;
+            const NCG_GLOBAL cx<double> *u = U[q] + (size_t)r * n, *v = V[q] + (size_t)r * n;
             for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
                 cx<double> a = u[i];
                 if ((conj_mask >> q) & 1) a.im = -a.im;
@@ -2607,8 +2601,8 @@ __global__ void __launch_bounds__(256) kg_near_spmv(const ncgk::GNear *items, co
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GNear it = items[d];
     if (it.skip && *it.skip) return;
-    near_spmv_item<S, NR>(blockIdx.x - start[d], it.n, seg, to_global(it.cb), to_global(it.w), to_global(it.row),
-                          to_global(it.off), to_global(it.col), to_global((const cx<S> *)it.val),
+    near_spmv_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cb, it.w, it.row,
+                          it.off, it.col, to_global((const cx<S> *)it.val),
                           to_global((const cx<double> *)it.x), to_global((cx<double> *)it.y));
 }
 
@@ -2687,8 +2681,8 @@ __global__ void __launch_bounds__(256) kg_near_cluster(const ncgk::GNear *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GNear it = items[d];
     if (it.skip && *it.skip) return;
-    near_cluster_item<S, NR>(blockIdx.x - start[d], it.n, seg, to_global(it.cptr), to_global(it.cb), to_global(it.w),
-                             to_global(it.row), to_global(it.off), to_global(it.col), to_global((const cx<S> *)it.val),
+    near_cluster_item<S, NR>(blockIdx.x - start[d], it.n, seg, it.cptr, it.cb, it.w,
+                             it.row, it.off, it.col, to_global((const cx<S> *)it.val),
                              to_global((const cx<double> *)it.x), to_global((cx<double> *)it.y));
 }
 
@@ -2724,17 +2718,17 @@ __device__ __forceinline__ void near_unit(const ncgk::GNear *items, const int *s
     const ncgk::GNear it = items[d];
     u.q = b - start[d];
     u.n = it.n;
-    u.cb = to_global(it.cb)[u.q];
+    u.cb = it.cb[u.q];
     // a converged problem's chunks are empty units (no loads, no stores)
-    u.w = (it.skip && *it.skip) ? 0 : to_global(it.w)[u.q];
-    u.col = to_global(it.col);
+    u.w = (it.skip && *it.skip) ? 0 : it.w[u.q];
+    u.col = it.col;
     u.x = to_global((const cx<double> *)it.x);
     u.y = to_global((cx<double> *)it.y);
 #pragma unroll
     for (int j = 0; j < RW; ++j) {
         const int a = ncgk::kNearRows * u.q + wv + 4 * j;
-        u.row[j] = to_global(it.row)[a];
-        u.vr[j] = to_global((const cx<S> *)it.val) + to_global(it.off)[a];
+        u.row[j] = it.row[a];
+        u.vr[j] = to_global((const cx<S> *)it.val) + it.off[a];
     }
 }
 
@@ -2838,7 +2832,7 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, to_global(it.cl_ptr), to_global(it.cl_elem),
+    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem,
                                to_global((const cx<S> *)it.Tv), to_global((const cx<double> *)it.x),
                                to_global((cx<W> *)it.tw));
 }
@@ -2849,7 +2843,7 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_D(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, to_global(it.fan_ptr), to_global(it.fan_idx),
+    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx,
                                   to_global((const cx<S> *)it.Dv), to_global((const cx<W> *)it.tw),
                                   to_global((cx<W> *)it.sw));
 }
@@ -2860,8 +2854,8 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_S(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_S_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, to_global(it.tl_cl), to_global(it.tl_a0),
-                               to_global(it.tl_m), to_global(it.cl_ptr), to_global(it.cl_elem),
+    apply_S_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.tl_cl, it.tl_a0,
+                               it.tl_m, it.cl_ptr, it.cl_elem,
                                to_global((const cx<S> *)it.Sv), to_global((const cx<W> *)it.sw),
                                to_global((cx<double> *)it.y));
 }

@@ -234,33 +234,44 @@ void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double 
 // does nothing while the item's flag *skip (device int, may be null) is nonzero, i.e. once
 // that problem's Krylov solve has converged.  Each item's arithmetic is the single-problem
 // kernel's, work-group for work-group, so results do not depend on the grouping.
+//
+// NCG_GLOBAL marks the descriptors' pointers as pointers into global memory in the device
+// pass of the kernels' unit (NCG_KERNELS_TU): a pointer read from memory is otherwise
+// generic to the compiler, and loads through it are flat loads, which are slower and are
+// waited for together with LDS operations.  Every other pass sees plain pointers; the
+// layout is the same (64-bit pointers).
+#if defined(__HIP_DEVICE_COMPILE__) && defined(NCG_KERNELS_TU)
+#define NCG_GLOBAL __attribute__((address_space(1)))
+#else
+#define NCG_GLOBAL
+#endif
 struct GNear {      // near field of the finest level (NearChunks, col, values)
-    int n, nch, ncl;                    // unknowns, row chunks, finest clusters
-    const int *cptr;                    // cluster c owns chunks cptr[c] .. cptr[c + 1]
-    const long long *cb, *off;
-    const int *w, *row, *col;
-    const void *val;
-    const void *x;
-    void *y;
-    const int *skip;
+    int n, nch, ncl;                        // unknowns, row chunks, finest clusters
+    const NCG_GLOBAL int *cptr;             // cluster c owns chunks cptr[c] .. cptr[c + 1]
+    const NCG_GLOBAL long long *cb, *off;
+    const NCG_GLOBAL int *w, *row, *col;
+    const NCG_GLOBAL void *val;
+    const NCG_GLOBAL void *x;
+    NCG_GLOBAL void *y;
+    const NCG_GLOBAL int *skip;
 };
 struct GLevel {     // one FMM level: T (x -> t), D (t -> s) and S (s -> y += ) of a problem
     int n, ncl, P, ntl;
-    const int *cl_ptr, *cl_elem, *fan_ptr, *fan_idx, *tl_cl, *tl_a0, *tl_m;
-    const void *Tv, *Dv, *Sv;
-    const void *x;
-    void *tw, *sw, *y;
-    const int *skip;
+    const NCG_GLOBAL int *cl_ptr, *cl_elem, *fan_ptr, *fan_idx, *tl_cl, *tl_a0, *tl_m;
+    const NCG_GLOBAL void *Tv, *Dv, *Sv;
+    const NCG_GLOBAL void *x;
+    NCG_GLOBAL void *tw, *sw, *y;
+    const NCG_GLOBAL int *skip;
 };
 struct GBj {        // block-Jacobi blocks (FMM finest clusters or dense diagonal blocks)
     int n, nr, nblk;
-    const int *b0, *m, *elem;
-    const long long *boff;
-    const void *minv;
-    const void *x;                      // input; with a step counter kp: x + *kp * xk doubles
+    const NCG_GLOBAL int *b0, *m, *elem;
+    const NCG_GLOBAL long long *boff;
+    const NCG_GLOBAL void *minv;
+    const NCG_GLOBAL void *x;               // input; with a step counter kp: x + *kp * xk doubles
     long long xk;
-    void *y;
-    const int *skip;
+    NCG_GLOBAL void *y;
+    const NCG_GLOBAL int *skip;
 };
 // GMRES(m) state of one problem of a group, on the device: the Hessenberg matrices, Givens
 // rotations, right-hand sides g and solutions y of every load case, the Arnoldi work
@@ -269,29 +280,29 @@ struct GBj {        // block-Jacobi blocks (FMM finest clusters or dense diagona
 // is set once every load case has converged.
 struct GGm {
     int n, nrhs, nsl, slice, gsb;
-    const void *b;                      // right-hand sides [nrhs][n] (the Krylov system's)
-    void *x;                            // solution [nrhs][n]
-    void *V;                            // basis [m + 1][nrhs][n]
-    void *W;                            // new Arnoldi vector / residual [nrhs][n]
-    void *Z, *T;                        // preconditioned vector and apply output [nrhs][n]
-    void *part;                         // multidot slice partials
-    void *h;                            // [2][nrhs][m + 1] projections of the two passes, then [nrhs][3] norms
-    double *npart;                      // [nrhs][gsb] partial squared norms
-    void *H, *cs, *sn, *g, *yv;         // [nrhs][(m+1) m], [nrhs][m], [nrhs][m], [nrhs][m+1], [nrhs][m]
-    double *sc;                         // [nrhs][4]
-    int *ic;                            // [nrhs][4]
-    int *skip;
-    double tol;                         // relative residual target of every load case
+    const NCG_GLOBAL void *b;               // right-hand sides [nrhs][n] (the Krylov system's)
+    NCG_GLOBAL void *x;                     // solution [nrhs][n]
+    NCG_GLOBAL void *V;                     // basis [m + 1][nrhs][n]
+    NCG_GLOBAL void *W;                     // new Arnoldi vector / residual [nrhs][n]
+    NCG_GLOBAL void *Z, *T;                 // preconditioned vector and apply output [nrhs][n]
+    NCG_GLOBAL void *part;                  // multidot slice partials
+    NCG_GLOBAL void *h;                     // [2][nrhs][m + 1] projections of the two passes, then [nrhs][3] norms
+    NCG_GLOBAL double *npart;               // [nrhs][gsb] partial squared norms
+    NCG_GLOBAL void *H, *cs, *sn, *g, *yv;  // [nrhs][(m+1) m], [nrhs][m], [nrhs][m], [nrhs][m+1], [nrhs][m]
+    NCG_GLOBAL double *sc;                  // [nrhs][4]
+    NCG_GLOBAL int *ic;                     // [nrhs][4]
+    NCG_GLOBAL int *skip;
+    double tol;                             // relative residual target of every load case
 };
 // element-wise vector operations of a group: per item y = a (op kVecCopy), y = a - b
 // (kVecSub), y += a (kVecAdd), y = 0 (kVecZero); len = complex entries
 enum { kVecCopy = 0, kVecSub = 1, kVecAdd = 2, kVecZero = 3 };
 struct GVec {
     long long len;
-    const void *a, *b;                  // with a step counter kp: a + *kp * ak doubles
+    const NCG_GLOBAL void *a, *b;           // with a step counter kp: a + *kp * ak doubles
     long long ak;
-    void *y;
-    const int *skip;
+    NCG_GLOBAL void *y;
+    const NCG_GLOBAL int *skip;
 };
 // launchers (blocks = start[nitems], the prefix the host uploaded)
 // mode 0: a work-group per row chunk; 1: per finest cluster (x gathered once for its chunks);
@@ -350,9 +361,9 @@ void g_gm_stats(hipStream_t s, const GGm *items, int nitems, int *its, double *r
 // work-groups (start: their prefix), npart [nrhs][gsb] scratch
 struct GNorm {
     int n, nrhs, gsb;
-    const void *v;
-    double *npart;
-    const int *skip;
+    const NCG_GLOBAL void *v;
+    NCG_GLOBAL double *npart;
+    const NCG_GLOBAL int *skip;
 };
 void g_norms(hipStream_t s, const GNorm *items, const int *start, int nitems, int blocks, double *out);
 
