@@ -2012,7 +2012,9 @@ void plan_apply(Stage &st, const std::vector<ncg_problem *> &P, const std::vecto
                     ncgk::GLevel g = level_item(i, l, c);
                     if (c != cls) continue;
                     it.push_back(g);
-                    cnt.push_back(g.ncl * ((g.P + 63) / 64));
+                    // grouped T and D: two sphere points per lane for FP32 values (P is even)
+                    const int pts = c < 2 ? 2 : 1;
+                    cnt.push_back(g.ncl * ((g.P + 64 * pts - 1) / (64 * pts)));
                     const double sf = c == 2 ? 16.0 : 8.0, sw = c == 0 ? 8.0 : 16.0, n = g.n, Pp = g.P, ncl = g.ncl;
                     const double nfan = P[i]->fmm.lev[l].nfan;
                     bytes += kind == 1 ? n * Pp * sf + nr * n * 16.0 + nr * ncl * Pp * sw

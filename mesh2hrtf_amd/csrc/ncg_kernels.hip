@@ -70,6 +70,32 @@ __device__ __forceinline__ NCG_GLOBAL T *to_global(T *p)
     return (NCG_GLOBAL T *)p;
 }
 
+// PL adjacent complex values q[0 .. PL) of one array; two FP32 complex values move as one
+// 16-byte load or store (the caller's index is even and the array 16-byte aligned)
+typedef float ncg_f4 __attribute__((ext_vector_type(4)));
+template <typename T, int PL>
+__device__ __forceinline__ void load_pts(const NCG_GLOBAL cx<T> *q, cx<T> (&o)[PL])
+{
+    if constexpr (PL == 2 && sizeof(T) == 4) {
+        const ncg_f4 v = *(const NCG_GLOBAL ncg_f4 *)q;
+        o[0] = cx<T>{v.x, v.y};
+        o[1] = cx<T>{v.z, v.w};
+    } else {
+#pragma unroll
+        for (int h = 0; h < PL; ++h) o[h] = q[h];
+    }
+}
+template <typename T, int PL>
+__device__ __forceinline__ void store_pts(NCG_GLOBAL cx<T> *q, const cx<T> (&o)[PL])
+{
+    if constexpr (PL == 2 && sizeof(T) == 4) {
+        *(NCG_GLOBAL ncg_f4 *)q = ncg_f4{o[0].re, o[0].im, o[1].re, o[1].im};
+    } else {
+#pragma unroll
+        for (int h = 0; h < PL; ++h) q[h] = o[h];
+    }
+}
+
 // ======================================================================================
 // geometry: centre, unit normal, area, Jacobian per element (NC_ReadMesh,
 // src/NC_Input.cpp:522-605; triangle centre s = t = 1/3 and area 0.5*J, quad centre
@@ -2213,48 +2239,63 @@ __global__ void __launch_bounds__(256) k_near_spmv(int n, int seg, const long lo
 // t[c, p] = sum over the cluster's elements of T * x.  One work-group per (cluster, tile
 // of 64 sphere points); its waves take every kSumWaves-th element of the cluster, four
 // per step, and their partial sums meet in LDS.  W is the storage type of t.
-template <typename S, typename W, int NR, int NW = kSumWaves>
+template <typename S, typename W, int NR, int NW = kSumWaves, int PL = 1>
 __device__ __forceinline__ void apply_T_item(int blk, int n, int ncl, int P, const NCG_GLOBAL int *cl_ptr,
                                              const NCG_GLOBAL int *cl_elem, const NCG_GLOBAL cx<S> *Tv,
                                              const NCG_GLOBAL cx<double> *x, NCG_GLOBAL cx<W> *tw)
 {
-    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
-    const int tiles = (P + 63) / 64;
+    // PL points per lane (p, p + 1 with PL = 2: P is even), each summed as with one
+    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR * PL];
+    const int tiles = (P + 64 * PL - 1) / (64 * PL);
     const int c = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int p = tile * 64 + lane;
+    const int p = (tile * 64 + lane) * PL;
     const bool on = p < P;
-    double re[NR], im[NR];
+    double re[NR * PL], im[NR * PL];   // [r * PL + h]
 #pragma unroll
-    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    for (int r = 0; r < NR * PL; ++r) re[r] = im[r] = 0.0;
     const int k1 = cl_ptr[c + 1];
     int k = cl_ptr[c] + wv;
     if (on) {
         for (; k + 3 * NW < k1; k += 4 * NW) {
-            cx<S> a[4];
+            cx<S> a[4][PL];
             int e[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                a[j] = Tv[(size_t)(k + NW * j) * P + p];
+                load_pts<S, PL>(Tv + (size_t)(k + NW * j) * P + p, a[j]);
                 e[j] = cl_elem[k + NW * j];
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), x[(size_t)r * n + e[j]]);
+                for (int r = 0; r < NR; ++r) {
+                    const cx<double> xv = x[(size_t)r * n + e[j]];
+#pragma unroll
+                    for (int h = 0; h < PL; ++h)
+                        cmac(re[r * PL + h], im[r * PL + h], double(a[j][h].re), double(a[j][h].im), xv);
+                }
         }
         for (; k < k1; k += NW) {
-            const cx<S> a = Tv[(size_t)k * P + p];
+            cx<S> a[PL];
+            load_pts<S, PL>(Tv + (size_t)k * P + p, a);
             const int e = cl_elem[k];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) cmac(re[r], im[r], double(a.re), double(a.im), x[(size_t)r * n + e]);
+            for (int r = 0; r < NR; ++r) {
+                const cx<double> xv = x[(size_t)r * n + e];
+#pragma unroll
+                for (int h = 0; h < PL; ++h) cmac(re[r * PL + h], im[r * PL + h], double(a[h].re), double(a[h].im), xv);
+            }
         }
     }
-    waves_sum<NR, NW>(re, im, red, wv, lane);
+    waves_sum<NR * PL, NW>(re, im, red, wv, lane);
     if (wv == 0 && on)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) tw[(size_t)r * ncl * P + (size_t)c * P + p] = cx<W>{W(re[r]), W(im[r])};
+        for (int r = 0; r < NR; ++r) {
+            cx<W> o[PL];
+#pragma unroll
+            for (int h = 0; h < PL; ++h) o[h] = cx<W>{W(re[r * PL + h]), W(im[r * PL + h])};
+            store_pts<W, PL>(tw + (size_t)r * ncl * P + (size_t)c * P + p, o);
+        }
 }
 
 template <typename S, typename W, int NR>
@@ -2269,54 +2310,69 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_T(int n, int ncl, 
 // s[i, p] = sum over the interaction list of D * t (D is diagonal in p); one work-group
 // per (cluster, tile of 64 points), the waves splitting the interaction list four
 // entries per step.  WI / WO are the storage types of t and s.
-template <typename S, typename WI, typename WO, int NR, int NW = kSumWaves>
+template <typename S, typename WI, typename WO, int NR, int NW = kSumWaves, int PL = 1>
 __device__ __forceinline__ void apply_D_item(int blk, int ncl, int P, const NCG_GLOBAL int *fan_ptr,
                                              const NCG_GLOBAL int *fan_idx, const NCG_GLOBAL cx<S> *Dv,
                                              const NCG_GLOBAL cx<WI> *tw, NCG_GLOBAL cx<WO> *sw)
 {
-    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR];
-    const int tiles = (P + 63) / 64;
+    // PL points per lane (p, p + 1 with PL = 2: P is even), each summed as with one
+    __shared__ double red[(NW > 1 ? NW / 2 : 1) * 64 * 2 * NR * PL];
+    const int tiles = (P + 64 * PL - 1) / (64 * PL);
     const int i = blk / tiles, tile = blk % tiles;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int p = tile * 64 + lane;
+    const int p = (tile * 64 + lane) * PL;
     const bool on = p < P;
     const size_t ld = (size_t)ncl * P;
-    double re[NR], im[NR];
+    double re[NR * PL], im[NR * PL];   // [r * PL + h]
 #pragma unroll
-    for (int r = 0; r < NR; ++r) re[r] = im[r] = 0.0;
+    for (int r = 0; r < NR * PL; ++r) re[r] = im[r] = 0.0;
     const int q1 = fan_ptr[i + 1];
     int q = fan_ptr[i] + wv;
     if (on) {
         for (; q + 3 * NW < q1; q += 4 * NW) {
-            cx<S> a[4];
+            cx<S> a[4][PL];
             int jj[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                a[j] = Dv[(size_t)(q + NW * j) * P + p];
+                load_pts<S, PL>(Dv + (size_t)(q + NW * j) * P + p, a[j]);
                 jj[j] = fan_idx[q + NW * j];
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    const cx<WI> t = tw[r * ld + (size_t)jj[j] * P + p];
-                    cmac(re[r], im[r], double(a[j].re), double(a[j].im), mk(double(t.re), double(t.im)));
+                    cx<WI> t[PL];
+                    load_pts<WI, PL>(tw + r * ld + (size_t)jj[j] * P + p, t);
+#pragma unroll
+                    for (int h = 0; h < PL; ++h)
+                        cmac(re[r * PL + h], im[r * PL + h], double(a[j][h].re), double(a[j][h].im),
+                             mk(double(t[h].re), double(t[h].im)));
                 }
         }
         for (; q < q1; q += NW) {
-            const cx<S> a = Dv[(size_t)q * P + p];
+            cx<S> a[PL];
+            load_pts<S, PL>(Dv + (size_t)q * P + p, a);
             const int j = fan_idx[q];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const cx<WI> t = tw[r * ld + (size_t)j * P + p];
-                cmac(re[r], im[r], double(a.re), double(a.im), mk(double(t.re), double(t.im)));
+                cx<WI> t[PL];
+                load_pts<WI, PL>(tw + r * ld + (size_t)j * P + p, t);
+#pragma unroll
+                for (int h = 0; h < PL; ++h)
+                    cmac(re[r * PL + h], im[r * PL + h], double(a[h].re), double(a[h].im),
+                         mk(double(t[h].re), double(t[h].im)));
             }
         }
     }
-    waves_sum<NR, NW>(re, im, red, wv, lane);
+    waves_sum<NR * PL, NW>(re, im, red, wv, lane);
     if (wv == 0 && on)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) sw[r * ld + (size_t)i * P + p] = cx<WO>{WO(re[r]), WO(im[r])};
+        for (int r = 0; r < NR; ++r) {
+            cx<WO> o[PL];
+#pragma unroll
+            for (int h = 0; h < PL; ++h) o[h] = cx<WO>{WO(re[r * PL + h]), WO(im[r * PL + h])};
+            store_pts<WO, PL>(sw + r * ld + (size_t)i * P + p, o);
+        }
 }
 
 template <typename S, typename WI, typename WO, int NR>
@@ -2826,13 +2882,18 @@ __global__ void __launch_bounds__(256) kq_near_spmv(const ncgk::GNear *items, co
     }
 }
 
+// grouped T and D: two sphere points per lane where the values are FP32 (one 16-byte load
+// for both), one for FP64 values; work-groups per item = ncl * ceil(P / (64 * points))
+template <typename S>
+constexpr int kGroupedPoints = sizeof(S) == 4 ? 2 : 1;
+
 template <typename S, typename W, int NR, int NW>
 __global__ void __launch_bounds__(64 * NW) kg_apply_T(const ncgk::GLevel *items, const int *start, int nitems)
 {
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_T_item<S, W, NR, NW>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem,
+    apply_T_item<S, W, NR, NW, kGroupedPoints<S>>(blockIdx.x - start[d], it.n, it.ncl, it.P, it.cl_ptr, it.cl_elem,
                                to_global((const cx<S> *)it.Tv), to_global((const cx<double> *)it.x),
                                to_global((cx<W> *)it.tw));
 }
@@ -2843,7 +2904,7 @@ __global__ void __launch_bounds__(64 * NW) kg_apply_D(const ncgk::GLevel *items,
     const int d = g_item(start, nitems, blockIdx.x);
     const ncgk::GLevel it = items[d];
     if (it.skip && *it.skip) return;
-    apply_D_item<S, W, W, NR, NW>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx,
+    apply_D_item<S, W, W, NR, NW, kGroupedPoints<S>>(blockIdx.x - start[d], it.ncl, it.P, it.fan_ptr, it.fan_idx,
                                   to_global((const cx<S> *)it.Dv), to_global((const cx<W> *)it.tw),
                                   to_global((cx<W> *)it.sw));
 }

@@ -313,7 +313,9 @@ void g_near_spmv(hipStream_t s, int mixed, int mode, int nr, int seg, const GNea
 // T and D of every item (each writes its own t or s); S adds into y, so a launch may hold at
 // most one level of a problem (the host launches one level index at a time).  sig64: t and
 // s in FP64 whatever the values' precision (the refinement residual).  nw: waves per
-// work-group (1, 2, 4, 8) splitting each item's sum, met in an LDS tree.
+// work-group (1, 2, 4, 8) splitting each item's sum, met in an LDS tree.  T and D take
+// ncl * ceil(P / (64 * pts)) work-groups per item, pts = 2 sphere points per lane for FP32
+// values (P is even), 1 for FP64; S one per row tile (GLevel::ntl).
 void g_apply_T(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
                int blocks);
 void g_apply_D(hipStream_t s, int f32, int sig64, int nw, int nr, const GLevel *items, const int *start, int nitems,
