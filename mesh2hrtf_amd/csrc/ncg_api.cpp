@@ -1227,6 +1227,14 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
         (rc = F.fgrad.ensure(48 * (size_t)nr * m)))
         return rc;
     const ncgk::BMParams b = p->bm();
+    // Arithmetic of the evaluation pairs' translation series (and of the solution's level-0
+    // signature feeding it): FP32 in the mixed form where the rule that keeps FMM levels in
+    // FP32 (fmm_operator: k d >= 1 and peak * k d <= 2e7) holds at the nearest far
+    // evaluation pair.  Evaluation points usually lie much farther from the level-0
+    // clusters than the clusters from each other, so the grid can be FP32 where level 0 is
+    // not.
+    const double kde = p->k * F.ev_dmin_far;
+    const int ev32 = p->precision == NCG_MIXED && F.npair > 0 && kde >= 1.0 && translation_peak(D0.L, kde) * kde <= 2e7;
     if (p->all_velocity && p->nsrc == 0) {
         // velocity elements with no prescribed velocity: the signature of the solution is
         // the level-0 aggregation T applied to the potential (same sum as below)
@@ -1234,11 +1242,11 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
                           D0.tv.d());
     } else {
         ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
-                           D0.tv.d(), D0.f32);
+                           D0.tv.d(), ev32);
     }
     ncgk::fmm_eval_hankel(s, D0.L, p->k, p->h, F.ecen, D0.ccen, F.pair_ev, F.pair_cl, F.npair, F.tab.d());
     ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
-                     nr, F.z.p, D0.f32);
+                     nr, F.z.p, ev32);
     ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
     {
         const int f32 = p->precision == NCG_MIXED;
