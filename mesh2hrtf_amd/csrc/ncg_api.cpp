@@ -1235,18 +1235,22 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
     // not.
     const double kde = p->k * F.ev_dmin_far;
     const int ev32 = p->precision == NCG_MIXED && F.npair > 0 && kde >= 1.0 && translation_peak(D0.L, kde) * kde <= 2e7;
-    if (p->all_velocity && p->nsrc == 0) {
+    // the signature itself stored in FP32 there too where it is formed here (the series'
+    // reads of it bound the evaluation); the velocity branch's T apply writes FP64
+    const bool tv_apply = p->all_velocity && p->nsrc == 0;
+    const int tv32 = ev32 && !tv_apply;
+    if (tv_apply) {
         // velocity elements with no prescribed velocity: the signature of the solution is
         // the level-0 aggregation T applied to the potential (same sum as below)
         ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, 1, pot,
                           D0.tv.d());
     } else {
         ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,
-                           D0.tv.d(), ev32);
+                           D0.tv.d(), ev32, tv32);
     }
     ncgk::fmm_eval_hankel(s, D0.L, p->k, p->h, F.ecen, D0.ccen, F.pair_ev, F.pair_cl, F.npair, F.tab.d());
     ncgk::fmm_eval_z(s, nev, D0.L, D0.P, p->h, F.ev_ptr, F.pair_cl, F.tab.d(), D0.u.d(), D0.w.d(), D0.tv.d(), D0.ncl,
-                     nr, F.z.p, ev32);
+                     nr, F.z.p, ev32, tv32);
     ncgk::fmm_eval_far(s, m, D0.P, nev, b.hk, p->eval, F.pt_ev, F.ecen, D0.u.d(), nr, F.z.p, F.fphi.p, F.fgrad.p);
     {
         const int f32 = p->precision == NCG_MIXED;

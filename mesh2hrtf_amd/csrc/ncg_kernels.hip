@@ -2463,11 +2463,11 @@ __global__ void __launch_bounds__(64 * kSumWaves) k_fmm_apply_S(int n, int ncl, 
 // src/NC_PostProcessing.cpp:302-364): thread per (cluster, sphere point)
 // T is the arithmetic of the phase and weight (FP32 in the mixed form, after the
 // geometry differences are formed in FP64); sums are FP64.
-template <typename T>
+template <typename T, typename TV>
 __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, int P, int ncl, const double *ccen,
                                                       const int *cl_ptr, const int *cl_elem, const double *u,
                                                       const cx<double> *pot, const cx<double> *vel, int nrhs,
-                                                      cx<double> *tv)
+                                                      cx<TV> *tv)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)ncl * P) return;
@@ -2501,7 +2501,7 @@ __global__ void __launch_bounds__(256) k_fmm_Tvec_sol(Elements E, BMConst bm, in
         }
         acc = acc * (bm.k / kPi4);
         if (bm.tau != 1.0) acc = acc * bm.tau;
-        tv[(size_t)r * ncl * P + t] = acc;
+        tv[(size_t)r * ncl * P + t] = cx<TV>{TV(acc.re), TV(acc.im)};
     }
 }
 
@@ -2538,11 +2538,13 @@ __global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, doubl
 // Second pass: one block per evaluation cluster, threads over sphere points, summing
 // its far pairs in registers (pairs of one evaluation cluster are contiguous and are
 // staged through LDS in batches).  T is the arithmetic of the series (FP32 in the mixed
-// form at regular levels); the sums over pairs are FP64.
-template <int NR, typename T>
+// form where the evaluation pairs allow it), TV the storage of the signature tv (FP32
+// there too: the kernel is bound by the tv reads, NR per pair and point); the sums over
+// pairs are FP64.
+template <int NR, typename T, typename TV>
 __global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, double h, const int *ev_ptr,
                                                     const int *pair_cl, const double *tab, const double *u,
-                                                    const double *wsph, const cx<double> *tv, int ncl,
+                                                    const double *wsph, const cx<TV> *tv, int ncl,
                                                     cx<double> *z)
 {
     constexpr int QB = 16;
@@ -2593,7 +2595,10 @@ __global__ void __launch_bounds__(256) k_fmm_eval_z(int nev, int L, int P, doubl
                 const cx<double> md = mk(double(mu.re * wp), double(mu.im * wp));
                 const int c = scl[qq];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) acc[r] += md * tv[r * ldt + (size_t)c * P + p];
+                for (int r = 0; r < NR; ++r) {
+                    const cx<TV> t = tv[r * ldt + (size_t)c * P + p];
+                    acc[r] += md * mk(double(t.re), double(t.im));
+                }
             }
         }
         if (on)
@@ -3958,16 +3963,20 @@ void fmm_apply_S(hipStream_t s, int n, int ncl, int P, int nrhs, const RowChunks
 }
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
-                  void *tv, int fp32)
+                  void *tv, int fp32, int tv32)
 {
     Elements E{D.n, D.X, D.nv, D.cen, D.nrm, D.area, D.jac, D.kind};
     const unsigned g = blocks_for((size_t)ncl * P, 256);
-    if (fp32)
-        ncg_launch(k_fmm_Tvec_sol<float>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
-                                                (const cx<double> *)vel, nrhs, (cx<double> *)tv);
+    const cx<double> *pp = (const cx<double> *)pot, *vv = (const cx<double> *)vel;
+    if (fp32 && tv32)
+        ncg_launch(k_fmm_Tvec_sol<float, float>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem,
+                   u, pp, vv, nrhs, (cx<float> *)tv);
+    else if (fp32)
+        ncg_launch(k_fmm_Tvec_sol<float, double>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr,
+                   cl_elem, u, pp, vv, nrhs, (cx<double> *)tv);
     else
-        ncg_launch(k_fmm_Tvec_sol<double>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr, cl_elem, u, (const cx<double> *)pot,
-                                                 (const cx<double> *)vel, nrhs, (cx<double> *)tv);
+        ncg_launch(k_fmm_Tvec_sol<double, double>, dim3(g), dim3(256), 0, s, E, make_bm(b), P, ncl, ccen, cl_ptr,
+                   cl_elem, u, pp, vv, nrhs, (cx<double> *)tv);
 }
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
                      const int *pair_ev, const int *pair_cl, int npair, double *tab)
@@ -3975,30 +3984,29 @@ void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ece
     if (npair == 0) return;
     ncg_launch(k_fmm_pair_hankel, dim3(blocks_for(npair, 128)), dim3(128), 0, s, npair, L, k, h, ecen, ccen, pair_ev, pair_cl, tab);
 }
-void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
-                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
-                int fp32)
+template <typename T, typename TV>
+static void eval_z_t(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
+                     const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z)
 {
-    if (nev == 0) return;
     rhs_chunks(nrhs, [&](int nr, int r0) {
-        const cx<double> *t = (const cx<double> *)tv + (size_t)r0 * ncl * P;
+        const cx<TV> *t = (const cx<TV> *)tv + (size_t)r0 * ncl * P;
         cx<double> *zz = (cx<double> *)z + (size_t)r0 * nev * P;
-        if (fp32) {
-            switch (nr) {
-            case 1: ncg_launch(k_fmm_eval_z<1, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 2: ncg_launch(k_fmm_eval_z<2, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 3: ncg_launch(k_fmm_eval_z<3, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            default: ncg_launch(k_fmm_eval_z<4, float>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            }
-        } else {
-            switch (nr) {
-            case 1: ncg_launch(k_fmm_eval_z<1, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 2: ncg_launch(k_fmm_eval_z<2, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            case 3: ncg_launch(k_fmm_eval_z<3, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            default: ncg_launch(k_fmm_eval_z<4, double>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
-            }
+        switch (nr) {
+        case 1: ncg_launch(k_fmm_eval_z<1, T, TV>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        case 2: ncg_launch(k_fmm_eval_z<2, T, TV>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        case 3: ncg_launch(k_fmm_eval_z<3, T, TV>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
+        default: ncg_launch(k_fmm_eval_z<4, T, TV>, dim3(nev), dim3(256), 0, s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, t, ncl, zz); break;
         }
     });
+}
+void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
+                const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
+                int fp32, int tv32)
+{
+    if (nev == 0) return;
+    if (fp32 && tv32) eval_z_t<float, float>(s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, tv, ncl, nrhs, z);
+    else if (fp32) eval_z_t<float, double>(s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, tv, ncl, nrhs, z);
+    else eval_z_t<double, double>(s, nev, L, P, h, ev_ptr, pair_cl, tab, u, w, tv, ncl, nrhs, z);
 }
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad)

@@ -215,14 +215,15 @@ void fmm_near(hipStream_t s, const DevElements &D, const BMParams &b, const doub
               int mixed, unsigned long long *evals = nullptr, int rows_mode = 0);
 void fmm_Tvec_sol(hipStream_t s, const DevElements &D, const BMParams &b, int P, int ncl, const double *ccen,
                   const int *cl_ptr, const int *cl_elem, const double *u, const void *pot, const void *vel, int nrhs,
-                  void *tv, int fp32);
+                  void *tv, int fp32, int tv32 = 0);
 // evaluation clusters: per far pair the direction and h_n (tab [npair][4 + 2L] doubles),
-// then z[r][ev][p] = sum over the pairs of mu * w * tv
+// then z[r][ev][p] = sum over the pairs of mu * w * tv.  fp32: the signature's phases and
+// the series in FP32; tv32 (with fp32): tv stored as FP32 complex, else FP64
 void fmm_eval_hankel(hipStream_t s, int L, double k, double h, const double *ecen, const double *ccen,
                      const int *pair_ev, const int *pair_cl, int npair, double *tab);
 void fmm_eval_z(hipStream_t s, int nev, int L, int P, double h, const int *ev_ptr, const int *pair_cl,
                 const double *tab, const double *u, const double *w, const void *tv, int ncl, int nrhs, void *z,
-                int fp32);
+                int fp32, int tv32 = 0);
 void fmm_eval_far(hipStream_t s, int m, int P, int nev, double hk, const double *pts, const int *pt_ev,
                   const double *ecen, const double *u, int nrhs, const void *z, void *fphi, void *fgrad);
 

@@ -215,6 +215,40 @@ def test_graded_large_mesh_mixed_vs_fp64(ncg, f):
     prob.close()
 
 
+@pytest.mark.parametrize("f", [300.0, 1000.0, 3000.0])
+def test_grid_fp32_pairs_where_level0_is_fp64(ncg, f):
+    """Low frequencies on a coarse head-like mesh (5120 elements) with the 1730-point grid
+    1.2 m out: the level-0 translations cancel (k d_min small), so level 0 stays FP64 in the
+    mixed form, while the grid's own far pairs (about 1 m) pass the FP32 rule and its series
+    runs in FP32.  Grid and surface of the mixed form within 1e-4 of the FP64 plain method
+    (measured about 2e-6 on the 20480-element head)."""
+    from mesh2hrtf_amd import project as P
+    mesh = P.head_like_mesh(freq=16, seed=2)
+    grid = P.fibonacci_grid(1730, 1.2)
+    src = P.ear_sources(mesh)
+    n = len(mesh.elems)
+    omega = 2 * np.pi * f
+    amp = 1.0 / (1j * 1.1839 * omega)
+    ps = [[s[0], s[1], s[2], amp.real, amp.imag, r] for r, s in enumerate(src)]
+    prob = ncg.Problem(np.full(n, 3), mesh.nodes[mesh.elems], np.zeros(n, np.int32), grid.nodes,
+                       node_index=mesh.elems, method_bem=4)
+    print("coarse mesh with grid, f", f, flush=True)
+    prob.assemble(omega / 343.0, nrhs=2, point_sources=ps, precision=ncg.FP64)
+    assert prob.solve(solver=ncg.CGS, precond=ncg.ILU, tolerance=1e-9)["converged"]
+    x64 = prob.solution()
+    p64, _ = prob.field(gradient=False)
+    prob.assemble(omega / 343.0, nrhs=2, point_sources=ps, precision=ncg.MIXED)
+    st = prob.solve(solver=ncg.GMRES, precond=ncg.BLOCK, refinement_steps=8, refinement_tolerance=1e-6)
+    assert st["converged"], st
+    x32 = prob.solution()
+    p32, _ = prob.field(gradient=False)
+    for r in range(2):
+        print("load case", r, "surface", R.rel_l2(x32[r], x64[r]), "grid", R.rel_l2(p32[r], p64[r]), flush=True)
+        assert R.rel_l2(x32[r], x64[r]) < 1e-4
+        assert R.rel_l2(p32[r], p64[r]) < 1e-4
+    prob.close()
+
+
 @pytest.mark.parametrize("f", [12400.0, 16000.0, 20000.0])
 def test_coarse_mesh_high_frequency_mixed_converges(ncg, f):
     """A coarse head-like mesh (5120 elements, about 4 per wavelength at 12 kHz) high in
