@@ -41,7 +41,9 @@ enum { NCG_BC_VELOCITY = 0, NCG_BC_PRESSURE = 1, NCG_BC_ADMITTANCE = 2 };
 enum {
     NCG_FP64 = 0,          /* plain method: everything FP64                                   */
     NCG_MIXED = 1          /* FP32-complex bulk quadrature, operator and Krylov matvec;
-                              FP64 singular/near integrals, Krylov dots and refinement residual */
+                              FP64 singular/near integrals, Krylov dots and refinement residual;
+                              FMM levels and the evaluation grid's translations stay FP64
+                              where their series would cancel (k d < 1 or a large peak term) */
 };
 
 enum { NCG_SOLVER_CGS = 0, NCG_SOLVER_GMRES = 1, NCG_SOLVER_LU = 4 };
