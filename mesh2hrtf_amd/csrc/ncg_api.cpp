@@ -1235,14 +1235,15 @@ int field_fmm(ncg_problem *p, double *pot, double *vel, double *dphi, double *dg
     // not.
     const double kde = p->k * F.ev_dmin_far;
     const int ev32 = p->precision == NCG_MIXED && F.npair > 0 && kde >= 1.0 && translation_peak(D0.L, kde) * kde <= 2e7;
-    // the signature itself stored in FP32 there too where it is formed here (the series'
-    // reads of it bound the evaluation); the velocity branch's T apply writes FP64
+    // the signature itself stored in FP32 there too (the series' reads of it bound the
+    // evaluation); where it is the stored T applied to the potential, only when level 0's
+    // T is FP32 as well
     const bool tv_apply = p->all_velocity && p->nsrc == 0;
-    const int tv32 = ev32 && !tv_apply;
+    const int tv32 = ev32 && (!tv_apply || D0.f32);
     if (tv_apply) {
         // velocity elements with no prescribed velocity: the signature of the solution is
         // the level-0 aggregation T applied to the potential (same sum as below)
-        ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, 1, pot,
+        ncgk::fmm_apply_T(s, p->n, D0.ncl, D0.P, nr, D0.cl_ptr, D0.cl_elem, D0.T.p, D0.f32, tv32 ? 0 : 1, pot,
                           D0.tv.d());
     } else {
         ncgk::fmm_Tvec_sol(s, p->elements(), b, D0.P, D0.ncl, D0.ccen, D0.cl_ptr, D0.cl_elem, D0.u.d(), pot, vel, nr,

@@ -2518,17 +2518,70 @@ __global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, doubl
     const int a = pair_ev[q], c = pair_cl[q];
     double d[3] = {ecen[3 * a] - ccen[3 * c], ecen[3 * a + 1] - ccen[3 * c + 1], ecen[3 * a + 2] - ccen[3 * c + 2]};
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double hre[kMaxTerms], him[kMaxTerms];
-    spherical_hankel(L, r * k, h, hre, him);
     double *o = tab + (size_t)q * (4 + 2 * L);
     o[0] = d[0] / r;
     o[1] = d[1] / r;
     o[2] = d[2] / r;
     o[3] = 0.0;
-    // series coefficients a_n = (i h)^n (2n + 1) h_n(k r)
+    // series coefficients a_n = (i h)^n (2n + 1) h_n(k r), h_n = j_n + i h y_n as
+    // spherical_hankel forms them (same operations), with the recurrences in registers
+    // instead of per-thread arrays: j_n goes to o[4 + 2n] first where it comes from the
+    // downward recurrence, then one upward pass forms y_n and a_n
+    const int N = L;
+    const double x = r * k;
+    const double sx = sin(x), cxs = cos(x);
+    const bool up = x > (double)N + 10.0;
+    if (!up) {
+        const double nu = (double)(N - 1);
+        double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
+        for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
+            const double b = (2.0 * (nu + j + 1.0) + 1.0) / x;
+            D = b - D;
+            if (D == 0.0) D = 1.0e-30;
+            D = 1.0 / D;
+            C = b - 1.0 / C;
+            if (C == 0.0) C = 1.0e-30;
+            di = di * C * D;
+            err = fabs(C * D - 1.0);
+        }
+        const double gnu = nu / x - 1.0 / di;
+        // downward recurrence twice: first for g_0 (the normalisation), then to store j_n
+        double g = 1.0, dg = gnu;
+        for (int i = N - 2; i >= 0; --i) {
+            const double gi = ((double)i + 2.0) / x * g + dg;
+            dg = (double)i / x * gi - g;
+            g = gi;
+        }
+        const double dp = fabs(g) > 1.0e-5 ? sx / x / g : (cxs - sx / x) / x / dg;
+        g = 1.0;
+        dg = gnu;
+        o[4 + 2 * (N - 1)] = dp * g;
+        for (int i = N - 2; i >= 0; --i) {
+            const double gi = ((double)i + 2.0) / x * g + dg;
+            dg = (double)i / x * gi - g;
+            g = gi;
+            o[4 + 2 * i] = dp * g;
+        }
+    }
+    double jm2 = 0.0, jm1 = 0.0, ym2 = 0.0, ym1 = 0.0;
     cx<double> zi = mk(1.0, 0.0);
-    for (int n = 0; n < L; ++n) {
-        const cx<double> an = zi * mk(hre[n], him[n]) * double(2 * n + 1);
+    for (int n = 0; n < N; ++n) {
+        double yn, jn;
+        if (n == 0) yn = -cxs / x;
+        else if (n == 1) yn = -(cxs / x + sx) / x;
+        else yn = ym1 * (double)(2 * n - 1) / x - ym2;
+        if (up) {
+            if (n == 0) jn = sx / x;
+            else if (n == 1) jn = (sx / x - cxs) / x;
+            else jn = jm1 * (double)(2 * n - 1) / x - jm2;
+            jm2 = jm1;
+            jm1 = jn;
+        } else {
+            jn = o[4 + 2 * n];
+        }
+        ym2 = ym1;
+        ym1 = yn;
+        const cx<double> an = zi * mk(jn, h * yn) * double(2 * n + 1);
         o[4 + 2 * n] = an.re;
         o[5 + 2 * n] = an.im;
         zi = mk(-zi.im * h, zi.re * h);
