@@ -1379,48 +1379,72 @@ __global__ void __launch_bounds__(256) k_field(Elements E, BMConst bm, const dou
 // ======================================================================================
 constexpr int kMaxTerms = 80;
 
-// h_n = j_n + i h y_n, n < N (NC_SphericalHankel, src/NC_3dFunctions.cpp:821-886): y_n by
-// upward recurrence, j_n by downward recurrence from the continued fraction of
-// j_N/j_{N-1} (modified Lentz), normalised with j_0 = sin x / x (or j_0' when j_0 ~ 0).
-// When x > N + 10 every order is well below the argument and j_n also comes from the
-// upward recurrence (the continued fraction would take O(x) steps there).
-__device__ void spherical_hankel(int N, double x, double h, double *hre, double *him)
+// Series coefficients a_n = (i h)^n (2n + 1) h_n(x), n < N, into o[2n], o[2n + 1], with
+// h_n = j_n + i h y_n (NC_SphericalHankel, src/NC_3dFunctions.cpp:821-886): y_n by upward
+// recurrence, j_n by downward recurrence from the continued fraction of j_N/j_{N-1}
+// (modified Lentz), normalised with j_0 = sin x / x (or j_0' when j_0 ~ 0).  When
+// x > N + 10 every order is well below the argument and j_n also comes from the upward
+// recurrence (the continued fraction would take O(x) steps there).  The recurrences run in
+// registers (per-thread arrays took 2.5 KB of scratch per thread): the downward one twice,
+// first for the normalisation, then storing j_n in o[2n]; one upward pass forms y_n and a_n.
+__device__ void hankel_series(int N, double x, double h, double *o)
 {
-    double gg[kMaxTerms], dg[kMaxTerms];
     const double sx = sin(x), cxs = cos(x);
-    him[0] = -cxs / x;
-    if (N > 1) him[1] = -(cxs / x + sx) / x;
-    for (int i = 2; i < N; ++i) him[i] = him[i - 1] * (double)(2 * i - 1) / x - him[i - 2];
-    if (x > (double)N + 10.0) {
-        hre[0] = sx / x;
-        if (N > 1) hre[1] = (sx / x - cxs) / x;
-        for (int i = 2; i < N; ++i) hre[i] = hre[i - 1] * (double)(2 * i - 1) / x - hre[i - 2];
-        for (int i = 0; i < N; ++i) him[i] = h * him[i];
-        return;
+    const bool up = x > (double)N + 10.0;
+    if (!up) {
+        const double nu = (double)(N - 1);
+        double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
+        for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
+            const double b = (2.0 * (nu + j + 1.0) + 1.0) / x;
+            D = b - D;
+            if (D == 0.0) D = 1.0e-30;
+            D = 1.0 / D;
+            C = b - 1.0 / C;
+            if (C == 0.0) C = 1.0e-30;
+            di = di * C * D;
+            err = fabs(C * D - 1.0);
+        }
+        const double gnu = nu / x - 1.0 / di;
+        // downward recurrence twice: first for g_0 (the normalisation), then to store j_n
+        double g = 1.0, dg = gnu;
+        for (int i = N - 2; i >= 0; --i) {
+            const double gi = ((double)i + 2.0) / x * g + dg;
+            dg = (double)i / x * gi - g;
+            g = gi;
+        }
+        const double dp = fabs(g) > 1.0e-5 ? sx / x / g : (cxs - sx / x) / x / dg;
+        g = 1.0;
+        dg = gnu;
+        o[2 * (N - 1)] = dp * g;
+        for (int i = N - 2; i >= 0; --i) {
+            const double gi = ((double)i + 2.0) / x * g + dg;
+            dg = (double)i / x * gi - g;
+            g = gi;
+            o[2 * i] = dp * g;
+        }
     }
-    const double nu = (double)(N - 1);
-    double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
-    for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
-        const double b = (2.0 * (nu + j + 1.0) + 1.0) / x;
-        D = b - D;
-        if (D == 0.0) D = 1.0e-30;
-        D = 1.0 / D;
-        C = b - 1.0 / C;
-        if (C == 0.0) C = 1.0e-30;
-        di = di * C * D;
-        err = fabs(C * D - 1.0);
-    }
-    const double gnu = nu / x - 1.0 / di;
-    gg[N - 1] = 1.0;
-    dg[N - 1] = gnu;
-    for (int i = N - 2; i >= 0; --i) {
-        gg[i] = ((double)i + 2.0) / x * gg[i + 1] + dg[i + 1];
-        dg[i] = (double)i / x * gg[i] - gg[i + 1];
-    }
-    const double dp = fabs(gg[0]) > 1.0e-5 ? sx / x / gg[0] : (cxs - sx / x) / x / dg[0];
-    for (int i = 0; i < N; ++i) {
-        hre[i] = dp * gg[i];
-        him[i] = h * him[i];
+    double jm2 = 0.0, jm1 = 0.0, ym2 = 0.0, ym1 = 0.0;
+    cx<double> zi = mk(1.0, 0.0);
+    for (int n = 0; n < N; ++n) {
+        double yn, jn;
+        if (n == 0) yn = -cxs / x;
+        else if (n == 1) yn = -(cxs / x + sx) / x;
+        else yn = ym1 * (double)(2 * n - 1) / x - ym2;
+        if (up) {
+            if (n == 0) jn = sx / x;
+            else if (n == 1) jn = (sx / x - cxs) / x;
+            else jn = jm1 * (double)(2 * n - 1) / x - jm2;
+            jm2 = jm1;
+            jm1 = jn;
+        } else {
+            jn = o[2 * n];
+        }
+        ym2 = ym1;
+        ym1 = yn;
+        const cx<double> an = zi * mk(jn, h * yn) * double(2 * n + 1);
+        o[2 * n] = an.re;
+        o[2 * n + 1] = an.im;
+        zi = mk(-zi.im * h, zi.re * h);
     }
 }
 
@@ -1457,19 +1481,11 @@ __global__ void __launch_bounds__(128) k_fmm_D_coef(int nfan, int L, double k, d
     const int i = fan_row[q], j = fan_idx[q];
     const double d[3] = {ccen[3 * i] - ccen[3 * j], ccen[3 * i + 1] - ccen[3 * j + 1], ccen[3 * i + 2] - ccen[3 * j + 2]};
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double hre[kMaxTerms], him[kMaxTerms];
-    spherical_hankel(L, r * k, h, hre, him);
     double *o = coef + (size_t)q * (3 + 2 * L);
     o[0] = d[0] / r;
     o[1] = d[1] / r;
     o[2] = d[2] / r;
-    cx<double> zi = mk(1.0, 0.0);
-    for (int n = 0; n < L; ++n) {
-        const cx<double> an = zi * mk(hre[n], him[n]) * double(2 * n + 1);
-        o[3 + 2 * n] = an.re;
-        o[4 + 2 * n] = an.im;
-        zi = mk(-zi.im * h, zi.re * h);
-    }
+    hankel_series(L, r * k, h, o + 3);
 }
 
 // Second pass: one block per interaction, threads over the sphere points, the series
@@ -2523,69 +2539,7 @@ __global__ void __launch_bounds__(128) k_fmm_pair_hankel(int npair, int L, doubl
     o[1] = d[1] / r;
     o[2] = d[2] / r;
     o[3] = 0.0;
-    // series coefficients a_n = (i h)^n (2n + 1) h_n(k r), h_n = j_n + i h y_n as
-    // spherical_hankel forms them (same operations), with the recurrences in registers
-    // instead of per-thread arrays: j_n goes to o[4 + 2n] first where it comes from the
-    // downward recurrence, then one upward pass forms y_n and a_n
-    const int N = L;
-    const double x = r * k;
-    const double sx = sin(x), cxs = cos(x);
-    const bool up = x > (double)N + 10.0;
-    if (!up) {
-        const double nu = (double)(N - 1);
-        double di = (2.0 * (nu + 1.0) + 1.0) / x, C = di, D = 0.0, err = 1.0;
-        for (int j = 1; err > 1.0e-9 && j < 100000; ++j) {
-            const double b = (2.0 * (nu + j + 1.0) + 1.0) / x;
-            D = b - D;
-            if (D == 0.0) D = 1.0e-30;
-            D = 1.0 / D;
-            C = b - 1.0 / C;
-            if (C == 0.0) C = 1.0e-30;
-            di = di * C * D;
-            err = fabs(C * D - 1.0);
-        }
-        const double gnu = nu / x - 1.0 / di;
-        // downward recurrence twice: first for g_0 (the normalisation), then to store j_n
-        double g = 1.0, dg = gnu;
-        for (int i = N - 2; i >= 0; --i) {
-            const double gi = ((double)i + 2.0) / x * g + dg;
-            dg = (double)i / x * gi - g;
-            g = gi;
-        }
-        const double dp = fabs(g) > 1.0e-5 ? sx / x / g : (cxs - sx / x) / x / dg;
-        g = 1.0;
-        dg = gnu;
-        o[4 + 2 * (N - 1)] = dp * g;
-        for (int i = N - 2; i >= 0; --i) {
-            const double gi = ((double)i + 2.0) / x * g + dg;
-            dg = (double)i / x * gi - g;
-            g = gi;
-            o[4 + 2 * i] = dp * g;
-        }
-    }
-    double jm2 = 0.0, jm1 = 0.0, ym2 = 0.0, ym1 = 0.0;
-    cx<double> zi = mk(1.0, 0.0);
-    for (int n = 0; n < N; ++n) {
-        double yn, jn;
-        if (n == 0) yn = -cxs / x;
-        else if (n == 1) yn = -(cxs / x + sx) / x;
-        else yn = ym1 * (double)(2 * n - 1) / x - ym2;
-        if (up) {
-            if (n == 0) jn = sx / x;
-            else if (n == 1) jn = (sx / x - cxs) / x;
-            else jn = jm1 * (double)(2 * n - 1) / x - jm2;
-            jm2 = jm1;
-            jm1 = jn;
-        } else {
-            jn = o[4 + 2 * n];
-        }
-        ym2 = ym1;
-        ym1 = yn;
-        const cx<double> an = zi * mk(jn, h * yn) * double(2 * n + 1);
-        o[4 + 2 * n] = an.re;
-        o[5 + 2 * n] = an.im;
-        zi = mk(-zi.im * h, zi.re * h);
-    }
+    hankel_series(L, r * k, h, o + 4);
 }
 
 // Second pass: one block per evaluation cluster, threads over sphere points, summing
